@@ -1,0 +1,231 @@
+/*
+ * rbgpu.h -- C ABI of librbgpu, the MI355X-native non-sequential optical
+ * tracer that stands in for ROBAST's AOpticsManager::TraceNonSequential.
+ *
+ * Everything here is plain C: fixed-size structs, raw pointers and sizes,
+ * int return codes (0 = ok, < 0 = error; rbgpu_last_error() has the text).
+ * No C++ or torch types cross this boundary, so the ROOT side (a cgo/ctypes/
+ * C++ shim) can bind it directly.
+ *
+ * Units follow ROBAST (include/AOpticsManager.h:59-71): lengths in cm,
+ * time in s, wavelength in cm (400 nm = 4e-5).
+ */
+#ifndef RBGPU_H
+#define RBGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- ray status: same codes and order as ARay's private enum
+ *      (include/ARay.h:26: kRun, kStop, kExit, kFocus, kSuspend, kAbsorb) --- */
+enum {
+  RB_RUN = 0,
+  RB_STOP = 1,
+  RB_EXIT = 2,
+  RB_FOCUS = 3,
+  RB_SUSPEND = 4,
+  RB_ABSORB = 5
+};
+
+/* ---- optical class of a volume: AOpticsManager's enum
+ *      (include/AOpticsManager.h:45-53). A volume's kind is decided by its
+ *      exact class, as IsA()== does there (subclasses do not match). ------- */
+enum {
+  RB_KLENS = 0,
+  RB_KOBS = 1,
+  RB_KMIRROR = 2,
+  RB_KFOCUS = 3,
+  RB_KOPT = 4,
+  RB_KOTHER = 5,
+  RB_KNULL = 6
+};
+
+/* ---- shape types and their parameter layout in rb_scene_desc.params ---- */
+enum {
+  RB_SHAPE_BOX = 1,        /* dx dy dz ox oy oz                     (TGeoBBox)        */
+  RB_SHAPE_TUBE = 2,       /* rmin rmax dz                          (TGeoTube)        */
+  RB_SHAPE_SPHERE = 3,     /* rmin rmax theta1 theta2 [deg]         (TGeoSphere, full phi) */
+  RB_SHAPE_PARABOLOID = 4, /* rlo rhi dz a b                        (TGeoParaboloid)  */
+  RB_SHAPE_PGON = 5,       /* phi1 dphi nedges nz {z rmin rmax}*nz  (TGeoPgon)        */
+  RB_SHAPE_PCON = 6,       /* phi1 dphi nz {z rmin rmax}*nz         (TGeoPcon)        */
+  RB_SHAPE_COMPOSITE = 7,  /* op left right lmat[12] rmat[12]       (TGeoBoolNode)    */
+  RB_SHAPE_ASPHERIC = 8,   /* z1 c1 z2 c2 rmax rmin kappa1 kappa2 n1 n2 K1[n1] K2[n2]
+                              (AGeoAsphericDisk)                                       */
+  RB_SHAPE_WINSTON = 9     /* r1 r2 polyN f theta dz                (AGeoWinstonConePoly) */
+};
+
+/* boolean operators of RB_SHAPE_COMPOSITE (TGeoUnion/TGeoSubtraction/TGeoIntersection) */
+enum { RB_OP_UNION = 0, RB_OP_SUBTRACTION = 1, RB_OP_INTERSECTION = 2 };
+
+/* refractive-index models of a lens volume */
+enum {
+  RB_INDEX_NONE = 0,      /* n = 1 (ALens without index, src/ALens.cxx:58-60)       */
+  RB_INDEX_TABLE = 1,     /* TGraph n(lambda), linear Eval                          */
+  RB_INDEX_SELLMEIER = 2, /* B1 B2 B3 C1 C2 C3, lambda in um (src/ASellmeierFormula.cxx:46-54) */
+  RB_INDEX_SCHOTT = 3     /* A0..A5, lambda in um (src/ASchottFormula.cxx:43-55)    */
+};
+
+/* A 4x3 affine transform in ROOT's convention: master = R * local + T,
+ * R row-major (TGeoMatrix::LocalToMaster). */
+typedef struct {
+  double rot[9];
+  double tr[3];
+} rb_matrix;
+
+typedef struct {
+  int32_t type;          /* RB_SHAPE_* */
+  int32_t param_offset;  /* first parameter in rb_scene_desc.params */
+  int32_t n_params;
+  int32_t pad;
+  double bbox_origin[3]; /* local bounding box (TGeoBBox::fOrigin, fDX..fDZ) */
+  double bbox_half[3];
+} rb_shape_desc;
+
+typedef struct {
+  double refl_const;       /* mirror reflectance when refl_table < 0 (default 1) */
+  int32_t shape;           /* index into shapes */
+  int32_t kind;            /* RB_K* */
+  int32_t refl_table;      /* mirror: TGraph R(lambda), -1 = constant            */
+  int32_t index_model;     /* lens: RB_INDEX_*                                   */
+  int32_t index_ref;       /* lens: table index (TABLE) or param offset (formula) */
+  int32_t k_table;         /* lens: TGraph k(lambda), -1 = none (k = 0)          */
+  int32_t qe_lambda_table; /* focal surface: TGraph QE(lambda), -1 = 1           */
+  int32_t qe_angle_table;  /* focal surface: TGraph QE(angle [rad]), -1 = none   */
+} rb_volume_desc;
+
+/* One placed instance of a volume (a path in the TGeo node tree), with its
+ * global matrix. Node 0 is the top volume. */
+typedef struct {
+  rb_matrix global;
+  int32_t volume;
+  int32_t parent;      /* -1 for the top node */
+  int32_t first_child; /* into rb_scene_desc.children */
+  int32_t n_children;
+} rb_node_desc;
+
+typedef struct {
+  int32_t offset; /* first (x, y) pair in table_xy */
+  int32_t n;      /* number of points, x ascending */
+} rb_table_desc;
+
+typedef struct {
+  int32_t n_nodes;
+  const rb_node_desc* nodes;
+  int32_t n_children;
+  const int32_t* children;
+  int32_t n_volumes;
+  const rb_volume_desc* volumes;
+  int32_t n_shapes;
+  const rb_shape_desc* shapes;
+  int32_t n_params;
+  const double* params;
+  int32_t n_tables;
+  const rb_table_desc* tables;
+  int32_t n_table_xy; /* number of doubles (2 per point) */
+  const double* table_xy;
+} rb_scene_desc;
+
+/* Rays in structure-of-arrays form. On input each ray holds its last point
+ * (x, y, z, t), unit direction, wavelength, status and point count, exactly
+ * what ARay carries (include/ARay.h:24-35). Rays whose status is not RB_RUN
+ * are left untouched (src/AOpticsManager.cxx:349-351). On output the arrays
+ * hold the terminal point, final direction, status, point count and the node
+ * index of the last AddNode (-1 = null node). */
+typedef struct {
+  double* x;
+  double* y;
+  double* z;
+  double* t;
+  double* dx;
+  double* dy;
+  double* dz;
+  double* lambda;
+  int32_t* status;
+  int32_t* npoints;
+  int32_t* last_node;
+  /* optional bounded path record (ARay's TGeoTrack points and node history):
+   * path_capacity == 0 disables it; otherwise path_xyzt is [n][capacity][4]
+   * and path_node [n][capacity]; slot k holds point k+1 (point 0 is the input) */
+  int32_t path_capacity;
+  int32_t pad;
+  double* path_xyzt;
+  int32_t* path_node;
+} rb_rays;
+
+enum { RB_PRECISION_FP64 = 0, RB_PRECISION_MIXED = 1 };
+
+typedef struct {
+  int32_t limit;       /* AOpticsManager::fLimit (default 100); SetLimit ignores n <= 0 */
+  int32_t fresnel;     /* 1 = Fresnel reflection on (default); 0 = DisableFresnelReflection(kTRUE) */
+  int32_t precision;   /* RB_PRECISION_* */
+  int32_t device_ptrs; /* 1 = rb_rays arrays are device pointers already in HBM */
+  uint64_t seed;       /* Philox4x32-10 key */
+  uint64_t ray_id_base;/* global id of ray 0 (Philox counter), for sharded runs */
+} rb_trace_opts;
+
+/* ---- synthetic ray sources (device-side ARayShooter equivalents) ---- */
+enum {
+  RB_RAYGEN_GRID = 0, /* ARayShooter::Square grid (src/ARayShooter.cxx:401-459)        */
+  RB_RAYGEN_DISK = 1  /* uniform disk of diameter `size`, normal to the direction,
+                         angle ~ U[theta_min, theta_max], lambda ~ U[l_min, l_max]   */
+};
+
+typedef struct {
+  int32_t kind;
+  int32_t n_side;     /* GRID: n rays per side                                  */
+  double size;        /* GRID: side length d; DISK: diameter                    */
+  double distance;    /* origin of ray plane at distance*(−sin th, 0, cos th)   */
+  double theta_min;   /* rad */
+  double theta_max;   /* rad */
+  double lambda_min;  /* cm  */
+  double lambda_max;  /* cm  */
+  uint64_t seed;
+} rb_raygen_desc;
+
+/* ---- API ---------------------------------------------------------------- */
+typedef struct rb_ctx rb_ctx;
+typedef struct rb_scene rb_scene;
+
+/* Create a context on HIP device `device` with its own stream (default
+ * priority). Contexts are not to be shared across host threads. */
+int rbgpu_ctx_create(int device, rb_ctx** out);
+void rbgpu_ctx_destroy(rb_ctx* ctx);
+const char* rbgpu_last_error(const rb_ctx* ctx);
+
+/* Upload a flattened scene (the AOpticsManager's closed TGeo tree). The
+ * library builds a BVH over each node's daughters on the host and keeps the
+ * scene in HBM. Replaces the geometry held by AOpticsManager (a TGeoManager,
+ * include/AOpticsManager.h:28) after CloseGeometry(). */
+int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* desc, rb_scene** out);
+void rbgpu_scene_free(rb_scene* scene);
+
+/* Trace n rays to termination: the non-sequential state machine of
+ * AOpticsManager::TraceNonSequential(TObjArray*) (src/AOpticsManager.cxx:335-520)
+ * applied to every running ray, the work ARayArray& dispatch does on host
+ * threads (src/AOpticsManager.cxx:523-587). Blocks until done. */
+int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
+                rb_rays* rays, int64_t n);
+
+/* Fill n rays on the device (pointers in `rays` must be device pointers). */
+int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* gen, rb_rays* rays,
+                        int64_t n);
+
+/* Milliseconds the last rbgpu_trace spent in its trace kernel(s), timed with
+ * HIP events on the context's stream, and the number of kernel launches. */
+double rbgpu_last_trace_ms(const rb_ctx* ctx);
+int rbgpu_last_trace_launches(const rb_ctx* ctx);
+
+/* Writes sizeof() of the ABI structs (rb_matrix, rb_shape_desc,
+ * rb_volume_desc, rb_node_desc, rb_table_desc, rb_scene_desc, rb_rays,
+ * rb_trace_opts, rb_raygen_desc) into out[0..8] for binding checks; returns 9.
+ * No counterpart in the reference (binding self-check only). */
+int rbgpu_abi_sizes(int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBGPU_H */

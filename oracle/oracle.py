@@ -1,0 +1,111 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU FP64 oracle
+(oracle/rb_oracle.cpp). Imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg -- never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from robast_amd import _abi as A
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call([os.path.join(_HERE, "build.sh")])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.oracle_trace.argtypes = [C.POINTER(A.rb_scene_desc), C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays),
+                                   C.c_int64, C.c_int]
+        L.oracle_trace.restype = C.c_int
+        L.oracle_shape_query.argtypes = [C.POINTER(A.rb_scene_desc), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_shape_query.restype = C.c_int
+        L.oracle_table_eval.argtypes = [C.POINTER(A.rb_scene_desc), C.c_int, C.c_double]
+        L.oracle_table_eval.restype = C.c_double
+        L.oracle_lens_index.argtypes = [C.POINTER(A.rb_scene_desc), C.c_int, C.c_double]
+        L.oracle_lens_index.restype = C.c_double
+        L.oracle_philox.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_philox.restype = None
+        L.oracle_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
+        L.oracle_uniform.restype = C.c_double
+        L.oracle_generate_rays.argtypes = [C.POINTER(A.rb_raygen_desc), C.POINTER(A.rb_rays), C.c_int64]
+        L.oracle_generate_rays.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def new_soa(n, path_capacity=0):
+    soa = {f: np.zeros(n) for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_")}
+    soa["status"] = np.zeros(n, dtype=np.int32)
+    soa["npoints"] = np.ones(n, dtype=np.int32)
+    soa["last_node"] = np.full(n, -1, dtype=np.int32)
+    if path_capacity:
+        soa["path_xyzt"] = np.zeros((n, path_capacity, 4))
+        soa["path_node"] = np.full((n, path_capacity), -1, dtype=np.int32)
+    return soa
+
+
+def rays_struct(soa):
+    r = A.rb_rays()
+    for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_", "status", "npoints", "last_node"):
+        setattr(r, f, A.ptr(soa[f]))
+    if "path_xyzt" in soa:
+        r.path_capacity = soa["path_xyzt"].shape[1]
+        r.path_xyzt = A.ptr(soa["path_xyzt"])
+        r.path_node = A.ptr(soa["path_node"])
+    return r
+
+
+def trace(flat, soa, opts, nthreads=1):
+    """Trace in place (soa: dict of host arrays)."""
+    n = len(soa["x"])
+    r = rays_struct(soa)
+    rc = lib().oracle_trace(C.byref(flat.desc), C.byref(opts), C.byref(r), n, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_trace failed ({rc})")
+    return soa
+
+
+def generate(gen, n):
+    soa = new_soa(n)
+    r = rays_struct(soa)
+    lib().oracle_generate_rays(C.byref(gen), C.byref(r), n)
+    return soa
+
+
+def shape_query(flat, shape, p, d):
+    out = np.zeros(6)
+    p = np.ascontiguousarray(p, dtype=float)
+    d = np.ascontiguousarray(d, dtype=float)
+    lib().oracle_shape_query(C.byref(flat.desc), shape, p.ctypes.data, d.ctypes.data, out.ctypes.data)
+    return bool(out[0]), out[1], out[2], out[3:6]
+
+
+def philox(ctr, key):
+    c = np.array(ctr, dtype=np.uint32)
+    k = np.array(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().oracle_philox(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def uniform(seed, ray_id, draw, stream=0):
+    return lib().oracle_uniform(seed, ray_id, draw, stream)
+
+
+def table_eval(flat, table, x):
+    return lib().oracle_table_eval(C.byref(flat.desc), table, x)
+
+
+def lens_index(flat, volume, lam):
+    return lib().oracle_lens_index(C.byref(flat.desc), volume, lam)

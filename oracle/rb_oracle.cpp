@@ -1,0 +1,1365 @@
+// rb_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU FP64 restatement of ROBAST's non-sequential tracer, used as the
+// correctness oracle for librbgpu and as the timed CPU baseline. Nothing in
+// the product (librbgpu, robast_amd) links, loads or calls this file; only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg do.
+//
+// It restates, from the reference source (read as text):
+//   src/AOpticsManager.cxx:335-520  per-ray state machine (trace_ray)
+//   src/AOpticsManager.cxx:170-247  DoReflection            (do_reflection)
+//   src/AOpticsManager.cxx:52-167   DoFresnel               (do_fresnel)
+//   src/AOpticsManager.cxx:523-587  ray-chunk dispatch      (oracle_trace)
+//   src/AMirror.cxx:39-60, src/ALens.cxx:36-60, include/ARefractiveIndex.h:36-65,
+//   src/ASellmeierFormula.cxx:46-54, src/ASchottFormula.cxx:43-55,
+//   src/AFocalSurface.cxx:35-52                              (physics tables)
+// and, because ROOT is not in the reference tree, the TGeo navigator and
+// primitive-shape behaviour it relies on (TGeoNavigator::FindNextBoundaryAndStep,
+// CrossBoundaryAndLocate, Step, FindNode; TGeoBBox/Tube/Sphere/Paraboloid/Pgon;
+// TGeoBoolNode) from ROOT's documented semantics. Those ROOT parts are
+// "parity unpinned" against ROOT itself; they are pinned against closed forms,
+// brute force and invariants in tests/.
+//
+// Deliberately plain: AoS, recursion, brute-force loops over daughters, no
+// BVH, no FMA contraction (build with -ffp-contract=off).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../include/rbgpu.h"
+
+namespace orc {
+
+constexpr double kBig = 1e30;        // TGeoShape::Big()
+constexpr double kTol = 1e-10;       // TGeoShape::Tolerance()
+constexpr double kEpsilon = 1e-6;    // src/AOpticsManager.cxx:20
+constexpr double kC = 2.99792458e10; // TMath::C() * AOpticsManager::m(), cm/s
+constexpr double kBoolPush = 1e-8;   // inside-test offset of the boolean iteration
+constexpr double kPi = 3.14159265358979323846;
+
+struct V3 {
+  double x, y, z;
+};
+static inline V3 v3(double x, double y, double z) { return {x, y, z}; }
+static inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 mul(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 axpy(V3 p, double s, V3 d) { return {p.x + s * d.x, p.y + s * d.y, p.z + s * d.z}; }
+
+// TGeoMatrix conventions: master = R local + T
+static inline V3 m2l(const rb_matrix& m, V3 p) {
+  V3 q = sub(p, v3(m.tr[0], m.tr[1], m.tr[2]));
+  const double* r = m.rot;
+  return {r[0] * q.x + r[3] * q.y + r[6] * q.z, r[1] * q.x + r[4] * q.y + r[7] * q.z,
+          r[2] * q.x + r[5] * q.y + r[8] * q.z};
+}
+static inline V3 m2lv(const rb_matrix& m, V3 q) {
+  const double* r = m.rot;
+  return {r[0] * q.x + r[3] * q.y + r[6] * q.z, r[1] * q.x + r[4] * q.y + r[7] * q.z,
+          r[2] * q.x + r[5] * q.y + r[8] * q.z};
+}
+static inline V3 l2mv(const rb_matrix& m, V3 q) {
+  const double* r = m.rot;
+  return {r[0] * q.x + r[1] * q.y + r[2] * q.z, r[3] * q.x + r[4] * q.y + r[5] * q.z,
+          r[6] * q.x + r[7] * q.y + r[8] * q.z};
+}
+
+struct Scene {
+  const rb_scene_desc* d;
+  const double* P(int shape) const { return d->params + d->shapes[shape].param_offset; }
+  int type(int shape) const { return d->shapes[shape].type; }
+  const rb_matrix& mat(const double* p) const { return *reinterpret_cast<const rb_matrix*>(p); }
+};
+
+// A distance result and the leaf surface it came from. `leaf` encodes the
+// path through composite shapes: bit k = right child taken at depth k,
+// depth in bits 24..31.
+struct Hit {
+  double s;
+  uint32_t leaf;
+};
+static inline uint32_t leaf_push(uint32_t leaf, int side) {
+  uint32_t depth = leaf >> 24;
+  return ((depth + 1) << 24) | (((leaf & 0xffffffu) << 1) | uint32_t(side));
+}
+
+// ---------------------------------------------------------------- box
+static bool box_contains(const double* p, V3 q) {
+  return std::fabs(q.x - p[3]) <= p[0] && std::fabs(q.y - p[4]) <= p[1] &&
+         std::fabs(q.z - p[5]) <= p[2];
+}
+static double box_dist_in(const double* p, V3 q, V3 d) {
+  double n[3] = {q.x - p[3], q.y - p[4], q.z - p[5]};
+  double dd[3] = {d.x, d.y, d.z};
+  double smin = kBig;
+  for (int i = 0; i < 3; i++) {
+    if (dd[i] != 0) {
+      double s = (dd[i] > 0 ? p[i] - n[i] : p[i] + n[i]) / std::fabs(dd[i]);
+      if (s < smin) smin = s;
+    }
+  }
+  return smin < 0 ? 0 : smin;
+}
+static double box_dist_out(const double* p, V3 q, V3 d) {
+  double n[3] = {q.x - p[3], q.y - p[4], q.z - p[5]};
+  double dd[3] = {d.x, d.y, d.z};
+  double saf[3];
+  bool in = true;
+  for (int i = 0; i < 3; i++) {
+    saf[i] = std::fabs(n[i]) - p[i];
+    if (saf[i] > 0) in = false;
+  }
+  if (in) return 0.0;
+  for (int i = 0; i < 3; i++) {
+    if (saf[i] < 0) continue;
+    if (n[i] * dd[i] >= 0) continue;
+    double s = saf[i] / std::fabs(dd[i]);
+    bool ok = true;
+    for (int j = 0; j < 3; j++) {
+      if (j == i) continue;
+      if (std::fabs(n[j] + s * dd[j]) > p[j]) {
+        ok = false;
+        break;
+      }
+    }
+    if (ok) return s;
+  }
+  return kBig;
+}
+static V3 box_normal(const double* p, V3 q, V3 d) {
+  double saf[3] = {std::fabs(std::fabs(q.x - p[3]) - p[0]), std::fabs(std::fabs(q.y - p[4]) - p[1]),
+                   std::fabs(std::fabs(q.z - p[5]) - p[2])};
+  int i = 0;
+  if (saf[1] < saf[i]) i = 1;
+  if (saf[2] < saf[i]) i = 2;
+  double dd[3] = {d.x, d.y, d.z};
+  double n[3] = {0, 0, 0};
+  n[i] = dd[i] > 0 ? 1 : -1;
+  return v3(n[0], n[1], n[2]);
+}
+
+// ---------------------------------------------------------------- tube
+// roots of |(q + s d)_xy|^2 = R^2 : s = -b -/+ sqrt(b^2 - c)
+static bool cyl_roots(V3 q, V3 d, double R, double& s1, double& s2) {
+  double nsq = d.x * d.x + d.y * d.y;
+  if (nsq < 1e-20) return false;
+  double b = (q.x * d.x + q.y * d.y) / nsq;
+  double c = (q.x * q.x + q.y * q.y - R * R) / nsq;
+  double disc = b * b - c;
+  if (disc <= 0) return false;
+  double sq = std::sqrt(disc);
+  s1 = -b - sq;
+  s2 = -b + sq;
+  return true;
+}
+static bool tube_contains(const double* p, V3 q) {
+  if (std::fabs(q.z) > p[2]) return false;
+  double r2 = q.x * q.x + q.y * q.y;
+  return r2 >= p[0] * p[0] && r2 <= p[1] * p[1];
+}
+static double tube_dist_in(const double* p, V3 q, V3 d) {
+  double rmin = p[0], rmax = p[1], dz = p[2];
+  double sz = kBig;
+  if (d.z != 0) {
+    sz = ((d.z > 0 ? dz : -dz) - q.z) / d.z;
+    if (sz <= 0) return 0.0;
+  }
+  double s1, s2, s = sz;
+  if (rmin > 0 && cyl_roots(q, d, rmin, s1, s2) && s1 > 0) s = std::min(s, s1);
+  if (cyl_roots(q, d, rmax, s1, s2)) {
+    if (s2 > 0) s = std::min(s, s2);
+    else s = 0;
+  } else if (d.x * d.x + d.y * d.y >= 1e-20) {
+    s = 0;  // no crossing of rmax although moving radially: numerically outside
+  }
+  return s < 0 ? 0 : s;
+}
+static double tube_dist_out(const double* p, V3 q, V3 d) {
+  double rmin = p[0], rmax = p[1], dz = p[2];
+  double best = kBig;
+  // z planes
+  if (std::fabs(q.z) >= dz && q.z * d.z < 0) {
+    double s = (std::fabs(q.z) - dz) / std::fabs(d.z);
+    double x = q.x + s * d.x, y = q.y + s * d.y, r2 = x * x + y * y;
+    if (r2 >= rmin * rmin && r2 <= rmax * rmax) best = s;
+  }
+  double s1, s2;
+  double r2 = q.x * q.x + q.y * q.y;
+  if (r2 >= rmax * rmax && cyl_roots(q, d, rmax, s1, s2) && s1 >= 0 && s1 < best) {
+    if (std::fabs(q.z + s1 * d.z) <= dz) best = s1;
+  }
+  if (rmin > 0 && r2 <= rmin * rmin && cyl_roots(q, d, rmin, s1, s2) && s2 >= 0 && s2 < best) {
+    if (std::fabs(q.z + s2 * d.z) <= dz) best = s2;
+  }
+  return best;
+}
+static V3 tube_normal(const double* p, V3 q, V3 d) {
+  double r = std::sqrt(q.x * q.x + q.y * q.y);
+  double saf0 = std::fabs(p[2] - std::fabs(q.z));
+  double saf1 = p[0] > 1e-10 ? std::fabs(r - p[0]) : kBig;
+  double saf2 = std::fabs(p[1] - r);
+  int i = 0;
+  double m = saf0;
+  if (saf1 < m) { i = 1; m = saf1; }
+  if (saf2 < m) { i = 2; }
+  if (i == 0) return v3(0, 0, d.z >= 0 ? 1.0 : -1.0);
+  if (r <= 0) return v3(1, 0, 0);
+  V3 n = v3(q.x / r, q.y / r, 0);
+  if (n.x * d.x + n.y * d.y < 0) n = mul(n, -1);
+  return n;
+}
+
+// ---------------------------------------------------------------- sphere
+// rmin rmax theta1 theta2 (deg), full phi. Constraint functions (<= 0 inside):
+//   g_out = r^2 - rmax^2 ; g_in = rmin^2 - r^2
+//   g_t1 = z - c1 r   (theta >= theta1)  when theta1 > 0
+//   g_t2 = c2 r - z   (theta <= theta2)  when theta2 < 180
+struct SphereP {
+  double rmin, rmax, c1, c2;
+  bool t1, t2;
+};
+static SphereP sphere_p(const double* p) {
+  SphereP s;
+  s.rmin = p[0];
+  s.rmax = p[1];
+  s.t1 = p[2] > 0;
+  s.t2 = p[3] < 180;
+  s.c1 = std::cos(p[2] * kPi / 180);
+  s.c2 = std::cos(p[3] * kPi / 180);
+  return s;
+}
+static bool sphere_contains(const double* p, V3 q) {
+  SphereP s = sphere_p(p);
+  double r2 = dot(q, q);
+  if (r2 < s.rmin * s.rmin || r2 > s.rmax * s.rmax) return false;
+  if (r2 < 1e-20) return true;
+  double r = std::sqrt(r2);
+  if (s.t1 && q.z > s.c1 * r) return false;
+  if (s.t2 && q.z < s.c2 * r) return false;
+  return true;
+}
+// crossings of the cone z = c r (nappe sign(z) == sign(c)); plane z=0 if c==0
+static int cone_roots(V3 q, V3 d, double c, double* out) {
+  int n = 0;
+  if (std::fabs(c) < 1e-15) {
+    if (d.z != 0) out[n++] = -q.z / d.z;
+    return n;
+  }
+  double c2 = c * c;
+  double A = d.z * d.z - c2 * dot(d, d);
+  double B = 2 * (q.z * d.z - c2 * dot(q, d));
+  double C = q.z * q.z - c2 * dot(q, q);
+  double r[2];
+  int m = 0;
+  if (std::fabs(A) < 1e-15) {
+    if (B != 0) r[m++] = -C / B;
+  } else {
+    double disc = B * B - 4 * A * C;
+    if (disc < 0) return 0;
+    double sq = std::sqrt(disc);
+    double qq = -0.5 * (B + (B >= 0 ? sq : -sq));
+    r[m++] = qq / A;
+    if (qq != 0) r[m++] = C / qq;
+  }
+  for (int i = 0; i < m; i++) {
+    double z = q.z + r[i] * d.z;
+    if ((c > 0 && z >= 0) || (c < 0 && z <= 0)) out[n++] = r[i];
+  }
+  if (n == 2 && out[0] > out[1]) std::swap(out[0], out[1]);
+  return n;
+}
+// constraint values and their rate along d at point x
+static void sphere_g(const SphereP& s, V3 x, V3 d, double* g, double* dg, bool* act) {
+  double r = std::sqrt(dot(x, x));
+  double rd = r > 0 ? dot(x, d) / r : 0;
+  g[0] = r - s.rmax;           dg[0] = rd;  act[0] = true;
+  g[1] = s.rmin - r;           dg[1] = -rd; act[1] = s.rmin > 0;
+  g[2] = x.z - s.c1 * r;       dg[2] = d.z - s.c1 * rd; act[2] = s.t1;
+  g[3] = s.c2 * r - x.z;       dg[3] = s.c2 * rd - d.z; act[3] = s.t2;
+}
+static int sphere_candidates(const SphereP& s, V3 q, V3 d, double* t, int* which) {
+  int n = 0;
+  double b = dot(q, d), c = dot(q, q) - s.rmax * s.rmax, disc = b * b - c;
+  if (disc > 0) {
+    double sq = std::sqrt(disc);
+    t[n] = -b - sq; which[n++] = 0;
+    t[n] = -b + sq; which[n++] = 0;
+  }
+  if (s.rmin > 0) {
+    c = dot(q, q) - s.rmin * s.rmin;
+    disc = b * b - c;
+    if (disc > 0) {
+      double sq = std::sqrt(disc);
+      t[n] = -b - sq; which[n++] = 1;
+      t[n] = -b + sq; which[n++] = 1;
+    }
+  }
+  double r[2];
+  if (s.t1) {
+    int m = cone_roots(q, d, s.c1, r);
+    for (int i = 0; i < m; i++) { t[n] = r[i]; which[n++] = 2; }
+  }
+  if (s.t2) {
+    int m = cone_roots(q, d, s.c2, r);
+    for (int i = 0; i < m; i++) { t[n] = r[i]; which[n++] = 3; }
+  }
+  return n;
+}
+static double sphere_dist_out(const double* p, V3 q, V3 d) {
+  SphereP s = sphere_p(p);
+  double t[8];
+  int w[8];
+  int n = sphere_candidates(s, q, d, t, w);
+  double best = kBig;
+  for (int i = 0; i < n; i++) {
+    if (t[i] < 0 || t[i] >= best) continue;
+    V3 x = axpy(q, t[i], d);
+    double g[4], dg[4];
+    bool act[4];
+    sphere_g(s, x, d, g, dg, act);
+    if (dg[w[i]] >= 0) continue;  // must be entering through this surface
+    bool ok = true;
+    for (int k = 0; k < 4; k++) {
+      if (k == w[i] || !act[k]) continue;
+      if (g[k] > 1e-9 * (1 + s.rmax)) { ok = false; break; }
+    }
+    if (ok) best = t[i];
+  }
+  return best;
+}
+static double sphere_dist_in(const double* p, V3 q, V3 d) {
+  SphereP s = sphere_p(p);
+  double t[8];
+  int w[8];
+  int n = sphere_candidates(s, q, d, t, w);
+  double best = kBig;
+  for (int i = 0; i < n; i++) {
+    if (t[i] < 0 || t[i] >= best) continue;
+    V3 x = axpy(q, t[i], d);
+    double g[4], dg[4];
+    bool act[4];
+    sphere_g(s, x, d, g, dg, act);
+    if (dg[w[i]] <= 0) continue;  // must be leaving through this surface
+    best = t[i];
+  }
+  return best >= kBig ? 0.0 : best;
+}
+static V3 sphere_normal(const double* p, V3 q, V3 d) {
+  SphereP s = sphere_p(p);
+  double r = std::sqrt(dot(q, q));
+  double dist[4] = {std::fabs(r - s.rmax), s.rmin > 0 ? std::fabs(r - s.rmin) : kBig, kBig, kBig};
+  double rho = std::sqrt(q.x * q.x + q.y * q.y);
+  double st1 = std::sqrt(std::max(0.0, 1 - s.c1 * s.c1)), st2 = std::sqrt(std::max(0.0, 1 - s.c2 * s.c2));
+  // distance to the cone line in the meridional plane: |rho*c - z*sin|
+  if (s.t1) dist[2] = std::fabs(rho * s.c1 - q.z * st1);
+  if (s.t2) dist[3] = std::fabs(rho * s.c2 - q.z * st2);
+  int k = 0;
+  for (int i = 1; i < 4; i++)
+    if (dist[i] < dist[k]) k = i;
+  V3 n;
+  if (k <= 1) {
+    n = r > 0 ? mul(q, 1 / r) : v3(0, 0, 1);
+  } else {
+    double c = k == 2 ? s.c1 : s.c2, st = k == 2 ? st1 : st2;
+    double cp = rho > 0 ? q.x / rho : 1, sp = rho > 0 ? q.y / rho : 0;
+    // gradient of (z*sin - rho*cos) direction
+    n = v3(-c * cp, -c * sp, st);
+  }
+  if (dot(n, d) < 0) n = mul(n, -1);
+  return n;
+}
+
+// ---------------------------------------------------------------- paraboloid
+// rlo rhi dz a b : z = a r^2 + b, solid = {|z|<=dz, a(z-b) >= a^2 r^2}
+static bool para_contains(const double* p, V3 q) {
+  if (std::fabs(q.z) > p[2]) return false;
+  double a = p[3], b = p[4];
+  double aa = a * (q.z - b);
+  if (aa < 0) return false;
+  return aa >= a * a * (q.x * q.x + q.y * q.y);
+}
+// roots of a (x^2+y^2) + b - z = 0 along the ray
+static int para_roots(const double* p, V3 q, V3 d, double* r) {
+  double a = p[3], b = p[4];
+  double A = a * (d.x * d.x + d.y * d.y);
+  double B = 2 * a * (q.x * d.x + q.y * d.y) - d.z;
+  double C = a * (q.x * q.x + q.y * q.y) + b - q.z;
+  if (std::fabs(A) < 1e-30) {
+    if (B == 0) return 0;
+    r[0] = -C / B;
+    return 1;
+  }
+  double disc = B * B - 4 * A * C;
+  if (disc < 0) return 0;
+  double sq = std::sqrt(disc);
+  double qq = -0.5 * (B + (B >= 0 ? sq : -sq));
+  int n = 0;
+  r[n++] = qq / A;
+  if (qq != 0) r[n++] = C / qq;
+  if (n == 2 && r[0] > r[1]) std::swap(r[0], r[1]);
+  return n;
+}
+static double para_dist_out(const double* p, V3 q, V3 d) {
+  double rlo = p[0], rhi = p[1], dz = p[2], a = p[3];
+  double best = kBig;
+  if (q.z <= -dz && d.z > 0) {
+    double s = -(dz + q.z) / d.z;
+    double x = q.x + s * d.x, y = q.y + s * d.y;
+    if (x * x + y * y <= rlo * rlo) best = s;
+  } else if (q.z >= dz && d.z < 0) {
+    double s = (dz - q.z) / d.z;
+    double x = q.x + s * d.x, y = q.y + s * d.y;
+    if (x * x + y * y <= rhi * rhi) best = s;
+  }
+  double r[2];
+  int n = para_roots(p, q, d, r);
+  for (int i = 0; i < n; i++) {
+    if (r[i] < 0 || r[i] >= best) continue;
+    V3 x = axpy(q, r[i], d);
+    if (std::fabs(x.z) > dz) continue;
+    // entering: d/ds [a(z-b) - a^2 r^2] > 0
+    double rate = a * d.z - 2 * a * a * (x.x * d.x + x.y * d.y);
+    if (rate <= 0) continue;
+    best = r[i];
+    break;
+  }
+  return best;
+}
+static double para_dist_in(const double* p, V3 q, V3 d) {
+  double dz = p[2], a = p[3];
+  double best = kBig;
+  if (d.z > 0) best = (dz - q.z) / d.z;
+  else if (d.z < 0) best = (-dz - q.z) / d.z;
+  if (best < 0) best = 0;
+  double r[2];
+  int n = para_roots(p, q, d, r);
+  for (int i = 0; i < n; i++) {
+    if (r[i] < 0 || r[i] >= best) continue;
+    V3 x = axpy(q, r[i], d);
+    double rate = a * d.z - 2 * a * a * (x.x * d.x + x.y * d.y);
+    if (rate >= 0) continue;  // leaving through the parabolic wall
+    best = r[i];
+    break;
+  }
+  return best;
+}
+static V3 para_normal(const double* p, V3 q, V3 d) {
+  double dz = p[2], a = p[3];
+  double r2 = q.x * q.x + q.y * q.y;
+  // distance to the parabolic surface along z, scaled by the slope
+  double zs = a * r2 + p[4];
+  double grad_len = std::sqrt(1 + 4 * a * a * r2);
+  double dpar = std::fabs(q.z - zs) / grad_len;
+  double dtop = std::fabs(dz - q.z), dbot = std::fabs(q.z + dz);
+  V3 n;
+  if (dpar <= dtop && dpar <= dbot) {
+    n = v3(2 * a * q.x, 2 * a * q.y, -1);
+    n = mul(n, 1 / grad_len);
+  } else {
+    n = v3(0, 0, 1);
+  }
+  if (dot(n, d) < 0) n = mul(n, -1);
+  return n;
+}
+
+// ---------------------------------------------------------------- pgon
+// phi1 dphi nedges nz {z rmin rmax}; rmin == 0 and dphi == 360 supported.
+// Each z segment is a convex frustum of a regular prism: half-spaces
+//   n_k . (x, y) <= rmax_i + slope_i (z - z_i),  z_i <= z <= z_{i+1}
+struct PgonSeg {
+  double z0, z1, r0, slope;
+};
+static int pgon_nedges(const double* p) { return int(p[2]); }
+static int pgon_nz(const double* p) { return int(p[3]); }
+static void pgon_face(const double* p, int k, double& cx, double& sy) {
+  double div = p[1] / pgon_nedges(p);
+  double ph = (p[0] + div * (k + 0.5)) * kPi / 180;
+  cx = std::cos(ph);
+  sy = std::sin(ph);
+}
+static bool pgon_seg(const double* p, int i, PgonSeg& s) {
+  const double* z = p + 4;
+  s.z0 = z[3 * i];
+  s.z1 = z[3 * (i + 1)];
+  if (s.z1 - s.z0 <= 0) return false;
+  s.r0 = z[3 * i + 2];
+  s.slope = (z[3 * (i + 1) + 2] - s.r0) / (s.z1 - s.z0);
+  return true;
+}
+// clip the ray against segment i: returns false if it misses
+static bool pgon_clip(const double* p, const PgonSeg& s, V3 q, V3 d, double& t0, double& t1) {
+  t0 = -kBig;
+  t1 = kBig;
+  // z slab
+  if (d.z != 0) {
+    double a = (s.z0 - q.z) / d.z, b = (s.z1 - q.z) / d.z;
+    if (a > b) std::swap(a, b);
+    t0 = std::max(t0, a);
+    t1 = std::min(t1, b);
+  } else if (q.z < s.z0 || q.z > s.z1) {
+    return false;
+  }
+  int ne = pgon_nedges(p);
+  for (int k = 0; k < ne; k++) {
+    double cx, sy;
+    pgon_face(p, k, cx, sy);
+    // f(x) = cx x + sy y - slope z - (r0 - slope z0) <= 0
+    double f0 = cx * q.x + sy * q.y - s.slope * (q.z - s.z0) - s.r0;
+    double fd = cx * d.x + sy * d.y - s.slope * d.z;
+    if (fd == 0) {
+      if (f0 > 0) return false;
+      continue;
+    }
+    double t = -f0 / fd;
+    if (fd > 0) t1 = std::min(t1, t);
+    else t0 = std::max(t0, t);
+  }
+  return t0 <= t1;
+}
+static bool pgon_contains(const double* p, V3 q) {
+  int nz = pgon_nz(p);
+  const double* z = p + 4;
+  if (q.z < z[0] || q.z > z[3 * (nz - 1)]) return false;
+  double div = p[1] / pgon_nedges(p);
+  double phi = std::atan2(q.y, q.x) * 180 / kPi;
+  while (phi < p[0]) phi += 360.0;
+  double ddp = phi - p[0];
+  if (ddp > p[1]) return false;
+  int ip = std::min(int(ddp / div), pgon_nedges(p) - 1);
+  double ph0 = (p[0] + div * (ip + 0.5)) * kPi / 180;
+  double r = q.x * std::cos(ph0) + q.y * std::sin(ph0);
+  for (int i = 0; i + 1 < nz; i++) {
+    double za = z[3 * i], zb = z[3 * (i + 1)];
+    if (q.z < za || q.z > zb) continue;
+    double rmax;
+    if (zb - za < 1e-8) {
+      rmax = std::max(z[3 * i + 2], z[3 * (i + 1) + 2]);
+    } else {
+      double f = (q.z - za) / (zb - za);
+      rmax = z[3 * i + 2] + f * (z[3 * (i + 1) + 2] - z[3 * i + 2]);
+    }
+    if (r <= rmax) return true;
+  }
+  return false;
+}
+static double pgon_dist_out(const double* p, V3 q, V3 d) {
+  int nz = pgon_nz(p);
+  double best = kBig;
+  for (int i = 0; i + 1 < nz; i++) {
+    PgonSeg s;
+    if (!pgon_seg(p, i, s)) continue;
+    double t0, t1;
+    if (!pgon_clip(p, s, q, d, t0, t1)) continue;
+    if (t1 <= 0 || t0 >= t1) continue;
+    double t = t0 < 0 ? 0 : t0;
+    if (t < best) best = t;
+  }
+  return best;
+}
+static double pgon_dist_in(const double* p, V3 q, V3 d) {
+  int nz = pgon_nz(p);
+  double cur = 0;
+  for (int guard = 0; guard < 2 * nz + 2; guard++) {
+    bool moved = false;
+    for (int i = 0; i + 1 < nz; i++) {
+      PgonSeg s;
+      if (!pgon_seg(p, i, s)) continue;
+      double t0, t1;
+      if (!pgon_clip(p, s, q, d, t0, t1)) continue;
+      if (t0 <= cur + 1e-9 && t1 > cur + 1e-9) {
+        cur = t1;
+        moved = true;
+        break;
+      }
+    }
+    if (!moved) break;
+  }
+  return cur;
+}
+static V3 pgon_normal(const double* p, V3 q, V3 d) {
+  int nz = pgon_nz(p);
+  const double* z = p + 4;
+  double best = kBig;
+  V3 n = v3(0, 0, 1);
+  for (int i = 0; i < nz; i++) {
+    double dzp = std::fabs(q.z - z[3 * i]);
+    if (dzp < best) {
+      best = dzp;
+      n = v3(0, 0, 1);
+    }
+  }
+  double div = p[1] / pgon_nedges(p);
+  double phi = std::atan2(q.y, q.x) * 180 / kPi;
+  while (phi < p[0]) phi += 360.0;
+  int ip = std::min(int((phi - p[0]) / div), pgon_nedges(p) - 1);
+  for (int i = 0; i + 1 < nz; i++) {
+    PgonSeg s;
+    if (!pgon_seg(p, i, s)) continue;
+    if (q.z < s.z0 - 1e-9 || q.z > s.z1 + 1e-9) continue;
+    double cx, sy;
+    pgon_face(p, ip, cx, sy);
+    double len = std::sqrt(1 + s.slope * s.slope);
+    double f = (cx * q.x + sy * q.y - s.slope * (q.z - s.z0) - s.r0) / len;
+    if (std::fabs(f) < best) {
+      best = std::fabs(f);
+      n = v3(cx / len, sy / len, -s.slope / len);
+    }
+  }
+  if (dot(n, d) < 0) n = mul(n, -1);
+  return n;
+}
+
+// ---------------------------------------------------------------- dispatch
+static bool contains(const Scene& sc, int shape, V3 q);
+static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d);
+static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d);
+
+struct Comp {
+  int op, left, right;
+  const rb_matrix* lm;
+  const rb_matrix* rm;
+};
+static Comp comp_of(const Scene& sc, int shape) {
+  const double* p = sc.P(shape);
+  Comp c;
+  c.op = int(p[0]);
+  c.left = int(p[1]);
+  c.right = int(p[2]);
+  c.lm = &sc.mat(p + 3);
+  c.rm = &sc.mat(p + 15);
+  return c;
+}
+
+static bool contains(const Scene& sc, int shape, V3 q) {
+  const double* p = sc.P(shape);
+  switch (sc.type(shape)) {
+    case RB_SHAPE_BOX: return box_contains(p, q);
+    case RB_SHAPE_TUBE: return tube_contains(p, q);
+    case RB_SHAPE_SPHERE: return sphere_contains(p, q);
+    case RB_SHAPE_PARABOLOID: return para_contains(p, q);
+    case RB_SHAPE_PGON: return pgon_contains(p, q);
+    case RB_SHAPE_COMPOSITE: {
+      Comp c = comp_of(sc, shape);
+      bool a = contains(sc, c.left, m2l(*c.lm, q));
+      if (c.op == RB_OP_UNION) return a || contains(sc, c.right, m2l(*c.rm, q));
+      if (c.op == RB_OP_SUBTRACTION) return a && !contains(sc, c.right, m2l(*c.rm, q));
+      return a && contains(sc, c.right, m2l(*c.rm, q));
+    }
+  }
+  return false;
+}
+
+static Hit child_out(const Scene& sc, int child, const rb_matrix& m, V3 q, V3 d, int side) {
+  Hit h = dist_out(sc, child, m2l(m, q), m2lv(m, d));
+  h.leaf = leaf_push(h.leaf, side);
+  return h;
+}
+static Hit child_in(const Scene& sc, int child, const rb_matrix& m, V3 q, V3 d, int side) {
+  Hit h = dist_in(sc, child, m2l(m, q), m2lv(m, d));
+  h.leaf = leaf_push(h.leaf, side);
+  return h;
+}
+
+// TGeoUnion/TGeoSubtraction/TGeoIntersection::DistFromOutside restated as
+// boundary-to-boundary iteration (inside tests taken kBoolPush past each point)
+static Hit comp_dist_out(const Scene& sc, int shape, V3 q, V3 d) {
+  Comp c = comp_of(sc, shape);
+  if (c.op == RB_OP_UNION) {
+    Hit a = child_out(sc, c.left, *c.lm, q, d, 0);
+    Hit b = child_out(sc, c.right, *c.rm, q, d, 1);
+    return b.s < a.s ? b : a;
+  }
+  double s = 0;
+  uint32_t leaf = 0;
+  for (int it = 0; it < 64; it++) {
+    V3 x = axpy(q, s, d);
+    V3 xp = axpy(x, kBoolPush, d);
+    bool inA = contains(sc, c.left, m2l(*c.lm, xp));
+    bool inB = contains(sc, c.right, m2l(*c.rm, xp));
+    double step;
+    if (c.op == RB_OP_SUBTRACTION) {
+      if (inA && !inB) return {s, leaf};
+      if (inB) {
+        Hit h = child_in(sc, c.right, *c.rm, x, d, 1);
+        step = h.s;
+        leaf = h.leaf;
+      } else {
+        Hit h = child_out(sc, c.left, *c.lm, x, d, 0);
+        if (h.s >= kBig) return {kBig, 0};
+        step = h.s;
+        leaf = h.leaf;
+      }
+    } else {  // intersection
+      if (inA && inB) return {s, leaf};
+      Hit ha = {0, 0}, hb = {0, 0};
+      if (!inA) {
+        ha = child_out(sc, c.left, *c.lm, x, d, 0);
+        if (ha.s >= kBig) return {kBig, 0};
+      }
+      if (!inB) {
+        hb = child_out(sc, c.right, *c.rm, x, d, 1);
+        if (hb.s >= kBig) return {kBig, 0};
+      }
+      if (!inA && (inB || ha.s >= hb.s)) {
+        step = ha.s;
+        leaf = ha.leaf;
+      } else {
+        step = hb.s;
+        leaf = hb.leaf;
+      }
+    }
+    if (step < kBoolPush) step = kBoolPush;
+    s += step;
+  }
+  return {kBig, 0};
+}
+static Hit comp_dist_in(const Scene& sc, int shape, V3 q, V3 d) {
+  Comp c = comp_of(sc, shape);
+  if (c.op == RB_OP_INTERSECTION) {
+    Hit a = child_in(sc, c.left, *c.lm, q, d, 0);
+    Hit b = child_in(sc, c.right, *c.rm, q, d, 1);
+    return b.s < a.s ? b : a;
+  }
+  if (c.op == RB_OP_SUBTRACTION) {
+    Hit a = child_in(sc, c.left, *c.lm, q, d, 0);
+    Hit b = child_out(sc, c.right, *c.rm, q, d, 1);
+    return b.s < a.s ? b : a;
+  }
+  // union: leave both children
+  double s = 0;
+  uint32_t leaf = 0;
+  for (int it = 0; it < 64; it++) {
+    V3 x = axpy(q, s, d);
+    V3 xp = axpy(x, kBoolPush, d);
+    bool inA = contains(sc, c.left, m2l(*c.lm, xp));
+    bool inB = contains(sc, c.right, m2l(*c.rm, xp));
+    if (!inA && !inB) return {s, leaf};
+    Hit ha = {0, 0}, hb = {0, 0};
+    if (inA) ha = child_in(sc, c.left, *c.lm, x, d, 0);
+    if (inB) hb = child_in(sc, c.right, *c.rm, x, d, 1);
+    Hit h = (inA && (!inB || ha.s >= hb.s)) ? ha : hb;
+    double step = h.s < kBoolPush ? kBoolPush : h.s;
+    leaf = h.leaf;
+    s += step;
+  }
+  return {s, leaf};
+}
+
+static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
+  const double* p = sc.P(shape);
+  switch (sc.type(shape)) {
+    case RB_SHAPE_BOX: return {box_dist_out(p, q, d), 0};
+    case RB_SHAPE_TUBE: return {tube_dist_out(p, q, d), 0};
+    case RB_SHAPE_SPHERE: return {sphere_dist_out(p, q, d), 0};
+    case RB_SHAPE_PARABOLOID: return {para_dist_out(p, q, d), 0};
+    case RB_SHAPE_PGON: return {pgon_dist_out(p, q, d), 0};
+    case RB_SHAPE_COMPOSITE: return comp_dist_out(sc, shape, q, d);
+  }
+  return {kBig, 0};
+}
+static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
+  const double* p = sc.P(shape);
+  switch (sc.type(shape)) {
+    case RB_SHAPE_BOX: return {box_dist_in(p, q, d), 0};
+    case RB_SHAPE_TUBE: return {tube_dist_in(p, q, d), 0};
+    case RB_SHAPE_SPHERE: return {sphere_dist_in(p, q, d), 0};
+    case RB_SHAPE_PARABOLOID: return {para_dist_in(p, q, d), 0};
+    case RB_SHAPE_PGON: return {pgon_dist_in(p, q, d), 0};
+    case RB_SHAPE_COMPOSITE: return comp_dist_in(sc, shape, q, d);
+  }
+  return {0, 0};
+}
+// normal of the surface `leaf` of `shape` at q, oriented along d
+static V3 shape_normal(const Scene& sc, int shape, uint32_t leaf, V3 q, V3 d) {
+  int depth = 0;
+  int maxdepth = int(leaf >> 24);
+  // walk composite path; matrices accumulate as nested master->local maps
+  const rb_matrix* stack[32];
+  int ns = 0;
+  while (sc.type(shape) == RB_SHAPE_COMPOSITE) {
+    Comp c = comp_of(sc, shape);
+    int side = depth < maxdepth ? int((leaf >> depth) & 1u) : 0;
+    const rb_matrix* m = side ? c.rm : c.lm;
+    q = m2l(*m, q);
+    d = m2lv(*m, d);
+    stack[ns++] = m;
+    shape = side ? c.right : c.left;
+    depth++;
+  }
+  const double* p = sc.P(shape);
+  V3 n;
+  switch (sc.type(shape)) {
+    case RB_SHAPE_BOX: n = box_normal(p, q, d); break;
+    case RB_SHAPE_TUBE: n = tube_normal(p, q, d); break;
+    case RB_SHAPE_SPHERE: n = sphere_normal(p, q, d); break;
+    case RB_SHAPE_PARABOLOID: n = para_normal(p, q, d); break;
+    case RB_SHAPE_PGON: n = pgon_normal(p, q, d); break;
+    default: n = v3(0, 0, 1);
+  }
+  while (ns > 0) n = l2mv(*stack[--ns], n);
+  return n;
+}
+
+// ---------------------------------------------------------------- physics
+// TGraph::Eval linear interpolation/extrapolation on x-ascending points
+static double table_eval(const rb_scene_desc* d, int table, double x) {
+  const rb_table_desc& t = d->tables[table];
+  const double* xy = d->table_xy + 2 * t.offset;
+  int n = t.n;
+  if (n == 0) return 0;
+  if (n == 1) return xy[1];
+  int low = -1, up = -1;
+  for (int i = 0; i < n; i++) {
+    double xi = xy[2 * i];
+    if (xi < x) low = i;
+    else if (xi > x) { up = i; break; }
+    else return xy[2 * i + 1];
+  }
+  if (up == -1) { up = low; low = low - 1; }
+  if (low == -1) { low = up; up = up + 1; }
+  double xl = xy[2 * low], yl = xy[2 * low + 1], xu = xy[2 * up], yu = xy[2 * up + 1];
+  if (xl == xu) return yl;
+  return yu + (x - xu) * (yl - yu) / (xl - xu);
+}
+static double mirror_reflectance(const rb_scene_desc* d, const rb_volume_desc& v, double lambda) {
+  double r = v.refl_table >= 0 ? table_eval(d, v.refl_table, lambda) : v.refl_const;
+  r = r > 1 ? 1 : r;
+  r = r < 0 ? 0 : r;
+  return r;
+}
+static double lens_index(const rb_scene_desc* d, const rb_volume_desc& v, double lambda) {
+  switch (v.index_model) {
+    case RB_INDEX_TABLE: return table_eval(d, v.index_ref, lambda);
+    case RB_INDEX_SELLMEIER: {
+      const double* b = d->params + v.index_ref;
+      double l = lambda / 1e-4;  // AOpticsManager::um()
+      double l2 = l * l;
+      return std::sqrt(1 + b[0] * l2 / (l2 - b[3]) + b[1] * l2 / (l2 - b[4]) + b[2] * l2 / (l2 - b[5]));
+    }
+    case RB_INDEX_SCHOTT: {
+      const double* a = d->params + v.index_ref;
+      double l = lambda / 1e-4;
+      return std::sqrt(a[0] + a[1] * std::pow(l, 2.) + a[2] * std::pow(l, -2.) + a[3] * std::pow(l, -4.) +
+                       a[4] * std::pow(l, -6.) + a[5] * std::pow(l, -8.));
+    }
+  }
+  return 1.0;
+}
+static double lens_k(const rb_scene_desc* d, const rb_volume_desc& v, double lambda) {
+  return v.k_table >= 0 ? table_eval(d, v.k_table, lambda) : 0.0;
+}
+static double lens_abs_length(const rb_scene_desc* d, const rb_volume_desc& v, double lambda) {
+  if (v.index_model == RB_INDEX_NONE && v.k_table < 0) return INFINITY;
+  double k = lens_k(d, v, lambda);
+  return k <= 0 ? INFINITY : lambda / (4 * kPi * k);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11), key = seed, counter = (id lo, id hi, draw, stream)
+static inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+    uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+    uint32_t h0 = uint32_t(p0 >> 32), l0 = uint32_t(p0);
+    uint32_t h1 = uint32_t(p1 >> 32), l1 = uint32_t(p1);
+    uint32_t n0 = h1 ^ c[1] ^ k0, n2 = h0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = l1; c[2] = n2; c[3] = l0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+struct Rng {
+  uint64_t seed, id;
+  uint32_t draw, stream;
+  double uniform() {  // (0, 1]
+    uint32_t c[4] = {uint32_t(id), uint32_t(id >> 32), draw++, stream};
+    philox(c, uint32_t(seed), uint32_t(seed >> 32));
+    uint64_t u = ((uint64_t(c[1]) << 32) | c[0]) >> 11;
+    return double(u + 1) * (1.0 / 9007199254740992.0);
+  }
+};
+
+// ---------------------------------------------------------------- navigator
+struct Nav {
+  const Scene* sc;
+  V3 p, d;            // current point and direction (fPoint, fDirection)
+  int cur;            // current node, -1 = outside (fIsOutside)
+  bool on_boundary;   // fIsOnBoundary
+  double step;        // fStep after the last FindNextBoundaryAndStep
+  // crossed surface of the last step, for FindNormal
+  int hit_node;
+  uint32_t hit_leaf;
+};
+
+static const rb_node_desc& node(const Scene& sc, int i) { return sc.d->nodes[i]; }
+static int node_shape(const Scene& sc, int i) { return sc.d->volumes[node(sc, i).volume].shape; }
+static bool node_contains(const Scene& sc, int i, V3 p) {
+  return contains(sc, node_shape(sc, i), m2l(node(sc, i).global, p));
+}
+
+// TGeoNavigator::SearchNode: from `start` go up until a node contains p
+// (skipping the containment test when start == skip), then down into the
+// first daughter (in order, excluding `skip`) that contains p.
+static int search_node(const Scene& sc, int start, V3 p, int skip, bool forced) {
+  int n = start;
+  bool inside = forced || (n == skip);
+  while (!inside) {
+    if (!node_contains(sc, n, p)) {
+      if (node(sc, n).parent < 0) return -1;
+      skip = n;
+      n = node(sc, n).parent;
+    } else {
+      inside = true;
+    }
+  }
+  for (;;) {
+    const rb_node_desc& nd = node(sc, n);
+    int found = -1;
+    for (int k = 0; k < nd.n_children; k++) {
+      int c = sc.d->children[nd.first_child + k];
+      if (c == skip) continue;
+      if (node_contains(sc, c, p)) {
+        found = c;
+        break;
+      }
+    }
+    if (found < 0) return n;
+    n = found;
+    skip = -1;
+  }
+}
+
+// TGeoNavigator::InitTrack / FindNode from the top
+static void init_track(Nav& nv, V3 p, V3 d) {
+  nv.p = p;
+  nv.d = d;
+  nv.on_boundary = false;
+  const Scene& sc = *nv.sc;
+  nv.cur = node_contains(sc, 0, p) ? search_node(sc, 0, p, -1, true) : -1;
+}
+
+// TGeoNavigator::CrossBoundaryAndLocate push: 100 (1 + |T| + step) tolerance
+static double locate_push(const Scene& sc, int n, double step) {
+  const double* t = node(sc, n).global.tr;
+  double trmax = 1. + std::fabs(t[0]) + std::fabs(t[1]) + std::fabs(t[2]);
+  return 100. * (trmax + step) * kTol;
+}
+
+// TGeoNavigator::FindNextBoundaryAndStep(); returns the next node (-1 = null)
+static int find_next_boundary_and_step(Nav& nv) {
+  const Scene& sc = *nv.sc;
+  double extra = nv.on_boundary ? kTol : 0.0;
+  nv.on_boundary = false;
+  nv.p = axpy(nv.p, extra, nv.d);
+  nv.hit_node = -1;
+  nv.hit_leaf = 0;
+  if (nv.cur < 0) {
+    const rb_node_desc& top = node(sc, 0);
+    Hit h = dist_out(sc, node_shape(sc, 0), m2l(top.global, nv.p), m2lv(top.global, nv.d));
+    if (h.s < kBig - kTol) {
+      nv.step = h.s + extra;
+      nv.p = axpy(nv.p, h.s, nv.d);
+      nv.on_boundary = true;
+      nv.hit_node = 0;
+      nv.hit_leaf = h.leaf;
+      V3 pp = axpy(nv.p, locate_push(sc, 0, nv.step), nv.d);
+      nv.cur = search_node(sc, 0, pp, -1, true);
+      return nv.cur;
+    }
+    nv.step = kBig;
+    return -1;
+  }
+  int cur = nv.cur;
+  const rb_node_desc& cn = node(sc, cur);
+  Hit hx = dist_in(sc, node_shape(sc, cur), m2l(cn.global, nv.p), m2lv(cn.global, nv.d));
+  if (hx.s <= kTol) {
+    nv.step = kTol;
+    nv.p = axpy(nv.p, nv.step, nv.d);
+    nv.on_boundary = true;
+    nv.hit_node = cur;
+    nv.hit_leaf = hx.leaf;
+    V3 pp = axpy(nv.p, locate_push(sc, cur, nv.step), nv.d);
+    nv.cur = search_node(sc, cur, pp, cur, false);
+    return nv.cur;
+  }
+  double best = kBig;
+  int crossed = -2;
+  uint32_t leaf = 0;
+  if (hx.s < best - kTol) {
+    best = hx.s;
+    crossed = -1;
+    leaf = hx.leaf;
+  }
+  for (int k = 0; k < cn.n_children; k++) {
+    int c = sc.d->children[cn.first_child + k];
+    const rb_node_desc& chn = node(sc, c);
+    Hit h = dist_out(sc, node_shape(sc, c), m2l(chn.global, nv.p), m2lv(chn.global, nv.d));
+    if (h.s < best - kTol) {
+      best = h.s;
+      crossed = c;
+      leaf = h.leaf;
+    }
+  }
+  nv.p = axpy(nv.p, best, nv.d);
+  nv.step = best + extra;
+  if (crossed == -2) return cur;
+  nv.on_boundary = true;
+  nv.hit_leaf = leaf;
+  if (crossed == -1) {
+    nv.hit_node = cur;
+    if (cn.parent < 0) {
+      nv.cur = -1;
+      return -1;
+    }
+    int mother = cn.parent;
+    V3 pp = axpy(nv.p, locate_push(sc, mother, nv.step), nv.d);
+    nv.cur = search_node(sc, mother, pp, cur, false);
+    return nv.cur;
+  }
+  nv.hit_node = crossed;
+  V3 pp = axpy(nv.p, locate_push(sc, crossed, nv.step), nv.d);
+  nv.cur = search_node(sc, crossed, pp, cur, true);
+  return nv.cur;
+}
+
+// TGeoNavigator::FindNormal: normal of the crossed surface, along d, master frame
+static V3 find_normal(const Nav& nv) {
+  const Scene& sc = *nv.sc;
+  if (nv.hit_node < 0) return v3(0, 0, 1);
+  const rb_node_desc& n = node(sc, nv.hit_node);
+  V3 lp = m2l(n.global, nv.p), ld = m2lv(n.global, nv.d);
+  V3 ln = shape_normal(sc, node_shape(sc, nv.hit_node), nv.hit_leaf, lp, ld);
+  return l2mv(n.global, ln);
+}
+
+// TGeoNavigator::Step() after SetStep(kEpsilon): moves fStep + 1e-6 and relocates
+static void step_back(Nav& nv, V3 back_dir) {
+  const Scene& sc = *nv.sc;
+  double s = kEpsilon + 1e-6;
+  nv.p = axpy(nv.p, s, back_dir);
+  int start = nv.cur < 0 ? 0 : nv.cur;
+  if (nv.cur < 0) {
+    nv.cur = node_contains(sc, 0, nv.p) ? search_node(sc, 0, nv.p, -1, true) : -1;
+  } else {
+    nv.cur = search_node(sc, start, nv.p, -1, false);
+  }
+  nv.on_boundary = true;
+}
+
+// ---------------------------------------------------------------- tracer
+struct Ray {
+  V3 x;      // last point
+  double t;
+  V3 d;      // direction (unit)
+  double lambda;
+  int status, npoints, last_node;
+  double* path;     // optional [cap][4]
+  int32_t* pnode;
+  int cap;
+  void add_point(V3 p, double tt, int nd) {
+    if (path && npoints - 1 < cap) {
+      double* s = path + 4 * (npoints - 1);
+      s[0] = p.x; s[1] = p.y; s[2] = p.z; s[3] = tt;
+      pnode[npoints - 1] = nd;
+    }
+    x = p;
+    t = tt;
+    last_node = nd;
+    npoints++;
+  }
+  void set_direction(V3 v) {  // ARay::SetDirection normalises (src/ARay.cxx:210-223)
+    double mag = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    if (mag > 0) d = v3(v.x / mag, v.y / mag, v.z / mag);
+  }
+};
+
+static int kind_of(const Scene& sc, int n) {
+  if (n < 0) return RB_KNULL;
+  return sc.d->volumes[node(sc, n).volume].kind;
+}
+static const rb_volume_desc& vol_of(const Scene& sc, int n) { return sc.d->volumes[node(sc, n).volume]; }
+
+struct Opts {
+  int limit;
+  bool fresnel;
+};
+
+// src/AOpticsManager.cxx:170-247
+static void do_reflection(double n1, Ray& ray, Nav& nv, int next, const V3* normal, Rng& rng) {
+  const Scene& sc = *nv.sc;
+  double step = nv.step;
+  V3 n = normal ? *normal : find_normal(nv);
+  V3 d1 = ray.d;
+  double cos1 = dot(d1, n);
+  bool absorbed = false;
+  if (kind_of(sc, next) == RB_KMIRROR) {
+    double ref = mirror_reflectance(sc.d, vol_of(sc, next), ray.lambda);
+    if (ref < rng.uniform()) {
+      absorbed = true;
+      ray.status = RB_ABSORB;
+    }
+  }
+  V3 d2 = v3(d1.x - 2 * n.x * cos1, d1.y - 2 * n.y * cos1, d1.z - 2 * n.z * cos1);
+  if (!absorbed) ray.set_direction(d2);
+  double t = ray.t + step / (kC / n1);
+  step_back(nv, mul(d1, -1));
+  nv.d = ray.d;
+  ray.add_point(nv.p, t, next);
+}
+
+// src/AOpticsManager.cxx:52-167 (multilayer branch out of scope)
+static void do_fresnel(double n1, double n2, double k2, Ray& ray, Nav& nv, int next, const Opts& o,
+                       Rng& rng) {
+  double step = nv.step;
+  V3 n = find_normal(nv);
+  V3 d1 = ray.d;
+  double cos1 = dot(d1, n);
+  double sin1 = std::sqrt(1 - cos1 * cos1);
+  double sin2 = n1 * sin1 / n2;
+  double cos2 = std::sqrt(1 - sin2 * sin2);
+  if (sin2 > 1.) {
+    do_reflection(n1, ray, nv, next, &n, rng);
+    return;
+  }
+  if (o.fresnel) {
+    double Rs, Rp;
+    auto sq = [](double v) { return v * v; };
+    if (k2 <= 0.) {
+      double eta1S = n1 * cos1, eta2S = n2 * cos2;
+      double eta1P = n1 / cos1, eta2P = n2 / cos2;
+      Rs = sq((eta1S - eta2S) / (eta1S + eta2S));
+      Rp = sq((eta1P - eta2P) / (eta1P + eta2P));
+    } else {
+      double eta1S = n1 * cos1, eta1P = n1 / cos1;
+      double x1S = eta1S, x1P = eta1P;
+      double u = sq(n2) - sq(k2) - sq(n1 * sin1);
+      double v = 2 * n2 * k2;
+      double tmp = std::sqrt(sq(u) + sq(v));
+      double cosxi2 = std::sqrt(1 + u / tmp) / std::sqrt(2.);
+      double sinxi2 = std::sqrt(1 - u / tmp) / std::sqrt(2.);
+      double x2S = std::sqrt(tmp) * cosxi2, y2S = std::sqrt(tmp) * sinxi2;
+      tmp = sq(x2S) + sq(y2S);
+      double x2P = (2 * n2 * k2 * y2S + (sq(n2) - sq(k2)) * x2S) / tmp;
+      double y2P = (2 * n2 * k2 * x2S - (sq(n2) - sq(k2)) * y2S) / tmp;
+      Rs = (sq(x1S - x2S) + sq(y2S)) / (sq(x1S + x2S) + sq(y2S));
+      Rp = (sq(x1P - x2P) + sq(y2P)) / (sq(x1P + x2P) + sq(y2P));
+    }
+    double R = (Rs + Rp) / 2.;
+    if (rng.uniform() < R) {
+      do_reflection(n1, ray, nv, next, &n, rng);
+      return;
+    }
+  }
+  V3 d2;
+  if (sin1 != 0) {
+    d2 = v3((d1.x - cos1 * n.x) * sin2 / sin1 + n.x * cos2, (d1.y - cos1 * n.y) * sin2 / sin1 + n.y * cos2,
+            (d1.z - cos1 * n.z) * sin2 / sin1 + n.z * cos2);
+  } else {
+    d2 = d1;
+  }
+  double t = ray.t + step / (kC / n1);
+  ray.add_point(nv.p, t, next);
+  ray.set_direction(d2);
+  nv.d = ray.d;
+}
+
+// src/AOpticsManager.cxx:346-519 for one ray
+static void trace_ray(const Scene& sc, const Opts& o, Ray& ray, Rng& rng) {
+  if (ray.status != RB_RUN) return;
+  Nav nv;
+  nv.sc = &sc;
+  nv.hit_node = -1;
+  nv.hit_leaf = 0;
+  nv.step = 0;
+  init_track(nv, ray.x, ray.d);
+  const double lambda = ray.lambda;
+  while (ray.status == RB_RUN) {
+    V3 x1 = ray.x;
+    double t1 = ray.t;
+    int current = nv.cur;
+    int next = find_next_boundary_and_step(nv);
+    double step = nv.step;
+    int tc = kind_of(sc, current), tn = kind_of(sc, next);
+
+    if (tc == RB_KLENS) {
+      double abs = lens_abs_length(sc.d, vol_of(sc, current), lambda);
+      if (abs > 0 && abs != INFINITY) {
+        double abs_step = -abs * std::log(rng.uniform());
+        if (abs_step < step) {
+          double n1 = lens_index(sc.d, vol_of(sc, current), lambda);
+          V3 x2 = axpy(x1, abs_step, ray.d);
+          ray.add_point(x2, t1 + abs_step / (kC / n1), next);
+          ray.status = RB_ABSORB;
+          continue;
+        }
+      }
+    }
+
+    bool c_free = tc == RB_KNULL || tc == RB_KOPT || tc == RB_KOTHER;
+    if ((c_free || tc == RB_KLENS) && tn == RB_KMIRROR) {
+      double n1 = tc == RB_KLENS ? lens_index(sc.d, vol_of(sc, current), lambda) : 1.;
+      do_reflection(n1, ray, nv, next, nullptr, rng);
+    } else if (c_free && tn == RB_KLENS) {
+      const rb_volume_desc& v = vol_of(sc, next);
+      do_fresnel(1, lens_index(sc.d, v, lambda), lens_k(sc.d, v, lambda), ray, nv, next, o, rng);
+    } else if ((c_free || tc == RB_KLENS) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
+      double n1 = tc == RB_KLENS ? lens_index(sc.d, vol_of(sc, current), lambda) : 1.;
+      ray.add_point(nv.p, t1 + step / (kC / n1), next);
+    } else if (c_free && (tn == RB_KOTHER || tn == RB_KOPT)) {
+      ray.add_point(nv.p, t1 + step / kC, next);
+    } else if (tc == RB_KLENS && tn == RB_KLENS) {
+      const rb_volume_desc& v = vol_of(sc, next);
+      do_fresnel(lens_index(sc.d, vol_of(sc, current), lambda), lens_index(sc.d, v, lambda),
+                 lens_k(sc.d, v, lambda), ray, nv, next, o, rng);
+    } else if (tc == RB_KLENS && (tn == RB_KNULL || tn == RB_KOPT || tn == RB_KOTHER)) {
+      do_fresnel(lens_index(sc.d, vol_of(sc, current), lambda), 1, 0, ray, nv, next, o, rng);
+    }
+
+    if (tn == RB_KNULL) {
+      ray.add_point(nv.p, t1 + step / kC, next);
+      ray.status = RB_EXIT;
+    } else if (tc == RB_KFOCUS || tc == RB_KOBS || tc == RB_KMIRROR || tn == RB_KOBS) {
+      ray.status = RB_STOP;
+    } else if (tn == RB_KFOCUS) {
+      const rb_volume_desc& v = vol_of(sc, next);
+      double angle = 0.;
+      if (v.qe_angle_table >= 0) {
+        V3 n = find_normal(nv);
+        angle = std::acos(dot(ray.d, n));
+      }
+      double qe = v.qe_lambda_table >= 0 ? table_eval(sc.d, v.qe_lambda_table, lambda) : 1.;
+      if (v.qe_angle_table >= 0) qe *= table_eval(sc.d, v.qe_angle_table, angle);
+      if (qe == 1 || rng.uniform() < qe) ray.status = RB_FOCUS;
+      else ray.status = RB_STOP;
+    }
+
+    if (ray.status == RB_RUN && ray.npoints >= o.limit) ray.status = RB_SUSPEND;
+  }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+static void trace_range(const Scene& sc, const Opts& o, const rb_trace_opts* opts, rb_rays* r, int64_t b,
+                        int64_t e) {
+  for (int64_t i = b; i < e; i++) {
+    Ray ray;
+    ray.x = v3(r->x[i], r->y[i], r->z[i]);
+    ray.t = r->t[i];
+    ray.lambda = r->lambda[i];
+    ray.status = r->status[i];
+    ray.npoints = r->npoints[i];
+    ray.last_node = r->last_node ? r->last_node[i] : -1;
+    ray.d = v3(r->dx[i], r->dy[i], r->dz[i]);  // already unit, as ARay stores it
+    ray.cap = r->path_capacity;
+    ray.path = ray.cap > 0 ? r->path_xyzt + int64_t(4) * ray.cap * i : nullptr;
+    ray.pnode = ray.cap > 0 ? r->path_node + int64_t(ray.cap) * i : nullptr;
+    Rng rng{opts->seed, opts->ray_id_base + uint64_t(i), 0u, 0u};
+    if (ray.status == RB_RUN) trace_ray(sc, o, ray, rng);
+    r->x[i] = ray.x.x; r->y[i] = ray.x.y; r->z[i] = ray.x.z; r->t[i] = ray.t;
+    r->dx[i] = ray.d.x; r->dy[i] = ray.d.y; r->dz[i] = ray.d.z;
+    r->status[i] = ray.status;
+    r->npoints[i] = ray.npoints;
+    if (r->last_node) r->last_node[i] = ray.last_node;
+  }
+}
+
+extern "C" {
+
+// Trace n rays with `nthreads` host threads in contiguous chunks of
+// (n / nthreads), the last taking the remainder (src/AOpticsManager.cxx:533-541).
+int oracle_trace(const rb_scene_desc* desc, const rb_trace_opts* opts, rb_rays* rays, int64_t n, int nthreads) {
+  if (!desc || !opts || !rays || n < 0) return -1;
+  Scene sc{desc};
+  Opts o{opts->limit > 0 ? opts->limit : 100, opts->fresnel != 0};
+  if (nthreads < 2 || n < 2 * nthreads) {
+    trace_range(sc, o, opts, rays, 0, n);
+    return 0;
+  }
+  std::vector<std::thread> th;
+  int64_t chunk = n / nthreads;
+  for (int k = 0; k < nthreads; k++) {
+    int64_t b = chunk * k, e = (k == nthreads - 1) ? n : chunk * (k + 1);
+    th.emplace_back(trace_range, std::cref(sc), std::cref(o), opts, rays, b, e);
+  }
+  for (auto& t : th) t.join();
+  return 0;
+}
+
+// Shape-level probes for tests (local frame of `shape`)
+int oracle_shape_query(const rb_scene_desc* desc, int shape, const double* p, const double* d, double* out) {
+  Scene sc{desc};
+  V3 q = v3(p[0], p[1], p[2]), dd = v3(d[0], d[1], d[2]);
+  out[0] = contains(sc, shape, q) ? 1.0 : 0.0;
+  Hit a = dist_out(sc, shape, q, dd);
+  Hit b = dist_in(sc, shape, q, dd);
+  out[1] = a.s;
+  out[2] = b.s;
+  V3 n = shape_normal(sc, shape, a.leaf, axpy(q, a.s < kBig ? a.s : 0, dd), dd);
+  out[3] = n.x; out[4] = n.y; out[5] = n.z;
+  return 0;
+}
+
+// Ray sources (host restatement of rbgpu_generate_rays): GRID follows
+// ARayShooter::Rectangle (src/ARayShooter.cxx:401-459) translated to
+// (-D sin th, 0, D cos th); DISK draws 4 Philox uniforms per ray (stream 1).
+int oracle_generate_rays(const rb_raygen_desc* g, rb_rays* r, int64_t n) {
+  for (int64_t i = 0; i < n; i++) {
+    double th, lam, px, py;
+    if (g->kind == RB_RAYGEN_GRID) {
+      int ns = g->n_side;
+      double delta = ns == 1 ? g->size / 2 : g->size / (ns - 1);
+      int64_t a = i / ns, b = i % ns;
+      px = a * delta - g->size / 2;
+      py = b * delta - g->size / 2;
+      th = g->theta_min;
+      lam = g->lambda_min;
+    } else {
+      Rng u{g->seed, uint64_t(i), 0u, 1u};
+      double u0 = u.uniform(), u1 = u.uniform(), u2 = u.uniform(), u3 = u.uniform();
+      th = g->theta_min + (g->theta_max - g->theta_min) * u2;
+      lam = g->lambda_min + (g->lambda_max - g->lambda_min) * u3;
+      double rr = 0.5 * g->size * std::sqrt(u0), ph = 2 * kPi * u1;
+      px = rr * std::cos(ph);
+      py = rr * std::sin(ph);
+    }
+    double st = std::sin(th), ct = std::cos(th), t2 = kPi - th;
+    double cx = -g->distance * st, cz = g->distance * ct;
+    if (g->kind == RB_RAYGEN_GRID) {
+      r->x[i] = px + cx; r->y[i] = py; r->z[i] = cz;
+    } else {
+      r->x[i] = cx + px * ct; r->y[i] = py; r->z[i] = cz + px * st;
+    }
+    r->t[i] = 0;
+    r->dx[i] = std::sin(t2); r->dy[i] = 0; r->dz[i] = std::cos(t2);
+    r->lambda[i] = lam;
+    r->status[i] = RB_RUN;
+    r->npoints[i] = 1;
+    if (r->last_node) r->last_node[i] = -1;
+  }
+  return 0;
+}
+
+double oracle_table_eval(const rb_scene_desc* desc, int table, double x) { return table_eval(desc, table, x); }
+
+double oracle_lens_index(const rb_scene_desc* desc, int volume, double lambda) {
+  return lens_index(desc, desc->volumes[volume], lambda);
+}
+
+void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  philox(c, key[0], key[1]);
+  for (int i = 0; i < 4; i++) out[i] = c[i];
+}
+
+double oracle_uniform(uint64_t seed, uint64_t id, uint32_t draw, uint32_t stream) {
+  Rng r{seed, id, draw, stream};
+  return r.uniform();
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+"""ctypes/numpy mirror of include/rbgpu.h (the librbgpu C ABI).
+
+Struct layouts here must match rbgpu.h byte for byte; tests/test_abi.py
+checks the sizes against the compiled library.
+"""
+import ctypes as C
+
+import numpy as np
+
+# ray status (include/ARay.h:26)
+RB_RUN, RB_STOP, RB_EXIT, RB_FOCUS, RB_SUSPEND, RB_ABSORB = range(6)
+STATUS_NAMES = ("running", "stopped", "exited", "focused", "suspended", "absorbed")
+
+# volume kinds (include/AOpticsManager.h:45-53)
+RB_KLENS, RB_KOBS, RB_KMIRROR, RB_KFOCUS, RB_KOPT, RB_KOTHER, RB_KNULL = range(7)
+
+RB_SHAPE_BOX = 1
+RB_SHAPE_TUBE = 2
+RB_SHAPE_SPHERE = 3
+RB_SHAPE_PARABOLOID = 4
+RB_SHAPE_PGON = 5
+RB_SHAPE_PCON = 6
+RB_SHAPE_COMPOSITE = 7
+RB_SHAPE_ASPHERIC = 8
+RB_SHAPE_WINSTON = 9
+
+RB_OP_UNION, RB_OP_SUBTRACTION, RB_OP_INTERSECTION = 0, 1, 2
+
+RB_INDEX_NONE, RB_INDEX_TABLE, RB_INDEX_SELLMEIER, RB_INDEX_SCHOTT = 0, 1, 2, 3
+
+RB_PRECISION_FP64, RB_PRECISION_MIXED = 0, 1
+
+RB_RAYGEN_GRID, RB_RAYGEN_DISK = 0, 1
+
+SHAPE_DTYPE = np.dtype(
+    [("type", "<i4"), ("param_offset", "<i4"), ("n_params", "<i4"), ("pad", "<i4"),
+     ("bbox_origin", "<f8", 3), ("bbox_half", "<f8", 3)], align=True)
+VOLUME_DTYPE = np.dtype(
+    [("refl_const", "<f8"), ("shape", "<i4"), ("kind", "<i4"), ("refl_table", "<i4"),
+     ("index_model", "<i4"), ("index_ref", "<i4"), ("k_table", "<i4"),
+     ("qe_lambda_table", "<i4"), ("qe_angle_table", "<i4")], align=True)
+NODE_DTYPE = np.dtype(
+    [("rot", "<f8", 9), ("tr", "<f8", 3), ("volume", "<i4"), ("parent", "<i4"),
+     ("first_child", "<i4"), ("n_children", "<i4")], align=True)
+TABLE_DTYPE = np.dtype([("offset", "<i4"), ("n", "<i4")], align=True)
+
+assert SHAPE_DTYPE.itemsize == 64
+assert VOLUME_DTYPE.itemsize == 40
+assert NODE_DTYPE.itemsize == 112
+assert TABLE_DTYPE.itemsize == 8
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class rb_scene_desc(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int32), ("nodes", C.c_void_p),
+        ("n_children", C.c_int32), ("children", C.c_void_p),
+        ("n_volumes", C.c_int32), ("volumes", C.c_void_p),
+        ("n_shapes", C.c_int32), ("shapes", C.c_void_p),
+        ("n_params", C.c_int32), ("params", C.c_void_p),
+        ("n_tables", C.c_int32), ("tables", C.c_void_p),
+        ("n_table_xy", C.c_int32), ("table_xy", C.c_void_p),
+    ]
+
+
+class rb_rays(C.Structure):
+    _fields_ = [
+        ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("t", C.c_void_p),
+        ("dx", C.c_void_p), ("dy", C.c_void_p), ("dz", C.c_void_p),
+        ("lambda_", C.c_void_p), ("status", C.c_void_p), ("npoints", C.c_void_p),
+        ("last_node", C.c_void_p),
+        ("path_capacity", C.c_int32), ("pad", C.c_int32),
+        ("path_xyzt", C.c_void_p), ("path_node", C.c_void_p),
+    ]
+
+
+class rb_trace_opts(C.Structure):
+    _fields_ = [
+        ("limit", C.c_int32), ("fresnel", C.c_int32), ("precision", C.c_int32),
+        ("device_ptrs", C.c_int32), ("seed", C.c_uint64), ("ray_id_base", C.c_uint64),
+    ]
+
+
+class rb_raygen_desc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("n_side", C.c_int32), ("size", C.c_double),
+        ("distance", C.c_double), ("theta_min", C.c_double), ("theta_max", C.c_double),
+        ("lambda_min", C.c_double), ("lambda_max", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+def ptr(a):
+    """Address of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data

@@ -1,0 +1,144 @@
+"""Loader and thin ctypes wrapper for librbgpu.so (the HIP tracer).
+
+There is no CPU fallback: if the library is missing or no GPU is visible,
+every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi as A
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librbgpu.so")
+
+_lib = None
+
+EXPORTS = (
+    "rbgpu_ctx_create", "rbgpu_ctx_destroy", "rbgpu_last_error", "rbgpu_scene_upload",
+    "rbgpu_scene_free", "rbgpu_trace", "rbgpu_generate_rays", "rbgpu_last_trace_ms",
+    "rbgpu_last_trace_launches", "rbgpu_abi_sizes",
+)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"librbgpu.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    lib.rbgpu_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    lib.rbgpu_ctx_create.restype = C.c_int
+    lib.rbgpu_ctx_destroy.argtypes = [C.c_void_p]
+    lib.rbgpu_ctx_destroy.restype = None
+    lib.rbgpu_last_error.argtypes = [C.c_void_p]
+    lib.rbgpu_last_error.restype = C.c_char_p
+    lib.rbgpu_scene_upload.argtypes = [C.c_void_p, C.POINTER(A.rb_scene_desc), C.POINTER(C.c_void_p)]
+    lib.rbgpu_scene_upload.restype = C.c_int
+    lib.rbgpu_scene_free.argtypes = [C.c_void_p]
+    lib.rbgpu_scene_free.restype = None
+    lib.rbgpu_trace.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays),
+                                C.c_int64]
+    lib.rbgpu_trace.restype = C.c_int
+    lib.rbgpu_generate_rays.argtypes = [C.c_void_p, C.POINTER(A.rb_raygen_desc), C.POINTER(A.rb_rays), C.c_int64]
+    lib.rbgpu_generate_rays.restype = C.c_int
+    lib.rbgpu_last_trace_ms.argtypes = [C.c_void_p]
+    lib.rbgpu_last_trace_ms.restype = C.c_double
+    lib.rbgpu_last_trace_launches.argtypes = [C.c_void_p]
+    lib.rbgpu_last_trace_launches.restype = C.c_int
+    lib.rbgpu_abi_sizes.argtypes = [C.POINTER(C.c_int32)]
+    lib.rbgpu_abi_sizes.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+class RbError(RuntimeError):
+    pass
+
+
+class Context:
+    """One librbgpu context (a HIP device + stream)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.rbgpu_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise RbError(f"rbgpu_ctx_create failed ({rc}); a HIP device is required")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.rbgpu_last_error(self.h)
+            raise RbError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def upload(self, flat):
+        h = C.c_void_p()
+        self._check(self.lib.rbgpu_scene_upload(self.h, C.byref(flat.desc), C.byref(h)), "rbgpu_scene_upload")
+        return Scene(self, h)
+
+    def trace(self, scene, rays, n, opts):
+        self._check(self.lib.rbgpu_trace(self.h, scene.h, C.byref(opts), C.byref(rays), C.c_int64(n)),
+                    "rbgpu_trace")
+
+    def generate(self, gen, rays, n):
+        self._check(self.lib.rbgpu_generate_rays(self.h, C.byref(gen), C.byref(rays), C.c_int64(n)),
+                    "rbgpu_generate_rays")
+
+    def last_trace_ms(self):
+        return self.lib.rbgpu_last_trace_ms(self.h)
+
+    def last_trace_launches(self):
+        return self.lib.rbgpu_last_trace_launches(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.rbgpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.rbgpu_scene_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def make_rays(soa, path_xyzt=None, path_node=None):
+    """rb_rays over host numpy SoA arrays (dict of field -> array)."""
+    r = A.rb_rays()
+    for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_", "status", "npoints", "last_node"):
+        a = soa[f]
+        setattr(r, f, A.ptr(a))
+    if path_xyzt is not None:
+        r.path_capacity = path_xyzt.shape[1]
+        r.path_xyzt = A.ptr(path_xyzt)
+        r.path_node = A.ptr(path_node)
+    return r
+
+
+def make_rays_device(t):
+    """rb_rays over torch device tensors (dict of field -> tensor)."""
+    r = A.rb_rays()
+    for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_", "status", "npoints", "last_node"):
+        setattr(r, f, t[f].data_ptr())
+    if "path_xyzt" in t:
+        r.path_capacity = t["path_xyzt"].shape[1]
+        r.path_xyzt = t["path_xyzt"].data_ptr()
+        r.path_node = t["path_node"].data_ptr()
+    return r

@@ -1,0 +1,806 @@
+// rb_device.h -- device-side tracer for librbgpu (gfx950).
+//
+// One lane owns one ray. Per segment the lane (1) finds the next boundary in
+// its current node: exit distance of the node's shape plus entry distances of
+// the daughters that a BVH over their global AABBs cannot cull, (2) crosses
+// and relocates the way TGeoNavigator::FindNextBoundaryAndStep /
+// CrossBoundaryAndLocate do, (3) applies the optics of
+// AOpticsManager::TraceNonSequential (src/AOpticsManager.cxx:335-520).
+// Shape code is templated on the arithmetic type T (double for the plain
+// form); the scene itself is always FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rbgpu.h"
+
+namespace rbd {
+
+constexpr double kBig = 1e30;        // TGeoShape::Big()
+constexpr double kTol = 1e-10;       // TGeoShape::Tolerance()
+constexpr double kEps = 1e-6;        // AOpticsManager kEpsilon (src/AOpticsManager.cxx:20)
+constexpr double kC = 2.99792458e10; // speed of light, cm/s
+constexpr double kBoolPush = 1e-8;   // inside-test offset in boolean-shape iteration
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kMaxCsgDepth = 3;
+constexpr int kBvhStack = 32;
+
+// ------------------------------------------------------------ scene on HBM
+struct DNode {                 // 128 B
+  double rot[9];
+  double tr[3];
+  int32_t shape, kind, volume, parent;
+  int32_t first_child, n_children, bvh_root, pad;
+};
+struct DBvh {                  // 32 B, FP32 boxes expanded outward
+  float lo[3];
+  int32_t left;                // inner: left child index (right = left + 1); leaf: first prim
+  float hi[3];
+  int32_t count;               // 0 = inner node, > 0 = leaf with `count` prims
+};
+struct DShape {
+  int32_t type, off;
+};
+struct DScene {
+  const DNode* nodes;
+  const int32_t* children;
+  const DBvh* bvh;
+  const int32_t* bvh_prims;    // node ids referenced by BVH leaves
+  const DShape* shapes;
+  const double* params;
+  const rb_volume_desc* volumes;
+  const rb_table_desc* tables;
+  const double* table_xy;
+  int32_t n_nodes;
+};
+
+template <typename T>
+struct V3 {
+  T x, y, z;
+};
+template <typename T> __device__ __forceinline__ V3<T> mk(T x, T y, T z) { return {x, y, z}; }
+template <typename T> __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> __device__ __forceinline__ V3<T> axpy(V3<T> p, T s, V3<T> d) {
+  return {p.x + s * d.x, p.y + s * d.y, p.z + s * d.z};
+}
+template <typename T> __device__ __forceinline__ V3<T> scale(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+
+// master -> local of a 4x3 matrix stored as rot[9], tr[3] (master = R local + T)
+template <typename T>
+__device__ __forceinline__ V3<T> m2l(const double* rot, const double* tr, V3<T> p) {
+  T qx = p.x - T(tr[0]), qy = p.y - T(tr[1]), qz = p.z - T(tr[2]);
+  return {T(rot[0]) * qx + T(rot[3]) * qy + T(rot[6]) * qz, T(rot[1]) * qx + T(rot[4]) * qy + T(rot[7]) * qz,
+          T(rot[2]) * qx + T(rot[5]) * qy + T(rot[8]) * qz};
+}
+template <typename T>
+__device__ __forceinline__ V3<T> m2lv(const double* rot, V3<T> q) {
+  return {T(rot[0]) * q.x + T(rot[3]) * q.y + T(rot[6]) * q.z, T(rot[1]) * q.x + T(rot[4]) * q.y + T(rot[7]) * q.z,
+          T(rot[2]) * q.x + T(rot[5]) * q.y + T(rot[8]) * q.z};
+}
+template <typename T>
+__device__ __forceinline__ V3<T> l2mv(const double* rot, V3<T> q) {
+  return {T(rot[0]) * q.x + T(rot[1]) * q.y + T(rot[2]) * q.z, T(rot[3]) * q.x + T(rot[4]) * q.y + T(rot[5]) * q.z,
+          T(rot[6]) * q.x + T(rot[7]) * q.y + T(rot[8]) * q.z};
+}
+
+struct Hit {
+  double s;
+  uint32_t leaf;   // composite path: bit k = right child at depth k; depth in bits 24..31
+};
+__device__ __forceinline__ uint32_t leaf_push(uint32_t leaf, int side) {
+  uint32_t depth = leaf >> 24;
+  return ((depth + 1u) << 24) | (((leaf & 0xffffffu) << 1) | uint32_t(side));
+}
+
+// ------------------------------------------------------------ box
+// params: dx dy dz ox oy oz
+template <typename T> __device__ bool box_contains(const double* p, V3<T> q) {
+  return fabs(q.x - T(p[3])) <= T(p[0]) && fabs(q.y - T(p[4])) <= T(p[1]) && fabs(q.z - T(p[5])) <= T(p[2]);
+}
+template <typename T> __device__ T box_dist_in(const double* p, V3<T> q, V3<T> d) {
+  T n[3] = {q.x - T(p[3]), q.y - T(p[4]), q.z - T(p[5])};
+  T dd[3] = {d.x, d.y, d.z};
+  T smin = T(kBig);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (dd[i] != T(0)) {
+      T s = (dd[i] > T(0) ? T(p[i]) - n[i] : T(p[i]) + n[i]) / fabs(dd[i]);
+      smin = s < smin ? s : smin;
+    }
+  }
+  return smin < T(0) ? T(0) : smin;
+}
+template <typename T> __device__ T box_dist_out(const double* p, V3<T> q, V3<T> d) {
+  T n[3] = {q.x - T(p[3]), q.y - T(p[4]), q.z - T(p[5])};
+  T dd[3] = {d.x, d.y, d.z};
+  T h[3] = {T(p[0]), T(p[1]), T(p[2])};
+  T saf[3];
+  bool in = true;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    saf[i] = fabs(n[i]) - h[i];
+    in = in && !(saf[i] > T(0));
+  }
+  if (in) return T(0);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (saf[i] < T(0) || n[i] * dd[i] >= T(0)) continue;
+    T s = saf[i] / fabs(dd[i]);
+    int j = (i + 1) % 3, k = (i + 2) % 3;
+    if (fabs(n[j] + s * dd[j]) > h[j] || fabs(n[k] + s * dd[k]) > h[k]) continue;
+    return s;
+  }
+  return T(kBig);
+}
+template <typename T> __device__ V3<T> box_normal(const double* p, V3<T> q, V3<T> d) {
+  T s0 = fabs(fabs(q.x - T(p[3])) - T(p[0])), s1 = fabs(fabs(q.y - T(p[4])) - T(p[1])),
+    s2 = fabs(fabs(q.z - T(p[5])) - T(p[2]));
+  int i = 0;
+  T m = s0;
+  if (s1 < m) { i = 1; m = s1; }
+  if (s2 < m) { i = 2; }
+  if (i == 0) return mk<T>(d.x > T(0) ? T(1) : T(-1), T(0), T(0));
+  if (i == 1) return mk<T>(T(0), d.y > T(0) ? T(1) : T(-1), T(0));
+  return mk<T>(T(0), T(0), d.z > T(0) ? T(1) : T(-1));
+}
+
+// ------------------------------------------------------------ tube
+// params: rmin rmax dz
+template <typename T> __device__ bool cyl_roots(V3<T> q, V3<T> d, T R, T& s1, T& s2) {
+  T nsq = d.x * d.x + d.y * d.y;
+  if (nsq < T(1e-20)) return false;
+  T b = (q.x * d.x + q.y * d.y) / nsq;
+  T c = (q.x * q.x + q.y * q.y - R * R) / nsq;
+  T disc = b * b - c;
+  if (disc <= T(0)) return false;
+  T sq = sqrt(disc);
+  s1 = -b - sq;
+  s2 = -b + sq;
+  return true;
+}
+template <typename T> __device__ bool tube_contains(const double* p, V3<T> q) {
+  if (fabs(q.z) > T(p[2])) return false;
+  T r2 = q.x * q.x + q.y * q.y;
+  return r2 >= T(p[0]) * T(p[0]) && r2 <= T(p[1]) * T(p[1]);
+}
+template <typename T> __device__ T tube_dist_in(const double* p, V3<T> q, V3<T> d) {
+  T rmin = T(p[0]), rmax = T(p[1]), dz = T(p[2]);
+  T s = T(kBig);
+  if (d.z != T(0)) {
+    s = ((d.z > T(0) ? dz : -dz) - q.z) / d.z;
+    if (s <= T(0)) return T(0);
+  }
+  T s1, s2;
+  if (rmin > T(0) && cyl_roots(q, d, rmin, s1, s2) && s1 > T(0)) s = s1 < s ? s1 : s;
+  if (cyl_roots(q, d, rmax, s1, s2)) {
+    if (s2 > T(0)) s = s2 < s ? s2 : s;
+    else s = T(0);
+  } else if (d.x * d.x + d.y * d.y >= T(1e-20)) {
+    s = T(0);
+  }
+  return s < T(0) ? T(0) : s;
+}
+template <typename T> __device__ T tube_dist_out(const double* p, V3<T> q, V3<T> d) {
+  T rmin = T(p[0]), rmax = T(p[1]), dz = T(p[2]);
+  T best = T(kBig);
+  if (fabs(q.z) >= dz && q.z * d.z < T(0)) {
+    T s = (fabs(q.z) - dz) / fabs(d.z);
+    T x = q.x + s * d.x, y = q.y + s * d.y, r2 = x * x + y * y;
+    if (r2 >= rmin * rmin && r2 <= rmax * rmax) best = s;
+  }
+  T s1, s2;
+  T r2 = q.x * q.x + q.y * q.y;
+  if (r2 >= rmax * rmax && cyl_roots(q, d, rmax, s1, s2) && s1 >= T(0) && s1 < best) {
+    if (fabs(q.z + s1 * d.z) <= dz) best = s1;
+  }
+  if (rmin > T(0) && r2 <= rmin * rmin && cyl_roots(q, d, rmin, s1, s2) && s2 >= T(0) && s2 < best) {
+    if (fabs(q.z + s2 * d.z) <= dz) best = s2;
+  }
+  return best;
+}
+template <typename T> __device__ V3<T> tube_normal(const double* p, V3<T> q, V3<T> d) {
+  T r = sqrt(q.x * q.x + q.y * q.y);
+  T a = fabs(T(p[2]) - fabs(q.z));
+  T b = T(p[0]) > T(1e-10) ? fabs(r - T(p[0])) : T(kBig);
+  T c = fabs(T(p[1]) - r);
+  int i = 0;
+  T m = a;
+  if (b < m) { i = 1; m = b; }
+  if (c < m) { i = 2; }
+  if (i == 0) return mk<T>(T(0), T(0), d.z >= T(0) ? T(1) : T(-1));
+  if (r <= T(0)) return mk<T>(T(1), T(0), T(0));
+  V3<T> n = mk<T>(q.x / r, q.y / r, T(0));
+  if (n.x * d.x + n.y * d.y < T(0)) n = scale(n, T(-1));
+  return n;
+}
+
+// ------------------------------------------------------------ sphere
+// params: rmin rmax theta1 theta2 (deg); constraint g <= 0 inside:
+//   0: r - rmax   1: rmin - r   2: z - c1 r (theta >= theta1)   3: c2 r - z (theta <= theta2)
+template <typename T> struct SphP {
+  T rmin, rmax, c1, c2;
+  bool t1, t2;
+};
+template <typename T> __device__ SphP<T> sph_p(const double* p) {
+  SphP<T> s;
+  s.rmin = T(p[0]);
+  s.rmax = T(p[1]);
+  s.t1 = p[2] > 0.0;
+  s.t2 = p[3] < 180.0;
+  s.c1 = T(cos(p[2] * (kPi / 180.0)));
+  s.c2 = T(cos(p[3] * (kPi / 180.0)));
+  return s;
+}
+template <typename T> __device__ bool sphere_contains(const double* p, V3<T> q) {
+  SphP<T> s = sph_p<T>(p);
+  T r2 = dot(q, q);
+  if (r2 < s.rmin * s.rmin || r2 > s.rmax * s.rmax) return false;
+  if (r2 < T(1e-20)) return true;
+  T r = sqrt(r2);
+  if (s.t1 && q.z > s.c1 * r) return false;
+  if (s.t2 && q.z < s.c2 * r) return false;
+  return true;
+}
+// evaluate constraint k and its rate along d at x
+template <typename T> __device__ __forceinline__ void sph_g(const SphP<T>& s, V3<T> x, V3<T> d, int k, T& g, T& dg) {
+  T r = sqrt(dot(x, x));
+  T rd = r > T(0) ? dot(x, d) / r : T(0);
+  if (k == 0) { g = r - s.rmax; dg = rd; }
+  else if (k == 1) { g = s.rmin - r; dg = -rd; }
+  else if (k == 2) { g = x.z - s.c1 * r; dg = d.z - s.c1 * rd; }
+  else { g = s.c2 * r - x.z; dg = s.c2 * rd - d.z; }
+}
+// A candidate root t on surface w: accept for dist_out if entering through w
+// and every other active constraint holds; for dist_in if leaving through w.
+template <typename T, bool OUT>
+__device__ __forceinline__ void sph_try(const SphP<T>& s, V3<T> q, V3<T> d, T t, int w, T& best) {
+  if (!(t >= T(0)) || t >= best) return;
+  V3<T> x = axpy(q, t, d);
+  T g, dg;
+  sph_g(s, x, d, w, g, dg);
+  if (OUT) {
+    if (dg >= T(0)) return;
+    T lim = T(1e-9) * (T(1) + s.rmax);
+    for (int k = 0; k < 4; k++) {
+      if (k == w) continue;
+      if (k == 1 && !(s.rmin > T(0))) continue;
+      if (k == 2 && !s.t1) continue;
+      if (k == 3 && !s.t2) continue;
+      T gk, dgk;
+      sph_g(s, x, d, k, gk, dgk);
+      if (gk > lim) return;
+    }
+  } else {
+    if (dg <= T(0)) return;
+  }
+  best = t;
+}
+template <typename T, bool OUT>
+__device__ void sph_cone(const SphP<T>& s, V3<T> q, V3<T> d, T c, int w, T& best) {
+  if (fabs(c) < T(1e-15)) {
+    if (d.z != T(0)) sph_try<T, OUT>(s, q, d, -q.z / d.z, w, best);
+    return;
+  }
+  T c2 = c * c;
+  T A = d.z * d.z - c2 * dot(d, d);
+  T B = T(2) * (q.z * d.z - c2 * dot(q, d));
+  T C = q.z * q.z - c2 * dot(q, q);
+  T r0 = T(-1), r1 = T(-1);
+  int m = 0;
+  if (fabs(A) < T(1e-15)) {
+    if (B != T(0)) { r0 = -C / B; m = 1; }
+  } else {
+    T disc = B * B - T(4) * A * C;
+    if (disc < T(0)) return;
+    T sq = sqrt(disc);
+    T qq = T(-0.5) * (B + (B >= T(0) ? sq : -sq));
+    r0 = qq / A;
+    m = 1;
+    if (qq != T(0)) { r1 = C / qq; m = 2; }
+  }
+  if (m == 2 && r0 > r1) { T t = r0; r0 = r1; r1 = t; }
+  for (int i = 0; i < m; i++) {
+    T r = i == 0 ? r0 : r1;
+    T z = q.z + r * d.z;
+    if ((c > T(0) && z >= T(0)) || (c < T(0) && z <= T(0))) sph_try<T, OUT>(s, q, d, r, w, best);
+  }
+}
+template <typename T, bool OUT> __device__ T sphere_dist(const double* p, V3<T> q, V3<T> d) {
+  SphP<T> s = sph_p<T>(p);
+  T best = T(kBig);
+  T b = dot(q, d), qq = dot(q, q);
+  T disc = b * b - (qq - s.rmax * s.rmax);
+  if (disc > T(0)) {
+    T sq = sqrt(disc);
+    sph_try<T, OUT>(s, q, d, -b - sq, 0, best);
+    sph_try<T, OUT>(s, q, d, -b + sq, 0, best);
+  }
+  if (s.rmin > T(0)) {
+    disc = b * b - (qq - s.rmin * s.rmin);
+    if (disc > T(0)) {
+      T sq = sqrt(disc);
+      sph_try<T, OUT>(s, q, d, -b - sq, 1, best);
+      sph_try<T, OUT>(s, q, d, -b + sq, 1, best);
+    }
+  }
+  if (s.t1) sph_cone<T, OUT>(s, q, d, s.c1, 2, best);
+  if (s.t2) sph_cone<T, OUT>(s, q, d, s.c2, 3, best);
+  if (!OUT && best >= T(kBig)) best = T(0);
+  return best;
+}
+template <typename T> __device__ V3<T> sphere_normal(const double* p, V3<T> q, V3<T> d) {
+  SphP<T> s = sph_p<T>(p);
+  T r = sqrt(dot(q, q));
+  T rho = sqrt(q.x * q.x + q.y * q.y);
+  T st1 = sqrt(fmax(T(0), T(1) - s.c1 * s.c1)), st2 = sqrt(fmax(T(0), T(1) - s.c2 * s.c2));
+  T dist[4] = {fabs(r - s.rmax), s.rmin > T(0) ? fabs(r - s.rmin) : T(kBig),
+               s.t1 ? fabs(rho * s.c1 - q.z * st1) : T(kBig), s.t2 ? fabs(rho * s.c2 - q.z * st2) : T(kBig)};
+  int k = 0;
+  for (int i = 1; i < 4; i++)
+    if (dist[i] < dist[k]) k = i;
+  V3<T> n;
+  if (k <= 1) {
+    n = r > T(0) ? scale(q, T(1) / r) : mk<T>(T(0), T(0), T(1));
+  } else {
+    T c = k == 2 ? s.c1 : s.c2, st = k == 2 ? st1 : st2;
+    T cp = rho > T(0) ? q.x / rho : T(1), sp = rho > T(0) ? q.y / rho : T(0);
+    n = mk<T>(-c * cp, -c * sp, st);
+  }
+  if (dot(n, d) < T(0)) n = scale(n, T(-1));
+  return n;
+}
+
+// ------------------------------------------------------------ paraboloid
+// params: rlo rhi dz a b ; z = a r^2 + b, solid: |z| <= dz and a (z - b) >= a^2 r^2
+template <typename T> __device__ bool para_contains(const double* p, V3<T> q) {
+  if (fabs(q.z) > T(p[2])) return false;
+  T a = T(p[3]), b = T(p[4]);
+  T aa = a * (q.z - b);
+  if (aa < T(0)) return false;
+  return aa >= a * a * (q.x * q.x + q.y * q.y);
+}
+template <typename T> __device__ int para_roots(const double* p, V3<T> q, V3<T> d, T& r0, T& r1) {
+  T a = T(p[3]), b = T(p[4]);
+  T A = a * (d.x * d.x + d.y * d.y);
+  T B = T(2) * a * (q.x * d.x + q.y * d.y) - d.z;
+  T C = a * (q.x * q.x + q.y * q.y) + b - q.z;
+  if (fabs(A) < T(1e-30)) {
+    if (B == T(0)) return 0;
+    r0 = -C / B;
+    return 1;
+  }
+  T disc = B * B - T(4) * A * C;
+  if (disc < T(0)) return 0;
+  T sq = sqrt(disc);
+  T qq = T(-0.5) * (B + (B >= T(0) ? sq : -sq));
+  r0 = qq / A;
+  if (qq == T(0)) return 1;
+  r1 = C / qq;
+  if (r0 > r1) { T t = r0; r0 = r1; r1 = t; }
+  return 2;
+}
+template <typename T> __device__ T para_dist_out(const double* p, V3<T> q, V3<T> d) {
+  T rlo = T(p[0]), rhi = T(p[1]), dz = T(p[2]), a = T(p[3]);
+  T best = T(kBig);
+  if (q.z <= -dz && d.z > T(0)) {
+    T s = -(dz + q.z) / d.z;
+    T x = q.x + s * d.x, y = q.y + s * d.y;
+    if (x * x + y * y <= rlo * rlo) best = s;
+  } else if (q.z >= dz && d.z < T(0)) {
+    T s = (dz - q.z) / d.z;
+    T x = q.x + s * d.x, y = q.y + s * d.y;
+    if (x * x + y * y <= rhi * rhi) best = s;
+  }
+  T r[2];
+  int n = para_roots(p, q, d, r[0], r[1]);
+  for (int i = 0; i < n; i++) {
+    if (r[i] < T(0) || r[i] >= best) continue;
+    V3<T> x = axpy(q, r[i], d);
+    if (fabs(x.z) > dz) continue;
+    T rate = a * d.z - T(2) * a * a * (x.x * d.x + x.y * d.y);
+    if (rate <= T(0)) continue;
+    best = r[i];
+    break;
+  }
+  return best;
+}
+template <typename T> __device__ T para_dist_in(const double* p, V3<T> q, V3<T> d) {
+  T dz = T(p[2]), a = T(p[3]);
+  T best = T(kBig);
+  if (d.z > T(0)) best = (dz - q.z) / d.z;
+  else if (d.z < T(0)) best = (-dz - q.z) / d.z;
+  if (best < T(0)) best = T(0);
+  T r[2];
+  int n = para_roots(p, q, d, r[0], r[1]);
+  for (int i = 0; i < n; i++) {
+    if (r[i] < T(0) || r[i] >= best) continue;
+    V3<T> x = axpy(q, r[i], d);
+    T rate = a * d.z - T(2) * a * a * (x.x * d.x + x.y * d.y);
+    if (rate >= T(0)) continue;
+    best = r[i];
+    break;
+  }
+  return best;
+}
+template <typename T> __device__ V3<T> para_normal(const double* p, V3<T> q, V3<T> d) {
+  T dz = T(p[2]), a = T(p[3]);
+  T r2 = q.x * q.x + q.y * q.y;
+  T zs = a * r2 + T(p[4]);
+  T gl = sqrt(T(1) + T(4) * a * a * r2);
+  T dpar = fabs(q.z - zs) / gl;
+  T dtop = fabs(dz - q.z), dbot = fabs(q.z + dz);
+  V3<T> n;
+  if (dpar <= dtop && dpar <= dbot) n = mk<T>(T(2) * a * q.x / gl, T(2) * a * q.y / gl, T(-1) / gl);
+  else n = mk<T>(T(0), T(0), T(1));
+  if (dot(n, d) < T(0)) n = scale(n, T(-1));
+  return n;
+}
+
+// ------------------------------------------------------------ pgon (rmin = 0, full phi)
+// params: phi1 dphi nedges nz {z rmin rmax}*nz. Each z segment is a convex
+// frustum: half-spaces n_k.(x,y) - slope (z - z0) <= r0 inside the z slab.
+template <typename T> __device__ __forceinline__ void pgon_face(const double* p, int k, T& cx, T& sy) {
+  double div = p[1] / p[2];
+  double ph = (p[0] + div * (k + 0.5)) * (kPi / 180.0);
+  cx = T(cos(ph));
+  sy = T(sin(ph));
+}
+template <typename T> __device__ bool pgon_clip(const double* p, int i, V3<T> q, V3<T> d, T& t0, T& t1) {
+  const double* z = p + 4;
+  T z0 = T(z[3 * i]), z1 = T(z[3 * (i + 1)]);
+  if (!(z1 - z0 > T(0))) return false;
+  T r0 = T(z[3 * i + 2]);
+  T slope = (T(z[3 * (i + 1) + 2]) - r0) / (z1 - z0);
+  t0 = T(-kBig);
+  t1 = T(kBig);
+  if (d.z != T(0)) {
+    T a = (z0 - q.z) / d.z, b = (z1 - q.z) / d.z;
+    if (a > b) { T t = a; a = b; b = t; }
+    t0 = a > t0 ? a : t0;
+    t1 = b < t1 ? b : t1;
+  } else if (q.z < z0 || q.z > z1) {
+    return false;
+  }
+  int ne = int(p[2]);
+  for (int k = 0; k < ne; k++) {
+    T cx, sy;
+    pgon_face(p, k, cx, sy);
+    T f0 = cx * q.x + sy * q.y - slope * (q.z - z0) - r0;
+    T fd = cx * d.x + sy * d.y - slope * d.z;
+    if (fd == T(0)) {
+      if (f0 > T(0)) return false;
+      continue;
+    }
+    T t = -f0 / fd;
+    if (fd > T(0)) t1 = t < t1 ? t : t1;
+    else t0 = t > t0 ? t : t0;
+  }
+  return t0 <= t1;
+}
+template <typename T> __device__ bool pgon_contains(const double* p, V3<T> q) {
+  int nz = int(p[3]);
+  const double* z = p + 4;
+  if (q.z < T(z[0]) || q.z > T(z[3 * (nz - 1)])) return false;
+  double div = p[1] / p[2];
+  double phi = atan2(double(q.y), double(q.x)) * (180.0 / kPi);
+  while (phi < p[0]) phi += 360.0;
+  double ddp = phi - p[0];
+  if (ddp > p[1]) return false;
+  int ip = min(int(ddp / div), int(p[2]) - 1);
+  double ph0 = (p[0] + div * (ip + 0.5)) * (kPi / 180.0);
+  T r = q.x * T(cos(ph0)) + q.y * T(sin(ph0));
+  for (int i = 0; i + 1 < nz; i++) {
+    T za = T(z[3 * i]), zb = T(z[3 * (i + 1)]);
+    if (q.z < za || q.z > zb) continue;
+    T rmax;
+    if (zb - za < T(1e-8)) {
+      rmax = T(fmax(z[3 * i + 2], z[3 * (i + 1) + 2]));
+    } else {
+      T f = (q.z - za) / (zb - za);
+      rmax = T(z[3 * i + 2]) + f * (T(z[3 * (i + 1) + 2]) - T(z[3 * i + 2]));
+    }
+    if (r <= rmax) return true;
+  }
+  return false;
+}
+template <typename T> __device__ T pgon_dist_out(const double* p, V3<T> q, V3<T> d) {
+  int nz = int(p[3]);
+  T best = T(kBig);
+  for (int i = 0; i + 1 < nz; i++) {
+    T t0, t1;
+    if (!pgon_clip(p, i, q, d, t0, t1)) continue;
+    if (t1 <= T(0) || t0 >= t1) continue;
+    T t = t0 < T(0) ? T(0) : t0;
+    best = t < best ? t : best;
+  }
+  return best;
+}
+template <typename T> __device__ T pgon_dist_in(const double* p, V3<T> q, V3<T> d) {
+  int nz = int(p[3]);
+  T cur = T(0);
+  for (int guard = 0; guard < 2 * nz + 2; guard++) {
+    bool moved = false;
+    for (int i = 0; i + 1 < nz; i++) {
+      T t0, t1;
+      if (!pgon_clip(p, i, q, d, t0, t1)) continue;
+      if (t0 <= cur + T(1e-9) && t1 > cur + T(1e-9)) {
+        cur = t1;
+        moved = true;
+        break;
+      }
+    }
+    if (!moved) break;
+  }
+  return cur;
+}
+template <typename T> __device__ V3<T> pgon_normal(const double* p, V3<T> q, V3<T> d) {
+  int nz = int(p[3]);
+  const double* z = p + 4;
+  T best = T(kBig);
+  V3<T> n = mk<T>(T(0), T(0), T(1));
+  for (int i = 0; i < nz; i++) {
+    T dzp = fabs(q.z - T(z[3 * i]));
+    if (dzp < best) { best = dzp; n = mk<T>(T(0), T(0), T(1)); }
+  }
+  double div = p[1] / p[2];
+  double phi = atan2(double(q.y), double(q.x)) * (180.0 / kPi);
+  while (phi < p[0]) phi += 360.0;
+  int ip = min(int((phi - p[0]) / div), int(p[2]) - 1);
+  T cx, sy;
+  pgon_face(p, ip, cx, sy);
+  for (int i = 0; i + 1 < nz; i++) {
+    T z0 = T(z[3 * i]), z1 = T(z[3 * (i + 1)]);
+    if (!(z1 - z0 > T(0))) continue;
+    if (q.z < z0 - T(1e-9) || q.z > z1 + T(1e-9)) continue;
+    T r0 = T(z[3 * i + 2]);
+    T slope = (T(z[3 * (i + 1) + 2]) - r0) / (z1 - z0);
+    T len = sqrt(T(1) + slope * slope);
+    T f = (cx * q.x + sy * q.y - slope * (q.z - z0) - r0) / len;
+    if (fabs(f) < best) {
+      best = fabs(f);
+      n = mk<T>(cx / len, sy / len, -slope / len);
+    }
+  }
+  if (dot(n, d) < T(0)) n = scale(n, T(-1));
+  return n;
+}
+
+// ------------------------------------------------------------ dispatch
+template <typename T> __device__ bool prim_contains(int type, const double* p, V3<T> q) {
+  switch (type) {
+    case RB_SHAPE_BOX: return box_contains<T>(p, q);
+    case RB_SHAPE_TUBE: return tube_contains<T>(p, q);
+    case RB_SHAPE_SPHERE: return sphere_contains<T>(p, q);
+    case RB_SHAPE_PARABOLOID: return para_contains<T>(p, q);
+    case RB_SHAPE_PGON: return pgon_contains<T>(p, q);
+  }
+  return false;
+}
+template <typename T> __device__ T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
+  switch (type) {
+    case RB_SHAPE_BOX: return box_dist_out<T>(p, q, d);
+    case RB_SHAPE_TUBE: return tube_dist_out<T>(p, q, d);
+    case RB_SHAPE_SPHERE: return sphere_dist<T, true>(p, q, d);
+    case RB_SHAPE_PARABOLOID: return para_dist_out<T>(p, q, d);
+    case RB_SHAPE_PGON: return pgon_dist_out<T>(p, q, d);
+  }
+  return T(kBig);
+}
+template <typename T> __device__ T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
+  switch (type) {
+    case RB_SHAPE_BOX: return box_dist_in<T>(p, q, d);
+    case RB_SHAPE_TUBE: return tube_dist_in<T>(p, q, d);
+    case RB_SHAPE_SPHERE: return sphere_dist<T, false>(p, q, d);
+    case RB_SHAPE_PARABOLOID: return para_dist_in<T>(p, q, d);
+    case RB_SHAPE_PGON: return pgon_dist_in<T>(p, q, d);
+  }
+  return T(0);
+}
+template <typename T> __device__ V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
+  switch (type) {
+    case RB_SHAPE_BOX: return box_normal<T>(p, q, d);
+    case RB_SHAPE_TUBE: return tube_normal<T>(p, q, d);
+    case RB_SHAPE_SPHERE: return sphere_normal<T>(p, q, d);
+    case RB_SHAPE_PARABOLOID: return para_normal<T>(p, q, d);
+    case RB_SHAPE_PGON: return pgon_normal<T>(p, q, d);
+  }
+  return mk<T>(T(0), T(0), T(1));
+}
+
+// Composite (TGeoBoolNode) params: op left right lmat[12] rmat[12].
+// Depth-templated so that nested expressions unroll without recursion.
+template <typename T, int D> struct Csg {
+  __device__ static bool contains(const DScene& S, int shape, V3<T> q) {
+    DShape sh = S.shapes[shape];
+    const double* p = S.params + sh.off;
+    if (sh.type != RB_SHAPE_COMPOSITE) return prim_contains<T>(sh.type, p, q);
+    int op = int(p[0]), L = int(p[1]), R = int(p[2]);
+    bool a = Csg<T, D - 1>::contains(S, L, m2l<T>(p + 3, p + 12, q));
+    if (op == RB_OP_UNION) return a || Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, q));
+    if (op == RB_OP_SUBTRACTION) return a && !Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, q));
+    return a && Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, q));
+  }
+  __device__ static Hit child(const DScene& S, const double* p, int side, bool out, V3<T> q, V3<T> d) {
+    const double* m = p + (side ? 15 : 3);
+    int c = int(p[side ? 2 : 1]);
+    V3<T> lq = m2l<T>(m, m + 9, q), ld = m2lv<T>(m, d);
+    Hit h = out ? Csg<T, D - 1>::dist_out(S, c, lq, ld) : Csg<T, D - 1>::dist_in(S, c, lq, ld);
+    h.leaf = leaf_push(h.leaf, side);
+    return h;
+  }
+  __device__ static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
+    DShape sh = S.shapes[shape];
+    const double* p = S.params + sh.off;
+    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T>(sh.type, p, q, d)), 0u};
+    int op = int(p[0]), L = int(p[1]), R = int(p[2]);
+    if (op == RB_OP_UNION) {
+      Hit a = child(S, p, 0, true, q, d), b = child(S, p, 1, true, q, d);
+      return b.s < a.s ? b : a;
+    }
+    T s = T(0);
+    uint32_t leaf = 0;
+    for (int it = 0; it < 64; it++) {
+      V3<T> x = axpy(q, s, d);
+      V3<T> xp = axpy(x, T(kBoolPush), d);
+      bool inA = Csg<T, D - 1>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
+      bool inB = Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
+      T step;
+      if (op == RB_OP_SUBTRACTION) {
+        if (inA && !inB) return {double(s), leaf};
+        Hit h = inB ? child(S, p, 1, false, x, d) : child(S, p, 0, true, x, d);
+        if (!inB && h.s >= kBig) return {kBig, 0u};
+        step = T(h.s);
+        leaf = h.leaf;
+      } else {
+        if (inA && inB) return {double(s), leaf};
+        Hit ha = {0.0, 0u}, hb = {0.0, 0u};
+        if (!inA) {
+          ha = child(S, p, 0, true, x, d);
+          if (ha.s >= kBig) return {kBig, 0u};
+        }
+        if (!inB) {
+          hb = child(S, p, 1, true, x, d);
+          if (hb.s >= kBig) return {kBig, 0u};
+        }
+        Hit h = (!inA && (inB || ha.s >= hb.s)) ? ha : hb;
+        step = T(h.s);
+        leaf = h.leaf;
+      }
+      if (step < T(kBoolPush)) step = T(kBoolPush);
+      s += step;
+    }
+    return {kBig, 0u};
+  }
+  __device__ static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
+    DShape sh = S.shapes[shape];
+    const double* p = S.params + sh.off;
+    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T>(sh.type, p, q, d)), 0u};
+    int op = int(p[0]), L = int(p[1]), R = int(p[2]);
+    if (op == RB_OP_INTERSECTION) {
+      Hit a = child(S, p, 0, false, q, d), b = child(S, p, 1, false, q, d);
+      return b.s < a.s ? b : a;
+    }
+    if (op == RB_OP_SUBTRACTION) {
+      Hit a = child(S, p, 0, false, q, d), b = child(S, p, 1, true, q, d);
+      return b.s < a.s ? b : a;
+    }
+    T s = T(0);
+    uint32_t leaf = 0;
+    for (int it = 0; it < 64; it++) {
+      V3<T> x = axpy(q, s, d);
+      V3<T> xp = axpy(x, T(kBoolPush), d);
+      bool inA = Csg<T, D - 1>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
+      bool inB = Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
+      if (!inA && !inB) return {double(s), leaf};
+      Hit ha = {0.0, 0u}, hb = {0.0, 0u};
+      if (inA) ha = child(S, p, 0, false, x, d);
+      if (inB) hb = child(S, p, 1, false, x, d);
+      Hit h = (inA && (!inB || ha.s >= hb.s)) ? ha : hb;
+      T step = T(h.s) < T(kBoolPush) ? T(kBoolPush) : T(h.s);
+      leaf = h.leaf;
+      s += step;
+    }
+    return {double(s), leaf};
+  }
+};
+// depth exhausted: primitives only (the host rejects deeper expressions)
+template <typename T> struct Csg<T, 0> {
+  __device__ static bool contains(const DScene& S, int shape, V3<T> q) {
+    DShape sh = S.shapes[shape];
+    return prim_contains<T>(sh.type, S.params + sh.off, q);
+  }
+  __device__ static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
+    DShape sh = S.shapes[shape];
+    return {double(prim_dist_out<T>(sh.type, S.params + sh.off, q, d)), 0u};
+  }
+  __device__ static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
+    DShape sh = S.shapes[shape];
+    return {double(prim_dist_in<T>(sh.type, S.params + sh.off, q, d)), 0u};
+  }
+};
+
+template <typename T>
+__device__ V3<T> shape_normal(const DScene& S, int shape, uint32_t leaf, V3<T> q, V3<T> d) {
+  const double* mats[kMaxCsgDepth + 1];
+  int ns = 0;
+  int maxdepth = int(leaf >> 24);
+  DShape sh = S.shapes[shape];
+  while (sh.type == RB_SHAPE_COMPOSITE && ns < kMaxCsgDepth + 1) {
+    const double* p = S.params + sh.off;
+    int side = ns < maxdepth ? int((leaf >> ns) & 1u) : 0;
+    const double* m = p + (side ? 15 : 3);
+    q = m2l<T>(m, m + 9, q);
+    d = m2lv<T>(m, d);
+    mats[ns++] = m;
+    sh = S.shapes[int(p[side ? 2 : 1])];
+  }
+  V3<T> n = prim_normal<T>(sh.type, S.params + sh.off, q, d);
+  while (ns > 0) n = l2mv<T>(mats[--ns], n);
+  return n;
+}
+
+// ------------------------------------------------------------ physics
+__device__ inline double table_eval(const DScene& S, int table, double x) {
+  rb_table_desc t = S.tables[table];
+  const double* xy = S.table_xy + 2 * t.offset;
+  int n = t.n;
+  if (n == 0) return 0.0;
+  if (n == 1) return xy[1];
+  // binary search on ascending x: first index with x_i >= x
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (xy[2 * mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < n && xy[2 * lo] == x) return xy[2 * lo + 1];
+  int low = lo - 1, up = lo;
+  if (up == n) { up = n - 1; low = n - 2; }
+  if (low < 0) { low = 0; up = 1; }
+  double xl = xy[2 * low], yl = xy[2 * low + 1], xu = xy[2 * up], yu = xy[2 * up + 1];
+  if (xl == xu) return yl;
+  return yu + (x - xu) * (yl - yu) / (xl - xu);
+}
+__device__ inline double lens_index(const DScene& S, const rb_volume_desc& v, double lambda) {
+  if (v.index_model == RB_INDEX_TABLE) return table_eval(S, v.index_ref, lambda);
+  if (v.index_model == RB_INDEX_SELLMEIER) {
+    const double* b = S.params + v.index_ref;
+    double l = lambda / 1e-4;
+    double l2 = l * l;
+    return sqrt(1 + b[0] * l2 / (l2 - b[3]) + b[1] * l2 / (l2 - b[4]) + b[2] * l2 / (l2 - b[5]));
+  }
+  if (v.index_model == RB_INDEX_SCHOTT) {
+    const double* a = S.params + v.index_ref;
+    double l = lambda / 1e-4;
+    return sqrt(a[0] + a[1] * pow(l, 2.) + a[2] * pow(l, -2.) + a[3] * pow(l, -4.) + a[4] * pow(l, -6.) +
+                a[5] * pow(l, -8.));
+  }
+  return 1.0;
+}
+__device__ inline double lens_k(const DScene& S, const rb_volume_desc& v, double lambda) {
+  return v.k_table >= 0 ? table_eval(S, v.k_table, lambda) : 0.0;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+__device__ __forceinline__ void philox(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
+                                       uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint32_t h0 = __umulhi(0xD2511F53u, c0), l0 = 0xD2511F53u * c0;
+    uint32_t h1 = __umulhi(0xCD9E8D57u, c2), l1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = h1 ^ c1 ^ k0, n2 = h0 ^ c3 ^ k1;
+    c0 = n0; c1 = l1; c2 = n2; c3 = l0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+// uniform in (0, 1] from counter (id, draw, stream) under key seed
+__device__ __forceinline__ double uniform01(uint64_t seed, uint64_t id, uint32_t draw, uint32_t stream) {
+  uint32_t c0 = uint32_t(id), c1 = uint32_t(id >> 32), c2 = draw, c3 = stream;
+  philox(c0, c1, c2, c3, uint32_t(seed), uint32_t(seed >> 32));
+  uint64_t u = ((uint64_t(c1) << 32) | c0) >> 11;
+  return double(u + 1) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace rbd

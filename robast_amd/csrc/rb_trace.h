@@ -1,0 +1,388 @@
+// rb_trace.h -- navigation and the per-ray optics state machine (device).
+#pragma once
+
+#include "rb_device.h"
+
+namespace rbd {
+
+struct RayIO {
+  double *x, *y, *z, *t, *dx, *dy, *dz;
+  const double* lambda;
+  int32_t *status, *npoints, *last_node;
+  double* path_xyzt;
+  int32_t* path_node;
+  int32_t path_cap;
+};
+
+struct TraceParams {
+  uint64_t seed, id_base;
+  int32_t limit, fresnel;
+};
+
+// per-lane ray and navigator state
+struct Lane {
+  V3<double> p, d;    // navigator point (== last recorded point between segments), unit direction
+  double t, lambda;
+  int32_t cur;        // current node, -1 = outside the top volume
+  int32_t status, npoints, last_node;
+  uint32_t draw;      // Philox draw counter
+  bool on_b;          // TGeoNavigator fIsOnBoundary
+  // crossing of the last step
+  double step;
+  int32_t hit_node;
+  uint32_t hit_leaf;
+};
+
+__device__ __forceinline__ bool node_contains(const DScene& S, int n, V3<double> p) {
+  const DNode& nd = S.nodes[n];
+  return Csg<double, kMaxCsgDepth>::contains(S, nd.shape, m2l<double>(nd.rot, nd.tr, p));
+}
+
+__device__ __forceinline__ bool box_has(const DBvh& b, float x, float y, float z) {
+  return x >= b.lo[0] && x <= b.hi[0] && y >= b.lo[1] && y <= b.hi[1] && z >= b.lo[2] && z <= b.hi[2];
+}
+
+// First daughter of n (lowest index, excluding skip) whose shape contains p.
+__device__ int first_daughter_containing(const DScene& S, int n, V3<double> p, int skip) {
+  const DNode& nd = S.nodes[n];
+  if (nd.n_children == 0) return -1;
+  float fx = float(p.x), fy = float(p.y), fz = float(p.z);
+  int found = 0x7fffffff;
+  int stack[kBvhStack];
+  int sp = 0;
+  stack[sp++] = nd.bvh_root;
+  while (sp > 0) {
+    const DBvh b = S.bvh[stack[--sp]];
+    if (!box_has(b, fx, fy, fz)) continue;
+    if (b.count == 0) {
+      if (sp + 2 <= kBvhStack) {
+        stack[sp++] = b.left;
+        stack[sp++] = b.left + 1;
+      }
+      continue;
+    }
+    for (int k = 0; k < b.count; k++) {
+      int c = S.bvh_prims[b.left + k];
+      if (c == skip || c >= found) continue;
+      if (node_contains(S, c, p)) found = c;
+    }
+  }
+  return found == 0x7fffffff ? -1 : found;
+}
+
+// TGeoNavigator::SearchNode (up until inside, then down through daughters)
+__device__ int search_node(const DScene& S, int start, V3<double> p, int skip, bool forced) {
+  int n = start;
+  bool inside = forced || n == skip;
+  while (!inside) {
+    if (node_contains(S, n, p)) {
+      inside = true;
+    } else {
+      int par = S.nodes[n].parent;
+      if (par < 0) return -1;
+      skip = n;
+      n = par;
+    }
+  }
+  for (int guard = 0; guard < 64; guard++) {
+    int c = first_daughter_containing(S, n, p, skip);
+    if (c < 0) return n;
+    n = c;
+    skip = -1;
+  }
+  return n;
+}
+
+__device__ __forceinline__ double locate_push(const DScene& S, int n, double step) {
+  const double* t = S.nodes[n].tr;
+  return 100. * ((1. + fabs(t[0]) + fabs(t[1]) + fabs(t[2])) + step) * kTol;
+}
+
+__device__ __forceinline__ Hit node_dist_out(const DScene& S, int c, V3<double> p, V3<double> d) {
+  const DNode& nd = S.nodes[c];
+  return Csg<double, kMaxCsgDepth>::dist_out(S, nd.shape, m2l<double>(nd.rot, nd.tr, p), m2lv<double>(nd.rot, d));
+}
+__device__ __forceinline__ Hit node_dist_in(const DScene& S, int c, V3<double> p, V3<double> d) {
+  const DNode& nd = S.nodes[c];
+  return Csg<double, kMaxCsgDepth>::dist_in(S, nd.shape, m2l<double>(nd.rot, nd.tr, p), m2lv<double>(nd.rot, d));
+}
+
+// TGeoNavigator::FindNextBoundaryAndStep; returns the next node (-1 = null)
+__device__ int next_boundary(const DScene& S, Lane& L) {
+  double extra = L.on_b ? kTol : 0.0;
+  L.on_b = false;
+  L.p = axpy(L.p, extra, L.d);
+  L.hit_node = -1;
+  L.hit_leaf = 0;
+  if (L.cur < 0) {
+    Hit h = node_dist_out(S, 0, L.p, L.d);
+    if (h.s < kBig - kTol) {
+      L.step = h.s + extra;
+      L.p = axpy(L.p, h.s, L.d);
+      L.on_b = true;
+      L.hit_node = 0;
+      L.hit_leaf = h.leaf;
+      V3<double> pp = axpy(L.p, locate_push(S, 0, L.step), L.d);
+      L.cur = search_node(S, 0, pp, -1, true);
+      return L.cur;
+    }
+    L.step = kBig;
+    return -1;
+  }
+  const int cur = L.cur;
+  const DNode& cn = S.nodes[cur];
+  Hit hx = node_dist_in(S, cur, L.p, L.d);
+  if (hx.s <= kTol) {
+    L.step = kTol;
+    L.p = axpy(L.p, kTol, L.d);
+    L.on_b = true;
+    L.hit_node = cur;
+    L.hit_leaf = hx.leaf;
+    V3<double> pp = axpy(L.p, locate_push(S, cur, L.step), L.d);
+    L.cur = search_node(S, cur, pp, cur, false);
+    return L.cur;
+  }
+  double best = kBig;
+  int crossed = -2;
+  uint32_t leaf = 0;
+  if (hx.s < best - kTol) {
+    best = hx.s;
+    crossed = -1;
+    leaf = hx.leaf;
+  }
+  if (cn.n_children > 0) {
+    // BVH over daughters, FP32 slab tests on conservatively expanded boxes
+    const float ox = float(L.p.x), oy = float(L.p.y), oz = float(L.p.z);
+    const float ix = 1.0f / float(L.d.x), iy = 1.0f / float(L.d.y), iz = 1.0f / float(L.d.z);
+    int stack[kBvhStack];
+    int sp = 0;
+    stack[sp++] = cn.bvh_root;
+    while (sp > 0) {
+      const DBvh b = S.bvh[stack[--sp]];
+      float t0x = (b.lo[0] - ox) * ix, t1x = (b.hi[0] - ox) * ix;
+      float t0y = (b.lo[1] - oy) * iy, t1y = (b.hi[1] - oy) * iy;
+      float t0z = (b.lo[2] - oz) * iz, t1z = (b.hi[2] - oz) * iz;
+      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+      float lim = float(best) * 1.0001f + 1e-2f;
+      if (!(tn <= tf * 1.0001f + 1e-3f) || tn > lim) continue;
+      if (b.count == 0) {
+        if (sp + 2 <= kBvhStack) {
+          stack[sp++] = b.left + 1;
+          stack[sp++] = b.left;
+        }
+        continue;
+      }
+      for (int k = 0; k < b.count; k++) {
+        int c = S.bvh_prims[b.left + k];
+        Hit h = node_dist_out(S, c, L.p, L.d);
+        if (h.s < best - kTol || (crossed >= 0 && c < crossed && h.s < best + kTol && h.s < kBig)) {
+          best = h.s;
+          crossed = c;
+          leaf = h.leaf;
+        }
+      }
+    }
+  }
+  L.p = axpy(L.p, best, L.d);
+  L.step = best + extra;
+  if (crossed == -2) return cur;
+  L.on_b = true;
+  L.hit_leaf = leaf;
+  if (crossed == -1) {
+    L.hit_node = cur;
+    if (cn.parent < 0) {
+      L.cur = -1;
+      return -1;
+    }
+    V3<double> pp = axpy(L.p, locate_push(S, cn.parent, L.step), L.d);
+    L.cur = search_node(S, cn.parent, pp, cur, false);
+    return L.cur;
+  }
+  L.hit_node = crossed;
+  V3<double> pp = axpy(L.p, locate_push(S, crossed, L.step), L.d);
+  L.cur = search_node(S, crossed, pp, cur, true);
+  return L.cur;
+}
+
+__device__ V3<double> find_normal(const DScene& S, const Lane& L) {
+  if (L.hit_node < 0) return mk(0.0, 0.0, 1.0);
+  const DNode& nd = S.nodes[L.hit_node];
+  V3<double> lp = m2l<double>(nd.rot, nd.tr, L.p), ld = m2lv<double>(nd.rot, L.d);
+  V3<double> ln = shape_normal<double>(S, nd.shape, L.hit_leaf, lp, ld);
+  return l2mv<double>(nd.rot, ln);
+}
+
+__device__ __forceinline__ int kind_of(const DScene& S, int n) { return n < 0 ? RB_KNULL : S.nodes[n].kind; }
+__device__ __forceinline__ const rb_volume_desc& vol_of(const DScene& S, int n) {
+  return S.volumes[S.nodes[n].volume];
+}
+
+__device__ __forceinline__ double draw(const TraceParams& P, uint64_t id, Lane& L) {
+  return uniform01(P.seed, id, L.draw++, 0u);
+}
+
+__device__ __forceinline__ void add_point(const RayIO& io, int64_t i, Lane& L, V3<double> x, double t, int node) {
+  if (io.path_cap > 0 && L.npoints - 1 < io.path_cap) {
+    double* s = io.path_xyzt + (int64_t(io.path_cap) * i + (L.npoints - 1)) * 4;
+    s[0] = x.x; s[1] = x.y; s[2] = x.z; s[3] = t;
+    io.path_node[int64_t(io.path_cap) * i + (L.npoints - 1)] = node;
+  }
+  L.t = t;
+  L.last_node = node;
+  L.npoints++;
+}
+
+__device__ __forceinline__ void set_direction(Lane& L, V3<double> v) {
+  double mag = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+  if (mag > 0) L.d = mk(v.x / mag, v.y / mag, v.z / mag);
+}
+
+// AOpticsManager::DoReflection (src/AOpticsManager.cxx:170-247)
+__device__ void do_reflection(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id,
+                              Lane& L, double n1, int next, V3<double> n) {
+  double step = L.step;
+  V3<double> d1 = L.d;
+  double cos1 = dot(d1, n);
+  bool absorbed = false;
+  if (kind_of(S, next) == RB_KMIRROR) {
+    const rb_volume_desc& v = vol_of(S, next);
+    double ref = v.refl_table >= 0 ? table_eval(S, v.refl_table, L.lambda) : v.refl_const;
+    ref = ref > 1 ? 1 : ref;
+    ref = ref < 0 ? 0 : ref;
+    if (ref < draw(P, id, L)) {
+      absorbed = true;
+      L.status = RB_ABSORB;
+    }
+  }
+  V3<double> d2 = mk(d1.x - 2 * n.x * cos1, d1.y - 2 * n.y * cos1, d1.z - 2 * n.z * cos1);
+  if (!absorbed) set_direction(L, d2);
+  double t = L.t + step / (kC / n1);
+  // TGeoNavigator::Step() of kEpsilon backwards moves kEpsilon + 1e-6, then relocates
+  L.p = axpy(L.p, -(kEps + 1e-6), d1);
+  L.cur = L.cur < 0 ? (node_contains(S, 0, L.p) ? search_node(S, 0, L.p, -1, true) : -1)
+                    : search_node(S, L.cur, L.p, -1, false);
+  L.on_b = true;
+  add_point(io, i, L, L.p, t, next);
+}
+
+// AOpticsManager::DoFresnel (src/AOpticsManager.cxx:52-167), no multilayer branch
+__device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id, Lane& L,
+                           double n1, double n2, double k2, int next) {
+  double step = L.step;
+  V3<double> n = find_normal(S, L);
+  V3<double> d1 = L.d;
+  double cos1 = dot(d1, n);
+  double sin1 = sqrt(1 - cos1 * cos1);
+  double sin2 = n1 * sin1 / n2;
+  double cos2 = sqrt(1 - sin2 * sin2);
+  if (sin2 > 1.) {
+    do_reflection(S, P, io, i, id, L, n1, next, n);
+    return;
+  }
+  if (P.fresnel) {
+    double Rs, Rp;
+    if (k2 <= 0.) {
+      double eta1S = n1 * cos1, eta2S = n2 * cos2, eta1P = n1 / cos1, eta2P = n2 / cos2;
+      double a = (eta1S - eta2S) / (eta1S + eta2S), b = (eta1P - eta2P) / (eta1P + eta2P);
+      Rs = a * a;
+      Rp = b * b;
+    } else {
+      double x1S = n1 * cos1, x1P = n1 / cos1;
+      double u = n2 * n2 - k2 * k2 - (n1 * sin1) * (n1 * sin1);
+      double v = 2 * n2 * k2;
+      double tmp = sqrt(u * u + v * v);
+      double cosxi2 = sqrt(1 + u / tmp) / sqrt(2.), sinxi2 = sqrt(1 - u / tmp) / sqrt(2.);
+      double x2S = sqrt(tmp) * cosxi2, y2S = sqrt(tmp) * sinxi2;
+      tmp = x2S * x2S + y2S * y2S;
+      double nk = n2 * n2 - k2 * k2;
+      double x2P = (2 * n2 * k2 * y2S + nk * x2S) / tmp;
+      double y2P = (2 * n2 * k2 * x2S - nk * y2S) / tmp;
+      Rs = ((x1S - x2S) * (x1S - x2S) + y2S * y2S) / ((x1S + x2S) * (x1S + x2S) + y2S * y2S);
+      Rp = ((x1P - x2P) * (x1P - x2P) + y2P * y2P) / ((x1P + x2P) * (x1P + x2P) + y2P * y2P);
+    }
+    double R = (Rs + Rp) / 2.;
+    if (draw(P, id, L) < R) {
+      do_reflection(S, P, io, i, id, L, n1, next, n);
+      return;
+    }
+  }
+  V3<double> d2 = d1;
+  if (sin1 != 0) {
+    double f = sin2 / sin1;
+    d2 = mk((d1.x - cos1 * n.x) * f + n.x * cos2, (d1.y - cos1 * n.y) * f + n.y * cos2,
+            (d1.z - cos1 * n.z) * f + n.z * cos2);
+  }
+  double t = L.t + step / (kC / n1);
+  add_point(io, i, L, L.p, t, next);
+  set_direction(L, d2);
+}
+
+// AOpticsManager::TraceNonSequential(TObjArray*) for one ray (src/AOpticsManager.cxx:353-518)
+__device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L) {
+  const uint64_t id = P.id_base + uint64_t(i);
+  L.on_b = false;
+  L.cur = node_contains(S, 0, L.p) ? search_node(S, 0, L.p, -1, true) : -1;
+  const double lambda = L.lambda;
+  while (L.status == RB_RUN) {
+    const V3<double> x1 = L.p;
+    const double t1 = L.t;
+    const int current = L.cur;
+    const int next = next_boundary(S, L);
+    const double step = L.step;
+    const int tc = kind_of(S, current), tn = kind_of(S, next);
+
+    if (tc == RB_KLENS) {
+      const rb_volume_desc& v = vol_of(S, current);
+      double k = lens_k(S, v, lambda);
+      double absl = k <= 0 ? 0.0 : lambda / (4 * kPi * k);
+      if (k > 0 && absl > 0 && isfinite(absl)) {
+        double abs_step = -absl * log(draw(P, id, L));
+        if (abs_step < step) {
+          double n1 = lens_index(S, v, lambda);
+          L.p = axpy(x1, abs_step, L.d);
+          add_point(io, i, L, L.p, t1 + abs_step / (kC / n1), next);
+          L.status = RB_ABSORB;
+          continue;
+        }
+      }
+    }
+
+    const bool c_free = tc == RB_KNULL || tc == RB_KOPT || tc == RB_KOTHER;
+    if ((c_free || tc == RB_KLENS) && tn == RB_KMIRROR) {
+      double n1 = tc == RB_KLENS ? lens_index(S, vol_of(S, current), lambda) : 1.;
+      do_reflection(S, P, io, i, id, L, n1, next, find_normal(S, L));
+    } else if (c_free && tn == RB_KLENS) {
+      const rb_volume_desc& v = vol_of(S, next);
+      do_fresnel(S, P, io, i, id, L, 1., lens_index(S, v, lambda), lens_k(S, v, lambda), next);
+    } else if ((c_free || tc == RB_KLENS) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
+      double n1 = tc == RB_KLENS ? lens_index(S, vol_of(S, current), lambda) : 1.;
+      add_point(io, i, L, L.p, t1 + step / (kC / n1), next);
+    } else if (c_free && (tn == RB_KOTHER || tn == RB_KOPT)) {
+      add_point(io, i, L, L.p, t1 + step / kC, next);
+    } else if (tc == RB_KLENS && tn == RB_KLENS) {
+      const rb_volume_desc& v = vol_of(S, next);
+      do_fresnel(S, P, io, i, id, L, lens_index(S, vol_of(S, current), lambda), lens_index(S, v, lambda),
+                 lens_k(S, v, lambda), next);
+    } else if (tc == RB_KLENS && (tn == RB_KNULL || tn == RB_KOPT || tn == RB_KOTHER)) {
+      do_fresnel(S, P, io, i, id, L, lens_index(S, vol_of(S, current), lambda), 1., 0., next);
+    }
+
+    if (tn == RB_KNULL) {
+      add_point(io, i, L, L.p, t1 + step / kC, next);
+      L.status = RB_EXIT;
+    } else if (tc == RB_KFOCUS || tc == RB_KOBS || tc == RB_KMIRROR || tn == RB_KOBS) {
+      L.status = RB_STOP;
+    } else if (tn == RB_KFOCUS) {
+      const rb_volume_desc& v = vol_of(S, next);
+      double angle = 0.;
+      if (v.qe_angle_table >= 0) angle = acos(dot(L.d, find_normal(S, L)));
+      double qe = v.qe_lambda_table >= 0 ? table_eval(S, v.qe_lambda_table, lambda) : 1.;
+      if (v.qe_angle_table >= 0) qe *= table_eval(S, v.qe_angle_table, angle);
+      L.status = (qe == 1 || draw(P, id, L) < qe) ? RB_FOCUS : RB_STOP;
+    }
+    if (L.status == RB_RUN && L.npoints >= P.limit) L.status = RB_SUSPEND;
+  }
+}
+
+}  // namespace rbd
