@@ -47,9 +47,10 @@ enum {
 enum {
   RB_SHAPE_BOX = 1,        /* dx dy dz ox oy oz                     (TGeoBBox)        */
   RB_SHAPE_TUBE = 2,       /* rmin rmax dz                          (TGeoTube)        */
-  RB_SHAPE_SPHERE = 3,     /* rmin rmax theta1 theta2 [deg]         (TGeoSphere, full phi) */
+  RB_SHAPE_SPHERE = 3,     /* rmin rmax theta1 theta2 [deg] cos1 cos2 sin1 sin2 (TGeoSphere, full phi) */
   RB_SHAPE_PARABOLOID = 4, /* rlo rhi dz a b                        (TGeoParaboloid)  */
-  RB_SHAPE_PGON = 5,       /* phi1 dphi nedges nz {z rmin rmax}*nz  (TGeoPgon)        */
+  RB_SHAPE_PGON = 5,       /* phi1 dphi nedges nz {z rmin rmax}*nz {cos sin}*nedges (TGeoPgon;
+                              side k's outward normal at phi1 + dphi/nedges (k + 1/2)) */
   RB_SHAPE_PCON = 6,       /* phi1 dphi nz {z rmin rmax}*nz         (TGeoPcon)        */
   RB_SHAPE_COMPOSITE = 7,  /* op left right lmat[12] rmat[12]       (TGeoBoolNode)    */
   RB_SHAPE_ASPHERIC = 8,   /* z1 c1 z2 c2 rmax rmin kappa1 kappa2 n1 n2 K1[n1] K2[n2]
@@ -106,9 +107,13 @@ typedef struct {
   int32_t n_children;
 } rb_node_desc;
 
+/* A TGraph for linear Eval: table_xy[offset ..] holds the n points sorted by
+ * x, then 4 more pairs: the (up, low) points TGraph::Eval extrapolates from
+ * above the range and the (low, up) points it uses below it. ROOT picks those
+ * by a scan in insertion order, so the flattener records them explicitly. */
 typedef struct {
   int32_t offset; /* first (x, y) pair in table_xy */
-  int32_t n;      /* number of points, x ascending */
+  int32_t n;      /* number of points (the 4 extrapolation pairs follow) */
 } rb_table_desc;
 
 typedef struct {

@@ -38,6 +38,8 @@ def lib():
         L.oracle_philox.restype = None
         L.oracle_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
         L.oracle_uniform.restype = C.c_double
+        L.oracle_contains_batch.argtypes = [C.POINTER(A.rb_scene_desc), C.c_int, C.c_int64, C.c_void_p, C.c_void_p]
+        L.oracle_contains_batch.restype = C.c_int
         L.oracle_generate_rays.argtypes = [C.POINTER(A.rb_raygen_desc), C.POINTER(A.rb_rays), C.c_int64]
         L.oracle_generate_rays.restype = C.c_int
         _lib = L
@@ -89,6 +91,13 @@ def shape_query(flat, shape, p, d):
     d = np.ascontiguousarray(d, dtype=float)
     lib().oracle_shape_query(C.byref(flat.desc), shape, p.ctypes.data, d.ctypes.data, out.ctypes.data)
     return bool(out[0]), out[1], out[2], out[3:6]
+
+
+def contains_batch(flat, shape, pts):
+    pts = np.ascontiguousarray(pts, dtype=float).reshape(-1, 3)
+    out = np.zeros(len(pts), dtype=np.uint8)
+    lib().oracle_contains_batch(C.byref(flat.desc), shape, len(pts), pts.ctypes.data, out.ctypes.data)
+    return out.astype(bool)
 
 
 def philox(ctr, key):

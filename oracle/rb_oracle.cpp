@@ -188,12 +188,13 @@ static double tube_dist_out(const double* p, V3 q, V3 d) {
     double x = q.x + s * d.x, y = q.y + s * d.y, r2 = x * x + y * y;
     if (r2 >= rmin * rmin && r2 <= rmax * rmax) best = s;
   }
+  // entering through the outer wall (first root), or out of the bore into
+  // the solid (second root of the inner wall), inside the z slab
   double s1, s2;
-  double r2 = q.x * q.x + q.y * q.y;
-  if (r2 >= rmax * rmax && cyl_roots(q, d, rmax, s1, s2) && s1 >= 0 && s1 < best) {
+  if (cyl_roots(q, d, rmax, s1, s2) && s1 >= 0 && s1 < best) {
     if (std::fabs(q.z + s1 * d.z) <= dz) best = s1;
   }
-  if (rmin > 0 && r2 <= rmin * rmin && cyl_roots(q, d, rmin, s1, s2) && s2 >= 0 && s2 < best) {
+  if (rmin > 0 && cyl_roots(q, d, rmin, s1, s2) && s2 >= 0 && s2 < best) {
     if (std::fabs(q.z + s2 * d.z) <= dz) best = s2;
   }
   return best;
@@ -229,8 +230,8 @@ static SphereP sphere_p(const double* p) {
   s.rmax = p[1];
   s.t1 = p[2] > 0;
   s.t2 = p[3] < 180;
-  s.c1 = std::cos(p[2] * kPi / 180);
-  s.c2 = std::cos(p[3] * kPi / 180);
+  s.c1 = p[4];  // cos(theta1), cos(theta2) as flattened
+  s.c2 = p[5];
   return s;
 }
 static bool sphere_contains(const double* p, V3 q) {
@@ -476,11 +477,11 @@ struct PgonSeg {
 };
 static int pgon_nedges(const double* p) { return int(p[2]); }
 static int pgon_nz(const double* p) { return int(p[3]); }
+// outward normal of side k (centre of its phi sector), stored after the sections
 static void pgon_face(const double* p, int k, double& cx, double& sy) {
-  double div = p[1] / pgon_nedges(p);
-  double ph = (p[0] + div * (k + 0.5)) * kPi / 180;
-  cx = std::cos(ph);
-  sy = std::sin(ph);
+  const double* f = p + 4 + 3 * pgon_nz(p) + 2 * k;
+  cx = f[0];
+  sy = f[1];
 }
 static bool pgon_seg(const double* p, int i, PgonSeg& s) {
   const double* z = p + 4;
@@ -531,8 +532,9 @@ static bool pgon_contains(const double* p, V3 q) {
   double ddp = phi - p[0];
   if (ddp > p[1]) return false;
   int ip = std::min(int(ddp / div), pgon_nedges(p) - 1);
-  double ph0 = (p[0] + div * (ip + 0.5)) * kPi / 180;
-  double r = q.x * std::cos(ph0) + q.y * std::sin(ph0);
+  double cx, sy;
+  pgon_face(p, ip, cx, sy);
+  double r = q.x * cx + q.y * sy;
   for (int i = 0; i + 1 < nz; i++) {
     double za = z[3 * i], zb = z[3 * (i + 1)];
     if (q.z < za || q.z > zb) continue;
@@ -820,9 +822,18 @@ static double table_eval(const rb_scene_desc* d, int table, double x) {
     else if (xi > x) { up = i; break; }
     else return xy[2 * i + 1];
   }
-  if (up == -1) { up = low; low = low - 1; }
-  if (low == -1) { low = up; up = up + 1; }
-  double xl = xy[2 * low], yl = xy[2 * low + 1], xu = xy[2 * up], yu = xy[2 * up + 1];
+  // outside the range: the insertion-order pairs stored after the points
+  const double* e = nullptr;
+  if (up == -1) e = xy + 2 * n;            // (up, low) above
+  else if (low == -1) e = xy + 2 * n + 4;  // (low, up) below
+  double xl, yl, xu, yu;
+  if (e) {
+    bool above = up == -1;
+    xu = e[above ? 0 : 2]; yu = e[above ? 1 : 3];
+    xl = e[above ? 2 : 0]; yl = e[above ? 3 : 1];
+  } else {
+    xl = xy[2 * low]; yl = xy[2 * low + 1]; xu = xy[2 * up]; yu = xy[2 * up + 1];
+  }
   if (xl == xu) return yl;
   return yu + (x - xu) * (yl - yu) / (xl - xu);
 }
@@ -1342,6 +1353,13 @@ int oracle_generate_rays(const rb_raygen_desc* g, rb_rays* r, int64_t n) {
     r->npoints[i] = 1;
     if (r->last_node) r->last_node[i] = -1;
   }
+  return 0;
+}
+
+// Batch containment probe for brute-force distance checks (local frame of `shape`).
+int oracle_contains_batch(const rb_scene_desc* desc, int shape, int64_t n, const double* pts, uint8_t* out) {
+  Scene sc{desc};
+  for (int64_t i = 0; i < n; i++) out[i] = contains(sc, shape, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2])) ? 1 : 0;
   return 0;
 }
 

@@ -28,6 +28,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"librbgpu.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    # torch ships its own libamdhip64 with the same SONAME; loading torch
+    # first makes librbgpu bind to that copy, so one HIP runtime serves both
+    # (two runtimes in one process leave the second without devices)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     lib.rbgpu_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
     lib.rbgpu_ctx_create.restype = C.c_int

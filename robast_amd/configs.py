@@ -68,7 +68,7 @@ C2_DISK_D = 12.5 * m
 def dc_facet_positions(n=C2_NFACETS, pitch=C2_MIRROR_D):
     """Centres of the n hexagonal facets nearest the axis on the lattice of
     tutorials/DaviesCotton.C:75-105 (x = 1.5 i dx, y = j dy, i + j even).
-    85 = every lattice point with r^2 <= 19 pitch^2 (complete shells)."""
+    85 = every lattice point with r^2 <= 21 pitch^2 (complete shells)."""
     dx = pitch / math.sqrt(3)
     dy = pitch / 2.0
     pts = []

@@ -10,7 +10,11 @@
 // form); the scene itself is always FP64.
 #pragma once
 
+#ifndef RB_HOST_EMULATION
 #include <hip/hip_runtime.h>
+#else
+#include "rb_host_emulation.h"  // CPU build of the device code, for debugging and tests
+#endif
 #include <stdint.h>
 
 #include "../../include/rbgpu.h"
@@ -23,8 +27,7 @@ constexpr double kEps = 1e-6;        // AOpticsManager kEpsilon (src/AOpticsMana
 constexpr double kC = 2.99792458e10; // speed of light, cm/s
 constexpr double kBoolPush = 1e-8;   // inside-test offset in boolean-shape iteration
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kMaxCsgDepth = 3;
-constexpr int kBvhStack = 32;
+constexpr int kMaxCsgDepth = 2;   // boolean nesting levels unrolled on the device
 
 // ------------------------------------------------------------ scene on HBM
 struct DNode {                 // 128 B
@@ -33,11 +36,11 @@ struct DNode {                 // 128 B
   int32_t shape, kind, volume, parent;
   int32_t first_child, n_children, bvh_root, pad;
 };
-struct DBvh {                  // 32 B, FP32 boxes expanded outward
+struct DBvh {                  // 32 B, FP32 boxes expanded outward, depth-first order
   float lo[3];
-  int32_t left;                // inner: left child index (right = left + 1); leaf: first prim
+  int32_t escape;              // next node when this box is missed (end of its subtree)
   float hi[3];
-  int32_t count;               // 0 = inner node, > 0 = leaf with `count` prims
+  int32_t prim;                // leaf: scene node id; inner: -1 (first child is the next entry)
 };
 struct DShape {
   int32_t type, off;
@@ -46,7 +49,6 @@ struct DScene {
   const DNode* nodes;
   const int32_t* children;
   const DBvh* bvh;
-  const int32_t* bvh_prims;    // node ids referenced by BVH leaves
   const DShape* shapes;
   const double* params;
   const rb_volume_desc* volumes;
@@ -86,7 +88,8 @@ __device__ __forceinline__ V3<T> l2mv(const double* rot, V3<T> q) {
 
 struct Hit {
   double s;
-  uint32_t leaf;   // composite path: bit k = right child at depth k; depth in bits 24..31
+  uint32_t leaf;     // composite path: bit k = right child at depth k; depth in bits 24..31
+  uint32_t exiting;  // 1 if the crossing leaves the leaf primitive, 0 if it enters it
 };
 __device__ __forceinline__ uint32_t leaf_push(uint32_t leaf, int side) {
   uint32_t depth = leaf >> 24;
@@ -189,12 +192,12 @@ template <typename T> __device__ T tube_dist_out(const double* p, V3<T> q, V3<T>
     T x = q.x + s * d.x, y = q.y + s * d.y, r2 = x * x + y * y;
     if (r2 >= rmin * rmin && r2 <= rmax * rmax) best = s;
   }
+  // outer wall entry (first root) or bore exit into the solid (second root)
   T s1, s2;
-  T r2 = q.x * q.x + q.y * q.y;
-  if (r2 >= rmax * rmax && cyl_roots(q, d, rmax, s1, s2) && s1 >= T(0) && s1 < best) {
+  if (cyl_roots(q, d, rmax, s1, s2) && s1 >= T(0) && s1 < best) {
     if (fabs(q.z + s1 * d.z) <= dz) best = s1;
   }
-  if (rmin > T(0) && r2 <= rmin * rmin && cyl_roots(q, d, rmin, s1, s2) && s2 >= T(0) && s2 < best) {
+  if (rmin > T(0) && cyl_roots(q, d, rmin, s1, s2) && s2 >= T(0) && s2 < best) {
     if (fabs(q.z + s2 * d.z) <= dz) best = s2;
   }
   return best;
@@ -228,8 +231,8 @@ template <typename T> __device__ SphP<T> sph_p(const double* p) {
   s.rmax = T(p[1]);
   s.t1 = p[2] > 0.0;
   s.t2 = p[3] < 180.0;
-  s.c1 = T(cos(p[2] * (kPi / 180.0)));
-  s.c2 = T(cos(p[3] * (kPi / 180.0)));
+  s.c1 = T(p[4]);
+  s.c2 = T(p[5]);
   return s;
 }
 template <typename T> __device__ bool sphere_contains(const double* p, V3<T> q) {
@@ -333,7 +336,7 @@ template <typename T> __device__ V3<T> sphere_normal(const double* p, V3<T> q, V
   SphP<T> s = sph_p<T>(p);
   T r = sqrt(dot(q, q));
   T rho = sqrt(q.x * q.x + q.y * q.y);
-  T st1 = sqrt(fmax(T(0), T(1) - s.c1 * s.c1)), st2 = sqrt(fmax(T(0), T(1) - s.c2 * s.c2));
+  T st1 = T(p[6]), st2 = T(p[7]);
   T dist[4] = {fabs(r - s.rmax), s.rmin > T(0) ? fabs(r - s.rmin) : T(kBig),
                s.t1 ? fabs(rho * s.c1 - q.z * st1) : T(kBig), s.t2 ? fabs(rho * s.c2 - q.z * st2) : T(kBig)};
   int k = 0;
@@ -441,10 +444,9 @@ template <typename T> __device__ V3<T> para_normal(const double* p, V3<T> q, V3<
 // params: phi1 dphi nedges nz {z rmin rmax}*nz. Each z segment is a convex
 // frustum: half-spaces n_k.(x,y) - slope (z - z0) <= r0 inside the z slab.
 template <typename T> __device__ __forceinline__ void pgon_face(const double* p, int k, T& cx, T& sy) {
-  double div = p[1] / p[2];
-  double ph = (p[0] + div * (k + 0.5)) * (kPi / 180.0);
-  cx = T(cos(ph));
-  sy = T(sin(ph));
+  const double* f = p + 4 + 3 * int(p[3]) + 2 * k;   // precomputed outward side normals
+  cx = T(f[0]);
+  sy = T(f[1]);
 }
 template <typename T> __device__ bool pgon_clip(const double* p, int i, V3<T> q, V3<T> d, T& t0, T& t1) {
   const double* z = p + 4;
@@ -478,18 +480,20 @@ template <typename T> __device__ bool pgon_clip(const double* p, int i, V3<T> q,
   }
   return t0 <= t1;
 }
+// Inside a regular full-phi polygon iff the projection on every side normal
+// is within the side's apothem; the largest projection is the one on the
+// side of the point's own phi sector (the test ROOT's TGeoPgon::Contains makes).
 template <typename T> __device__ bool pgon_contains(const double* p, V3<T> q) {
-  int nz = int(p[3]);
+  int nz = int(p[3]), ne = int(p[2]);
   const double* z = p + 4;
   if (q.z < T(z[0]) || q.z > T(z[3 * (nz - 1)])) return false;
-  double div = p[1] / p[2];
-  double phi = atan2(double(q.y), double(q.x)) * (180.0 / kPi);
-  while (phi < p[0]) phi += 360.0;
-  double ddp = phi - p[0];
-  if (ddp > p[1]) return false;
-  int ip = min(int(ddp / div), int(p[2]) - 1);
-  double ph0 = (p[0] + div * (ip + 0.5)) * (kPi / 180.0);
-  T r = q.x * T(cos(ph0)) + q.y * T(sin(ph0));
+  T r = T(-kBig);
+  for (int k = 0; k < ne; k++) {
+    T cx, sy;
+    pgon_face(p, k, cx, sy);
+    T v = q.x * cx + q.y * sy;
+    r = v > r ? v : r;
+  }
   for (int i = 0; i + 1 < nz; i++) {
     T za = T(z[3 * i]), zb = T(z[3 * (i + 1)]);
     if (q.z < za || q.z > zb) continue;
@@ -535,7 +539,7 @@ template <typename T> __device__ T pgon_dist_in(const double* p, V3<T> q, V3<T> 
   return cur;
 }
 template <typename T> __device__ V3<T> pgon_normal(const double* p, V3<T> q, V3<T> d) {
-  int nz = int(p[3]);
+  int nz = int(p[3]), ne = int(p[2]);
   const double* z = p + 4;
   T best = T(kBig);
   V3<T> n = mk<T>(T(0), T(0), T(1));
@@ -543,12 +547,16 @@ template <typename T> __device__ V3<T> pgon_normal(const double* p, V3<T> q, V3<
     T dzp = fabs(q.z - T(z[3 * i]));
     if (dzp < best) { best = dzp; n = mk<T>(T(0), T(0), T(1)); }
   }
-  double div = p[1] / p[2];
-  double phi = atan2(double(q.y), double(q.x)) * (180.0 / kPi);
-  while (phi < p[0]) phi += 360.0;
-  int ip = min(int((phi - p[0]) / div), int(p[2]) - 1);
-  T cx, sy;
-  pgon_face(p, ip, cx, sy);
+  // the side of the point's phi sector: largest projection
+  int ip = 0;
+  T cx = T(0), sy = T(0), rmx = T(-kBig);
+  for (int k = 0; k < ne; k++) {
+    T a, b;
+    pgon_face(p, k, a, b);
+    T v = q.x * a + q.y * b;
+    if (v > rmx) { rmx = v; ip = k; cx = a; sy = b; }
+  }
+  (void)ip;
   for (int i = 0; i + 1 < nz; i++) {
     T z0 = T(z[3 * i]), z1 = T(z[3 * (i + 1)]);
     if (!(z1 - z0 > T(0))) continue;
@@ -567,7 +575,7 @@ template <typename T> __device__ V3<T> pgon_normal(const double* p, V3<T> q, V3<
 }
 
 // ------------------------------------------------------------ dispatch
-template <typename T> __device__ bool prim_contains(int type, const double* p, V3<T> q) {
+template <typename T> __device__ __noinline__ bool prim_contains(int type, const double* p, V3<T> q) {
   switch (type) {
     case RB_SHAPE_BOX: return box_contains<T>(p, q);
     case RB_SHAPE_TUBE: return tube_contains<T>(p, q);
@@ -577,7 +585,7 @@ template <typename T> __device__ bool prim_contains(int type, const double* p, V
   }
   return false;
 }
-template <typename T> __device__ T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ __noinline__ T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_dist_out<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_dist_out<T>(p, q, d);
@@ -587,7 +595,7 @@ template <typename T> __device__ T prim_dist_out(int type, const double* p, V3<T
   }
   return T(kBig);
 }
-template <typename T> __device__ T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ __noinline__ T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_dist_in<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_dist_in<T>(p, q, d);
@@ -597,7 +605,7 @@ template <typename T> __device__ T prim_dist_in(int type, const double* p, V3<T>
   }
   return T(0);
 }
-template <typename T> __device__ V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ __noinline__ V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_normal<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_normal<T>(p, q, d);
@@ -610,8 +618,13 @@ template <typename T> __device__ V3<T> prim_normal(int type, const double* p, V3
 
 // Composite (TGeoBoolNode) params: op left right lmat[12] rmat[12].
 // Depth-templated so that nested expressions unroll without recursion.
+// The inside tests of the boundary-to-boundary iteration are taken `push`
+// past each boundary point (kBoolPush in FP64; larger in FP32 search).
+template <typename T> __device__ __forceinline__ T bool_push() { return T(kBoolPush); }
+template <> __device__ __forceinline__ float bool_push<float>() { return 2e-4f; }
+
 template <typename T, int D> struct Csg {
-  __device__ static bool contains(const DScene& S, int shape, V3<T> q) {
+  __device__ __noinline__ static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return prim_contains<T>(sh.type, p, q);
@@ -629,53 +642,52 @@ template <typename T, int D> struct Csg {
     h.leaf = leaf_push(h.leaf, side);
     return h;
   }
-  __device__ static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ __noinline__ static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T>(sh.type, p, q, d)), 0u};
+    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T>(sh.type, p, q, d)), 0u, 0u};
     int op = int(p[0]), L = int(p[1]), R = int(p[2]);
     if (op == RB_OP_UNION) {
       Hit a = child(S, p, 0, true, q, d), b = child(S, p, 1, true, q, d);
       return b.s < a.s ? b : a;
     }
+    const T push = bool_push<T>();
     T s = T(0);
-    uint32_t leaf = 0;
+    Hit last = {0.0, 0u, 0u};
     for (int it = 0; it < 64; it++) {
       V3<T> x = axpy(q, s, d);
-      V3<T> xp = axpy(x, T(kBoolPush), d);
+      V3<T> xp = axpy(x, push, d);
       bool inA = Csg<T, D - 1>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
       bool inB = Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
-      T step;
+      Hit h;
       if (op == RB_OP_SUBTRACTION) {
-        if (inA && !inB) return {double(s), leaf};
-        Hit h = inB ? child(S, p, 1, false, x, d) : child(S, p, 0, true, x, d);
-        if (!inB && h.s >= kBig) return {kBig, 0u};
-        step = T(h.s);
-        leaf = h.leaf;
+        if (inA && !inB) return {double(s), last.leaf, last.exiting};
+        h = inB ? child(S, p, 1, false, x, d) : child(S, p, 0, true, x, d);
+        if (!inB && h.s >= kBig) return {kBig, 0u, 0u};
       } else {
-        if (inA && inB) return {double(s), leaf};
-        Hit ha = {0.0, 0u}, hb = {0.0, 0u};
+        if (inA && inB) return {double(s), last.leaf, last.exiting};
+        Hit ha = {0.0, 0u, 0u}, hb = {0.0, 0u, 0u};
         if (!inA) {
           ha = child(S, p, 0, true, x, d);
-          if (ha.s >= kBig) return {kBig, 0u};
+          if (ha.s >= kBig) return {kBig, 0u, 0u};
         }
         if (!inB) {
           hb = child(S, p, 1, true, x, d);
-          if (hb.s >= kBig) return {kBig, 0u};
+          if (hb.s >= kBig) return {kBig, 0u, 0u};
         }
-        Hit h = (!inA && (inB || ha.s >= hb.s)) ? ha : hb;
-        step = T(h.s);
-        leaf = h.leaf;
+        h = (!inA && (inB || ha.s >= hb.s)) ? ha : hb;
       }
-      if (step < T(kBoolPush)) step = T(kBoolPush);
+      T step = T(h.s);
+      last = h;
+      if (step < push) step = push;
       s += step;
     }
-    return {kBig, 0u};
+    return {kBig, 0u, 0u};
   }
-  __device__ static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ __noinline__ static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T>(sh.type, p, q, d)), 0u};
+    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T>(sh.type, p, q, d)), 0u, 1u};
     int op = int(p[0]), L = int(p[1]), R = int(p[2]);
     if (op == RB_OP_INTERSECTION) {
       Hit a = child(S, p, 0, false, q, d), b = child(S, p, 1, false, q, d);
@@ -685,59 +697,87 @@ template <typename T, int D> struct Csg {
       Hit a = child(S, p, 0, false, q, d), b = child(S, p, 1, true, q, d);
       return b.s < a.s ? b : a;
     }
+    const T push = bool_push<T>();
     T s = T(0);
-    uint32_t leaf = 0;
+    Hit last = {0.0, 0u, 1u};
     for (int it = 0; it < 64; it++) {
       V3<T> x = axpy(q, s, d);
-      V3<T> xp = axpy(x, T(kBoolPush), d);
+      V3<T> xp = axpy(x, push, d);
       bool inA = Csg<T, D - 1>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
       bool inB = Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
-      if (!inA && !inB) return {double(s), leaf};
-      Hit ha = {0.0, 0u}, hb = {0.0, 0u};
+      if (!inA && !inB) return {double(s), last.leaf, last.exiting};
+      Hit ha = {0.0, 0u, 1u}, hb = {0.0, 0u, 1u};
       if (inA) ha = child(S, p, 0, false, x, d);
       if (inB) hb = child(S, p, 1, false, x, d);
       Hit h = (inA && (!inB || ha.s >= hb.s)) ? ha : hb;
-      T step = T(h.s) < T(kBoolPush) ? T(kBoolPush) : T(h.s);
-      leaf = h.leaf;
+      T step = T(h.s) < push ? push : T(h.s);
+      last = h;
       s += step;
     }
-    return {double(s), leaf};
+    return {double(s), last.leaf, last.exiting};
+  }
+  // normal of the leaf surface `leaf` (path bits consumed from bit 0)
+  __device__ __noinline__ static V3<T> normal(const DScene& S, int shape, uint32_t path, int depth, V3<T> q, V3<T> d) {
+    DShape sh = S.shapes[shape];
+    const double* p = S.params + sh.off;
+    if (sh.type != RB_SHAPE_COMPOSITE) return prim_normal<T>(sh.type, p, q, d);
+    int side = depth > 0 ? int(path & 1u) : 0;
+    const double* m = p + (side ? 15 : 3);
+    V3<T> n = Csg<T, D - 1>::normal(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0,
+                                    m2l<T>(m, m + 9, q), m2lv<T>(m, d));
+    return l2mv<T>(m, n);
+  }
+  // FP64 crossing of the leaf primitive `path` near the estimate s0 along
+  // the ray: the primitive's own entry/exit distance from a point backed
+  // off by `back` (the mixed form's root polish). Returns kBig on failure.
+  __device__ __noinline__ static double polish(const DScene& S, int shape, uint32_t path, int depth, bool exiting, V3<T> q,
+                                  V3<T> d, T s0, T back) {
+    DShape sh = S.shapes[shape];
+    const double* p = S.params + sh.off;
+    if (sh.type != RB_SHAPE_COMPOSITE) {
+      T t0 = s0 - back > T(0) ? s0 - back : T(0);
+      V3<T> q0 = axpy(q, t0, d);
+      T t = exiting ? prim_dist_in<T>(sh.type, p, q0, d) : prim_dist_out<T>(sh.type, p, q0, d);
+      return t >= T(kBig) ? kBig : double(t0 + t);
+    }
+    int side = depth > 0 ? int(path & 1u) : 0;
+    const double* m = p + (side ? 15 : 3);
+    return Csg<T, D - 1>::polish(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0, exiting,
+                                 m2l<T>(m, m + 9, q), m2lv<T>(m, d), s0, back);
   }
 };
 // depth exhausted: primitives only (the host rejects deeper expressions)
 template <typename T> struct Csg<T, 0> {
-  __device__ static bool contains(const DScene& S, int shape, V3<T> q) {
+  __device__ __noinline__ static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
     return prim_contains<T>(sh.type, S.params + sh.off, q);
   }
-  __device__ static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ __noinline__ static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
-    return {double(prim_dist_out<T>(sh.type, S.params + sh.off, q, d)), 0u};
+    return {double(prim_dist_out<T>(sh.type, S.params + sh.off, q, d)), 0u, 0u};
   }
-  __device__ static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ __noinline__ static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
-    return {double(prim_dist_in<T>(sh.type, S.params + sh.off, q, d)), 0u};
+    return {double(prim_dist_in<T>(sh.type, S.params + sh.off, q, d)), 0u, 1u};
+  }
+  __device__ __noinline__ static V3<T> normal(const DScene& S, int shape, uint32_t, int, V3<T> q, V3<T> d) {
+    DShape sh = S.shapes[shape];
+    return prim_normal<T>(sh.type, S.params + sh.off, q, d);
+  }
+  __device__ __noinline__ static double polish(const DScene& S, int shape, uint32_t, int, bool exiting, V3<T> q, V3<T> d, T s0,
+                                  T back) {
+    DShape sh = S.shapes[shape];
+    const double* p = S.params + sh.off;
+    T t0 = s0 - back > T(0) ? s0 - back : T(0);
+    V3<T> q0 = axpy(q, t0, d);
+    T t = exiting ? prim_dist_in<T>(sh.type, p, q0, d) : prim_dist_out<T>(sh.type, p, q0, d);
+    return t >= T(kBig) ? kBig : double(t0 + t);
   }
 };
 
 template <typename T>
-__device__ V3<T> shape_normal(const DScene& S, int shape, uint32_t leaf, V3<T> q, V3<T> d) {
-  const double* mats[kMaxCsgDepth + 1];
-  int ns = 0;
-  int maxdepth = int(leaf >> 24);
-  DShape sh = S.shapes[shape];
-  while (sh.type == RB_SHAPE_COMPOSITE && ns < kMaxCsgDepth + 1) {
-    const double* p = S.params + sh.off;
-    int side = ns < maxdepth ? int((leaf >> ns) & 1u) : 0;
-    const double* m = p + (side ? 15 : 3);
-    q = m2l<T>(m, m + 9, q);
-    d = m2lv<T>(m, d);
-    mats[ns++] = m;
-    sh = S.shapes[int(p[side ? 2 : 1])];
-  }
-  V3<T> n = prim_normal<T>(sh.type, S.params + sh.off, q, d);
-  while (ns > 0) n = l2mv<T>(mats[--ns], n);
-  return n;
+__device__ __forceinline__ V3<T> shape_normal(const DScene& S, int shape, uint32_t leaf, V3<T> q, V3<T> d) {
+  return Csg<T, kMaxCsgDepth>::normal(S, shape, leaf & 0xffffffu, int(leaf >> 24), q, d);
 }
 
 // ------------------------------------------------------------ physics
@@ -755,10 +795,16 @@ __device__ inline double table_eval(const DScene& S, int table, double x) {
     else hi = mid;
   }
   if (lo < n && xy[2 * lo] == x) return xy[2 * lo + 1];
-  int low = lo - 1, up = lo;
-  if (up == n) { up = n - 1; low = n - 2; }
-  if (low < 0) { low = 0; up = 1; }
-  double xl = xy[2 * low], yl = xy[2 * low + 1], xu = xy[2 * up], yu = xy[2 * up + 1];
+  double xl, yl, xu, yu;
+  if (lo == n) {  // above the range: stored (up, low) pair
+    const double* e = xy + 2 * n;
+    xu = e[0]; yu = e[1]; xl = e[2]; yl = e[3];
+  } else if (lo == 0) {  // below the range: stored (low, up) pair
+    const double* e = xy + 2 * n + 4;
+    xl = e[0]; yl = e[1]; xu = e[2]; yu = e[3];
+  } else {
+    xl = xy[2 * (lo - 1)]; yl = xy[2 * (lo - 1) + 1]; xu = xy[2 * lo]; yu = xy[2 * lo + 1];
+  }
   if (xl == xu) return yl;
   return yu + (x - xu) * (yl - yu) / (xl - xu);
 }

@@ -1,0 +1,284 @@
+// rb_scene_build.h -- host side of rbgpu_scene_upload: validation of a
+// flattened scene, the per-node BVH build and packing into one blob whose
+// layout the device reads through DScene. Plain C++ (no HIP calls).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rbgpu.h"
+#include "rb_device.h"
+
+namespace rbd {
+
+inline int fail_msg(std::string& err, int code, const std::string& msg) {
+  err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------- BVH build
+struct Box {
+  double lo[3], hi[3];
+};
+
+Box node_box(const rb_scene_desc* d, int n) {
+  const rb_node_desc& nd = d->nodes[n];
+  const rb_shape_desc& sh = d->shapes[d->volumes[nd.volume].shape];
+  Box b;
+  for (int k = 0; k < 3; k++) {
+    b.lo[k] = 1e300;
+    b.hi[k] = -1e300;
+  }
+  for (int c = 0; c < 8; c++) {
+    double l[3];
+    for (int k = 0; k < 3; k++) l[k] = sh.bbox_origin[k] + ((c >> k) & 1 ? 1 : -1) * sh.bbox_half[k];
+    for (int r = 0; r < 3; r++) {
+      double m = nd.global.tr[r] + nd.global.rot[3 * r] * l[0] + nd.global.rot[3 * r + 1] * l[1] +
+                 nd.global.rot[3 * r + 2] * l[2];
+      b.lo[r] = std::min(b.lo[r], m);
+      b.hi[r] = std::max(b.hi[r], m);
+    }
+  }
+  // conservative expansion so FP32 slab tests never cull a true hit
+  for (int k = 0; k < 3; k++) {
+    b.lo[k] -= 1e-3 + 1e-6 * std::fabs(b.lo[k]);
+    b.hi[k] += 1e-3 + 1e-6 * std::fabs(b.hi[k]);
+  }
+  return b;
+}
+
+float down(double v) {
+  float f = float(v);
+  return double(f) > v ? std::nextafter(f, -INFINITY) : f;
+}
+float up(double v) {
+  float f = float(v);
+  return double(f) < v ? std::nextafter(f, INFINITY) : f;
+}
+
+// Depth-first BVH with escape links (stackless traversal): node i's first
+// child is i + 1; `escape` is the index just past its subtree. Leaves hold
+// one scene node. Median split on the longest centroid axis.
+struct BvhBuilder {
+  std::vector<DBvh> nodes;
+
+  int build(std::vector<std::pair<int, Box>>& items) {
+    int root = int(nodes.size());
+    fill(items, 0, int(items.size()));
+    return root;
+  }
+  void fill(std::vector<std::pair<int, Box>>& items, int b, int e) {
+    Box bb;
+    for (int k = 0; k < 3; k++) {
+      bb.lo[k] = 1e300;
+      bb.hi[k] = -1e300;
+    }
+    for (int i = b; i < e; i++)
+      for (int k = 0; k < 3; k++) {
+        bb.lo[k] = std::min(bb.lo[k], items[i].second.lo[k]);
+        bb.hi[k] = std::max(bb.hi[k], items[i].second.hi[k]);
+      }
+    int idx = int(nodes.size());
+    nodes.push_back(DBvh{});
+    DBvh nd{};
+    for (int k = 0; k < 3; k++) {
+      nd.lo[k] = down(bb.lo[k]);
+      nd.hi[k] = up(bb.hi[k]);
+    }
+    if (e - b == 1) {
+      nd.prim = items[b].first;
+      nd.escape = idx + 1;
+      nodes[idx] = nd;
+      return;
+    }
+    double cl[3], ch[3];
+    for (int k = 0; k < 3; k++) {
+      cl[k] = 1e300;
+      ch[k] = -1e300;
+    }
+    for (int i = b; i < e; i++)
+      for (int k = 0; k < 3; k++) {
+        double c = 0.5 * (items[i].second.lo[k] + items[i].second.hi[k]);
+        cl[k] = std::min(cl[k], c);
+        ch[k] = std::max(ch[k], c);
+      }
+    int ax = 0;
+    for (int k = 1; k < 3; k++)
+      if (ch[k] - cl[k] > ch[ax] - cl[ax]) ax = k;
+    int mid = (b + e) / 2;
+    std::nth_element(items.begin() + b, items.begin() + mid, items.begin() + e,
+                     [ax](const std::pair<int, Box>& x, const std::pair<int, Box>& y) {
+                       double cx = x.second.lo[ax] + x.second.hi[ax], cy = y.second.lo[ax] + y.second.hi[ax];
+                       return cx < cy || (cx == cy && x.first < y.first);
+                     });
+    fill(items, b, mid);
+    fill(items, mid, e);
+    nd.prim = -1;
+    nd.escape = int(nodes.size());
+    nodes[idx] = nd;
+  }
+};
+
+int csg_depth(const rb_scene_desc* d, int shape, int guard) {
+  if (guard > 16) return 99;
+  const rb_shape_desc& s = d->shapes[shape];
+  if (s.type != RB_SHAPE_COMPOSITE) return 0;
+  const double* p = d->params + s.param_offset;
+  int l = int(p[1]), r = int(p[2]);
+  if (l < 0 || l >= d->n_shapes || r < 0 || r >= d->n_shapes) return 99;
+  return 1 + std::max(csg_depth(d, l, guard + 1), csg_depth(d, r, guard + 1));
+}
+
+int validate(std::string& err, const rb_scene_desc* d) {
+  if (!d || d->n_nodes < 1 || !d->nodes || !d->volumes || !d->shapes || !d->params)
+    return fail_msg(err, -1, "scene: empty or null arrays");
+  for (int i = 0; i < d->n_shapes; i++) {
+    const rb_shape_desc& s = d->shapes[i];
+    if (s.param_offset < 0 || s.param_offset + s.n_params > d->n_params)
+      return fail_msg(err, -1, "scene: shape parameters out of range");
+    switch (s.type) {
+      case RB_SHAPE_BOX:
+        if (s.n_params != 6) return fail_msg(err, -1, "scene: box needs 6 parameters");
+        break;
+      case RB_SHAPE_TUBE:
+        if (s.n_params != 3) return fail_msg(err, -1, "scene: tube needs 3 parameters");
+        break;
+      case RB_SHAPE_SPHERE:
+        if (s.n_params != 8) return fail_msg(err, -1, "scene: sphere needs 8 parameters");
+        break;
+      case RB_SHAPE_PARABOLOID:
+        if (s.n_params != 5) return fail_msg(err, -1, "scene: paraboloid needs 5 parameters");
+        break;
+      case RB_SHAPE_PGON: {
+        const double* p = d->params + s.param_offset;
+        int nz = int(p[3]);
+        if (nz < 2 || int(p[2]) < 3 || s.n_params != 4 + 3 * nz + 2 * int(p[2]) || p[1] != 360.0)
+          return fail_msg(err, -3, "scene: unsupported TGeoPgon parameters");
+        for (int k = 0; k < nz; k++)
+          if (p[4 + 3 * k + 1] != 0.0) return fail_msg(err, -3, "scene: TGeoPgon with rmin > 0 is unsupported");
+        break;
+      }
+      case RB_SHAPE_COMPOSITE:
+        if (s.n_params != 27) return fail_msg(err, -1, "scene: composite needs 27 parameters");
+        if (csg_depth(d, i, 0) > kMaxCsgDepth)
+          return fail_msg(err, -3, "scene: composite nesting deeper than " + std::to_string(kMaxCsgDepth) + " boolean levels");
+        break;
+      default:
+        return fail_msg(err, -3, "scene: unsupported shape type " + std::to_string(s.type));
+    }
+  }
+  for (int i = 0; i < d->n_volumes; i++) {
+    const rb_volume_desc& v = d->volumes[i];
+    if (v.shape < 0 || v.shape >= d->n_shapes) return fail_msg(err, -1, "scene: volume shape out of range");
+    if (v.kind < RB_KLENS || v.kind > RB_KOTHER) return fail_msg(err, -1, "scene: bad volume kind");
+    int tabs[5] = {v.refl_table, v.k_table, v.qe_lambda_table, v.qe_angle_table,
+                   v.index_model == RB_INDEX_TABLE ? v.index_ref : -1};
+    for (int t : tabs)
+      if (t >= d->n_tables) return fail_msg(err, -1, "scene: table index out of range");
+    if (v.index_model == RB_INDEX_TABLE && v.index_ref < 0) return fail_msg(err, -1, "scene: missing index table");
+    if ((v.index_model == RB_INDEX_SELLMEIER || v.index_model == RB_INDEX_SCHOTT) &&
+        (v.index_ref < 0 || v.index_ref + 6 > d->n_params))
+      return fail_msg(err, -1, "scene: index formula parameters out of range");
+  }
+  for (int i = 0; i < d->n_tables; i++) {
+    const rb_table_desc& t = d->tables[i];
+    if (t.offset < 0 || t.n < 0 || 2 * (int64_t(t.offset) + t.n + 4) > d->n_table_xy)
+      return fail_msg(err, -1, "scene: table out of range");
+    const double* xy = d->table_xy + 2 * t.offset;
+    for (int k = 1; k < t.n; k++)
+      if (xy[2 * k] < xy[2 * (k - 1)]) return fail_msg(err, -1, "scene: table x must be ascending");
+  }
+  if (d->nodes[0].parent != -1) return fail_msg(err, -1, "scene: node 0 must be the top node");
+  for (int i = 0; i < d->n_nodes; i++) {
+    const rb_node_desc& n = d->nodes[i];
+    if (n.volume < 0 || n.volume >= d->n_volumes) return fail_msg(err, -1, "scene: node volume out of range");
+    if (n.first_child < 0 || n.n_children < 0 || n.first_child + n.n_children > std::max(d->n_children, 0))
+      return fail_msg(err, -1, "scene: child range out of bounds");
+    for (int k = 0; k < n.n_children; k++) {
+      int c = d->children[n.first_child + k];
+      if (c <= 0 || c >= d->n_nodes || d->nodes[c].parent != i)
+        return fail_msg(err, -1, "scene: inconsistent child list");
+      if (k > 0 && c <= d->children[n.first_child + k - 1])
+        return fail_msg(err, -1, "scene: child ids must ascend");
+    }
+  }
+  return 0;
+}
+
+template <typename X>
+size_t put(std::vector<char>& blob, const X* src, size_t count) {
+  size_t off = (blob.size() + 255) & ~size_t(255);
+  blob.resize(off + sizeof(X) * std::max<size_t>(count, 1));
+  if (count) std::memcpy(blob.data() + off, src, sizeof(X) * count);
+  return off;
+}
+
+
+// Offsets of the DScene arrays inside the packed blob.
+struct SceneLayout {
+  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy;
+  int32_t n_nodes;
+};
+
+// Validate d, build BVHs and pack everything into blob. 0 = ok.
+inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayout& lay, std::string& err) {
+  int rc = validate(err, d);
+  if (rc) return rc;
+  std::vector<DNode> nodes(d->n_nodes);
+  BvhBuilder bvh;
+  for (int i = 0; i < d->n_nodes; i++) {
+    const rb_node_desc& n = d->nodes[i];
+    DNode& o = nodes[i];
+    std::memcpy(o.rot, n.global.rot, sizeof(o.rot));
+    std::memcpy(o.tr, n.global.tr, sizeof(o.tr));
+    o.volume = n.volume;
+    o.shape = d->volumes[n.volume].shape;
+    o.kind = d->volumes[n.volume].kind;
+    o.parent = n.parent;
+    o.first_child = n.first_child;
+    o.n_children = n.n_children;
+    o.bvh_root = -1;
+    o.pad = 0;
+    if (n.n_children > 0) {
+      std::vector<std::pair<int, Box>> items;
+      for (int k = 0; k < n.n_children; k++) {
+        int c = d->children[n.first_child + k];
+        items.emplace_back(c, node_box(d, c));
+      }
+      o.bvh_root = bvh.build(items);
+    }
+  }
+  std::vector<DShape> shapes(d->n_shapes);
+  for (int i = 0; i < d->n_shapes; i++) shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset};
+  blob.clear();
+  lay.nodes = put(blob, nodes.data(), nodes.size());
+  lay.children = put(blob, d->children, size_t(std::max(d->n_children, 0)));
+  lay.bvh = put(blob, bvh.nodes.data(), bvh.nodes.size());
+  lay.shapes = put(blob, shapes.data(), shapes.size());
+  lay.params = put(blob, d->params, size_t(d->n_params));
+  lay.volumes = put(blob, d->volumes, size_t(d->n_volumes));
+  lay.tables = put(blob, d->tables, size_t(std::max(d->n_tables, 0)));
+  lay.table_xy = put(blob, d->table_xy, size_t(std::max(d->n_table_xy, 0)));
+  lay.n_nodes = d->n_nodes;
+  return 0;
+}
+
+// DScene view of a packed blob at `base` (host or device address).
+inline DScene scene_view(const char* base, const SceneLayout& lay) {
+  DScene s;
+  s.nodes = reinterpret_cast<const DNode*>(base + lay.nodes);
+  s.children = reinterpret_cast<const int32_t*>(base + lay.children);
+  s.bvh = reinterpret_cast<const DBvh*>(base + lay.bvh);
+  s.shapes = reinterpret_cast<const DShape*>(base + lay.shapes);
+  s.params = reinterpret_cast<const double*>(base + lay.params);
+  s.volumes = reinterpret_cast<const rb_volume_desc*>(base + lay.volumes);
+  s.tables = reinterpret_cast<const rb_table_desc*>(base + lay.tables);
+  s.table_xy = reinterpret_cast<const double*>(base + lay.table_xy);
+  s.n_nodes = lay.n_nodes;
+  return s;
+}
+
+}  // namespace rbd

@@ -5,6 +5,10 @@
 
 namespace rbd {
 
+#ifdef RB_HOST_EMULATION
+inline long rb_emul_fallbacks = 0;
+#endif
+
 struct RayIO {
   double *x, *y, *z, *t, *dx, *dy, *dz;
   const double* lambda;
@@ -33,7 +37,7 @@ struct Lane {
   uint32_t hit_leaf;
 };
 
-__device__ __forceinline__ bool node_contains(const DScene& S, int n, V3<double> p) {
+__device__ __noinline__ bool node_contains(const DScene& S, int n, V3<double> p) {
   const DNode& nd = S.nodes[n];
   return Csg<double, kMaxCsgDepth>::contains(S, nd.shape, m2l<double>(nd.rot, nd.tr, p));
 }
@@ -42,36 +46,29 @@ __device__ __forceinline__ bool box_has(const DBvh& b, float x, float y, float z
   return x >= b.lo[0] && x <= b.hi[0] && y >= b.lo[1] && y <= b.hi[1] && z >= b.lo[2] && z <= b.hi[2];
 }
 
-// First daughter of n (lowest index, excluding skip) whose shape contains p.
+// First daughter of n (lowest node id, excluding skip) whose shape contains
+// p; stackless walk of n's BVH (depth-first order with escape links).
 __device__ int first_daughter_containing(const DScene& S, int n, V3<double> p, int skip) {
   const DNode& nd = S.nodes[n];
   if (nd.n_children == 0) return -1;
-  float fx = float(p.x), fy = float(p.y), fz = float(p.z);
+  const float fx = float(p.x), fy = float(p.y), fz = float(p.z);
   int found = 0x7fffffff;
-  int stack[kBvhStack];
-  int sp = 0;
-  stack[sp++] = nd.bvh_root;
-  while (sp > 0) {
-    const DBvh b = S.bvh[stack[--sp]];
-    if (!box_has(b, fx, fy, fz)) continue;
-    if (b.count == 0) {
-      if (sp + 2 <= kBvhStack) {
-        stack[sp++] = b.left;
-        stack[sp++] = b.left + 1;
-      }
+  int i = nd.bvh_root;
+  const int end = S.bvh[i].escape;
+  while (i < end) {
+    const DBvh b = S.bvh[i];
+    if (!box_has(b, fx, fy, fz)) {
+      i = b.escape;
       continue;
     }
-    for (int k = 0; k < b.count; k++) {
-      int c = S.bvh_prims[b.left + k];
-      if (c == skip || c >= found) continue;
-      if (node_contains(S, c, p)) found = c;
-    }
+    if (b.prim >= 0 && b.prim != skip && b.prim < found && node_contains(S, b.prim, p)) found = b.prim;
+    i++;
   }
   return found == 0x7fffffff ? -1 : found;
 }
 
 // TGeoNavigator::SearchNode (up until inside, then down through daughters)
-__device__ int search_node(const DScene& S, int start, V3<double> p, int skip, bool forced) {
+__device__ __noinline__ int search_node(const DScene& S, int start, V3<double> p, int skip, bool forced) {
   int n = start;
   bool inside = forced || n == skip;
   while (!inside) {
@@ -98,16 +95,158 @@ __device__ __forceinline__ double locate_push(const DScene& S, int n, double ste
   return 100. * ((1. + fabs(t[0]) + fabs(t[1]) + fabs(t[2])) + step) * kTol;
 }
 
-__device__ __forceinline__ Hit node_dist_out(const DScene& S, int c, V3<double> p, V3<double> d) {
+// distance from a ray (origin p, unit d, FP64) to node c's shape, the
+// search itself in precision T; the origin is first rebased to `base`
+// along the ray (the node's box entry) so FP32 works near the node's frame
+template <typename T>
+__device__ __noinline__ Hit node_dist(const DScene& S, int c, V3<double> p, V3<double> d, double base, bool out) {
   const DNode& nd = S.nodes[c];
-  return Csg<double, kMaxCsgDepth>::dist_out(S, nd.shape, m2l<double>(nd.rot, nd.tr, p), m2lv<double>(nd.rot, d));
-}
-__device__ __forceinline__ Hit node_dist_in(const DScene& S, int c, V3<double> p, V3<double> d) {
-  const DNode& nd = S.nodes[c];
-  return Csg<double, kMaxCsgDepth>::dist_in(S, nd.shape, m2l<double>(nd.rot, nd.tr, p), m2lv<double>(nd.rot, d));
+  V3<double> pb = axpy(p, base, d);
+  V3<double> q = m2l<double>(nd.rot, nd.tr, pb), ld = m2lv<double>(nd.rot, d);
+  V3<T> qt = mk<T>(T(q.x), T(q.y), T(q.z)), dt = mk<T>(T(ld.x), T(ld.y), T(ld.z));
+  Hit h = out ? Csg<T, kMaxCsgDepth>::dist_out(S, nd.shape, qt, dt) : Csg<T, kMaxCsgDepth>::dist_in(S, nd.shape, qt, dt);
+  if (h.s < kBig) h.s += base;
+  return h;
 }
 
-// TGeoNavigator::FindNextBoundaryAndStep; returns the next node (-1 = null)
+// FP64 root polish of a crossing found in FP32 (leaf surface of node c)
+__device__ __noinline__ double node_polish(const DScene& S, int c, const Hit& h, V3<double> p, V3<double> d) {
+  const DNode& nd = S.nodes[c];
+  V3<double> q = m2l<double>(nd.rot, nd.tr, p), ld = m2lv<double>(nd.rot, d);
+  const double back = 2e-3 + 1e-6 * h.s;
+  double s = Csg<double, kMaxCsgDepth>::polish(S, nd.shape, h.leaf & 0xffffffu, int(h.leaf >> 24), h.exiting != 0,
+                                                q, ld, h.s, back);
+  return (s < kBig && fabs(s - h.s) <= back) ? s : -1.0;
+}
+
+// Find the nearest boundary of the current node `cur` for the ray at p:
+// its own exit (crossed = -1) or a daughter's entry (crossed = node id),
+// all shape distances in FP64 (the plain form).
+__device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<double> d, double& best,
+                                     int& crossed, Hit& hit) {
+  const DNode& cn = S.nodes[cur];
+  Hit hx = node_dist<double>(S, cur, p, d, 0.0, false);
+  best = kBig;
+  crossed = -2;
+  if (hx.s < best - kTol) {
+    best = hx.s;
+    crossed = -1;
+    hit = hx;
+  }
+  if (cn.n_children == 0) return;
+  const float ox = float(p.x), oy = float(p.y), oz = float(p.z);
+  const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
+  int i = cn.bvh_root;
+  const int end = S.bvh[i].escape;
+  while (i < end) {
+    const DBvh b = S.bvh[i];
+    float t0x = (b.lo[0] - ox) * ix, t1x = (b.hi[0] - ox) * ix;
+    float t0y = (b.lo[1] - oy) * iy, t1y = (b.hi[1] - oy) * iy;
+    float t0z = (b.lo[2] - oz) * iz, t1z = (b.hi[2] - oz) * iz;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    float lim = float(best) * 1.0001f + 1e-2f;
+    if (!(tn <= tf * 1.0001f + 1e-3f) || tn > lim) {
+      i = b.escape;
+      continue;
+    }
+    if (b.prim >= 0) {
+      const int c = b.prim;
+      Hit h = node_dist<double>(S, c, p, d, 0.0, true);
+      if (h.s < best - kTol || (crossed >= 0 && c < crossed && h.s < best + kTol && h.s < kBig)) {
+        best = h.s;
+        crossed = c;
+        hit = h;
+      }
+    }
+    i++;
+  }
+}
+
+// Mixed form of the same search. Candidates are ranked with FP32 shape
+// distances in frames rebased to each daughter's box entry; a candidate
+// within kNearF32 of the ray origin (the surface just left, or one the ray
+// sits on) is resolved in FP64 on the spot, since FP32 cannot tell which
+// side of it the ray is. The winner is then polished in FP64 and checked
+// with an FP64 inside test; near-ties and failed checks redo the search
+// in FP64. Returns false when the caller must run the FP64 search.
+constexpr double kNearF32 = 2e-2;
+__device__ bool search_boundary_mixed(const DScene& S, int cur, V3<double> p, V3<double> d, double& best,
+                                      int& crossed, Hit& hit) {
+  const DNode& cn = S.nodes[cur];
+  best = kBig;
+  crossed = -2;
+  bool exact = false;
+  double second = kBig;
+  Hit hx = node_dist<float>(S, cur, p, d, 0.0, false);
+  bool hx_exact = false;
+  if (hx.s < kNearF32) {
+    hx = node_dist<double>(S, cur, p, d, 0.0, false);
+    hx_exact = true;
+  }
+  if (hx.s < kBig) {
+    best = hx.s;
+    crossed = -1;
+    hit = hx;
+    exact = hx_exact;
+  }
+  if (cn.n_children > 0) {
+    const float ox = float(p.x), oy = float(p.y), oz = float(p.z);
+    const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
+    int i = cn.bvh_root;
+    const int end = S.bvh[i].escape;
+    while (i < end) {
+      const DBvh b = S.bvh[i];
+      float t0x = (b.lo[0] - ox) * ix, t1x = (b.hi[0] - ox) * ix;
+      float t0y = (b.lo[1] - oy) * iy, t1y = (b.hi[1] - oy) * iy;
+      float t0z = (b.lo[2] - oz) * iz, t1z = (b.hi[2] - oz) * iz;
+      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+      float lim = float(fmin(best, second)) * 1.0001f + 1e-2f;
+      if (!(tn <= tf * 1.0001f + 1e-3f) || tn > lim) {
+        i = b.escape;
+        continue;
+      }
+      if (b.prim >= 0) {
+        const int c = b.prim;
+        const double base = fmax(0.0, double(tn) * (1.0 - 1e-6) - 1.0);
+        Hit h = node_dist<float>(S, c, p, d, base, true);
+        bool hexact = false;
+        if (h.s < kNearF32) {
+          h = node_dist<double>(S, c, p, d, 0.0, true);
+          hexact = true;
+        }
+        if (h.s < best) {
+          second = best;
+          best = h.s;
+          crossed = c;
+          hit = h;
+          exact = hexact;
+        } else if (h.s < second) {
+          second = h.s;
+        }
+      }
+      i++;
+    }
+  }
+  if (crossed == -2) return true;
+  const double margin = 1e-2 + 1e-6 * best;
+  if (second < best + margin) return false;  // near-tie: rank in FP64
+  if (exact) return true;
+  const int node = crossed == -1 ? cur : crossed;
+  double s = node_polish(S, node, hit, p, d);
+  if (s < 0.0) return false;
+  // the polished crossing must really enter (leave) the node in FP64
+  const bool inside = node_contains(S, node, axpy(p, s + kBoolPush, d));
+  if (inside != (crossed != -1)) return false;
+  best = s;
+  hit.s = s;
+  return true;
+}
+
+// TGeoNavigator::FindNextBoundaryAndStep; returns the next node (-1 = null).
+// MIXED: the search runs in FP32 and the winning crossing is polished in FP64.
+template <bool MIXED>
 __device__ int next_boundary(const DScene& S, Lane& L) {
   double extra = L.on_b ? kTol : 0.0;
   L.on_b = false;
@@ -115,7 +254,7 @@ __device__ int next_boundary(const DScene& S, Lane& L) {
   L.hit_node = -1;
   L.hit_leaf = 0;
   if (L.cur < 0) {
-    Hit h = node_dist_out(S, 0, L.p, L.d);
+    Hit h = node_dist<double>(S, 0, L.p, L.d, 0.0, true);
     if (h.s < kBig - kTol) {
       L.step = h.s + extra;
       L.p = axpy(L.p, h.s, L.d);
@@ -131,64 +270,31 @@ __device__ int next_boundary(const DScene& S, Lane& L) {
   }
   const int cur = L.cur;
   const DNode& cn = S.nodes[cur];
-  Hit hx = node_dist_in(S, cur, L.p, L.d);
-  if (hx.s <= kTol) {
+  double best;
+  int crossed;
+  Hit hit = {kBig, 0u, 0u};
+  if (!MIXED || !search_boundary_mixed(S, cur, L.p, L.d, best, crossed, hit)) {
+#ifdef RB_HOST_EMULATION
+    if (MIXED) rb_emul_fallbacks++;
+#endif
+    search_boundary_fp64(S, cur, L.p, L.d, best, crossed, hit);
+  }
+  if (crossed == -1 && best <= kTol) {
+    // TGeoNavigator: do not stay on the exit surface; step by the tolerance
     L.step = kTol;
     L.p = axpy(L.p, kTol, L.d);
     L.on_b = true;
     L.hit_node = cur;
-    L.hit_leaf = hx.leaf;
+    L.hit_leaf = hit.leaf;
     V3<double> pp = axpy(L.p, locate_push(S, cur, L.step), L.d);
     L.cur = search_node(S, cur, pp, cur, false);
     return L.cur;
-  }
-  double best = kBig;
-  int crossed = -2;
-  uint32_t leaf = 0;
-  if (hx.s < best - kTol) {
-    best = hx.s;
-    crossed = -1;
-    leaf = hx.leaf;
-  }
-  if (cn.n_children > 0) {
-    // BVH over daughters, FP32 slab tests on conservatively expanded boxes
-    const float ox = float(L.p.x), oy = float(L.p.y), oz = float(L.p.z);
-    const float ix = 1.0f / float(L.d.x), iy = 1.0f / float(L.d.y), iz = 1.0f / float(L.d.z);
-    int stack[kBvhStack];
-    int sp = 0;
-    stack[sp++] = cn.bvh_root;
-    while (sp > 0) {
-      const DBvh b = S.bvh[stack[--sp]];
-      float t0x = (b.lo[0] - ox) * ix, t1x = (b.hi[0] - ox) * ix;
-      float t0y = (b.lo[1] - oy) * iy, t1y = (b.hi[1] - oy) * iy;
-      float t0z = (b.lo[2] - oz) * iz, t1z = (b.hi[2] - oz) * iz;
-      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
-      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-      float lim = float(best) * 1.0001f + 1e-2f;
-      if (!(tn <= tf * 1.0001f + 1e-3f) || tn > lim) continue;
-      if (b.count == 0) {
-        if (sp + 2 <= kBvhStack) {
-          stack[sp++] = b.left + 1;
-          stack[sp++] = b.left;
-        }
-        continue;
-      }
-      for (int k = 0; k < b.count; k++) {
-        int c = S.bvh_prims[b.left + k];
-        Hit h = node_dist_out(S, c, L.p, L.d);
-        if (h.s < best - kTol || (crossed >= 0 && c < crossed && h.s < best + kTol && h.s < kBig)) {
-          best = h.s;
-          crossed = c;
-          leaf = h.leaf;
-        }
-      }
-    }
   }
   L.p = axpy(L.p, best, L.d);
   L.step = best + extra;
   if (crossed == -2) return cur;
   L.on_b = true;
-  L.hit_leaf = leaf;
+  L.hit_leaf = hit.leaf;
   if (crossed == -1) {
     L.hit_node = cur;
     if (cn.parent < 0) {
@@ -319,6 +425,7 @@ __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& i
 }
 
 // AOpticsManager::TraceNonSequential(TObjArray*) for one ray (src/AOpticsManager.cxx:353-518)
+template <bool MIXED>
 __device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L) {
   const uint64_t id = P.id_base + uint64_t(i);
   L.on_b = false;
@@ -328,7 +435,7 @@ __device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& i
     const V3<double> x1 = L.p;
     const double t1 = L.t;
     const int current = L.cur;
-    const int next = next_boundary(S, L);
+    const int next = next_boundary<MIXED>(S, L);
     const double step = L.step;
     const int tc = kind_of(S, current), tn = kind_of(S, next);
 

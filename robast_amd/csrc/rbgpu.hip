@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/rbgpu.h"
+#include "rb_scene_build.h"
 #include "rb_trace.h"
 
 using namespace rbd;
@@ -42,199 +43,6 @@ int fail(rb_ctx* c, int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(ctx, -2, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-// ---------------------------------------------------------------- BVH build
-struct Box {
-  double lo[3], hi[3];
-};
-
-Box node_box(const rb_scene_desc* d, int n) {
-  const rb_node_desc& nd = d->nodes[n];
-  const rb_shape_desc& sh = d->shapes[d->volumes[nd.volume].shape];
-  Box b;
-  for (int k = 0; k < 3; k++) {
-    b.lo[k] = 1e300;
-    b.hi[k] = -1e300;
-  }
-  for (int c = 0; c < 8; c++) {
-    double l[3];
-    for (int k = 0; k < 3; k++) l[k] = sh.bbox_origin[k] + ((c >> k) & 1 ? 1 : -1) * sh.bbox_half[k];
-    for (int r = 0; r < 3; r++) {
-      double m = nd.global.tr[r] + nd.global.rot[3 * r] * l[0] + nd.global.rot[3 * r + 1] * l[1] +
-                 nd.global.rot[3 * r + 2] * l[2];
-      b.lo[r] = std::min(b.lo[r], m);
-      b.hi[r] = std::max(b.hi[r], m);
-    }
-  }
-  // conservative expansion so FP32 slab tests never cull a true hit
-  for (int k = 0; k < 3; k++) {
-    b.lo[k] -= 1e-3 + 1e-6 * std::fabs(b.lo[k]);
-    b.hi[k] += 1e-3 + 1e-6 * std::fabs(b.hi[k]);
-  }
-  return b;
-}
-
-float down(double v) {
-  float f = float(v);
-  return double(f) > v ? std::nextafter(f, -INFINITY) : f;
-}
-float up(double v) {
-  float f = float(v);
-  return double(f) < v ? std::nextafter(f, INFINITY) : f;
-}
-
-struct BvhBuilder {
-  std::vector<DBvh> nodes;
-  std::vector<int32_t> prims;
-
-  // Median split on the longest centroid axis; leaves hold <= 2 prims.
-  // Returns the index of the subtree root; inner nodes keep both children
-  // adjacent (right = left + 1).
-  int build(std::vector<std::pair<int, Box>>& items, int b, int e) {
-    int idx = int(nodes.size());
-    nodes.push_back(DBvh{});
-    fill(idx, items, b, e);
-    return idx;
-  }
-  void fill(int idx, std::vector<std::pair<int, Box>>& items, int b, int e) {
-    Box bb;
-    for (int k = 0; k < 3; k++) {
-      bb.lo[k] = 1e300;
-      bb.hi[k] = -1e300;
-    }
-    for (int i = b; i < e; i++)
-      for (int k = 0; k < 3; k++) {
-        bb.lo[k] = std::min(bb.lo[k], items[i].second.lo[k]);
-        bb.hi[k] = std::max(bb.hi[k], items[i].second.hi[k]);
-      }
-    DBvh nd{};
-    for (int k = 0; k < 3; k++) {
-      nd.lo[k] = down(bb.lo[k]);
-      nd.hi[k] = up(bb.hi[k]);
-    }
-    if (e - b <= 2) {
-      nd.left = int(prims.size());
-      nd.count = e - b;
-      for (int i = b; i < e; i++) prims.push_back(items[i].first);
-      nodes[idx] = nd;
-      return;
-    }
-    double cl[3], ch[3];
-    for (int k = 0; k < 3; k++) {
-      cl[k] = 1e300;
-      ch[k] = -1e300;
-    }
-    for (int i = b; i < e; i++)
-      for (int k = 0; k < 3; k++) {
-        double c = 0.5 * (items[i].second.lo[k] + items[i].second.hi[k]);
-        cl[k] = std::min(cl[k], c);
-        ch[k] = std::max(ch[k], c);
-      }
-    int ax = 0;
-    for (int k = 1; k < 3; k++)
-      if (ch[k] - cl[k] > ch[ax] - cl[ax]) ax = k;
-    int mid = (b + e) / 2;
-    std::nth_element(items.begin() + b, items.begin() + mid, items.begin() + e,
-                     [ax](const std::pair<int, Box>& x, const std::pair<int, Box>& y) {
-                       return x.second.lo[ax] + x.second.hi[ax] < y.second.lo[ax] + y.second.hi[ax];
-                     });
-    int left = int(nodes.size());
-    nodes.push_back(DBvh{});
-    nodes.push_back(DBvh{});
-    nd.left = left;
-    nd.count = 0;
-    nodes[idx] = nd;
-    fill(left, items, b, mid);
-    fill(left + 1, items, mid, e);
-  }
-};
-
-int csg_depth(const rb_scene_desc* d, int shape, int guard) {
-  if (guard > 16) return 99;
-  const rb_shape_desc& s = d->shapes[shape];
-  if (s.type != RB_SHAPE_COMPOSITE) return 0;
-  const double* p = d->params + s.param_offset;
-  int l = int(p[1]), r = int(p[2]);
-  if (l < 0 || l >= d->n_shapes || r < 0 || r >= d->n_shapes) return 99;
-  return 1 + std::max(csg_depth(d, l, guard + 1), csg_depth(d, r, guard + 1));
-}
-
-int validate(rb_ctx* ctx, const rb_scene_desc* d) {
-  if (!d || d->n_nodes < 1 || !d->nodes || !d->volumes || !d->shapes || !d->params)
-    return fail(ctx, -1, "scene: empty or null arrays");
-  for (int i = 0; i < d->n_shapes; i++) {
-    const rb_shape_desc& s = d->shapes[i];
-    if (s.param_offset < 0 || s.param_offset + s.n_params > d->n_params)
-      return fail(ctx, -1, "scene: shape parameters out of range");
-    switch (s.type) {
-      case RB_SHAPE_BOX:
-      case RB_SHAPE_TUBE:
-      case RB_SHAPE_SPHERE:
-      case RB_SHAPE_PARABOLOID:
-        break;
-      case RB_SHAPE_PGON: {
-        const double* p = d->params + s.param_offset;
-        int nz = int(p[3]);
-        if (nz < 2 || int(p[2]) < 3 || s.n_params != 4 + 3 * nz || p[1] != 360.0)
-          return fail(ctx, -3, "scene: unsupported TGeoPgon parameters");
-        for (int k = 0; k < nz; k++)
-          if (p[4 + 3 * k + 1] != 0.0) return fail(ctx, -3, "scene: TGeoPgon with rmin > 0 is unsupported");
-        break;
-      }
-      case RB_SHAPE_COMPOSITE:
-        if (s.n_params != 27) return fail(ctx, -1, "scene: composite needs 27 parameters");
-        if (csg_depth(d, i, 0) > kMaxCsgDepth)
-          return fail(ctx, -3, "scene: composite nesting deeper than 3 boolean levels");
-        break;
-      default:
-        return fail(ctx, -3, "scene: unsupported shape type " + std::to_string(s.type));
-    }
-  }
-  for (int i = 0; i < d->n_volumes; i++) {
-    const rb_volume_desc& v = d->volumes[i];
-    if (v.shape < 0 || v.shape >= d->n_shapes) return fail(ctx, -1, "scene: volume shape out of range");
-    if (v.kind < RB_KLENS || v.kind > RB_KOTHER) return fail(ctx, -1, "scene: bad volume kind");
-    int tabs[5] = {v.refl_table, v.k_table, v.qe_lambda_table, v.qe_angle_table,
-                   v.index_model == RB_INDEX_TABLE ? v.index_ref : -1};
-    for (int t : tabs)
-      if (t >= d->n_tables) return fail(ctx, -1, "scene: table index out of range");
-    if (v.index_model == RB_INDEX_TABLE && v.index_ref < 0) return fail(ctx, -1, "scene: missing index table");
-    if ((v.index_model == RB_INDEX_SELLMEIER || v.index_model == RB_INDEX_SCHOTT) &&
-        (v.index_ref < 0 || v.index_ref + 6 > d->n_params))
-      return fail(ctx, -1, "scene: index formula parameters out of range");
-  }
-  for (int i = 0; i < d->n_tables; i++) {
-    const rb_table_desc& t = d->tables[i];
-    if (t.offset < 0 || t.n < 0 || 2 * (t.offset + t.n) > d->n_table_xy)
-      return fail(ctx, -1, "scene: table out of range");
-    const double* xy = d->table_xy + 2 * t.offset;
-    for (int k = 1; k < t.n; k++)
-      if (xy[2 * k] < xy[2 * (k - 1)]) return fail(ctx, -1, "scene: table x must be ascending");
-  }
-  if (d->nodes[0].parent != -1) return fail(ctx, -1, "scene: node 0 must be the top node");
-  for (int i = 0; i < d->n_nodes; i++) {
-    const rb_node_desc& n = d->nodes[i];
-    if (n.volume < 0 || n.volume >= d->n_volumes) return fail(ctx, -1, "scene: node volume out of range");
-    if (n.first_child < 0 || n.n_children < 0 || n.first_child + n.n_children > std::max(d->n_children, 0))
-      return fail(ctx, -1, "scene: child range out of bounds");
-    for (int k = 0; k < n.n_children; k++) {
-      int c = d->children[n.first_child + k];
-      if (c <= 0 || c >= d->n_nodes || d->nodes[c].parent != i)
-        return fail(ctx, -1, "scene: inconsistent child list");
-      if (k > 0 && c <= d->children[n.first_child + k - 1])
-        return fail(ctx, -1, "scene: child ids must ascend");
-    }
-  }
-  return 0;
-}
-
-template <typename X>
-size_t put(std::vector<char>& blob, const X* src, size_t count) {
-  size_t off = (blob.size() + 255) & ~size_t(255);
-  blob.resize(off + sizeof(X) * std::max<size_t>(count, 1));
-  if (count) std::memcpy(blob.data() + off, src, sizeof(X) * count);
-  return off;
-}
-
 }  // namespace
 
 namespace {
@@ -257,6 +65,7 @@ struct DevRays {
 }  // namespace
 
 // ---------------------------------------------------------------- kernels
+template <bool MIXED>
 __global__ void __launch_bounds__(256) k_trace(DScene S, RayIO io, int64_t n, TraceParams P) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -273,7 +82,7 @@ __global__ void __launch_bounds__(256) k_trace(DScene S, RayIO io, int64_t n, Tr
   L.step = 0;
   L.hit_node = -1;
   L.hit_leaf = 0;
-  trace_lane(S, P, io, i, L);
+  trace_lane<MIXED>(S, P, io, i, L);
   io.x[i] = L.p.x;
   io.y[i] = L.p.y;
   io.z[i] = L.p.z;
@@ -375,48 +184,12 @@ int rbgpu_abi_sizes(int32_t* out) {
 int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
   if (!ctx || !out) return -1;
   *out = nullptr;
-  int rc = validate(ctx, d);
+  if (!d) return fail(ctx, -1, "scene: null descriptor");
+  std::vector<char> blob;
+  SceneLayout lay;
+  int rc = pack_scene(d, blob, lay, ctx->err);
   if (rc) return rc;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-
-  std::vector<DNode> nodes(d->n_nodes);
-  BvhBuilder bvh;
-  for (int i = 0; i < d->n_nodes; i++) {
-    const rb_node_desc& n = d->nodes[i];
-    DNode& o = nodes[i];
-    std::memcpy(o.rot, n.global.rot, sizeof(o.rot));
-    std::memcpy(o.tr, n.global.tr, sizeof(o.tr));
-    o.volume = n.volume;
-    o.shape = d->volumes[n.volume].shape;
-    o.kind = d->volumes[n.volume].kind;
-    o.parent = n.parent;
-    o.first_child = n.first_child;
-    o.n_children = n.n_children;
-    o.bvh_root = -1;
-    o.pad = 0;
-    if (n.n_children > 0) {
-      std::vector<std::pair<int, Box>> items;
-      for (int k = 0; k < n.n_children; k++) {
-        int c = d->children[n.first_child + k];
-        items.emplace_back(c, node_box(d, c));
-      }
-      o.bvh_root = bvh.build(items, 0, int(items.size()));
-    }
-  }
-  std::vector<DShape> shapes(d->n_shapes);
-  for (int i = 0; i < d->n_shapes; i++) shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset};
-
-  std::vector<char> blob;
-  size_t o_nodes = put(blob, nodes.data(), nodes.size());
-  size_t o_children = put(blob, d->children, size_t(std::max(d->n_children, 0)));
-  size_t o_bvh = put(blob, bvh.nodes.data(), bvh.nodes.size());
-  size_t o_prims = put(blob, bvh.prims.data(), bvh.prims.size());
-  size_t o_shapes = put(blob, shapes.data(), shapes.size());
-  size_t o_params = put(blob, d->params, size_t(d->n_params));
-  size_t o_vol = put(blob, d->volumes, size_t(d->n_volumes));
-  size_t o_tab = put(blob, d->tables, size_t(std::max(d->n_tables, 0)));
-  size_t o_txy = put(blob, d->table_xy, size_t(std::max(d->n_table_xy, 0)));
-
   rb_scene* sc = new rb_scene();
   sc->ctx = ctx;
   hipError_t e = hipMalloc(&sc->dmem, blob.size());
@@ -430,17 +203,7 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
     delete sc;
     return fail(ctx, -2, std::string("hipMemcpy scene: ") + hipGetErrorString(e));
   }
-  char* base = static_cast<char*>(sc->dmem);
-  sc->s.nodes = reinterpret_cast<const DNode*>(base + o_nodes);
-  sc->s.children = reinterpret_cast<const int32_t*>(base + o_children);
-  sc->s.bvh = reinterpret_cast<const DBvh*>(base + o_bvh);
-  sc->s.bvh_prims = reinterpret_cast<const int32_t*>(base + o_prims);
-  sc->s.shapes = reinterpret_cast<const DShape*>(base + o_shapes);
-  sc->s.params = reinterpret_cast<const double*>(base + o_params);
-  sc->s.volumes = reinterpret_cast<const rb_volume_desc*>(base + o_vol);
-  sc->s.tables = reinterpret_cast<const rb_table_desc*>(base + o_tab);
-  sc->s.table_xy = reinterpret_cast<const double*>(base + o_txy);
-  sc->s.n_nodes = d->n_nodes;
+  sc->s = scene_view(static_cast<const char*>(sc->dmem), lay);
   *out = sc;
   return 0;
 }
@@ -516,7 +279,10 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
   const int block = 256;
   const int64_t grid = (n + block - 1) / block;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  hipLaunchKernelGGL(k_trace, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P);
+  if (opts->precision == RB_PRECISION_MIXED)
+    hipLaunchKernelGGL(k_trace<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P);
+  else
+    hipLaunchKernelGGL(k_trace<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   ctx->launches = 1;

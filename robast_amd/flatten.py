@@ -76,6 +76,8 @@ def flatten(top_volume):
         tables.append((len(table_xy) // 2, len(xs)))
         for x, y in zip(xs, ys):
             table_xy.extend((x, y))
+        for x, y in g.extrapolation_points():
+            table_xy.extend((x, y))
         table_ids[id(g)] = idx
         return idx
 

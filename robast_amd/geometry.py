@@ -230,7 +230,9 @@ class TGeoSphere(TGeoShape):
             raise NotImplementedError("TGeoSphere with a phi range is not supported")
 
     def params(self):
-        return [self.rmin, self.rmax, self.theta1, self.theta2]
+        t1, t2 = math.radians(self.theta1), math.radians(self.theta2)
+        return [self.rmin, self.rmax, self.theta1, self.theta2,
+                math.cos(t1), math.cos(t2), math.sin(t1), math.sin(t2)]
 
     def local_bbox(self):
         r = self.rmax
@@ -322,6 +324,11 @@ class TGeoPgon(TGeoPcon):
         p = [self.phi1, self.dphi, float(self.nedges), float(self.nz)]
         for i in range(self.nz):
             p += [self.z[i], self.rmin[i], self.rmax[i]]
+        # outward normal (cos, sin) of each side: the middle of its phi sector
+        div = self.dphi / self.nedges
+        for k in range(self.nedges):
+            ph = math.radians(self.phi1 + div * (k + 0.5))
+            p += [math.cos(ph), math.sin(ph)]
         return p
 
     def local_bbox(self):

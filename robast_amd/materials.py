@@ -59,8 +59,37 @@ class TGraph:
         return Y[up] + (x - X[up]) * (Y[low] - Y[up]) / (X[low] - X[up])
 
     def sorted_points(self):
+        """Points sorted by x (stable, so equal abscissae keep insertion order)."""
         pts = sorted(zip(self.x, self.y), key=lambda p: p[0])
         return [p[0] for p in pts], [p[1] for p in pts]
+
+    def _pick(self, above):
+        # the (up, low) pair Eval's scan settles on when x lies above (below)
+        # every abscissa; it depends on insertion order, not only on x
+        X = self.x
+        a = b = -1
+        for i in range(len(X)):
+            if above:
+                if a == -1 or X[i] > X[a]:
+                    b, a = a, i
+                elif b == -1:
+                    b = i
+            else:
+                if a == -1 or X[i] < X[a]:
+                    b, a = a, i
+                elif b == -1:
+                    b = i
+        return (a, b) if above else (b, a)
+
+    def extrapolation_points(self):
+        """[(x, y) of up, low] used above the range, then [low, up] below it."""
+        if len(self.x) < 2:
+            p = (self.x[0], self.y[0]) if self.x else (0.0, 0.0)
+            return [p, p, p, p]
+        up, low = self._pick(True)
+        low2, up2 = self._pick(False)
+        return [(self.x[up], self.y[up]), (self.x[low], self.y[low]),
+                (self.x[low2], self.y[low2]), (self.x[up2], self.y[up2])]
 
 
 class ARefractiveIndex:

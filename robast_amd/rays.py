@@ -243,3 +243,15 @@ class ARayShooter:
     @staticmethod
     def Square(lam, d, n, rot=None, tr=None, v=None):
         return ARayShooter.Rectangle(lam, d, d, n, n, rot, tr, v)
+
+    @staticmethod
+    def RandomSphere(lam, n, tr=None, seed=0):
+        """n rays from the origin (or tr) with isotropic directions
+        (src/ARayShooter.cxx RandomSphere; numpy's seeded generator replaces gRandom)."""
+        rng = np.random.default_rng(seed)
+        cos_t = rng.uniform(-1, 1, n)
+        phi = rng.uniform(0, 2 * math.pi, n)
+        sin_t = np.sqrt(1 - cos_t * cos_t)
+        o = np.zeros((n, 3)) if tr is None else np.tile(tr.tr, (n, 1))
+        return ARayArray.from_arrays(o[:, 0], o[:, 1], o[:, 2], sin_t * np.cos(phi), sin_t * np.sin(phi), cos_t,
+                                     np.full(n, float(lam)))

@@ -1,0 +1,148 @@
+"""Per-ray parity of the HIP tracer (through librbgpu's C ABI) with the CPU
+FP64 oracle on identical inputs: status, point count and last node
+identical; terminal point within 1e-7 cm (1e-9 m); direction within 1e-9."""
+import numpy as np
+import pytest
+
+from helpers import compare, copy_soa, counts
+from oracle import oracle as O
+from robast_amd import _abi as A
+from robast_amd import configs
+from robast_amd._lib import RbError, make_rays, make_rays_device
+
+pytestmark = pytest.mark.gpu
+
+
+def _trace_both(ctx, mgr, soa, nthreads=8, path_cap=0):
+    g = copy_soa(soa)
+    o = copy_soa(soa)
+    if path_cap:
+        for d in (g, o):
+            d["path_xyzt"] = np.zeros((len(d["x"]), path_cap, 4))
+            d["path_node"] = np.full((len(d["x"]), path_cap), -1, dtype=np.int32)
+    scene = ctx.upload(mgr.flat())
+    rays = make_rays(g, g.get("path_xyzt"), g.get("path_node"))
+    ctx.trace(scene, rays, len(g["x"]), mgr.trace_opts())
+    O.trace(mgr.flat(), o, mgr.trace_opts(), nthreads=nthreads)
+    return g, o
+
+
+@pytest.mark.parametrize("deg", configs.C1_ANGLES_DEG)
+def test_c1_parabolic_per_ray(ctx, deg):
+    print(f"C1 {deg} deg, 316x316 grid", flush=True)
+    mgr = configs.c1_scene()
+    g, o = _trace_both(ctx, mgr, configs.c1_rays(deg).soa())
+    compare(g, o)
+    c = counts(g["status"])
+    assert c[A.RB_FOCUS] > 60000 and c[A.RB_RUN] == 0
+
+
+def test_c1_path_record_matches(ctx):
+    mgr = configs.c1_scene()
+    soa = configs.c1_rays(1.0, nside=64).soa()
+    g, o = _trace_both(ctx, mgr, soa, path_cap=4)
+    compare(g, o)
+    assert np.array_equal(g["path_node"], o["path_node"])
+    assert np.abs(g["path_xyzt"][..., :3] - o["path_xyzt"][..., :3]).max() < 1e-7
+
+
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_c2_davies_cotton_sample_per_ray(ctx, deterministic):
+    print(f"C2 sample deterministic={deterministic}", flush=True)
+    mgr = configs.c2_scene(deterministic=deterministic)
+    soa = O.generate(configs.c2_raygen(), 200000)
+    g, o = _trace_both(ctx, mgr, soa)
+    compare(g, o)
+    c = counts(g["status"])
+    if deterministic:
+        assert c[A.RB_ABSORB] == 0
+    else:
+        assert c[A.RB_ABSORB] > 0
+
+
+def test_generator_matches_oracle(ctx):
+    import torch
+    n = 4096
+    dev = torch.device("cuda")
+    t = {f: torch.zeros(n, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_")}
+    for f in ("status", "npoints", "last_node"):
+        t[f] = torch.zeros(n, dtype=torch.int32, device=dev)
+    ctx.generate(configs.c2_raygen(), make_rays_device(t), n)
+    got = {k: v.cpu().numpy() for k, v in t.items()}
+    want = O.generate(configs.c2_raygen(), n)
+    for f in ("x", "y", "z", "dx", "dy", "dz", "lambda_"):
+        assert np.abs(got[f] - want[f]).max() < 1e-9, f
+    assert np.all(got["status"] == A.RB_RUN) and np.all(got["npoints"] == 1)
+
+
+def test_c2_full_size_properties_and_prefix_parity(ctx):
+    """1e7 rays resident in HBM: terminal-state invariants at full size, and
+    the first 2e5 rays identical to the oracle (Philox streams are keyed by
+    ray id, so a prefix of the big batch is a self-contained parity case)."""
+    import torch
+    print("C2 full size 1e7", flush=True)
+    N, P = 10_000_000, 200_000
+    mgr = configs.c2_scene()
+    scene = ctx.upload(mgr.flat())
+    dev = torch.device("cuda")
+    t = {f: torch.empty(N, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_")}
+    for f in ("status", "npoints", "last_node"):
+        t[f] = torch.empty(N, dtype=torch.int32, device=dev)
+    rays = make_rays_device(t)
+    ctx.generate(configs.c2_raygen(), rays, N)
+    src = {k: v[:P].cpu().numpy().copy() for k, v in t.items()}
+    ctx.trace(scene, rays, N, mgr.trace_opts(device_ptrs=1))
+    st = t["status"]
+    c = torch.bincount(st, minlength=6).cpu().numpy()
+    assert c.sum() == N and c[A.RB_RUN] == 0 and c[A.RB_SUSPEND] == 0
+    norm = torch.sqrt(t["dx"] ** 2 + t["dy"] ** 2 + t["dz"] ** 2)
+    assert float((norm - 1).abs().max()) < 1e-12
+    foc = st == A.RB_FOCUS
+    # focused rays end on the camera's lower face z = f (the tube spans f .. f + 2 mm)
+    assert float((t["z"][foc] - configs.C2_F).abs().max()) < 1e-9
+    assert float(torch.sqrt(t["x"][foc] ** 2 + t["y"][foc] ** 2).max()) <= configs.C2_CAMERA_R + 1e-9
+    o = copy_soa(src)
+    O.trace(mgr.flat(), o, mgr.trace_opts(), nthreads=8)
+    g = {k: v[:P].cpu().numpy() for k, v in t.items()}
+    compare(g, o)
+    # the sample's focused fraction is the full run's within binomial error
+    p_full = c[A.RB_FOCUS] / N
+    p_samp = (o["status"] == A.RB_FOCUS).mean()
+    assert abs(p_samp - p_full) < 4 * np.sqrt(p_full * (1 - p_full) / P)
+
+
+def test_device_pointer_path_equals_host_path(ctx):
+    import torch
+    mgr = configs.c1_scene()
+    soa = configs.c1_rays(0.5, nside=128).soa()
+    h = copy_soa(soa)
+    scene = ctx.upload(mgr.flat())
+    ctx.trace(scene, make_rays(h), len(h["x"]), mgr.trace_opts())
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in soa.items()}
+    ctx.trace(scene, make_rays_device(t), len(h["x"]), mgr.trace_opts(device_ptrs=1))
+    for k in h:
+        assert np.array_equal(t[k].cpu().numpy(), h[k]), k
+    assert ctx.last_trace_ms() > 0 and ctx.last_trace_launches() >= 1
+
+
+def test_non_running_rays_untouched(ctx):
+    mgr = configs.c1_scene()
+    soa = configs.c1_rays(0.0, nside=32).soa()
+    soa["status"][::3] = A.RB_STOP
+    before = copy_soa(soa)
+    scene = ctx.upload(mgr.flat())
+    ctx.trace(scene, make_rays(soa), len(soa["x"]), mgr.trace_opts())
+    for k in soa:
+        assert np.array_equal(soa[k][::3], before[k][::3]), k
+
+
+def test_unsupported_scene_is_reported(ctx):
+    mgr = configs.c1_scene()
+    flat = mgr.flat()
+    flat.shapes[0]["type"] = 42
+    with pytest.raises(RbError, match="unsupported shape type"):
+        ctx.upload(flat)
+    flat.shapes[0]["type"] = A.RB_SHAPE_BOX
+    flat.nodes[1]["volume"] = 99
+    with pytest.raises(RbError, match="node volume out of range"):
+        ctx.upload(flat)

@@ -1,0 +1,239 @@
+"""Pins the oracle's geometry: shape distances against brute-force marching
+of the containment test, containment against independent closed forms, and
+the whole C1 reflection chain against an exact mpmath construction."""
+import math
+import zlib
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from robast_amd import _abi as A
+from robast_amd import configs
+from robast_amd import geometry as G
+from robast_amd.flatten import flatten
+
+
+def _shape_scene(shape):
+    top = G.AOpticalComponent("world", G.TGeoBBox("w", 100, 100, 100))
+    top.AddNode(G.AOpticalComponent("obj", shape), 1)
+    flat = flatten(top)
+    return flat, int(flat.volumes[flat.nodes[1]["volume"]]["shape"])
+
+
+def _shapes():
+    G._reset_registry()
+    out = {}
+    out["box"] = G.TGeoBBox("b", 10, 8, 6, (1.0, -2.0, 0.5))
+    out["tube"] = G.TGeoTube("t", 3, 7, 5)
+    out["sphere"] = G.TGeoSphere("s", 4, 9, 30, 120)
+    out["sphere_cap"] = G.TGeoSphere("s2", 6, 9, 140, 180)
+    out["paraboloid"] = G.TGeoParaboloid("p", 1, 6, 4)
+    pg = G.TGeoPgon("pg", 0, 360, 6, 3)
+    pg.DefineSection(0, -5, 0, 5)
+    pg.DefineSection(1, 0, 0, 7)
+    pg.DefineSection(2, 4, 0, 4)
+    out["pgon"] = pg
+    G.TGeoBBox("cb", 6, 6, 6)
+    G.TGeoTube("ct", 0, 3, 9)
+    G.TGeoTranslation("ctr", 1.5, 0, 0).RegisterYourself()
+    out["box_minus_tube"] = G.TGeoCompositeShape("c1", "cb - ct:ctr")
+    G.TGeoSphere("cs", 0, 8)
+    out["sphere_and_box"] = G.TGeoCompositeShape("c2", "cs * cb")
+    G.TGeoBBox("cb2", 3, 3, 9)
+    G.TGeoRotation("crot", 30, 20, 0).RegisterYourself()
+    out["union_rot"] = G.TGeoCompositeShape("c3", "cb + cb2:crot")
+    return out
+
+
+SHAPES = _shapes()
+
+
+def _march(flat, shape, p, d, inside_start, tmax=60.0, n=6000):
+    """First t where containment flips, refined by bisection."""
+    ts = np.linspace(0, tmax, n + 1)[1:]
+    pts = p[None, :] + ts[:, None] * d[None, :]
+    c = O.contains_batch(flat, shape, pts)
+    flip = np.nonzero(c != inside_start)[0]
+    if len(flip) == 0:
+        return None
+    k = flip[0]
+    lo = ts[k - 1] if k > 0 else 0.0
+    hi = ts[k]
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if O.contains_batch(flat, shape, p + mid * d)[0] == inside_start:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+@pytest.mark.parametrize("name", sorted(SHAPES))
+def test_distances_match_marching(name):
+    flat, sid = _shape_scene(SHAPES[name])
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    checked = 0
+    for _ in range(400):
+        p = rng.uniform(-14, 14, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        inside, dout, din, _n = O.shape_query(flat, sid, p, d)
+        t = _march(flat, sid, p, d, inside)
+        if inside:
+            assert t is not None, (name, p, d)
+            assert din == pytest.approx(t, abs=2e-7), (name, p, d, din, t)
+        elif t is not None and abs(dout - t) <= 2e-7:
+            pass
+        elif dout < 1e29 and (t is None or dout < t):
+            # an entry the 0.01 cm marching grid stepped over (a sliver):
+            # the point just past it must be inside
+            assert O.contains_batch(flat, sid, p + (dout + 1e-7) * d)[0], (name, p, d, dout, t)
+        else:
+            assert t is None and dout >= 1e29, (name, p, d, dout, t)
+        checked += 1
+    assert checked == 400
+
+
+def _contains_closed_form(name, q):
+    x, y, z = q[:, 0], q[:, 1], q[:, 2]
+    r2 = x * x + y * y
+    R = np.sqrt(r2 + z * z)
+    if name == "box":
+        return (np.abs(x - 1) <= 10) & (np.abs(y + 2) <= 8) & (np.abs(z - 0.5) <= 6)
+    if name == "tube":
+        return (np.abs(z) <= 5) & (r2 >= 9) & (r2 <= 49)
+    if name == "sphere":
+        th = np.degrees(np.arccos(np.clip(z / np.maximum(R, 1e-300), -1, 1)))
+        return (R >= 4) & (R <= 9) & (th >= 30) & (th <= 120)
+    if name == "paraboloid":
+        a = 2 * 4 / (36 - 1)
+        b = -4 * (1 + 36) / (36 - 1)
+        return (np.abs(z) <= 4) & (z >= a * r2 + b)
+    if name == "pgon":
+        phi = np.degrees(np.arctan2(y, x)) % 360
+        ph0 = np.radians((np.floor(phi / 60) + 0.5) * 60)
+        r = x * np.cos(ph0) + y * np.sin(ph0)
+        rmax = np.where(z <= 0, 5 + (z + 5) / 5 * 2, 7 + z / 4 * (-3))
+        return (z >= -5) & (z <= 4) & (r <= rmax)
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["box", "tube", "sphere", "paraboloid", "pgon"])
+def test_contains_matches_closed_form(name):
+    flat, sid = _shape_scene(SHAPES[name])
+    q = np.random.default_rng(5).uniform(-12, 12, (20000, 3))
+    got = O.contains_batch(flat, sid, q)
+    want = _contains_closed_form(name, q)
+    assert np.mean(got == want) > 0.9999  # boundary rounding only
+
+
+def _c1_soa(deg):
+    return configs.c1_rays(deg).soa()
+
+
+def test_c1_on_axis_status_counts_closed_form():
+    mgr = configs.c1_scene()
+    soa = _c1_soa(0.0)
+    r = np.hypot(soa["x"], soa["y"])
+    O.trace(mgr.flat(), soa, mgr.trace_opts(), nthreads=8)
+    sag = configs.C1_R ** 2 / 4 / configs.C1_F
+    rb_top = math.sqrt(4 * configs.C1_F * (sag - 1e-4))  # rim of the inner surface at the shell top
+    stopped = r <= 60.0                                  # obscuration shadow
+    focused = (r > 60.0) & (r < rb_top)                  # paraboloid -> (0, 0, f)
+    np.testing.assert_array_equal(soa["status"] == A.RB_STOP, stopped)
+    np.testing.assert_array_equal(soa["status"] == A.RB_FOCUS, focused)
+    np.testing.assert_array_equal(soa["status"] == A.RB_EXIT, ~(stopped | focused))
+    f = soa["status"] == A.RB_FOCUS
+    # perfect paraboloid: every focused ray within the step-back offset of the focus
+    assert np.all(np.abs(soa["z"][f] - configs.C1_F) == 0)
+    assert np.hypot(soa["x"][f], soa["y"][f]).max() < 1e-6
+
+
+@pytest.mark.parametrize("deg", [0.0, 1.5])
+def test_c1_reflection_chain_matches_exact_geometry(deg):
+    """Exact (50-digit) paraboloid hit, normal, specular reflection, the
+    2e-6 cm step back along -d1 and the focal-plane crossing, for a sample."""
+    mp.mp.dps = 50
+    mgr = configs.c1_scene()
+    flat = mgr.flat()
+    soa = _c1_soa(deg)
+    src = {k: v.copy() for k, v in soa.items()}
+    O.trace(flat, soa, mgr.trace_opts(), nthreads=8)
+    # paraboloid coefficients as flattened
+    para = [i for i in range(len(flat.shapes)) if flat.shapes[i]["type"] == A.RB_SHAPE_PARABOLOID][0]
+    off = flat.shapes[para]["param_offset"]
+    a = mp.mpf(float(flat.params[off + 3]))
+    b = mp.mpf(float(flat.params[off + 4]))
+    sag = configs.C1_R ** 2 / 4.0 / configs.C1_F
+    shift = mp.mpf(sag / 2.0)  # mirror_tr1: inner (reflecting) surface z = a r^2 + b + shift
+    f = mp.mpf(configs.C1_F)
+    idx = np.nonzero(soa["status"] == A.RB_FOCUS)[0][::997]
+    assert len(idx) > 50
+    c = mp.mpf(2.99792458e10)
+    for i in idx:
+        o = [mp.mpf(float(src[k][i])) for k in ("x", "y", "z")]
+        d = [mp.mpf(float(src[k][i])) for k in ("dx", "dy", "dz")]
+        A2 = a * (d[0] ** 2 + d[1] ** 2)
+        B2 = 2 * a * (o[0] * d[0] + o[1] * d[1]) - d[2]
+        C2 = a * (o[0] ** 2 + o[1] ** 2) + b + shift - o[2]
+        sq = mp.sqrt(B2 ** 2 - 4 * A2 * C2)
+        qq = -(B2 + (sq if B2 >= 0 else -sq)) / 2
+        roots = [C2 / qq] + ([qq / A2] if A2 != 0 else [])
+        t = min(r for r in roots if r > 0)
+        h = [o[k] + t * d[k] for k in range(3)]
+        n = [2 * a * h[0], 2 * a * h[1], mp.mpf(-1)]
+        nn = mp.sqrt(sum(v * v for v in n))
+        n = [v / nn for v in n]
+        cs = sum(d[k] * n[k] for k in range(3))
+        d2 = [d[k] - 2 * cs * n[k] for k in range(3)]
+        m = mp.sqrt(sum(v * v for v in d2))
+        d2 = [v / m for v in d2]
+        hb = [h[k] - mp.mpf("2e-6") * d[k] for k in range(3)]
+        s2 = (f - hb[2]) / d2[2]
+        end = [hb[k] + s2 * d2[k] for k in range(3)]
+        for k, key in enumerate(("x", "y", "z")):
+            assert abs(float(end[k]) - soa[key][i]) < 1e-9, (i, key, float(end[k]), soa[key][i])
+        tt = t / c + (s2 + mp.mpf("1e-10")) / c
+        assert abs(float(tt) - soa["t"][i]) < 1e-18
+        assert soa["npoints"][i] == 3
+
+
+def test_c1_ray_grid_matches_arayshooter_formula():
+    # src/ARayShooter.cxx:425-431: x_i = i d/(n-1) - d/2, then translated
+    arr = configs.c1_rays(1.0, nside=5)
+    th = math.radians(1.0)
+    xs = [i * 300.0 / 4 - 150.0 for i in range(5)]
+    for i in range(5):
+        for j in range(5):
+            k = i * 5 + j
+            assert arr.x[k] == xs[i] + (-800.0 * 2 * math.sin(th))
+            assert arr.y[k] == xs[j]
+            assert arr.z[k] == 800.0 * 2 * math.cos(th)
+    assert arr.dx[0] == pytest.approx(math.sin(th), abs=1e-15)
+    assert arr.dz[0] == pytest.approx(-math.cos(th), abs=1e-15)
+
+
+def test_dc_facets_closed_shells():
+    pos = configs.dc_facet_positions()
+    assert len(pos) == 85
+    r2 = np.array([x * x + y * y for x, y in pos]) / configs.C2_MIRROR_D ** 2
+    assert r2.max() == pytest.approx(21.0)
+    # lattice nearest-neighbour distance equals the flat-to-flat width
+    p = np.array(pos)
+    dmin = min(np.linalg.norm(p[i] - p[j]) for i in range(20) for j in range(len(p)) if i != j)
+    assert dmin == pytest.approx(configs.C2_MIRROR_D)
+
+
+def test_dc_facet_normals_aim_at_2f():
+    mgr = configs.c2_scene(deterministic=True)
+    flat = mgr.flat()
+    for i in range(1, 86):
+        rot = flat.nodes[i]["rot"].reshape(3, 3)
+        tr = flat.nodes[i]["tr"]
+        axis = rot @ np.array([0, 0, 1.0])
+        want = np.array([0, 0, 2 * configs.C2_F]) - tr
+        want /= np.linalg.norm(want)
+        assert np.allclose(axis, want, atol=1e-12)
+        assert np.hypot(tr[0], tr[1]) ** 2 + (tr[2] - configs.C2_F) ** 2 == pytest.approx(configs.C2_F ** 2)
