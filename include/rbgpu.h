@@ -54,7 +54,7 @@ enum {
   RB_SHAPE_PCON = 6,       /* phi1 dphi nz {z rmin rmax}*nz         (TGeoPcon)        */
   RB_SHAPE_COMPOSITE = 7,  /* op left right lmat[12] rmat[12]       (TGeoBoolNode)    */
   RB_SHAPE_ASPHERIC = 8,   /* z1 c1 z2 c2 rmax rmin kappa1 kappa2 n1 n2 K1[n1] K2[n2]
-                              (AGeoAsphericDisk)                                       */
+                              Kn1[n1] Kn2[n2] with Kn_k = K_k rmax^2k (AGeoAsphericDisk) */
   RB_SHAPE_WINSTON = 9     /* r1 r2 polyN f theta dz                (AGeoWinstonConePoly) */
 };
 

@@ -616,6 +616,167 @@ static V3 pgon_normal(const double* p, V3 q, V3 d) {
   return n;
 }
 
+
+// ---------------------------------------------------------------- aspheric disk
+// AGeoAsphericDisk (src/AGeoAsphericDisk.cxx). params: z1 c1 z2 c2 rmax rmin
+// kappa1 kappa2 n1 n2 K1[n1] K2[n2]; surface s in {0, 1} = ROOT's 1, 2.
+struct Asph {
+  double z[2], c[2], kap[2], rmax, rmin;
+  int n[2];
+  const double* K[2];
+};
+static Asph asph_p(const double* p) {
+  Asph a;
+  a.z[0] = p[0]; a.c[0] = p[1]; a.z[1] = p[2]; a.c[1] = p[3];
+  a.rmax = p[4]; a.rmin = p[5];
+  a.kap[0] = p[6]; a.kap[1] = p[7];
+  a.n[0] = int(p[8]); a.n[1] = int(p[9]);
+  a.K[0] = p + 10; a.K[1] = p + 10 + a.n[0];
+  return a;
+}
+// CalcF1/CalcF2 (:126-153); false where ROOT throws (1 - p < 0)
+static bool asph_sag(const Asph& a, int s, double r, double& f) {
+  double p = r * r * a.c[s] * a.c[s] * a.kap[s];
+  if (1 - p < 0) return false;
+  f = a.z[s] + r * r * a.c[s] / (1 + std::sqrt(1 - p));
+  for (int i = 0; i < a.n[s]; i++) f += a.K[s][i] * std::pow(r, 2 * (i + 1));
+  return true;
+}
+// CalcdF1dr/CalcdF2dr (:96-123); false where ROOT throws (1 - p <= 0)
+static bool asph_dsag(const Asph& a, int s, double r, double& df) {
+  double p = r * r * a.c[s] * a.c[s] * a.kap[s];
+  if (1 - p <= 0) return false;
+  df = r * a.c[s] / std::sqrt(1 - p);
+  for (int i = 0; i < a.n[s]; i++) df += 2 * (i + 1) * a.K[s][i] * std::pow(r, 2 * (i + 1) - 1);
+  return true;
+}
+// DistToAsphere (:415-526): Matsui's spherical seed, then Newton steps
+// along the ray until |e| < 1e-10 (at most 100)
+static double asph_dist_surface(const Asph& a, int s, V3 P, V3 D) {
+  const double c = a.c[s], kap = a.kap[s], d = a.z[s];
+  double H2 = P.x * P.x + P.y * P.y;
+  double p = -((P.z - d) * D.z + P.x * D.x + P.y * D.y);
+  double M = p * D.z + P.z - d;
+  double M2 = (P.z - d) * (P.z - d) + H2 - p * p;
+  double check = 1 - (M2 * c - 2 * M) * c / D.z / D.z;
+  if (check < 0) return kBig;
+  double q = p + (M2 * c - 2 * M) / (D.z * (1 + std::sqrt(check)));
+  double np[3] = {P.x + q * D.x, P.y + q * D.y, P.z + q * D.z - d};
+  for (int i = 0;; i++) {
+    if (i > 100) return kBig;
+    H2 = np[0] * np[0] + np[1] * np[1];
+    check = 1 - kap * H2 * c * c;
+    if (check < 0) return kBig;
+    double l = std::sqrt(check);
+    double x = 0;
+    if (c != 0) x += (1 - l) / c / kap;
+    for (int j = 0; j < a.n[s]; j++) x += a.K[s][j] * std::pow(H2, j + 1);
+    double v = 0;
+    for (int j = 0; j < a.n[s]; j++) v += 2 * (j + 1) * a.K[s][j] * std::pow(H2, j);
+    v = c * kap + l * v;
+    double m = -np[0] * v, nn = -np[1] * v;
+    double norm = std::sqrt(l * l + m * m + nn * nn);
+    l /= norm; m /= norm; nn /= norm;
+    check = D.z * l + D.x * m + D.y * nn;
+    if (check == 0) return kBig;
+    double e = l * (x - np[2]) / check;
+    np[0] += e * D.x; np[1] += e * D.y; np[2] += e * D.z;
+    if (std::fabs(e) < 1e-10) break;
+  }
+  np[2] += d;
+  check = D.x * (np[0] - P.x) + D.y * (np[1] - P.y) + D.z * (np[2] - P.z);
+  if (check < 0) return kBig;
+  double rz = std::pow(np[0] * np[0] + np[1] * np[1], 0.5);
+  if (rz < a.rmin || rz > a.rmax) return kBig;
+  return std::sqrt(std::pow(np[0] - P.x, 2) + std::pow(np[1] - P.y, 2) + std::pow(np[2] - P.z, 2));
+}
+// DistToInner / DistToOuter (:529-684): the cylinder r = R between the two sags
+static double asph_dist_wall(const Asph& a, bool outer, V3 P, V3 D) {
+  if (!outer && !(a.rmin > 0)) return kBig;
+  const double R = outer ? a.rmax : a.rmin;
+  double rsq = P.x * P.x + P.y * P.y, nsq = D.x * D.x + D.y * D.y;
+  if (std::sqrt(nsq) < kTol) return kBig;
+  double rdotn = P.x * D.x + P.y * D.y;
+  double t1i = 1. / nsq, b = t1i * rdotn, cc = t1i * (rsq - R * R);
+  double delta = b * b - cc;
+  if (delta > 0) delta = std::sqrt(delta);
+  else return kBig;
+  double t1 = -b + delta, t2 = -b - delta;
+  if (t1 < 0 && t2 < 0) return kBig;
+  double zmin, zmax;
+  if (!asph_sag(a, 0, R, zmin) || !asph_sag(a, 1, R, zmax)) return kBig;
+  if (t2 > 0) {
+    double z1 = t1 * D.z + P.z, z2 = t2 * D.z + P.z;
+    if (z1 < zmin || zmax < z1) t1 = kBig;
+    if (z2 < zmin || zmax < z2) t2 = kBig;
+    return t1 < t2 ? t1 : t2;
+  } else if (t2 == 0) {
+    if (t1 > 0) {
+      if (zmin <= P.z && P.z <= zmax) return 0;
+      double z1 = t1 * D.z + P.z;
+      if (zmin <= z1 && z1 <= zmax) return t1;
+    } else if (t1 == 0) {
+      if (zmin <= P.z && P.z <= zmax) return 0;
+    }
+  } else {
+    if (t1 > 0) {
+      double z1 = t1 * D.z + P.z;
+      if (zmin <= z1 && z1 <= zmax) return t1;
+    } else if (t1 == 0) {
+      if (zmin <= P.z && P.z <= zmax) return 0;
+    }
+  }
+  return kBig;
+}
+// DistFromInside and DistFromOutside are the same minimum (:363-412)
+static double asph_dist(const double* p, V3 q, V3 d) {
+  Asph a = asph_p(p);
+  double v[4] = {asph_dist_surface(a, 0, q, d), asph_dist_surface(a, 1, q, d), asph_dist_wall(a, false, q, d),
+                 asph_dist_wall(a, true, q, d)};
+  int k = 0;
+  for (int i = 1; i < 4; i++)
+    if (v[i] < v[k]) k = i;
+  return v[k];
+}
+static bool asph_contains(const double* p, V3 q) {
+  Asph a = asph_p(p);
+  double r = std::sqrt(q.x * q.x + q.y * q.y);
+  if (r > a.rmax || r < a.rmin) return false;
+  double f1, f2;
+  if (!asph_sag(a, 0, r, f1) || !asph_sag(a, 1, r, f2)) return false;
+  return !(q.z < f1 || f2 < q.z);
+}
+// ComputeNormal (:246-327): closest of the four surfaces, rotated by phi
+static V3 asph_normal(const double* p, V3 q, V3 d) {
+  Asph a = asph_p(p);
+  double r = std::sqrt(q.x * q.x + q.y * q.y);
+  double phi = std::atan2(q.y, q.x);
+  double saf[4];
+  saf[0] = a.rmin > 0 ? std::fabs(r - a.rmin) : kBig;
+  saf[1] = std::fabs(r - a.rmax);
+  double df[2] = {kBig, kBig};
+  for (int s = 0; s < 2; s++) {
+    double f;
+    if (!asph_sag(a, s, r, f)) { saf[2 + s] = kBig; continue; }
+    if (!asph_dsag(a, s, r, df[s])) { saf[2 + s] = kBig; continue; }
+    saf[2 + s] = std::fabs(f - q.z) / std::sqrt(1 + df[s] * df[s]);
+  }
+  int k = 0;
+  for (int i = 1; i < 4; i++)
+    if (saf[i] < saf[k]) k = i;
+  double n0 = 0, n2 = 0;
+  if (k <= 1) {
+    n0 = 1;
+  } else {
+    double g = df[k - 2];
+    if (g == 0) n2 = 1;
+    else { n0 = g / std::sqrt(1 + g * g); n2 = -1 / std::sqrt(1 + g * g); }
+  }
+  V3 n = v3(n0 * std::cos(phi), n0 * std::sin(phi), n2);
+  if (dot(n, d) < 0) n = mul(n, -1);
+  return n;
+}
+
 // ---------------------------------------------------------------- dispatch
 static bool contains(const Scene& sc, int shape, V3 q);
 static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d);
@@ -645,6 +806,7 @@ static bool contains(const Scene& sc, int shape, V3 q) {
     case RB_SHAPE_SPHERE: return sphere_contains(p, q);
     case RB_SHAPE_PARABOLOID: return para_contains(p, q);
     case RB_SHAPE_PGON: return pgon_contains(p, q);
+    case RB_SHAPE_ASPHERIC: return asph_contains(p, q);
     case RB_SHAPE_COMPOSITE: {
       Comp c = comp_of(sc, shape);
       bool a = contains(sc, c.left, m2l(*c.lm, q));
@@ -760,6 +922,7 @@ static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
     case RB_SHAPE_SPHERE: return {sphere_dist_out(p, q, d), 0};
     case RB_SHAPE_PARABOLOID: return {para_dist_out(p, q, d), 0};
     case RB_SHAPE_PGON: return {pgon_dist_out(p, q, d), 0};
+    case RB_SHAPE_ASPHERIC: return {asph_dist(p, q, d), 0};
     case RB_SHAPE_COMPOSITE: return comp_dist_out(sc, shape, q, d);
   }
   return {kBig, 0};
@@ -772,6 +935,7 @@ static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
     case RB_SHAPE_SPHERE: return {sphere_dist_in(p, q, d), 0};
     case RB_SHAPE_PARABOLOID: return {para_dist_in(p, q, d), 0};
     case RB_SHAPE_PGON: return {pgon_dist_in(p, q, d), 0};
+    case RB_SHAPE_ASPHERIC: return {asph_dist(p, q, d), 0};
     case RB_SHAPE_COMPOSITE: return comp_dist_in(sc, shape, q, d);
   }
   return {0, 0};
@@ -801,6 +965,7 @@ static V3 shape_normal(const Scene& sc, int shape, uint32_t leaf, V3 q, V3 d) {
     case RB_SHAPE_SPHERE: n = sphere_normal(p, q, d); break;
     case RB_SHAPE_PARABOLOID: n = para_normal(p, q, d); break;
     case RB_SHAPE_PGON: n = pgon_normal(p, q, d); break;
+    case RB_SHAPE_ASPHERIC: n = asph_normal(p, q, d); break;
     default: n = v3(0, 0, 1);
   }
   while (ns > 0) n = l2mv(*stack[--ns], n);

@@ -2,13 +2,14 @@
 
 C1 parabolic telescope (after tutorials/SimpleParabolicTelescope.C:20-88)
 C2 Davies-Cotton 85-facet reflector (after tutorials/DaviesCotton.C:58-157)
+C3 Schwarzschild-Couder aspheric two-mirror system (after tutorials/SchwarzschildCouder.C:33-90)
 """
 import math
 
 import numpy as np
 
 from . import _abi as A
-from .geometry import (AFocalSurface, AMirror, AObscuration, AOpticalComponent, TGeoBBox, TGeoCombiTrans,
+from .geometry import (AFocalSurface, AGeoAsphericDisk, AMirror, AObscuration, AOpticalComponent, TGeoBBox, TGeoCombiTrans,
                        TGeoCompositeShape, TGeoParaboloid, TGeoPgon, TGeoRotation, TGeoSphere, TGeoTranslation,
                        TGeoTube, cm, m, mm, nm, um)
 from .manager import AOpticsManager
@@ -173,5 +174,75 @@ def c2_raygen(seed=0xC2, theta_max_deg=4.0):
     g.theta_max = theta_max_deg * math.pi / 180.0
     g.lambda_min = 300 * nm
     g.lambda_max = 600 * nm
+    g.seed = seed
+    return g
+
+
+# ------------------------------------------------------------------ C3
+C3_FP = 16.915 * m      # primary focal length (SchwarzschildCouder.C:33-45)
+C3_FS = -3.553 * m      # secondary focal length
+C3_ZS = 9.980 * m       # secondary position
+C3_ZF = 7.631 * m       # focal surface position
+C3_RP_MAX, C3_RP_MIN = 4.8 * m, 2.2 * m
+C3_RS_MAX = 2.7 * m
+C3_FOCAL_RC = 1.0 * m   # focal surface: spherical cap, radius of curvature 1 m
+C3_FOCAL_R = 0.2 * m    # 0.4 m across
+C3_NPOL = 8             # r^2 .. r^16
+
+
+def c3_coefficients(seed=0xC3):
+    """Seeded draws: conic ~ U[-1, 0]; polynomial coefficient k (of r^2k) ~
+    N(0, 1e-3 / R^(2k-1)) with R the surface's outer radius in cm."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    out = {}
+    for name, rmax in (("primary", C3_RP_MAX), ("secondary", C3_RS_MAX)):
+        conic = rng.uniform(-1.0, 0.0)
+        ks = [rng.normal(0.0, 1e-3 / rmax ** (2 * k - 1)) for k in range(1, C3_NPOL + 1)]
+        out[name] = (conic, ks)
+    return out
+
+
+def c3_scene(seed=0xC3):
+    """Primary (R 4.8 m, hole 2.2 m) and secondary (R 2.7 m) 1 um thick
+    AGeoAsphericDisk mirrors with base curvature 1/(2F) from the tutorial,
+    seeded conics and r^2..r^16 terms; spherical-cap focal surface (R_c = 1 m,
+    0.4 m across) with an obscuration disk behind it. World -> top -> optics,
+    as the tutorial nests them."""
+    co = c3_coefficients(seed)
+    mgr = AOpticsManager("manager", "SC")
+    world = AOpticalComponent("world", TGeoBBox("worldbox", 30 * m, 30 * m, 30 * m))
+    mgr.SetTopVolume(world)
+    top = AOpticalComponent("top", TGeoBBox("topbox", 25 * m, 25 * m, 25 * m))
+    cp, cs = 0.5 / C3_FP, 0.5 / C3_FS
+    kp, polp = co["primary"]
+    prim = AGeoAsphericDisk("primaryV", -1 * um, cp, 0.0, cp, C3_RP_MAX, C3_RP_MIN)
+    prim.SetConicConstants(kp, kp)
+    prim.SetPolynomials(C3_NPOL, polp, C3_NPOL, polp)
+    top.AddNode(AMirror("primaryMirror", prim), 1)
+    ks, pols = co["secondary"]
+    sec = AGeoAsphericDisk("secondaryV", C3_ZS, cs, C3_ZS + 1 * um, cs, C3_RS_MAX, 0.0)
+    sec.SetConicConstants(ks, ks)
+    sec.SetPolynomials(C3_NPOL, pols, C3_NPOL, pols)
+    top.AddNode(AMirror("secondaryMirror", sec), 1)
+    cf = -1.0 / C3_FOCAL_RC
+    foc = AGeoAsphericDisk("focalV", C3_ZF - 1 * mm, cf, C3_ZF, cf, C3_FOCAL_R, 0.0)
+    top.AddNode(AFocalSurface("focalPlane", foc), 1)
+    zlow = foc.CalcF1(C3_FOCAL_R)
+    obs = AObscuration("focalObs", TGeoTube("focalObsV", 0, C3_FOCAL_R * 1.25, 0.5 * cm))
+    top.AddNode(obs, 1, TGeoTranslation(0, 0, zlow - 1 * cm))
+    world.AddNode(top, 1)
+    mgr.CloseGeometry()
+    return mgr
+
+
+def c3_raygen(seed=0xC3):
+    """Parallel beams over a 10 m disk from 1.2 z_s up the axis, 0-5 deg."""
+    g = A.rb_raygen_desc()
+    g.kind = A.RB_RAYGEN_DISK
+    g.size = 10 * m
+    g.distance = 1.2 * C3_ZS
+    g.theta_min = 0.0
+    g.theta_max = 5.0 * math.pi / 180.0
+    g.lambda_min = g.lambda_max = 400 * nm
     g.seed = seed
     return g

@@ -574,6 +574,191 @@ template <typename T> __device__ V3<T> pgon_normal(const double* p, V3<T> q, V3<
   return n;
 }
 
+// ------------------------------------------------------------ aspheric disk
+// AGeoAsphericDisk (src/AGeoAsphericDisk.cxx). params: z1 c1 z2 c2 rmax rmin
+// kappa1 kappa2 n1 n2 K1[n1] K2[n2]. Both distances are the minimum over the
+// two aspheres and the two walls (:363-412); an asphere crossing is the
+// Matsui spherical seed refined by Newton steps along the ray (:415-526).
+template <typename T> __device__ __forceinline__ T newton_eps() { return T(1e-10); }
+template <> __device__ __forceinline__ float newton_eps<float>() { return 2e-4f; }
+
+// polynomial part with normalised coefficients: sum Kn_k h^k, h = r^2 / rmax^2
+template <typename T>
+__device__ __forceinline__ const double* asph_kn(const double* p, int s) {
+  return p + 10 + int(p[8]) + int(p[9]) + (s ? int(p[8]) : 0);
+}
+template <typename T>
+__device__ bool asph_sag(const double* p, int s, T r, T& f) {
+  const T c = T(p[1 + 2 * s]), kap = T(p[6 + s]);
+  const T pp = r * r * c * c * kap;
+  if (T(1) - pp < T(0)) return false;
+  const int n = int(p[8 + s]);
+  const double* K = asph_kn<T>(p, s);
+  const T r2 = r * r, h = r2 / T(p[4] * p[4]);
+  T acc = r2 * c / (T(1) + sqrt(T(1) - pp)), hk = h;
+  for (int i = 0; i < n; i++) {
+    acc += T(K[i]) * hk;
+    hk *= h;
+  }
+  f = T(p[2 * s]) + acc;
+  return true;
+}
+template <typename T>
+__device__ bool asph_dsag(const double* p, int s, T r, T& df) {
+  const T c = T(p[1 + 2 * s]), kap = T(p[6 + s]);
+  const T pp = r * r * c * c * kap;
+  if (T(1) - pp <= T(0)) return false;
+  const int n = int(p[8 + s]);
+  const double* K = asph_kn<T>(p, s);
+  const T R2inv = T(1.0 / (p[4] * p[4])), h = r * r * R2inv;
+  T acc = r * c / sqrt(T(1) - pp), poly = T(0), hk = T(1);
+  for (int i = 0; i < n; i++) {
+    poly += T(2 * (i + 1)) * T(K[i]) * hk;
+    hk *= h;
+  }
+  df = acc + poly * r * R2inv;
+  return true;
+}
+template <typename T>
+__device__ T asph_surface(const double* p, int s, V3<T> P, V3<T> D) {
+  const T c = T(p[1 + 2 * s]), kap = T(p[6 + s]), dz = T(p[2 * s]);
+  const int n = int(p[8 + s]);
+  const double* K = asph_kn<T>(p, s);
+  const T R2inv = T(1.0 / (p[4] * p[4]));
+  T H2 = P.x * P.x + P.y * P.y;
+  T pz = P.z - dz;
+  T pr = -(pz * D.z + P.x * D.x + P.y * D.y);
+  T M = pr * D.z + pz;
+  T M2 = pz * pz + H2 - pr * pr;
+  T chk = T(1) - (M2 * c - T(2) * M) * c / D.z / D.z;
+  if (!(chk >= T(0))) return T(kBig);
+  T q = pr + (M2 * c - T(2) * M) / (D.z * (T(1) + sqrt(chk)));
+  T nx = P.x + q * D.x, ny = P.y + q * D.y, nz = pz + q * D.z;
+  const T eps = newton_eps<T>();
+  T e = T(kBig);
+  for (int i = 0;; i++) {
+    if (i > 100) {
+      // FP32 may stall at its noise floor: keep a close iterate for the FP64 polish
+      if (sizeof(T) == 4 && fabs(e) < T(1e-2)) break;
+      return T(kBig);
+    }
+    H2 = nx * nx + ny * ny;
+    chk = T(1) - kap * H2 * c * c;
+    if (chk < T(0)) return T(kBig);
+    T l = sqrt(chk);
+    // conic sag c r^2 / (1 + l): the same value as the reference's
+    // (1 - l) / (c kappa) without its cancellation
+    T x = H2 * c / (T(1) + l);
+    const T h = H2 * R2inv;
+    T v = T(0), hk = T(1);
+    for (int j = 0; j < n; j++) {
+      T kj = T(K[j]);
+      v += T(2 * (j + 1)) * kj * hk;
+      hk *= h;
+      x += kj * hk;
+    }
+    v = c * kap + l * v * R2inv;
+    T m = -nx * v, mm = -ny * v;
+    T inv = T(1) / sqrt(l * l + m * m + mm * mm);
+    l *= inv; m *= inv; mm *= inv;
+    chk = D.z * l + D.x * m + D.y * mm;
+    if (chk == T(0)) return T(kBig);
+    e = l * (x - nz) / chk;
+    nx += e * D.x; ny += e * D.y; nz += e * D.z;
+    if (fabs(e) < eps) break;
+  }
+  nz += dz;
+  T ex = nx - P.x, ey = ny - P.y, ez = nz - P.z;
+  if (D.x * ex + D.y * ey + D.z * ez < T(0)) return T(kBig);
+  T rz = sqrt(nx * nx + ny * ny);
+  if (rz < T(p[5]) || rz > T(p[4])) return T(kBig);
+  return sqrt(ex * ex + ey * ey + ez * ez);
+}
+template <typename T>
+__device__ T asph_wall(const double* p, bool outer, V3<T> P, V3<T> D) {
+  const T R = outer ? T(p[4]) : T(p[5]);
+  if (!outer && !(R > T(0))) return T(kBig);
+  T rsq = P.x * P.x + P.y * P.y, nsq = D.x * D.x + D.y * D.y;
+  if (sqrt(nsq) < T(kTol)) return T(kBig);
+  T rdotn = P.x * D.x + P.y * D.y;
+  T t1i = T(1) / nsq, b = t1i * rdotn, cc = t1i * (rsq - R * R);
+  T delta = b * b - cc;
+  if (!(delta > T(0))) return T(kBig);
+  delta = sqrt(delta);
+  T t1 = -b + delta, t2 = -b - delta;
+  if (t1 < T(0) && t2 < T(0)) return T(kBig);
+  T zmin, zmax;
+  if (!asph_sag<T>(p, 0, R, zmin) || !asph_sag<T>(p, 1, R, zmax)) return T(kBig);
+  if (t2 > T(0)) {
+    T z1 = t1 * D.z + P.z, z2 = t2 * D.z + P.z;
+    if (z1 < zmin || zmax < z1) t1 = T(kBig);
+    if (z2 < zmin || zmax < z2) t2 = T(kBig);
+    return t1 < t2 ? t1 : t2;
+  } else if (t2 == T(0)) {
+    if (t1 > T(0)) {
+      if (zmin <= P.z && P.z <= zmax) return T(0);
+      T z1 = t1 * D.z + P.z;
+      if (zmin <= z1 && z1 <= zmax) return t1;
+    } else if (t1 == T(0)) {
+      if (zmin <= P.z && P.z <= zmax) return T(0);
+    }
+  } else {
+    if (t1 > T(0)) {
+      T z1 = t1 * D.z + P.z;
+      if (zmin <= z1 && z1 <= zmax) return t1;
+    } else if (t1 == T(0)) {
+      if (zmin <= P.z && P.z <= zmax) return T(0);
+    }
+  }
+  return T(kBig);
+}
+template <typename T> __device__ T asph_dist(const double* p, V3<T> q, V3<T> d) {
+  T a = asph_surface<T>(p, 0, q, d), b = asph_surface<T>(p, 1, q, d);
+  T c = asph_wall<T>(p, false, q, d), e = asph_wall<T>(p, true, q, d);
+  T m = a;
+  m = b < m ? b : m;
+  m = c < m ? c : m;
+  m = e < m ? e : m;
+  return m;
+}
+template <typename T> __device__ bool asph_contains(const double* p, V3<T> q) {
+  T r = sqrt(q.x * q.x + q.y * q.y);
+  if (r > T(p[4]) || r < T(p[5])) return false;
+  T f1, f2;
+  if (!asph_sag<T>(p, 0, r, f1) || !asph_sag<T>(p, 1, r, f2)) return false;
+  return !(q.z < f1 || f2 < q.z);
+}
+template <typename T> __device__ V3<T> asph_normal(const double* p, V3<T> q, V3<T> d) {
+  T r = sqrt(q.x * q.x + q.y * q.y);
+  T saf0 = T(p[5]) > T(0) ? fabs(r - T(p[5])) : T(kBig);
+  T saf1 = fabs(r - T(p[4]));
+  T saf2 = T(kBig), saf3 = T(kBig), df1 = T(kBig), df2 = T(kBig), f;
+  if (asph_sag<T>(p, 0, r, f) && asph_dsag<T>(p, 0, r, df1)) saf2 = fabs(f - q.z) / sqrt(T(1) + df1 * df1);
+  if (asph_sag<T>(p, 1, r, f) && asph_dsag<T>(p, 1, r, df2)) saf3 = fabs(f - q.z) / sqrt(T(1) + df2 * df2);
+  int k = 0;
+  T m = saf0;
+  if (saf1 < m) { k = 1; m = saf1; }
+  if (saf2 < m) { k = 2; m = saf2; }
+  if (saf3 < m) { k = 3; }
+  T n0 = T(0), n2 = T(0);
+  if (k <= 1) {
+    n0 = T(1);
+  } else {
+    T g = k == 2 ? df1 : df2;
+    if (g == T(0)) {
+      n2 = T(1);
+    } else {
+      T inv = T(1) / sqrt(T(1) + g * g);
+      n0 = g * inv;
+      n2 = -inv;
+    }
+  }
+  T cp = r > T(0) ? q.x / r : T(1), sp = r > T(0) ? q.y / r : T(0);
+  V3<T> n = mk<T>(n0 * cp, n0 * sp, n2);
+  if (dot(n, d) < T(0)) n = scale(n, T(-1));
+  return n;
+}
+
 // ------------------------------------------------------------ dispatch
 template <typename T> __device__ __noinline__ bool prim_contains(int type, const double* p, V3<T> q) {
   switch (type) {
@@ -582,6 +767,7 @@ template <typename T> __device__ __noinline__ bool prim_contains(int type, const
     case RB_SHAPE_SPHERE: return sphere_contains<T>(p, q);
     case RB_SHAPE_PARABOLOID: return para_contains<T>(p, q);
     case RB_SHAPE_PGON: return pgon_contains<T>(p, q);
+    case RB_SHAPE_ASPHERIC: return asph_contains<T>(p, q);
   }
   return false;
 }
@@ -592,6 +778,7 @@ template <typename T> __device__ __noinline__ T prim_dist_out(int type, const do
     case RB_SHAPE_SPHERE: return sphere_dist<T, true>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_dist_out<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_dist_out<T>(p, q, d);
+    case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
   }
   return T(kBig);
 }
@@ -602,6 +789,7 @@ template <typename T> __device__ __noinline__ T prim_dist_in(int type, const dou
     case RB_SHAPE_SPHERE: return sphere_dist<T, false>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_dist_in<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_dist_in<T>(p, q, d);
+    case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
   }
   return T(0);
 }
@@ -612,6 +800,7 @@ template <typename T> __device__ __noinline__ V3<T> prim_normal(int type, const 
     case RB_SHAPE_SPHERE: return sphere_normal<T>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_normal<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_normal<T>(p, q, d);
+    case RB_SHAPE_ASPHERIC: return asph_normal<T>(p, q, d);
   }
   return mk<T>(T(0), T(0), T(1));
 }

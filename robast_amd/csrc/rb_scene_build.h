@@ -161,6 +161,12 @@ int validate(std::string& err, const rb_scene_desc* d) {
           if (p[4 + 3 * k + 1] != 0.0) return fail_msg(err, -3, "scene: TGeoPgon with rmin > 0 is unsupported");
         break;
       }
+      case RB_SHAPE_ASPHERIC: {
+        const double* p = d->params + s.param_offset;
+        if (s.n_params < 10 || int(p[8]) < 0 || int(p[9]) < 0 || s.n_params != 10 + 2 * (int(p[8]) + int(p[9])))
+          return fail_msg(err, -1, "scene: aspheric disk parameter count");
+        break;
+      }
       case RB_SHAPE_COMPOSITE:
         if (s.n_params != 27) return fail_msg(err, -1, "scene: composite needs 27 parameters");
         if (csg_depth(d, i, 0) > kMaxCsgDepth)

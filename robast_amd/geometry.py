@@ -337,6 +337,73 @@ class TGeoPgon(TGeoPcon):
         return np.array([0, 0, 0.5 * (z0 + z1)]), np.array([r, r, 0.5 * (z1 - z0)])
 
 
+class AGeoAsphericDisk(TGeoShape):
+    """Disk bounded by two aspheric surfaces z = z_i + c r^2 / (1 + sqrt(1 -
+    (1+K) c^2 r^2)) + sum_k K_k r^(2k) and cylinders at rmin / rmax
+    (src/AGeoAsphericDisk.cxx; SetAsphDimensions orders z1 < z2 and rmin < rmax)."""
+    shape_type = A.RB_SHAPE_ASPHERIC
+
+    def __init__(self, *args):
+        name = ""
+        if args and isinstance(args[0], str):
+            name, args = args[0], args[1:]
+        super().__init__(name)
+        z1, c1, z2, c2, rmax = (float(v) for v in args[:5])
+        rmin = float(args[5]) if len(args) > 5 else 0.0
+        if z1 < z2:
+            self.z1, self.z2, self.c1, self.c2 = z1, z2, c1, c2
+        else:
+            self.z1, self.z2, self.c1, self.c2 = z2, z1, c2, c1
+        rmax, rmin = abs(rmax), abs(rmin)
+        self.rmax, self.rmin = (rmax, rmin) if rmax > rmin else (rmin, rmax)
+        self.conic1 = self.conic2 = 0.0
+        self.k1, self.k2 = [], []
+
+    def SetConicConstants(self, conic1, conic2):
+        self.conic1, self.conic2 = float(conic1), float(conic2)
+
+    def SetPolynomials(self, n1, k1, n2, k2):
+        self.k1 = [float(k1[i]) for i in range(n1)] if n1 > 0 else []
+        self.k2 = [float(k2[i]) for i in range(n2)] if n2 > 0 else []
+
+    def _sag(self, which, r):
+        z, c, kap, ks = ((self.z1, self.c1, self.conic1 + 1, self.k1) if which == 1 else
+                         (self.z2, self.c2, self.conic2 + 1, self.k2))
+        p = r * r * c * c * kap
+        if 1 - p < 0:
+            return None
+        v = z + r * r * c / (1 + math.sqrt(1 - p))
+        for i, k in enumerate(ks):
+            v += k * r ** (2 * (i + 1))
+        return v
+
+    def CalcF1(self, r):
+        return self._sag(1, r)
+
+    def CalcF2(self, r):
+        return self._sag(2, r)
+
+    def params(self):
+        # the K_k r^2k terms again as Kn_k (r / rmax)^2k, Kn_k = K_k rmax^2k, so
+        # single precision never forms r^16 (1e42 cm^16 overflows FP32)
+        R = self.rmax if self.rmax > 0 else 1.0
+        kn1 = [k * R ** (2 * (i + 1)) for i, k in enumerate(self.k1)]
+        kn2 = [k * R ** (2 * (i + 1)) for i, k in enumerate(self.k2)]
+        return [self.z1, self.c1, self.z2, self.c2, self.rmax, self.rmin, self.conic1 + 1, self.conic2 + 1,
+                float(len(self.k1)), float(len(self.k2)), *self.k1, *self.k2, *kn1, *kn2]
+
+    def local_bbox(self):
+        rs = np.linspace(self.rmin, self.rmax, 4001)
+        f1 = [self._sag(1, r) for r in rs]
+        f2 = [self._sag(2, r) for r in rs]
+        f1 = [v for v in f1 if v is not None] or [self.z1]
+        f2 = [v for v in f2 if v is not None] or [self.z2]
+        zmin, zmax = min(f1), max(f2)
+        pad = 1e-3 * (zmax - zmin) + 0.1
+        zmin, zmax = zmin - pad, zmax + pad
+        return np.array([0, 0, 0.5 * (zmin + zmax)]), np.array([self.rmax, self.rmax, 0.5 * (zmax - zmin)])
+
+
 class TGeoBoolNode:
     def __init__(self, op, left, right, lmat, rmat):
         self.op, self.left, self.right = op, left, right

@@ -237,3 +237,78 @@ def test_dc_facet_normals_aim_at_2f():
         want /= np.linalg.norm(want)
         assert np.allclose(axis, want, atol=1e-12)
         assert np.hypot(tr[0], tr[1]) ** 2 + (tr[2] - configs.C2_F) ** 2 == pytest.approx(configs.C2_F ** 2)
+
+
+def _asph_shape():
+    G._reset_registry()
+    a = G.AGeoAsphericDisk("asph", 0.0, 1 / 60.0, 4.0, 1 / 80.0, 10.0, 2.0)
+    a.SetConicConstants(-0.4, 0.3)
+    a.SetPolynomials(3, [1e-4, -2e-7, 3e-10], 2, [-5e-5, 1e-7])
+    return a
+
+
+def test_aspheric_distances_match_marching_for_axial_rays():
+    # DistToAsphere finds one root per surface (src/AGeoAsphericDisk.cxx:415-526),
+    # so the check uses rays within ~45 deg of the axis, as its comment advises
+    shape = _asph_shape()
+    flat, sid = _shape_scene(shape)
+    rng = np.random.default_rng(11)
+    n_hit = 0
+    for _ in range(400):
+        p = np.array([rng.uniform(-12, 12), rng.uniform(-12, 12), rng.choice([-8.0, 14.0])])
+        d = np.array([rng.normal() * 0.4, rng.normal() * 0.4, -np.sign(p[2] - 2.0)])
+        d /= np.linalg.norm(d)
+        inside, dout, din, _n = O.shape_query(flat, sid, p, d)
+        assert not inside
+        t = _march(flat, sid, p, d, False, tmax=40.0, n=8000)
+        if t is None:
+            assert dout >= 1e29
+        else:
+            n_hit += 1
+            assert dout == pytest.approx(t, abs=2e-7), (p, d, dout, t)
+    assert n_hit > 50
+
+
+def test_aspheric_newton_root_is_exact():
+    """The converged crossing satisfies z = sag(r) to 1e-10 (mpmath check)."""
+    mp.mp.dps = 40
+    shape = _asph_shape()
+    flat, sid = _shape_scene(shape)
+    rng = np.random.default_rng(12)
+
+    def sag(which, r):
+        z, c, kap, ks = ((shape.z1, shape.c1, shape.conic1 + 1, shape.k1) if which == 1 else
+                         (shape.z2, shape.c2, shape.conic2 + 1, shape.k2))
+        r = mp.mpf(r)
+        v = mp.mpf(z) + r * r * c / (1 + mp.sqrt(1 - r * r * c * c * kap))
+        for i, k in enumerate(ks):
+            v += k * r ** (2 * (i + 1))
+        return v
+
+    checked = 0
+    for _ in range(100):
+        p = np.array([rng.uniform(-6, 6), rng.uniform(-6, 6), 14.0])
+        d = np.array([rng.normal() * 0.2, rng.normal() * 0.2, -1.0])
+        d /= np.linalg.norm(d)
+        _, dout, _, _ = O.shape_query(flat, sid, p, d)
+        if dout >= 1e29:
+            continue
+        h = p + dout * d
+        r = math.hypot(h[0], h[1])
+        if not (2.0 < r < 10.0):
+            continue
+        assert abs(float(sag(2, r)) - h[2]) < 1e-9, (p, d, h)
+        checked += 1
+    assert checked > 40
+
+
+def test_c3_schwarzschild_couder_scene_traces():
+    mgr = configs.c3_scene()
+    soa = O.generate(configs.c3_raygen(), 20000)
+    O.trace(mgr.flat(), soa, mgr.trace_opts(), nthreads=8)
+    c = np.bincount(soa["status"], minlength=6)
+    assert c[A.RB_RUN] == 0 and c[A.RB_FOCUS] > 1000 and c[A.RB_EXIT] > 1000
+    f = soa["status"] == A.RB_FOCUS
+    # focused rays end on the focal cap: z within its sag range
+    assert soa["z"][f].max() <= configs.C3_ZF + 1e-9
+    assert soa["z"][f].min() >= configs.C3_ZF - 0.1 - configs.C3_FOCAL_R ** 2 / 100.0
