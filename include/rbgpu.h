@@ -55,7 +55,8 @@ enum {
   RB_SHAPE_COMPOSITE = 7,  /* op left right lmat[12] rmat[12]       (TGeoBoolNode)    */
   RB_SHAPE_ASPHERIC = 8,   /* z1 c1 z2 c2 rmax rmin kappa1 kappa2 n1 n2 K1[n1] K2[n2]
                               Kn1[n1] Kn2[n2] with Kn_k = K_k rmax^2k (AGeoAsphericDisk) */
-  RB_SHAPE_WINSTON = 9     /* r1 r2 polyN f theta dz                (AGeoWinstonConePoly) */
+  RB_SHAPE_WINSTON = 9     /* r1 r2 polyN theta dz f cos(theta) sin(theta) tan(pi/N)
+                              {cos sin}(2 pi i / N) * N          (AGeoWinstonConePoly) */
 };
 
 /* boolean operators of RB_SHAPE_COMPOSITE (TGeoUnion/TGeoSubtraction/TGeoIntersection) */
@@ -174,8 +175,13 @@ typedef struct {
 /* ---- synthetic ray sources (device-side ARayShooter equivalents) ---- */
 enum {
   RB_RAYGEN_GRID = 0, /* ARayShooter::Square grid (src/ARayShooter.cxx:401-459)        */
-  RB_RAYGEN_DISK = 1  /* uniform disk of diameter `size`, normal to the direction,
+  RB_RAYGEN_DISK = 1, /* uniform disk of diameter `size`, normal to the direction,
                          angle ~ U[theta_min, theta_max], lambda ~ U[l_min, l_max]   */
+  RB_RAYGEN_CONE = 2  /* converging beam: from a pupil disk of diameter `size` at
+                         height `distance` above the image plane z = z0 toward the
+                         image of a field angle th ~ U[theta_min, theta_max] at
+                         azimuth ~ U[0, 2 pi) (radius distance * tan th); rays start
+                         `start` above the image plane                              */
 };
 
 typedef struct {
@@ -188,6 +194,8 @@ typedef struct {
   double lambda_min;  /* cm  */
   double lambda_max;  /* cm  */
   uint64_t seed;
+  double z0;          /* CONE: image plane */
+  double start;       /* CONE: start height above the image plane */
 } rb_raygen_desc;
 
 /* ---- API ---------------------------------------------------------------- */

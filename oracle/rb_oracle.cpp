@@ -69,8 +69,14 @@ static inline V3 l2mv(const rb_matrix& m, V3 q) {
           r[6] * q.x + r[7] * q.y + r[8] * q.z};
 }
 
+struct Box {
+  double lo[3], hi[3];
+};
+
 struct Scene {
   const rb_scene_desc* d;
+  std::vector<Box> boxes;  // per node, global frame: the bounding-box pre-check
+                           // every TGeo shape makes before its exact distance
   const double* P(int shape) const { return d->params + d->shapes[shape].param_offset; }
   int type(int shape) const { return d->shapes[shape].type; }
   const rb_matrix& mat(const double* p) const { return *reinterpret_cast<const rb_matrix*>(p); }
@@ -777,6 +783,150 @@ static V3 asph_normal(const double* p, V3 q, V3 d) {
   return n;
 }
 
+
+// ---------------------------------------------------------------- Winston cone (polygonal)
+// AGeoWinstonConePoly (src/AGeoWinstonConePoly.cxx) over AGeoWinstonCone2D's
+// parabola math (src/AGeoWinstonCone2D.cxx). params: r1 r2 polyN theta dz f ...
+struct Wins {
+  double r1, r2, theta, dz, f;
+  int n;
+};
+static Wins wins_p(const double* p) {
+  Wins w;
+  w.r1 = p[0]; w.r2 = p[1]; w.n = int(p[2]); w.theta = p[3]; w.dz = p[4]; w.f = p[5];
+  return w;
+}
+// AGeoWinstonCone2D::CalcR (:82-98); false where it throws (|z| > dz + 1e-10)
+static bool wins_r(const Wins& w, double z, double& r) {
+  if (std::fabs(z) > w.dz + 1e-10) return false;
+  double sint = std::sin(w.theta), cost = std::cos(w.theta);
+  double t = z + w.dz;
+  double a0 = t * t * sint * sint - 4. * w.f * (t * cost + w.f);
+  double a1 = 2. * t * sint * cost + 4. * w.f * sint;
+  double a2 = cost * cost;
+  r = (-a1 + std::sqrt(a1 * a1 - 4. * a0 * a2)) / (2 * a2) - w.r2;
+  return true;
+}
+static bool wins_drdz(const Wins& w, double z, double& g) {
+  if (std::fabs(z) > w.dz + 1e-10) return false;
+  double sint = std::sin(w.theta), cost = std::cos(w.theta);
+  double t = z + w.dz;
+  double a0 = t * t * sint * sint - 4. * w.f * (t * cost + w.f);
+  double a1 = 2. * t * sint * cost + 4. * w.f * sint;
+  double a2 = cost * cost;
+  double da0dt = 2 * t * sint * sint - 4 * w.f * cost, da1dt = 2 * sint * cost;
+  g = (-da1dt + (a1 * da1dt - 2 * da0dt * a2) / std::sqrt(a1 * a1 - 4 * a0 * a2)) / (2 * a2);
+  return true;
+}
+// AGeoWinstonConePoly::InsidePolygon (:293-309)
+static bool wins_inside_polygon(const Wins& w, double x, double y, double r) {
+  double th = std::atan2(y, x);
+  while (th > kPi / w.n) th -= 2 * kPi / w.n;
+  while (th < -kPi / w.n) th += 2 * kPi / w.n;
+  return !(std::sqrt(x * x + y * y) * std::cos(th) > r);
+}
+// AGeoWinstonCone2D::DistToParabola (:325-428): the side at azimuth phi,
+// in the tilted parabola frame; `open` is the accepted azimuth width
+static double wins_parabola(const Wins& w, V3 P, V3 D, double phi, double open) {
+  double x = std::cos(phi) * P.x + std::sin(phi) * P.y;
+  double y = -std::sin(phi) * P.x + std::cos(phi) * P.y;
+  double z = P.z;
+  double px = std::cos(phi) * D.x + std::sin(phi) * D.y;
+  double py = -std::sin(phi) * D.x + std::cos(phi) * D.y;
+  double pz = D.z;
+  if (px == 0 && pz == 0) return kBig;
+  double cost = std::cos(w.theta), sint = std::sin(w.theta);
+  double X = cost * (x + w.r2) + (z + w.dz) * sint;
+  double Z = -sint * (x + w.r2) + (z + w.dz) * cost + w.f;
+  double alpha = std::atan2(pz, px);
+  double tanA = std::tan(alpha - w.theta);
+  double dist[2] = {kBig, kBig};
+  double tmp = tanA * tanA - (X * tanA - Z) / w.f;
+  if (tmp < 0) return kBig;
+  double Xp, Xm;
+  if (w.dz * 2 / std::fabs(tanA) < kTol) {
+    Xp = X;
+    Xm = X;
+  } else {
+    Xp = 2 * w.f * (tanA + std::sqrt(tmp));
+    Xm = 2 * w.f * (tanA - std::sqrt(tmp));
+  }
+  double Xs[2] = {Xp, Xm};
+  for (int k = 0; k < 2; k++) {
+    double Xc = Xs[k];
+    double Zc = Xc * Xc / 4. / w.f;
+    double xc = cost * Xc - sint * (Zc - w.f) - w.r2;
+    double zc = sint * Xc + cost * (Zc - w.f) - w.dz;
+    double yc;
+    if (std::fabs(px) <= std::fabs(pz) && std::fabs(py) <= std::fabs(pz)) {
+      yc = y + (zc - z) * py / pz;
+    } else if (std::fabs(py) <= std::fabs(px) && std::fabs(pz) <= std::fabs(px)) {
+      yc = y + (xc - x) * py / px;
+    } else {
+      yc = y + (std::fabs(px) < 1e-5 ? (zc - z) * py / pz : (xc - x) * py / px);
+    }
+    double dx = xc - x, dy = yc - y, dz = zc - z;
+    if (xc < w.r2 || w.r1 < xc || zc < -w.dz || w.dz < zc || dx * px + dz * pz < 0) continue;
+    if (std::fabs(std::atan2(yc, xc)) <= open / 2.) dist[k] = std::sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  return std::min(dist[0], dist[1]);
+}
+static bool wins_contains(const double* p, V3 q) {
+  Wins w = wins_p(p);
+  if (std::fabs(q.z) > w.dz) return false;
+  double r;
+  if (!wins_r(w, q.z, r)) return false;
+  return wins_inside_polygon(w, q.x, q.y, r);
+}
+static double wins_dist_in(const double* p, V3 q, V3 d) {
+  Wins w = wins_p(p);
+  double best = kBig;
+  if (d.z < 0) best = (-q.z - w.dz) / d.z;
+  else if (d.z > 0) best = (w.dz - q.z) / d.z;
+  for (int i = 0; i < w.n; i++) best = std::min(best, wins_parabola(w, q, d, i * 2 * kPi / w.n, kPi));
+  return best;
+}
+static double wins_dist_out(const double* p, V3 q, V3 d) {
+  Wins w = wins_p(p);
+  if (q.z <= -w.dz) {
+    if (d.z <= 0) return kBig;
+    double s = -(w.dz + q.z) / d.z;
+    if (wins_inside_polygon(w, q.x + s * d.x, q.y + s * d.y, w.r2)) return s;
+  } else if (q.z >= w.dz) {
+    if (d.z >= 0) return kBig;
+    double s = (w.dz - q.z) / d.z;
+    if (wins_inside_polygon(w, q.x + s * d.x, q.y + s * d.y, w.r1)) return s;
+  }
+  double best = kBig;
+  for (int i = 0; i < w.n; i++) best = std::min(best, wins_parabola(w, q, d, i * 2 * kPi / w.n, 2 * kPi / w.n));
+  return best;
+}
+// AGeoWinstonConePoly::ComputeNormal (:67-122)
+static V3 wins_normal(const double* p, V3 q, V3 d) {
+  Wins w = wins_p(p);
+  double saf0 = std::fabs(std::fabs(w.dz) - std::fabs(q.z)), saf1;
+  double phi = std::atan2(q.y, q.x);
+  while (phi > kPi / w.n) phi -= 2 * kPi / w.n;
+  while (phi < -kPi / w.n) phi += 2 * kPi / w.n;
+  double r;
+  saf1 = wins_r(w, q.z, r) ? std::fabs(r - std::sqrt(q.x * q.x + q.y * q.y) * std::cos(phi)) : kBig;
+  V3 n;
+  if (saf0 <= saf1) {
+    n = v3(0, 0, 1);
+  } else {
+    phi = std::atan2(q.y, q.x);
+    if (phi < -kPi / w.n) phi += 2 * kPi;
+    int k = int(std::floor((phi + kPi / w.n) / (2 * kPi / w.n)));
+    double g = 0;
+    wins_drdz(w, q.z, g);
+    n = v3(std::cos(k * 2 * kPi / w.n), std::sin(k * 2 * kPi / w.n), -g);
+    double m = std::sqrt(dot(n, n));
+    n = mul(n, 1 / m);
+  }
+  if (dot(n, d) < 0) n = mul(n, -1);
+  return n;
+}
+
 // ---------------------------------------------------------------- dispatch
 static bool contains(const Scene& sc, int shape, V3 q);
 static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d);
@@ -807,6 +957,7 @@ static bool contains(const Scene& sc, int shape, V3 q) {
     case RB_SHAPE_PARABOLOID: return para_contains(p, q);
     case RB_SHAPE_PGON: return pgon_contains(p, q);
     case RB_SHAPE_ASPHERIC: return asph_contains(p, q);
+    case RB_SHAPE_WINSTON: return wins_contains(p, q);
     case RB_SHAPE_COMPOSITE: {
       Comp c = comp_of(sc, shape);
       bool a = contains(sc, c.left, m2l(*c.lm, q));
@@ -923,6 +1074,7 @@ static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
     case RB_SHAPE_PARABOLOID: return {para_dist_out(p, q, d), 0};
     case RB_SHAPE_PGON: return {pgon_dist_out(p, q, d), 0};
     case RB_SHAPE_ASPHERIC: return {asph_dist(p, q, d), 0};
+    case RB_SHAPE_WINSTON: return {wins_dist_out(p, q, d), 0};
     case RB_SHAPE_COMPOSITE: return comp_dist_out(sc, shape, q, d);
   }
   return {kBig, 0};
@@ -936,6 +1088,7 @@ static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
     case RB_SHAPE_PARABOLOID: return {para_dist_in(p, q, d), 0};
     case RB_SHAPE_PGON: return {pgon_dist_in(p, q, d), 0};
     case RB_SHAPE_ASPHERIC: return {asph_dist(p, q, d), 0};
+    case RB_SHAPE_WINSTON: return {wins_dist_in(p, q, d), 0};
     case RB_SHAPE_COMPOSITE: return comp_dist_in(sc, shape, q, d);
   }
   return {0, 0};
@@ -966,6 +1119,7 @@ static V3 shape_normal(const Scene& sc, int shape, uint32_t leaf, V3 q, V3 d) {
     case RB_SHAPE_PARABOLOID: n = para_normal(p, q, d); break;
     case RB_SHAPE_PGON: n = pgon_normal(p, q, d); break;
     case RB_SHAPE_ASPHERIC: n = asph_normal(p, q, d); break;
+    case RB_SHAPE_WINSTON: n = wins_normal(p, q, d); break;
     default: n = v3(0, 0, 1);
   }
   while (ns > 0) n = l2mv(*stack[--ns], n);
@@ -1060,6 +1214,53 @@ struct Rng {
 };
 
 // ---------------------------------------------------------------- navigator
+static void build_boxes(Scene& sc) {
+  const rb_scene_desc* d = sc.d;
+  sc.boxes.resize(d->n_nodes);
+  for (int n = 0; n < d->n_nodes; n++) {
+    const rb_node_desc& nd = d->nodes[n];
+    const rb_shape_desc& sh = d->shapes[d->volumes[nd.volume].shape];
+    Box b;
+    for (int k = 0; k < 3; k++) { b.lo[k] = 1e300; b.hi[k] = -1e300; }
+    for (int c = 0; c < 8; c++) {
+      double l[3];
+      for (int k = 0; k < 3; k++) l[k] = sh.bbox_origin[k] + ((c >> k) & 1 ? 1 : -1) * sh.bbox_half[k];
+      for (int r = 0; r < 3; r++) {
+        double m = nd.global.tr[r] + nd.global.rot[3 * r] * l[0] + nd.global.rot[3 * r + 1] * l[1] +
+                   nd.global.rot[3 * r + 2] * l[2];
+        b.lo[r] = std::min(b.lo[r], m);
+        b.hi[r] = std::max(b.hi[r], m);
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      double pad = 1e-6 * (1 + std::fabs(b.lo[k]) + std::fabs(b.hi[k]));
+      b.lo[k] -= pad;
+      b.hi[k] += pad;
+    }
+    sc.boxes[n] = b;
+  }
+}
+static bool box_has_point(const Box& b, V3 p) {
+  return p.x >= b.lo[0] && p.x <= b.hi[0] && p.y >= b.lo[1] && p.y <= b.hi[1] && p.z >= b.lo[2] && p.z <= b.hi[2];
+}
+// slab test: can the ray meet the box before `limit`?
+static bool box_hit(const Box& b, V3 p, V3 d, double limit) {
+  double t0 = 0, t1 = limit + 1e-6 * (1 + limit);
+  double o[3] = {p.x, p.y, p.z}, v[3] = {d.x, d.y, d.z};
+  for (int k = 0; k < 3; k++) {
+    if (v[k] == 0) {
+      if (o[k] < b.lo[k] || o[k] > b.hi[k]) return false;
+      continue;
+    }
+    double a = (b.lo[k] - o[k]) / v[k], c = (b.hi[k] - o[k]) / v[k];
+    if (a > c) std::swap(a, c);
+    t0 = std::max(t0, a);
+    t1 = std::min(t1, c);
+    if (t0 > t1) return false;
+  }
+  return true;
+}
+
 struct Nav {
   const Scene* sc;
   V3 p, d;            // current point and direction (fPoint, fDirection)
@@ -1098,6 +1299,7 @@ static int search_node(const Scene& sc, int start, V3 p, int skip, bool forced) 
     for (int k = 0; k < nd.n_children; k++) {
       int c = sc.d->children[nd.first_child + k];
       if (c == skip) continue;
+      if (!box_has_point(sc.boxes[c], p)) continue;
       if (node_contains(sc, c, p)) {
         found = c;
         break;
@@ -1172,6 +1374,7 @@ static int find_next_boundary_and_step(Nav& nv) {
   }
   for (int k = 0; k < cn.n_children; k++) {
     int c = sc.d->children[cn.first_child + k];
+    if (!box_hit(sc.boxes[c], nv.p, nv.d, best)) continue;
     const rb_node_desc& chn = node(sc, c);
     Hit h = dist_out(sc, node_shape(sc, c), m2l(chn.global, nv.p), m2lv(chn.global, nv.d));
     if (h.s < best - kTol) {
@@ -1451,7 +1654,8 @@ extern "C" {
 // (n / nthreads), the last taking the remainder (src/AOpticsManager.cxx:533-541).
 int oracle_trace(const rb_scene_desc* desc, const rb_trace_opts* opts, rb_rays* rays, int64_t n, int nthreads) {
   if (!desc || !opts || !rays || n < 0) return -1;
-  Scene sc{desc};
+  Scene sc{desc, {}};
+  build_boxes(sc);
   Opts o{opts->limit > 0 ? opts->limit : 100, opts->fresnel != 0};
   if (nthreads < 2 || n < 2 * nthreads) {
     trace_range(sc, o, opts, rays, 0, n);
@@ -1469,7 +1673,7 @@ int oracle_trace(const rb_scene_desc* desc, const rb_trace_opts* opts, rb_rays* 
 
 // Shape-level probes for tests (local frame of `shape`)
 int oracle_shape_query(const rb_scene_desc* desc, int shape, const double* p, const double* d, double* out) {
-  Scene sc{desc};
+  Scene sc{desc, {}};
   V3 q = v3(p[0], p[1], p[2]), dd = v3(d[0], d[1], d[2]);
   out[0] = contains(sc, shape, q) ? 1.0 : 0.0;
   Hit a = dist_out(sc, shape, q, dd);
@@ -1486,33 +1690,50 @@ int oracle_shape_query(const rb_scene_desc* desc, int shape, const double* p, co
 // (-D sin th, 0, D cos th); DISK draws 4 Philox uniforms per ray (stream 1).
 int oracle_generate_rays(const rb_raygen_desc* g, rb_rays* r, int64_t n) {
   for (int64_t i = 0; i < n; i++) {
-    double th, lam, px, py;
-    if (g->kind == RB_RAYGEN_GRID) {
-      int ns = g->n_side;
-      double delta = ns == 1 ? g->size / 2 : g->size / (ns - 1);
-      int64_t a = i / ns, b = i % ns;
-      px = a * delta - g->size / 2;
-      py = b * delta - g->size / 2;
-      th = g->theta_min;
-      lam = g->lambda_min;
-    } else {
+    double ox, oy, oz, dx, dy, dz, lam;
+    if (g->kind == RB_RAYGEN_CONE) {
       Rng u{g->seed, uint64_t(i), 0u, 1u};
-      double u0 = u.uniform(), u1 = u.uniform(), u2 = u.uniform(), u3 = u.uniform();
-      th = g->theta_min + (g->theta_max - g->theta_min) * u2;
-      lam = g->lambda_min + (g->lambda_max - g->lambda_min) * u3;
-      double rr = 0.5 * g->size * std::sqrt(u0), ph = 2 * kPi * u1;
-      px = rr * std::cos(ph);
-      py = rr * std::sin(ph);
-    }
-    double st = std::sin(th), ct = std::cos(th), t2 = kPi - th;
-    double cx = -g->distance * st, cz = g->distance * ct;
-    if (g->kind == RB_RAYGEN_GRID) {
-      r->x[i] = px + cx; r->y[i] = py; r->z[i] = cz;
+      double u0 = u.uniform(), u1 = u.uniform(), u2 = u.uniform(), u3 = u.uniform(), u4 = u.uniform();
+      double rho = 0.5 * g->size * std::sqrt(u0), al = 2 * kPi * u1;
+      double th = g->theta_min + (g->theta_max - g->theta_min) * u2, ph = 2 * kPi * u3;
+      lam = g->lambda_min + (g->lambda_max - g->lambda_min) * u4;
+      double ri = g->distance * std::tan(th);
+      double ix = ri * std::cos(ph), iy = ri * std::sin(ph);
+      double vx = ix - rho * std::cos(al), vy = iy - rho * std::sin(al), vz = -g->distance;
+      double m = std::sqrt(vx * vx + vy * vy + vz * vz);
+      dx = vx / m; dy = vy / m; dz = vz / m;
+      double s = g->start / -dz;
+      ox = ix - s * dx; oy = iy - s * dy; oz = g->z0 + g->start;
     } else {
-      r->x[i] = cx + px * ct; r->y[i] = py; r->z[i] = cz + px * st;
+      double th, px, py;
+      if (g->kind == RB_RAYGEN_GRID) {
+        int ns = g->n_side;
+        double delta = ns == 1 ? g->size / 2 : g->size / (ns - 1);
+        int64_t a = i / ns, b = i % ns;
+        px = a * delta - g->size / 2;
+        py = b * delta - g->size / 2;
+        th = g->theta_min;
+        lam = g->lambda_min;
+      } else {
+        Rng u{g->seed, uint64_t(i), 0u, 1u};
+        double u0 = u.uniform(), u1 = u.uniform(), u2 = u.uniform(), u3 = u.uniform();
+        th = g->theta_min + (g->theta_max - g->theta_min) * u2;
+        lam = g->lambda_min + (g->lambda_max - g->lambda_min) * u3;
+        double rr = 0.5 * g->size * std::sqrt(u0), ph = 2 * kPi * u1;
+        px = rr * std::cos(ph);
+        py = rr * std::sin(ph);
+      }
+      double st = std::sin(th), ct = std::cos(th), t2 = kPi - th;
+      double cx = -g->distance * st, cz = g->distance * ct;
+      if (g->kind == RB_RAYGEN_GRID) {
+        ox = px + cx; oy = py; oz = cz;
+      } else {
+        ox = cx + px * ct; oy = py; oz = cz + px * st;
+      }
+      dx = std::sin(t2); dy = 0; dz = std::cos(t2);
     }
-    r->t[i] = 0;
-    r->dx[i] = std::sin(t2); r->dy[i] = 0; r->dz[i] = std::cos(t2);
+    r->x[i] = ox; r->y[i] = oy; r->z[i] = oz; r->t[i] = 0;
+    r->dx[i] = dx; r->dy[i] = dy; r->dz[i] = dz;
     r->lambda[i] = lam;
     r->status[i] = RB_RUN;
     r->npoints[i] = 1;
@@ -1523,7 +1744,7 @@ int oracle_generate_rays(const rb_raygen_desc* g, rb_rays* r, int64_t n) {
 
 // Batch containment probe for brute-force distance checks (local frame of `shape`).
 int oracle_contains_batch(const rb_scene_desc* desc, int shape, int64_t n, const double* pts, uint8_t* out) {
-  Scene sc{desc};
+  Scene sc{desc, {}};
   for (int64_t i = 0; i < n; i++) out[i] = contains(sc, shape, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2])) ? 1 : 0;
   return 0;
 }

@@ -30,7 +30,7 @@ RB_INDEX_NONE, RB_INDEX_TABLE, RB_INDEX_SELLMEIER, RB_INDEX_SCHOTT = 0, 1, 2, 3
 
 RB_PRECISION_FP64, RB_PRECISION_MIXED = 0, 1
 
-RB_RAYGEN_GRID, RB_RAYGEN_DISK = 0, 1
+RB_RAYGEN_GRID, RB_RAYGEN_DISK, RB_RAYGEN_CONE = 0, 1, 2
 
 SHAPE_DTYPE = np.dtype(
     [("type", "<i4"), ("param_offset", "<i4"), ("n_params", "<i4"), ("pad", "<i4"),
@@ -88,6 +88,7 @@ class rb_raygen_desc(C.Structure):
         ("kind", C.c_int32), ("n_side", C.c_int32), ("size", C.c_double),
         ("distance", C.c_double), ("theta_min", C.c_double), ("theta_max", C.c_double),
         ("lambda_min", C.c_double), ("lambda_max", C.c_double), ("seed", C.c_uint64),
+        ("z0", C.c_double), ("start", C.c_double),
     ]
 
 

@@ -3,17 +3,18 @@
 C1 parabolic telescope (after tutorials/SimpleParabolicTelescope.C:20-88)
 C2 Davies-Cotton 85-facet reflector (after tutorials/DaviesCotton.C:58-157)
 C3 Schwarzschild-Couder aspheric two-mirror system (after tutorials/SchwarzschildCouder.C:33-90)
+C4 BK7 window + 1,027 hexagonal Winston-cone light guides (after tutorials/HexWinstonCone.C:29-75)
 """
 import math
 
 import numpy as np
 
 from . import _abi as A
-from .geometry import (AFocalSurface, AGeoAsphericDisk, AMirror, AObscuration, AOpticalComponent, TGeoBBox, TGeoCombiTrans,
+from .geometry import (AFocalSurface, AGeoAsphericDisk, AGeoWinstonConePoly, ALens, AMirror, AObscuration, AOpticalComponent, TGeoBBox, TGeoCombiTrans,
                        TGeoCompositeShape, TGeoParaboloid, TGeoPgon, TGeoRotation, TGeoSphere, TGeoTranslation,
                        TGeoTube, cm, m, mm, nm, um)
 from .manager import AOpticsManager
-from .materials import TGraph
+from .materials import ASellmeierFormula, ARefractiveIndex, TGraph
 from .rays import ARayArray, ARayShooter
 
 # ------------------------------------------------------------------ C1
@@ -245,4 +246,90 @@ def c3_raygen(seed=0xC3):
     g.theta_max = 5.0 * math.pi / 180.0
     g.lambda_min = g.lambda_max = 400 * nm
     g.seed = seed
+    return g
+
+
+# ------------------------------------------------------------------ C4
+C4_R1, C4_R2 = 12.5 * mm, 5.0 * mm     # entrance / exit apothems (25 mm, 10 mm flat-to-flat)
+C4_RINGS = 18                           # 1 + 3 * 18 * 19 = 1027 cones
+C4_WALL_R = 0.9
+C4_BK7 = (1.0396, 0.23179, 1.01047, 6.0007e-3, 2.0018e-2, 103.56)
+C4_ABS_400 = 1.0 * m                    # absorption length at 400 nm
+C4_FIELD_DEG = 6.0
+C4_FNUMBER = 1.2
+
+
+def c4_positions(rings=C4_RINGS, pitch=None):
+    """Centres of a hexagonal array of hexagonal cones whose flats face +-x
+    (the cone's side normals are at 0, 60, ... deg): all lattice points
+    i a1 + j a2 with a1 = pitch (1, 0), a2 = pitch (1/2, sqrt(3)/2) and hex
+    distance max(|i|, |j|, |i + j|) <= rings."""
+    if pitch is None:
+        pitch = 2 * C4_R1 * 1.002
+    pts = []
+    for i in range(-rings, rings + 1):
+        for j in range(-rings, rings + 1):
+            if max(abs(i), abs(j), abs(i + j)) <= rings:
+                pts.append((pitch * (i + 0.5 * j), pitch * (math.sqrt(3) / 2) * j))
+    return pts
+
+
+def c4_scene(rings=C4_RINGS):
+    """BK7 window (Sellmeier; k from a 1 m absorption length at 400 nm, so the
+    length scales with lambda) above a hexagonal array of hexagonal Winston
+    cones, each a mirror block "pgon:rot30 - hexV" with wall reflectance 0.9
+    and a thin hexagonal focal surface at its exit (HexWinstonCone.C mode 0).
+    The cone length follows from r1, r2: 2 dz = 40.1 mm."""
+    mgr = AOpticsManager("manager", "light guides")
+    world = AOpticalComponent("world", TGeoBBox("worldbox", 1 * m, 1 * m, 1 * m))
+    mgr.SetTopVolume(world)
+    hexv = AGeoWinstonConePoly("hexV", C4_R1, C4_R2, 6)
+    dz = hexv.GetDZ()
+    TGeoRotation("rot30", 30, 0, 0).RegisterYourself()
+    pgon = TGeoPgon("pgon", 0, 360, 6, 4)
+    pgon.DefineSection(0, -dz * 0.999, 0, C4_R2 * 1.1)
+    pgon.DefineSection(1, -dz * 0.5, 0, C4_R1 * 0.9)
+    pgon.DefineSection(2, -dz * 0.0, 0, C4_R1 * 0.99)
+    pgon.DefineSection(3, dz * 0.999, 0, C4_R1 * 1.001)
+    mirror = AMirror("coneMirror", TGeoCompositeShape("coneComp1", "pgon:rot30 - hexV"))
+    mirror.SetReflectance(C4_WALL_R)
+    pmt_shape = TGeoPgon("pgonPMT", 0, 360, 6, 2)
+    pmt_shape.DefineSection(0, -dz - 0.01 * mm, 0, C4_R2 * 1.01)
+    pmt_shape.DefineSection(1, -dz, 0, C4_R2 * 1.01)
+    pmt = AFocalSurface("pmt", pmt_shape)
+    rot30 = TGeoRotation("", 30, 0, 0)
+    pos = c4_positions(rings)
+    for k, (x, y) in enumerate(pos):
+        world.AddNode(mirror, k + 1, TGeoTranslation(x, y, 0.0))
+        world.AddNode(pmt, k + 1, TGeoCombiTrans(x, y, 0.0, rot30))
+    rmax = max(math.hypot(x, y) for x, y in pos) + 2 * C4_R1
+    bk7 = ASellmeierFormula(*C4_BK7)
+    kg = TGraph()
+    kg.SetPoint(0, 400 * nm, ARefractiveIndex.AbsorptionLengthToExtinctionCoefficient(C4_ABS_400, 400 * nm))
+    bk7.SetExtinctionCoefficient(kg)
+    lens = ALens("lens", TGeoTube("lensV", 0, rmax, 0.5 * cm))
+    lens.SetRefractiveIndex(bk7)
+    world.AddNode(lens, 1, TGeoTranslation(0, 0, dz + 2 * cm))
+    mgr.CloseGeometry()
+    mgr.c4_dz = dz
+    mgr.c4_rmax = rmax
+    return mgr
+
+
+def c4_raygen(dz, seed=0xC4, rings=C4_RINGS):
+    """f/1.2 beams converging on the cone entrance plane z = dz, field angle
+    0-6 deg (the 6 deg image at the array's edge), lambda 300-700 nm."""
+    g = A.rb_raygen_desc()
+    g.kind = A.RB_RAYGEN_CONE
+    r_edge = rings * 2 * C4_R1 * 1.002
+    F = r_edge / math.tan(math.radians(C4_FIELD_DEG))
+    g.distance = F
+    g.size = F / C4_FNUMBER
+    g.theta_min = 0.0
+    g.theta_max = math.radians(C4_FIELD_DEG)
+    g.lambda_min = 300 * nm
+    g.lambda_max = 700 * nm
+    g.seed = seed
+    g.z0 = dz
+    g.start = 5 * cm
     return g

@@ -759,6 +759,147 @@ template <typename T> __device__ V3<T> asph_normal(const double* p, V3<T> q, V3<
   return n;
 }
 
+// ------------------------------------------------------------ Winston cone (polygonal)
+// AGeoWinstonConePoly (src/AGeoWinstonConePoly.cxx) with AGeoWinstonCone2D's
+// parabola crossing (src/AGeoWinstonCone2D.cxx:325-428). params: r1 r2 N theta
+// dz f cos(theta) sin(theta) tan(pi/N) {cos, sin}(2 pi i / N) * N.
+// Trigonometry-free on the device: tan(atan2(pz, px) - theta) is formed from
+// the direction components, and the sector tests become projections.
+template <typename T> __device__ __forceinline__ bool wins_r(const double* p, T z, T& r) {
+  const T dz = T(p[4]), f = T(p[5]), ct = T(p[6]), st = T(p[7]);
+  if (fabs(z) > dz + T(1e-10)) return false;
+  T t = z + dz;
+  T a0 = t * t * st * st - T(4) * f * (t * ct + f);
+  T a1 = T(2) * t * st * ct + T(4) * f * st;
+  T a2 = ct * ct;
+  r = (-a1 + sqrt(a1 * a1 - T(4) * a0 * a2)) / (T(2) * a2) - T(p[1]);
+  return true;
+}
+template <typename T> __device__ __forceinline__ bool wins_drdz(const double* p, T z, T& g) {
+  const T dz = T(p[4]), f = T(p[5]), ct = T(p[6]), st = T(p[7]);
+  if (fabs(z) > dz + T(1e-10)) return false;
+  T t = z + dz;
+  T a0 = t * t * st * st - T(4) * f * (t * ct + f);
+  T a1 = T(2) * t * st * ct + T(4) * f * st;
+  T a2 = ct * ct;
+  T da0 = T(2) * t * st * st - T(4) * f * ct, da1 = T(2) * st * ct;
+  g = (-da1 + (a1 * da1 - T(2) * da0 * a2) / sqrt(a1 * a1 - T(4) * a0 * a2)) / (T(2) * a2);
+  return true;
+}
+// InsidePolygon: the projection on the side normal of the point's own sector
+// (the largest projection) must not exceed the apothem r
+template <typename T> __device__ __forceinline__ bool wins_inside_polygon(const double* p, T x, T y, T r) {
+  const int n = int(p[2]);
+  T m = T(-kBig);
+  for (int i = 0; i < n; i++) {
+    T v = x * T(p[9 + 2 * i]) + y * T(p[10 + 2 * i]);
+    m = v > m ? v : m;
+  }
+  return !(m > r);
+}
+// side i's parabola; half_plane: accept |azimuth| <= pi/2 (DistFromInside),
+// else |azimuth| <= pi/N (DistFromOutside)
+template <typename T>
+__device__ T wins_parabola(const double* p, int i, V3<T> P, V3<T> D, bool half_plane) {
+  const T r1 = T(p[0]), r2 = T(p[1]), dz = T(p[4]), f = T(p[5]), ct = T(p[6]), st = T(p[7]);
+  const T cph = T(p[9 + 2 * i]), sph = T(p[10 + 2 * i]);
+  T x = cph * P.x + sph * P.y, y = -sph * P.x + cph * P.y, z = P.z;
+  T px = cph * D.x + sph * D.y, py = -sph * D.x + cph * D.y, pz = D.z;
+  if (px == T(0) && pz == T(0)) return T(kBig);
+  T X = ct * (x + r2) + (z + dz) * st;
+  T Z = -st * (x + r2) + (z + dz) * ct + f;
+  T num = pz * ct - px * st, den = px * ct + pz * st;  // tanA = num / den
+  T Xc[2];
+  if (T(2) * dz * fabs(den) < T(kTol) * fabs(num)) {  // direction almost along the parabola axis
+    Xc[0] = Xc[1] = X;
+  } else {
+    T tanA = num / den;
+    T tmp = tanA * tanA - (X * tanA - Z) / f;
+    if (tmp < T(0)) return T(kBig);
+    T sq = sqrt(tmp);
+    Xc[0] = T(2) * f * (tanA + sq);
+    Xc[1] = T(2) * f * (tanA - sq);
+  }
+  const T tn = T(p[8]);
+  T best = T(kBig);
+  for (int k = 0; k < 2; k++) {
+    T Zc = Xc[k] * Xc[k] / T(4) / f;
+    T xc = ct * Xc[k] - st * (Zc - f) - r2;
+    T zc = st * Xc[k] + ct * (Zc - f) - dz;
+    T yc;
+    if (fabs(px) <= fabs(pz) && fabs(py) <= fabs(pz)) yc = y + (zc - z) * py / pz;
+    else if (fabs(py) <= fabs(px) && fabs(pz) <= fabs(px)) yc = y + (xc - x) * py / px;
+    else yc = y + (fabs(px) < T(1e-5) ? (zc - z) * py / pz : (xc - x) * py / px);
+    T ex = xc - x, ey = yc - y, ez = zc - z;
+    if (xc < r2 || r1 < xc || zc < -dz || dz < zc || ex * px + ez * pz < T(0)) continue;
+    bool in_sector = half_plane ? xc >= T(0) : (xc > T(0) && fabs(yc) <= xc * tn);
+    if (!in_sector) continue;
+    T dd = sqrt(ex * ex + ey * ey + ez * ez);
+    best = dd < best ? dd : best;
+  }
+  return best;
+}
+template <typename T> __device__ bool wins_contains(const double* p, V3<T> q) {
+  if (fabs(q.z) > T(p[4])) return false;
+  T r;
+  if (!wins_r<T>(p, q.z, r)) return false;
+  return wins_inside_polygon<T>(p, q.x, q.y, r);
+}
+template <typename T> __device__ T wins_dist_in(const double* p, V3<T> q, V3<T> d) {
+  const T dz = T(p[4]);
+  T best = T(kBig);
+  if (d.z < T(0)) best = (-q.z - dz) / d.z;
+  else if (d.z > T(0)) best = (dz - q.z) / d.z;
+  const int n = int(p[2]);
+  for (int i = 0; i < n; i++) {
+    T s = wins_parabola<T>(p, i, q, d, true);
+    best = s < best ? s : best;
+  }
+  return best;
+}
+template <typename T> __device__ T wins_dist_out(const double* p, V3<T> q, V3<T> d) {
+  const T dz = T(p[4]);
+  if (q.z <= -dz) {
+    if (d.z <= T(0)) return T(kBig);
+    T s = -(dz + q.z) / d.z;
+    if (wins_inside_polygon<T>(p, q.x + s * d.x, q.y + s * d.y, T(p[1]))) return s;
+  } else if (q.z >= dz) {
+    if (d.z >= T(0)) return T(kBig);
+    T s = (dz - q.z) / d.z;
+    if (wins_inside_polygon<T>(p, q.x + s * d.x, q.y + s * d.y, T(p[0]))) return s;
+  }
+  T best = T(kBig);
+  const int n = int(p[2]);
+  for (int i = 0; i < n; i++) {
+    T s = wins_parabola<T>(p, i, q, d, false);
+    best = s < best ? s : best;
+  }
+  return best;
+}
+template <typename T> __device__ V3<T> wins_normal(const double* p, V3<T> q, V3<T> d) {
+  const int n = int(p[2]);
+  T saf0 = fabs(fabs(T(p[4])) - fabs(q.z));
+  int k = 0;
+  T m = T(-kBig);
+  for (int i = 0; i < n; i++) {
+    T v = q.x * T(p[9 + 2 * i]) + q.y * T(p[10 + 2 * i]);
+    if (v > m) { m = v; k = i; }
+  }
+  T r, saf1 = T(kBig);
+  if (wins_r<T>(p, q.z, r)) saf1 = fabs(r - m);
+  V3<T> nn;
+  if (saf0 <= saf1) {
+    nn = mk<T>(T(0), T(0), T(1));
+  } else {
+    T g = T(0);
+    wins_drdz<T>(p, q.z, g);
+    nn = mk<T>(T(p[9 + 2 * k]), T(p[10 + 2 * k]), -g);
+    nn = scale(nn, T(1) / sqrt(dot(nn, nn)));
+  }
+  if (dot(nn, d) < T(0)) nn = scale(nn, T(-1));
+  return nn;
+}
+
 // ------------------------------------------------------------ dispatch
 template <typename T> __device__ __noinline__ bool prim_contains(int type, const double* p, V3<T> q) {
   switch (type) {
@@ -768,6 +909,7 @@ template <typename T> __device__ __noinline__ bool prim_contains(int type, const
     case RB_SHAPE_PARABOLOID: return para_contains<T>(p, q);
     case RB_SHAPE_PGON: return pgon_contains<T>(p, q);
     case RB_SHAPE_ASPHERIC: return asph_contains<T>(p, q);
+    case RB_SHAPE_WINSTON: return wins_contains<T>(p, q);
   }
   return false;
 }
@@ -779,6 +921,7 @@ template <typename T> __device__ __noinline__ T prim_dist_out(int type, const do
     case RB_SHAPE_PARABOLOID: return para_dist_out<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_dist_out<T>(p, q, d);
     case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
+    case RB_SHAPE_WINSTON: return wins_dist_out<T>(p, q, d);
   }
   return T(kBig);
 }
@@ -790,6 +933,7 @@ template <typename T> __device__ __noinline__ T prim_dist_in(int type, const dou
     case RB_SHAPE_PARABOLOID: return para_dist_in<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_dist_in<T>(p, q, d);
     case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
+    case RB_SHAPE_WINSTON: return wins_dist_in<T>(p, q, d);
   }
   return T(0);
 }
@@ -801,6 +945,7 @@ template <typename T> __device__ __noinline__ V3<T> prim_normal(int type, const 
     case RB_SHAPE_PARABOLOID: return para_normal<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_normal<T>(p, q, d);
     case RB_SHAPE_ASPHERIC: return asph_normal<T>(p, q, d);
+    case RB_SHAPE_WINSTON: return wins_normal<T>(p, q, d);
   }
   return mk<T>(T(0), T(0), T(1));
 }

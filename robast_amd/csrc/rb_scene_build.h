@@ -167,6 +167,12 @@ int validate(std::string& err, const rb_scene_desc* d) {
           return fail_msg(err, -1, "scene: aspheric disk parameter count");
         break;
       }
+      case RB_SHAPE_WINSTON: {
+        const double* p = d->params + s.param_offset;
+        if (s.n_params < 9 || int(p[2]) < 3 || s.n_params != 9 + 2 * int(p[2]))
+          return fail_msg(err, -1, "scene: Winston cone parameter count");
+        break;
+      }
       case RB_SHAPE_COMPOSITE:
         if (s.n_params != 27) return fail_msg(err, -1, "scene: composite needs 27 parameters");
         if (csg_depth(d, i, 0) > kMaxCsgDepth)

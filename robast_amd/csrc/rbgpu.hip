@@ -98,39 +98,56 @@ __global__ void __launch_bounds__(256) k_trace(DScene S, RayIO io, int64_t n, Tr
 __global__ void __launch_bounds__(256) k_generate(rb_raygen_desc g, RayIO io, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double th, lam, px, py;
-  if (g.kind == RB_RAYGEN_GRID) {
-    int ns = g.n_side;
-    double delta = ns == 1 ? g.size / 2 : g.size / (ns - 1);
-    int64_t a = i / ns, b = i % ns;
-    px = a * delta - g.size / 2;
-    py = b * delta - g.size / 2;
-    th = g.theta_min;
-    lam = g.lambda_min;
-  } else {
+  double ox, oy, oz, dx, dy, dz, lam;
+  if (g.kind == RB_RAYGEN_CONE) {
     double u0 = uniform01(g.seed, uint64_t(i), 0u, 1u), u1 = uniform01(g.seed, uint64_t(i), 1u, 1u);
     double u2 = uniform01(g.seed, uint64_t(i), 2u, 1u), u3 = uniform01(g.seed, uint64_t(i), 3u, 1u);
-    th = g.theta_min + (g.theta_max - g.theta_min) * u2;
-    lam = g.lambda_min + (g.lambda_max - g.lambda_min) * u3;
-    double rr = 0.5 * g.size * sqrt(u0), ph = 2 * kPi * u1;
-    px = rr * cos(ph);
-    py = rr * sin(ph);
-  }
-  double st = sin(th), ct = cos(th), t2 = kPi - th;
-  double cx = -g.distance * st, cz = g.distance * ct;
-  if (g.kind == RB_RAYGEN_GRID) {
-    io.x[i] = px + cx;
-    io.y[i] = py;
-    io.z[i] = cz;
+    double u4 = uniform01(g.seed, uint64_t(i), 4u, 1u);
+    double rho = 0.5 * g.size * sqrt(u0), al = 2 * kPi * u1;
+    double th = g.theta_min + (g.theta_max - g.theta_min) * u2, ph = 2 * kPi * u3;
+    lam = g.lambda_min + (g.lambda_max - g.lambda_min) * u4;
+    double ri = g.distance * tan(th);
+    double ix = ri * cos(ph), iy = ri * sin(ph);
+    double vx = ix - rho * cos(al), vy = iy - rho * sin(al), vz = -g.distance;
+    double m = sqrt(vx * vx + vy * vy + vz * vz);
+    dx = vx / m; dy = vy / m; dz = vz / m;
+    double s = g.start / -dz;
+    ox = ix - s * dx; oy = iy - s * dy; oz = g.z0 + g.start;
   } else {
-    io.x[i] = cx + px * ct;
-    io.y[i] = py;
-    io.z[i] = cz + px * st;
+    double th, px, py;
+    if (g.kind == RB_RAYGEN_GRID) {
+      int ns = g.n_side;
+      double delta = ns == 1 ? g.size / 2 : g.size / (ns - 1);
+      int64_t a = i / ns, b = i % ns;
+      px = a * delta - g.size / 2;
+      py = b * delta - g.size / 2;
+      th = g.theta_min;
+      lam = g.lambda_min;
+    } else {
+      double u0 = uniform01(g.seed, uint64_t(i), 0u, 1u), u1 = uniform01(g.seed, uint64_t(i), 1u, 1u);
+      double u2 = uniform01(g.seed, uint64_t(i), 2u, 1u), u3 = uniform01(g.seed, uint64_t(i), 3u, 1u);
+      th = g.theta_min + (g.theta_max - g.theta_min) * u2;
+      lam = g.lambda_min + (g.lambda_max - g.lambda_min) * u3;
+      double rr = 0.5 * g.size * sqrt(u0), ph = 2 * kPi * u1;
+      px = rr * cos(ph);
+      py = rr * sin(ph);
+    }
+    double st = sin(th), ct = cos(th), t2 = kPi - th;
+    double cx = -g.distance * st, cz = g.distance * ct;
+    if (g.kind == RB_RAYGEN_GRID) {
+      ox = px + cx; oy = py; oz = cz;
+    } else {
+      ox = cx + px * ct; oy = py; oz = cz + px * st;
+    }
+    dx = sin(t2); dy = 0; dz = cos(t2);
   }
+  io.x[i] = ox;
+  io.y[i] = oy;
+  io.z[i] = oz;
   io.t[i] = 0;
-  io.dx[i] = sin(t2);
-  io.dy[i] = 0;
-  io.dz[i] = cos(t2);
+  io.dx[i] = dx;
+  io.dy[i] = dy;
+  io.dz[i] = dz;
   const_cast<double*>(io.lambda)[i] = lam;
   io.status[i] = RB_RUN;
   io.npoints[i] = 1;
@@ -310,7 +327,8 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
 int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* g, rb_rays* rays, int64_t n) {
   if (!ctx) return -1;
   if (!g || !rays || n < 0) return fail(ctx, -1, "rbgpu_generate_rays: null argument");
-  if (g->kind != RB_RAYGEN_GRID && g->kind != RB_RAYGEN_DISK) return fail(ctx, -1, "rbgpu_generate_rays: bad kind");
+  if (g->kind != RB_RAYGEN_GRID && g->kind != RB_RAYGEN_DISK && g->kind != RB_RAYGEN_CONE)
+    return fail(ctx, -1, "rbgpu_generate_rays: bad kind");
   if (g->kind == RB_RAYGEN_GRID && (g->n_side < 1 || int64_t(g->n_side) * g->n_side < n))
     return fail(ctx, -1, "rbgpu_generate_rays: grid smaller than n");
   if (n == 0) return 0;

@@ -404,6 +404,41 @@ class AGeoAsphericDisk(TGeoShape):
         return np.array([0, 0, 0.5 * (zmin + zmax)]), np.array([self.rmax, self.rmax, 0.5 * (zmax - zmin)])
 
 
+class AGeoWinstonConePoly(TGeoShape):
+    """Polygonal Winston cone (src/AGeoWinstonConePoly.cxx:356-386): r1, r2
+    are the entrance/exit apothems, n >= 3 sides; theta = asin(r2/r1),
+    half length dz = (r1 + r2) / tan(theta) / 2, parabola focal length
+    f = r2 (1 + sin theta). The wide end is at z = +dz."""
+    shape_type = A.RB_SHAPE_WINSTON
+
+    def __init__(self, *args):
+        name = ""
+        if args and isinstance(args[0], str):
+            name, args = args[0], args[1:]
+        super().__init__(name)
+        r1, r2, n = abs(float(args[0])), abs(float(args[1])), int(args[2])
+        self.r1, self.r2 = (r1, r2) if r1 > r2 else (r2, r1)
+        self.n = n if n >= 3 else 3
+        self.theta = math.asin(self.r2 / self.r1)
+        self.dz = (self.r1 + self.r2) / math.tan(self.theta) / 2.0
+        self.f = self.r2 * (1 + math.sin(self.theta))
+
+    def GetDZ(self):
+        return self.dz
+
+    def params(self):
+        p = [self.r1, self.r2, float(self.n), self.theta, self.dz, self.f,
+             math.cos(self.theta), math.sin(self.theta), math.tan(math.pi / self.n)]
+        for i in range(self.n):
+            a = i * 2 * math.pi / self.n
+            p += [math.cos(a), math.sin(a)]
+        return p
+
+    def local_bbox(self):
+        r = self.r1 / math.cos(math.pi / self.n)
+        return np.zeros(3), np.array([r, r, self.dz])
+
+
 class TGeoBoolNode:
     def __init__(self, op, left, right, lmat, rmat):
         self.op, self.left, self.right = op, left, right

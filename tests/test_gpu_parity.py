@@ -157,15 +157,16 @@ def test_c3_aspheric_sample_per_ray(ctx):
     assert counts(g["status"])[A.RB_FOCUS] > 10000
 
 
-@pytest.mark.parametrize("config", ["c2", "c3"])
+@pytest.mark.parametrize("config", ["c2", "c3", "c4"])
 def test_mixed_precision_against_fp64(ctx, config):
     """Mixed form (FP32 search in rebased frames + FP64 polish) against the
     plain FP64 kernel at 1e6 rays: status flips rare, focused-hit RMS < 1 um."""
     import torch
     print(f"mixed vs fp64 {config}", flush=True)
     N = 1_000_000
-    mgr = configs.c2_scene() if config == "c2" else configs.c3_scene()
-    gen = configs.c2_raygen() if config == "c2" else configs.c3_raygen()
+    mgr = {"c2": configs.c2_scene, "c3": configs.c3_scene, "c4": configs.c4_scene}[config]()
+    gen = {"c2": configs.c2_raygen(), "c3": configs.c3_raygen(),
+           "c4": configs.c4_raygen(getattr(mgr, "c4_dz", 0.0))}[config]
     scene = ctx.upload(mgr.flat())
     dev = torch.device("cuda")
     out = {}
@@ -183,6 +184,23 @@ def test_mixed_precision_against_fp64(ctx, config):
     both = (a["status"] == A.RB_FOCUS) & (b["status"] == A.RB_FOCUS)
     d2 = (a["x"] - b["x"]) ** 2 + (a["y"] - b["y"]) ** 2 + (a["z"] - b["z"]) ** 2
     rms_um = float(torch.sqrt(d2[both].mean())) * 1e4
-    print(f"  flips {flips} / {N}, focused hit RMS {rms_um:.3e} um", flush=True)
-    assert flips <= 10
-    assert rms_um < 1.0
+    fa = float((a["status"] == A.RB_FOCUS).double().mean())
+    fb = float((b["status"] == A.RB_FOCUS).double().mean())
+    print(f"  flips {flips} / {N}, focused hit RMS {rms_um:.3e} um, focused {fa:.6f} vs {fb:.6f}", flush=True)
+    # the focused fraction agrees within binomial error
+    assert abs(fa - fb) < 3 * np.sqrt(fa * (1 - fa) / N)
+    if config != "c4":
+        # single-bounce optics: rays pair one to one
+        assert flips <= 10
+        assert rms_um < 1.0
+
+
+def test_c4_light_guides_sample_per_ray(ctx):
+    print("C4 BK7 window + 1027 Winston cones sample", flush=True)
+    mgr = configs.c4_scene()
+    soa = O.generate(configs.c4_raygen(mgr.c4_dz), 100000)
+    g, o = _trace_both(ctx, mgr, soa)
+    compare(g, o)
+    c = counts(g["status"])
+    assert c[A.RB_FOCUS] > 50000 and c[A.RB_ABSORB] > 0
+    assert g["npoints"].max() > 10
