@@ -226,6 +226,32 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
 int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* gen, rb_rays* rays,
                         int64_t n);
 
+/* ---- adjoint of a focal-plane loss (differentiable alignment) ------------
+ * Supported optics: chains of up to 4 reflections on spherical facets, each
+ * the node shape `sphere(rmin = R) : T(0, 0, R) * cut` placed with rotation
+ * M = M0 Rx(tip) Ry(tilt) (a Davies-Cotton facet), ending on a focal surface
+ * whose entry face is the plane z = z_cam. The loss seen by each focused ray
+ * is weight * |h - centroid|^2 / 2 on its hit (h_x, h_y), so weight = 2 / N
+ * and centroid = mean hit give d(spot RMS^2)/dparams. */
+typedef struct {
+  const int32_t* node_param;   /* device [n_nodes][3]: tip, tilt, R parameter index or -1 */
+  const double* param_values;  /* device [n_params]: current tip, tilt (rad), R (cm), ... */
+  int32_t n_params;
+  int32_t z_param;             /* index of the focal-plane z offset parameter, or -1 */
+  int32_t focal_node;          /* node id whose entry plane carries z_param, or -1 */
+  int32_t pad;
+  double centroid_x, centroid_y;
+  double weight;
+} rb_adjoint_desc;
+
+/* Re-trace the n rays (left unchanged) and accumulate d(loss)/d(params)
+ * into grad (device, FP64, n_params; added to, not overwritten). Per-ray
+ * arithmetic is FP32, accumulation FP64. The FP64 loss itself comes from a
+ * forward rbgpu_trace. No counterpart in the reference: Optimize.C fits with
+ * Minuit and no gradient (tutorials/Optimize.C:188-205). */
+int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const rb_rays* rays, int64_t n,
+                  const rb_adjoint_desc* adj, double* grad);
+
 /* Milliseconds the last rbgpu_trace spent in its trace kernel(s), timed with
  * HIP events on the context's stream, and the number of kernel launches. */
 double rbgpu_last_trace_ms(const rb_ctx* ctx);

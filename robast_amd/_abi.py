@@ -92,6 +92,14 @@ class rb_raygen_desc(C.Structure):
     ]
 
 
+class rb_adjoint_desc(C.Structure):
+    _fields_ = [
+        ("node_param", C.c_void_p), ("param_values", C.c_void_p), ("n_params", C.c_int32),
+        ("z_param", C.c_int32), ("focal_node", C.c_int32), ("pad", C.c_int32),
+        ("centroid_x", C.c_double), ("centroid_y", C.c_double), ("weight", C.c_double),
+    ]
+
+
 def ptr(a):
     """Address of a contiguous numpy array (or None)."""
     if a is None:

@@ -18,7 +18,7 @@ _lib = None
 EXPORTS = (
     "rbgpu_ctx_create", "rbgpu_ctx_destroy", "rbgpu_last_error", "rbgpu_scene_upload",
     "rbgpu_scene_free", "rbgpu_trace", "rbgpu_generate_rays", "rbgpu_last_trace_ms",
-    "rbgpu_last_trace_launches", "rbgpu_abi_sizes",
+    "rbgpu_last_trace_launches", "rbgpu_abi_sizes", "rbgpu_adjoint",
 )
 
 
@@ -55,6 +55,9 @@ def load():
     lib.rbgpu_last_trace_ms.restype = C.c_double
     lib.rbgpu_last_trace_launches.argtypes = [C.c_void_p]
     lib.rbgpu_last_trace_launches.restype = C.c_int
+    lib.rbgpu_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays), C.c_int64,
+                                  C.POINTER(A.rb_adjoint_desc), C.c_void_p]
+    lib.rbgpu_adjoint.restype = C.c_int
     lib.rbgpu_abi_sizes.argtypes = [C.POINTER(C.c_int32)]
     lib.rbgpu_abi_sizes.restype = C.c_int
     _lib = lib
@@ -93,6 +96,10 @@ class Context:
     def generate(self, gen, rays, n):
         self._check(self.lib.rbgpu_generate_rays(self.h, C.byref(gen), C.byref(rays), C.c_int64(n)),
                     "rbgpu_generate_rays")
+
+    def adjoint(self, scene, rays, n, opts, adj, grad_ptr):
+        self._check(self.lib.rbgpu_adjoint(self.h, scene.h, C.byref(opts), C.byref(rays), C.c_int64(n), C.byref(adj),
+                                           C.c_void_p(grad_ptr)), "rbgpu_adjoint")
 
     def last_trace_ms(self):
         return self.lib.rbgpu_last_trace_ms(self.h)

@@ -37,6 +37,29 @@ struct Lane {
   uint32_t hit_leaf;
 };
 
+// Reflection recorders: trace_lane reports every specular reflection
+// (node, boundary point before the step back, incoming direction).
+struct NoRecord {
+  __device__ __forceinline__ void reflect(int, V3<double>, V3<double>) {}
+};
+struct ChainRecord {  // the adjoint's recomputed bounce chain
+  static constexpr int kMax = 4;
+  int n = 0;
+  bool overflow = false;
+  int node[kMax];
+  V3<double> P[kMax], din[kMax];
+  __device__ __forceinline__ void reflect(int nd, V3<double> p, V3<double> d) {
+    if (n < kMax) {
+      node[n] = nd;
+      P[n] = p;
+      din[n] = d;
+      n++;
+    } else {
+      overflow = true;
+    }
+  }
+};
+
 __device__ __noinline__ bool node_contains(const DScene& S, int n, V3<double> p) {
   const DNode& nd = S.nodes[n];
   return Csg<double, kMaxCsgDepth>::contains(S, nd.shape, m2l<double>(nd.rot, nd.tr, p));
@@ -345,8 +368,9 @@ __device__ __forceinline__ void set_direction(Lane& L, V3<double> v) {
 }
 
 // AOpticsManager::DoReflection (src/AOpticsManager.cxx:170-247)
+template <typename Rec>
 __device__ void do_reflection(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id,
-                              Lane& L, double n1, int next, V3<double> n) {
+                              Lane& L, double n1, int next, V3<double> n, Rec& rec) {
   double step = L.step;
   V3<double> d1 = L.d;
   double cos1 = dot(d1, n);
@@ -362,7 +386,10 @@ __device__ void do_reflection(const DScene& S, const TraceParams& P, const RayIO
     }
   }
   V3<double> d2 = mk(d1.x - 2 * n.x * cos1, d1.y - 2 * n.y * cos1, d1.z - 2 * n.z * cos1);
-  if (!absorbed) set_direction(L, d2);
+  if (!absorbed) {
+    set_direction(L, d2);
+    rec.reflect(next, L.p, d1);
+  }
   double t = L.t + step / (kC / n1);
   // TGeoNavigator::Step() of kEpsilon backwards moves kEpsilon + 1e-6, then relocates
   L.p = axpy(L.p, -(kEps + 1e-6), d1);
@@ -373,8 +400,9 @@ __device__ void do_reflection(const DScene& S, const TraceParams& P, const RayIO
 }
 
 // AOpticsManager::DoFresnel (src/AOpticsManager.cxx:52-167), no multilayer branch
+template <typename Rec>
 __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id, Lane& L,
-                           double n1, double n2, double k2, int next) {
+                           double n1, double n2, double k2, int next, Rec& rec) {
   double step = L.step;
   V3<double> n = find_normal(S, L);
   V3<double> d1 = L.d;
@@ -383,7 +411,7 @@ __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& i
   double sin2 = n1 * sin1 / n2;
   double cos2 = sqrt(1 - sin2 * sin2);
   if (sin2 > 1.) {
-    do_reflection(S, P, io, i, id, L, n1, next, n);
+    do_reflection(S, P, io, i, id, L, n1, next, n, rec);
     return;
   }
   if (P.fresnel) {
@@ -409,7 +437,7 @@ __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& i
     }
     double R = (Rs + Rp) / 2.;
     if (draw(P, id, L) < R) {
-      do_reflection(S, P, io, i, id, L, n1, next, n);
+      do_reflection(S, P, io, i, id, L, n1, next, n, rec);
       return;
     }
   }
@@ -425,8 +453,8 @@ __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& i
 }
 
 // AOpticsManager::TraceNonSequential(TObjArray*) for one ray (src/AOpticsManager.cxx:353-518)
-template <bool MIXED>
-__device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L) {
+template <bool MIXED, typename Rec>
+__device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
   const uint64_t id = P.id_base + uint64_t(i);
   L.on_b = false;
   L.cur = node_contains(S, 0, L.p) ? search_node(S, 0, L.p, -1, true) : -1;
@@ -458,10 +486,10 @@ __device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& i
     const bool c_free = tc == RB_KNULL || tc == RB_KOPT || tc == RB_KOTHER;
     if ((c_free || tc == RB_KLENS) && tn == RB_KMIRROR) {
       double n1 = tc == RB_KLENS ? lens_index(S, vol_of(S, current), lambda) : 1.;
-      do_reflection(S, P, io, i, id, L, n1, next, find_normal(S, L));
+      do_reflection(S, P, io, i, id, L, n1, next, find_normal(S, L), rec);
     } else if (c_free && tn == RB_KLENS) {
       const rb_volume_desc& v = vol_of(S, next);
-      do_fresnel(S, P, io, i, id, L, 1., lens_index(S, v, lambda), lens_k(S, v, lambda), next);
+      do_fresnel(S, P, io, i, id, L, 1., lens_index(S, v, lambda), lens_k(S, v, lambda), next, rec);
     } else if ((c_free || tc == RB_KLENS) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
       double n1 = tc == RB_KLENS ? lens_index(S, vol_of(S, current), lambda) : 1.;
       add_point(io, i, L, L.p, t1 + step / (kC / n1), next);
@@ -470,9 +498,9 @@ __device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& i
     } else if (tc == RB_KLENS && tn == RB_KLENS) {
       const rb_volume_desc& v = vol_of(S, next);
       do_fresnel(S, P, io, i, id, L, lens_index(S, vol_of(S, current), lambda), lens_index(S, v, lambda),
-                 lens_k(S, v, lambda), next);
+                 lens_k(S, v, lambda), next, rec);
     } else if (tc == RB_KLENS && (tn == RB_KNULL || tn == RB_KOPT || tn == RB_KOTHER)) {
-      do_fresnel(S, P, io, i, id, L, lens_index(S, vol_of(S, current), lambda), 1., 0., next);
+      do_fresnel(S, P, io, i, id, L, lens_index(S, vol_of(S, current), lambda), 1., 0., next, rec);
     }
 
     if (tn == RB_KNULL) {

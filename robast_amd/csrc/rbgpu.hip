@@ -12,6 +12,7 @@
 #include "../../include/rbgpu.h"
 #include "rb_scene_build.h"
 #include "rb_trace.h"
+#include "rb_adjoint.h"
 
 using namespace rbd;
 
@@ -82,7 +83,8 @@ __global__ void __launch_bounds__(256) k_trace(DScene S, RayIO io, int64_t n, Tr
   L.step = 0;
   L.hit_node = -1;
   L.hit_leaf = 0;
-  trace_lane<MIXED>(S, P, io, i, L);
+  NoRecord rec;
+  trace_lane<MIXED>(S, P, io, i, L, rec);
   io.x[i] = L.p.x;
   io.y[i] = L.p.y;
   io.z[i] = L.p.z;
@@ -93,6 +95,42 @@ __global__ void __launch_bounds__(256) k_trace(DScene S, RayIO io, int64_t n, Tr
   io.status[i] = L.status;
   io.npoints[i] = L.npoints;
   if (io.last_node) io.last_node[i] = L.last_node;
+}
+
+// Adjoint of the focal-plane loss: each lane recomputes its ray's bounce
+// chain (nothing is stored by the forward pass) and accumulates parameter
+// gradients in LDS (FP64), flushed with one global atomic per parameter.
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_adjoint(DScene S, RayIO io, int64_t n, TraceParams P, AdjParams A,
+                                                 double* grad) {
+  __shared__ double acc[kMaxAdjParams];
+  for (int k = threadIdx.x; k < A.n_params; k += blockDim.x) acc[k] = 0.0;
+  __syncthreads();
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && io.status[i] == RB_RUN) {
+    Lane L;
+    L.p = mk(io.x[i], io.y[i], io.z[i]);
+    L.d = mk(io.dx[i], io.dy[i], io.dz[i]);
+    L.t = io.t[i];
+    L.lambda = io.lambda[i];
+    L.status = RB_RUN;
+    L.npoints = io.npoints[i];
+    L.last_node = -1;
+    L.draw = 0;
+    L.step = 0;
+    L.hit_node = -1;
+    L.hit_leaf = 0;
+    const V3<double> origin = L.p;
+    ChainRecord rec;
+    RayIO ro = io;
+    ro.path_cap = 0;
+    trace_lane<MIXED>(S, P, ro, i, L, rec);
+    if (L.status == RB_FOCUS && rec.n > 0 && !rec.overflow && L.last_node == A.focal_node)
+      adjoint_ray(S, A, rec, origin, L.p, acc);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < A.n_params; k += blockDim.x)
+    if (acc[k] != 0.0) atomicAdd(&grad[k], acc[k]);
 }
 
 __global__ void __launch_bounds__(256) k_generate(rb_raygen_desc g, RayIO io, int64_t n) {
@@ -342,6 +380,55 @@ int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* g, rb_rays* rays, int
   hipLaunchKernelGGL(k_generate, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, *g, io, n);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const rb_rays* rays, int64_t n,
+                  const rb_adjoint_desc* adj, double* grad) {
+  if (!ctx) return -1;
+  if (!scene || !opts || !rays || !adj || !grad || n < 0) return fail(ctx, -1, "rbgpu_adjoint: null argument");
+  if (!opts->device_ptrs) return fail(ctx, -1, "rbgpu_adjoint: rays must be device pointers");
+  if (!adj->node_param || !adj->param_values || adj->n_params <= 0 || adj->n_params > kMaxAdjParams)
+    return fail(ctx, -1, "rbgpu_adjoint: parameter table missing or larger than " + std::to_string(kMaxAdjParams));
+  if (adj->z_param >= adj->n_params || adj->focal_node >= scene->s.n_nodes)
+    return fail(ctx, -1, "rbgpu_adjoint: z parameter or focal node out of range");
+  ctx->launches = 0;
+  ctx->last_ms = 0;
+  if (n == 0) return 0;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  TraceParams P;
+  P.seed = opts->seed;
+  P.id_base = opts->ray_id_base;
+  P.limit = opts->limit > 0 ? opts->limit : 100;
+  P.fresnel = opts->fresnel ? 1 : 0;
+  RayIO io{};
+  io.x = rays->x; io.y = rays->y; io.z = rays->z; io.t = rays->t;
+  io.dx = rays->dx; io.dy = rays->dy; io.dz = rays->dz; io.lambda = rays->lambda;
+  io.status = rays->status; io.npoints = rays->npoints; io.last_node = rays->last_node;
+  io.path_cap = 0;
+  AdjParams A;
+  A.node_param = adj->node_param;
+  A.values = adj->param_values;
+  A.n_params = adj->n_params;
+  A.z_param = adj->z_param;
+  A.focal_node = adj->focal_node;
+  A.cx = adj->centroid_x;
+  A.cy = adj->centroid_y;
+  A.weight = adj->weight;
+  const int block = 256;
+  const int64_t grid = (n + block - 1) / block;
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (opts->precision == RB_PRECISION_MIXED)
+    hipLaunchKernelGGL(k_adjoint<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, A, grad);
+  else
+    hipLaunchKernelGGL(k_adjoint<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, A, grad);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ms;
+  ctx->launches = 1;
   return 0;
 }
 
