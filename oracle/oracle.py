@@ -42,6 +42,9 @@ def lib():
         L.oracle_contains_batch.restype = C.c_int
         L.oracle_generate_rays.argtypes = [C.POINTER(A.rb_raygen_desc), C.POINTER(A.rb_rays), C.c_int64]
         L.oracle_generate_rays.restype = C.c_int
+        L.oracle_adjoint_dual.argtypes = [C.POINTER(A.rb_scene_desc), C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays),
+                                          C.c_int64, C.POINTER(A.rb_adjoint_desc), C.c_void_p, C.c_int]
+        L.oracle_adjoint_dual.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -76,6 +79,19 @@ def trace(flat, soa, opts, nthreads=1):
     if rc != 0:
         raise RuntimeError(f"oracle_trace failed ({rc})")
     return soa
+
+
+def adjoint_dual(flat, soa, opts, adj, n_params, nthreads=8):
+    """Dual-number loss gradient (forward mode) of the rays in `soa` (not
+    modified); adj is an rb_adjoint_desc. Returns (grad[n_params], rays used)."""
+    n = len(soa["x"])
+    grad = np.zeros(n_params)
+    r = rays_struct(soa)
+    used = lib().oracle_adjoint_dual(C.byref(flat.desc), C.byref(opts), C.byref(r), n, C.byref(adj),
+                                     grad.ctypes.data, nthreads)
+    if used < 0:
+        raise RuntimeError("oracle_adjoint_dual failed")
+    return grad, int(used)
 
 
 def generate(gen, n):

@@ -1620,6 +1620,201 @@ static void trace_ray(const Scene& sc, const Opts& o, Ray& ray, Rng& rng) {
   }
 }
 
+// ---------------------------------------------------------------- dual-number adjoint reference
+// Forward-mode restatement of the focal-plane loss gradient for chains of
+// spherical-facet reflections, used to check the device adjoint. Each focused
+// ray is traced as above with its path recorded; the chain is then recomputed
+// with dual numbers whose tangents are the (tip, tilt, R) of each facet the
+// ray reflects on and the focal-plane z. Nothing here is hand-differentiated:
+// the facet rotation is rebuilt as M0 Rx(tip) Ry(tilt) and every step follows
+// the forward formulas of reflect() and the navigator's step back.
+constexpr int kDualN = 13;  // 3 per reflection (at most 4) + focal-plane z
+struct Dual {
+  double v;
+  double d[kDualN];
+};
+static inline Dual dc(double v) {
+  Dual r;
+  r.v = v;
+  for (int k = 0; k < kDualN; k++) r.d[k] = 0;
+  return r;
+}
+static inline Dual dvar(double v, int k) {
+  Dual r = dc(v);
+  r.d[k] = 1;
+  return r;
+}
+static inline Dual operator+(const Dual& a, const Dual& b) {
+  Dual r;
+  r.v = a.v + b.v;
+  for (int k = 0; k < kDualN; k++) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+static inline Dual operator-(const Dual& a, const Dual& b) {
+  Dual r;
+  r.v = a.v - b.v;
+  for (int k = 0; k < kDualN; k++) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+static inline Dual operator*(const Dual& a, const Dual& b) {
+  Dual r;
+  r.v = a.v * b.v;
+  for (int k = 0; k < kDualN; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+static inline Dual operator/(const Dual& a, const Dual& b) {
+  Dual r;
+  r.v = a.v / b.v;
+  for (int k = 0; k < kDualN; k++) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+  return r;
+}
+static inline Dual dsqrt(const Dual& a) {
+  Dual r;
+  r.v = std::sqrt(a.v);
+  for (int k = 0; k < kDualN; k++) r.d[k] = a.d[k] / (2 * r.v);
+  return r;
+}
+static inline Dual dsin(const Dual& a) {
+  Dual r;
+  r.v = std::sin(a.v);
+  const double c = std::cos(a.v);
+  for (int k = 0; k < kDualN; k++) r.d[k] = c * a.d[k];
+  return r;
+}
+static inline Dual dcos(const Dual& a) {
+  Dual r;
+  r.v = std::cos(a.v);
+  const double s = -std::sin(a.v);
+  for (int k = 0; k < kDualN; k++) r.d[k] = s * a.d[k];
+  return r;
+}
+struct DV3 {
+  Dual x, y, z;
+};
+static inline DV3 dv(V3 a) { return {dc(a.x), dc(a.y), dc(a.z)}; }
+static inline DV3 operator+(const DV3& a, const DV3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline DV3 operator-(const DV3& a, const DV3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline DV3 operator*(const DV3& a, const Dual& s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline Dual ddot(const DV3& a, const DV3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 val(const DV3& a) { return v3(a.x.v, a.y.v, a.z.v); }
+
+// Loss gradient of one focused ray; returns false when the chain holds a
+// reflection on a node without parameters (the device adjoint skips those too).
+static bool dual_ray(const rb_scene_desc* d, const rb_adjoint_desc* A, V3 origin, V3 d0, const double* path,
+                     const int32_t* pnode, int K, V3 hit, double* grad) {
+  for (int j = 0; j < K; j++) {
+    const int32_t* np = A->node_param + 3 * pnode[j];
+    if (np[0] < 0 || np[1] < 0 || np[2] < 0) return false;
+  }
+  const double eps = kEpsilon + 1e-6;
+  DV3 o = dv(origin), din = dv(d0);
+  for (int j = 0; j < K; j++) {
+    const int32_t* np = A->node_param + 3 * pnode[j];
+    const rb_matrix& g = d->nodes[pnode[j]].global;
+    const double tip0 = A->param_values[np[0]], tilt0 = A->param_values[np[1]];
+    // M0 = M Ry(tilt0)^T Rx(tip0)^T: the placement without the tip/tilt
+    double rx[9] = {1, 0, 0, 0, std::cos(tip0), -std::sin(tip0), 0, std::sin(tip0), std::cos(tip0)};
+    double ry[9] = {std::cos(tilt0), 0, std::sin(tilt0), 0, 1, 0, -std::sin(tilt0), 0, std::cos(tilt0)};
+    double e[9], m0[9];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        double s = 0;
+        for (int k = 0; k < 3; k++) s += rx[c * 3 + k] * ry[k * 3 + r];  // (Rx Ry)^T
+        e[r * 3 + c] = s;
+      }
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        double s = 0;
+        for (int k = 0; k < 3; k++) s += g.rot[r * 3 + k] * e[k * 3 + c];
+        m0[r * 3 + c] = s;
+      }
+    const Dual tip = dvar(tip0, 3 * j), tilt = dvar(tilt0, 3 * j + 1), R = dvar(A->param_values[np[2]], 3 * j + 2);
+    // local axis of the sphere centre: Rx(tip) Ry(tilt) e_z = (sin tilt, -sin tip cos tilt, cos tip cos tilt)
+    const Dual ct = dcos(tilt);
+    const DV3 a = {dsin(tilt), dc(0) - dsin(tip) * ct, dcos(tip) * ct};
+    DV3 C;
+    C.x = dc(g.tr[0]) + (dc(m0[0]) * a.x + dc(m0[1]) * a.y + dc(m0[2]) * a.z) * R;
+    C.y = dc(g.tr[1]) + (dc(m0[3]) * a.x + dc(m0[4]) * a.y + dc(m0[5]) * a.z) * R;
+    C.z = dc(g.tr[2]) + (dc(m0[6]) * a.x + dc(m0[7]) * a.y + dc(m0[8]) * a.z) * R;
+    // |o + t din - C| = R; the root nearest the recorded crossing
+    const DV3 oc = o - C;
+    const Dual b = ddot(oc, din), c = ddot(oc, oc) - R * R;
+    const Dual q = dsqrt(b * b - c);
+    const Dual t1 = dc(0) - b - q, t2 = dc(0) - b + q;
+    const V3 rec = axpy(v3(path[4 * j], path[4 * j + 1], path[4 * j + 2]), eps, val(din));
+    const V3 p1 = axpy(val(o), t1.v, val(din)), p2 = axpy(val(o), t2.v, val(din));
+    const V3 e1 = sub(p1, rec), e2 = sub(p2, rec);
+    const Dual t = dot(e1, e1) <= dot(e2, e2) ? t1 : t2;
+    const DV3 P = o + din * t;
+    const DV3 n = (P - C) * (dc(1) / R);
+    const Dual k = ddot(din, n);
+    DV3 dout = din - n * (dc(2) * k);
+    dout = dout * (dc(1) / dsqrt(ddot(dout, dout)));
+    o = P - din * dc(eps);
+    din = dout;
+  }
+  // focal plane z = hit.z, moved by the z parameter
+  const Dual zc = A->z_param >= 0 ? dvar(hit.z, kDualN - 1) : dc(hit.z);
+  const Dual s = (zc - o.z) / din.z;
+  const DV3 h = o + din * s;
+  const double gx = A->weight * (h.x.v - A->centroid_x), gy = A->weight * (h.y.v - A->centroid_y);
+  auto put = [&](int param, int k) {
+    if (param >= 0) grad[param] += gx * h.x.d[k] + gy * h.y.d[k];
+  };
+  for (int j = 0; j < K; j++) {
+    const int32_t* np = A->node_param + 3 * pnode[j];
+    put(np[0], 3 * j);
+    put(np[1], 3 * j + 1);
+    put(np[2], 3 * j + 2);
+  }
+  put(A->z_param, kDualN - 1);
+  return true;
+}
+
+static void dual_range(const Scene& sc, const Opts& o, const rb_trace_opts* opts, const rb_rays* r,
+                       const rb_adjoint_desc* A, int64_t b, int64_t e, double* grad, int64_t* used) {
+  constexpr int kCap = 8;
+  double path[4 * kCap];
+  int32_t pnode[kCap];
+  for (int64_t i = b; i < e; i++) {
+    if (r->status[i] != RB_RUN) continue;
+    Ray ray;
+    ray.x = v3(r->x[i], r->y[i], r->z[i]);
+    ray.t = r->t[i];
+    ray.lambda = r->lambda[i];
+    ray.status = RB_RUN;
+    ray.npoints = 1;
+    ray.last_node = -1;
+    ray.d = v3(r->dx[i], r->dy[i], r->dz[i]);
+    ray.cap = kCap;
+    ray.path = path;
+    ray.pnode = pnode;
+    Rng rng{opts->seed, opts->ray_id_base + uint64_t(i), 0u, 0u};
+    trace_ray(sc, o, ray, rng);
+    if (ray.status != RB_FOCUS || ray.last_node != A->focal_node || ray.npoints - 1 > kCap) continue;
+    // the points between the origin and the focal hit: reflections on
+    // mirrors, or boundary crossings that leave the line unchanged
+    double rpath[4 * kCap];
+    int32_t rnode[kCap];
+    int K = 0;
+    bool ok = true;
+    for (int j = 0; j + 2 < ray.npoints; j++) {
+      const int kind = kind_of(sc, pnode[j]);
+      if (kind == RB_KMIRROR) {
+        if (K == 4) ok = false;
+        if (!ok) break;
+        for (int c = 0; c < 4; c++) rpath[4 * K + c] = path[4 * j + c];
+        rnode[K++] = pnode[j];
+      } else if (kind == RB_KLENS) {
+        ok = false;  // refraction: outside the model
+      }
+    }
+    if (ok && dual_ray(sc.d, A, v3(r->x[i], r->y[i], r->z[i]), v3(r->dx[i], r->dy[i], r->dz[i]), rpath, rnode, K,
+                       ray.x, grad))
+      (*used)++;
+  }
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1669,6 +1864,32 @@ int oracle_trace(const rb_scene_desc* desc, const rb_trace_opts* opts, rb_rays* 
   }
   for (auto& t : th) t.join();
   return 0;
+}
+
+// Dual-number reference of rbgpu_adjoint: adds d(loss)/d(params) of the rays
+// (not modified) into grad[n_params]; returns the number of rays that contributed.
+int64_t oracle_adjoint_dual(const rb_scene_desc* desc, const rb_trace_opts* opts, const rb_rays* rays, int64_t n,
+                            const rb_adjoint_desc* adj, double* grad, int nthreads) {
+  if (!desc || !opts || !rays || !adj || !grad || n < 0) return -1;
+  Scene sc{desc, {}};
+  build_boxes(sc);
+  Opts o{opts->limit > 0 ? opts->limit : 100, opts->fresnel != 0};
+  nthreads = std::max(1, std::min<int>(nthreads, int(n / 64) + 1));
+  std::vector<std::vector<double>> g(nthreads, std::vector<double>(adj->n_params, 0.0));
+  std::vector<int64_t> used(nthreads, 0);
+  std::vector<std::thread> th;
+  int64_t chunk = n / nthreads;
+  for (int k = 0; k < nthreads; k++) {
+    int64_t b = chunk * k, e = (k == nthreads - 1) ? n : chunk * (k + 1);
+    th.emplace_back(dual_range, std::cref(sc), std::cref(o), opts, rays, adj, b, e, g[k].data(), &used[k]);
+  }
+  for (auto& t : th) t.join();
+  int64_t total = 0;
+  for (int k = 0; k < nthreads; k++) {
+    for (int p = 0; p < adj->n_params; p++) grad[p] += g[k][p];
+    total += used[k];
+  }
+  return total;
 }
 
 // Shape-level probes for tests (local frame of `shape`)

@@ -39,17 +39,18 @@ __device__ __forceinline__ V3f mulf(V3f a, float s) { return {a.x * s, a.y * s, 
 // Accumulates d(loss)/d(params) of one focused ray into `acc` (LDS, FP64).
 // Returns false if the chain holds a reflection the model does not cover.
 __device__ bool adjoint_ray(const DScene& S, const AdjParams& A, const ChainRecord& rec, V3<double> origin,
-                            V3<double> hit, double* acc) {
+                            V3<double> d0, V3<double> hit, double* acc) {
   const int K = rec.n;
   for (int j = 0; j < K; j++) {
     const int32_t* np = A.node_param + 3 * rec.node[j];
     if (np[0] < 0 || np[1] < 0 || np[2] < 0) return false;
   }
-  // focal plane
+  // focal plane: reached from the last step-back point along the last
+  // reflection's outgoing direction (recomputed in FP64), or straight from
+  // the origin for a ray that reflected nowhere
   const double eps = kEps + 1e-6;
-  V3<double> dlast = rec.din[K - 1];
-  {
-    // outgoing direction of the last reflection, recomputed in FP64
+  V3<double> dlast = K > 0 ? rec.din[K - 1] : d0;
+  if (K > 0) {
     const DNode& nd = S.nodes[rec.node[K - 1]];
     const double R = A.values[A.node_param[3 * rec.node[K - 1] + 2]];
     V3<double> C = mk(nd.tr[0] + nd.rot[2] * R, nd.tr[1] + nd.rot[5] * R, nd.tr[2] + nd.rot[8] * R);
@@ -60,7 +61,7 @@ __device__ bool adjoint_ray(const DScene& S, const AdjParams& A, const ChainReco
     double mag = sqrt(dot(dlast, dlast));
     dlast = scale(dlast, 1.0 / mag);
   }
-  const V3<double> Pp = axpy(rec.P[K - 1], -eps, rec.din[K - 1]);
+  const V3<double> Pp = K > 0 ? axpy(rec.P[K - 1], -eps, rec.din[K - 1]) : origin;
   const float s = float((hit.z - Pp.z) / dlast.z);
   V3f dout = f3(dlast);
   V3f hbar = {float(A.weight * (hit.x - A.cx)), float(A.weight * (hit.y - A.cy)), 0.0f};
@@ -95,11 +96,18 @@ __device__ bool adjoint_ray(const DScene& S, const AdjParams& A, const ChainReco
     V3f nbar = mulf(doutbar, -2.0f * k);
     dinbar = addf(dinbar, mulf(n, kbar));
     nbar = addf(nbar, mulf(din, kbar));
-    // n = (P - C) / R
+    // facet frame: C = T + R ez, w = P - T = m + R ez (the hit relative to the
+    // facet vertex, formed in FP64 so that ez + n = w / R keeps its precision)
+    const V3f ex = {float(nd.rot[0]), float(nd.rot[3]), float(nd.rot[6])};
+    const V3f ey = {float(nd.rot[1]), float(nd.rot[4]), float(nd.rot[7])};
+    const V3f ez = {float(nd.rot[2]), float(nd.rot[5]), float(nd.rot[8])};
+    const V3f w = f3(mk(rec.P[j].x - nd.tr[0], rec.P[j].y - nd.tr[1], rec.P[j].z - nd.tr[2]));
+    // n = (P - C) / R: dn/dP = 1/R, dn/dC = -1/R, dn/dR (C = T + R ez) = -w / R^2
     Pbar = addf(Pbar, mulf(nbar, 1.0f / Rf));
     V3f Cbar = mulf(nbar, -1.0f / Rf);
-    float Rbar = -dotf(nbar, n) / Rf;
-    // P = o + t din with t from |P - C| = R (implicit): m.din = R k
+    float Rbar = -dotf(nbar, w) / (Rf * Rf);
+    // P = o + t din with t from |P - C| = R (implicit), m.din = R k:
+    // dt = (m.dC - m.do - t m.ddin) / md, and through R: (w.ez) / md
     const float tbar = dotf(Pbar, din);
     V3f obar = Pbar;
     dinbar = addf(dinbar, mulf(Pbar, t));
@@ -107,14 +115,11 @@ __device__ bool adjoint_ray(const DScene& S, const AdjParams& A, const ChainReco
     obar = addf(obar, mulf(m, -tbar / md));
     dinbar = addf(dinbar, mulf(m, -tbar * t / md));
     Cbar = addf(Cbar, mulf(m, tbar / md));
-    Rbar += tbar * Rf / md;
-    // facet parameters: C = T + M (0, 0, R)
-    const V3f ex = {float(nd.rot[0]), float(nd.rot[3]), float(nd.rot[6])};
-    const V3f ey = {float(nd.rot[1]), float(nd.rot[4]), float(nd.rot[7])};
-    const V3f ez = {float(nd.rot[2]), float(nd.rot[5]), float(nd.rot[8])};
+    Rbar += tbar * dotf(w, ez) / md;
+    // tip and tilt move C at fixed R: dC/dtip = -R cos(tilt) ey, dC/dtilt = R ex
     atomicAdd(&acc[np[0]], double(-Rf * cosf(float(tilt)) * dotf(Cbar, ey)));
     atomicAdd(&acc[np[1]], double(Rf * dotf(Cbar, ex)));
-    atomicAdd(&acc[np[2]], double(Rbar + dotf(Cbar, ez)));
+    atomicAdd(&acc[np[2]], double(Rbar));
     // into the previous segment: its step-back point and outgoing direction
     Ppbar = obar;
     doutbar = dinbar;

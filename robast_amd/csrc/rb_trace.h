@@ -41,11 +41,16 @@ struct Lane {
 // (node, boundary point before the step back, incoming direction).
 struct NoRecord {
   __device__ __forceinline__ void reflect(int, V3<double>, V3<double>) {}
+  __device__ __forceinline__ void refract() {}
 };
-struct ChainRecord {  // the adjoint's recomputed bounce chain
+// The adjoint's recomputed bounce chain: up to kMax reflections (node,
+// crossing point before the step back, incoming direction). A longer chain or
+// a refraction marks the ray as outside the adjoint's model.
+struct ChainRecord {
   static constexpr int kMax = 4;
   int n = 0;
   bool overflow = false;
+  __device__ __forceinline__ void refract() { overflow = true; }
   int node[kMax];
   V3<double> P[kMax], din[kMax];
   __device__ __forceinline__ void reflect(int nd, V3<double> p, V3<double> d) {
@@ -450,6 +455,7 @@ __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& i
   double t = L.t + step / (kC / n1);
   add_point(io, i, L, L.p, t, next);
   set_direction(L, d2);
+  rec.refract();
 }
 
 // AOpticsManager::TraceNonSequential(TObjArray*) for one ray (src/AOpticsManager.cxx:353-518)

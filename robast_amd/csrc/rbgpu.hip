@@ -120,13 +120,13 @@ __global__ void __launch_bounds__(256) k_adjoint(DScene S, RayIO io, int64_t n, 
     L.step = 0;
     L.hit_node = -1;
     L.hit_leaf = 0;
-    const V3<double> origin = L.p;
+    const V3<double> origin = L.p, d0 = L.d;
     ChainRecord rec;
     RayIO ro = io;
     ro.path_cap = 0;
     trace_lane<MIXED>(S, P, ro, i, L, rec);
-    if (L.status == RB_FOCUS && rec.n > 0 && !rec.overflow && L.last_node == A.focal_node)
-      adjoint_ray(S, A, rec, origin, L.p, acc);
+    if (L.status == RB_FOCUS && !rec.overflow && L.last_node == A.focal_node)
+      adjoint_ray(S, A, rec, origin, d0, L.p, acc);
   }
   __syncthreads();
   for (int k = threadIdx.x; k < A.n_params; k += blockDim.x)
