@@ -3,7 +3,11 @@ reflector at 1e7 rays on one MI355X (rays resident in HBM), with the trace
 kernel's roofline position and the CPU oracle timed beside it.
 
 A step = one rbgpu_trace of the whole 1e7-ray batch (restored from a
-pristine HBM copy first). Prints one JSON line.
+pristine HBM copy first). `value` is the FP64 tracer (the drop-in); the same
+run also times the mixed FP32/FP64 tracer on the same batch (with its error
+against the FP64 result: status flips and focused-hit RMS) and the adjoint
+alignment gradient (loss forward + rbgpu_adjoint over 8 angles x 1.25e6 rays,
+per-facet tip/tilt/R + camera z). Prints one JSON line.
 """
 import argparse
 import json
@@ -35,11 +39,11 @@ def cpu_baseline(mgr, gen, target_s=10.0):
     from oracle import oracle as O
     cores = min(len(os.sched_getaffinity(0)), 16)
     flat = mgr.flat()
-    probe = O.generate(gen, 20000)
+    probe = O.generate(gen, 500_000)
     t0 = time.perf_counter()
     O.trace(flat, probe, mgr.trace_opts(), nthreads=cores)
-    rate = 20000 / max(time.perf_counter() - t0, 1e-6)
-    n = int(min(max(rate * target_s, 50000), 3_000_000))
+    rate = 500_000 / max(time.perf_counter() - t0, 1e-6)
+    n = int(min(max(rate * target_s, 500_000), 100_000_000))
     soa = O.generate(gen, n)
     t0 = time.perf_counter()
     O.trace(flat, soa, mgr.trace_opts(), nthreads=cores)
@@ -57,10 +61,11 @@ def main():
     ap.add_argument("--rays", type=int, default=10_000_000)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(rays shard by contiguous ranges with the scene replicated; see DESIGN.md)", file=sys.stderr)
+              "(rays would shard by contiguous ranges with the scene replicated per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -71,7 +76,6 @@ def main():
 
     N = args.rays
     mgr = configs.c2_scene()
-    mgr.SetPrecision(args.precision == "mixed")
     ctx = Context(0)
     scene = ctx.upload(mgr.flat())
     dev = torch.device("cuda")
@@ -83,28 +87,69 @@ def main():
     gen = configs.c2_raygen()
     ctx.generate(gen, make_rays_device(src), N)
     rays = make_rays_device(work)
-    opts = mgr.trace_opts(device_ptrs=1)
 
-    def step():
-        for f in src:
-            work[f].copy_(src[f])
-        torch.cuda.current_stream().synchronize()
-        ctx.trace(scene, rays, N, opts)
-        return ctx.last_trace_ms()
+    def run(precision, steps, warmup):
+        mgr.SetPrecision(precision == "mixed")
+        opts = mgr.trace_opts(device_ptrs=1)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        kernel_ms.append(step())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
+        def step():
+            for f in src:
+                work[f].copy_(src[f])
+            torch.cuda.current_stream().synchronize()
+            ctx.trace(scene, rays, N, opts)
+            return ctx.last_trace_ms()
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        kernel_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            kernel_ms.append(step())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall * 1e3 / steps, sum(kernel_ms) / len(kernel_ms)
+
+    other = "mixed" if args.precision == "fp64" else "fp64"
+    ms_o, kms_o = run(other, args.steps, args.warmup)
+    res_o = {f: v.clone() for f, v in work.items()}
+    ms_per_step, kms = run(args.precision, args.steps, args.warmup)
     value = N / (ms_per_step * 1e-3) / 1e6
-    kms = sum(kernel_ms) / len(kernel_ms)
     status = torch.bincount(work["status"], minlength=6).cpu().tolist()
+    a, b = (work, res_o) if args.precision == "fp64" else (res_o, work)
+    flips = int((a["status"] != b["status"]).sum())
+    both = (a["status"] == A.RB_FOCUS) & (b["status"] == A.RB_FOCUS)
+    d2 = (a["x"] - b["x"]) ** 2 + (a["y"] - b["y"]) ** 2 + (a["z"] - b["z"]) ** 2
+    rms_um = float(torch.sqrt(d2[both].mean())) * 1e4
+    ms_fp64, ms_mixed = (ms_per_step, ms_o) if args.precision == "fp64" else (ms_o, ms_per_step)
+    kms_fp64, kms_mixed = (kms, kms_o) if args.precision == "fp64" else (kms_o, kms)
+    del src, work, res_o, rays
+    torch.cuda.empty_cache()
+
+    adjoint = None
+    if not args.no_adjoint:
+        from robast_amd import alignment as AL
+        model = AL.DaviesCottonAlignment(ctx, rays_per_angle=N // len(AL.C5_ANGLES_DEG))
+        p0 = AL.c5_initial_params()
+        model.loss(p0)          # warm-up evaluation
+        model.gradient()
+        torch.cuda.synchronize()
+        trace_ms, adj_ms = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            model.loss(p0)
+            model.gradient()
+            trace_ms.append(model.last_trace_ms)
+            adj_ms.append(model.last_adjoint_ms)
+        torch.cuda.synchronize()
+        ms_eval = (time.perf_counter() - t0) * 1e3 / args.steps
+        adjoint = {"grad_evals_per_s": round(1e3 / ms_eval, 3), "ms_per_eval": round(ms_eval, 3),
+                   "forward_kernel_ms": round(sum(trace_ms) / len(trace_ms), 3),
+                   "adjoint_kernel_ms": round(sum(adj_ms) / len(adj_ms), 3),
+                   "rays": model.n * len(model.angles), "params": model.n_params,
+                   "workload": "Davies-Cotton alignment: 170 tip/tilt + 85 radii + camera z, loss = sum of "
+                               "spot RMS^2 over 8 angles 0-3.5 deg, unit reflectance, FP64 forward, "
+                               "adjoint FP32 per ray with FP64 accumulation"}
 
     achieved = BYTES_PER_RAY * N / (kms * 1e-3) / 1e9
     traffic = _profile_traffic(f"k_trace_c2_{args.precision}_pmc.txt")
@@ -127,11 +172,18 @@ def main():
         "kernel_ms": round(kms, 3),
         "kernel_mrays_s": round(N / (kms * 1e-3) / 1e6, 3),
         "status_counts": dict(zip(A.STATUS_NAMES, status)),
+        "fp64_mrays_s": round(N / (ms_fp64 * 1e-3) / 1e6, 3),
+        "mixed_mrays_s": round(N / (ms_mixed * 1e-3) / 1e6, 3),
+        "mixed_speedup": round(ms_fp64 / ms_mixed, 3),
+        "mixed_kernel_speedup": round(kms_fp64 / kms_mixed, 3),
+        "mixed_vs_fp64": {"status_flips": flips, "focused_hit_rms_um": round(rms_um, 6)},
+        "adjoint": adjoint,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "note": "algorithmic ray-state bytes (132 B/ray) over the k_trace launch time"},
     }
     if not args.no_cpu_baseline:
+        mgr.SetPrecision(False)
         out["cpu_baseline"] = cpu_baseline(mgr, gen)
     print(json.dumps(out), flush=True)
 
