@@ -44,6 +44,8 @@ struct DBvh {                  // 32 B, FP32 boxes expanded outward, depth-first
 };
 struct DShape {
   int32_t type, off;
+  int32_t depth;  // boolean nesting levels below this shape (0 = primitive)
+  int32_t pad;
 };
 struct DScene {
   const DNode* nodes;
@@ -900,8 +902,25 @@ template <typename T> __device__ V3<T> wins_normal(const double* p, V3<T> q, V3<
   return nn;
 }
 
+// Call structure of the shape code. Each level is a real call by default:
+// it keeps the megakernel's register allocation per function. A build can
+// inline a level (-DRB_PRIM_FN=__forceinline__ ...) to trade registers for
+// call overhead.
+#ifndef RB_PRIM_FN
+#define RB_PRIM_FN __noinline__
+#endif
+#ifndef RB_CSG0_FN
+#define RB_CSG0_FN __noinline__
+#endif
+#ifndef RB_CSG_FN
+#define RB_CSG_FN __noinline__
+#endif
+#ifndef RB_NODE_FN
+#define RB_NODE_FN __noinline__
+#endif
+
 // ------------------------------------------------------------ dispatch
-template <typename T> __device__ __noinline__ bool prim_contains(int type, const double* p, V3<T> q) {
+template <typename T> __device__ RB_PRIM_FN bool prim_contains(int type, const double* p, V3<T> q) {
   switch (type) {
     case RB_SHAPE_BOX: return box_contains<T>(p, q);
     case RB_SHAPE_TUBE: return tube_contains<T>(p, q);
@@ -913,7 +932,7 @@ template <typename T> __device__ __noinline__ bool prim_contains(int type, const
   }
   return false;
 }
-template <typename T> __device__ __noinline__ T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ RB_PRIM_FN T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_dist_out<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_dist_out<T>(p, q, d);
@@ -925,7 +944,7 @@ template <typename T> __device__ __noinline__ T prim_dist_out(int type, const do
   }
   return T(kBig);
 }
-template <typename T> __device__ __noinline__ T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ RB_PRIM_FN T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_dist_in<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_dist_in<T>(p, q, d);
@@ -937,7 +956,7 @@ template <typename T> __device__ __noinline__ T prim_dist_in(int type, const dou
   }
   return T(0);
 }
-template <typename T> __device__ __noinline__ V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ RB_PRIM_FN V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_normal<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_normal<T>(p, q, d);
@@ -958,7 +977,7 @@ template <typename T> __device__ __forceinline__ T bool_push() { return T(kBoolP
 template <> __device__ __forceinline__ float bool_push<float>() { return 2e-4f; }
 
 template <typename T, int D> struct Csg {
-  __device__ __noinline__ static bool contains(const DScene& S, int shape, V3<T> q) {
+  __device__ RB_CSG_FN static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return prim_contains<T>(sh.type, p, q);
@@ -976,7 +995,7 @@ template <typename T, int D> struct Csg {
     h.leaf = leaf_push(h.leaf, side);
     return h;
   }
-  __device__ __noinline__ static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ RB_CSG_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T>(sh.type, p, q, d)), 0u, 0u};
@@ -1018,7 +1037,7 @@ template <typename T, int D> struct Csg {
     }
     return {kBig, 0u, 0u};
   }
-  __device__ __noinline__ static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ RB_CSG_FN static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T>(sh.type, p, q, d)), 0u, 1u};
@@ -1051,7 +1070,7 @@ template <typename T, int D> struct Csg {
     return {double(s), last.leaf, last.exiting};
   }
   // normal of the leaf surface `leaf` (path bits consumed from bit 0)
-  __device__ __noinline__ static V3<T> normal(const DScene& S, int shape, uint32_t path, int depth, V3<T> q, V3<T> d) {
+  __device__ RB_CSG_FN static V3<T> normal(const DScene& S, int shape, uint32_t path, int depth, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return prim_normal<T>(sh.type, p, q, d);
@@ -1064,7 +1083,7 @@ template <typename T, int D> struct Csg {
   // FP64 crossing of the leaf primitive `path` near the estimate s0 along
   // the ray: the primitive's own entry/exit distance from a point backed
   // off by `back` (the mixed form's root polish). Returns kBig on failure.
-  __device__ __noinline__ static double polish(const DScene& S, int shape, uint32_t path, int depth, bool exiting, V3<T> q,
+  __device__ RB_CSG_FN static double polish(const DScene& S, int shape, uint32_t path, int depth, bool exiting, V3<T> q,
                                   V3<T> d, T s0, T back) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
@@ -1082,23 +1101,23 @@ template <typename T, int D> struct Csg {
 };
 // depth exhausted: primitives only (the host rejects deeper expressions)
 template <typename T> struct Csg<T, 0> {
-  __device__ __noinline__ static bool contains(const DScene& S, int shape, V3<T> q) {
+  __device__ RB_CSG0_FN static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
     return prim_contains<T>(sh.type, S.params + sh.off, q);
   }
-  __device__ __noinline__ static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ RB_CSG0_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     return {double(prim_dist_out<T>(sh.type, S.params + sh.off, q, d)), 0u, 0u};
   }
-  __device__ __noinline__ static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ RB_CSG0_FN static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     return {double(prim_dist_in<T>(sh.type, S.params + sh.off, q, d)), 0u, 1u};
   }
-  __device__ __noinline__ static V3<T> normal(const DScene& S, int shape, uint32_t, int, V3<T> q, V3<T> d) {
+  __device__ RB_CSG0_FN static V3<T> normal(const DScene& S, int shape, uint32_t, int, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     return prim_normal<T>(sh.type, S.params + sh.off, q, d);
   }
-  __device__ __noinline__ static double polish(const DScene& S, int shape, uint32_t, int, bool exiting, V3<T> q, V3<T> d, T s0,
+  __device__ RB_CSG0_FN static double polish(const DScene& S, int shape, uint32_t, int, bool exiting, V3<T> q, V3<T> d, T s0,
                                   T back) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
@@ -1109,9 +1128,44 @@ template <typename T> struct Csg<T, 0> {
   }
 };
 
+// Entry points by the shape's own nesting depth, so that a primitive or a
+// single boolean level does not pay for the deepest unrolled form.
 template <typename T>
 __device__ __forceinline__ V3<T> shape_normal(const DScene& S, int shape, uint32_t leaf, V3<T> q, V3<T> d) {
-  return Csg<T, kMaxCsgDepth>::normal(S, shape, leaf & 0xffffffu, int(leaf >> 24), q, d);
+  const uint32_t path = leaf & 0xffffffu;
+  const int depth = int(leaf >> 24);
+  switch (S.shapes[shape].depth) {
+    case 0: return Csg<T, 0>::normal(S, shape, path, depth, q, d);
+    case 1: return Csg<T, 1>::normal(S, shape, path, depth, q, d);
+  }
+  return Csg<T, kMaxCsgDepth>::normal(S, shape, path, depth, q, d);
+}
+template <typename T>
+__device__ __forceinline__ bool shape_contains(const DScene& S, int shape, V3<T> q) {
+  switch (S.shapes[shape].depth) {
+    case 0: return Csg<T, 0>::contains(S, shape, q);
+    case 1: return Csg<T, 1>::contains(S, shape, q);
+  }
+  return Csg<T, kMaxCsgDepth>::contains(S, shape, q);
+}
+template <typename T>
+__device__ __forceinline__ Hit shape_dist(const DScene& S, int shape, V3<T> q, V3<T> d, bool out) {
+  switch (S.shapes[shape].depth) {
+    case 0: return out ? Csg<T, 0>::dist_out(S, shape, q, d) : Csg<T, 0>::dist_in(S, shape, q, d);
+    case 1: return out ? Csg<T, 1>::dist_out(S, shape, q, d) : Csg<T, 1>::dist_in(S, shape, q, d);
+  }
+  return out ? Csg<T, kMaxCsgDepth>::dist_out(S, shape, q, d) : Csg<T, kMaxCsgDepth>::dist_in(S, shape, q, d);
+}
+template <typename T>
+__device__ __forceinline__ double shape_polish(const DScene& S, int shape, uint32_t leaf, bool exiting, V3<T> q,
+                                               V3<T> d, T s0, T back) {
+  const uint32_t path = leaf & 0xffffffu;
+  const int depth = int(leaf >> 24);
+  switch (S.shapes[shape].depth) {
+    case 0: return Csg<T, 0>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+    case 1: return Csg<T, 1>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+  }
+  return Csg<T, kMaxCsgDepth>::polish(S, shape, path, depth, exiting, q, d, s0, back);
 }
 
 // ------------------------------------------------------------ physics

@@ -264,7 +264,8 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
     }
   }
   std::vector<DShape> shapes(d->n_shapes);
-  for (int i = 0; i < d->n_shapes; i++) shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset};
+  for (int i = 0; i < d->n_shapes; i++)
+    shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset, csg_depth(d, i, 0), 0};
   blob.clear();
   lay.nodes = put(blob, nodes.data(), nodes.size());
   lay.children = put(blob, d->children, size_t(std::max(d->n_children, 0)));

@@ -24,7 +24,20 @@ struct TraceParams {
 };
 
 // per-lane ray and navigator state
+// Candidate leaves of one BVH traversal, in LDS: a lane's slot k lives at
+// [k * kCandStride], so adjacent lanes hit adjacent banks.
+#ifndef RB_POSTPONE
+#define RB_POSTPONE 1
+#endif
+constexpr int kCand = 8;
+constexpr int kCandStride = 256;  // the tracing kernels' block size
+struct Cand {
+  int32_t* node;
+  float* tn;
+};
+
 struct Lane {
+  Cand cand;          // this lane's candidate slots
   V3<double> p, d;    // navigator point (== last recorded point between segments), unit direction
   double t, lambda;
   int32_t cur;        // current node, -1 = outside the top volume
@@ -65,9 +78,9 @@ struct ChainRecord {
   }
 };
 
-__device__ __noinline__ bool node_contains(const DScene& S, int n, V3<double> p) {
+__device__ RB_NODE_FN bool node_contains(const DScene& S, int n, V3<double> p) {
   const DNode& nd = S.nodes[n];
-  return Csg<double, kMaxCsgDepth>::contains(S, nd.shape, m2l<double>(nd.rot, nd.tr, p));
+  return shape_contains<double>(S, nd.shape, m2l<double>(nd.rot, nd.tr, p));
 }
 
 __device__ __forceinline__ bool box_has(const DBvh& b, float x, float y, float z) {
@@ -96,7 +109,7 @@ __device__ int first_daughter_containing(const DScene& S, int n, V3<double> p, i
 }
 
 // TGeoNavigator::SearchNode (up until inside, then down through daughters)
-__device__ __noinline__ int search_node(const DScene& S, int start, V3<double> p, int skip, bool forced) {
+__device__ RB_NODE_FN int search_node(const DScene& S, int start, V3<double> p, int skip, bool forced) {
   int n = start;
   bool inside = forced || n == skip;
   while (!inside) {
@@ -127,31 +140,85 @@ __device__ __forceinline__ double locate_push(const DScene& S, int n, double ste
 // search itself in precision T; the origin is first rebased to `base`
 // along the ray (the node's box entry) so FP32 works near the node's frame
 template <typename T>
-__device__ __noinline__ Hit node_dist(const DScene& S, int c, V3<double> p, V3<double> d, double base, bool out) {
+__device__ RB_NODE_FN Hit node_dist(const DScene& S, int c, V3<double> p, V3<double> d, double base, bool out) {
   const DNode& nd = S.nodes[c];
   V3<double> pb = axpy(p, base, d);
   V3<double> q = m2l<double>(nd.rot, nd.tr, pb), ld = m2lv<double>(nd.rot, d);
   V3<T> qt = mk<T>(T(q.x), T(q.y), T(q.z)), dt = mk<T>(T(ld.x), T(ld.y), T(ld.z));
-  Hit h = out ? Csg<T, kMaxCsgDepth>::dist_out(S, nd.shape, qt, dt) : Csg<T, kMaxCsgDepth>::dist_in(S, nd.shape, qt, dt);
+  Hit h = shape_dist<T>(S, nd.shape, qt, dt, out);
   if (h.s < kBig) h.s += base;
   return h;
 }
 
 // FP64 root polish of a crossing found in FP32 (leaf surface of node c)
-__device__ __noinline__ double node_polish(const DScene& S, int c, const Hit& h, V3<double> p, V3<double> d) {
+__device__ RB_NODE_FN double node_polish(const DScene& S, int c, const Hit& h, V3<double> p, V3<double> d) {
   const DNode& nd = S.nodes[c];
   V3<double> q = m2l<double>(nd.rot, nd.tr, p), ld = m2lv<double>(nd.rot, d);
   const double back = 2e-3 + 1e-6 * h.s;
-  double s = Csg<double, kMaxCsgDepth>::polish(S, nd.shape, h.leaf & 0xffffffu, int(h.leaf >> 24), h.exiting != 0,
-                                                q, ld, h.s, back);
+  double s = shape_polish<double>(S, nd.shape, h.leaf, h.exiting != 0, q, ld, h.s, back);
   return (s < kBig && fabs(s - h.s) <= back) ? s : -1.0;
+}
+
+// Slab test of a BVH box (FP32, boxes rounded outward on the host): entry
+// distance tn, and whether the box is crossed no farther than lim.
+__device__ __forceinline__ bool box_pass(const DBvh& b, float ox, float oy, float oz, float ix, float iy, float iz,
+                                         float lim, float& tn) {
+  float t0x = (b.lo[0] - ox) * ix, t1x = (b.hi[0] - ox) * ix;
+  float t0y = (b.lo[1] - oy) * iy, t1y = (b.hi[1] - oy) * iy;
+  float t0z = (b.lo[2] - oz) * iz, t1z = (b.hi[2] - oz) * iz;
+  tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+  float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+  return tn <= tf * 1.0001f + 1e-3f && tn <= lim;
+}
+
+// Gather the leaves (daughter node ids, in depth-first order) whose boxes
+// the ray crosses no farther than lim. Returns the count, or -1 if more than
+// kCand qualify.
+__device__ __forceinline__ int gather_leaves(const DScene& S, int root, float ox, float oy, float oz, float ix,
+                                             float iy, float iz, float lim, Cand cand) {
+  int i = root, nc = 0;
+  const int end = S.bvh[i].escape;
+  while (i < end) {
+    const DBvh b = S.bvh[i];
+    float tn;
+    if (!box_pass(b, ox, oy, oz, ix, iy, iz, lim, tn)) {
+      i = b.escape;
+      continue;
+    }
+    if (b.prim >= 0) {
+      if (nc == kCand) return -1;
+      cand.node[nc * kCandStride] = b.prim;
+      cand.tn[nc * kCandStride] = tn;
+      nc++;
+    }
+    i++;
+  }
+  return nc;
+}
+
+// FP64 ranking rule: a daughter must beat the best by the tolerance; within
+// the tolerance the lower node id wins, as the reference's loop over
+// daughters in index order decides.
+__device__ __forceinline__ void rank_fp64(int c, const Hit& h, double& best, int& crossed, Hit& hit) {
+  if (h.s < best - kTol || (crossed >= 0 && c < crossed && h.s < best + kTol && h.s < kBig)) {
+    best = h.s;
+    crossed = c;
+    hit = h;
+  }
 }
 
 // Find the nearest boundary of the current node `cur` for the ray at p:
 // its own exit (crossed = -1) or a daughter's entry (crossed = node id),
 // all shape distances in FP64 (the plain form).
+//
+// The daughters are tested in two phases: the BVH walk only gathers the
+// leaves whose boxes the ray crosses, then every lane tests its k-th
+// candidate together, so the shape code runs with the wave's lanes in step
+// instead of one leaf-holding lane at a time. Each candidate is re-checked
+// against the current best before its test, in the walk's order, which
+// reproduces the one-pass walk's tests and ranking exactly.
 __device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<double> d, double& best,
-                                     int& crossed, Hit& hit) {
+                                     int& crossed, Hit& hit, Cand cand) {
   const DNode& cn = S.nodes[cur];
   Hit hx = node_dist<double>(S, cur, p, d, 0.0, false);
   best = kBig;
@@ -164,29 +231,27 @@ __device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<
   if (cn.n_children == 0) return;
   const float ox = float(p.x), oy = float(p.y), oz = float(p.z);
   const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
+#if RB_POSTPONE
+  const int nc = gather_leaves(S, cn.bvh_root, ox, oy, oz, ix, iy, iz, float(best) * 1.0001f + 1e-2f, cand);
+  if (nc >= 0) {
+    for (int k = 0; k < nc; k++) {
+      if (cand.tn[k * kCandStride] > float(best) * 1.0001f + 1e-2f) continue;
+      const int c = cand.node[k * kCandStride];
+      rank_fp64(c, node_dist<double>(S, c, p, d, 0.0, true), best, crossed, hit);
+    }
+    return;
+  }
+#endif
   int i = cn.bvh_root;
   const int end = S.bvh[i].escape;
   while (i < end) {
     const DBvh b = S.bvh[i];
-    float t0x = (b.lo[0] - ox) * ix, t1x = (b.hi[0] - ox) * ix;
-    float t0y = (b.lo[1] - oy) * iy, t1y = (b.hi[1] - oy) * iy;
-    float t0z = (b.lo[2] - oz) * iz, t1z = (b.hi[2] - oz) * iz;
-    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
-    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    float lim = float(best) * 1.0001f + 1e-2f;
-    if (!(tn <= tf * 1.0001f + 1e-3f) || tn > lim) {
+    float tn;
+    if (!box_pass(b, ox, oy, oz, ix, iy, iz, float(best) * 1.0001f + 1e-2f, tn)) {
       i = b.escape;
       continue;
     }
-    if (b.prim >= 0) {
-      const int c = b.prim;
-      Hit h = node_dist<double>(S, c, p, d, 0.0, true);
-      if (h.s < best - kTol || (crossed >= 0 && c < crossed && h.s < best + kTol && h.s < kBig)) {
-        best = h.s;
-        crossed = c;
-        hit = h;
-      }
-    }
+    if (b.prim >= 0) rank_fp64(b.prim, node_dist<double>(S, b.prim, p, d, 0.0, true), best, crossed, hit);
     i++;
   }
 }
@@ -199,8 +264,31 @@ __device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<
 // with an FP64 inside test; near-ties and failed checks redo the search
 // in FP64. Returns false when the caller must run the FP64 search.
 constexpr double kNearF32 = 2e-2;
+
+// One daughter of the mixed search: FP32 distance from its box entry, FP64
+// when it is within kNearF32 of the origin; keeps the best and second best.
+__device__ __forceinline__ void rank_mixed(const DScene& S, int c, float tn, V3<double> p, V3<double> d,
+                                           double& best, double& second, int& crossed, Hit& hit, bool& exact) {
+  const double base = fmax(0.0, double(tn) * (1.0 - 1e-6) - 1.0);
+  Hit h = node_dist<float>(S, c, p, d, base, true);
+  bool hexact = false;
+  if (h.s < kNearF32) {
+    h = node_dist<double>(S, c, p, d, 0.0, true);
+    hexact = true;
+  }
+  if (h.s < best) {
+    second = best;
+    best = h.s;
+    crossed = c;
+    hit = h;
+    exact = hexact;
+  } else if (h.s < second) {
+    second = h.s;
+  }
+}
+
 __device__ bool search_boundary_mixed(const DScene& S, int cur, V3<double> p, V3<double> d, double& best,
-                                      int& crossed, Hit& hit) {
+                                      int& crossed, Hit& hit, Cand cand) {
   const DNode& cn = S.nodes[cur];
   best = kBig;
   crossed = -2;
@@ -221,40 +309,28 @@ __device__ bool search_boundary_mixed(const DScene& S, int cur, V3<double> p, V3
   if (cn.n_children > 0) {
     const float ox = float(p.x), oy = float(p.y), oz = float(p.z);
     const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
-    int i = cn.bvh_root;
-    const int end = S.bvh[i].escape;
-    while (i < end) {
-      const DBvh b = S.bvh[i];
-      float t0x = (b.lo[0] - ox) * ix, t1x = (b.hi[0] - ox) * ix;
-      float t0y = (b.lo[1] - oy) * iy, t1y = (b.hi[1] - oy) * iy;
-      float t0z = (b.lo[2] - oz) * iz, t1z = (b.hi[2] - oz) * iz;
-      float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
-      float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-      float lim = float(fmin(best, second)) * 1.0001f + 1e-2f;
-      if (!(tn <= tf * 1.0001f + 1e-3f) || tn > lim) {
-        i = b.escape;
-        continue;
-      }
-      if (b.prim >= 0) {
-        const int c = b.prim;
-        const double base = fmax(0.0, double(tn) * (1.0 - 1e-6) - 1.0);
-        Hit h = node_dist<float>(S, c, p, d, base, true);
-        bool hexact = false;
-        if (h.s < kNearF32) {
-          h = node_dist<double>(S, c, p, d, 0.0, true);
-          hexact = true;
+    int nc = -1;
+#if RB_POSTPONE
+    nc = gather_leaves(S, cn.bvh_root, ox, oy, oz, ix, iy, iz, float(fmin(best, second)) * 1.0001f + 1e-2f, cand);
+    for (int k = 0; k < nc; k++) {
+      const float tn = cand.tn[k * kCandStride];
+      if (tn > float(fmin(best, second)) * 1.0001f + 1e-2f) continue;
+      rank_mixed(S, cand.node[k * kCandStride], tn, p, d, best, second, crossed, hit, exact);
+    }
+#endif
+    if (nc < 0) {
+      int i = cn.bvh_root;
+      const int end = S.bvh[i].escape;
+      while (i < end) {
+        const DBvh b = S.bvh[i];
+        float tn;
+        if (!box_pass(b, ox, oy, oz, ix, iy, iz, float(fmin(best, second)) * 1.0001f + 1e-2f, tn)) {
+          i = b.escape;
+          continue;
         }
-        if (h.s < best) {
-          second = best;
-          best = h.s;
-          crossed = c;
-          hit = h;
-          exact = hexact;
-        } else if (h.s < second) {
-          second = h.s;
-        }
+        if (b.prim >= 0) rank_mixed(S, b.prim, tn, p, d, best, second, crossed, hit, exact);
+        i++;
       }
-      i++;
     }
   }
   if (crossed == -2) return true;
@@ -301,11 +377,11 @@ __device__ int next_boundary(const DScene& S, Lane& L) {
   double best;
   int crossed;
   Hit hit = {kBig, 0u, 0u};
-  if (!MIXED || !search_boundary_mixed(S, cur, L.p, L.d, best, crossed, hit)) {
+  if (!MIXED || !search_boundary_mixed(S, cur, L.p, L.d, best, crossed, hit, L.cand)) {
 #ifdef RB_HOST_EMULATION
     if (MIXED) rb_emul_fallbacks++;
 #endif
-    search_boundary_fp64(S, cur, L.p, L.d, best, crossed, hit);
+    search_boundary_fp64(S, cur, L.p, L.d, best, crossed, hit, L.cand);
   }
   if (crossed == -1 && best <= kTol) {
     // TGeoNavigator: do not stay on the exit surface; step by the tolerance

@@ -66,12 +66,31 @@ struct DevRays {
 }  // namespace
 
 // ---------------------------------------------------------------- kernels
+// Block size of the tracing kernels: the LDS candidate slots are laid out
+// for exactly this many lanes.
+constexpr int kTraceBlock = kCandStride;
+// Occupancy request for the tracing kernels (waves per SIMD); 0 leaves the
+// register budget to the compiler. Two waves (256 registers per lane, the
+// rest spilled) measured faster than one wave with 512 on MI355X: the
+// second wave hides the latency of the scene and BVH loads.
+#ifndef RB_WAVES_PER_EU
+#define RB_WAVES_PER_EU 2
+#endif
+#if RB_WAVES_PER_EU > 0
+#define RB_TRACE_ATTR __attribute__((amdgpu_waves_per_eu(RB_WAVES_PER_EU)))
+#else
+#define RB_TRACE_ATTR
+#endif
+
 template <bool MIXED>
-__global__ void __launch_bounds__(256) k_trace(DScene S, RayIO io, int64_t n, TraceParams P) {
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, RayIO io, int64_t n, TraceParams P) {
+  __shared__ int32_t cand_node[kCand * kCandStride];
+  __shared__ float cand_tn[kCand * kCandStride];
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (io.status[i] != RB_RUN) return;
   Lane L;
+  L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
   L.p = mk(io.x[i], io.y[i], io.z[i]);
   L.d = mk(io.dx[i], io.dy[i], io.dz[i]);
   L.t = io.t[i];
@@ -101,14 +120,17 @@ __global__ void __launch_bounds__(256) k_trace(DScene S, RayIO io, int64_t n, Tr
 // chain (nothing is stored by the forward pass) and accumulates parameter
 // gradients in LDS (FP64), flushed with one global atomic per parameter.
 template <bool MIXED>
-__global__ void __launch_bounds__(256) k_adjoint(DScene S, RayIO io, int64_t n, TraceParams P, AdjParams A,
-                                                 double* grad) {
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DScene S, RayIO io, int64_t n, TraceParams P, AdjParams A,
+                                                         double* grad) {
   __shared__ double acc[kMaxAdjParams];
+  __shared__ int32_t cand_node[kCand * kCandStride];
+  __shared__ float cand_tn[kCand * kCandStride];
   for (int k = threadIdx.x; k < A.n_params; k += blockDim.x) acc[k] = 0.0;
   __syncthreads();
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n && io.status[i] == RB_RUN) {
     Lane L;
+    L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
     L.p = mk(io.x[i], io.y[i], io.z[i]);
     L.d = mk(io.dx[i], io.dy[i], io.dz[i]);
     L.t = io.t[i];
@@ -331,7 +353,7 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
     io.path_node = rays->path_node;
   }
 
-  const int block = 256;
+  const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (opts->precision == RB_PRECISION_MIXED)
@@ -375,7 +397,7 @@ int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* g, rb_rays* rays, int
   io.x = rays->x; io.y = rays->y; io.z = rays->z; io.t = rays->t;
   io.dx = rays->dx; io.dy = rays->dy; io.dz = rays->dz; io.lambda = rays->lambda;
   io.status = rays->status; io.npoints = rays->npoints; io.last_node = rays->last_node;
-  const int block = 256;
+  const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
   hipLaunchKernelGGL(k_generate, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, *g, io, n);
   HIPCHK(ctx, hipGetLastError());
@@ -415,7 +437,7 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   A.cx = adj->centroid_x;
   A.cy = adj->centroid_y;
   A.weight = adj->weight;
-  const int block = 256;
+  const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (opts->precision == RB_PRECISION_MIXED)
