@@ -29,7 +29,10 @@ struct TraceParams {
 #ifndef RB_POSTPONE
 #define RB_POSTPONE 1
 #endif
-constexpr int kCand = 8;
+#ifndef RB_NCAND
+#define RB_NCAND 8
+#endif
+constexpr int kCand = RB_NCAND;
 constexpr int kCandStride = 256;  // the tracing kernels' block size
 struct Cand {
   int32_t* node;

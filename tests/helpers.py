@@ -1,4 +1,5 @@
 """Shared helpers for the parity tests."""
+import mpmath as mp
 import numpy as np
 
 from robast_amd import _abi as A
@@ -30,3 +31,40 @@ def compare(gpu, ora, pos_tol=1e-7, dir_tol=1e-9, t_tol=1e-15, check_nodes=True)
 
 def counts(status):
     return np.bincount(np.asarray(status), minlength=6)
+
+
+def c1_exact_chain(flat, src, i):
+    """50-digit construction of ray i of a C1 (paraboloid) batch: the hit on
+    the inner surface z = a r^2 + b + sag/2, the normal, the specular
+    reflection, the 2e-6 cm step back along -d1 and the focal-plane (z = f)
+    crossing. Returns (end point [x, y, z], arrival time)."""
+    from robast_amd import configs
+    mp.mp.dps = 50
+    para = [k for k in range(len(flat.shapes)) if flat.shapes[k]["type"] == A.RB_SHAPE_PARABOLOID][0]
+    off = flat.shapes[para]["param_offset"]
+    a = mp.mpf(float(flat.params[off + 3]))
+    b = mp.mpf(float(flat.params[off + 4]))
+    shift = mp.mpf(configs.C1_R ** 2 / 4.0 / configs.C1_F / 2.0)
+    f = mp.mpf(configs.C1_F)
+    c = mp.mpf(2.99792458e10)
+    o = [mp.mpf(float(src[k][i])) for k in ("x", "y", "z")]
+    d = [mp.mpf(float(src[k][i])) for k in ("dx", "dy", "dz")]
+    A2 = a * (d[0] ** 2 + d[1] ** 2)
+    B2 = 2 * a * (o[0] * d[0] + o[1] * d[1]) - d[2]
+    C2 = a * (o[0] ** 2 + o[1] ** 2) + b + shift - o[2]
+    sq = mp.sqrt(B2 ** 2 - 4 * A2 * C2)
+    qq = -(B2 + (sq if B2 >= 0 else -sq)) / 2
+    roots = [C2 / qq] + ([qq / A2] if A2 != 0 else [])
+    t = min(r for r in roots if r > 0)
+    h = [o[k] + t * d[k] for k in range(3)]
+    n = [2 * a * h[0], 2 * a * h[1], mp.mpf(-1)]
+    nn = mp.sqrt(sum(v * v for v in n))
+    n = [v / nn for v in n]
+    cs = sum(d[k] * n[k] for k in range(3))
+    d2 = [d[k] - 2 * cs * n[k] for k in range(3)]
+    m = mp.sqrt(sum(v * v for v in d2))
+    d2 = [v / m for v in d2]
+    hb = [h[k] - mp.mpf("2e-6") * d[k] for k in range(3)]
+    s2 = (f - hb[2]) / d2[2]
+    end = [hb[k] + s2 * d2[k] for k in range(3)]
+    return end, t / c + (s2 + mp.mpf("1e-10")) / c

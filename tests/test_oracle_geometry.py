@@ -8,6 +8,7 @@ import mpmath as mp
 import numpy as np
 import pytest
 
+from helpers import c1_exact_chain
 from oracle import oracle as O
 from robast_amd import _abi as A
 from robast_amd import configs
@@ -161,41 +162,12 @@ def test_c1_reflection_chain_matches_exact_geometry(deg):
     soa = _c1_soa(deg)
     src = {k: v.copy() for k, v in soa.items()}
     O.trace(flat, soa, mgr.trace_opts(), nthreads=8)
-    # paraboloid coefficients as flattened
-    para = [i for i in range(len(flat.shapes)) if flat.shapes[i]["type"] == A.RB_SHAPE_PARABOLOID][0]
-    off = flat.shapes[para]["param_offset"]
-    a = mp.mpf(float(flat.params[off + 3]))
-    b = mp.mpf(float(flat.params[off + 4]))
-    sag = configs.C1_R ** 2 / 4.0 / configs.C1_F
-    shift = mp.mpf(sag / 2.0)  # mirror_tr1: inner (reflecting) surface z = a r^2 + b + shift
-    f = mp.mpf(configs.C1_F)
     idx = np.nonzero(soa["status"] == A.RB_FOCUS)[0][::997]
     assert len(idx) > 50
-    c = mp.mpf(2.99792458e10)
     for i in idx:
-        o = [mp.mpf(float(src[k][i])) for k in ("x", "y", "z")]
-        d = [mp.mpf(float(src[k][i])) for k in ("dx", "dy", "dz")]
-        A2 = a * (d[0] ** 2 + d[1] ** 2)
-        B2 = 2 * a * (o[0] * d[0] + o[1] * d[1]) - d[2]
-        C2 = a * (o[0] ** 2 + o[1] ** 2) + b + shift - o[2]
-        sq = mp.sqrt(B2 ** 2 - 4 * A2 * C2)
-        qq = -(B2 + (sq if B2 >= 0 else -sq)) / 2
-        roots = [C2 / qq] + ([qq / A2] if A2 != 0 else [])
-        t = min(r for r in roots if r > 0)
-        h = [o[k] + t * d[k] for k in range(3)]
-        n = [2 * a * h[0], 2 * a * h[1], mp.mpf(-1)]
-        nn = mp.sqrt(sum(v * v for v in n))
-        n = [v / nn for v in n]
-        cs = sum(d[k] * n[k] for k in range(3))
-        d2 = [d[k] - 2 * cs * n[k] for k in range(3)]
-        m = mp.sqrt(sum(v * v for v in d2))
-        d2 = [v / m for v in d2]
-        hb = [h[k] - mp.mpf("2e-6") * d[k] for k in range(3)]
-        s2 = (f - hb[2]) / d2[2]
-        end = [hb[k] + s2 * d2[k] for k in range(3)]
+        end, tt = c1_exact_chain(flat, src, i)
         for k, key in enumerate(("x", "y", "z")):
             assert abs(float(end[k]) - soa[key][i]) < 1e-9, (i, key, float(end[k]), soa[key][i])
-        tt = t / c + (s2 + mp.mpf("1e-10")) / c
         assert abs(float(tt) - soa["t"][i]) < 1e-18
         assert soa["npoints"][i] == 3
 
