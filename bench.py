@@ -21,6 +21,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s to termination, FP64 vs mixed FP32; adjoint grad evals/s at 1e7 rays"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
 BYTES_PER_RAY = 132  # in: 8 doubles + status; out: status, point(4), dir(3), npoints, node
 
 
@@ -30,6 +31,16 @@ def _profile_traffic(name):
     if not os.path.exists(path):
         return None
     m = re.search(r"hbm_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def _algorithmic_flops(key):
+    """FP64 FLOPs per ray from profiles/algorithmic_flops.md (instrumented
+    oracle counts x nominal per-operation costs), or None."""
+    path = os.path.join(ROOT, "profiles", "algorithmic_flops.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(key + r"\s*:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -182,6 +193,13 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "note": "algorithmic ray-state bytes (132 B/ray) over the k_trace launch time"},
     }
+    flops = _algorithmic_flops("c2_fp64_flops_per_ray")
+    if flops is not None:
+        tf = flops * N / (kms * 1e-3) / 1e12
+        out["valu_roofline"] = {"bound": "valu_fp64", "achieved": round(tf, 4), "peak": FP64_PEAK_TFS,
+                                "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFS, 6), "flops_per_ray": flops,
+                                "note": "algorithmic FP64 FLOPs per ray (profiles/algorithmic_flops.md) over the "
+                                        "k_trace launch time; MI355X vector FP64 peak"}
     if not args.no_cpu_baseline:
         mgr.SetPrecision(False)
         out["cpu_baseline"] = cpu_baseline(mgr, gen)

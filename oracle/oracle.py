@@ -45,6 +45,9 @@ def lib():
         L.oracle_adjoint_dual.argtypes = [C.POINTER(A.rb_scene_desc), C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays),
                                           C.c_int64, C.POINTER(A.rb_adjoint_desc), C.c_void_p, C.c_int]
         L.oracle_adjoint_dual.restype = C.c_int64
+        L.oracle_trace_counts.argtypes = [C.POINTER(A.rb_scene_desc), C.POINTER(A.rb_trace_opts),
+                                          C.POINTER(A.rb_rays), C.c_int64, C.c_void_p, C.c_int]
+        L.oracle_trace_counts.restype = C.c_int
         _lib = L
     return _lib
 
@@ -92,6 +95,24 @@ def adjoint_dual(flat, soa, opts, adj, n_params, nthreads=8):
     if used < 0:
         raise RuntimeError("oracle_adjoint_dual failed")
     return grad, int(used)
+
+
+OP_NAMES = ("xform_point", "xform_dir", "dist_box", "dist_tube", "dist_sphere", "dist_paraboloid", "dist_pgon",
+            "dist_winston_cap", "asph_seed", "asph_newton", "asph_wall", "winston_side", "in_box", "in_tube",
+            "in_sphere", "in_paraboloid", "in_pgon", "in_aspheric", "in_winston", "normal", "reflect", "refract",
+            "box_test", "step")
+
+
+def trace_counts(flat, soa, opts):
+    """Single-threaded trace in place that also returns the operation counts
+    (dict OP_NAMES -> count) the FLOP figure is built from."""
+    n = len(soa["x"])
+    c = np.zeros(64, dtype=np.uint64)
+    r = rays_struct(soa)
+    k = lib().oracle_trace_counts(C.byref(flat.desc), C.byref(opts), C.byref(r), n, c.ctypes.data, 64)
+    if k != len(OP_NAMES):
+        raise RuntimeError(f"oracle_trace_counts: {k} counters, expected {len(OP_NAMES)}")
+    return dict(zip(OP_NAMES, (int(v) for v in c[:k])))
 
 
 def generate(gen, n):

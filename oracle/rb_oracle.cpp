@@ -41,6 +41,23 @@ constexpr double kC = 2.99792458e10; // TMath::C() * AOpticsManager::m(), cm/s
 constexpr double kBoolPush = 1e-8;   // inside-test offset of the boolean iteration
 constexpr double kPi = 3.14159265358979323846;
 
+// Operation counts for the algorithmic FLOP figure (oracle_trace_counts):
+// every transform, primitive distance / inside test, aspheric Newton step,
+// Winston side solve, normal, reflection, refraction and bounding-box test
+// the trace performs. Off (null) for ordinary traces.
+enum OpKind {
+  OP_XFORM_P, OP_XFORM_V,
+  OP_DIST_BOX, OP_DIST_TUBE, OP_DIST_SPHERE, OP_DIST_PARA, OP_DIST_PGON, OP_DIST_WINS_CAP,
+  OP_ASPH_SEED, OP_ASPH_NEWTON, OP_ASPH_WALL, OP_WINS_SIDE,
+  OP_IN_BOX, OP_IN_TUBE, OP_IN_SPHERE, OP_IN_PARA, OP_IN_PGON, OP_IN_ASPH, OP_IN_WINS,
+  OP_NORMAL, OP_REFLECT, OP_REFRACT, OP_BOX_TEST, OP_STEP,
+  OP_COUNT
+};
+static thread_local uint64_t* g_ops = nullptr;
+static inline void count_op(int k) {
+  if (g_ops) g_ops[k]++;
+}
+
 struct V3 {
   double x, y, z;
 };
@@ -53,12 +70,14 @@ static inline V3 axpy(V3 p, double s, V3 d) { return {p.x + s * d.x, p.y + s * d
 
 // TGeoMatrix conventions: master = R local + T
 static inline V3 m2l(const rb_matrix& m, V3 p) {
+  count_op(OP_XFORM_P);
   V3 q = sub(p, v3(m.tr[0], m.tr[1], m.tr[2]));
   const double* r = m.rot;
   return {r[0] * q.x + r[3] * q.y + r[6] * q.z, r[1] * q.x + r[4] * q.y + r[7] * q.z,
           r[2] * q.x + r[5] * q.y + r[8] * q.z};
 }
 static inline V3 m2lv(const rb_matrix& m, V3 q) {
+  count_op(OP_XFORM_V);
   const double* r = m.rot;
   return {r[0] * q.x + r[3] * q.y + r[6] * q.z, r[1] * q.x + r[4] * q.y + r[7] * q.z,
           r[2] * q.x + r[5] * q.y + r[8] * q.z};
@@ -659,6 +678,7 @@ static bool asph_dsag(const Asph& a, int s, double r, double& df) {
 // DistToAsphere (:415-526): Matsui's spherical seed, then Newton steps
 // along the ray until |e| < 1e-10 (at most 100)
 static double asph_dist_surface(const Asph& a, int s, V3 P, V3 D) {
+  count_op(OP_ASPH_SEED);
   const double c = a.c[s], kap = a.kap[s], d = a.z[s];
   double H2 = P.x * P.x + P.y * P.y;
   double p = -((P.z - d) * D.z + P.x * D.x + P.y * D.y);
@@ -670,6 +690,7 @@ static double asph_dist_surface(const Asph& a, int s, V3 P, V3 D) {
   double np[3] = {P.x + q * D.x, P.y + q * D.y, P.z + q * D.z - d};
   for (int i = 0;; i++) {
     if (i > 100) return kBig;
+    count_op(OP_ASPH_NEWTON);
     H2 = np[0] * np[0] + np[1] * np[1];
     check = 1 - kap * H2 * c * c;
     if (check < 0) return kBig;
@@ -698,6 +719,7 @@ static double asph_dist_surface(const Asph& a, int s, V3 P, V3 D) {
 }
 // DistToInner / DistToOuter (:529-684): the cylinder r = R between the two sags
 static double asph_dist_wall(const Asph& a, bool outer, V3 P, V3 D) {
+  count_op(OP_ASPH_WALL);
   if (!outer && !(a.rmin > 0)) return kBig;
   const double R = outer ? a.rmax : a.rmin;
   double rsq = P.x * P.x + P.y * P.y, nsq = D.x * D.x + D.y * D.y;
@@ -828,6 +850,7 @@ static bool wins_inside_polygon(const Wins& w, double x, double y, double r) {
 // AGeoWinstonCone2D::DistToParabola (:325-428): the side at azimuth phi,
 // in the tilted parabola frame; `open` is the accepted azimuth width
 static double wins_parabola(const Wins& w, V3 P, V3 D, double phi, double open) {
+  count_op(OP_WINS_SIDE);
   double x = std::cos(phi) * P.x + std::sin(phi) * P.y;
   double y = -std::sin(phi) * P.x + std::cos(phi) * P.y;
   double z = P.z;
@@ -948,8 +971,14 @@ static Comp comp_of(const Scene& sc, int shape) {
   return c;
 }
 
+static const int kInOp[] = {-1, OP_IN_BOX, OP_IN_TUBE, OP_IN_SPHERE, OP_IN_PARA, OP_IN_PGON, OP_IN_PGON, -1,
+                            OP_IN_ASPH, OP_IN_WINS};
+static const int kDistOp[] = {-1, OP_DIST_BOX, OP_DIST_TUBE, OP_DIST_SPHERE, OP_DIST_PARA, OP_DIST_PGON,
+                              OP_DIST_PGON, -1, -1, OP_DIST_WINS_CAP};
 static bool contains(const Scene& sc, int shape, V3 q) {
   const double* p = sc.P(shape);
+  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON && kInOp[sc.type(shape)] >= 0)
+    count_op(kInOp[sc.type(shape)]);
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return box_contains(p, q);
     case RB_SHAPE_TUBE: return tube_contains(p, q);
@@ -1067,6 +1096,8 @@ static Hit comp_dist_in(const Scene& sc, int shape, V3 q, V3 d) {
 
 static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
   const double* p = sc.P(shape);
+  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON && kDistOp[sc.type(shape)] >= 0)
+    count_op(kDistOp[sc.type(shape)]);
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return {box_dist_out(p, q, d), 0};
     case RB_SHAPE_TUBE: return {tube_dist_out(p, q, d), 0};
@@ -1081,6 +1112,8 @@ static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
 }
 static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
   const double* p = sc.P(shape);
+  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON && kDistOp[sc.type(shape)] >= 0)
+    count_op(kDistOp[sc.type(shape)]);
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return {box_dist_in(p, q, d), 0};
     case RB_SHAPE_TUBE: return {tube_dist_in(p, q, d), 0};
@@ -1095,6 +1128,7 @@ static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
 }
 // normal of the surface `leaf` of `shape` at q, oriented along d
 static V3 shape_normal(const Scene& sc, int shape, uint32_t leaf, V3 q, V3 d) {
+  count_op(OP_NORMAL);
   int depth = 0;
   int maxdepth = int(leaf >> 24);
   // walk composite path; matrices accumulate as nested master->local maps
@@ -1245,6 +1279,7 @@ static bool box_has_point(const Box& b, V3 p) {
 }
 // slab test: can the ray meet the box before `limit`?
 static bool box_hit(const Box& b, V3 p, V3 d, double limit) {
+  count_op(OP_BOX_TEST);
   double t0 = 0, t1 = limit + 1e-6 * (1 + limit);
   double o[3] = {p.x, p.y, p.z}, v[3] = {d.x, d.y, d.z};
   for (int k = 0; k < 3; k++) {
@@ -1329,6 +1364,7 @@ static double locate_push(const Scene& sc, int n, double step) {
 
 // TGeoNavigator::FindNextBoundaryAndStep(); returns the next node (-1 = null)
 static int find_next_boundary_and_step(Nav& nv) {
+  count_op(OP_STEP);
   const Scene& sc = *nv.sc;
   double extra = nv.on_boundary ? kTol : 0.0;
   nv.on_boundary = false;
@@ -1469,6 +1505,7 @@ struct Opts {
 
 // src/AOpticsManager.cxx:170-247
 static void do_reflection(double n1, Ray& ray, Nav& nv, int next, const V3* normal, Rng& rng) {
+  count_op(OP_REFLECT);
   const Scene& sc = *nv.sc;
   double step = nv.step;
   V3 n = normal ? *normal : find_normal(nv);
@@ -1493,6 +1530,7 @@ static void do_reflection(double n1, Ray& ray, Nav& nv, int next, const V3* norm
 // src/AOpticsManager.cxx:52-167 (multilayer branch out of scope)
 static void do_fresnel(double n1, double n2, double k2, Ray& ray, Nav& nv, int next, const Opts& o,
                        Rng& rng) {
+  count_op(OP_REFRACT);
   double step = nv.step;
   V3 n = find_normal(nv);
   V3 d1 = ray.d;
@@ -1890,6 +1928,22 @@ int64_t oracle_adjoint_dual(const rb_scene_desc* desc, const rb_trace_opts* opts
     total += used[k];
   }
   return total;
+}
+
+// Single-threaded trace that also counts the operations it performs
+// (OP_COUNT counters added into `counts`); the FLOP figure's input.
+int oracle_trace_counts(const rb_scene_desc* desc, const rb_trace_opts* opts, rb_rays* rays, int64_t n,
+                        uint64_t* counts, int n_counts) {
+  if (!desc || !opts || !rays || !counts || n < 0 || n_counts < OP_COUNT) return -1;
+  Scene sc{desc, {}};
+  build_boxes(sc);
+  Opts o{opts->limit > 0 ? opts->limit : 100, opts->fresnel != 0};
+  std::vector<uint64_t> c(OP_COUNT, 0);
+  g_ops = c.data();
+  trace_range(sc, o, opts, rays, 0, n);
+  g_ops = nullptr;
+  for (int k = 0; k < OP_COUNT; k++) counts[k] += c[k];
+  return OP_COUNT;
 }
 
 // Shape-level probes for tests (local frame of `shape`)

@@ -284,3 +284,19 @@ def test_c3_schwarzschild_couder_scene_traces():
     # focused rays end on the focal cap: z within its sag range
     assert soa["z"][f].max() <= configs.C3_ZF + 1e-9
     assert soa["z"][f].min() >= configs.C3_ZF - 0.1 - configs.C3_FOCAL_R ** 2 / 100.0
+
+
+def test_operation_counts_do_not_change_the_trace_and_count_reflections():
+    """The FLOP figure's instrumented trace: same results as the plain trace,
+    and on-axis C1 rays reflect exactly once per focused ray."""
+    mgr = configs.c1_scene()
+    soa = configs.c1_rays(0.0, nside=64).soa()
+    a = {k: v.copy() for k, v in soa.items()}
+    b = {k: v.copy() for k, v in soa.items()}
+    O.trace(mgr.flat(), a, mgr.trace_opts())
+    counts = O.trace_counts(mgr.flat(), b, mgr.trace_opts())
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert counts["reflect"] == int((a["status"] == A.RB_FOCUS).sum())
+    assert counts["normal"] == counts["reflect"]
+    assert counts["dist_paraboloid"] > 0 and counts["dist_sphere"] == 0
