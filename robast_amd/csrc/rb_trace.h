@@ -538,13 +538,19 @@ __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& i
 }
 
 // AOpticsManager::TraceNonSequential(TObjArray*) for one ray (src/AOpticsManager.cxx:353-518)
-template <bool MIXED, typename Rec>
-__device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
-  const uint64_t id = P.id_base + uint64_t(i);
+// Locate the ray's starting point (TGeoNavigator::InitTrack).
+__device__ __forceinline__ void begin_lane(const DScene& S, Lane& L) {
   L.on_b = false;
   L.cur = node_contains(S, 0, L.p) ? search_node(S, 0, L.p, -1, true) : -1;
+}
+
+// One segment of the per-ray state machine: next boundary, then the
+// interaction the (current, next) volume kinds select.
+template <bool MIXED, typename Rec>
+__device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
+  const uint64_t id = P.id_base + uint64_t(i);
   const double lambda = L.lambda;
-  while (L.status == RB_RUN) {
+  {
     const V3<double> x1 = L.p;
     const double t1 = L.t;
     const int current = L.cur;
@@ -563,7 +569,7 @@ __device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& i
           L.p = axpy(x1, abs_step, L.d);
           add_point(io, i, L, L.p, t1 + abs_step / (kC / n1), next);
           L.status = RB_ABSORB;
-          continue;
+          return;
         }
       }
     }
@@ -603,6 +609,12 @@ __device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& i
     }
     if (L.status == RB_RUN && L.npoints >= P.limit) L.status = RB_SUSPEND;
   }
+}
+
+template <bool MIXED, typename Rec>
+__device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
+  begin_lane(S, L);
+  while (L.status == RB_RUN) step_lane<MIXED>(S, P, io, i, L, rec);
 }
 
 }  // namespace rbd

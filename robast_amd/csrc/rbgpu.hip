@@ -18,6 +18,8 @@ using namespace rbd;
 
 struct rb_ctx {
   int device = 0;
+  int cus = 0;                                // compute units (persistent grid size)
+  unsigned long long* next_ray = nullptr;     // persistent kernels' ray counter (device)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
@@ -69,6 +71,12 @@ struct DevRays {
 // Block size of the tracing kernels: the LDS candidate slots are laid out
 // for exactly this many lanes.
 constexpr int kTraceBlock = kCandStride;
+// Trace schedule: persistent grid with per-lane ray fetch (1) or one lane
+// per ray over the whole batch (0).
+#ifndef RB_PERSISTENT
+#define RB_PERSISTENT 0
+#endif
+constexpr bool kPersistent = RB_PERSISTENT != 0;
 // Occupancy request for the tracing kernels (waves per SIMD); 0 leaves the
 // register budget to the compiler. Two waves (256 registers per lane, the
 // rest spilled) measured faster than one wave with 512 on MI355X: the
@@ -82,15 +90,7 @@ constexpr int kTraceBlock = kCandStride;
 #define RB_TRACE_ATTR
 #endif
 
-template <bool MIXED>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, RayIO io, int64_t n, TraceParams P) {
-  __shared__ int32_t cand_node[kCand * kCandStride];
-  __shared__ float cand_tn[kCand * kCandStride];
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (io.status[i] != RB_RUN) return;
-  Lane L;
-  L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
+__device__ __forceinline__ void load_lane(const RayIO& io, int64_t i, Lane& L) {
   L.p = mk(io.x[i], io.y[i], io.z[i]);
   L.d = mk(io.dx[i], io.dy[i], io.dz[i]);
   L.t = io.t[i];
@@ -102,8 +102,8 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, R
   L.step = 0;
   L.hit_node = -1;
   L.hit_leaf = 0;
-  NoRecord rec;
-  trace_lane<MIXED>(S, P, io, i, L, rec);
+}
+__device__ __forceinline__ void store_lane(const RayIO& io, int64_t i, const Lane& L) {
   io.x[i] = L.p.x;
   io.y[i] = L.p.y;
   io.z[i] = L.p.z;
@@ -114,6 +114,74 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, R
   io.status[i] = L.status;
   io.npoints[i] = L.npoints;
   if (io.last_node) io.last_node[i] = L.last_node;
+}
+
+// Persistent form: a grid sized to the machine, each lane taking the next
+// ray from a global counter as soon as its current one terminates, so a
+// wave's lanes stay busy instead of waiting for the longest path among 64
+// fixed rays. The counter is claimed once per wave (one vector atomic for
+// all lanes that need a ray). A ray is traced from start to end by the lane
+// that claimed it, with the same per-ray Philox stream, so results equal
+// k_trace's. Every wave leaves when the counter has passed n and none of
+// its lanes holds a ray.
+template <bool MIXED>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DScene S, RayIO io, int64_t n,
+                                                                          TraceParams P,
+                                                                          unsigned long long* next_ray) {
+  __shared__ int32_t cand_node[kCand * kCandStride];
+  __shared__ float cand_tn[kCand * kCandStride];
+  Lane L;
+  L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
+  NoRecord rec;
+  int64_t i = 0;
+  bool active = false, exhausted = false;
+  for (;;) {
+    const bool need = !active && !exhausted;
+    const unsigned long long want = __ballot(need);
+    if (want != 0ull) {
+      const int leader = __ffsll(want) - 1;
+      const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(want >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(want), 0u));
+      unsigned long long base = 0;
+      if (need && rank == 0) base = atomicAdd(next_ray, (unsigned long long)__popcll(want));
+      const unsigned lo = __shfl(unsigned(base), leader), hi = __shfl(unsigned(base >> 32), leader);
+      if (need) {
+        i = int64_t((static_cast<unsigned long long>(hi) << 32) | lo) + rank;
+        if (i >= n) {
+          exhausted = true;
+        } else if (io.status[i] == RB_RUN) {
+          load_lane(io, i, L);
+          begin_lane(S, L);
+          active = true;
+        }
+      }
+    }
+    if (!__any(active)) {
+      if (!__any(!exhausted)) break;
+      continue;
+    }
+    if (active) {
+      step_lane<MIXED>(S, P, io, i, L, rec);
+      if (L.status != RB_RUN) {
+        store_lane(io, i, L);
+        active = false;
+      }
+    }
+  }
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, RayIO io, int64_t n, TraceParams P) {
+  __shared__ int32_t cand_node[kCand * kCandStride];
+  __shared__ float cand_tn[kCand * kCandStride];
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (io.status[i] != RB_RUN) return;
+  Lane L;
+  L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
+  load_lane(io, i, L);
+  NoRecord rec;
+  trace_lane<MIXED>(S, P, io, i, L, rec);
+  store_lane(io, i, L);
 }
 
 // Adjoint of the focal-plane loss: each lane recomputes its ray's bounce
@@ -237,6 +305,7 @@ int rbgpu_ctx_create(int device, rb_ctx** out) {
 void rbgpu_ctx_destroy(rb_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  if (c->next_ray) (void)hipFree(c->next_ray);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -355,11 +424,32 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
 
   const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
+  const bool mixed = opts->precision == RB_PRECISION_MIXED;
+  if (kPersistent && !ctx->next_ray) HIPCHK(ctx, hipMalloc(&ctx->next_ray, sizeof(unsigned long long)));
+  if (kPersistent && ctx->cus == 0) {
+    hipDeviceProp_t prop;
+    HIPCHK(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    ctx->cus = prop.multiProcessorCount;
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  if (opts->precision == RB_PRECISION_MIXED)
+  if (kPersistent) {
+    int per_cu = 0;
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, mixed ? (const void*)k_trace_persist<true> : (const void*)k_trace_persist<false>,
+                    block, 0));
+    const int64_t pgrid = std::min<int64_t>(grid, int64_t(std::max(per_cu, 1)) * ctx->cus);
+    HIPCHK(ctx, hipMemsetAsync(ctx->next_ray, 0, sizeof(unsigned long long), ctx->stream));
+    if (mixed)
+      hipLaunchKernelGGL(k_trace_persist<true>, dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream, scene->s, io, n,
+                         P, ctx->next_ray);
+    else
+      hipLaunchKernelGGL(k_trace_persist<false>, dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream, scene->s, io,
+                         n, P, ctx->next_ray);
+  } else if (mixed) {
     hipLaunchKernelGGL(k_trace<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P);
-  else
+  } else {
     hipLaunchKernelGGL(k_trace<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P);
+  }
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   ctx->launches = 1;
