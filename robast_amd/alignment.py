@@ -79,6 +79,61 @@ def c5_raygen(angle_deg, index):
     return g
 
 
+class FacetSceneUpdater:
+    """Moves an existing per-facet scene to new parameter values by patching
+    its flattened arrays in place (facet rotations, sphere radii, the
+    sphere-to-cap translation and the affected bounding boxes), through the
+    same shape code a rebuild uses, so the arrays equal a rebuild's bit for
+    bit. Tip/tilt act as M = M0 Rx(tip) Ry(tilt) on the facet's placement."""
+
+    def __init__(self, mgr, nfacets=C5_NFACETS):
+        self.flat = flat = mgr.flat()
+        self.nfacets = nfacets
+        index = {name: k for k, name in enumerate(flat.node_names)}
+        self.nodes = [index[f"mirror{i}_{i + 1}"] for i in range(nfacets)]
+        self.base_rot = [np.asarray(configs._dc_facet_matrix(x, y).rot) for x, y in configs.dc_facet_positions()]
+        self.comp = []      # (composite object, composite shape index, sphere shape index)
+        for k in self.nodes:
+            v = int(flat.nodes[k]["volume"])
+            cs = int(flat.volumes[v]["shape"])
+            sph = int(flat.params[flat.shapes[cs]["param_offset"] + 1])
+            self.comp.append((flat.volume_objs[v].shape, cs, sph))
+
+    def _put_shape(self, idx, sh, params):
+        f = self.flat
+        off, n = int(f.shapes[idx]["param_offset"]), int(f.shapes[idx]["n_params"])
+        assert len(params) == n
+        for j, v in enumerate(params):
+            if v is not None:
+                f.params[off + j] = v
+        org, half = sh.local_bbox()
+        f.shapes[idx]["bbox_origin"] = org
+        f.shapes[idx]["bbox_half"] = half
+
+    def update(self, params):
+        p = np.asarray(params, dtype=float)
+        nf = self.nfacets
+        f = self.flat
+        for i, k in enumerate(self.nodes):
+            rot = self.base_rot[i] @ configs._small_rotation(float(p[2 * i]), float(p[2 * i + 1]))
+            f.nodes[k]["rot"] = rot.ravel()
+            comp, cs, sph = self.comp[i]
+            s = comp.node.left
+            R = float(p[2 * nf + i])
+            if s.rmin != R:
+                s.rmin, s.rmax = R, R + configs.C2_MIRROR_T
+                s.theta1 = 180.0 - math.asin(configs.C2_MIRROR_D / math.sqrt(3) / R) * 180.0 / math.pi
+                comp.node.lmat.tr = np.array([0.0, 0.0, R])
+                self._put_shape(sph, s, s.params())
+                self._put_shape(cs, comp, comp.params())
+        cam = f.node_names.index("focalPlane_1")
+        obs = f.node_names.index("cameraBox_1")
+        zc = configs.C2_F + float(p[3 * nf])
+        f.nodes[cam]["tr"] = [0.0, 0.0, zc + 1 * configs.mm]
+        f.nodes[obs]["tr"] = [0.0, 0.0, zc + 2 * configs.mm + 1 * configs.cm + 5 * configs.cm]
+        return f
+
+
 def node_param_table(flat, nfacets=C5_NFACETS):
     """[n_nodes][3] int32: the (tip, tilt, R) parameter indices of each facet
     node, -1 elsewhere; plus the focal-plane node index."""
@@ -145,11 +200,13 @@ class DaviesCottonAlignment:
         p = np.asarray(params, dtype=float)
         assert p.shape == (self.n_params,)
         self.params = p.copy()
-        self.mgr = c5_scene(p, self.nfacets)
-        flat = self.mgr.flat()
+        if self.mgr is None:
+            self.mgr = c5_scene(p, self.nfacets)
+            self.updater = FacetSceneUpdater(self.mgr, self.nfacets)
+            table, self.focal_node = node_param_table(self.mgr.flat(), self.nfacets)
+            self.node_param = torch.from_numpy(table).to(self.dev)
+        flat = self.updater.update(p)
         self.scene = self.ctx.upload(flat)
-        table, self.focal_node = node_param_table(flat, self.nfacets)
-        self.node_param = torch.from_numpy(table).to(self.dev)
         self.values = torch.from_numpy(p).to(self.dev)
 
     def loss(self, params):

@@ -242,3 +242,23 @@ def test_gpu_adam_fit_reduces_spot(ctx):
     print(f"  loss {hist[0]:.4f} -> {hist[-1]:.4f} cm^2", flush=True)
     assert hist[-1] < 0.5 * hist[0]
     assert np.abs(p[:2 * NF]).max() < 5e-3
+
+
+def test_scene_updater_equals_rebuild():
+    """Patching the flattened scene to new parameters gives the same arrays,
+    bit for bit, as building the scene at those parameters."""
+    p0 = AL.c5_initial_params()
+    mgr = AL.c5_scene(p0)
+    up = AL.FacetSceneUpdater(mgr)
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        p = p0.copy()
+        p[:2 * NF] += rng.normal(0, 1e-3, 2 * NF)
+        p[2 * NF:3 * NF] *= 1 + rng.uniform(-0.02, 0.02, NF)
+        p[ZP] = rng.normal(0, 2.0)
+        got = up.update(p)
+        want = AL.c5_scene(p).flat()
+        for name in ("nodes", "children", "volumes", "shapes", "params", "tables", "table_xy"):
+            a, b = getattr(got, name), getattr(want, name)
+            assert a.dtype == b.dtype and a.shape == b.shape, name
+            assert a.tobytes() == b.tobytes(), name
