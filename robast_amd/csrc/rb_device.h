@@ -902,12 +902,14 @@ template <typename T> __device__ V3<T> wins_normal(const double* p, V3<T> q, V3<
   return nn;
 }
 
-// Call structure of the shape code. Each level is a real call by default:
-// it keeps the megakernel's register allocation per function. A build can
-// inline a level (-DRB_PRIM_FN=__forceinline__ ...) to trade registers for
-// call overhead.
+// Call structure of the shape code. The node-level queries and the
+// primitive dispatch are inlined; the composite levels (Csg<T, D>) stay real
+// calls, which keeps each boolean level's register allocation to itself.
+// Measured on MI355X (85-facet Davies-Cotton, 1e7 rays, FP64): this split
+// 15.2 ms; composite level 0 inlined too 20.3 ms; everything inlined 27.5 ms;
+// nothing inlined 19.6 ms. A build can override each level.
 #ifndef RB_PRIM_FN
-#define RB_PRIM_FN __noinline__
+#define RB_PRIM_FN __forceinline__
 #endif
 #ifndef RB_CSG0_FN
 #define RB_CSG0_FN __noinline__
@@ -916,7 +918,7 @@ template <typename T> __device__ V3<T> wins_normal(const double* p, V3<T> q, V3<
 #define RB_CSG_FN __noinline__
 #endif
 #ifndef RB_NODE_FN
-#define RB_NODE_FN __noinline__
+#define RB_NODE_FN __forceinline__
 #endif
 
 // ------------------------------------------------------------ dispatch
