@@ -170,7 +170,17 @@ typedef struct {
   int32_t device_ptrs; /* 1 = rb_rays arrays are device pointers already in HBM */
   uint64_t seed;       /* Philox4x32-10 key */
   uint64_t ray_id_base;/* global id of ray 0 (Philox counter), for sharded runs */
+  int32_t order;       /* RB_ORDER_*: the order rays are traced in (results do not depend on it) */
+  int32_t pad;
 } rb_trace_opts;
+
+/* Trace order. Every ray keeps its own Philox stream (keyed by its index),
+ * so per-ray results are the same in any order; the order only changes how
+ * coherent the rays sharing a wavefront are. COHERENT sorts running rays by
+ * a Morton code of the point where each first enters a daughter box of the
+ * top volume, and traces them in that order through a permutation. AUTO
+ * sorts batches of 2^20 rays or more. */
+enum { RB_ORDER_AUTO = 0, RB_ORDER_INPUT = 1, RB_ORDER_COHERENT = 2 };
 
 /* ---- synthetic ray sources (device-side ARayShooter equivalents) ---- */
 enum {
@@ -252,14 +262,19 @@ typedef struct {
 int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const rb_rays* rays, int64_t n,
                   const rb_adjoint_desc* adj, double* grad);
 
-/* Milliseconds the last rbgpu_trace spent in its trace kernel(s), timed with
- * HIP events on the context's stream, and the number of kernel launches. */
+/* Milliseconds the last rbgpu_trace (or rbgpu_adjoint) spent on the device,
+ * ordering included, timed with HIP events on the context's stream, and the
+ * number of kernel launches it made. */
 double rbgpu_last_trace_ms(const rb_ctx* ctx);
+/* Milliseconds of the last call's trace (or adjoint) kernel alone, without
+ * the ordering kernels. No counterpart in the reference (measurement). */
+double rbgpu_last_kernel_ms(const rb_ctx* ctx);
 int rbgpu_last_trace_launches(const rb_ctx* ctx);
 
 /* Writes sizeof() of the ABI structs (rb_matrix, rb_shape_desc,
  * rb_volume_desc, rb_node_desc, rb_table_desc, rb_scene_desc, rb_rays,
- * rb_trace_opts, rb_raygen_desc) into out[0..8] for binding checks; returns 9.
+ * rb_trace_opts, rb_raygen_desc, rb_adjoint_desc) into out[0..9] for binding
+ * checks; returns 10.
  * No counterpart in the reference (binding self-check only). */
 int rbgpu_abi_sizes(int32_t* out);
 

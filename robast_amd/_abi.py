@@ -30,6 +30,8 @@ RB_INDEX_NONE, RB_INDEX_TABLE, RB_INDEX_SELLMEIER, RB_INDEX_SCHOTT = 0, 1, 2, 3
 
 RB_PRECISION_FP64, RB_PRECISION_MIXED = 0, 1
 
+RB_ORDER_AUTO, RB_ORDER_INPUT, RB_ORDER_COHERENT = 0, 1, 2
+
 RB_RAYGEN_GRID, RB_RAYGEN_DISK, RB_RAYGEN_CONE = 0, 1, 2
 
 SHAPE_DTYPE = np.dtype(
@@ -80,6 +82,7 @@ class rb_trace_opts(C.Structure):
     _fields_ = [
         ("limit", C.c_int32), ("fresnel", C.c_int32), ("precision", C.c_int32),
         ("device_ptrs", C.c_int32), ("seed", C.c_uint64), ("ray_id_base", C.c_uint64),
+        ("order", C.c_int32), ("pad", C.c_int32),
     ]
 
 

@@ -18,7 +18,7 @@ _lib = None
 EXPORTS = (
     "rbgpu_ctx_create", "rbgpu_ctx_destroy", "rbgpu_last_error", "rbgpu_scene_upload",
     "rbgpu_scene_free", "rbgpu_trace", "rbgpu_generate_rays", "rbgpu_last_trace_ms",
-    "rbgpu_last_trace_launches", "rbgpu_abi_sizes", "rbgpu_adjoint",
+    "rbgpu_last_trace_launches", "rbgpu_abi_sizes", "rbgpu_adjoint", "rbgpu_last_kernel_ms",
 )
 
 
@@ -53,6 +53,8 @@ def load():
     lib.rbgpu_generate_rays.restype = C.c_int
     lib.rbgpu_last_trace_ms.argtypes = [C.c_void_p]
     lib.rbgpu_last_trace_ms.restype = C.c_double
+    lib.rbgpu_last_kernel_ms.argtypes = [C.c_void_p]
+    lib.rbgpu_last_kernel_ms.restype = C.c_double
     lib.rbgpu_last_trace_launches.argtypes = [C.c_void_p]
     lib.rbgpu_last_trace_launches.restype = C.c_int
     lib.rbgpu_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays), C.c_int64,
@@ -103,6 +105,9 @@ class Context:
 
     def last_trace_ms(self):
         return self.lib.rbgpu_last_trace_ms(self.h)
+
+    def last_kernel_ms(self):
+        return self.lib.rbgpu_last_kernel_ms(self.h)
 
     def last_trace_launches(self):
         return self.lib.rbgpu_last_trace_launches(self.h)

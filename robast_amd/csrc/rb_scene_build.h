@@ -233,6 +233,8 @@ size_t put(std::vector<char>& blob, const X* src, size_t count) {
 struct SceneLayout {
   size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy;
   int32_t n_nodes;
+  bool has_top_box = false;  // the top node has daughters: their bounding box follows
+  float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -276,6 +278,14 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
   lay.tables = put(blob, d->tables, size_t(std::max(d->n_tables, 0)));
   lay.table_xy = put(blob, d->table_xy, size_t(std::max(d->n_table_xy, 0)));
   lay.n_nodes = d->n_nodes;
+  lay.has_top_box = d->n_nodes > 0 && nodes[0].bvh_root >= 0;
+  if (lay.has_top_box) {
+    const DBvh& r = bvh.nodes[size_t(nodes[0].bvh_root)];
+    for (int k = 0; k < 3; k++) {
+      lay.top_lo[k] = r.lo[k];
+      lay.top_hi[k] = r.hi[k];
+    }
+  }
   return 0;
 }
 

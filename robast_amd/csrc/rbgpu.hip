@@ -1,6 +1,7 @@
 // rbgpu.hip -- librbgpu: C ABI (include/rbgpu.h), scene upload with per-node
 // BVH build, ray sources and the trace kernel launch for gfx950.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -21,8 +22,14 @@ struct rb_ctx {
   int cus = 0;                                // compute units (persistent grid size)
   unsigned long long* next_ray = nullptr;     // persistent kernels' ray counter (device)
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
+  double last_ms = 0, kernel_ms = 0;
+  // coherence-order buffers, grown on demand: keys and ray indices (double
+  // buffered for the radix sort) and the sort's temporary storage
+  uint32_t* order_buf = nullptr;
+  size_t order_cap = 0;
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
   int launches = 0;
   std::string err;
 };
@@ -31,6 +38,8 @@ struct rb_scene {
   rb_ctx* ctx = nullptr;
   void* dmem = nullptr;
   DScene s{};
+  bool has_top_box = false;
+  float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
 };
 
 namespace {
@@ -116,6 +125,52 @@ __device__ __forceinline__ void store_lane(const RayIO& io, int64_t i, const Lan
   if (io.last_node) io.last_node[i] = L.last_node;
 }
 
+// Coherence key of each ray: a 30-bit Morton code of the point where it first
+// enters one of the top volume's daughter boxes (box tests only, over the top
+// node's BVH), quantised over the daughters' joint box. Rays that are not
+// running, or cross no daughter box, get the largest key and sort last.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n, float lox, float loy, float loz,
+                                                   float sx, float sy, float sz, uint32_t* key, uint32_t* idx) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = 0xffffffffu;
+  if (io.status[i] == RB_RUN) {
+    const float ox = float(io.x[i]), oy = float(io.y[i]), oz = float(io.z[i]);
+    const float dx = float(io.dx[i]), dy = float(io.dy[i]), dz = float(io.dz[i]);
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const int root = S.nodes[0].bvh_root;
+    float best = 3.0e38f;
+    int b = root;
+    const int end = S.bvh[root].escape;
+    while (b < end) {
+      const DBvh box = S.bvh[b];
+      float tn;
+      if (!box_pass(box, ox, oy, oz, ix, iy, iz, best, tn)) {
+        b = box.escape;
+        continue;
+      }
+      if (box.prim >= 0) best = fminf(best, tn);
+      b++;
+    }
+    if (best < 3.0e38f) {
+      const float px = (ox + best * dx - lox) * sx, py = (oy + best * dy - loy) * sy, pz = (oz + best * dz - loz) * sz;
+      const uint32_t qx = uint32_t(fminf(fmaxf(px, 0.0f), 1023.0f)), qy = uint32_t(fminf(fmaxf(py, 0.0f), 1023.0f)),
+                     qz = uint32_t(fminf(fmaxf(pz, 0.0f), 1023.0f));
+      k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
+    }
+  }
+  key[i] = k;
+  idx[i] = uint32_t(i);
+}
+
 // Persistent form: a grid sized to the machine, each lane taking the next
 // ray from a global counter as soon as its current one terminates, so a
 // wave's lanes stay busy instead of waiting for the longest path among 64
@@ -169,12 +224,16 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
   }
 }
 
+// One lane per ray. With `perm`, lane t traces ray perm[t] (the coherence
+// order); the ray keeps its own index for its Philox stream and its slots.
 template <bool MIXED>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, RayIO io, int64_t n, TraceParams P) {
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, RayIO io, int64_t n, TraceParams P,
+                                                                  const uint32_t* perm) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t i = perm ? int64_t(perm[t]) : t;
   if (io.status[i] != RB_RUN) return;
   Lane L;
   L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
@@ -189,13 +248,14 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, R
 // gradients in LDS (FP64), flushed with one global atomic per parameter.
 template <bool MIXED>
 __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DScene S, RayIO io, int64_t n, TraceParams P, AdjParams A,
-                                                         double* grad) {
+                                                         double* grad, const uint32_t* perm) {
   __shared__ double acc[kMaxAdjParams];
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   for (int k = threadIdx.x; k < A.n_params; k += blockDim.x) acc[k] = 0.0;
   __syncthreads();
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = t < n ? (perm ? int64_t(perm[t]) : t) : n;
   if (i < n && io.status[i] == RB_RUN) {
     Lane L;
     L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
@@ -283,6 +343,50 @@ __global__ void __launch_bounds__(256) k_generate(rb_raygen_desc g, RayIO io, in
 }
 
 // ---------------------------------------------------------------- C ABI
+// Batches at least this large are traced in coherence order by default.
+constexpr int64_t kAutoOrderMin = int64_t(1) << 20;
+
+// Coherence order of n rays: keys from k_order_key, then a radix sort of
+// (key, index) pairs. *perm receives the sorted indices (device).
+static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm) {
+  if (size_t(n) > ctx->order_cap) {
+    if (ctx->order_buf) HIPCHK(ctx, hipFree(ctx->order_buf));
+    ctx->order_buf = nullptr;
+    ctx->order_cap = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->order_buf, 4 * sizeof(uint32_t) * size_t(n)));
+    ctx->order_cap = size_t(n);
+  }
+  uint32_t* key_in = ctx->order_buf;
+  uint32_t* key_out = key_in + ctx->order_cap;
+  uint32_t* idx_in = key_out + ctx->order_cap;
+  uint32_t* idx_out = idx_in + ctx->order_cap;
+  const unsigned count = unsigned(n);
+  size_t bytes = 0;
+  HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, idx_in, idx_out, count, 0u, 32u,
+                                        ctx->stream));
+  if (bytes > ctx->sort_tmp_bytes) {
+    if (ctx->sort_tmp) HIPCHK(ctx, hipFree(ctx->sort_tmp));
+    ctx->sort_tmp = nullptr;
+    ctx->sort_tmp_bytes = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->sort_tmp, bytes));
+    ctx->sort_tmp_bytes = bytes;
+  }
+  float s[3];
+  for (int k = 0; k < 3; k++) {
+    const float ext = scene->top_hi[k] - scene->top_lo[k];
+    s[k] = ext > 0.0f ? 1023.0f / ext : 0.0f;
+  }
+  const int64_t grid = (n + 255) / 256;
+  hipLaunchKernelGGL(k_order_key, dim3(unsigned(grid)), dim3(256), 0, ctx->stream, scene->s, io, n, scene->top_lo[0],
+                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], key_in, idx_in);
+  HIPCHK(ctx, hipGetLastError());
+  bytes = ctx->sort_tmp_bytes;
+  HIPCHK(ctx, rocprim::radix_sort_pairs(ctx->sort_tmp, bytes, key_in, key_out, idx_in, idx_out, count, 0u, 32u,
+                                        ctx->stream));
+  *perm = idx_out;
+  return 0;
+}
+
 extern "C" {
 
 int rbgpu_ctx_create(int device, rb_ctx** out) {
@@ -294,7 +398,8 @@ int rbgpu_ctx_create(int device, rb_ctx** out) {
   rb_ctx* c = new rb_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess) {
     delete c;
     return -2;
   }
@@ -306,6 +411,10 @@ void rbgpu_ctx_destroy(rb_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->next_ray) (void)hipFree(c->next_ray);
+  if (c->order_buf) (void)hipFree(c->order_buf);
+  if (c->sort_tmp) (void)hipFree(c->sort_tmp);
+  if (c->evk0) (void)hipEventDestroy(c->evk0);
+  if (c->evk1) (void)hipEventDestroy(c->evk1);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -324,7 +433,8 @@ int rbgpu_abi_sizes(int32_t* out) {
   out[6] = sizeof(rb_rays);
   out[7] = sizeof(rb_trace_opts);
   out[8] = sizeof(rb_raygen_desc);
-  return 9;
+  out[9] = sizeof(rb_adjoint_desc);
+  return 10;
 }
 
 int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
@@ -350,6 +460,11 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
     return fail(ctx, -2, std::string("hipMemcpy scene: ") + hipGetErrorString(e));
   }
   sc->s = scene_view(static_cast<const char*>(sc->dmem), lay);
+  sc->has_top_box = lay.has_top_box;
+  for (int k = 0; k < 3; k++) {
+    sc->top_lo[k] = lay.top_lo[k];
+    sc->top_hi[k] = lay.top_hi[k];
+  }
   *out = sc;
   return 0;
 }
@@ -375,6 +490,7 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
     return fail(ctx, -1, "rbgpu_trace: bad path record");
   ctx->launches = 0;
   ctx->last_ms = 0;
+  ctx->kernel_ms = 0;
   if (n == 0) return 0;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   TraceParams P;
@@ -425,13 +541,21 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
   const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
   const bool mixed = opts->precision == RB_PRECISION_MIXED;
+  const bool coherent = !kPersistent && scene->has_top_box && n < (int64_t(1) << 31) &&
+                        (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
+  const uint32_t* perm = nullptr;
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (coherent) {
+    const int rc = order_rays(ctx, scene, io, n, &perm);
+    if (rc) return rc;
+  }
   if (kPersistent && !ctx->next_ray) HIPCHK(ctx, hipMalloc(&ctx->next_ray, sizeof(unsigned long long)));
   if (kPersistent && ctx->cus == 0) {
     hipDeviceProp_t prop;
     HIPCHK(ctx, hipGetDeviceProperties(&prop, ctx->device));
     ctx->cus = prop.multiProcessorCount;
   }
-  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
   if (kPersistent) {
     int per_cu = 0;
     HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -446,13 +570,14 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
       hipLaunchKernelGGL(k_trace_persist<false>, dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream, scene->s, io,
                          n, P, ctx->next_ray);
   } else if (mixed) {
-    hipLaunchKernelGGL(k_trace<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P);
+    hipLaunchKernelGGL(k_trace<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, perm);
   } else {
-    hipLaunchKernelGGL(k_trace<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P);
+    hipLaunchKernelGGL(k_trace<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, perm);
   }
   HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  ctx->launches = 1;
+  ctx->launches = coherent ? 3 : 1;  // key kernel + radix sort (counted once) + trace
 
   if (host) {
     double* hs[7] = {rays->x, rays->y, rays->z, rays->t, rays->dx, rays->dy, rays->dz};
@@ -468,9 +593,11 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
     }
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  float ms = 0;
+  float ms = 0, kms = 0;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  HIPCHK(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
   ctx->last_ms = ms;
+  ctx->kernel_ms = kms;
   return 0;
 }
 
@@ -506,6 +633,7 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
     return fail(ctx, -1, "rbgpu_adjoint: z parameter or focal node out of range");
   ctx->launches = 0;
   ctx->last_ms = 0;
+  ctx->kernel_ms = 0;
   if (n == 0) return 0;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   TraceParams P;
@@ -529,20 +657,35 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   A.weight = adj->weight;
   const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
+  const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
+                        (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
+  const uint32_t* perm = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (coherent) {
+    const int rc = order_rays(ctx, scene, io, n, &perm);
+    if (rc) return rc;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
   if (opts->precision == RB_PRECISION_MIXED)
-    hipLaunchKernelGGL(k_adjoint<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, A, grad);
+    hipLaunchKernelGGL(k_adjoint<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, A, grad,
+                       perm);
   else
-    hipLaunchKernelGGL(k_adjoint<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, A, grad);
+    hipLaunchKernelGGL(k_adjoint<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, A,
+                       grad, perm);
   HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  float ms = 0;
+  float ms = 0, kms = 0;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  HIPCHK(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
   ctx->last_ms = ms;
-  ctx->launches = 1;
+  ctx->kernel_ms = kms;
+  ctx->launches = coherent ? 3 : 1;
   return 0;
 }
+
+double rbgpu_last_kernel_ms(const rb_ctx* c) { return c ? c->kernel_ms : 0.0; }
 
 double rbgpu_last_trace_ms(const rb_ctx* c) { return c ? c->last_ms : 0.0; }
 int rbgpu_last_trace_launches(const rb_ctx* c) { return c ? c->launches : 0; }

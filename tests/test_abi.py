@@ -30,8 +30,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_binding():
     lib = _lib.load()
     out = (C.c_int32 * 16)()
-    assert lib.rbgpu_abi_sizes(out) == 9
-    sizes = list(out)[:9]
+    assert lib.rbgpu_abi_sizes(out) == 10
+    sizes = list(out)[:10]
     assert sizes[0] == 96                              # rb_matrix
     assert sizes[1] == A.SHAPE_DTYPE.itemsize
     assert sizes[2] == A.VOLUME_DTYPE.itemsize
@@ -41,6 +41,8 @@ def test_struct_sizes_match_binding():
     assert sizes[6] == C.sizeof(A.rb_rays)
     assert sizes[7] == C.sizeof(A.rb_trace_opts)
     assert sizes[8] == C.sizeof(A.rb_raygen_desc)
+    assert sizes[9] == C.sizeof(A.rb_adjoint_desc)
+    assert C.sizeof(A.rb_trace_opts) == 40
 
 
 def test_null_context_is_an_error_not_a_crash():

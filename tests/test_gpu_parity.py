@@ -204,3 +204,28 @@ def test_c4_light_guides_sample_per_ray(ctx):
     c = counts(g["status"])
     assert c[A.RB_FOCUS] > 50000 and c[A.RB_ABSORB] > 0
     assert g["npoints"].max() > 10
+
+
+@pytest.mark.parametrize("config", ["c2", "c4"])
+def test_coherence_order_gives_identical_results(ctx, config):
+    """Tracing in coherence order (sorted by first daughter-box entry, traced
+    through a permutation) and in input order give bit-identical rays."""
+    import torch
+    print(f"order input vs coherent {config}", flush=True)
+    N = 1 << 20
+    mgr = {"c2": configs.c2_scene, "c4": configs.c4_scene}[config]()
+    gen = configs.c2_raygen() if config == "c2" else configs.c4_raygen(mgr.c4_dz)
+    scene = ctx.upload(mgr.flat())
+    dev = torch.device("cuda")
+    out = {}
+    for order in (A.RB_ORDER_INPUT, A.RB_ORDER_COHERENT):
+        t = {f: torch.empty(N, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_")}
+        for f in ("status", "npoints", "last_node"):
+            t[f] = torch.empty(N, dtype=torch.int32, device=dev)
+        ctx.generate(gen, make_rays_device(t), N)
+        t["status"][::7] = A.RB_STOP   # some rays not running: left untouched in both orders
+        ctx.trace(scene, make_rays_device(t), N, mgr.trace_opts(device_ptrs=1, order=order))
+        assert ctx.last_trace_launches() == (3 if order == A.RB_ORDER_COHERENT else 1)
+        out[order] = t
+    for k in out[A.RB_ORDER_INPUT]:
+        assert torch.equal(out[A.RB_ORDER_INPUT][k], out[A.RB_ORDER_COHERENT][k]), k
