@@ -41,6 +41,7 @@ struct Cand {
 
 struct Lane {
   Cand cand;          // this lane's candidate slots
+  uint64_t ray;       // Philox counter id of the ray (ray_id_base + its index)
   V3<double> p, d;    // navigator point (== last recorded point between segments), unit direction
   double t, lambda;
   int32_t cur;        // current node, -1 = outside the top volume
@@ -548,7 +549,7 @@ __device__ __forceinline__ void begin_lane(const DScene& S, Lane& L) {
 // interaction the (current, next) volume kinds select.
 template <bool MIXED, typename Rec>
 __device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
-  const uint64_t id = P.id_base + uint64_t(i);
+  const uint64_t id = L.ray;
   const double lambda = L.lambda;
   {
     const V3<double> x1 = L.p;
