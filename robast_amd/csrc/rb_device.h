@@ -19,6 +19,13 @@
 
 #include "../../include/rbgpu.h"
 
+// The aspheric and Winston-cone solvers (Newton iterations, per-side
+// quadratics) are the register-heaviest primitives; kept as calls
+// (RB_HEAVY_FN) they do not size the frame of every primitive dispatch.
+#ifndef RB_HEAVY_FN
+#define RB_HEAVY_FN
+#endif
+
 namespace rbd {
 
 constexpr double kBig = 1e30;        // TGeoShape::Big()
@@ -714,7 +721,7 @@ __device__ T asph_wall(const double* p, bool outer, V3<T> P, V3<T> D) {
   }
   return T(kBig);
 }
-template <typename T> __device__ T asph_dist(const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ RB_HEAVY_FN T asph_dist(const double* p, V3<T> q, V3<T> d) {
   T a = asph_surface<T>(p, 0, q, d), b = asph_surface<T>(p, 1, q, d);
   T c = asph_wall<T>(p, false, q, d), e = asph_wall<T>(p, true, q, d);
   T m = a;
@@ -723,14 +730,14 @@ template <typename T> __device__ T asph_dist(const double* p, V3<T> q, V3<T> d) 
   m = e < m ? e : m;
   return m;
 }
-template <typename T> __device__ bool asph_contains(const double* p, V3<T> q) {
+template <typename T> __device__ RB_HEAVY_FN bool asph_contains(const double* p, V3<T> q) {
   T r = sqrt(q.x * q.x + q.y * q.y);
   if (r > T(p[4]) || r < T(p[5])) return false;
   T f1, f2;
   if (!asph_sag<T>(p, 0, r, f1) || !asph_sag<T>(p, 1, r, f2)) return false;
   return !(q.z < f1 || f2 < q.z);
 }
-template <typename T> __device__ V3<T> asph_normal(const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ RB_HEAVY_FN V3<T> asph_normal(const double* p, V3<T> q, V3<T> d) {
   T r = sqrt(q.x * q.x + q.y * q.y);
   T saf0 = T(p[5]) > T(0) ? fabs(r - T(p[5])) : T(kBig);
   T saf1 = fabs(r - T(p[4]));
@@ -841,13 +848,13 @@ __device__ T wins_parabola(const double* p, int i, V3<T> P, V3<T> D, bool half_p
   }
   return best;
 }
-template <typename T> __device__ bool wins_contains(const double* p, V3<T> q) {
+template <typename T> __device__ RB_HEAVY_FN bool wins_contains(const double* p, V3<T> q) {
   if (fabs(q.z) > T(p[4])) return false;
   T r;
   if (!wins_r<T>(p, q.z, r)) return false;
   return wins_inside_polygon<T>(p, q.x, q.y, r);
 }
-template <typename T> __device__ T wins_dist_in(const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ RB_HEAVY_FN T wins_dist_in(const double* p, V3<T> q, V3<T> d) {
   const T dz = T(p[4]);
   T best = T(kBig);
   if (d.z < T(0)) best = (-q.z - dz) / d.z;
@@ -859,7 +866,7 @@ template <typename T> __device__ T wins_dist_in(const double* p, V3<T> q, V3<T> 
   }
   return best;
 }
-template <typename T> __device__ T wins_dist_out(const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ RB_HEAVY_FN T wins_dist_out(const double* p, V3<T> q, V3<T> d) {
   const T dz = T(p[4]);
   if (q.z <= -dz) {
     if (d.z <= T(0)) return T(kBig);
@@ -878,7 +885,7 @@ template <typename T> __device__ T wins_dist_out(const double* p, V3<T> q, V3<T>
   }
   return best;
 }
-template <typename T> __device__ V3<T> wins_normal(const double* p, V3<T> q, V3<T> d) {
+template <typename T> __device__ RB_HEAVY_FN V3<T> wins_normal(const double* p, V3<T> q, V3<T> d) {
   const int n = int(p[2]);
   T saf0 = fabs(fabs(T(p[4])) - fabs(q.z));
   int k = 0;
