@@ -7,7 +7,8 @@ pristine HBM copy first). `value` is the FP64 tracer (the drop-in); the same
 run also times the mixed FP32/FP64 tracer on the same batch (with its error
 against the FP64 result: status flips and focused-hit RMS) and the adjoint
 alignment gradient (loss forward + rbgpu_adjoint over 8 angles x 1.25e6 rays,
-per-facet tip/tilt/R + camera z). Prints one JSON line.
+per-facet tip/tilt/R + camera z; one adjoint pass per angle gives loss and
+gradient). Prints one JSON line.
 """
 import argparse
 import json
@@ -144,21 +145,18 @@ def main():
         from robast_amd import alignment as AL
         model = AL.DaviesCottonAlignment(ctx, rays_per_angle=N // len(AL.C5_ANGLES_DEG))
         p0 = AL.c5_initial_params()
-        model.loss(p0)          # warm-up evaluation
-        model.gradient()
+        model.loss_and_gradient(p0)          # warm-up evaluation (sets the reference centroids)
         torch.cuda.synchronize()
-        trace_ms, adj_ms = [], []
+        adj_ms = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            model.loss(p0)
-            model.gradient()
-            trace_ms.append(model.last_trace_ms)
+            model.loss_and_gradient(p0)
             adj_ms.append(model.last_adjoint_ms)
         torch.cuda.synchronize()
         ms_eval = (time.perf_counter() - t0) * 1e3 / args.steps
         adjoint = {"grad_evals_per_s": round(1e3 / ms_eval, 3), "ms_per_eval": round(ms_eval, 3),
-                   "forward_kernel_ms": round(sum(trace_ms) / len(trace_ms), 3),
-                   "adjoint_kernel_ms": round(sum(adj_ms) / len(adj_ms), 3),
+                   "adjoint_device_ms": round(sum(adj_ms) / len(adj_ms), 3),
+                   "form": "one adjoint pass per angle returning loss and gradient (moments form)",
                    "rays": model.n * len(model.angles), "params": model.n_params,
                    "workload": "Davies-Cotton alignment: 170 tip/tilt + 85 radii + camera z, loss = sum of "
                                "spot RMS^2 over 8 angles 0-3.5 deg, unit reflectance, FP64 forward, "

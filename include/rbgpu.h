@@ -249,16 +249,28 @@ typedef struct {
   int32_t n_params;
   int32_t z_param;             /* index of the focal-plane z offset parameter, or -1 */
   int32_t focal_node;          /* node id whose entry plane carries z_param, or -1 */
-  int32_t pad;
-  double centroid_x, centroid_y;
-  double weight;
+  int32_t mode;                /* RB_ADJ_CENTROID or RB_ADJ_MOMENTS */
+  double centroid_x, centroid_y; /* RB_ADJ_CENTROID only */
+  double weight;                 /* RB_ADJ_CENTROID only */
 } rb_adjoint_desc;
 
-/* Re-trace the n rays (left unchanged) and accumulate d(loss)/d(params)
- * into grad (device, FP64, n_params; added to, not overwritten). Per-ray
- * arithmetic is FP32, accumulation FP64. The FP64 loss itself comes from a
- * forward rbgpu_trace. No counterpart in the reference: Optimize.C fits with
- * Minuit and no gradient (tutorials/Optimize.C:188-205). */
+/* RB_ADJ_CENTROID: grad[p] += weight * sum over focused rays of
+ *   (h - centroid) . dh/dp                    (grad has n_params entries).
+ * RB_ADJ_MOMENTS: one pass that needs no centroid. grad receives
+ *   [0, P)    sum h . dh/dp
+ *   [P, 2P)   sum dh_x/dp
+ *   [2P, 3P)  sum dh_y/dp
+ *   3P..3P+3  N, sum h_x, sum h_y, sum |h|^2  (P = n_params),
+ * from which the caller forms, with c = (sum h) / N,
+ *   spot RMS^2 = sum |h|^2 / N - |c|^2 and its gradient
+ *   (2 / N) (sum h.dh/dp - c_x sum dh_x/dp - c_y sum dh_y/dp). */
+enum { RB_ADJ_CENTROID = 0, RB_ADJ_MOMENTS = 1 };
+
+/* Re-trace the n rays (left unchanged) and accumulate into grad (device,
+ * FP64; added to, not overwritten) as `adj->mode` selects. Per-ray
+ * derivative arithmetic is FP32, accumulation FP64. No counterpart in the
+ * reference: Optimize.C fits with Minuit and no gradient
+ * (tutorials/Optimize.C:188-205). */
 int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const rb_rays* rays, int64_t n,
                   const rb_adjoint_desc* adj, double* grad);
 

@@ -32,6 +32,8 @@ RB_PRECISION_FP64, RB_PRECISION_MIXED = 0, 1
 
 RB_ORDER_AUTO, RB_ORDER_INPUT, RB_ORDER_COHERENT = 0, 1, 2
 
+RB_ADJ_CENTROID, RB_ADJ_MOMENTS = 0, 1
+
 RB_RAYGEN_GRID, RB_RAYGEN_DISK, RB_RAYGEN_CONE = 0, 1, 2
 
 SHAPE_DTYPE = np.dtype(
@@ -98,7 +100,7 @@ class rb_raygen_desc(C.Structure):
 class rb_adjoint_desc(C.Structure):
     _fields_ = [
         ("node_param", C.c_void_p), ("param_values", C.c_void_p), ("n_params", C.c_int32),
-        ("z_param", C.c_int32), ("focal_node", C.c_int32), ("pad", C.c_int32),
+        ("z_param", C.c_int32), ("focal_node", C.c_int32), ("mode", C.c_int32),
         ("centroid_x", C.c_double), ("centroid_y", C.c_double), ("weight", C.c_double),
     ]
 

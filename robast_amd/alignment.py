@@ -18,8 +18,9 @@ bounce chain and back-propagates dL/dhit through the reflections.
 Parameter vector layout (256 for the 85-facet array): p[2i] = tip_i,
 p[2i+1] = tilt_i, p[2F + i] = R_i, p[3F] = camera dz.
 
-`SpotLoss` is a torch.autograd.Function: forward traces, backward calls the
-adjoint. Per angle k with N_k focused rays, centroid c_k and hits h,
+`SpotLoss` is a torch.autograd.Function: its forward runs one adjoint pass
+per angle that returns the loss and the gradient together (the moments form
+of rbgpu_adjoint); backward hands the gradient on. Per angle k with N_k focused rays, centroid c_k and hits h,
 L = sum_k (1/N_k) sum |h - c_k|^2, so dL/dh = (2/N_k)(h - c_k) (the centroid's
 own dependence drops out because sum (h - c_k) = 0).
 """
@@ -145,10 +146,11 @@ def node_param_table(flat, nfacets=C5_NFACETS):
     return t, index["focalPlane_1"]
 
 
-def adjoint_desc(node_param, values, n_params, z_param, focal_node, cx, cy, weight):
+def adjoint_desc(node_param, values, n_params, z_param, focal_node, cx, cy, weight, mode=A.RB_ADJ_CENTROID):
     """rb_adjoint_desc over caller-owned buffers (host or device pointers, as
     the tensors are)."""
     d = A.rb_adjoint_desc()
+    d.mode = mode
     d.node_param = node_param.data_ptr()
     d.param_values = values.data_ptr()
     d.n_params = n_params
@@ -183,6 +185,7 @@ class DaviesCottonAlignment:
         self.work = self._alloc()
         self.scene = None
         self.mgr = None
+        self.c0 = None
         self.last_adjoint_ms = 0.0
         self.last_trace_ms = 0.0
 
@@ -234,6 +237,40 @@ class DaviesCottonAlignment:
             total += rms2
         return total
 
+    def loss_and_gradient(self, params):
+        """Loss and gradient from one adjoint pass per angle (no forward
+        trace): the kernel accumulates the hit moments and sum (h - c0).J,
+        sum J about a reference point c0 (the previous evaluation's centroid,
+        so the FP32 products stay small), and the centroid terms are formed
+        here in FP64."""
+        if self.c0 is None:
+            self.loss(params)
+            self.c0 = [(cx, cy) for _, cx, cy in self.stats]
+        self.set_params(params)
+        opts = self._opts()
+        P = self.n_params
+        acc = torch.zeros((len(self.src), 3 * P + 4), dtype=torch.float64, device=self.dev)
+        self.last_adjoint_ms = 0.0
+        for k, src in enumerate(self.src):
+            d = adjoint_desc(self.node_param, self.values, P, self.z_param, self.focal_node, self.c0[k][0],
+                             self.c0[k][1], 0.0, A.RB_ADJ_MOMENTS)
+            self.ctx.adjoint(self.scene, make_rays_device(src), self.n, opts, d, acc[k].data_ptr())
+            self.last_adjoint_ms += self.ctx.last_trace_ms()
+        a = acc.cpu().numpy()
+        loss, grad, stats = 0.0, np.zeros(P), []
+        for k in range(len(self.src)):
+            n, sx, sy, s2 = a[k, 3 * P:]
+            if n == 0:
+                stats.append((0, 0.0, 0.0))
+                continue
+            ux, uy = sx / n, sy / n
+            loss += s2 / n - ux * ux - uy * uy
+            grad += 2.0 / n * (a[k, :P] - ux * a[k, P:2 * P] - uy * a[k, 2 * P:3 * P])
+            stats.append((int(n), self.c0[k][0] + ux, self.c0[k][1] + uy))
+        self.stats = stats
+        self.c0 = [(cx, cy) if cnt else c for (cnt, cx, cy), c in zip(stats, self.c0)]
+        return loss, grad
+
     def gradient(self):
         """dL/dparams for the parameters of the last loss() call."""
         opts = self._opts()
@@ -255,15 +292,13 @@ class SpotLoss(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, params, model):
-        loss = model.loss(params.detach().cpu().numpy())
-        ctx.model = model
-        ctx.like = params
+        loss, grad = model.loss_and_gradient(params.detach().cpu().numpy())
+        ctx.grad = torch.from_numpy(grad).to(params.device)
         return torch.tensor(loss, dtype=torch.float64, device=params.device)
 
     @staticmethod
     def backward(ctx, gout):
-        g = torch.from_numpy(ctx.model.gradient()).to(ctx.like.device)
-        return gout * g, None
+        return gout * ctx.grad, None
 
 
 def adam_fit(model, p0, steps=200, lr=0.05, scale=None, callback=None):
@@ -280,7 +315,7 @@ def adam_fit(model, p0, steps=200, lr=0.05, scale=None, callback=None):
         loss = SpotLoss.apply(p0 + s * u, model)
         loss.backward()
         opt.step()
-        history.append(float(loss))
+        history.append(float(loss.detach()))
         if callback is not None:
             callback(it, float(loss))
     return (p0 + s * u.detach()).numpy(), history

@@ -24,9 +24,13 @@ constexpr int kMaxAdjParams = 1024;
 struct AdjParams {
   const int32_t* node_param;  // [n_nodes][3]
   const double* values;
-  int32_t n_params, z_param, focal_node;
+  int32_t n_params, z_param, focal_node, mode;
   double cx, cy, weight;
 };
+// accumulator entries a mode needs
+__host__ __device__ __forceinline__ int adj_acc_size(int mode, int n_params) {
+  return mode == 0 ? n_params : 3 * n_params + 4;
+}
 
 struct V3f {
   float x, y, z;
@@ -36,15 +40,20 @@ __device__ __forceinline__ float dotf(V3f a, V3f b) { return a.x * b.x + a.y * b
 __device__ __forceinline__ V3f addf(V3f a, V3f b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ V3f mulf(V3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 
-// Accumulates d(loss)/d(params) of one focused ray into `acc` (LDS, FP64).
-// Returns false if the chain holds a reflection the model does not cover.
-__device__ bool adjoint_ray(const DScene& S, const AdjParams& A, const ChainRecord& rec, V3<double> origin,
-                            V3<double> d0, V3<double> hit, double* acc) {
-  const int K = rec.n;
-  for (int j = 0; j < K; j++) {
+// True if every reflection of the chain is on a facet with parameters.
+__device__ __forceinline__ bool chain_covered(const AdjParams& A, const ChainRecord& rec) {
+  for (int j = 0; j < rec.n; j++) {
     const int32_t* np = A.node_param + 3 * rec.node[j];
     if (np[0] < 0 || np[1] < 0 || np[2] < 0) return false;
   }
+  return true;
+}
+
+// Accumulates hbar . dh/dparams of one focused ray into `acc` (LDS, FP64):
+// the reverse pass seeded with the loss adjoint hbar of the hit's (x, y).
+__device__ void adjoint_ray(const DScene& S, const AdjParams& A, const ChainRecord& rec, V3<double> origin,
+                            V3<double> d0, V3<double> hit, V3f hbar, double* acc) {
+  const int K = rec.n;
   // focal plane: reached from the last step-back point along the last
   // reflection's outgoing direction (recomputed in FP64), or straight from
   // the origin for a ray that reflected nowhere
@@ -64,7 +73,6 @@ __device__ bool adjoint_ray(const DScene& S, const AdjParams& A, const ChainReco
   const V3<double> Pp = K > 0 ? axpy(rec.P[K - 1], -eps, rec.din[K - 1]) : origin;
   const float s = float((hit.z - Pp.z) / dlast.z);
   V3f dout = f3(dlast);
-  V3f hbar = {float(A.weight * (hit.x - A.cx)), float(A.weight * (hit.y - A.cy)), 0.0f};
   // h = P' + s d', s = (z_cam - P'_z) / d'_z
   float sbar = dotf(hbar, dout);
   V3f Ppbar = hbar;
@@ -124,7 +132,28 @@ __device__ bool adjoint_ray(const DScene& S, const AdjParams& A, const ChainReco
     Ppbar = obar;
     doutbar = dinbar;
   }
-  return true;
+}
+
+// One focused ray's contribution in the descriptor's mode: the centroid
+// form seeds hbar = weight (h - c); the moments form seeds (h - c0), (1, 0)
+// and (0, 1) into three consecutive n_params blocks and adds the ray's
+// shifted moments (1, h_x - c0_x, h_y - c0_y, |h - c0|^2) after them.
+__device__ void adjoint_accumulate(const DScene& S, const AdjParams& A, const ChainRecord& rec, V3<double> origin,
+                                   V3<double> d0, V3<double> hit, double* acc) {
+  if (!chain_covered(A, rec)) return;
+  const double ux = hit.x - A.cx, uy = hit.y - A.cy;
+  if (A.mode == 0) {
+    adjoint_ray(S, A, rec, origin, d0, hit, V3f{float(A.weight * ux), float(A.weight * uy), 0.0f}, acc);
+    return;
+  }
+  const int P = A.n_params;
+  adjoint_ray(S, A, rec, origin, d0, hit, V3f{float(ux), float(uy), 0.0f}, acc);
+  adjoint_ray(S, A, rec, origin, d0, hit, V3f{1.0f, 0.0f, 0.0f}, acc + P);
+  adjoint_ray(S, A, rec, origin, d0, hit, V3f{0.0f, 1.0f, 0.0f}, acc + 2 * P);
+  atomicAdd(&acc[3 * P], 1.0);
+  atomicAdd(&acc[3 * P + 1], ux);
+  atomicAdd(&acc[3 * P + 2], uy);
+  atomicAdd(&acc[3 * P + 3], ux * ux + uy * uy);
 }
 
 }  // namespace rbd

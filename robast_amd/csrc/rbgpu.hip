@@ -265,10 +265,11 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, R
 template <bool MIXED>
 __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DScene S, RayIO io, int64_t n, TraceParams P, AdjParams A,
                                                          double* grad, const uint32_t* perm) {
-  __shared__ double acc[kMaxAdjParams];
+  __shared__ double acc[3 * kMaxAdjParams + 4];
+  const int nacc = adj_acc_size(A.mode, A.n_params);
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
-  for (int k = threadIdx.x; k < A.n_params; k += blockDim.x) acc[k] = 0.0;
+  for (int k = threadIdx.x; k < nacc; k += blockDim.x) acc[k] = 0.0;
   __syncthreads();
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = t < n ? (perm ? int64_t(perm[t]) : t) : n;
@@ -293,10 +294,10 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DScene S,
     ro.path_cap = 0;
     trace_lane<MIXED>(S, P, ro, i, L, rec);
     if (L.status == RB_FOCUS && !rec.overflow && L.last_node == A.focal_node)
-      adjoint_ray(S, A, rec, origin, d0, L.p, acc);
+      adjoint_accumulate(S, A, rec, origin, d0, L.p, acc);
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < A.n_params; k += blockDim.x)
+  for (int k = threadIdx.x; k < nacc; k += blockDim.x)
     if (acc[k] != 0.0) atomicAdd(&grad[k], acc[k]);
 }
 
@@ -652,6 +653,8 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
     return fail(ctx, -1, "rbgpu_adjoint: parameter table missing or larger than " + std::to_string(kMaxAdjParams));
   if (adj->z_param >= adj->n_params || adj->focal_node >= scene->s.n_nodes)
     return fail(ctx, -1, "rbgpu_adjoint: z parameter or focal node out of range");
+  if (adj->mode != RB_ADJ_CENTROID && adj->mode != RB_ADJ_MOMENTS)
+    return fail(ctx, -1, "rbgpu_adjoint: unknown mode");
   ctx->launches = 0;
   ctx->last_ms = 0;
   ctx->kernel_ms = 0;
@@ -673,6 +676,7 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   A.n_params = adj->n_params;
   A.z_param = adj->z_param;
   A.focal_node = adj->focal_node;
+  A.mode = adj->mode;
   A.cx = adj->centroid_x;
   A.cy = adj->centroid_y;
   A.weight = adj->weight;

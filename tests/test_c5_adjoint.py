@@ -262,3 +262,28 @@ def test_scene_updater_equals_rebuild():
             a, b = getattr(got, name), getattr(want, name)
             assert a.dtype == b.dtype and a.shape == b.shape, name
             assert a.tobytes() == b.tobytes(), name
+
+
+@pytest.mark.gpu
+def test_gpu_moments_form_matches_centroid_form(ctx):
+    """The one-pass moments form (loss and gradient from rbgpu_adjoint alone)
+    equals the forward loss and the centroid-form gradient."""
+    print("C5 moments form vs forward + centroid form, 8 angles x 50k rays", flush=True)
+    model = AL.DaviesCottonAlignment(ctx, rays_per_angle=50000)
+    p0 = AL.c5_initial_params()
+    loss_f = model.loss(p0)
+    g_c = model.gradient()
+    loss_m, g_m = model.loss_and_gradient(p0)
+    assert loss_m == pytest.approx(loss_f, rel=1e-12)
+    nz = g_c != 0
+    rel = (np.abs(g_m - g_c)[nz] / np.abs(g_c[nz])).max()
+    print(f"  loss {loss_m:.6f} vs {loss_f:.6f}, max relative gradient difference {rel:.2e}", flush=True)
+    assert rel <= 1e-5
+    # a second evaluation at moved parameters reuses the previous centroids as the reference
+    p1 = p0.copy()
+    p1[:2 * NF] *= 0.5
+    loss1, g1 = model.loss_and_gradient(p1)
+    assert loss1 == pytest.approx(model.loss(p1), rel=1e-12)
+    g1c = model.gradient()
+    nz = g1c != 0
+    assert (np.abs(g1 - g1c)[nz] / np.abs(g1c[nz])).max() <= 1e-5
