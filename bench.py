@@ -65,6 +65,49 @@ def cpu_baseline(mgr, gen, target_s=10.0):
                       f"std::thread x {cores}, {dt:.1f} s"}
 
 
+def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3):
+    """C1-C4 at 1e6 rays each: FP64 and mixed device Mrays/s, and the mixed
+    form's error against the FP64 result (status flips, focused-hit RMS)."""
+    import math
+    dev = torch.device("cuda")
+    out = {}
+    for name in ("c1", "c2", "c3", "c4"):
+        mgr = {"c1": configs.c1_scene, "c2": configs.c2_scene, "c3": configs.c3_scene, "c4": configs.c4_scene}[name]()
+        scene = ctx.upload(mgr.flat())
+        src = {f: torch.empty(n, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz",
+                                                                            "lambda_")}
+        src.update({f: torch.empty(n, dtype=torch.int32, device=dev) for f in ("status", "npoints", "last_node")})
+        if name == "c1":
+            side = int(math.isqrt(n))
+            g = configs.c1_raygen(1.0, side)
+            n_c = side * side
+        else:
+            g = {"c2": configs.c2_raygen, "c3": configs.c3_raygen}.get(name)
+            g = g() if g else configs.c4_raygen(mgr.c4_dz)
+            n_c = n
+        ctx.generate(g, make_rays_device(src), n_c)
+        res = {}
+        for prec in ("fp64", "mixed"):
+            mgr.SetPrecision(prec == "mixed")
+            opts = mgr.trace_opts(device_ptrs=1)
+            best = None
+            for _ in range(steps):
+                work = {f: v[:n_c].clone() for f, v in src.items()}
+                torch.cuda.synchronize()
+                ctx.trace(scene, make_rays_device(work), n_c, opts)
+                best = ctx.last_trace_ms() if best is None else min(best, ctx.last_trace_ms())
+            res[prec] = (best, work)
+        a, b = res["fp64"][1], res["mixed"][1]
+        both = (a["status"] == A.RB_FOCUS) & (b["status"] == A.RB_FOCUS)
+        d2 = (a["x"] - b["x"]) ** 2 + (a["y"] - b["y"]) ** 2 + (a["z"] - b["z"]) ** 2
+        out[name] = {"rays": n_c, "fp64_mrays_s": round(n_c / res["fp64"][0] / 1e3, 2),
+                     "mixed_mrays_s": round(n_c / res["mixed"][0] / 1e3, 2),
+                     "mixed_status_flips": int((a["status"] != b["status"]).sum()),
+                     "mixed_focused_hit_rms_um": round(float(torch.sqrt(d2[both].mean())) * 1e4, 6),
+                     "focused_fraction": round(float((a["status"] == A.RB_FOCUS).double().mean()), 6)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +117,7 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -162,6 +206,8 @@ def main():
                                "spot RMS^2 over 8 angles 0-3.5 deg, unit reflectance, FP64 forward, "
                                "adjoint FP32 per ray with FP64 accumulation"}
 
+    per_config = None if args.no_configs else config_sweep(ctx, torch, A, configs, make_rays_device)
+
     achieved = BYTES_PER_RAY * N / (kms * 1e-3) / 1e9
     traffic = _profile_traffic(f"k_trace_c2_{args.precision}_pmc.txt")
     out = {
@@ -192,6 +238,7 @@ def main():
         "mixed_kernel_speedup": round(kms_fp64 / kms_mixed, 3),
         "mixed_vs_fp64": {"status_flips": flips, "focused_hit_rms_um": round(rms_um, 6)},
         "adjoint": adjoint,
+        "configs": per_config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "note": "algorithmic ray-state bytes (132 B/ray) over the k_trace launch time (ordering "

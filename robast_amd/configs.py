@@ -56,6 +56,19 @@ def c1_rays(deg, nside=C1_NSIDE, lam=400 * nm):
     return ARayShooter.Square(lam, 2 * C1_R, nside, None, tr, v)
 
 
+def c1_raygen(deg, nside=C1_NSIDE, lam=400 * nm):
+    """The device-side source of c1_rays: a GRID of nside x nside rays over
+    the 3 m square at distance 2f."""
+    g = A.rb_raygen_desc()
+    g.kind = A.RB_RAYGEN_GRID
+    g.n_side = nside
+    g.size = 2 * C1_R
+    g.distance = 2 * C1_F
+    g.theta_min = g.theta_max = deg * math.pi / 180.0
+    g.lambda_min = g.lambda_max = lam
+    return g
+
+
 # ------------------------------------------------------------------ C2
 C2_F = 16 * m
 C2_MIRROR_R = 2 * C2_F

@@ -300,3 +300,12 @@ def test_operation_counts_do_not_change_the_trace_and_count_reflections():
     assert counts["reflect"] == int((a["status"] == A.RB_FOCUS).sum())
     assert counts["normal"] == counts["reflect"]
     assert counts["dist_paraboloid"] > 0 and counts["dist_sphere"] == 0
+
+
+@pytest.mark.parametrize("deg", [0.0, 1.5])
+def test_c1_grid_source_matches_arayshooter_rays(deg):
+    """The GRID ray source reproduces ARayShooter::Square's C1 rays."""
+    want = configs.c1_rays(deg, nside=50).soa()
+    got = O.generate(configs.c1_raygen(deg, nside=50), 50 * 50)
+    for k in ("x", "y", "z", "dx", "dy", "dz", "lambda_"):
+        assert np.abs(got[k] - want[k]).max() < 1e-12, k
