@@ -260,7 +260,10 @@ __device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<
   }
 }
 
-// Mixed form of the same search. Candidates are ranked with FP32 shape
+// Mixed form of the same search. Composite shapes are always measured in
+// FP64: their boolean iteration steps past each boundary by an inside-test
+// offset, and the FP32 offset (2 um) exceeds features such as a 1 um mirror
+// shell, which FP32 would step straight over. Primitives are ranked with FP32 shape
 // distances in frames rebased to each daughter's box entry; a candidate
 // within kNearF32 of the ray origin (the surface just left, or one the ray
 // sits on) is resolved in FP64 on the spot, since FP32 cannot tell which
@@ -271,12 +274,18 @@ constexpr double kNearF32 = 2e-2;
 
 // One daughter of the mixed search: FP32 distance from its box entry, FP64
 // when it is within kNearF32 of the origin; keeps the best and second best.
+__device__ __forceinline__ bool fp64_only(const DScene& S, int node) {
+  return S.shapes[S.nodes[node].shape].depth > 0;
+}
 __device__ __forceinline__ void rank_mixed(const DScene& S, int c, float tn, V3<double> p, V3<double> d,
                                            double& best, double& second, int& crossed, Hit& hit, bool& exact) {
-  const double base = fmax(0.0, double(tn) * (1.0 - 1e-6) - 1.0);
-  Hit h = node_dist<float>(S, c, p, d, base, true);
-  bool hexact = false;
-  if (h.s < kNearF32) {
+  Hit h;
+  bool hexact = fp64_only(S, c);
+  if (!hexact) {
+    const double base = fmax(0.0, double(tn) * (1.0 - 1e-6) - 1.0);
+    h = node_dist<float>(S, c, p, d, base, true);
+  }
+  if (hexact || h.s < kNearF32) {
     h = node_dist<double>(S, c, p, d, 0.0, true);
     hexact = true;
   }
@@ -298,9 +307,10 @@ __device__ bool search_boundary_mixed(const DScene& S, int cur, V3<double> p, V3
   crossed = -2;
   bool exact = false;
   double second = kBig;
-  Hit hx = node_dist<float>(S, cur, p, d, 0.0, false);
-  bool hx_exact = false;
-  if (hx.s < kNearF32) {
+  Hit hx;
+  bool hx_exact = fp64_only(S, cur);
+  if (!hx_exact) hx = node_dist<float>(S, cur, p, d, 0.0, false);
+  if (hx_exact || hx.s < kNearF32) {
     hx = node_dist<double>(S, cur, p, d, 0.0, false);
     hx_exact = true;
   }

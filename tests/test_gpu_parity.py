@@ -157,15 +157,15 @@ def test_c3_aspheric_sample_per_ray(ctx):
     assert counts(g["status"])[A.RB_FOCUS] > 10000
 
 
-@pytest.mark.parametrize("config", ["c2", "c3", "c4"])
+@pytest.mark.parametrize("config", ["c1", "c2", "c3", "c4"])
 def test_mixed_precision_against_fp64(ctx, config):
     """Mixed form (FP32 search in rebased frames + FP64 polish) against the
     plain FP64 kernel at 1e6 rays: status flips rare, focused-hit RMS < 1 um."""
     import torch
     print(f"mixed vs fp64 {config}", flush=True)
     N = 1_000_000
-    mgr = {"c2": configs.c2_scene, "c3": configs.c3_scene, "c4": configs.c4_scene}[config]()
-    gen = {"c2": configs.c2_raygen(), "c3": configs.c3_raygen(),
+    mgr = {"c1": configs.c1_scene, "c2": configs.c2_scene, "c3": configs.c3_scene, "c4": configs.c4_scene}[config]()
+    gen = {"c1": configs.c1_raygen(1.0, 1000), "c2": configs.c2_raygen(), "c3": configs.c3_raygen(),
            "c4": configs.c4_raygen(getattr(mgr, "c4_dz", 0.0))}[config]
     scene = ctx.upload(mgr.flat())
     dev = torch.device("cuda")
