@@ -120,7 +120,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(rays would shard by contiguous ranges with the scene replicated per GPU)", file=sys.stderr)
         sys.exit(2)
 
