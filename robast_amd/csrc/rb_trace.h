@@ -175,28 +175,47 @@ __device__ __forceinline__ bool box_pass(const DBvh& b, float ox, float oy, floa
   return tn <= tf * 1.0001f + 1e-3f && tn <= lim;
 }
 
-// Gather the leaves (daughter node ids, in depth-first order) whose boxes
-// the ray crosses no farther than lim. Returns the count, or -1 if more than
-// kCand qualify.
+// Gather the leaves (daughter node ids) whose boxes the ray crosses no
+// farther than lim, keeping the kCand nearest by box entry, sorted by entry
+// distance. Returns the count kept; *cut receives the smallest entry
+// distance among the boxes that did not fit (+inf if none was dropped).
 __device__ __forceinline__ int gather_leaves(const DScene& S, int root, float ox, float oy, float oz, float ix,
-                                             float iy, float iz, float lim, Cand cand) {
+                                             float iy, float iz, float lim, Cand cand, float* cut) {
   int i = root, nc = 0;
+  float drop = 3.0e38f;
   const int end = S.bvh[i].escape;
   while (i < end) {
     const DBvh b = S.bvh[i];
     float tn;
-    if (!box_pass(b, ox, oy, oz, ix, iy, iz, lim, tn)) {
+    if (!box_pass(b, ox, oy, oz, ix, iy, iz, fminf(lim, drop), tn)) {
       i = b.escape;
       continue;
     }
     if (b.prim >= 0) {
-      if (nc == kCand) return -1;
-      cand.node[nc * kCandStride] = b.prim;
-      cand.tn[nc * kCandStride] = tn;
-      nc++;
+      // insertion into the sorted slots; a full list drops its farthest
+      int k = nc < kCand ? nc : kCand - 1;
+      if (nc == kCand) {
+        const float last = cand.tn[(kCand - 1) * kCandStride];
+        if (tn >= last) {
+          drop = fminf(drop, tn);
+          i++;
+          continue;
+        }
+        drop = fminf(drop, last);
+      } else {
+        nc++;
+      }
+      while (k > 0 && cand.tn[(k - 1) * kCandStride] > tn) {
+        cand.tn[k * kCandStride] = cand.tn[(k - 1) * kCandStride];
+        cand.node[k * kCandStride] = cand.node[(k - 1) * kCandStride];
+        k--;
+      }
+      cand.tn[k * kCandStride] = tn;
+      cand.node[k * kCandStride] = b.prim;
     }
     i++;
   }
+  *cut = drop;
   return nc;
 }
 
@@ -216,11 +235,13 @@ __device__ __forceinline__ void rank_fp64(int c, const Hit& h, double& best, int
 // all shape distances in FP64 (the plain form).
 //
 // The daughters are tested in two phases: the BVH walk only gathers the
-// leaves whose boxes the ray crosses, then every lane tests its k-th
-// candidate together, so the shape code runs with the wave's lanes in step
-// instead of one leaf-holding lane at a time. Each candidate is re-checked
-// against the current best before its test, in the walk's order, which
-// reproduces the one-pass walk's tests and ranking exactly.
+// (up to kCand nearest) leaves whose boxes the ray crosses, then every lane
+// tests its k-th candidate together, so the shape code runs with the wave's
+// lanes in step instead of one leaf-holding lane at a time. Candidates are
+// tested in order of box entry and the test stops at the first entry beyond
+// the best crossing (a shape is never met before its box). If a box that
+// did not fit could still hold a nearer crossing, the one-pass walk decides.
+// The ranking rule makes the result independent of the test order.
 __device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<double> d, double& best,
                                      int& crossed, Hit& hit, Cand cand) {
   const DNode& cn = S.nodes[cur];
@@ -236,14 +257,21 @@ __device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<
   const float ox = float(p.x), oy = float(p.y), oz = float(p.z);
   const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
 #if RB_POSTPONE
-  const int nc = gather_leaves(S, cn.bvh_root, ox, oy, oz, ix, iy, iz, float(best) * 1.0001f + 1e-2f, cand);
-  if (nc >= 0) {
-    for (int k = 0; k < nc; k++) {
-      if (cand.tn[k * kCandStride] > float(best) * 1.0001f + 1e-2f) continue;
-      const int c = cand.node[k * kCandStride];
-      rank_fp64(c, node_dist<double>(S, c, p, d, 0.0, true), best, crossed, hit);
-    }
-    return;
+  float cut;
+  const int nc = gather_leaves(S, cn.bvh_root, ox, oy, oz, ix, iy, iz, float(best) * 1.0001f + 1e-2f, cand, &cut);
+  for (int k = 0; k < nc; k++) {
+    if (cand.tn[k * kCandStride] > float(best) * 1.0001f + 1e-2f) break;
+    const int c = cand.node[k * kCandStride];
+    rank_fp64(c, node_dist<double>(S, c, p, d, 0.0, true), best, crossed, hit);
+  }
+  if (!(cut <= float(best) * 1.0001f + 1e-2f)) return;
+  // a dropped box may hold a nearer crossing: rank all daughters afresh
+  best = kBig;
+  crossed = -2;
+  if (hx.s < best - kTol) {
+    best = hx.s;
+    crossed = -1;
+    hit = hx;
   }
 #endif
   int i = cn.bvh_root;
@@ -323,16 +351,31 @@ __device__ bool search_boundary_mixed(const DScene& S, int cur, V3<double> p, V3
   if (cn.n_children > 0) {
     const float ox = float(p.x), oy = float(p.y), oz = float(p.z);
     const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
-    int nc = -1;
+    bool walk = true;
 #if RB_POSTPONE
-    nc = gather_leaves(S, cn.bvh_root, ox, oy, oz, ix, iy, iz, float(fmin(best, second)) * 1.0001f + 1e-2f, cand);
+    float cut;
+    const int nc =
+        gather_leaves(S, cn.bvh_root, ox, oy, oz, ix, iy, iz, float(fmin(best, second)) * 1.0001f + 1e-2f, cand, &cut);
     for (int k = 0; k < nc; k++) {
       const float tn = cand.tn[k * kCandStride];
-      if (tn > float(fmin(best, second)) * 1.0001f + 1e-2f) continue;
+      if (tn > float(fmin(best, second)) * 1.0001f + 1e-2f) break;
       rank_mixed(S, cand.node[k * kCandStride], tn, p, d, best, second, crossed, hit, exact);
     }
+    walk = cut <= float(fmin(best, second)) * 1.0001f + 1e-2f;
+    if (walk) {  // a dropped box may hold a nearer crossing: rank all daughters afresh
+      best = kBig;
+      second = kBig;
+      crossed = -2;
+      exact = false;
+      if (hx.s < kBig) {
+        best = hx.s;
+        crossed = -1;
+        hit = hx;
+        exact = hx_exact;
+      }
+    }
 #endif
-    if (nc < 0) {
+    if (walk) {
       int i = cn.bvh_root;
       const int end = S.bvh[i].escape;
       while (i < end) {
