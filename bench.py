@@ -60,9 +60,24 @@ def cpu_baseline(mgr, gen, target_s=10.0):
     t0 = time.perf_counter()
     O.trace(flat, soa, mgr.trace_opts(), nthreads=cores)
     dt = time.perf_counter() - t0
-    return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"first {n} rays of the same seeded Davies-Cotton batch, oracle/rb_oracle.cpp, "
-                      f"std::thread x {cores}, {dt:.1f} s"}
+    out = {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
+           "sample": f"first {n} rays of the same seeded Davies-Cotton batch, oracle/rb_oracle.cpp, "
+                     f"std::thread x {cores}, {dt:.1f} s"}
+    # the other configurations, on samples of about 2 s each
+    from robast_amd import configs
+    per = {}
+    for name, m, g, k in (("c1", configs.c1_scene(), configs.c1_raygen(1.0, 1000), 1_000_000),
+                          ("c3", configs.c3_scene(), configs.c3_raygen(), 400_000),
+                          ("c4", None, None, 200_000)):
+        if name == "c4":
+            m = configs.c4_scene()
+            g = configs.c4_raygen(m.c4_dz)
+        soa = O.generate(g, k)
+        t0 = time.perf_counter()
+        O.trace(m.flat(), soa, m.trace_opts(), nthreads=cores)
+        per[name] = {"value": round(k / (time.perf_counter() - t0) / 1e6, 4), "rays": k}
+    out["configs"] = per
+    return out
 
 
 def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3):

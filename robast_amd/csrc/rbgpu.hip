@@ -361,6 +361,27 @@ __global__ void __launch_bounds__(256) k_generate(rb_raygen_desc g, RayIO io, in
 }
 
 // ---------------------------------------------------------------- C ABI
+// True if the GPU can dereference p: device or managed memory, or host
+// memory pinned and mapped by HIP. Pageable host memory would fault in a
+// kernel, so callers that promise device pointers are checked with this.
+static bool gpu_accessible(const void* p) {
+  if (!p) return true;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+         (a.type == hipMemoryTypeHost && a.devicePointer != nullptr);
+}
+static bool rays_accessible(const rb_rays* r, bool with_path) {
+  const void* ptrs[13] = {r->x, r->y, r->z, r->t, r->dx, r->dy, r->dz, r->lambda,
+                          r->status, r->npoints, r->last_node, r->path_xyzt, r->path_node};
+  for (int k = 0; k < (with_path ? 13 : 11); k++)
+    if (!gpu_accessible(ptrs[k])) return false;
+  return true;
+}
+
 // Batches at least this large are traced in coherence order by default.
 constexpr int64_t kAutoOrderMin = int64_t(1) << 20;
 
@@ -552,6 +573,7 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
       HIPCHK(ctx, hipMemcpyAsync(io.path_node, rays->path_node, ni * cap, hipMemcpyHostToDevice, ctx->stream));
     }
   } else {
+    if (!rays_accessible(rays, cap > 0)) return fail(ctx, -1, "rbgpu_trace: device_ptrs set but a ray array is not GPU memory");
     io.x = rays->x; io.y = rays->y; io.z = rays->z; io.t = rays->t;
     io.dx = rays->dx; io.dy = rays->dy; io.dz = rays->dz; io.lambda = rays->lambda;
     io.status = rays->status; io.npoints = rays->npoints; io.last_node = rays->last_node;
@@ -632,6 +654,10 @@ int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* g, rb_rays* rays, int
     return fail(ctx, -1, "rbgpu_generate_rays: grid smaller than n");
   if (n == 0) return 0;
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!rays->x || !rays->y || !rays->z || !rays->t || !rays->dx || !rays->dy || !rays->dz || !rays->lambda ||
+      !rays->status || !rays->npoints)
+    return fail(ctx, -1, "rbgpu_generate_rays: missing ray array");
+  if (!rays_accessible(rays, false)) return fail(ctx, -1, "rbgpu_generate_rays: a ray array is not GPU memory");
   RayIO io{};
   io.x = rays->x; io.y = rays->y; io.z = rays->z; io.t = rays->t;
   io.dx = rays->dx; io.dy = rays->dy; io.dz = rays->dz; io.lambda = rays->lambda;
@@ -655,6 +681,13 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
     return fail(ctx, -1, "rbgpu_adjoint: z parameter or focal node out of range");
   if (adj->mode != RB_ADJ_CENTROID && adj->mode != RB_ADJ_MOMENTS)
     return fail(ctx, -1, "rbgpu_adjoint: unknown mode");
+  if (!rays->x || !rays->y || !rays->z || !rays->t || !rays->dx || !rays->dy || !rays->dz || !rays->lambda ||
+      !rays->status || !rays->npoints)
+    return fail(ctx, -1, "rbgpu_adjoint: missing ray array");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!rays_accessible(rays, false) || !gpu_accessible(adj->node_param) || !gpu_accessible(adj->param_values) ||
+      !gpu_accessible(grad))
+    return fail(ctx, -1, "rbgpu_adjoint: rays, parameter table, values and grad must be GPU memory");
   ctx->launches = 0;
   ctx->last_ms = 0;
   ctx->kernel_ms = 0;

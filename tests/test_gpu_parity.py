@@ -229,3 +229,19 @@ def test_coherence_order_gives_identical_results(ctx, config):
         out[order] = t
     for k in out[A.RB_ORDER_INPUT]:
         assert torch.equal(out[A.RB_ORDER_INPUT][k], out[A.RB_ORDER_COHERENT][k]), k
+
+
+def test_host_memory_passed_as_device_pointers_is_an_error(ctx):
+    """device_ptrs = 1 with pageable host arrays returns an error (checked on
+    the host before any launch) instead of faulting in a kernel."""
+    import torch
+    mgr = configs.c1_scene()
+    scene = ctx.upload(mgr.flat())
+    soa = configs.c1_rays(0.0, nside=16).soa()
+    with pytest.raises(RbError, match="not GPU memory"):
+        ctx.trace(scene, make_rays(soa), len(soa["x"]), mgr.trace_opts(device_ptrs=1))
+    with pytest.raises(RbError, match="not GPU memory"):
+        ctx.generate(configs.c2_raygen(), make_rays(soa), len(soa["x"]))
+    # the same arrays in HBM are accepted
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in soa.items()}
+    ctx.trace(scene, make_rays_device(t), len(soa["x"]), mgr.trace_opts(device_ptrs=1))
