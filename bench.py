@@ -7,8 +7,8 @@ pristine HBM copy first). `value` is the FP64 tracer (the drop-in); the same
 run also times the mixed FP32/FP64 tracer on the same batch (with its error
 against the FP64 result: status flips and focused-hit RMS) and the adjoint
 alignment gradient (loss forward + rbgpu_adjoint over 8 angles x 1.25e6 rays,
-per-facet tip/tilt/R + camera z; one adjoint pass per angle gives loss and
-gradient). Prints one JSON line.
+per-facet tip/tilt/R + camera z; one adjoint launch over all angles gives
+loss and gradient). Prints one JSON line.
 """
 import argparse
 import json
@@ -215,7 +215,7 @@ def main():
         ms_eval = (time.perf_counter() - t0) * 1e3 / args.steps
         adjoint = {"grad_evals_per_s": round(1e3 / ms_eval, 3), "ms_per_eval": round(ms_eval, 3),
                    "adjoint_device_ms": round(sum(adj_ms) / len(adj_ms), 3),
-                   "form": "one adjoint pass per angle returning loss and gradient (moments form)",
+                   "form": "one adjoint launch over the 8 angles returning loss and gradient (moments form)",
                    "rays": model.n * len(model.angles), "params": model.n_params,
                    "workload": "Davies-Cotton alignment: 170 tip/tilt + 85 radii + camera z, loss = sum of "
                                "spot RMS^2 over 8 angles 0-3.5 deg, unit reflectance, FP64 forward, "

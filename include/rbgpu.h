@@ -250,20 +250,32 @@ typedef struct {
   int32_t z_param;             /* index of the focal-plane z offset parameter, or -1 */
   int32_t focal_node;          /* node id whose entry plane carries z_param, or -1 */
   int32_t mode;                /* RB_ADJ_CENTROID or RB_ADJ_MOMENTS */
-  double centroid_x, centroid_y; /* RB_ADJ_CENTROID only */
+  double centroid_x, centroid_y; /* CENTROID: the centroid; MOMENTS with no ref_xy: the reference point */
   double weight;                 /* RB_ADJ_CENTROID only */
+  /* RB_ADJ_MOMENTS only: the rays form n_groups consecutive groups of
+   * group_size rays (ray i belongs to group i / group_size), each with its
+   * own reference point ref_xy[2 g], ref_xy[2 g + 1] (device memory) and its
+   * own block of 3 n_params + 4 sums in grad. n_groups = 0 means one group
+   * over all n rays, referenced to (centroid_x, centroid_y). */
+  const double* ref_xy;
+  int64_t group_size;
+  int32_t n_groups;
+  int32_t pad;
 } rb_adjoint_desc;
 
 /* RB_ADJ_CENTROID: grad[p] += weight * sum over focused rays of
  *   (h - centroid) . dh/dp                    (grad has n_params entries).
- * RB_ADJ_MOMENTS: one pass that needs no centroid. grad receives
- *   [0, P)    sum h . dh/dp
+ * RB_ADJ_MOMENTS: one pass that needs no centroid. With u = h - r, r the
+ * group's reference point, each group's block of grad receives
+ *   [0, P)    sum u . dh/dp
  *   [P, 2P)   sum dh_x/dp
  *   [2P, 3P)  sum dh_y/dp
- *   3P..3P+3  N, sum h_x, sum h_y, sum |h|^2  (P = n_params),
- * from which the caller forms, with c = (sum h) / N,
- *   spot RMS^2 = sum |h|^2 / N - |c|^2 and its gradient
- *   (2 / N) (sum h.dh/dp - c_x sum dh_x/dp - c_y sum dh_y/dp). */
+ *   3P..3P+3  N, sum u_x, sum u_y, sum |u|^2  (P = n_params),
+ * from which the caller forms, with m = (sum u) / N (centroid c = r + m),
+ *   spot RMS^2 = sum |u|^2 / N - |m|^2 and its gradient
+ *   (2 / N) (sum u.dh/dp - m_x sum dh_x/dp - m_y sum dh_y/dp).
+ * A reference point near the centroid keeps the FP32 products small. The
+ * accumulators of all groups must fit: groups x (3 P + 4) <= 6400. */
 enum { RB_ADJ_CENTROID = 0, RB_ADJ_MOMENTS = 1 };
 
 /* Re-trace the n rays (left unchanged) and accumulate into grad (device,

@@ -102,6 +102,7 @@ class rb_adjoint_desc(C.Structure):
         ("node_param", C.c_void_p), ("param_values", C.c_void_p), ("n_params", C.c_int32),
         ("z_param", C.c_int32), ("focal_node", C.c_int32), ("mode", C.c_int32),
         ("centroid_x", C.c_double), ("centroid_y", C.c_double), ("weight", C.c_double),
+        ("ref_xy", C.c_void_p), ("group_size", C.c_int64), ("n_groups", C.c_int32), ("pad", C.c_int32),
     ]
 
 

@@ -18,9 +18,10 @@ bounce chain and back-propagates dL/dhit through the reflections.
 Parameter vector layout (256 for the 85-facet array): p[2i] = tip_i,
 p[2i+1] = tilt_i, p[2F + i] = R_i, p[3F] = camera dz.
 
-`SpotLoss` is a torch.autograd.Function: its forward runs one adjoint pass
-per angle that returns the loss and the gradient together (the moments form
-of rbgpu_adjoint); backward hands the gradient on. Per angle k with N_k focused rays, centroid c_k and hits h,
+`SpotLoss` is a torch.autograd.Function: its forward runs one adjoint launch
+over all angles that returns the loss and the gradient together (the moments
+form of rbgpu_adjoint, one group of sums per angle); backward hands the
+gradient on. Per angle k with N_k focused rays, centroid c_k and hits h,
 L = sum_k (1/N_k) sum |h - c_k|^2, so dL/dh = (2/N_k)(h - c_k) (the centroid's
 own dependence drops out because sum (h - c_k) = 0).
 """
@@ -177,9 +178,12 @@ class DaviesCottonAlignment:
         self.mixed = precision == "mixed"
         dev = torch.device("cuda")
         self.dev = dev
+        # all angles' rays in one buffer (angle k at [k n, (k + 1) n)), so one
+        # adjoint launch covers them; self.src[k] are views of angle k
+        self.all = self._alloc(len(self.angles) * self.n)
         self.src = []
         for k, a in enumerate(self.angles):
-            t = self._alloc()
+            t = {f: v[k * self.n:(k + 1) * self.n] for f, v in self.all.items()}
             self.ctx.generate(c5_raygen(a, k), make_rays_device(t), self.n)
             self.src.append(t)
         self.work = self._alloc()
@@ -189,10 +193,11 @@ class DaviesCottonAlignment:
         self.last_adjoint_ms = 0.0
         self.last_trace_ms = 0.0
 
-    def _alloc(self):
-        t = {f: torch.empty(self.n, dtype=torch.float64, device=self.dev) for f in _FIELDS_F64}
+    def _alloc(self, n=None):
+        n = self.n if n is None else n
+        t = {f: torch.empty(n, dtype=torch.float64, device=self.dev) for f in _FIELDS_F64}
         for f in _FIELDS_I32:
-            t[f] = torch.empty(self.n, dtype=torch.int32, device=self.dev)
+            t[f] = torch.empty(n, dtype=torch.int32, device=self.dev)
         return t
 
     def _opts(self):
@@ -238,24 +243,26 @@ class DaviesCottonAlignment:
         return total
 
     def loss_and_gradient(self, params):
-        """Loss and gradient from one adjoint pass per angle (no forward
-        trace): the kernel accumulates the hit moments and sum (h - c0).J,
-        sum J about a reference point c0 (the previous evaluation's centroid,
-        so the FP32 products stay small), and the centroid terms are formed
-        here in FP64."""
+        """Loss and gradient from one adjoint launch over all angles (no
+        forward trace): per angle, the kernel accumulates the hit moments and
+        sum (h - c0).J, sum J about a reference point c0 (the previous
+        evaluation's centroid, so the FP32 products stay small), and the
+        centroid terms are formed here in FP64."""
         if self.c0 is None:
             self.loss(params)
             self.c0 = [(cx, cy) for _, cx, cy in self.stats]
         self.set_params(params)
         opts = self._opts()
-        P = self.n_params
-        acc = torch.zeros((len(self.src), 3 * P + 4), dtype=torch.float64, device=self.dev)
-        self.last_adjoint_ms = 0.0
-        for k, src in enumerate(self.src):
-            d = adjoint_desc(self.node_param, self.values, P, self.z_param, self.focal_node, self.c0[k][0],
-                             self.c0[k][1], 0.0, A.RB_ADJ_MOMENTS)
-            self.ctx.adjoint(self.scene, make_rays_device(src), self.n, opts, d, acc[k].data_ptr())
-            self.last_adjoint_ms += self.ctx.last_trace_ms()
+        P, G = self.n_params, len(self.src)
+        acc = torch.zeros((G, 3 * P + 4), dtype=torch.float64, device=self.dev)
+        ref = torch.tensor(self.c0, dtype=torch.float64, device=self.dev)
+        d = adjoint_desc(self.node_param, self.values, P, self.z_param, self.focal_node, 0.0, 0.0, 0.0,
+                         A.RB_ADJ_MOMENTS)
+        d.ref_xy = ref.data_ptr()
+        d.group_size = self.n
+        d.n_groups = G
+        self.ctx.adjoint(self.scene, make_rays_device(self.all), G * self.n, opts, d, acc.data_ptr())
+        self.last_adjoint_ms = self.ctx.last_trace_ms()
         a = acc.cpu().numpy()
         loss, grad, stats = 0.0, np.zeros(P), []
         for k in range(len(self.src)):

@@ -26,10 +26,17 @@ struct AdjParams {
   const double* values;
   int32_t n_params, z_param, focal_node, mode;
   double cx, cy, weight;
+  const double* ref_xy;       // moments: per-group reference points, or null
+  int64_t group_size;         // moments: rays per group
+  int32_t n_groups;           // moments: groups (>= 1)
 };
-// accumulator entries a mode needs
-__host__ __device__ __forceinline__ int adj_acc_size(int mode, int n_params) {
+constexpr int kMaxAdjAcc = 6400;  // LDS accumulator entries (FP64)
+// accumulator entries per group, and in all
+__host__ __device__ __forceinline__ int adj_block(int mode, int n_params) {
   return mode == 0 ? n_params : 3 * n_params + 4;
+}
+__host__ __device__ __forceinline__ int adj_acc_size(int mode, int n_params, int n_groups) {
+  return mode == 0 ? n_params : n_groups * (3 * n_params + 4);
 }
 
 struct V3f {
@@ -139,9 +146,18 @@ __device__ void adjoint_ray(const DScene& S, const AdjParams& A, const ChainReco
 // and (0, 1) into three consecutive n_params blocks and adds the ray's
 // shifted moments (1, h_x - c0_x, h_y - c0_y, |h - c0|^2) after them.
 __device__ void adjoint_accumulate(const DScene& S, const AdjParams& A, const ChainRecord& rec, V3<double> origin,
-                                   V3<double> d0, V3<double> hit, double* acc) {
+                                   V3<double> d0, V3<double> hit, int64_t ray, double* acc) {
   if (!chain_covered(A, rec)) return;
-  const double ux = hit.x - A.cx, uy = hit.y - A.cy;
+  double rx = A.cx, ry = A.cy;
+  if (A.mode != 0) {
+    const int64_t g = ray / A.group_size;
+    acc += g * adj_block(A.mode, A.n_params);
+    if (A.ref_xy) {
+      rx = A.ref_xy[2 * g];
+      ry = A.ref_xy[2 * g + 1];
+    }
+  }
+  const double ux = hit.x - rx, uy = hit.y - ry;
   if (A.mode == 0) {
     adjoint_ray(S, A, rec, origin, d0, hit, V3f{float(A.weight * ux), float(A.weight * uy), 0.0f}, acc);
     return;

@@ -265,8 +265,8 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, R
 template <bool MIXED>
 __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DScene S, RayIO io, int64_t n, TraceParams P, AdjParams A,
                                                          double* grad, const uint32_t* perm) {
-  __shared__ double acc[3 * kMaxAdjParams + 4];
-  const int nacc = adj_acc_size(A.mode, A.n_params);
+  __shared__ double acc[kMaxAdjAcc];
+  const int nacc = adj_acc_size(A.mode, A.n_params, A.n_groups);
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   for (int k = threadIdx.x; k < nacc; k += blockDim.x) acc[k] = 0.0;
@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DScene S,
     ro.path_cap = 0;
     trace_lane<MIXED>(S, P, ro, i, L, rec);
     if (L.status == RB_FOCUS && !rec.overflow && L.last_node == A.focal_node)
-      adjoint_accumulate(S, A, rec, origin, d0, L.p, acc);
+      adjoint_accumulate(S, A, rec, origin, d0, L.p, i, acc);
   }
   __syncthreads();
   for (int k = threadIdx.x; k < nacc; k += blockDim.x)
@@ -681,12 +681,19 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
     return fail(ctx, -1, "rbgpu_adjoint: z parameter or focal node out of range");
   if (adj->mode != RB_ADJ_CENTROID && adj->mode != RB_ADJ_MOMENTS)
     return fail(ctx, -1, "rbgpu_adjoint: unknown mode");
+  const int groups = adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 ? adj->n_groups : 1;
+  const int64_t group_size = adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 ? adj->group_size : std::max<int64_t>(n, 1);
+  if (adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 &&
+      (adj->group_size <= 0 || int64_t(adj->n_groups) * adj->group_size < n || !adj->ref_xy))
+    return fail(ctx, -1, "rbgpu_adjoint: groups must cover the n rays and carry reference points");
+  if (adj_acc_size(adj->mode, adj->n_params, groups) > kMaxAdjAcc)
+    return fail(ctx, -1, "rbgpu_adjoint: accumulators exceed " + std::to_string(kMaxAdjAcc) + " entries");
   if (!rays->x || !rays->y || !rays->z || !rays->t || !rays->dx || !rays->dy || !rays->dz || !rays->lambda ||
       !rays->status || !rays->npoints)
     return fail(ctx, -1, "rbgpu_adjoint: missing ray array");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   if (!rays_accessible(rays, false) || !gpu_accessible(adj->node_param) || !gpu_accessible(adj->param_values) ||
-      !gpu_accessible(grad))
+      !gpu_accessible(grad) || (adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 && !gpu_accessible(adj->ref_xy)))
     return fail(ctx, -1, "rbgpu_adjoint: rays, parameter table, values and grad must be GPU memory");
   ctx->launches = 0;
   ctx->last_ms = 0;
@@ -710,6 +717,9 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   A.z_param = adj->z_param;
   A.focal_node = adj->focal_node;
   A.mode = adj->mode;
+  A.ref_xy = adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 ? adj->ref_xy : nullptr;
+  A.group_size = group_size;
+  A.n_groups = groups;
   A.cx = adj->centroid_x;
   A.cy = adj->centroid_y;
   A.weight = adj->weight;
