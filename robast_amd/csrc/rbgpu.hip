@@ -526,15 +526,17 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
   if (!scene || !opts || !rays || n < 0) return fail(ctx, -1, "rbgpu_trace: null argument or negative n");
   if (opts->precision != RB_PRECISION_FP64 && opts->precision != RB_PRECISION_MIXED)
     return fail(ctx, -1, "rbgpu_trace: unknown precision mode");
+  if (opts->order != RB_ORDER_AUTO && opts->order != RB_ORDER_INPUT && opts->order != RB_ORDER_COHERENT)
+    return fail(ctx, -1, "rbgpu_trace: unknown order");
+  ctx->launches = 0;
+  ctx->last_ms = 0;
+  ctx->kernel_ms = 0;
+  if (n == 0) return 0;  // an empty batch: nothing to read, arrays may be null
   if (!rays->x || !rays->y || !rays->z || !rays->t || !rays->dx || !rays->dy || !rays->dz || !rays->lambda ||
       !rays->status || !rays->npoints)
     return fail(ctx, -1, "rbgpu_trace: missing ray array");
   if (rays->path_capacity < 0 || (rays->path_capacity > 0 && (!rays->path_xyzt || !rays->path_node)))
     return fail(ctx, -1, "rbgpu_trace: bad path record");
-  ctx->launches = 0;
-  ctx->last_ms = 0;
-  ctx->kernel_ms = 0;
-  if (n == 0) return 0;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   TraceParams P;
   P.seed = opts->seed;

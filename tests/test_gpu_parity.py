@@ -245,3 +245,47 @@ def test_host_memory_passed_as_device_pointers_is_an_error(ctx):
     # the same arrays in HBM are accepted
     t = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in soa.items()}
     ctx.trace(scene, make_rays_device(t), len(soa["x"]), mgr.trace_opts(device_ptrs=1))
+
+
+@pytest.mark.parametrize("n", [0, 1, 257])
+def test_empty_and_ragged_batches(ctx, n):
+    """n = 0 is a no-op; batches that do not fill a workgroup trace exactly
+    as the oracle does, through host and device pointers."""
+    import torch
+    mgr = configs.c2_scene()
+    scene = ctx.upload(mgr.flat())
+    soa = O.generate(configs.c2_raygen(), max(n, 1))
+    soa = {k: v[:n].copy() for k, v in soa.items()}
+    g = copy_soa(soa)
+    ctx.trace(scene, make_rays(g), n, mgr.trace_opts())
+    o = copy_soa(soa)
+    O.trace(mgr.flat(), o, mgr.trace_opts())
+    if n:
+        compare(g, o)
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in soa.items()}
+    if n:
+        ctx.trace(scene, make_rays_device(t), n, mgr.trace_opts(device_ptrs=1))
+        for k in g:
+            assert np.array_equal(t[k].cpu().numpy(), g[k]), k
+    assert ctx.last_trace_launches() == (1 if n else 0)
+
+
+def test_auto_order_just_above_threshold(ctx):
+    """A batch of 2^20 + 3 rays (auto order sorts it, in a partial last tile)
+    equals the same batch traced in input order."""
+    import torch
+    N = (1 << 20) + 3
+    mgr = configs.c2_scene()
+    scene = ctx.upload(mgr.flat())
+    dev = torch.device("cuda")
+    out = {}
+    for order in (A.RB_ORDER_AUTO, A.RB_ORDER_INPUT):
+        t = {f: torch.empty(N, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_")}
+        for f in ("status", "npoints", "last_node"):
+            t[f] = torch.empty(N, dtype=torch.int32, device=dev)
+        ctx.generate(configs.c2_raygen(), make_rays_device(t), N)
+        ctx.trace(scene, make_rays_device(t), N, mgr.trace_opts(device_ptrs=1, order=order))
+        assert ctx.last_trace_launches() == (3 if order == A.RB_ORDER_AUTO else 1)
+        out[order] = t
+    for k in out[A.RB_ORDER_AUTO]:
+        assert torch.equal(out[A.RB_ORDER_AUTO][k], out[A.RB_ORDER_INPUT][k]), k
