@@ -608,6 +608,7 @@ __device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io
     const V3<double> x1 = L.p;
     const double t1 = L.t;
     const int current = L.cur;
+    const int np0 = L.npoints;
     const int next = next_boundary<MIXED>(S, L);
     const double step = L.step;
     const int tc = kind_of(S, current), tn = kind_of(S, next);
@@ -652,7 +653,10 @@ __device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io
       add_point(io, i, L, L.p, t1 + step / kC, next);
       L.status = RB_EXIT;
     } else if (tc == RB_KFOCUS || tc == RB_KOBS || tc == RB_KMIRROR || tn == RB_KOBS) {
+      // ray->Stop() without AddPoint: the ray's last point stays the one
+      // recorded before this step, not the navigator's new position
       L.status = RB_STOP;
+      if (L.npoints == np0) L.p = x1;
     } else if (tn == RB_KFOCUS) {
       const rb_volume_desc& v = vol_of(S, next);
       double angle = 0.;
