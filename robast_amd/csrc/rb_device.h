@@ -1205,6 +1205,20 @@ __device__ inline double table_eval(const DScene& S, int table, double x) {
   if (xl == xu) return yl;
   return yu + (x - xu) * (yl - yu) / (xl - xu);
 }
+// Rarely taken math (absorption depth, focal-surface angle, Schott index) is
+// kept out of line: inlined into the tracing loop, the compiler hoists the
+// polynomial constants of log/acos/pow out of the loop into registers and
+// then spills them for every ray.
+#ifndef RB_COLD_FN
+#define RB_COLD_FN __noinline__
+#endif
+__device__ RB_COLD_FN double cold_log(double x) { return log(x); }
+__device__ RB_COLD_FN double cold_acos(double x) { return acos(x); }
+__device__ RB_COLD_FN double schott_index(const double* a, double lambda) {
+  double l = lambda / 1e-4;
+  return sqrt(a[0] + a[1] * pow(l, 2.) + a[2] * pow(l, -2.) + a[3] * pow(l, -4.) + a[4] * pow(l, -6.) +
+              a[5] * pow(l, -8.));
+}
 __device__ inline double lens_index(const DScene& S, const rb_volume_desc& v, double lambda) {
   if (v.index_model == RB_INDEX_TABLE) return table_eval(S, v.index_ref, lambda);
   if (v.index_model == RB_INDEX_SELLMEIER) {
@@ -1213,12 +1227,7 @@ __device__ inline double lens_index(const DScene& S, const rb_volume_desc& v, do
     double l2 = l * l;
     return sqrt(1 + b[0] * l2 / (l2 - b[3]) + b[1] * l2 / (l2 - b[4]) + b[2] * l2 / (l2 - b[5]));
   }
-  if (v.index_model == RB_INDEX_SCHOTT) {
-    const double* a = S.params + v.index_ref;
-    double l = lambda / 1e-4;
-    return sqrt(a[0] + a[1] * pow(l, 2.) + a[2] * pow(l, -2.) + a[3] * pow(l, -4.) + a[4] * pow(l, -6.) +
-                a[5] * pow(l, -8.));
-  }
+  if (v.index_model == RB_INDEX_SCHOTT) return schott_index(S.params + v.index_ref, lambda);
   return 1.0;
 }
 __device__ inline double lens_k(const DScene& S, const rb_volume_desc& v, double lambda) {

@@ -113,7 +113,10 @@ __device__ int first_daughter_containing(const DScene& S, int n, V3<double> p, i
 }
 
 // TGeoNavigator::SearchNode (up until inside, then down through daughters)
-__device__ RB_NODE_FN int search_node(const DScene& S, int start, V3<double> p, int skip, bool forced) {
+#ifndef RB_SEARCH_FN
+#define RB_SEARCH_FN RB_NODE_FN
+#endif
+__device__ RB_SEARCH_FN int search_node(const DScene& S, int start, V3<double> p, int skip, bool forced) {
   int n = start;
   bool inside = forced || n == skip;
   while (!inside) {
@@ -537,21 +540,20 @@ __device__ void do_reflection(const DScene& S, const TraceParams& P, const RayIO
   add_point(io, i, L, L.p, t, next);
 }
 
-// AOpticsManager::DoFresnel (src/AOpticsManager.cxx:52-167), no multilayer branch
+// AOpticsManager::DoFresnel (src/AOpticsManager.cxx:52-167), no multilayer
+// branch. Refracts the ray, or returns true when it is reflected instead
+// (total internal reflection, or the Fresnel draw): the caller then runs
+// DoReflection with the same normal, so each is inlined once.
 template <typename Rec>
-__device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id, Lane& L,
-                           double n1, double n2, double k2, int next, Rec& rec) {
+__device__ bool do_fresnel(const TraceParams& P, const RayIO& io, int64_t i, uint64_t id, Lane& L, double n1,
+                           double n2, double k2, int next, V3<double> n, Rec& rec) {
   double step = L.step;
-  V3<double> n = find_normal(S, L);
   V3<double> d1 = L.d;
   double cos1 = dot(d1, n);
   double sin1 = sqrt(1 - cos1 * cos1);
   double sin2 = n1 * sin1 / n2;
   double cos2 = sqrt(1 - sin2 * sin2);
-  if (sin2 > 1.) {
-    do_reflection(S, P, io, i, id, L, n1, next, n, rec);
-    return;
-  }
+  if (sin2 > 1.) return true;
   if (P.fresnel) {
     double Rs, Rp;
     if (k2 <= 0.) {
@@ -574,10 +576,7 @@ __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& i
       Rp = ((x1P - x2P) * (x1P - x2P) + y2P * y2P) / ((x1P + x2P) * (x1P + x2P) + y2P * y2P);
     }
     double R = (Rs + Rp) / 2.;
-    if (draw(P, id, L) < R) {
-      do_reflection(S, P, io, i, id, L, n1, next, n, rec);
-      return;
-    }
+    if (draw(P, id, L) < R) return true;
   }
   V3<double> d2 = d1;
   if (sin1 != 0) {
@@ -589,6 +588,7 @@ __device__ void do_fresnel(const DScene& S, const TraceParams& P, const RayIO& i
   add_point(io, i, L, L.p, t, next);
   set_direction(L, d2);
   rec.refract();
+  return false;
 }
 
 // AOpticsManager::TraceNonSequential(TObjArray*) for one ray (src/AOpticsManager.cxx:353-518)
@@ -618,7 +618,7 @@ __device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io
       double k = lens_k(S, v, lambda);
       double absl = k <= 0 ? 0.0 : lambda / (4 * kPi * k);
       if (k > 0 && absl > 0 && isfinite(absl)) {
-        double abs_step = -absl * log(draw(P, id, L));
+        double abs_step = -absl * cold_log(draw(P, id, L));
         if (abs_step < step) {
           double n1 = lens_index(S, v, lambda);
           L.p = axpy(x1, abs_step, L.d);
@@ -629,24 +629,29 @@ __device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io
       }
     }
 
+    // (current, next) transition table; the mirror and the Fresnel cases
+    // share one normal and one DoReflection
     const bool c_free = tc == RB_KNULL || tc == RB_KOPT || tc == RB_KOTHER;
-    if ((c_free || tc == RB_KLENS) && tn == RB_KMIRROR) {
-      double n1 = tc == RB_KLENS ? lens_index(S, vol_of(S, current), lambda) : 1.;
-      do_reflection(S, P, io, i, id, L, n1, next, find_normal(S, L), rec);
-    } else if (c_free && tn == RB_KLENS) {
-      const rb_volume_desc& v = vol_of(S, next);
-      do_fresnel(S, P, io, i, id, L, 1., lens_index(S, v, lambda), lens_k(S, v, lambda), next, rec);
-    } else if ((c_free || tc == RB_KLENS) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
-      double n1 = tc == RB_KLENS ? lens_index(S, vol_of(S, current), lambda) : 1.;
+    const bool c_lens = tc == RB_KLENS;
+    const bool n_free = tn == RB_KNULL || tn == RB_KOPT || tn == RB_KOTHER;
+    const bool mirror = (c_free || c_lens) && tn == RB_KMIRROR;
+    const bool fresnel = (c_free && tn == RB_KLENS) || (c_lens && (tn == RB_KLENS || n_free));
+    if (mirror || fresnel) {
+      const double n1 = c_lens ? lens_index(S, vol_of(S, current), lambda) : 1.;
+      double n2 = 1., k2 = 0.;
+      if (tn == RB_KLENS) {
+        const rb_volume_desc& v = vol_of(S, next);
+        n2 = lens_index(S, v, lambda);
+        k2 = lens_k(S, v, lambda);
+      }
+      const V3<double> n = find_normal(S, L);
+      if (mirror || do_fresnel(P, io, i, id, L, n1, n2, k2, next, n, rec))
+        do_reflection(S, P, io, i, id, L, n1, next, n, rec);
+    } else if ((c_free || c_lens) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
+      double n1 = c_lens ? lens_index(S, vol_of(S, current), lambda) : 1.;
       add_point(io, i, L, L.p, t1 + step / (kC / n1), next);
     } else if (c_free && (tn == RB_KOTHER || tn == RB_KOPT)) {
       add_point(io, i, L, L.p, t1 + step / kC, next);
-    } else if (tc == RB_KLENS && tn == RB_KLENS) {
-      const rb_volume_desc& v = vol_of(S, next);
-      do_fresnel(S, P, io, i, id, L, lens_index(S, vol_of(S, current), lambda), lens_index(S, v, lambda),
-                 lens_k(S, v, lambda), next, rec);
-    } else if (tc == RB_KLENS && (tn == RB_KNULL || tn == RB_KOPT || tn == RB_KOTHER)) {
-      do_fresnel(S, P, io, i, id, L, lens_index(S, vol_of(S, current), lambda), 1., 0., next, rec);
     }
 
     if (tn == RB_KNULL) {
@@ -660,7 +665,7 @@ __device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io
     } else if (tn == RB_KFOCUS) {
       const rb_volume_desc& v = vol_of(S, next);
       double angle = 0.;
-      if (v.qe_angle_table >= 0) angle = acos(dot(L.d, find_normal(S, L)));
+      if (v.qe_angle_table >= 0) angle = cold_acos(dot(L.d, find_normal(S, L)));
       double qe = v.qe_lambda_table >= 0 ? table_eval(S, v.qe_lambda_table, lambda) : 1.;
       if (v.qe_angle_table >= 0) qe *= table_eval(S, v.qe_angle_table, angle);
       L.status = (qe == 1 || draw(P, id, L) < qe) ? RB_FOCUS : RB_STOP;

@@ -112,6 +112,16 @@ __device__ __forceinline__ void load_lane(const RayIO& io, int64_t i, Lane& L) {
   L.hit_node = -1;
   L.hit_leaf = 0;
 }
+// The index handed back through an empty asm: the store addresses are then
+// formed again after the trace instead of being kept (and spilled) across it
+// from the loads.
+#ifndef RB_ADDR_BARRIER
+#define RB_ADDR_BARRIER 1
+#endif
+__device__ __forceinline__ int64_t opaque_index(int64_t i) {
+  if (RB_ADDR_BARRIER) asm volatile("" : "+v"(i));
+  return i;
+}
 __device__ __forceinline__ void store_lane(const RayIO& io, int64_t i, const Lane& L) {
   io.x[i] = L.p.x;
   io.y[i] = L.p.y;
@@ -256,7 +266,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, R
   L.ray = P.id_base + uint64_t(i);
   NoRecord rec;
   trace_lane<MIXED>(S, P, io, i, L, rec);
-  store_lane(io, i, L);
+  store_lane(io, opaque_index(i), L);
 }
 
 // Adjoint of the focal-plane loss: each lane recomputes its ray's bounce
