@@ -64,7 +64,12 @@ struct DScene {
   const rb_table_desc* tables;
   const double* table_xy;
   int32_t n_nodes;
+  // BVH entries [stage_lo, stage_lo + stage_n) -- the top node's BVH, at
+  // most kBvhStage entries -- are read from an LDS copy that each tracing
+  // kernel makes at entry (stage_n = 0: none).
+  int32_t stage_lo, stage_n;
 };
+constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 
 template <typename T>
 struct V3 {

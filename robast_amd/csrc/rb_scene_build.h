@@ -235,6 +235,7 @@ struct SceneLayout {
   int32_t n_nodes;
   bool has_top_box = false;  // the top node has daughters: their bounding box follows
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
+  int32_t top_bvh_lo = 0, top_bvh_n = 0;  // the top node's BVH entries
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -285,6 +286,8 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
       lay.top_lo[k] = r.lo[k];
       lay.top_hi[k] = r.hi[k];
     }
+    lay.top_bvh_lo = nodes[0].bvh_root;
+    lay.top_bvh_n = r.escape - nodes[0].bvh_root;
   }
   return 0;
 }
@@ -301,6 +304,8 @@ inline DScene scene_view(const char* base, const SceneLayout& lay) {
   s.tables = reinterpret_cast<const rb_table_desc*>(base + lay.tables);
   s.table_xy = reinterpret_cast<const double*>(base + lay.table_xy);
   s.n_nodes = lay.n_nodes;
+  s.stage_lo = 0;
+  s.stage_n = 0;
   return s;
 }
 

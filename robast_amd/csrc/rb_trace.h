@@ -7,6 +7,28 @@ namespace rbd {
 
 #ifdef RB_HOST_EMULATION
 inline long rb_emul_fallbacks = 0;
+// host views are never staged (stage_n = 0)
+inline DBvh bvh_node(const DScene& S, int i) { return S.bvh[i]; }
+#else
+// LDS copy of the top node's BVH (DScene::stage_lo / stage_n), written by
+// stage_bvh() at the entry of every kernel that navigates.
+__shared__ __attribute__((aligned(16))) DBvh g_bvh_stage[kBvhStage];  // read as float4 pairs
+
+// BVH entry i, from the LDS copy when staged. The two reads stay in separate
+// branches (and types) so that the LDS one is a ds_read, not a flat load
+// through a selected generic pointer.
+__device__ __forceinline__ DBvh bvh_node(const DScene& S, int i) {
+  const unsigned k = unsigned(i - S.stage_lo);
+  DBvh b;
+  if (k < unsigned(S.stage_n)) {
+    const float4* l = reinterpret_cast<const float4*>(&g_bvh_stage[k]);
+    const float4 a = l[0], c = l[1];
+    b = {{a.x, a.y, a.z}, __float_as_int(a.w), {c.x, c.y, c.z}, __float_as_int(c.w)};
+  } else {
+    b = S.bvh[i];
+  }
+  return b;
+}
 #endif
 
 struct RayIO {
@@ -99,9 +121,9 @@ __device__ int first_daughter_containing(const DScene& S, int n, V3<double> p, i
   const float fx = float(p.x), fy = float(p.y), fz = float(p.z);
   int found = 0x7fffffff;
   int i = nd.bvh_root;
-  const int end = S.bvh[i].escape;
+  const int end = bvh_node(S, i).escape;
   while (i < end) {
-    const DBvh b = S.bvh[i];
+    const DBvh b = bvh_node(S, i);
     if (!box_has(b, fx, fy, fz)) {
       i = b.escape;
       continue;
@@ -186,9 +208,9 @@ __device__ __forceinline__ int gather_leaves(const DScene& S, int root, float ox
                                              float iy, float iz, float lim, Cand cand, float* cut) {
   int i = root, nc = 0;
   float drop = 3.0e38f;
-  const int end = S.bvh[i].escape;
+  const int end = bvh_node(S, i).escape;
   while (i < end) {
-    const DBvh b = S.bvh[i];
+    const DBvh b = bvh_node(S, i);
     float tn;
     if (!box_pass(b, ox, oy, oz, ix, iy, iz, fminf(lim, drop), tn)) {
       i = b.escape;
@@ -278,9 +300,9 @@ __device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<
   }
 #endif
   int i = cn.bvh_root;
-  const int end = S.bvh[i].escape;
+  const int end = bvh_node(S, i).escape;
   while (i < end) {
-    const DBvh b = S.bvh[i];
+    const DBvh b = bvh_node(S, i);
     float tn;
     if (!box_pass(b, ox, oy, oz, ix, iy, iz, float(best) * 1.0001f + 1e-2f, tn)) {
       i = b.escape;
@@ -380,9 +402,9 @@ __device__ bool search_boundary_mixed(const DScene& S, int cur, V3<double> p, V3
 #endif
     if (walk) {
       int i = cn.bvh_root;
-      const int end = S.bvh[i].escape;
+      const int end = bvh_node(S, i).escape;
       while (i < end) {
-        const DBvh b = S.bvh[i];
+        const DBvh b = bvh_node(S, i);
         float tn;
         if (!box_pass(b, ox, oy, oz, ix, iy, iz, float(fmin(best, second)) * 1.0001f + 1e-2f, tn)) {
           i = b.escape;

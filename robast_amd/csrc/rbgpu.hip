@@ -19,7 +19,7 @@ using namespace rbd;
 
 struct rb_ctx {
   int device = 0;
-  int cus = 0;                                // compute units (persistent grid size)
+  int cus = 0;                                // compute units (grid sizes of the striding kernels)
   unsigned long long* next_ray = nullptr;     // persistent kernels' ray counter (device)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
@@ -135,6 +135,13 @@ __device__ __forceinline__ void store_lane(const RayIO& io, int64_t i, const Lan
   if (io.last_node) io.last_node[i] = L.last_node;
 }
 
+// Copies the top node's BVH into LDS. Called by every navigating kernel at
+// entry, by all threads of the block, before any thread returns.
+__device__ __forceinline__ void stage_bvh(const DScene& S) {
+  for (int k = threadIdx.x; k < S.stage_n; k += blockDim.x) g_bvh_stage[k] = S.bvh[S.stage_lo + k];
+  __syncthreads();
+}
+
 // Coherence key of each ray: a 30-bit Morton code of the point where it first
 // enters one of the top volume's daughter boxes (box tests only, over the top
 // node's BVH), quantised over the daughters' joint box. Rays that are not
@@ -161,38 +168,53 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
 #define RB_ORDER_TILE_BITS 19
 #endif
 constexpr int kOrderTileBits = RB_ORDER_TILE_BITS;
+#ifndef RB_ORDER_AXIS_BITS
+#define RB_ORDER_AXIS_BITS 10
+#endif
+constexpr int kOrderAxisBits = RB_ORDER_AXIS_BITS;
+// The top node's BVH (its first kOrderStage entries) is staged in LDS once
+// per block: the walk is a chain of dependent loads, and LDS answers them at
+// a fraction of the L2 latency. Blocks stride over the rays so that the
+// staging is paid once per block, not once per 256 rays.
+constexpr int kOrderStage = 1024;  // 32 KB of DBvh
 __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n, float lox, float loy, float loz,
                                                    float sx, float sy, float sz, uint64_t* key, uint32_t* idx) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t k = 0xffffffffu;
-  if (io.status[i] == RB_RUN) {
-    const float ox = float(io.x[i]), oy = float(io.y[i]), oz = float(io.z[i]);
-    const float dx = float(io.dx[i]), dy = float(io.dy[i]), dz = float(io.dz[i]);
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-    const int root = S.nodes[0].bvh_root;
-    float best = 3.0e38f;
-    int b = root;
-    const int end = S.bvh[root].escape;
-    while (b < end) {
-      const DBvh box = S.bvh[b];
-      float tn;
-      if (!box_pass(box, ox, oy, oz, ix, iy, iz, best, tn)) {
-        b = box.escape;
-        continue;
+  __shared__ DBvh stage[kOrderStage];
+  const int root = S.nodes[0].bvh_root;
+  const int end = S.bvh[root].escape;
+  const int staged = min(end - root, kOrderStage);
+  for (int k = threadIdx.x; k < staged; k += blockDim.x) stage[k] = S.bvh[root + k];
+  __syncthreads();
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t k = 0xffffffffu;
+    if (io.status[i] == RB_RUN) {
+      const float ox = float(io.x[i]), oy = float(io.y[i]), oz = float(io.z[i]);
+      const float dx = float(io.dx[i]), dy = float(io.dy[i]), dz = float(io.dz[i]);
+      const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+      float best = 3.0e38f;
+      int b = root;
+      while (b < end) {
+        const DBvh box = b - root < staged ? stage[b - root] : S.bvh[b];
+        float tn;
+        if (!box_pass(box, ox, oy, oz, ix, iy, iz, best, tn)) {
+          b = box.escape;
+          continue;
+        }
+        if (box.prim >= 0) best = fminf(best, tn);
+        b++;
       }
-      if (box.prim >= 0) best = fminf(best, tn);
-      b++;
+      if (best < 3.0e38f) {
+        const float px = (ox + best * dx - lox) * sx, py = (oy + best * dy - loy) * sy,
+                    pz = (oz + best * dz - loz) * sz;
+        const uint32_t qx = uint32_t(fminf(fmaxf(px, 0.0f), 1023.0f)), qy = uint32_t(fminf(fmaxf(py, 0.0f), 1023.0f)),
+                       qz = uint32_t(fminf(fmaxf(pz, 0.0f), 1023.0f));
+        k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
+      }
     }
-    if (best < 3.0e38f) {
-      const float px = (ox + best * dx - lox) * sx, py = (oy + best * dy - loy) * sy, pz = (oz + best * dz - loz) * sz;
-      const uint32_t qx = uint32_t(fminf(fmaxf(px, 0.0f), 1023.0f)), qy = uint32_t(fminf(fmaxf(py, 0.0f), 1023.0f)),
-                     qz = uint32_t(fminf(fmaxf(pz, 0.0f), 1023.0f));
-      k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
-    }
+    key[i] = (uint64_t(i >> kOrderTileBits) << 32) | k;
+    idx[i] = uint32_t(i);
   }
-  key[i] = (uint64_t(i >> kOrderTileBits) << 32) | k;
-  idx[i] = uint32_t(i);
 }
 
 // Persistent form: a grid sized to the machine, each lane taking the next
@@ -209,6 +231,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
                                                                           unsigned long long* next_ray) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
+  stage_bvh(S);
   Lane L;
   L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
   NoRecord rec;
@@ -256,6 +279,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, R
                                                                   const uint32_t* perm) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
+  stage_bvh(S);
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t i = perm ? int64_t(perm[t]) : t;
@@ -280,7 +304,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DScene S,
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   for (int k = threadIdx.x; k < nacc; k += blockDim.x) acc[k] = 0.0;
-  __syncthreads();
+  stage_bvh(S);
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = t < n ? (perm ? int64_t(perm[t]) : t) : n;
   if (i < n && io.status[i] == RB_RUN) {
@@ -414,8 +438,10 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
   unsigned tile_bits = 0;
   while ((int64_t(1) << tile_bits) < ((n - 1) >> kOrderTileBits) + 1) tile_bits++;
   const unsigned end_bit = 32u + tile_bits;
+  // only the top kOrderAxisBits of each axis' 10-bit cell index are sorted on
+  const unsigned begin_bit = 30u - 3u * unsigned(kOrderAxisBits);
   size_t bytes = 0;
-  HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, idx_in, idx_out, count, 0u, end_bit,
+  HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, idx_in, idx_out, count, begin_bit, end_bit,
                                         ctx->stream));
   if (bytes > ctx->sort_tmp_bytes) {
     if (ctx->sort_tmp) HIPCHK(ctx, hipFree(ctx->sort_tmp));
@@ -429,12 +455,13 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
     const float ext = scene->top_hi[k] - scene->top_lo[k];
     s[k] = ext > 0.0f ? 1023.0f / ext : 0.0f;
   }
-  const int64_t grid = (n + 255) / 256;
+  // 8 blocks of 256 per CU, each striding over the rays
+  const int64_t grid = std::min<int64_t>((n + 255) / 256, int64_t(ctx->cus) * 8);
   hipLaunchKernelGGL(k_order_key, dim3(unsigned(grid)), dim3(256), 0, ctx->stream, scene->s, io, n, scene->top_lo[0],
                      scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], key_in, idx_in);
   HIPCHK(ctx, hipGetLastError());
   bytes = ctx->sort_tmp_bytes;
-  HIPCHK(ctx, rocprim::radix_sort_pairs(ctx->sort_tmp, bytes, key_in, key_out, idx_in, idx_out, count, 0u, end_bit,
+  HIPCHK(ctx, rocprim::radix_sort_pairs(ctx->sort_tmp, bytes, key_in, key_out, idx_in, idx_out, count, begin_bit, end_bit,
                                         ctx->stream));
   *perm = idx_out;
   return 0;
@@ -452,8 +479,9 @@ int rbgpu_ctx_create(int device, rb_ctx** out) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess) {
-    delete c;
+      hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess ||
+      hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->cus <= 0) {
+    rbgpu_ctx_destroy(c);
     return -2;
   }
   *out = c;
@@ -513,6 +541,8 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
     return fail(ctx, -2, std::string("hipMemcpy scene: ") + hipGetErrorString(e));
   }
   sc->s = scene_view(static_cast<const char*>(sc->dmem), lay);
+  sc->s.stage_lo = lay.top_bvh_lo;
+  sc->s.stage_n = std::min(lay.top_bvh_n, int32_t(kBvhStage));
   sc->has_top_box = lay.has_top_box;
   for (int k = 0; k < 3; k++) {
     sc->top_lo[k] = lay.top_lo[k];
@@ -606,11 +636,6 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
     if (rc) return rc;
   }
   if (kPersistent && !ctx->next_ray) HIPCHK(ctx, hipMalloc(&ctx->next_ray, sizeof(unsigned long long)));
-  if (kPersistent && ctx->cus == 0) {
-    hipDeviceProp_t prop;
-    HIPCHK(ctx, hipGetDeviceProperties(&prop, ctx->device));
-    ctx->cus = prop.multiProcessorCount;
-  }
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
   if (kPersistent) {
     int per_cu = 0;
