@@ -145,7 +145,7 @@ __device__ __forceinline__ void stage_bvh(const DScene& S) {
 // Coherence key of each ray: a 30-bit Morton code of the point where it first
 // enters one of the top volume's daughter boxes (box tests only, over the top
 // node's BVH), quantised over the daughters' joint box. Rays that are not
-// running, or cross no daughter box, get the largest key and sort last.
+// running, or cross no daughter box, get the last cell and sort last.
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {
   v &= 0x3ffu;
   v = (v | (v << 16)) & 0x030000ffu;
@@ -168,26 +168,23 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
 #define RB_ORDER_TILE_BITS 19
 #endif
 constexpr int kOrderTileBits = RB_ORDER_TILE_BITS;
-#ifndef RB_ORDER_AXIS_BITS
-#define RB_ORDER_AXIS_BITS 10
-#endif
-constexpr int kOrderAxisBits = RB_ORDER_AXIS_BITS;
-// The top node's BVH (its first kOrderStage entries) is staged in LDS once
-// per block: the walk is a chain of dependent loads, and LDS answers them at
-// a fraction of the L2 latency. Blocks stride over the rays so that the
-// staging is paid once per block, not once per 256 rays.
-constexpr int kOrderStage = 1024;  // 32 KB of DBvh
+// Keys are 32-bit: the tile index in the high bits and the top
+// (32 - tile bits) bits of the 30-bit Morton code below it (all 30 up to
+// 4 tiles; 27 = 9 bits per axis at 1e7 rays), so that the radix sort makes
+// 4 passes over 8-byte pairs instead of 5 over 12-byte ones.
+// The walk reads the top node's BVH through the same LDS copy as the
+// tracing kernels (stage_bvh / bvh_node): it is a chain of dependent loads,
+// and LDS answers them at a fraction of the L2 latency. Blocks stride over
+// the rays so that the staging is paid once per block.
 __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n, float lox, float loy, float loz,
-                                                   float sx, float sy, float sz, uint64_t* key, uint32_t* idx) {
-  __shared__ DBvh stage[kOrderStage];
+                                                   float sx, float sy, float sz, int mbits, uint32_t* key,
+                                                   uint32_t* idx) {
+  stage_bvh(S);
   const int root = S.nodes[0].bvh_root;
-  const int end = S.bvh[root].escape;
-  const int staged = min(end - root, kOrderStage);
-  for (int k = threadIdx.x; k < staged; k += blockDim.x) stage[k] = S.bvh[root + k];
-  __syncthreads();
+  const int end = bvh_node(S, root).escape;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint32_t k = 0xffffffffu;
+    uint32_t k = 0x3fffffffu;  // not running or no daughter box crossed: the last cell
     if (io.status[i] == RB_RUN) {
       const float ox = float(io.x[i]), oy = float(io.y[i]), oz = float(io.z[i]);
       const float dx = float(io.dx[i]), dy = float(io.dy[i]), dz = float(io.dz[i]);
@@ -195,7 +192,7 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
       float best = 3.0e38f;
       int b = root;
       while (b < end) {
-        const DBvh box = b - root < staged ? stage[b - root] : S.bvh[b];
+        const DBvh box = bvh_node(S, b);
         float tn;
         if (!box_pass(box, ox, oy, oz, ix, iy, iz, best, tn)) {
           b = box.escape;
@@ -212,7 +209,8 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
         k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
       }
     }
-    key[i] = (uint64_t(i >> kOrderTileBits) << 32) | k;
+    // tile index above the top mbits (<= 30) Morton bits
+    key[i] = (uint32_t(i >> kOrderTileBits) << mbits) | (k >> (30 - mbits));
     idx[i] = uint32_t(i);
   }
 }
@@ -426,20 +424,19 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
     if (ctx->order_buf) HIPCHK(ctx, hipFree(ctx->order_buf));
     ctx->order_buf = nullptr;
     ctx->order_cap = 0;
-    HIPCHK(ctx, hipMalloc(&ctx->order_buf, (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t)) * size_t(n)));
+    HIPCHK(ctx, hipMalloc(&ctx->order_buf, 4 * sizeof(uint32_t) * size_t(n)));
     ctx->order_cap = size_t(n);
   }
-  uint64_t* key_in = reinterpret_cast<uint64_t*>(ctx->order_buf);
-  uint64_t* key_out = key_in + ctx->order_cap;
+  uint32_t* key_in = reinterpret_cast<uint32_t*>(ctx->order_buf);
+  uint32_t* key_out = key_in + ctx->order_cap;
   uint32_t* idx_in = reinterpret_cast<uint32_t*>(key_out + ctx->order_cap);
   uint32_t* idx_out = idx_in + ctx->order_cap;
   const unsigned count = unsigned(n);
   // sort on the Morton bits and as many tile-index bits as there are tiles
   unsigned tile_bits = 0;
   while ((int64_t(1) << tile_bits) < ((n - 1) >> kOrderTileBits) + 1) tile_bits++;
-  const unsigned end_bit = 32u + tile_bits;
-  // only the top kOrderAxisBits of each axis' 10-bit cell index are sorted on
-  const unsigned begin_bit = 30u - 3u * unsigned(kOrderAxisBits);
+  const int mbits = std::min(30, 32 - int(tile_bits));
+  const unsigned begin_bit = 0u, end_bit = unsigned(mbits) + tile_bits;
   size_t bytes = 0;
   HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, idx_in, idx_out, count, begin_bit, end_bit,
                                         ctx->stream));
@@ -458,7 +455,7 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
   // 8 blocks of 256 per CU, each striding over the rays
   const int64_t grid = std::min<int64_t>((n + 255) / 256, int64_t(ctx->cus) * 8);
   hipLaunchKernelGGL(k_order_key, dim3(unsigned(grid)), dim3(256), 0, ctx->stream, scene->s, io, n, scene->top_lo[0],
-                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], key_in, idx_in);
+                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], mbits, key_in, idx_in);
   HIPCHK(ctx, hipGetLastError());
   bytes = ctx->sort_tmp_bytes;
   HIPCHK(ctx, rocprim::radix_sort_pairs(ctx->sort_tmp, bytes, key_in, key_out, idx_in, idx_out, count, begin_bit, end_bit,
