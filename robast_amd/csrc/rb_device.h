@@ -68,8 +68,16 @@ struct DScene {
   // most kBvhStage entries -- are read from an LDS copy that each tracing
   // kernel makes at entry (stage_n = 0: none).
   int32_t stage_lo, stage_n;
+  // deepest boolean nesting the navigation code is compiled for
+  static constexpr int max_depth = kMaxCsgDepth;
 };
 constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
+// The same scene, for kernels compiled for scenes whose composites nest at
+// most MAXD levels: the deeper composite call sites, and the registers they
+// make the kernel spill around them, are left out.
+template <int MAXD> struct DSceneOf : DScene {
+  static constexpr int max_depth = MAXD;
+};
 
 template <typename T>
 struct V3 {
@@ -1143,35 +1151,36 @@ template <typename T> struct Csg<T, 0> {
 };
 
 // Entry points by the shape's own nesting depth, so that a primitive or a
-// single boolean level does not pay for the deepest unrolled form.
-template <typename T>
-__device__ __forceinline__ V3<T> shape_normal(const DScene& S, int shape, uint32_t leaf, V3<T> q, V3<T> d) {
+// single boolean level does not pay for the deepest unrolled form. Scenes
+// typed DSceneOf<1> compile no deeper form at all.
+template <typename T, typename SC>
+__device__ __forceinline__ V3<T> shape_normal(const SC& S, int shape, uint32_t leaf, V3<T> q, V3<T> d) {
   const uint32_t path = leaf & 0xffffffu;
   const int depth = int(leaf >> 24);
   switch (S.shapes[shape].depth) {
     case 0: return Csg<T, 0>::normal(S, shape, path, depth, q, d);
     case 1: return Csg<T, 1>::normal(S, shape, path, depth, q, d);
   }
-  return Csg<T, kMaxCsgDepth>::normal(S, shape, path, depth, q, d);
+  return Csg<T, SC::max_depth>::normal(S, shape, path, depth, q, d);
 }
-template <typename T>
-__device__ __forceinline__ bool shape_contains(const DScene& S, int shape, V3<T> q) {
+template <typename T, typename SC>
+__device__ __forceinline__ bool shape_contains(const SC& S, int shape, V3<T> q) {
   switch (S.shapes[shape].depth) {
     case 0: return Csg<T, 0>::contains(S, shape, q);
     case 1: return Csg<T, 1>::contains(S, shape, q);
   }
-  return Csg<T, kMaxCsgDepth>::contains(S, shape, q);
+  return Csg<T, SC::max_depth>::contains(S, shape, q);
 }
-template <typename T>
-__device__ __forceinline__ Hit shape_dist(const DScene& S, int shape, V3<T> q, V3<T> d, bool out) {
+template <typename T, typename SC>
+__device__ __forceinline__ Hit shape_dist(const SC& S, int shape, V3<T> q, V3<T> d, bool out) {
   switch (S.shapes[shape].depth) {
     case 0: return out ? Csg<T, 0>::dist_out(S, shape, q, d) : Csg<T, 0>::dist_in(S, shape, q, d);
     case 1: return out ? Csg<T, 1>::dist_out(S, shape, q, d) : Csg<T, 1>::dist_in(S, shape, q, d);
   }
-  return out ? Csg<T, kMaxCsgDepth>::dist_out(S, shape, q, d) : Csg<T, kMaxCsgDepth>::dist_in(S, shape, q, d);
+  return out ? Csg<T, SC::max_depth>::dist_out(S, shape, q, d) : Csg<T, SC::max_depth>::dist_in(S, shape, q, d);
 }
-template <typename T>
-__device__ __forceinline__ double shape_polish(const DScene& S, int shape, uint32_t leaf, bool exiting, V3<T> q,
+template <typename T, typename SC>
+__device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t leaf, bool exiting, V3<T> q,
                                                V3<T> d, T s0, T back) {
   const uint32_t path = leaf & 0xffffffu;
   const int depth = int(leaf >> 24);
@@ -1179,7 +1188,7 @@ __device__ __forceinline__ double shape_polish(const DScene& S, int shape, uint3
     case 0: return Csg<T, 0>::polish(S, shape, path, depth, exiting, q, d, s0, back);
     case 1: return Csg<T, 1>::polish(S, shape, path, depth, exiting, q, d, s0, back);
   }
-  return Csg<T, kMaxCsgDepth>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+  return Csg<T, SC::max_depth>::polish(S, shape, path, depth, exiting, q, d, s0, back);
 }
 
 // ------------------------------------------------------------ physics

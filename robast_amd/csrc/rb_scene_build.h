@@ -236,6 +236,7 @@ struct SceneLayout {
   bool has_top_box = false;  // the top node has daughters: their bounding box follows
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
   int32_t top_bvh_lo = 0, top_bvh_n = 0;  // the top node's BVH entries
+  int32_t max_depth = 0;                  // deepest boolean nesting of any shape
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -267,8 +268,11 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
     }
   }
   std::vector<DShape> shapes(d->n_shapes);
-  for (int i = 0; i < d->n_shapes; i++)
+  lay.max_depth = 0;
+  for (int i = 0; i < d->n_shapes; i++) {
     shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset, csg_depth(d, i, 0), 0};
+    lay.max_depth = std::max(lay.max_depth, shapes[i].depth);
+  }
   blob.clear();
   lay.nodes = put(blob, nodes.data(), nodes.size());
   lay.children = put(blob, d->children, size_t(std::max(d->n_children, 0)));

@@ -104,7 +104,8 @@ struct ChainRecord {
   }
 };
 
-__device__ RB_NODE_FN bool node_contains(const DScene& S, int n, V3<double> p) {
+template <typename SC>
+__device__ RB_NODE_FN bool node_contains(const SC& S, int n, V3<double> p) {
   const DNode& nd = S.nodes[n];
   return shape_contains<double>(S, nd.shape, m2l<double>(nd.rot, nd.tr, p));
 }
@@ -115,7 +116,8 @@ __device__ __forceinline__ bool box_has(const DBvh& b, float x, float y, float z
 
 // First daughter of n (lowest node id, excluding skip) whose shape contains
 // p; stackless walk of n's BVH (depth-first order with escape links).
-__device__ int first_daughter_containing(const DScene& S, int n, V3<double> p, int skip) {
+template <typename SC>
+__device__ int first_daughter_containing(const SC& S, int n, V3<double> p, int skip) {
   const DNode& nd = S.nodes[n];
   if (nd.n_children == 0) return -1;
   const float fx = float(p.x), fy = float(p.y), fz = float(p.z);
@@ -138,7 +140,8 @@ __device__ int first_daughter_containing(const DScene& S, int n, V3<double> p, i
 #ifndef RB_SEARCH_FN
 #define RB_SEARCH_FN RB_NODE_FN
 #endif
-__device__ RB_SEARCH_FN int search_node(const DScene& S, int start, V3<double> p, int skip, bool forced) {
+template <typename SC>
+__device__ RB_SEARCH_FN int search_node(const SC& S, int start, V3<double> p, int skip, bool forced) {
   int n = start;
   bool inside = forced || n == skip;
   while (!inside) {
@@ -168,8 +171,8 @@ __device__ __forceinline__ double locate_push(const DScene& S, int n, double ste
 // distance from a ray (origin p, unit d, FP64) to node c's shape, the
 // search itself in precision T; the origin is first rebased to `base`
 // along the ray (the node's box entry) so FP32 works near the node's frame
-template <typename T>
-__device__ RB_NODE_FN Hit node_dist(const DScene& S, int c, V3<double> p, V3<double> d, double base, bool out) {
+template <typename T, typename SC>
+__device__ RB_NODE_FN Hit node_dist(const SC& S, int c, V3<double> p, V3<double> d, double base, bool out) {
   const DNode& nd = S.nodes[c];
   V3<double> pb = axpy(p, base, d);
   V3<double> q = m2l<double>(nd.rot, nd.tr, pb), ld = m2lv<double>(nd.rot, d);
@@ -180,7 +183,8 @@ __device__ RB_NODE_FN Hit node_dist(const DScene& S, int c, V3<double> p, V3<dou
 }
 
 // FP64 root polish of a crossing found in FP32 (leaf surface of node c)
-__device__ RB_NODE_FN double node_polish(const DScene& S, int c, const Hit& h, V3<double> p, V3<double> d) {
+template <typename SC>
+__device__ RB_NODE_FN double node_polish(const SC& S, int c, const Hit& h, V3<double> p, V3<double> d) {
   const DNode& nd = S.nodes[c];
   V3<double> q = m2l<double>(nd.rot, nd.tr, p), ld = m2lv<double>(nd.rot, d);
   const double back = 2e-3 + 1e-6 * h.s;
@@ -267,7 +271,8 @@ __device__ __forceinline__ void rank_fp64(int c, const Hit& h, double& best, int
 // the best crossing (a shape is never met before its box). If a box that
 // did not fit could still hold a nearer crossing, the one-pass walk decides.
 // The ranking rule makes the result independent of the test order.
-__device__ void search_boundary_fp64(const DScene& S, int cur, V3<double> p, V3<double> d, double& best,
+template <typename SC>
+__device__ void search_boundary_fp64(const SC& S, int cur, V3<double> p, V3<double> d, double& best,
                                      int& crossed, Hit& hit, Cand cand) {
   const DNode& cn = S.nodes[cur];
   Hit hx = node_dist<double>(S, cur, p, d, 0.0, false);
@@ -330,7 +335,8 @@ constexpr double kNearF32 = 2e-2;
 __device__ __forceinline__ bool fp64_only(const DScene& S, int node) {
   return S.shapes[S.nodes[node].shape].depth > 0;
 }
-__device__ __forceinline__ void rank_mixed(const DScene& S, int c, float tn, V3<double> p, V3<double> d,
+template <typename SC>
+__device__ __forceinline__ void rank_mixed(const SC& S, int c, float tn, V3<double> p, V3<double> d,
                                            double& best, double& second, int& crossed, Hit& hit, bool& exact) {
   Hit h;
   bool hexact = fp64_only(S, c);
@@ -353,7 +359,8 @@ __device__ __forceinline__ void rank_mixed(const DScene& S, int c, float tn, V3<
   }
 }
 
-__device__ bool search_boundary_mixed(const DScene& S, int cur, V3<double> p, V3<double> d, double& best,
+template <typename SC>
+__device__ bool search_boundary_mixed(const SC& S, int cur, V3<double> p, V3<double> d, double& best,
                                       int& crossed, Hit& hit, Cand cand) {
   const DNode& cn = S.nodes[cur];
   best = kBig;
@@ -432,8 +439,8 @@ __device__ bool search_boundary_mixed(const DScene& S, int cur, V3<double> p, V3
 
 // TGeoNavigator::FindNextBoundaryAndStep; returns the next node (-1 = null).
 // MIXED: the search runs in FP32 and the winning crossing is polished in FP64.
-template <bool MIXED>
-__device__ int next_boundary(const DScene& S, Lane& L) {
+template <bool MIXED, typename SC>
+__device__ int next_boundary(const SC& S, Lane& L) {
   double extra = L.on_b ? kTol : 0.0;
   L.on_b = false;
   L.p = axpy(L.p, extra, L.d);
@@ -497,7 +504,8 @@ __device__ int next_boundary(const DScene& S, Lane& L) {
   return L.cur;
 }
 
-__device__ V3<double> find_normal(const DScene& S, const Lane& L) {
+template <typename SC>
+__device__ V3<double> find_normal(const SC& S, const Lane& L) {
   if (L.hit_node < 0) return mk(0.0, 0.0, 1.0);
   const DNode& nd = S.nodes[L.hit_node];
   V3<double> lp = m2l<double>(nd.rot, nd.tr, L.p), ld = m2lv<double>(nd.rot, L.d);
@@ -531,8 +539,8 @@ __device__ __forceinline__ void set_direction(Lane& L, V3<double> v) {
 }
 
 // AOpticsManager::DoReflection (src/AOpticsManager.cxx:170-247)
-template <typename Rec>
-__device__ void do_reflection(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id,
+template <typename Rec, typename SC>
+__device__ void do_reflection(const SC& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id,
                               Lane& L, double n1, int next, V3<double> n, Rec& rec) {
   double step = L.step;
   V3<double> d1 = L.d;
@@ -615,15 +623,16 @@ __device__ bool do_fresnel(const TraceParams& P, const RayIO& io, int64_t i, uin
 
 // AOpticsManager::TraceNonSequential(TObjArray*) for one ray (src/AOpticsManager.cxx:353-518)
 // Locate the ray's starting point (TGeoNavigator::InitTrack).
-__device__ __forceinline__ void begin_lane(const DScene& S, Lane& L) {
+template <typename SC>
+__device__ __forceinline__ void begin_lane(const SC& S, Lane& L) {
   L.on_b = false;
   L.cur = node_contains(S, 0, L.p) ? search_node(S, 0, L.p, -1, true) : -1;
 }
 
 // One segment of the per-ray state machine: next boundary, then the
 // interaction the (current, next) volume kinds select.
-template <bool MIXED, typename Rec>
-__device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
+template <bool MIXED, typename Rec, typename SC>
+__device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
   const uint64_t id = L.ray;
   const double lambda = L.lambda;
   {
@@ -696,8 +705,8 @@ __device__ void step_lane(const DScene& S, const TraceParams& P, const RayIO& io
   }
 }
 
-template <bool MIXED, typename Rec>
-__device__ void trace_lane(const DScene& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
+template <bool MIXED, typename Rec, typename SC>
+__device__ void trace_lane(const SC& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
   begin_lane(S, L);
   while (L.status == RB_RUN) step_lane<MIXED>(S, P, io, i, L, rec);
 }

@@ -38,11 +38,19 @@ struct rb_scene {
   rb_ctx* ctx = nullptr;
   void* dmem = nullptr;
   DScene s{};
+  int max_depth = 0;  // deepest boolean nesting among the scene's shapes
   bool has_top_box = false;
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
 };
 
 namespace {
+
+// The scene as the kernels compiled for composites at most MAXD deep take it.
+template <int MAXD> DSceneOf<MAXD> scene_of(const rb_scene* sc) {
+  DSceneOf<MAXD> s;
+  static_cast<DScene&>(s) = sc->s;
+  return s;
+}
 
 int fail(rb_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -272,9 +280,9 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
 
 // One lane per ray. With `perm`, lane t traces ray perm[t] (the coherence
 // order); the ray keeps its own index for its Philox stream and its slots.
-template <bool MIXED>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, RayIO io, int64_t n, TraceParams P,
-                                                                  const uint32_t* perm) {
+template <bool MIXED, int MAXD>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD> S, RayIO io, int64_t n,
+                                                                  TraceParams P, const uint32_t* perm) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   stage_bvh(S);
@@ -294,9 +302,10 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DScene S, R
 // Adjoint of the focal-plane loss: each lane recomputes its ray's bounce
 // chain (nothing is stored by the forward pass) and accumulates parameter
 // gradients in LDS (FP64), flushed with one global atomic per parameter.
-template <bool MIXED>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DScene S, RayIO io, int64_t n, TraceParams P, AdjParams A,
-                                                         double* grad, const uint32_t* perm) {
+template <bool MIXED, int MAXD>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DSceneOf<MAXD> S, RayIO io, int64_t n,
+                                                                    TraceParams P, AdjParams A, double* grad,
+                                                                    const uint32_t* perm) {
   __shared__ double acc[kMaxAdjAcc];
   const int nacc = adj_acc_size(A.mode, A.n_params, A.n_groups);
   __shared__ int32_t cand_node[kCand * kCandStride];
@@ -540,6 +549,7 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
   sc->s = scene_view(static_cast<const char*>(sc->dmem), lay);
   sc->s.stage_lo = lay.top_bvh_lo;
   sc->s.stage_n = std::min(lay.top_bvh_n, int32_t(kBvhStage));
+  sc->max_depth = lay.max_depth;
   sc->has_top_box = lay.has_top_box;
   for (int k = 0; k < 3; k++) {
     sc->top_lo[k] = lay.top_lo[k];
@@ -648,9 +658,19 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
       hipLaunchKernelGGL(k_trace_persist<false>, dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream, scene->s, io,
                          n, P, ctx->next_ray);
   } else if (mixed) {
-    hipLaunchKernelGGL(k_trace<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, perm);
+    if (scene->max_depth <= 1)
+      hipLaunchKernelGGL((k_trace<true, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene), io,
+                         n, P, perm);
+    else
+      hipLaunchKernelGGL((k_trace<true, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                         scene_of<kMaxCsgDepth>(scene), io, n, P, perm);
   } else {
-    hipLaunchKernelGGL(k_trace<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, perm);
+    if (scene->max_depth <= 1)
+      hipLaunchKernelGGL((k_trace<false, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene),
+                         io, n, P, perm);
+    else
+      hipLaunchKernelGGL((k_trace<false, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                         scene_of<kMaxCsgDepth>(scene), io, n, P, perm);
   }
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
@@ -768,12 +788,19 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
     if (rc) return rc;
   }
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-  if (opts->precision == RB_PRECISION_MIXED)
-    hipLaunchKernelGGL(k_adjoint<true>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, A, grad,
-                       perm);
+  const bool shallow = scene->max_depth <= 1;
+  if (opts->precision == RB_PRECISION_MIXED && shallow)
+    hipLaunchKernelGGL((k_adjoint<true, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene), io,
+                       n, P, A, grad, perm);
+  else if (opts->precision == RB_PRECISION_MIXED)
+    hipLaunchKernelGGL((k_adjoint<true, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                       scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm);
+  else if (shallow)
+    hipLaunchKernelGGL((k_adjoint<false, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene),
+                       io, n, P, A, grad, perm);
   else
-    hipLaunchKernelGGL(k_adjoint<false>, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene->s, io, n, P, A,
-                       grad, perm);
+    hipLaunchKernelGGL((k_adjoint<false, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                       scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
