@@ -201,6 +201,22 @@ def main():
     rms_um = float(torch.sqrt(d2[both].mean())) * 1e4
     ms_fp64, ms_mixed = (ms_per_step, ms_o) if args.precision == "fp64" else (ms_o, ms_per_step)
     kms_fp64, kms_mixed = (kms, kms_o) if args.precision == "fp64" else (kms_o, kms)
+    # spot analysis after the trace: containment radius (D80) of the batch's
+    # focal-plane hits, histogram filled and searched on the device
+    hist = A.rb_hist2d_desc()
+    hist.nx = hist.ny = 1000
+    hist.xlo, hist.xhi, hist.ylo, hist.yhi = -120.0, 120.0, -120.0, 120.0
+    hist.status, hist.device_ptrs = A.RB_FOCUS, 1
+    spot_ms, search_ms = [], []
+    for _ in range(3):
+        d80 = ctx.containment_radius(rays, N, hist, 0.8)
+        spot_ms.append(ctx.last_trace_ms())
+        search_ms.append(ctx.last_kernel_ms())
+    spot = {"d80_radius_cm": round(d80[0], 4), "centre_cm": [round(d80[1], 4), round(d80[2], 4)],
+            "device_ms": round(min(spot_ms), 3), "search_ms": round(min(search_ms), 3),
+            "fill_and_compact_ms": round(min(s - k for s, k in zip(spot_ms, search_ms)), 3),
+            "workload": "AGeoUtil::ContainmentRadius(0.80) of the focused rays of the batch (all field angles), "
+                        "TH2D 1000 x 1000 bins over +-120 cm"}
     del src, work, res_o, rays
     torch.cuda.empty_cache()
 
@@ -258,6 +274,7 @@ def main():
         "mixed_kernel_speedup": round(kms_fp64 / kms_mixed, 3),
         "mixed_vs_fp64": {"status_flips": flips, "focused_hit_rms_um": round(rms_um, 6)},
         "adjoint": adjoint,
+        "spot": spot,
         "configs": per_config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,

@@ -286,19 +286,44 @@ enum { RB_ADJ_CENTROID = 0, RB_ADJ_MOMENTS = 1 };
 int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const rb_rays* rays, int64_t n,
                   const rb_adjoint_desc* adj, double* grad);
 
-/* Milliseconds the last rbgpu_trace (or rbgpu_adjoint) spent on the device,
- * ordering included, timed with HIP events on the context's stream, and the
- * number of kernel launches it made. */
+/* Milliseconds the last rbgpu_trace (or rbgpu_adjoint, or
+ * rbgpu_containment_radius) spent on the device, ordering (histogram fill)
+ * included, timed with HIP events on the context's stream, and the number of
+ * kernel launches it made. */
 double rbgpu_last_trace_ms(const rb_ctx* ctx);
-/* Milliseconds of the last call's trace (or adjoint) kernel alone, without
- * the ordering kernels. No counterpart in the reference (measurement). */
+/* Milliseconds of the last call's trace (adjoint, containment search) kernel
+ * alone, without the ordering (fill) kernels. No counterpart in the
+ * reference (measurement). */
 double rbgpu_last_kernel_ms(const rb_ctx* ctx);
 int rbgpu_last_trace_launches(const rb_ctx* ctx);
 
+/* ---- spot analysis after the trace -------------------------------------- */
+/* Binning of the spot histogram, as ROOT's TH2D("", "", nx, xlo, xhi, ny,
+ * ylo, yhi) that the reference's callers fill with the focal-plane hits
+ * (tutorials/MST.C:272-277, tutorials/unittest_robast.py:590-610). */
+typedef struct rb_hist2d_desc {
+  int32_t nx, ny;          /* bins per axis; nx * ny <= 1 << 26               */
+  double xlo, xhi;         /* x axis range, xhi > xlo                          */
+  double ylo, yhi;         /* y axis range, yhi > ylo                          */
+  int32_t status;          /* the rays counted (RB_FOCUS: the focal spot)      */
+  int32_t device_ptrs;     /* rays->x, y and status are GPU memory             */
+} rb_hist2d_desc;
+
+/* AGeoUtil::ContainmentRadius(TH2*, fraction, r, x, y)
+ * (src/AGeoUtil.cxx:198-308) of the terminal (x, y) of the n rays with
+ * status hist->status. The histogram is filled on the device as TH2::Fill
+ * fills it (entries outside either axis go to no bin and stay out of the
+ * statistics), and the radius search runs on the device.
+ * out[0..2] = r, x, y. If not NULL, counts (host, ny * nx, bin (bx, by) at
+ * (by - 1) * nx + bx - 1) and stats (host, {sum w, sum x, sum x^2, sum y,
+ * sum y^2}) receive the histogram. 0 < fraction <= 1. */
+int rbgpu_containment_radius(rb_ctx* ctx, const rb_rays* rays, int64_t n, const rb_hist2d_desc* hist,
+                             double fraction, double* out, uint32_t* counts, double* stats);
+
 /* Writes sizeof() of the ABI structs (rb_matrix, rb_shape_desc,
  * rb_volume_desc, rb_node_desc, rb_table_desc, rb_scene_desc, rb_rays,
- * rb_trace_opts, rb_raygen_desc, rb_adjoint_desc) into out[0..9] for binding
- * checks; returns 10.
+ * rb_trace_opts, rb_raygen_desc, rb_adjoint_desc, rb_hist2d_desc) into
+ * out[0..10] for binding checks; returns 11.
  * No counterpart in the reference (binding self-check only). */
 int rbgpu_abi_sizes(int32_t* out);
 

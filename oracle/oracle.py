@@ -48,6 +48,12 @@ def lib():
         L.oracle_trace_counts.argtypes = [C.POINTER(A.rb_scene_desc), C.POINTER(A.rb_trace_opts),
                                           C.POINTER(A.rb_rays), C.c_int64, C.c_void_p, C.c_int]
         L.oracle_trace_counts.restype = C.c_int
+        L.oracle_hist2d.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int, C.c_double,
+                                    C.c_double, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+        L.oracle_hist2d.restype = C.c_int
+        L.oracle_containment.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
+                                         C.c_double, C.c_double, C.c_void_p]
+        L.oracle_containment.restype = None
         _lib = L
     return _lib
 
@@ -155,3 +161,27 @@ def table_eval(flat, table, x):
 
 def lens_index(flat, volume, lam):
     return lib().oracle_lens_index(C.byref(flat.desc), volume, lam)
+
+
+def hist2d(x, y, status, want, nx, xlo, xhi, ny, ylo, yhi):
+    """TH2D(nx, xlo, xhi, ny, ylo, yhi) filled with the (x, y) of the rays with
+    status `want`: counts [ny, nx] (uint32) and stats {sum w, x, x^2, y, y^2}."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    status = np.ascontiguousarray(status, dtype=np.int32)
+    counts = np.zeros((ny, nx), dtype=np.uint32)
+    stats = np.zeros(5)
+    rc = lib().oracle_hist2d(x.ctypes.data, y.ctypes.data, status.ctypes.data, len(x), want, nx, xlo, xhi, ny, ylo,
+                             yhi, counts.ctypes.data, stats.ctypes.data)
+    assert rc == 0, rc
+    return counts, stats
+
+
+def containment(counts, stats, nx, xlo, xhi, ny, ylo, yhi, fraction):
+    """AGeoUtil::ContainmentRadius over a filled histogram: (r, x, y)."""
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    stats = np.ascontiguousarray(stats, dtype=np.float64)
+    out = np.zeros(3)
+    lib().oracle_containment(counts.ctypes.data, stats.ctypes.data, nx, xlo, xhi, ny, ylo, yhi, fraction,
+                             out.ctypes.data)
+    return float(out[0]), float(out[1]), float(out[2])

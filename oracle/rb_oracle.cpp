@@ -2041,4 +2041,167 @@ double oracle_uniform(uint64_t seed, uint64_t id, uint32_t draw, uint32_t stream
   return r.uniform();
 }
 
+// ---------------------------------------------------------------------------
+// Spot analysis after the trace: ROOT's TH2D fill of the focal-plane hits and
+// AGeoUtil::ContainmentRadius (src/AGeoUtil.cxx:198-308) over it.
+//
+// TH2D(nx, xlo, xhi, ny, ylo, yhi) semantics restated from ROOT's documented
+// behaviour (ROOT is not in the reference tree; version unpinned):
+//   TAxis::FindBin   x < lo -> 0; !(x < hi) -> n + 1; else 1 + int(n (x - lo) / (hi - lo))
+//   TAxis::GetBinCenter(b) = lo + (b - 1) w + 0.5 w,  w = (hi - lo) / n
+//   TH2::Fill        an entry outside either axis goes to an under/overflow
+//                    bin and does not enter the statistics (sum w, x, x^2, y, y^2)
+//   TH1::GetMean     sum x / sum w;  GetStdDev = sqrt(max(sum x^2 / sum w - mean^2, 0))
+//   TH1::Integral    sum of the in-range bins
+// counts[(by - 1) nx + (bx - 1)] holds bin (bx, by); stats = {sum w, sum x,
+// sum x^2, sum y, sum y^2}, summed in ray order.
+static int find_bin(double x, int n, double lo, double hi) {
+  if (x < lo) return 0;
+  if (!(x < hi)) return n + 1;
+  return 1 + int(n * (x - lo) / (hi - lo));
+}
+static double bin_center(int b, int n, double lo, double hi) {
+  const double w = (hi - lo) / double(n);
+  return lo + (b - 1) * w + 0.5 * w;
+}
+
+int oracle_hist2d(const double* x, const double* y, const int32_t* status, int64_t n, int32_t want, int nx, double xlo,
+                  double xhi, int ny, double ylo, double yhi, uint32_t* counts, double* stats) {
+  if (nx <= 0 || ny <= 0 || !(xhi > xlo) || !(yhi > ylo)) return -1;
+  std::fill(counts, counts + int64_t(nx) * ny, 0u);
+  for (int k = 0; k < 5; k++) stats[k] = 0.0;
+  for (int64_t i = 0; i < n; i++) {
+    if (status[i] != want) continue;
+    const int bx = find_bin(x[i], nx, xlo, xhi), by = find_bin(y[i], ny, ylo, yhi);
+    if (bx < 1 || bx > nx || by < 1 || by > ny) continue;
+    counts[int64_t(by - 1) * nx + (bx - 1)]++;
+    stats[0] += 1.0;
+    stats[1] += x[i];
+    stats[2] += x[i] * x[i];
+    stats[3] += y[i];
+    stats[4] += y[i] * y[i];
+  }
+  return 0;
+}
+
+// SumInRadius (src/AGeoUtil.cxx:18-42): contents of the bins whose centres lie
+// within r of (x, y).
+static double sum_in_radius(const uint32_t* counts, int nx, double xlo, double xhi, int ny, double ylo, double yhi,
+                            double x, double y, double r) {
+  const double r2 = r * r;
+  double total = 0.;
+  for (int ix = 1; ix <= nx; ++ix) {
+    const double cx = bin_center(ix, nx, xlo, xhi);
+    for (int iy = 1; iy <= ny; ++iy) {
+      const double cy = bin_center(iy, ny, ylo, yhi);
+      const double c = counts[int64_t(iy - 1) * nx + (ix - 1)];
+      if (c <= 0) continue;
+      const double d2 = (cx - x) * (cx - x) + (cy - y) * (cy - y);
+      if (d2 <= r2) total += c;
+    }
+  }
+  return total;
+}
+
+// ContainmentRadius (src/AGeoUtil.cxx:198-308): radius and centre of the
+// circle holding `fraction` of the histogram. out = {r, x, y}. Each
+// hill-climbing run of the centre is capped at kMaxClimb steps, as on the
+// device (the reference's loops are uncapped; the cap is never reached on
+// real spots, since every step raises the contained count).
+static const int kMaxClimb = 1 << 16;
+void oracle_containment(const uint32_t* counts, const double* stats, int nx, double xlo, double xhi, int ny,
+                        double ylo, double yhi, double fraction, double* out) {
+  auto sir = [&](double x, double y, double r) { return sum_in_radius(counts, nx, xlo, xhi, ny, ylo, yhi, x, y, r); };
+  int no_shift = 0, no_stable = 0;
+  const double sw = stats[0];
+  double x = sw == 0 ? 0. : stats[1] / sw;
+  double y = sw == 0 ? 0. : stats[3] / sw;
+  const double vx = sw == 0 ? 0. : std::max(stats[2] / sw - x * x, 0.0);
+  const double vy = sw == 0 ? 0. : std::max(stats[4] / sw - y * y, 0.0);
+  double r = std::sqrt(std::sqrt(vx) * std::sqrt(vx) + std::sqrt(vy) * std::sqrt(vy)) * 1.5;
+  double dr = 0.1 * r;
+  double integral = 0.;
+  for (int64_t b = 0; b < int64_t(nx) * ny; b++) integral += counts[b];
+  const double sum_goal = integral * fraction;
+  for (int i = 0; i < 100 && no_shift < 30; i++) {
+    bool stable_r = false, stable_x = true, stable_y = true;
+    const double sum0 = sir(x, y, r);
+    double next_r = r;
+    if (sum0 < sum_goal) {
+      const double sum1 = sir(x, y, r + dr);
+      if (sum1 == sum0) {
+        dr *= 2.;
+        continue;
+      }
+      next_r = r + dr * (sum_goal - sum0) / (sum1 - sum0);
+    } else if (sum0 != sum_goal) {
+      const double sum1 = sir(x, y, r - dr);
+      if (sum1 == sum0) {
+        dr *= 2.;
+        continue;
+      }
+      next_r = r - dr * (sum0 - sum_goal) / (sum0 - sum1);
+    }
+    if (next_r < 0.) next_r = 0.5 * r;
+    if (next_r < 0.5 * r) next_r = 0.5 * r;
+    if (next_r > 2. * r) next_r = 2. * r;
+    stable_r = std::fabs(next_r - r) < 0.0001 * r;
+    r = next_r;
+    {
+      double sum1 = sir(x, y, r);
+      dr *= sum0 != sum_goal ? std::fabs((sum1 - sum_goal) / (sum0 - sum_goal)) : 0.5;
+      if (dr > 0.5 * r) dr = 0.5 * r;
+      if (dr < 0.0005 * r) dr = 0.0005 * r;
+      no_shift++;
+      for (double dx = 0.25 * r; dx > 0.1 * dr; dx *= 0.25) {
+        double sum_x1 = sir(x + dx, y, r);
+        double sum_x2 = sir(x - dx, y, r);
+        for (int climb = 0; climb < kMaxClimb && sum_x1 > sum1; climb++) {
+          no_shift = 0;
+          x += dx;
+          sum_x2 = sum1;
+          sum1 = sum_x1;
+          sum_x1 = sir(x + dx, y, r);
+          stable_x = false;
+        }
+        for (int climb = 0; climb < kMaxClimb && sum_x2 > sum1; climb++) {
+          no_shift = 0;
+          x -= dx;
+          sum_x1 = sum1;
+          sum1 = sum_x2;
+          sum_x2 = sir(x - dx, y, r);
+          stable_x = false;
+        }
+      }
+    }
+    for (double dy = 0.1 * r; dy > 0.1 * dr; dy *= 0.25) {
+      double sum1 = sir(x, y, r);
+      double sum_y1 = sir(x, y + dy, r);
+      double sum_y2 = sir(x, y - dy, r);
+      for (int climb = 0; climb < kMaxClimb && sum_y1 > sum1; climb++) {
+        no_shift = 0;
+        y += dy;
+        sum_y2 = sum1;
+        sum1 = sum_y1;
+        sum_y1 = sir(x, y + dy, r);
+        stable_y = false;
+      }
+      for (int climb = 0; climb < kMaxClimb && sum_y2 > sum1; climb++) {
+        no_shift = 0;
+        y -= dy;
+        sum_y1 = sum1;
+        sum1 = sum_y2;
+        sum_y2 = sir(x, y - dy, r);
+        stable_y = false;
+      }
+    }
+    if (stable_r && stable_x && stable_y) no_stable++;
+    else no_stable = 0;
+    if (no_stable >= 4) break;
+  }
+  out[0] = r;
+  out[1] = x;
+  out[2] = y;
+}
+
 }  // extern "C"

@@ -106,6 +106,13 @@ class rb_adjoint_desc(C.Structure):
     ]
 
 
+class rb_hist2d_desc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("xlo", C.c_double), ("xhi", C.c_double),
+        ("ylo", C.c_double), ("yhi", C.c_double), ("status", C.c_int32), ("device_ptrs", C.c_int32),
+    ]
+
+
 def ptr(a):
     """Address of a contiguous numpy array (or None)."""
     if a is None:

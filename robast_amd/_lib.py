@@ -19,6 +19,7 @@ EXPORTS = (
     "rbgpu_ctx_create", "rbgpu_ctx_destroy", "rbgpu_last_error", "rbgpu_scene_upload",
     "rbgpu_scene_free", "rbgpu_trace", "rbgpu_generate_rays", "rbgpu_last_trace_ms",
     "rbgpu_last_trace_launches", "rbgpu_abi_sizes", "rbgpu_adjoint", "rbgpu_last_kernel_ms",
+    "rbgpu_containment_radius",
 )
 
 
@@ -60,6 +61,10 @@ def load():
     lib.rbgpu_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays), C.c_int64,
                                   C.POINTER(A.rb_adjoint_desc), C.c_void_p]
     lib.rbgpu_adjoint.restype = C.c_int
+    lib.rbgpu_containment_radius.argtypes = [C.c_void_p, C.POINTER(A.rb_rays), C.c_int64,
+                                             C.POINTER(A.rb_hist2d_desc), C.c_double, C.c_void_p, C.c_void_p,
+                                             C.c_void_p]
+    lib.rbgpu_containment_radius.restype = C.c_int
     lib.rbgpu_abi_sizes.argtypes = [C.POINTER(C.c_int32)]
     lib.rbgpu_abi_sizes.restype = C.c_int
     _lib = lib
@@ -102,6 +107,18 @@ class Context:
     def adjoint(self, scene, rays, n, opts, adj, grad_ptr):
         self._check(self.lib.rbgpu_adjoint(self.h, scene.h, C.byref(opts), C.byref(rays), C.c_int64(n), C.byref(adj),
                                            C.c_void_p(grad_ptr)), "rbgpu_adjoint")
+
+    def containment_radius(self, rays, n, hist, fraction, with_histogram=False):
+        """rbgpu_containment_radius: (r, x, y), plus (counts [ny, nx] uint32,
+        stats {sum w, x, x^2, y, y^2}) with_histogram."""
+        out = np.zeros(3)
+        counts = np.zeros((hist.ny, hist.nx), dtype=np.uint32) if with_histogram else None
+        stats = np.zeros(5) if with_histogram else None
+        self._check(self.lib.rbgpu_containment_radius(self.h, C.byref(rays), C.c_int64(n), C.byref(hist),
+                                                      C.c_double(fraction), A.ptr(out), A.ptr(counts), A.ptr(stats)),
+                    "rbgpu_containment_radius")
+        r = (float(out[0]), float(out[1]), float(out[2]))
+        return (r, counts, stats) if with_histogram else r
 
     def last_trace_ms(self):
         return self.lib.rbgpu_last_trace_ms(self.h)

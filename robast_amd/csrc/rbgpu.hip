@@ -2,6 +2,7 @@
 // BVH build, ray sources and the trace kernel launch for gfx950.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -14,6 +15,7 @@
 #include "rb_scene_build.h"
 #include "rb_trace.h"
 #include "rb_adjoint.h"
+#include "rb_analysis.h"
 
 using namespace rbd;
 
@@ -401,6 +403,88 @@ __global__ void __launch_bounds__(256) k_generate(rb_raygen_desc g, RayIO io, in
   if (io.last_node) io.last_node[i] = -1;
 }
 
+// ---------------------------------------------------------------- spot analysis
+constexpr int kHistBlock = 256;
+constexpr int kSearchBlock = 1024;
+
+// TH2::Fill of the terminal (x, y) of the rays with status H.status: one
+// 32-bit atomic per in-range entry. Each block also sums its entries'
+// statistics, in a fixed tree order, into partial[block][0..4].
+__global__ void __launch_bounds__(kHistBlock) k_hist2d(const double* x, const double* y, const int32_t* status,
+                                                       int64_t n, HistParams H, uint32_t* counts, double* partial) {
+  double s[5] = {0., 0., 0., 0., 0.};
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (status[i] != H.status) continue;
+    const double xi = x[i], yi = y[i];
+    const int bx = hist_bin(xi, H.nx, H.xlo, H.xhi), by = hist_bin(yi, H.ny, H.ylo, H.yhi);
+    if (bx < 1 || bx > H.nx || by < 1 || by > H.ny) continue;
+    atomicAdd(&counts[int64_t(by - 1) * H.nx + (bx - 1)], 1u);
+    s[0] += 1.0;
+    s[1] += xi;
+    s[2] += xi * xi;
+    s[3] += yi;
+    s[4] += yi * yi;
+  }
+  __shared__ double red[5][kHistBlock];
+  for (int k = 0; k < 5; k++) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = kHistBlock / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w)
+      for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 5; k++) partial[int64_t(blockIdx.x) * 5 + k] = red[k][0];
+}
+
+// Compaction of the non-empty bins (after an exclusive scan of the flags):
+// entry pos[b] holds bin b; row_start[by - 1] is the first entry of row by.
+__global__ void __launch_bounds__(kHistBlock) k_hist_flags(const uint32_t* counts, int nb, int32_t* flag) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) flag[b] = counts[b] != 0u;
+}
+__global__ void __launch_bounds__(kHistBlock) k_hist_compact(const uint32_t* counts, const int32_t* pos, int nb,
+                                                             int nx, int ny, int32_t* bin, uint32_t* count,
+                                                             int32_t* row_start) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+    const uint32_t c = counts[b];
+    if (c != 0u) {
+      bin[pos[b]] = b;
+      count[pos[b]] = c;
+    }
+    if (b % nx == 0) row_start[b / nx] = pos[b];
+    if (b == nb - 1) row_start[ny] = pos[b] + (c != 0u ? 1 : 0);
+  }
+}
+
+// One block: the statistics (block partials added in block order), the
+// integral, then the containment search. out = {r, x, y, sum w, sum x,
+// sum x^2, sum y, sum y^2}.
+__global__ void __launch_bounds__(kSearchBlock) k_containment(SparseHist Hs, const double* partial, int nparts,
+                                                              HistParams H, double fraction, double* out) {
+  __shared__ uint64_t red[kSearchBlock];
+  __shared__ double st_sh[5];
+  if (threadIdx.x == 0) {
+    double t[5] = {0., 0., 0., 0., 0.};
+    for (int b = 0; b < nparts; b++)
+      for (int k = 0; k < 5; k++) t[k] += partial[int64_t(b) * 5 + k];
+    for (int k = 0; k < 5; k++) st_sh[k] = t[k];
+  }
+  __syncthreads();
+  double st[5];
+  for (int k = 0; k < 5; k++) st[k] = st_sh[k];
+  const int ne = Hs.row_start[H.ny];
+  uint64_t acc = 0;
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) acc += Hs.count[e];
+  const double integral = double(block_sum_u64(acc, red));
+  double res[3];
+  containment_search(Hs, H, st, integral, fraction, red, res);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 3; k++) out[k] = res[k];
+    for (int k = 0; k < 5; k++) out[3 + k] = st[k];
+  }
+}
+
 // ---------------------------------------------------------------- C ABI
 // True if the GPU can dereference p: device or managed memory, or host
 // memory pinned and mapped by HIP. Pageable host memory would fault in a
@@ -521,7 +605,8 @@ int rbgpu_abi_sizes(int32_t* out) {
   out[7] = sizeof(rb_trace_opts);
   out[8] = sizeof(rb_raygen_desc);
   out[9] = sizeof(rb_adjoint_desc);
-  return 10;
+  out[10] = sizeof(rb_hist2d_desc);
+  return 11;
 }
 
 int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
@@ -815,6 +900,101 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
 }
 
 double rbgpu_last_kernel_ms(const rb_ctx* c) { return c ? c->kernel_ms : 0.0; }
+
+int rbgpu_containment_radius(rb_ctx* ctx, const rb_rays* rays, int64_t n, const rb_hist2d_desc* h,
+                             double fraction, double* out, uint32_t* counts, double* stats) {
+  if (!ctx) return -1;
+  if (!rays || !h || !out || n < 0) return fail(ctx, -1, "rbgpu_containment_radius: null argument or negative n");
+  if (h->nx < 1 || h->ny < 1 || int64_t(h->nx) * h->ny > (int64_t(1) << 26))
+    return fail(ctx, -1, "rbgpu_containment_radius: bin counts must be >= 1 with nx * ny <= 2^26");
+  if (!(h->xhi > h->xlo) || !(h->yhi > h->ylo) || !std::isfinite(h->xhi - h->xlo) || !std::isfinite(h->yhi - h->ylo))
+    return fail(ctx, -1, "rbgpu_containment_radius: axis range must be finite with hi > lo");
+  if (!(fraction > 0.0 && fraction <= 1.0)) return fail(ctx, -1, "rbgpu_containment_radius: fraction must lie in (0, 1]");
+  if (n > 0 && (!rays->x || !rays->y || !rays->status))
+    return fail(ctx, -1, "rbgpu_containment_radius: missing x, y or status array");
+  ctx->launches = 0;
+  ctx->last_ms = 0;
+  ctx->kernel_ms = 0;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  DevRays dev;
+  const double* x = rays->x;
+  const double* y = rays->y;
+  const int32_t* status = rays->status;
+  if (n > 0 && !h->device_ptrs) {
+    double *dx, *dy;
+    int32_t* ds;
+    HIPCHK(ctx, dev.alloc(&dx, n));
+    HIPCHK(ctx, dev.alloc(&dy, n));
+    HIPCHK(ctx, dev.alloc(&ds, n));
+    HIPCHK(ctx, hipMemcpyAsync(dx, x, size_t(n) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(dy, y, size_t(n) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ds, status, size_t(n) * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    x = dx;
+    y = dy;
+    status = ds;
+  } else if (n > 0 && (!gpu_accessible(x) || !gpu_accessible(y) || !gpu_accessible(status))) {
+    return fail(ctx, -1, "rbgpu_containment_radius: device_ptrs set but a ray array is not GPU memory");
+  }
+  const int nb = h->nx * h->ny;
+  const int grid = int(std::max<int64_t>(1, std::min<int64_t>((n + kHistBlock - 1) / kHistBlock, int64_t(ctx->cus) * 4)));
+  uint32_t* dcounts;
+  double *partial, *dout;
+  HIPCHK(ctx, dev.alloc(&dcounts, nb));
+  HIPCHK(ctx, dev.alloc(&partial, int64_t(grid) * 5));
+  HIPCHK(ctx, dev.alloc(&dout, 8));
+  HistParams H{h->nx, h->ny, h->xlo, h->xhi, h->ylo, h->yhi, h->status};
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(dcounts, 0, size_t(nb) * sizeof(uint32_t), ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(partial, 0, size_t(grid) * 5 * sizeof(double), ctx->stream));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_hist2d, dim3(unsigned(grid)), dim3(kHistBlock), 0, ctx->stream, x, y, status, n, H, dcounts,
+                       partial);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  // non-empty bins, row by row
+  int32_t *flag, *pos, *sbin, *row_start;
+  uint32_t* scount;
+  HIPCHK(ctx, dev.alloc(&flag, nb));
+  HIPCHK(ctx, dev.alloc(&pos, nb));
+  HIPCHK(ctx, dev.alloc(&sbin, nb));
+  HIPCHK(ctx, dev.alloc(&scount, nb));
+  HIPCHK(ctx, dev.alloc(&row_start, int64_t(h->ny) + 1));
+  size_t scan_bytes = 0;
+  HIPCHK(ctx, rocprim::exclusive_scan(nullptr, scan_bytes, flag, pos, int32_t(0), size_t(nb), rocprim::plus<int32_t>(),
+                                      ctx->stream));
+  char* scan_tmp;
+  HIPCHK(ctx, dev.alloc(&scan_tmp, int64_t(scan_bytes)));
+  const int cgrid = std::max(1, std::min((nb + kHistBlock - 1) / kHistBlock, ctx->cus * 8));
+  hipLaunchKernelGGL(k_hist_flags, dim3(unsigned(cgrid)), dim3(kHistBlock), 0, ctx->stream, dcounts, nb, flag);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, rocprim::exclusive_scan(scan_tmp, scan_bytes, flag, pos, int32_t(0), size_t(nb), rocprim::plus<int32_t>(),
+                                      ctx->stream));
+  hipLaunchKernelGGL(k_hist_compact, dim3(unsigned(cgrid)), dim3(kHistBlock), 0, ctx->stream, dcounts, pos, nb, h->nx,
+                     h->ny, sbin, scount, row_start);
+  HIPCHK(ctx, hipGetLastError());
+  SparseHist Hs{sbin, scount, row_start};
+  HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+  hipLaunchKernelGGL(k_containment, dim3(1), dim3(kSearchBlock), 0, ctx->stream, Hs, partial, grid, H, fraction,
+                     dout);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  double res[8];
+  HIPCHK(ctx, hipMemcpyAsync(res, dout, sizeof(res), hipMemcpyDeviceToHost, ctx->stream));
+  if (counts)
+    HIPCHK(ctx, hipMemcpyAsync(counts, dcounts, size_t(nb) * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0, kms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  HIPCHK(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
+  ctx->last_ms = ms;
+  ctx->kernel_ms = kms;
+  ctx->launches = n > 0 ? 5 : 4;  // fill, flags, scan, compaction, search
+  for (int k = 0; k < 3; k++) out[k] = res[k];
+  if (stats)
+    for (int k = 0; k < 5; k++) stats[k] = res[3 + k];
+  return 0;
+}
 
 double rbgpu_last_trace_ms(const rb_ctx* c) { return c ? c->last_ms : 0.0; }
 int rbgpu_last_trace_launches(const rb_ctx* c) { return c ? c->launches : 0; }

@@ -30,8 +30,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_binding():
     lib = _lib.load()
     out = (C.c_int32 * 16)()
-    assert lib.rbgpu_abi_sizes(out) == 10
-    sizes = list(out)[:10]
+    assert lib.rbgpu_abi_sizes(out) == 11
+    sizes = list(out)[:11]
     assert sizes[0] == 96                              # rb_matrix
     assert sizes[1] == A.SHAPE_DTYPE.itemsize
     assert sizes[2] == A.VOLUME_DTYPE.itemsize
@@ -42,6 +42,7 @@ def test_struct_sizes_match_binding():
     assert sizes[7] == C.sizeof(A.rb_trace_opts)
     assert sizes[8] == C.sizeof(A.rb_raygen_desc)
     assert sizes[9] == C.sizeof(A.rb_adjoint_desc)
+    assert sizes[10] == C.sizeof(A.rb_hist2d_desc) == 48
     assert C.sizeof(A.rb_trace_opts) == 40
 
 
@@ -50,3 +51,4 @@ def test_null_context_is_an_error_not_a_crash():
     assert lib.rbgpu_trace(None, None, None, None, 0) == -1
     assert lib.rbgpu_scene_upload(None, None, None) == -1
     assert lib.rbgpu_last_error(None) == b"null context"
+    assert lib.rbgpu_containment_radius(None, None, 0, None, 0.8, None, None, None) == -1
