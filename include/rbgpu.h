@@ -171,7 +171,7 @@ typedef struct {
   uint64_t seed;       /* Philox4x32-10 key */
   uint64_t ray_id_base;/* global id of ray 0 (Philox counter), for sharded runs */
   int32_t order;       /* RB_ORDER_*: the order rays are traced in (results do not depend on it) */
-  int32_t pad;
+  int32_t schedule;    /* RB_SCHEDULE_*: how rays are handed to lanes (results do not depend on it) */
 } rb_trace_opts;
 
 /* Trace order. Every ray keeps its own Philox stream (keyed by its index),
@@ -181,6 +181,13 @@ typedef struct {
  * top volume, and traces them in that order through a permutation. AUTO
  * sorts batches of 2^20 rays or more. */
 enum { RB_ORDER_AUTO = 0, RB_ORDER_INPUT = 1, RB_ORDER_COHERENT = 2 };
+
+/* Lane schedule. LANE: one lane per ray for the whole trace. PERSISTENT: a
+ * grid sized to the machine whose lanes take the next ray (in trace order)
+ * from a work counter as soon as theirs terminates, so a wavefront does not
+ * wait for its longest path. AUTO picks PERSISTENT for scenes with refracting
+ * volumes (multi-bounce paths of very unequal length) and LANE otherwise. */
+enum { RB_SCHEDULE_AUTO = 0, RB_SCHEDULE_LANE = 1, RB_SCHEDULE_PERSISTENT = 2 };
 
 /* ---- synthetic ray sources (device-side ARayShooter equivalents) ---- */
 enum {

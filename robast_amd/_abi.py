@@ -9,6 +9,7 @@ import numpy as np
 
 # ray status (include/ARay.h:26)
 RB_RUN, RB_STOP, RB_EXIT, RB_FOCUS, RB_SUSPEND, RB_ABSORB = range(6)
+RB_SCHEDULE_AUTO, RB_SCHEDULE_LANE, RB_SCHEDULE_PERSISTENT = range(3)
 STATUS_NAMES = ("running", "stopped", "exited", "focused", "suspended", "absorbed")
 
 # volume kinds (include/AOpticsManager.h:45-53)
@@ -84,7 +85,7 @@ class rb_trace_opts(C.Structure):
     _fields_ = [
         ("limit", C.c_int32), ("fresnel", C.c_int32), ("precision", C.c_int32),
         ("device_ptrs", C.c_int32), ("seed", C.c_uint64), ("ray_id_base", C.c_uint64),
-        ("order", C.c_int32), ("pad", C.c_int32),
+        ("order", C.c_int32), ("schedule", C.c_int32),
     ]
 
 

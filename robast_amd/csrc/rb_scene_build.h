@@ -237,6 +237,7 @@ struct SceneLayout {
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
   int32_t top_bvh_lo = 0, top_bvh_n = 0;  // the top node's BVH entries
   int32_t max_depth = 0;                  // deepest boolean nesting of any shape
+  bool has_lens = false;                  // a placed node is an ALens
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -258,6 +259,7 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
     o.n_children = n.n_children;
     o.bvh_root = -1;
     o.pad = 0;
+    if (o.kind == RB_KLENS) lay.has_lens = true;
     if (n.n_children > 0) {
       std::vector<std::pair<int, Box>> items;
       for (int k = 0; k < n.n_children; k++) {

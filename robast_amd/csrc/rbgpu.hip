@@ -41,6 +41,7 @@ struct rb_scene {
   void* dmem = nullptr;
   DScene s{};
   int max_depth = 0;  // deepest boolean nesting among the scene's shapes
+  bool has_lens = false;  // a placed volume refracts (ALens)
   bool has_top_box = false;
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
 };
@@ -90,12 +91,6 @@ struct DevRays {
 // Block size of the tracing kernels: the LDS candidate slots are laid out
 // for exactly this many lanes.
 constexpr int kTraceBlock = kCandStride;
-// Trace schedule: persistent grid with per-lane ray fetch (1) or one lane
-// per ray over the whole batch (0).
-#ifndef RB_PERSISTENT
-#define RB_PERSISTENT 0
-#endif
-constexpr bool kPersistent = RB_PERSISTENT != 0;
 // Occupancy request for the tracing kernels (waves per SIMD); 0 leaves the
 // register budget to the compiler. Two waves (256 registers per lane, the
 // rest spilled) measured faster than one wave with 512 on MI355X: the
@@ -229,14 +224,15 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
 // ray from a global counter as soon as its current one terminates, so a
 // wave's lanes stay busy instead of waiting for the longest path among 64
 // fixed rays. The counter is claimed once per wave (one vector atomic for
-// all lanes that need a ray). A ray is traced from start to end by the lane
-// that claimed it, with the same per-ray Philox stream, so results equal
-// k_trace's. Every wave leaves when the counter has passed n and none of
-// its lanes holds a ray.
-template <bool MIXED>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DScene S, RayIO io, int64_t n,
+// all lanes that need a ray) and walks the trace order (`perm`, or the
+// input order). A ray is traced from start to end by the lane that claimed
+// it, with the same per-ray Philox stream, so results equal k_trace's. Every
+// wave leaves when the counter has passed n and none of its lanes holds a ray.
+template <bool MIXED, int MAXD>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSceneOf<MAXD> S, RayIO io, int64_t n,
                                                                           TraceParams P,
-                                                                          unsigned long long* next_ray) {
+                                                                          unsigned long long* next_ray,
+                                                                          const uint32_t* perm) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   stage_bvh(S);
@@ -255,10 +251,10 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
       if (need && rank == 0) base = atomicAdd(next_ray, (unsigned long long)__popcll(want));
       const unsigned lo = __shfl(unsigned(base), leader), hi = __shfl(unsigned(base >> 32), leader);
       if (need) {
-        i = int64_t((static_cast<unsigned long long>(hi) << 32) | lo) + rank;
-        if (i >= n) {
+        const int64_t c = int64_t((static_cast<unsigned long long>(hi) << 32) | lo) + rank;
+        if (c >= n) {
           exhausted = true;
-        } else if (io.status[i] == RB_RUN) {
+        } else if (i = perm ? int64_t(perm[c]) : c; io.status[i] == RB_RUN) {
           load_lane(io, i, L);
           L.ray = P.id_base + uint64_t(i);
           begin_lane(S, L);
@@ -485,6 +481,29 @@ __global__ void __launch_bounds__(kSearchBlock) k_containment(SparseHist Hs, con
   }
 }
 
+// ---------------------------------------------------------------- launches
+// The trace kernel for the scene's depth and the lane schedule.
+template <bool MIXED, int MAXD>
+static int launch_trace(rb_ctx* ctx, const rb_scene* scene, bool persistent, const RayIO& io, int64_t n,
+                        const TraceParams& P, const uint32_t* perm) {
+  const int block = kTraceBlock;
+  const int64_t grid = (n + block - 1) / block;
+  if (persistent) {
+    int per_cu = 0;
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace_persist<MIXED, MAXD>,
+                                                             block, 0));
+    const int64_t pgrid = std::min<int64_t>(grid, int64_t(std::max(per_cu, 1)) * ctx->cus);
+    HIPCHK(ctx, hipMemsetAsync(ctx->next_ray, 0, sizeof(unsigned long long), ctx->stream));
+    hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
+                       scene_of<MAXD>(scene), io, n, P, ctx->next_ray, perm);
+  } else {
+    hipLaunchKernelGGL((k_trace<MIXED, MAXD>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                       scene_of<MAXD>(scene), io, n, P, perm);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return 0;
+}
+
 // ---------------------------------------------------------------- C ABI
 // True if the GPU can dereference p: device or managed memory, or host
 // memory pinned and mapped by HIP. Pageable host memory would fault in a
@@ -635,6 +654,7 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
   sc->s.stage_lo = lay.top_bvh_lo;
   sc->s.stage_n = std::min(lay.top_bvh_n, int32_t(kBvhStage));
   sc->max_depth = lay.max_depth;
+  sc->has_lens = lay.has_lens;
   sc->has_top_box = lay.has_top_box;
   for (int k = 0; k < 3; k++) {
     sc->top_lo[k] = lay.top_lo[k];
@@ -660,6 +680,9 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
     return fail(ctx, -1, "rbgpu_trace: unknown precision mode");
   if (opts->order != RB_ORDER_AUTO && opts->order != RB_ORDER_INPUT && opts->order != RB_ORDER_COHERENT)
     return fail(ctx, -1, "rbgpu_trace: unknown order");
+  if (opts->schedule != RB_SCHEDULE_AUTO && opts->schedule != RB_SCHEDULE_LANE &&
+      opts->schedule != RB_SCHEDULE_PERSISTENT)
+    return fail(ctx, -1, "rbgpu_trace: unknown schedule");
   ctx->launches = 0;
   ctx->last_ms = 0;
   ctx->kernel_ms = 0;
@@ -716,47 +739,25 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
     io.path_node = rays->path_node;
   }
 
-  const int block = kTraceBlock;
-  const int64_t grid = (n + block - 1) / block;
   const bool mixed = opts->precision == RB_PRECISION_MIXED;
-  const bool coherent = !kPersistent && scene->has_top_box && n < (int64_t(1) << 31) &&
+  const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
                         (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
+  const bool persistent = opts->schedule == RB_SCHEDULE_PERSISTENT ||
+                          (opts->schedule == RB_SCHEDULE_AUTO && scene->has_lens);
   const uint32_t* perm = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (coherent) {
     const int rc = order_rays(ctx, scene, io, n, &perm);
     if (rc) return rc;
   }
-  if (kPersistent && !ctx->next_ray) HIPCHK(ctx, hipMalloc(&ctx->next_ray, sizeof(unsigned long long)));
+  if (persistent && !ctx->next_ray) HIPCHK(ctx, hipMalloc(&ctx->next_ray, sizeof(unsigned long long)));
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-  if (kPersistent) {
-    int per_cu = 0;
-    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, mixed ? (const void*)k_trace_persist<true> : (const void*)k_trace_persist<false>,
-                    block, 0));
-    const int64_t pgrid = std::min<int64_t>(grid, int64_t(std::max(per_cu, 1)) * ctx->cus);
-    HIPCHK(ctx, hipMemsetAsync(ctx->next_ray, 0, sizeof(unsigned long long), ctx->stream));
-    if (mixed)
-      hipLaunchKernelGGL(k_trace_persist<true>, dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream, scene->s, io, n,
-                         P, ctx->next_ray);
-    else
-      hipLaunchKernelGGL(k_trace_persist<false>, dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream, scene->s, io,
-                         n, P, ctx->next_ray);
-  } else if (mixed) {
-    if (scene->max_depth <= 1)
-      hipLaunchKernelGGL((k_trace<true, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene), io,
-                         n, P, perm);
-    else
-      hipLaunchKernelGGL((k_trace<true, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
-                         scene_of<kMaxCsgDepth>(scene), io, n, P, perm);
-  } else {
-    if (scene->max_depth <= 1)
-      hipLaunchKernelGGL((k_trace<false, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene),
-                         io, n, P, perm);
-    else
-      hipLaunchKernelGGL((k_trace<false, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
-                         scene_of<kMaxCsgDepth>(scene), io, n, P, perm);
-  }
+  const bool shallow = scene->max_depth <= 1;
+  const int rc = mixed ? (shallow ? launch_trace<true, 1>(ctx, scene, persistent, io, n, P, perm)
+                                  : launch_trace<true, kMaxCsgDepth>(ctx, scene, persistent, io, n, P, perm))
+                       : (shallow ? launch_trace<false, 1>(ctx, scene, persistent, io, n, P, perm)
+                                  : launch_trace<false, kMaxCsgDepth>(ctx, scene, persistent, io, n, P, perm));
+  if (rc) return rc;
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));

@@ -79,9 +79,10 @@ class AOpticsManager:
             self.CloseGeometry()
         return self._flat
 
-    def trace_opts(self, device_ptrs=0, ray_id_base=0, order=A.RB_ORDER_AUTO):
+    def trace_opts(self, device_ptrs=0, ray_id_base=0, order=A.RB_ORDER_AUTO, schedule=A.RB_SCHEDULE_AUTO):
         o = A.rb_trace_opts()
         o.order = order
+        o.schedule = schedule
         o.limit = self.limit
         o.fresnel = 0 if self.disable_fresnel else 1
         o.precision = self.precision
