@@ -96,7 +96,19 @@ typedef struct {
   int32_t k_table;         /* lens: TGraph k(lambda), -1 = none (k = 0)          */
   int32_t qe_lambda_table; /* focal surface: TGraph QE(lambda), -1 = 1           */
   int32_t qe_angle_table;  /* focal surface: TGraph QE(angle [rad]), -1 = none   */
+  int32_t border_first;    /* this volume's border-surface conditions:           */
+  int32_t border_count;    /*   borders[border_first .. + border_count)          */
 } rb_volume_desc;
+
+/* ABorderSurfaceCondition(component1, component2) (include/ABorderSurfaceCondition.h:24-45),
+ * listed under component1's volume in registration order; the first whose
+ * component2 matches the volume entered applies
+ * (AOpticalComponent::FindBorderSurfaceCondition, src/AOpticalComponent.cxx:51-65). */
+typedef struct {
+  int32_t volume2;    /* component2's volume; -1 = the null node (leaving the top volume) */
+  int32_t lambertian; /* EnableLambertian(kTRUE): diffuse reflection                 */
+  double sigma;       /* SetGaussianRoughness(sigma) [rad]; 0 = smooth                 */
+} rb_border_desc;
 
 /* One placed instance of a volume (a path in the TGeo node tree), with its
  * global matrix. Node 0 is the top volume. */
@@ -132,6 +144,8 @@ typedef struct {
   const rb_table_desc* tables;
   int32_t n_table_xy; /* number of doubles (2 per point) */
   const double* table_xy;
+  int32_t n_borders;
+  const rb_border_desc* borders;
 } rb_scene_desc;
 
 /* Rays in structure-of-arrays form. On input each ray holds its last point
@@ -329,8 +343,8 @@ int rbgpu_containment_radius(rb_ctx* ctx, const rb_rays* rays, int64_t n, const 
 
 /* Writes sizeof() of the ABI structs (rb_matrix, rb_shape_desc,
  * rb_volume_desc, rb_node_desc, rb_table_desc, rb_scene_desc, rb_rays,
- * rb_trace_opts, rb_raygen_desc, rb_adjoint_desc, rb_hist2d_desc) into
- * out[0..10] for binding checks; returns 11.
+ * rb_trace_opts, rb_raygen_desc, rb_adjoint_desc, rb_hist2d_desc,
+ * rb_border_desc) into out[0..11] for binding checks; returns 12.
  * No counterpart in the reference (binding self-check only). */
 int rbgpu_abi_sizes(int32_t* out);
 

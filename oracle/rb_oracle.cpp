@@ -1503,12 +1503,89 @@ struct Opts {
   bool fresnel;
 };
 
+// ---- border-surface conditions (include/ABorderSurfaceCondition.h:24-45)
+// AOpticalComponent::FindBorderSurfaceCondition (src/AOpticalComponent.cxx:51-65):
+// the first condition of the current node's volume whose second component is
+// the next node's volume (-1: the null node).
+static const rb_border_desc* find_border(const Scene& sc, int cur, int next) {
+  if (cur < 0 || sc.d->n_borders <= 0) return nullptr;
+  const rb_volume_desc& v = vol_of(sc, cur);
+  const int v2 = next < 0 ? -1 : node(sc, next).volume;
+  for (int k = 0; k < v.border_count; k++)
+    if (sc.d->borders[v.border_first + k].volume2 == v2) return &sc.d->borders[v.border_first + k];
+  return nullptr;
+}
+
+// Uniforms one stochastic surface sample may consume before the smooth
+// normal is kept (the reference's loops are unbounded; the device caps them
+// the same way so that every lane ends).
+static const uint32_t kMaxSurfaceDraws = 1u << 16;
+
+// TVector3::RotateUz(u) (u a unit vector)
+static V3 rotate_uz(V3 v, V3 u) {
+  double up = u.x * u.x + u.y * u.y;
+  if (up != 0.) {
+    up = std::sqrt(up);
+    return v3((u.x * u.z * v.x - u.y * v.y + u.x * up * v.z) / up, (u.y * u.z * v.x + u.x * v.y + u.y * up * v.z) / up,
+              (u.z * u.z * v.x - v.x + u.z * up * v.z) / up);
+  }
+  if (u.z < 0.) return v3(-v.x, v.y, -v.z);
+  return v;
+}
+
+// GetFacetNormal (src/AOpticsManager.cxx:250-301): the smooth normal, or for
+// a rough condition the Geant4-style facet normal. TRandom::Gaus is replaced
+// by Box-Muller on two Philox uniforms.
+static V3 facet_normal(const Nav& nv, const Ray& ray, const rb_border_desc* bc, Rng& rng) {
+  V3 normal = find_normal(nv);
+  if (!bc || bc->lambertian || bc->sigma == 0.) return normal;
+  const double sigma = bc->sigma;
+  const double f_max = std::min(1., 4. * sigma);
+  const uint32_t stop = rng.draw + kMaxSurfaceDraws;
+  while (rng.draw < stop) {
+    double alpha = 0.;
+    bool ok = false;
+    while (rng.draw < stop) {
+      const double u1 = rng.uniform(), u2 = rng.uniform();
+      alpha = sigma * (std::sqrt(-2. * std::log(u1)) * std::cos(2. * kPi * u2));
+      if (!(f_max * rng.uniform() > std::sin(alpha) || alpha >= kPi / 2)) {
+        ok = true;
+        break;
+      }
+    }
+    if (!ok) break;
+    const double phi = 2. * kPi * rng.uniform();
+    const double sa = std::sin(alpha), ca = std::cos(alpha), sp = std::sin(phi), cp = std::cos(phi);
+    const V3 f = rotate_uz(v3(sa * cp, sa * sp, ca), normal);
+    if (dot(ray.d, f) > 0.0) return f;
+  }
+  return normal;
+}
+
+// DoReflection's Lambertian direction (src/AOpticsManager.cxx:209-224)
+static V3 lambert_dir(V3 n, Rng& rng) {
+  const double y = 0.5 * rng.uniform();
+  const double theta = std::asin(std::sqrt(2 * y));
+  const double phi = 2. * kPi * rng.uniform();
+  const double rad2deg = 180. / kPi, deg2rad = kPi / 180.;
+  const double theta_n =
+      (n.x == 0. && n.y == 0. && n.z == 0. ? 0. : std::atan2(std::sqrt(n.x * n.x + n.y * n.y), n.z)) * rad2deg;
+  const double phi_n = (n.x == 0. && n.y == 0. ? 0. : std::atan2(n.y, n.x)) * rad2deg;
+  // TGeoRotation("", phi_n + 90, theta_n + 180, 0): Euler z-x-z, LocalToMaster
+  const double sph = std::sin(deg2rad * (phi_n + 90)), cph = std::cos(deg2rad * (phi_n + 90));
+  const double sth = std::sin(deg2rad * (theta_n + 180)), cth = std::cos(deg2rad * (theta_n + 180));
+  const V3 v = v3(std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta));
+  return v3(cph * v.x - cth * sph * v.y + sth * sph * v.z, sph * v.x + cth * cph * v.y - sth * cph * v.z,
+            sth * v.y + cth * v.z);
+}
+
 // src/AOpticsManager.cxx:170-247
-static void do_reflection(double n1, Ray& ray, Nav& nv, int next, const V3* normal, Rng& rng) {
+static void do_reflection(double n1, Ray& ray, Nav& nv, int current, int next, const V3* normal, Rng& rng) {
   count_op(OP_REFLECT);
   const Scene& sc = *nv.sc;
   double step = nv.step;
-  V3 n = normal ? *normal : find_normal(nv);
+  const rb_border_desc* bc = find_border(sc, current, next);
+  V3 n = normal ? *normal : facet_normal(nv, ray, bc, rng);
   V3 d1 = ray.d;
   double cos1 = dot(d1, n);
   bool absorbed = false;
@@ -1520,6 +1597,7 @@ static void do_reflection(double n1, Ray& ray, Nav& nv, int next, const V3* norm
     }
   }
   V3 d2 = v3(d1.x - 2 * n.x * cos1, d1.y - 2 * n.y * cos1, d1.z - 2 * n.z * cos1);
+  if (bc && bc->lambertian) d2 = lambert_dir(n, rng);  // drawn even for an absorbed ray
   if (!absorbed) ray.set_direction(d2);
   double t = ray.t + step / (kC / n1);
   step_back(nv, mul(d1, -1));
@@ -1528,18 +1606,18 @@ static void do_reflection(double n1, Ray& ray, Nav& nv, int next, const V3* norm
 }
 
 // src/AOpticsManager.cxx:52-167 (multilayer branch out of scope)
-static void do_fresnel(double n1, double n2, double k2, Ray& ray, Nav& nv, int next, const Opts& o,
+static void do_fresnel(double n1, double n2, double k2, Ray& ray, Nav& nv, int current, int next, const Opts& o,
                        Rng& rng) {
   count_op(OP_REFRACT);
   double step = nv.step;
-  V3 n = find_normal(nv);
+  V3 n = facet_normal(nv, ray, find_border(*nv.sc, current, next), rng);
   V3 d1 = ray.d;
   double cos1 = dot(d1, n);
   double sin1 = std::sqrt(1 - cos1 * cos1);
   double sin2 = n1 * sin1 / n2;
   double cos2 = std::sqrt(1 - sin2 * sin2);
   if (sin2 > 1.) {
-    do_reflection(n1, ray, nv, next, &n, rng);
+    do_reflection(n1, ray, nv, current, next, &n, rng);
     return;
   }
   if (o.fresnel) {
@@ -1567,7 +1645,7 @@ static void do_fresnel(double n1, double n2, double k2, Ray& ray, Nav& nv, int n
     }
     double R = (Rs + Rp) / 2.;
     if (rng.uniform() < R) {
-      do_reflection(n1, ray, nv, next, &n, rng);
+      do_reflection(n1, ray, nv, current, next, &n, rng);
       return;
     }
   }
@@ -1619,10 +1697,10 @@ static void trace_ray(const Scene& sc, const Opts& o, Ray& ray, Rng& rng) {
     bool c_free = tc == RB_KNULL || tc == RB_KOPT || tc == RB_KOTHER;
     if ((c_free || tc == RB_KLENS) && tn == RB_KMIRROR) {
       double n1 = tc == RB_KLENS ? lens_index(sc.d, vol_of(sc, current), lambda) : 1.;
-      do_reflection(n1, ray, nv, next, nullptr, rng);
+      do_reflection(n1, ray, nv, current, next, nullptr, rng);
     } else if (c_free && tn == RB_KLENS) {
       const rb_volume_desc& v = vol_of(sc, next);
-      do_fresnel(1, lens_index(sc.d, v, lambda), lens_k(sc.d, v, lambda), ray, nv, next, o, rng);
+      do_fresnel(1, lens_index(sc.d, v, lambda), lens_k(sc.d, v, lambda), ray, nv, current, next, o, rng);
     } else if ((c_free || tc == RB_KLENS) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
       double n1 = tc == RB_KLENS ? lens_index(sc.d, vol_of(sc, current), lambda) : 1.;
       ray.add_point(nv.p, t1 + step / (kC / n1), next);
@@ -1631,9 +1709,9 @@ static void trace_ray(const Scene& sc, const Opts& o, Ray& ray, Rng& rng) {
     } else if (tc == RB_KLENS && tn == RB_KLENS) {
       const rb_volume_desc& v = vol_of(sc, next);
       do_fresnel(lens_index(sc.d, vol_of(sc, current), lambda), lens_index(sc.d, v, lambda),
-                 lens_k(sc.d, v, lambda), ray, nv, next, o, rng);
+                 lens_k(sc.d, v, lambda), ray, nv, current, next, o, rng);
     } else if (tc == RB_KLENS && (tn == RB_KNULL || tn == RB_KOPT || tn == RB_KOTHER)) {
-      do_fresnel(lens_index(sc.d, vol_of(sc, current), lambda), 1, 0, ray, nv, next, o, rng);
+      do_fresnel(lens_index(sc.d, vol_of(sc, current), lambda), 1, 0, ray, nv, current, next, o, rng);
     }
 
     if (tn == RB_KNULL) {
@@ -1645,7 +1723,7 @@ static void trace_ray(const Scene& sc, const Opts& o, Ray& ray, Rng& rng) {
       const rb_volume_desc& v = vol_of(sc, next);
       double angle = 0.;
       if (v.qe_angle_table >= 0) {
-        V3 n = find_normal(nv);
+        V3 n = facet_normal(nv, ray, find_border(sc, current, next), rng);
         angle = std::acos(dot(ray.d, n));
       }
       double qe = v.qe_lambda_table >= 0 ? table_eval(sc.d, v.qe_lambda_table, lambda) : 1.;

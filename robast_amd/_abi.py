@@ -43,14 +43,17 @@ SHAPE_DTYPE = np.dtype(
 VOLUME_DTYPE = np.dtype(
     [("refl_const", "<f8"), ("shape", "<i4"), ("kind", "<i4"), ("refl_table", "<i4"),
      ("index_model", "<i4"), ("index_ref", "<i4"), ("k_table", "<i4"),
-     ("qe_lambda_table", "<i4"), ("qe_angle_table", "<i4")], align=True)
+     ("qe_lambda_table", "<i4"), ("qe_angle_table", "<i4"), ("border_first", "<i4"), ("border_count", "<i4")],
+    align=True)
+BORDER_DTYPE = np.dtype([("volume2", "<i4"), ("lambertian", "<i4"), ("sigma", "<f8")], align=True)
 NODE_DTYPE = np.dtype(
     [("rot", "<f8", 9), ("tr", "<f8", 3), ("volume", "<i4"), ("parent", "<i4"),
      ("first_child", "<i4"), ("n_children", "<i4")], align=True)
 TABLE_DTYPE = np.dtype([("offset", "<i4"), ("n", "<i4")], align=True)
 
 assert SHAPE_DTYPE.itemsize == 64
-assert VOLUME_DTYPE.itemsize == 40
+assert VOLUME_DTYPE.itemsize == 48
+assert BORDER_DTYPE.itemsize == 16
 assert NODE_DTYPE.itemsize == 112
 assert TABLE_DTYPE.itemsize == 8
 
@@ -67,6 +70,7 @@ class rb_scene_desc(C.Structure):
         ("n_params", C.c_int32), ("params", C.c_void_p),
         ("n_tables", C.c_int32), ("tables", C.c_void_p),
         ("n_table_xy", C.c_int32), ("table_xy", C.c_void_p),
+        ("n_borders", C.c_int32), ("borders", C.c_void_p),
     ]
 
 

@@ -63,20 +63,25 @@ struct DScene {
   const rb_volume_desc* volumes;
   const rb_table_desc* tables;
   const double* table_xy;
+  const rb_border_desc* borders;
   int32_t n_nodes;
   // BVH entries [stage_lo, stage_lo + stage_n) -- the top node's BVH, at
   // most kBvhStage entries -- are read from an LDS copy that each tracing
   // kernel makes at entry (stage_n = 0: none).
   int32_t stage_lo, stage_n;
-  // deepest boolean nesting the navigation code is compiled for
+  // deepest boolean nesting the navigation code is compiled for, and
+  // whether it handles border-surface conditions (roughness, Lambertian)
   static constexpr int max_depth = kMaxCsgDepth;
+  static constexpr bool borders_on = true;
 };
 constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // The same scene, for kernels compiled for scenes whose composites nest at
 // most MAXD levels: the deeper composite call sites, and the registers they
-// make the kernel spill around them, are left out.
-template <int MAXD> struct DSceneOf : DScene {
+// make the kernel spill around them, are left out; likewise the border-surface
+// code unless BORDERS.
+template <int MAXD, bool BORDERS = false> struct DSceneOf : DScene {
   static constexpr int max_depth = MAXD;
+  static constexpr bool borders_on = BORDERS;
 };
 
 template <typename T>

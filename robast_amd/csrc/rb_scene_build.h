@@ -194,6 +194,14 @@ int validate(std::string& err, const rb_scene_desc* d) {
     if ((v.index_model == RB_INDEX_SELLMEIER || v.index_model == RB_INDEX_SCHOTT) &&
         (v.index_ref < 0 || v.index_ref + 6 > d->n_params))
       return fail_msg(err, -1, "scene: index formula parameters out of range");
+    if (v.border_count < 0 || v.border_first < 0 ||
+        int64_t(v.border_first) + v.border_count > std::max(d->n_borders, 0) || (v.border_count > 0 && !d->borders))
+      return fail_msg(err, -1, "scene: border-condition range out of bounds");
+  }
+  for (int i = 0; i < d->n_borders; i++) {
+    const rb_border_desc& b = d->borders[i];
+    if (b.volume2 < -2 || b.volume2 >= d->n_volumes) return fail_msg(err, -1, "scene: border volume out of range");
+    if (!(b.sigma >= 0.0) || !std::isfinite(b.sigma)) return fail_msg(err, -1, "scene: border roughness must be >= 0");
   }
   for (int i = 0; i < d->n_tables; i++) {
     const rb_table_desc& t = d->tables[i];
@@ -231,13 +239,14 @@ size_t put(std::vector<char>& blob, const X* src, size_t count) {
 
 // Offsets of the DScene arrays inside the packed blob.
 struct SceneLayout {
-  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy;
+  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy, borders;
   int32_t n_nodes;
   bool has_top_box = false;  // the top node has daughters: their bounding box follows
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
   int32_t top_bvh_lo = 0, top_bvh_n = 0;  // the top node's BVH entries
   int32_t max_depth = 0;                  // deepest boolean nesting of any shape
   bool has_lens = false;                  // a placed node is an ALens
+  bool has_borders = false;               // border-surface conditions are defined
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -284,6 +293,8 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
   lay.volumes = put(blob, d->volumes, size_t(d->n_volumes));
   lay.tables = put(blob, d->tables, size_t(std::max(d->n_tables, 0)));
   lay.table_xy = put(blob, d->table_xy, size_t(std::max(d->n_table_xy, 0)));
+  lay.borders = put(blob, d->borders, size_t(std::max(d->n_borders, 0)));
+  lay.has_borders = d->n_borders > 0;
   lay.n_nodes = d->n_nodes;
   lay.has_top_box = d->n_nodes > 0 && nodes[0].bvh_root >= 0;
   if (lay.has_top_box) {
@@ -309,6 +320,7 @@ inline DScene scene_view(const char* base, const SceneLayout& lay) {
   s.volumes = reinterpret_cast<const rb_volume_desc*>(base + lay.volumes);
   s.tables = reinterpret_cast<const rb_table_desc*>(base + lay.tables);
   s.table_xy = reinterpret_cast<const double*>(base + lay.table_xy);
+  s.borders = reinterpret_cast<const rb_border_desc*>(base + lay.borders);
   s.n_nodes = lay.n_nodes;
   s.stage_lo = 0;
   s.stage_n = 0;

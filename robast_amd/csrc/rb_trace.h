@@ -522,6 +522,111 @@ __device__ __forceinline__ double draw(const TraceParams& P, uint64_t id, Lane& 
   return uniform01(P.seed, id, L.draw++, 0u);
 }
 
+// ---- border-surface conditions (ABorderSurfaceCondition)
+// AOpticalComponent::FindBorderSurfaceCondition (src/AOpticalComponent.cxx:51-65):
+// the first condition of node `cur`'s volume whose second component is node
+// `next`'s volume (-1: the null node). Kernels built for scenes without
+// conditions compile none of this.
+template <typename SC>
+__device__ __forceinline__ const rb_border_desc* find_border(const SC& S, int cur, int next) {
+  if constexpr (!SC::borders_on) {
+    return nullptr;
+  } else {
+    if (cur < 0) return nullptr;
+    const rb_volume_desc& v = vol_of(S, cur);
+    const int v2 = next < 0 ? -1 : S.nodes[next].volume;
+    for (int k = 0; k < v.border_count; k++)
+      if (S.borders[v.border_first + k].volume2 == v2) return &S.borders[v.border_first + k];
+    return nullptr;
+  }
+}
+
+// The draws of one stochastic surface sample are bounded: a rejection loop
+// that has not accepted after kMaxSurfaceDraws uniforms keeps the smooth
+// normal (the reference's loops are unbounded; the cap only guarantees that
+// every lane ends).
+constexpr uint32_t kMaxSurfaceDraws = 1u << 16;
+struct SurfaceSample {
+  V3<double> v;
+  uint32_t draw;
+};
+
+// TVector3::RotateUz(u) of v (u a unit vector): the frame whose z axis is u.
+__device__ inline V3<double> rotate_uz(V3<double> v, V3<double> u) {
+  double up = u.x * u.x + u.y * u.y;
+  if (up != 0.) {
+    up = sqrt(up);
+    return mk((u.x * u.z * v.x - u.y * v.y + u.x * up * v.z) / up, (u.y * u.z * v.x + u.x * v.y + u.y * up * v.z) / up,
+              (u.z * u.z * v.x - v.x + u.z * up * v.z) / up);
+  }
+  if (u.z < 0.) return mk(-v.x, v.y, -v.z);
+  return v;
+}
+
+// GetFacetNormal's roughness branch (src/AOpticsManager.cxx:266-298, after
+// Geant4's G4OpBoundaryProcess::GetFacetNormal): alpha ~ Gaus(0, sigma),
+// accepted with probability sin(alpha) / min(1, 4 sigma) below pi/2; a
+// uniform azimuth; the tilted normal about n, drawn again until it faces the
+// ray (d . n' > 0). Gaus is Box-Muller on two Philox uniforms.
+__device__ RB_COLD_FN SurfaceSample rough_normal(uint64_t seed, uint64_t id, uint32_t draw, V3<double> n, V3<double> d,
+                                                 double sigma) {
+  const uint32_t stop = draw + kMaxSurfaceDraws;
+  const double f_max = fmin(1., 4. * sigma);
+  while (draw < stop) {
+    double alpha = 0.;
+    bool ok = false;
+    while (draw < stop) {
+      const double u1 = uniform01(seed, id, draw++, 0u), u2 = uniform01(seed, id, draw++, 0u);
+      alpha = sigma * (sqrt(-2. * log(u1)) * cos(2. * kPi * u2));
+      if (!(f_max * uniform01(seed, id, draw++, 0u) > sin(alpha) || alpha >= kPi / 2)) {
+        ok = true;
+        break;
+      }
+    }
+    if (!ok) break;
+    const double phi = 2. * kPi * uniform01(seed, id, draw++, 0u);
+    const double sa = sin(alpha), ca = cos(alpha), sp = sin(phi), cp = cos(phi);
+    const V3<double> f = rotate_uz(mk(sa * cp, sa * sp, ca), n);
+    if (dot(d, f) > 0.0) return {f, draw};
+  }
+  return {n, draw};
+}
+
+// DoReflection's Lambertian branch (src/AOpticsManager.cxx:209-224): theta =
+// asin(sqrt(2 y)), y ~ U(0, 1/2), phi ~ U(0, 2 pi), turned by
+// TGeoRotation(phi_n + 90, theta_n + 180, 0) of the normal's angles (degrees).
+__device__ RB_COLD_FN SurfaceSample lambert_dir(uint64_t seed, uint64_t id, uint32_t draw, V3<double> n) {
+  const double y = 0.5 * uniform01(seed, id, draw++, 0u);
+  const double theta = asin(sqrt(2 * y));
+  const double phi = 2. * kPi * uniform01(seed, id, draw++, 0u);
+  const double rad2deg = 180. / kPi, deg2rad = kPi / 180.;
+  const double theta_n = (n.x == 0. && n.y == 0. && n.z == 0. ? 0. : atan2(sqrt(n.x * n.x + n.y * n.y), n.z)) * rad2deg;
+  const double phi_n = (n.x == 0. && n.y == 0. ? 0. : atan2(n.y, n.x)) * rad2deg;
+  const double sph = sin(deg2rad * (phi_n + 90)), cph = cos(deg2rad * (phi_n + 90));
+  const double sth = sin(deg2rad * (theta_n + 180)), cth = cos(deg2rad * (theta_n + 180));
+  const V3<double> v = mk(sin(theta) * cos(phi), sin(theta) * sin(phi), cos(theta));
+  // TGeoRotation::SetAngles(phi, theta, psi = 0), LocalToMaster
+  return {mk(cph * v.x - cth * sph * v.y + sth * sph * v.z, sph * v.x + cth * cph * v.y - sth * cph * v.z,
+             sth * v.y + cth * v.z),
+          draw};
+}
+
+// AOpticsManager::GetFacetNormal (src/AOpticsManager.cxx:250-301): the
+// navigator's normal, tilted by the border condition's roughness.
+template <typename SC>
+__device__ __forceinline__ V3<double> facet_normal(const SC& S, const TraceParams& P, Lane& L,
+                                                   const rb_border_desc* bc) {
+  V3<double> n = find_normal(S, L);
+  if constexpr (SC::borders_on) {
+    if (bc && !bc->lambertian && bc->sigma != 0.) {
+      const SurfaceSample r = rough_normal(P.seed, L.ray, L.draw, n, L.d, bc->sigma);
+      L.draw = r.draw;
+      n = r.v;
+    }
+  }
+  return n;
+}
+
 __device__ __forceinline__ void add_point(const RayIO& io, int64_t i, Lane& L, V3<double> x, double t, int node) {
   if (io.path_cap > 0 && L.npoints - 1 < io.path_cap) {
     double* s = io.path_xyzt + (int64_t(io.path_cap) * i + (L.npoints - 1)) * 4;
@@ -541,7 +646,7 @@ __device__ __forceinline__ void set_direction(Lane& L, V3<double> v) {
 // AOpticsManager::DoReflection (src/AOpticsManager.cxx:170-247)
 template <typename Rec, typename SC>
 __device__ void do_reflection(const SC& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id,
-                              Lane& L, double n1, int next, V3<double> n, Rec& rec) {
+                              Lane& L, double n1, int next, V3<double> n, Rec& rec, const rb_border_desc* bc) {
   double step = L.step;
   V3<double> d1 = L.d;
   double cos1 = dot(d1, n);
@@ -557,6 +662,13 @@ __device__ void do_reflection(const SC& S, const TraceParams& P, const RayIO& io
     }
   }
   V3<double> d2 = mk(d1.x - 2 * n.x * cos1, d1.y - 2 * n.y * cos1, d1.z - 2 * n.z * cos1);
+  if constexpr (SC::borders_on) {
+    if (bc && bc->lambertian) {  // drawn even for an absorbed ray, as the reference does
+      const SurfaceSample l = lambert_dir(P.seed, id, L.draw, n);
+      L.draw = l.draw;
+      d2 = l.v;
+    }
+  }
   if (!absorbed) {
     set_direction(L, d2);
     rec.reflect(next, L.p, d1);
@@ -675,9 +787,10 @@ __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, in
         n2 = lens_index(S, v, lambda);
         k2 = lens_k(S, v, lambda);
       }
-      const V3<double> n = find_normal(S, L);
+      const rb_border_desc* bc = find_border(S, current, next);
+      const V3<double> n = facet_normal(S, P, L, bc);
       if (mirror || do_fresnel(P, io, i, id, L, n1, n2, k2, next, n, rec))
-        do_reflection(S, P, io, i, id, L, n1, next, n, rec);
+        do_reflection(S, P, io, i, id, L, n1, next, n, rec, bc);
     } else if ((c_free || c_lens) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
       double n1 = c_lens ? lens_index(S, vol_of(S, current), lambda) : 1.;
       add_point(io, i, L, L.p, t1 + step / (kC / n1), next);
@@ -696,7 +809,7 @@ __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, in
     } else if (tn == RB_KFOCUS) {
       const rb_volume_desc& v = vol_of(S, next);
       double angle = 0.;
-      if (v.qe_angle_table >= 0) angle = cold_acos(dot(L.d, find_normal(S, L)));
+      if (v.qe_angle_table >= 0) angle = cold_acos(dot(L.d, facet_normal(S, P, L, find_border(S, current, next))));
       double qe = v.qe_lambda_table >= 0 ? table_eval(S, v.qe_lambda_table, lambda) : 1.;
       if (v.qe_angle_table >= 0) qe *= table_eval(S, v.qe_angle_table, angle);
       L.status = (qe == 1 || draw(P, id, L) < qe) ? RB_FOCUS : RB_STOP;

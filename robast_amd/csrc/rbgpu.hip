@@ -42,6 +42,7 @@ struct rb_scene {
   DScene s{};
   int max_depth = 0;  // deepest boolean nesting among the scene's shapes
   bool has_lens = false;  // a placed volume refracts (ALens)
+  bool has_borders = false;  // border-surface conditions are defined
   bool has_top_box = false;
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
 };
@@ -49,8 +50,8 @@ struct rb_scene {
 namespace {
 
 // The scene as the kernels compiled for composites at most MAXD deep take it.
-template <int MAXD> DSceneOf<MAXD> scene_of(const rb_scene* sc) {
-  DSceneOf<MAXD> s;
+template <int MAXD, bool BORDERS = false> DSceneOf<MAXD, BORDERS> scene_of(const rb_scene* sc) {
+  DSceneOf<MAXD, BORDERS> s;
   static_cast<DScene&>(s) = sc->s;
   return s;
 }
@@ -228,8 +229,9 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
 // input order). A ray is traced from start to end by the lane that claimed
 // it, with the same per-ray Philox stream, so results equal k_trace's. Every
 // wave leaves when the counter has passed n and none of its lanes holds a ray.
-template <bool MIXED, int MAXD>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSceneOf<MAXD> S, RayIO io, int64_t n,
+template <bool MIXED, int MAXD, bool BORDERS>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSceneOf<MAXD, BORDERS> S, RayIO io,
+                                                                          int64_t n,
                                                                           TraceParams P,
                                                                           unsigned long long* next_ray,
                                                                           const uint32_t* perm) {
@@ -278,8 +280,8 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
 
 // One lane per ray. With `perm`, lane t traces ray perm[t] (the coherence
 // order); the ray keeps its own index for its Philox stream and its slots.
-template <bool MIXED, int MAXD>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD> S, RayIO io, int64_t n,
+template <bool MIXED, int MAXD, bool BORDERS>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD, BORDERS> S, RayIO io, int64_t n,
                                                                   TraceParams P, const uint32_t* perm) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
@@ -483,22 +485,22 @@ __global__ void __launch_bounds__(kSearchBlock) k_containment(SparseHist Hs, con
 
 // ---------------------------------------------------------------- launches
 // The trace kernel for the scene's depth and the lane schedule.
-template <bool MIXED, int MAXD>
+template <bool MIXED, int MAXD, bool BORDERS>
 static int launch_trace(rb_ctx* ctx, const rb_scene* scene, bool persistent, const RayIO& io, int64_t n,
                         const TraceParams& P, const uint32_t* perm) {
   const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
   if (persistent) {
     int per_cu = 0;
-    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace_persist<MIXED, MAXD>,
-                                                             block, 0));
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, (const void*)k_trace_persist<MIXED, MAXD, BORDERS>, block, 0));
     const int64_t pgrid = std::min<int64_t>(grid, int64_t(std::max(per_cu, 1)) * ctx->cus);
     HIPCHK(ctx, hipMemsetAsync(ctx->next_ray, 0, sizeof(unsigned long long), ctx->stream));
-    hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
-                       scene_of<MAXD>(scene), io, n, P, ctx->next_ray, perm);
+    hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
+                       scene_of<MAXD, BORDERS>(scene), io, n, P, ctx->next_ray, perm);
   } else {
-    hipLaunchKernelGGL((k_trace<MIXED, MAXD>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
-                       scene_of<MAXD>(scene), io, n, P, perm);
+    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                       scene_of<MAXD, BORDERS>(scene), io, n, P, perm);
   }
   HIPCHK(ctx, hipGetLastError());
   return 0;
@@ -625,7 +627,8 @@ int rbgpu_abi_sizes(int32_t* out) {
   out[8] = sizeof(rb_raygen_desc);
   out[9] = sizeof(rb_adjoint_desc);
   out[10] = sizeof(rb_hist2d_desc);
-  return 11;
+  out[11] = sizeof(rb_border_desc);
+  return 12;
 }
 
 int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
@@ -655,6 +658,7 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
   sc->s.stage_n = std::min(lay.top_bvh_n, int32_t(kBvhStage));
   sc->max_depth = lay.max_depth;
   sc->has_lens = lay.has_lens;
+  sc->has_borders = lay.has_borders;
   sc->has_top_box = lay.has_top_box;
   for (int k = 0; k < 3; k++) {
     sc->top_lo[k] = lay.top_lo[k];
@@ -752,11 +756,18 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
   }
   if (persistent && !ctx->next_ray) HIPCHK(ctx, hipMalloc(&ctx->next_ray, sizeof(unsigned long long)));
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-  const bool shallow = scene->max_depth <= 1;
-  const int rc = mixed ? (shallow ? launch_trace<true, 1>(ctx, scene, persistent, io, n, P, perm)
-                                  : launch_trace<true, kMaxCsgDepth>(ctx, scene, persistent, io, n, P, perm))
-                       : (shallow ? launch_trace<false, 1>(ctx, scene, persistent, io, n, P, perm)
-                                  : launch_trace<false, kMaxCsgDepth>(ctx, scene, persistent, io, n, P, perm));
+  // kernels: composites at most one level deep and no border conditions;
+  // deeper composites; border conditions (the general form)
+  const int form = scene->has_borders ? 2 : scene->max_depth <= 1 ? 0 : 1;
+  int rc;
+  if (mixed)
+    rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, persistent, io, n, P, perm)
+       : form == 1 ? launch_trace<true, kMaxCsgDepth, false>(ctx, scene, persistent, io, n, P, perm)
+                   : launch_trace<true, kMaxCsgDepth, true>(ctx, scene, persistent, io, n, P, perm);
+  else
+    rc = form == 0 ? launch_trace<false, 1, false>(ctx, scene, persistent, io, n, P, perm)
+       : form == 1 ? launch_trace<false, kMaxCsgDepth, false>(ctx, scene, persistent, io, n, P, perm)
+                   : launch_trace<false, kMaxCsgDepth, true>(ctx, scene, persistent, io, n, P, perm);
   if (rc) return rc;
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
@@ -821,6 +832,8 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
     return fail(ctx, -1, "rbgpu_adjoint: z parameter or focal node out of range");
   if (adj->mode != RB_ADJ_CENTROID && adj->mode != RB_ADJ_MOMENTS)
     return fail(ctx, -1, "rbgpu_adjoint: unknown mode");
+  if (scene->has_borders)
+    return fail(ctx, -3, "rbgpu_adjoint: rough or Lambertian border surfaces are not differentiated");
   const int groups = adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 ? adj->n_groups : 1;
   const int64_t group_size = adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 ? adj->group_size : std::max<int64_t>(n, 1);
   if (adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 &&

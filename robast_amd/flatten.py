@@ -32,7 +32,11 @@ def volume_kind(vol):
 class FlatScene:
     """Owns the numpy arrays behind an rb_scene_desc."""
 
-    def __init__(self, nodes, children, volumes, shapes, params, tables, table_xy, node_names, volume_objs):
+    def __init__(self, nodes, children, volumes, shapes, params, tables, table_xy, node_names, volume_objs,
+                 borders=None):
+        if borders is None:
+            borders = np.zeros(0, dtype=A.BORDER_DTYPE)
+        self.borders = borders
         self.nodes = nodes
         self.children = children
         self.volumes = volumes
@@ -50,6 +54,7 @@ class FlatScene:
         d.n_params, d.params = len(params), A.ptr(params)
         d.n_tables, d.tables = len(tables), A.ptr(tables)
         d.n_table_xy, d.table_xy = len(table_xy), A.ptr(table_xy)
+        d.n_borders, d.borders = len(borders), (A.ptr(borders) if len(borders) else None)
         self.desc = d
 
     def desc_ptr(self):
@@ -172,6 +177,22 @@ def flatten(top_volume):
         for k, v in rec.items():
             vol_arr[i][k] = v
 
+    # border-surface conditions, per component1 volume in registration order;
+    # a component2 that is not placed can never be entered (-2 matches nothing)
+    border_recs = []
+    for i, vol in enumerate(volume_objs):
+        conds = getattr(vol, "border_conditions", [])
+        vol_arr[i]["border_first"] = len(border_recs)
+        vol_arr[i]["border_count"] = len(conds)
+        for c in conds:
+            if c.multilayer is not None:
+                raise NotImplementedError("ABorderSurfaceCondition: multilayer coatings are not supported")
+            v2 = -1 if c.component2 is None else volume_ids.get(id(c.component2), -2)
+            border_recs.append((v2, 1 if c.lambertian else 0, c.sigma))
+    borders = np.zeros(len(border_recs), dtype=A.BORDER_DTYPE)
+    for i, (v2, lam, sig) in enumerate(border_recs):
+        borders[i]["volume2"], borders[i]["lambertian"], borders[i]["sigma"] = v2, lam, sig
+
     shp = np.zeros(len(shapes), dtype=A.SHAPE_DTYPE)
     for i, (t, off, n, org, half) in enumerate(shapes):
         shp[i]["type"], shp[i]["param_offset"], shp[i]["n_params"] = t, off, n
@@ -183,7 +204,7 @@ def flatten(top_volume):
 
     par = np.array(params if params else [0.0], dtype=np.float64)
     txy = np.array(table_xy if table_xy else [0.0, 0.0], dtype=np.float64)
-    fs = FlatScene(nodes, children, vol_arr, shp, par, tab, txy, [r[3] for r in node_recs], volume_objs)
+    fs = FlatScene(nodes, children, vol_arr, shp, par, tab, txy, [r[3] for r in node_recs], volume_objs, borders)
     fs.desc.n_tables = len(tables)
     fs.desc.n_table_xy = len(table_xy)
     return fs

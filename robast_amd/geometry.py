@@ -598,6 +598,61 @@ class TGeoVolume:
 class AOpticalComponent(TGeoVolume):
     kind = A.RB_KOPT
 
+    def AddBorderSurfaceCondition(self, condition):
+        """src/AOpticalComponent.cxx:40-48."""
+        if not hasattr(self, "border_conditions"):
+            self.border_conditions = []
+        self.border_conditions.append(condition)
+
+    def FindBorderSurfaceCondition(self, component2):
+        """src/AOpticalComponent.cxx:51-65: the first condition registered
+        with this component whose second component is component2."""
+        for c in getattr(self, "border_conditions", []):
+            if c.component2 is component2:
+                return c
+        return None
+
+
+class ABorderSurfaceCondition:
+    """include/ABorderSurfaceCondition.h:24-45 (src/ABorderSurfaceCondition.cxx):
+    the surface between component1 and component2 (None = leaving the top
+    volume), registered with component1. Gaussian roughness (rad) perturbs
+    the facet normal; Lambertian makes reflections diffuse. Multilayer
+    coatings are not supported by the flattener."""
+
+    def __init__(self, component1, component2):
+        self.component1 = component1
+        self.component2 = component2
+        self.sigma = 0.0
+        self.lambertian = False
+        self.multilayer = None
+        if component1 is not None:
+            component1.AddBorderSurfaceCondition(self)
+
+    def GetComponent1(self):
+        return self.component1
+
+    def GetComponent2(self):
+        return self.component2
+
+    def SetGaussianRoughness(self, sigma):
+        self.sigma = abs(float(sigma))
+
+    def GetGaussianRoughness(self):
+        return self.sigma
+
+    def EnableLambertian(self, enable):
+        self.lambertian = bool(enable)
+
+    def IsLambertian(self):
+        return self.lambertian
+
+    def SetMultilayer(self, layer):
+        self.multilayer = layer
+
+    def GetMultilayer(self):
+        return self.multilayer
+
 
 class AObscuration(AOpticalComponent):
     kind = A.RB_KOBS

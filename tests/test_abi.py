@@ -30,8 +30,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_binding():
     lib = _lib.load()
     out = (C.c_int32 * 16)()
-    assert lib.rbgpu_abi_sizes(out) == 11
-    sizes = list(out)[:11]
+    assert lib.rbgpu_abi_sizes(out) == 12
+    sizes = list(out)[:12]
     assert sizes[0] == 96                              # rb_matrix
     assert sizes[1] == A.SHAPE_DTYPE.itemsize
     assert sizes[2] == A.VOLUME_DTYPE.itemsize
@@ -43,6 +43,7 @@ def test_struct_sizes_match_binding():
     assert sizes[8] == C.sizeof(A.rb_raygen_desc)
     assert sizes[9] == C.sizeof(A.rb_adjoint_desc)
     assert sizes[10] == C.sizeof(A.rb_hist2d_desc) == 48
+    assert sizes[11] == A.BORDER_DTYPE.itemsize == 16
     assert C.sizeof(A.rb_trace_opts) == 40
 
 
