@@ -257,6 +257,104 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
 int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* gen, rb_rays* rays,
                         int64_t n);
 
+/* ---- ARayShooter on the device ------------------------------------------
+ * The ray sources of src/ARayShooter.cxx, written straight into HBM. Every
+ * kind places its local pattern with `rot` (positions, and the direction
+ * where the reference rotates it) and then `tr` (positions only), as
+ * TGeoRotation/TGeoTranslation::LocalToMaster do there; a null rotation or
+ * translation of the reference is the identity / zero here. Directions are
+ * normalised as ARay::SetDirection does (src/ARay.cxx:217-223; a zero vector
+ * is left (0, 0, 0)). Rays start with t = 0, status RB_RUN, one point.
+ * The random kinds draw from Philox streams keyed by (seed, id_base + i):
+ * the reference draws from the shared gRandom, so only the distribution can
+ * agree, and it does (the disk and sphere are drawn by inverse CDF where the
+ * reference rejects: the same uniform measure). */
+enum {
+  RB_SHOOT_RECTANGLE = 0,             /* Rectangle(lambda, dx=a, dy=b, nx, ny, rot, tr, v) :401-452;
+                                         Square(lambda, d, n, ...) :455-459 is a = b = d, nx = ny = n */
+  RB_SHOOT_CIRCLE = 1,                /* Circle(lambda, rmax=a, nr=nx, nphi=ny, rot, tr, v) :122-183   */
+  RB_SHOOT_RANDOM_CIRCLE = 2,         /* RandomCircle(lambda, rmax=a, n, rot, tr, v) :186-237          */
+  RB_SHOOT_RANDOM_CONE = 3,           /* RandomCone(lambda, r=a, d=b, n, rot, tr) :240-285 (dir unused) */
+  RB_SHOOT_RANDOM_RECTANGLE = 4,      /* RandomRectangle(lambda, dx=a, dy=b, n, rot, tr, v) :288-335;
+                                         RandomSquare(lambda, d, n, ...) :394-398 is a = b = d        */
+  RB_SHOOT_RANDOM_SPHERE = 5,         /* RandomSphere(lambda, n, tr) :338-356 (rot and dir unused)     */
+  RB_SHOOT_RANDOM_SPHERICAL_CONE = 6  /* RandomSphericalCone(lambda, n, theta=a [deg], rot, tr)
+                                         :359-391 (dir unused)                                        */
+};
+
+typedef struct {
+  int32_t kind;
+  int32_t nx;        /* RECTANGLE: nx; CIRCLE: nr                                 */
+  int32_t ny;        /* RECTANGLE: ny; CIRCLE: nphi                               */
+  int32_t pad;
+  double lambda;     /* cm                                                        */
+  double a;          /* see the kinds above                                       */
+  double b;
+  double rot[9];     /* rotation matrix, row-major, local -> master               */
+  double tr[3];      /* translation                                               */
+  double dir[3];     /* the `v` direction argument ((0, 0, 1) when the reference's
+                        v is null)                                                */
+  uint64_t seed;     /* Philox key of the random kinds                            */
+  uint64_t id_base;  /* stream of ray i is id_base + i                            */
+} rb_shooter_desc;
+
+/* Number of rays the reference's function makes for `desc` (n: its ray-count
+ * argument, used by the random kinds): nx * ny for RECTANGLE,
+ * 1 + nphi * nr (nr + 1) / 2 for CIRCLE, n for the random kinds, and 0 where
+ * the reference returns an empty array (negative sizes, counts below 1).
+ * Returns -1 for an unknown kind. */
+int64_t rbgpu_shooter_count(const rb_shooter_desc* desc, int64_t n);
+
+/* Write the rays of `desc` into `rays` (device pointers); n must equal
+ * rbgpu_shooter_count(desc, n). */
+int rbgpu_shoot_rays(rb_ctx* ctx, const rb_shooter_desc* desc, rb_rays* rays, int64_t n);
+
+/* ---- CORSIKA photon bunches -> rays ---------------------------------------
+ * The conversion of ACorsikaIACTFile::GetRayArray(telNo, arrayNo, z,
+ * refractiveIndex) (src/ACorsikaIACTFile.cxx:71-131) over the bunch table its
+ * reader fills (one entry per bunch, the TTree branches of :330-342, FP32 as
+ * there; cz = -sqrt(1 - cx^2 - cy^2) as the reader computes it, :452). */
+typedef struct {
+  const int32_t* tel_no;
+  const int32_t* array_no;
+  const float* x;        /* cm, at the telescope's observation level     */
+  const float* y;
+  const float* zem;      /* emission height (carried, not used)          */
+  const float* time;     /* ns                                           */
+  const float* cx;
+  const float* cy;
+  const float* cz;
+  const float* lambda;   /* nm; 0 = not set by CORSIKA                   */
+  const float* photons;  /* photons in the bunch (<= 1 << 24)            */
+} rb_bunches;
+
+typedef struct {
+  int32_t tel_no;            /* telNo                                              */
+  int32_t array_no;          /* arrayNo                                            */
+  double z;                  /* cm: start height of the photons (argument z)       */
+  double tel_z;              /* cm: GetTelescopeZ(telNo)                           */
+  double refractive_index;   /* argument refractiveIndex                           */
+  double lambda_min;         /* nm: fMinWavelength                                 */
+  double lambda_max;         /* nm: upper end of the drawn wavelength; the reference
+                                uses fMaxPhotonBunches here (:123-125)             */
+  uint64_t seed;             /* Philox key of the wavelength draws                 */
+  uint64_t id_base;          /* stream of output ray i is id_base + i              */
+  int32_t device_ptrs;       /* the bunch arrays are GPU memory                    */
+  int32_t pad;
+} rb_bunch_opts;
+
+/* Each bunch of telescope tel_no and array array_no becomes as many rays as
+ * the reference's `for (j = 0; j < photons; j++)` makes (ceil(photons), none
+ * for photons <= 0), in bunch order: start (x - D cx, y - D cy, z) with
+ * D = (z - tel_z) (-1 / cz), time time * 1 ns - D n / c, direction
+ * (cx, cy, cz) normalised, wavelength lambda nm, or for lambda == 0
+ * 1 / (1 / lambda_min - U (1 / lambda_min - 1 / lambda_max)) nm.
+ * *n_rays receives the number of rays. With rays == NULL only the count is
+ * computed; otherwise `rays` (device pointers) must hold `capacity` rays and
+ * the call fails without writing rays when the count exceeds it. */
+int rbgpu_bunches_to_rays(rb_ctx* ctx, const rb_bunches* bunches, int64_t n_bunches, const rb_bunch_opts* opts,
+                          rb_rays* rays, int64_t capacity, int64_t* n_rays);
+
 /* ---- adjoint of a focal-plane loss (differentiable alignment) ------------
  * Supported optics: chains of up to 4 reflections on spherical facets, each
  * the node shape `sphere(rmin = R) : T(0, 0, R) * cut` placed with rotation
@@ -344,7 +442,8 @@ int rbgpu_containment_radius(rb_ctx* ctx, const rb_rays* rays, int64_t n, const 
 /* Writes sizeof() of the ABI structs (rb_matrix, rb_shape_desc,
  * rb_volume_desc, rb_node_desc, rb_table_desc, rb_scene_desc, rb_rays,
  * rb_trace_opts, rb_raygen_desc, rb_adjoint_desc, rb_hist2d_desc,
- * rb_border_desc) into out[0..11] for binding checks; returns 12.
+ * rb_border_desc, rb_shooter_desc, rb_bunches, rb_bunch_opts) into
+ * out[0..14] for binding checks; returns 15.
  * No counterpart in the reference (binding self-check only). */
 int rbgpu_abi_sizes(int32_t* out);
 

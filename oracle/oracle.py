@@ -42,6 +42,13 @@ def lib():
         L.oracle_contains_batch.restype = C.c_int
         L.oracle_generate_rays.argtypes = [C.POINTER(A.rb_raygen_desc), C.POINTER(A.rb_rays), C.c_int64]
         L.oracle_generate_rays.restype = C.c_int
+        L.oracle_shooter_count.argtypes = [C.POINTER(A.rb_shooter_desc), C.c_int64]
+        L.oracle_shooter_count.restype = C.c_int64
+        L.oracle_shoot_rays.argtypes = [C.POINTER(A.rb_shooter_desc), C.POINTER(A.rb_rays), C.c_int64]
+        L.oracle_shoot_rays.restype = C.c_int
+        L.oracle_bunches_to_rays.argtypes = [C.POINTER(A.rb_bunches), C.c_int64, C.POINTER(A.rb_bunch_opts),
+                                             C.POINTER(A.rb_rays), C.c_int64]
+        L.oracle_bunches_to_rays.restype = C.c_int64
         L.oracle_adjoint_dual.argtypes = [C.POINTER(A.rb_scene_desc), C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays),
                                           C.c_int64, C.POINTER(A.rb_adjoint_desc), C.c_void_p, C.c_int]
         L.oracle_adjoint_dual.restype = C.c_int64
@@ -125,6 +132,32 @@ def generate(gen, n):
     soa = new_soa(n)
     r = rays_struct(soa)
     lib().oracle_generate_rays(C.byref(gen), C.byref(r), n)
+    return soa
+
+
+def shooter_count(desc, n=0):
+    return int(lib().oracle_shooter_count(C.byref(desc), n))
+
+
+def shoot(desc, n=0):
+    """oracle_shoot_rays: the ARayShooter restatement; returns the SoA."""
+    count = shooter_count(desc, n)
+    soa = new_soa(count)
+    rc = lib().oracle_shoot_rays(C.byref(desc), C.byref(rays_struct(soa)), count)
+    assert rc == 0, rc
+    return soa
+
+
+def bunches_to_rays(bunches, opts):
+    """oracle_bunches_to_rays over a CorsikaBunches table; returns the SoA
+    (or raises for a bunch above 2^24 photons)."""
+    d = bunches.desc()
+    total = lib().oracle_bunches_to_rays(C.byref(d), bunches.n, C.byref(opts), None, 0)
+    if total < 0:
+        raise ValueError("a selected bunch holds more than 2^24 photons")
+    soa = new_soa(total)
+    got = lib().oracle_bunches_to_rays(C.byref(d), bunches.n, C.byref(opts), C.byref(rays_struct(soa)), total)
+    assert got == total
     return soa
 
 

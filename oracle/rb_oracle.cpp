@@ -2095,6 +2095,217 @@ int oracle_generate_rays(const rb_raygen_desc* g, rb_rays* r, int64_t n) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// ARayShooter (src/ARayShooter.cxx:122-459), restated with its loops: the
+// pattern is built in the reference's order and each ray placed with
+// TGeoRotation::LocalToMaster (m_i = l0 R[3i] + l1 R[3i+1] + l2 R[3i+2]) and
+// TGeoTranslation::LocalToMaster (m_i = l_i + T_i), then ARay's constructor
+// (src/ARay.cxx:28-36: AddPoint, SetDirection normalising a non-zero vector,
+// :217-223). The random kinds draw Philox uniforms (stream 2, key seed,
+// counter id_base + ray) in place of gRandom; the disk and the sphere use
+// inverse-CDF draws where the reference rejects (RandomCircle :211-215,
+// RandomCone :250-256, TRandom::Sphere): the same uniform measure.
+namespace shoot {
+static void rot(const double* R, const double* l, double* m) {
+  for (int i = 0; i < 3; i++) m[i] = l[0] * R[3 * i] + l[1] * R[3 * i + 1] + l[2] * R[3 * i + 2];
+}
+static void trn(const double* T, const double* l, double* m) {
+  for (int i = 0; i < 3; i++) m[i] = l[i] + T[i];
+}
+struct Out {
+  rb_rays* r;
+  int64_t k = 0;
+  void add(double lam, const double* x, const double* d) {  // new ARay(0, lam, x, 0, d)
+    double mag = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double u[3] = {0, 0, 0};
+    if (mag > 0) { u[0] = d[0] / mag; u[1] = d[1] / mag; u[2] = d[2] / mag; }
+    r->x[k] = x[0]; r->y[k] = x[1]; r->z[k] = x[2]; r->t[k] = 0;
+    r->dx[k] = u[0]; r->dy[k] = u[1]; r->dz[k] = u[2];
+    r->lambda[k] = lam;
+    r->status[k] = RB_RUN;
+    r->npoints[k] = 1;
+    if (r->last_node) r->last_node[k] = -1;
+    k++;
+  }
+};
+}  // namespace shoot
+
+int64_t oracle_shooter_count(const rb_shooter_desc* g, int64_t n) {
+  switch (g->kind) {
+    case RB_SHOOT_RECTANGLE:
+      return (g->a < 0 || g->b < 0 || g->nx < 1 || g->ny < 1) ? 0 : int64_t(g->nx) * g->ny;
+    case RB_SHOOT_CIRCLE: {
+      if (g->a < 0 || g->nx < 1 || g->ny < 1) return 0;
+      int64_t c = 1;
+      for (int64_t i = 0; i < g->nx; i++) c += int64_t(g->ny) * (i + 1);
+      return c;
+    }
+    case RB_SHOOT_RANDOM_CIRCLE: return g->a < 0 ? 0 : std::max<int64_t>(n, 0);
+    case RB_SHOOT_RANDOM_RECTANGLE: return (g->a < 0 || g->b < 0 || n < 1) ? 0 : n;
+    case RB_SHOOT_RANDOM_CONE:
+    case RB_SHOOT_RANDOM_SPHERE:
+    case RB_SHOOT_RANDOM_SPHERICAL_CONE: return std::max<int64_t>(n, 0);
+  }
+  return -1;
+}
+
+int oracle_shoot_rays(const rb_shooter_desc* g, rb_rays* r, int64_t n) {
+  using namespace shoot;
+  if (oracle_shooter_count(g, n) != n) return -1;
+  Out out{r};
+  const double* R = g->rot;
+  const double* T = g->tr;
+  double dir[3], new_dir[3];
+  rot(R, g->dir, new_dir);  // the v direction, rotated (Rectangle :419-423 and the others alike)
+  switch (g->kind) {
+    case RB_SHOOT_RECTANGLE: {  // :401-452
+      const double dx = g->a, dy = g->b;
+      const int nx = g->nx, ny = g->ny;
+      double deltax = nx == 1 ? dx / 2 : dx / (nx - 1);
+      double deltay = ny == 1 ? dy / 2 : dy / (ny - 1);
+      for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+          double x[3] = {i * deltax - dx / 2, j * deltay - dy / 2, 0}, np[3];
+          rot(R, x, np);
+          trn(T, np, x);
+          out.add(g->lambda, x, new_dir);
+        }
+      break;
+    }
+    case RB_SHOOT_CIRCLE: {  // :122-183
+      const double rmax = g->a;
+      const int nr = g->nx, nphi = g->ny;
+      double position[3] = {0, 0, 0}, np[3];
+      trn(T, position, np);
+      out.add(g->lambda, np, new_dir);
+      for (int i = 0; i < nr; i++) {
+        double rr = rmax * (i + 1) / nr;
+        for (int j = 0; j < nphi * (i + 1); j++) {
+          double phi = 2 * kPi / nphi / (i + 1) * j;
+          double x[3] = {rr * std::cos(phi), rr * std::sin(phi), 0};
+          rot(R, x, np);
+          trn(T, np, x);
+          out.add(g->lambda, x, new_dir);
+        }
+      }
+      break;
+    }
+    case RB_SHOOT_RANDOM_CIRCLE:  // :186-237
+      for (int64_t i = 0; i < n; i++) {
+        Rng u{g->seed, g->id_base + uint64_t(i), 0u, 2u};
+        double rho = g->a * std::sqrt(u.uniform()), phi = 2 * kPi * u.uniform();
+        double x[3] = {rho * std::cos(phi), rho * std::sin(phi), 0}, np[3];
+        rot(R, x, np);
+        trn(T, np, x);
+        out.add(g->lambda, x, new_dir);
+      }
+      break;
+    case RB_SHOOT_RANDOM_CONE:  // :240-285: from tr toward a disk of radius r at z = d
+      for (int64_t i = 0; i < n; i++) {
+        Rng u{g->seed, g->id_base + uint64_t(i), 0u, 2u};
+        double rho = g->a * std::sqrt(u.uniform()), phi = 2 * kPi * u.uniform();
+        double goal[3] = {rho * std::cos(phi), rho * std::sin(phi), g->b}, tmp[3];
+        rot(R, goal, tmp);
+        trn(T, tmp, goal);
+        double start[3] = {0, 0, 0};
+        trn(T, start, tmp);
+        double d[3] = {goal[0] - tmp[0], goal[1] - tmp[1], goal[2] - tmp[2]};
+        out.add(g->lambda, tmp, d);
+      }
+      break;
+    case RB_SHOOT_RANDOM_RECTANGLE:  // :288-335
+      for (int64_t i = 0; i < n; i++) {
+        Rng u{g->seed, g->id_base + uint64_t(i), 0u, 2u};
+        double lo_x = -g->a / 2., lo_y = -g->b / 2.;
+        double px = lo_x + (g->a / 2. - lo_x) * u.uniform();  // TRandom::Uniform(x1, x2) = x1 + (x2 - x1) U
+        double py = lo_y + (g->b / 2. - lo_y) * u.uniform();
+        double x[3] = {px, py, 0}, np[3];
+        rot(R, x, np);
+        trn(T, np, x);
+        out.add(g->lambda, x, new_dir);
+      }
+      break;
+    case RB_SHOOT_RANDOM_SPHERE:  // :338-356: isotropic from tr, no rotation
+      for (int64_t i = 0; i < n; i++) {
+        Rng u{g->seed, g->id_base + uint64_t(i), 0u, 2u};
+        double cz = 2 * u.uniform() - 1, phi = 2 * kPi * u.uniform();
+        double s = std::sqrt(std::max(0.0, 1 - cz * cz));
+        dir[0] = s * std::cos(phi); dir[1] = s * std::sin(phi); dir[2] = cz;
+        double p[3] = {0, 0, 0}, np[3];
+        trn(T, p, np);
+        out.add(g->lambda, np, dir);
+      }
+      break;
+    case RB_SHOOT_RANDOM_SPHERICAL_CONE: {  // :359-391
+      const double c = std::cos(g->a * (kPi / 180.0));
+      for (int64_t i = 0; i < n; i++) {
+        Rng u{g->seed, g->id_base + uint64_t(i), 0u, 2u};
+        double ran = c + (1 - c) * u.uniform();
+        double th = std::acos(ran);
+        double phi = 0 + (2 * kPi - 0) * u.uniform();
+        dir[0] = std::sin(th) * std::cos(phi); dir[1] = std::sin(th) * std::sin(phi); dir[2] = std::cos(th);
+        double nd[3], p[3] = {0, 0, 0}, np[3];
+        rot(R, dir, nd);
+        trn(T, p, np);
+        out.add(g->lambda, np, nd);
+      }
+      break;
+    }
+    default: return -1;
+  }
+  return out.k == n ? 0 : -2;
+}
+
+// ACorsikaIACTFile::GetRayArray (src/ACorsikaIACTFile.cxx:71-131) over a
+// bunch table: the loop over bunches, the telescope/array filter, the shift of
+// each bunch from the observation level up to height z along its direction,
+// and `for (j = 0; j < photons; j++)` rays per bunch (float compare as there).
+// A bunch with lambda == 0 draws its wavelength per ray (Philox stream 3,
+// counter id_base + output ray). Returns the ray count; rays are written only
+// when `r` is not null and the count fits `capacity`. -1: photons above 2^24.
+int64_t oracle_bunches_to_rays(const rb_bunches* b, int64_t nb, const rb_bunch_opts* o, rb_rays* r,
+                               int64_t capacity) {
+  const double m = 1e2 * 1.0, cm = 1.0, nm = 1e-9 * m, ns = 1e-9 * 1.0;  // AOpticsManager units
+  int64_t total = 0;
+  for (int64_t i = 0; i < nb; i++) {
+    if (o->tel_no != b->tel_no[i] || o->array_no != b->array_no[i]) continue;
+    const float photons = b->photons[i];
+    if (photons > 16777216.0f) return -1;
+    for (int64_t j = 0; float(j) < photons; j++) total++;
+  }
+  if (!r || total > capacity) return total;
+  int64_t k = 0;
+  for (int64_t i = 0; i < nb; i++) {
+    if (o->tel_no != b->tel_no[i] || o->array_no != b->array_no[i]) continue;
+    const float x = b->x[i], y = b->y[i], time = b->time[i], cx = b->cx[i], cy = b->cy[i], cz = b->cz[i];
+    const float lambda = b->lambda[i], photons = b->photons[i];
+    double airmass = -1. / cz;
+    double tel_dist = (o->z - o->tel_z * cm) * airmass;
+    double speed = kC / o->refractive_index;
+    double px = x * cm - tel_dist * cx;
+    double py = y * cm - tel_dist * cy;
+    double pt = time * ns - tel_dist / speed;
+    for (int64_t j = 0; float(j) < photons; j++, k++) {
+      double lam = lambda;
+      if (lambda == 0) {
+        Rng u{o->seed, o->id_base + uint64_t(k), 0u, 3u};
+        lam = 1. / (1. / o->lambda_min - u.uniform() * (1. / o->lambda_min - 1. / o->lambda_max));
+      }
+      double d[3] = {cx, cy, cz};
+      double mag = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      double u3[3] = {0, 0, 0};
+      if (mag > 0) { u3[0] = d[0] / mag; u3[1] = d[1] / mag; u3[2] = d[2] / mag; }
+      r->x[k] = px; r->y[k] = py; r->z[k] = o->z; r->t[k] = pt;
+      r->dx[k] = u3[0]; r->dy[k] = u3[1]; r->dz[k] = u3[2];
+      r->lambda[k] = lam * nm;
+      r->status[k] = RB_RUN;
+      r->npoints[k] = 1;
+      if (r->last_node) r->last_node[k] = -1;
+    }
+  }
+  return total;
+}
+
 // Batch containment probe for brute-force distance checks (local frame of `shape`).
 int oracle_contains_batch(const rb_scene_desc* desc, int shape, int64_t n, const double* pts, uint8_t* out) {
   Scene sc{desc, {}};

@@ -36,6 +36,8 @@ RB_ORDER_AUTO, RB_ORDER_INPUT, RB_ORDER_COHERENT = 0, 1, 2
 RB_ADJ_CENTROID, RB_ADJ_MOMENTS = 0, 1
 
 RB_RAYGEN_GRID, RB_RAYGEN_DISK, RB_RAYGEN_CONE = 0, 1, 2
+(RB_SHOOT_RECTANGLE, RB_SHOOT_CIRCLE, RB_SHOOT_RANDOM_CIRCLE, RB_SHOOT_RANDOM_CONE, RB_SHOOT_RANDOM_RECTANGLE,
+ RB_SHOOT_RANDOM_SPHERE, RB_SHOOT_RANDOM_SPHERICAL_CONE) = range(7)
 
 SHAPE_DTYPE = np.dtype(
     [("type", "<i4"), ("param_offset", "<i4"), ("n_params", "<i4"), ("pad", "<i4"),
@@ -115,6 +117,31 @@ class rb_hist2d_desc(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("xlo", C.c_double), ("xhi", C.c_double),
         ("ylo", C.c_double), ("yhi", C.c_double), ("status", C.c_int32), ("device_ptrs", C.c_int32),
+    ]
+
+
+class rb_shooter_desc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("pad", C.c_int32),
+        ("lambda_", C.c_double), ("a", C.c_double), ("b", C.c_double),
+        ("rot", C.c_double * 9), ("tr", C.c_double * 3), ("dir", C.c_double * 3),
+        ("seed", C.c_uint64), ("id_base", C.c_uint64),
+    ]
+
+
+BUNCH_FIELDS_I = ("tel_no", "array_no")
+BUNCH_FIELDS_F = ("x", "y", "zem", "time", "cx", "cy", "cz", "lambda_", "photons")
+
+
+class rb_bunches(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in BUNCH_FIELDS_I + BUNCH_FIELDS_F]
+
+
+class rb_bunch_opts(C.Structure):
+    _fields_ = [
+        ("tel_no", C.c_int32), ("array_no", C.c_int32), ("z", C.c_double), ("tel_z", C.c_double),
+        ("refractive_index", C.c_double), ("lambda_min", C.c_double), ("lambda_max", C.c_double),
+        ("seed", C.c_uint64), ("id_base", C.c_uint64), ("device_ptrs", C.c_int32), ("pad", C.c_int32),
     ]
 
 

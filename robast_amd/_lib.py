@@ -19,7 +19,7 @@ EXPORTS = (
     "rbgpu_ctx_create", "rbgpu_ctx_destroy", "rbgpu_last_error", "rbgpu_scene_upload",
     "rbgpu_scene_free", "rbgpu_trace", "rbgpu_generate_rays", "rbgpu_last_trace_ms",
     "rbgpu_last_trace_launches", "rbgpu_abi_sizes", "rbgpu_adjoint", "rbgpu_last_kernel_ms",
-    "rbgpu_containment_radius",
+    "rbgpu_containment_radius", "rbgpu_shooter_count", "rbgpu_shoot_rays", "rbgpu_bunches_to_rays",
 )
 
 
@@ -65,6 +65,13 @@ def load():
                                              C.POINTER(A.rb_hist2d_desc), C.c_double, C.c_void_p, C.c_void_p,
                                              C.c_void_p]
     lib.rbgpu_containment_radius.restype = C.c_int
+    lib.rbgpu_shooter_count.argtypes = [C.POINTER(A.rb_shooter_desc), C.c_int64]
+    lib.rbgpu_shooter_count.restype = C.c_int64
+    lib.rbgpu_shoot_rays.argtypes = [C.c_void_p, C.POINTER(A.rb_shooter_desc), C.POINTER(A.rb_rays), C.c_int64]
+    lib.rbgpu_shoot_rays.restype = C.c_int
+    lib.rbgpu_bunches_to_rays.argtypes = [C.c_void_p, C.POINTER(A.rb_bunches), C.c_int64, C.POINTER(A.rb_bunch_opts),
+                                          C.POINTER(A.rb_rays), C.c_int64, C.POINTER(C.c_int64)]
+    lib.rbgpu_bunches_to_rays.restype = C.c_int
     lib.rbgpu_abi_sizes.argtypes = [C.POINTER(C.c_int32)]
     lib.rbgpu_abi_sizes.restype = C.c_int
     _lib = lib
@@ -80,6 +87,7 @@ class Context:
 
     def __init__(self, device=0):
         self.lib = load()
+        self.device = int(device)
         h = C.c_void_p()
         rc = self.lib.rbgpu_ctx_create(device, C.byref(h))
         if rc != 0:
@@ -103,6 +111,19 @@ class Context:
     def generate(self, gen, rays, n):
         self._check(self.lib.rbgpu_generate_rays(self.h, C.byref(gen), C.byref(rays), C.c_int64(n)),
                     "rbgpu_generate_rays")
+
+    def shoot(self, desc, rays, n):
+        """rbgpu_shoot_rays: the ARayShooter pattern `desc` into device rays."""
+        self._check(self.lib.rbgpu_shoot_rays(self.h, C.byref(desc), C.byref(rays), C.c_int64(n)),
+                    "rbgpu_shoot_rays")
+
+    def bunches_to_rays(self, bunches, n_bunches, opts, rays=None, capacity=0):
+        """rbgpu_bunches_to_rays: returns the ray count (rays=None: count only)."""
+        n = C.c_int64(0)
+        self._check(self.lib.rbgpu_bunches_to_rays(self.h, C.byref(bunches), C.c_int64(n_bunches), C.byref(opts),
+                                                   C.byref(rays) if rays is not None else None,
+                                                   C.c_int64(capacity), C.byref(n)), "rbgpu_bunches_to_rays")
+        return int(n.value)
 
     def adjoint(self, scene, rays, n, opts, adj, grad_ptr):
         self._check(self.lib.rbgpu_adjoint(self.h, scene.h, C.byref(opts), C.byref(rays), C.c_int64(n), C.byref(adj),

@@ -401,6 +401,199 @@ __global__ void __launch_bounds__(256) k_generate(rb_raygen_desc g, RayIO io, in
   if (io.last_node) io.last_node[i] = -1;
 }
 
+// ---------------------------------------------------------------- ray sources
+// ARayShooter (src/ARayShooter.cxx:122-459) on the device: lane i finds its
+// place in the pattern from i alone, places it with the rotation and then the
+// translation (TGeoRotation / TGeoTranslation::LocalToMaster), and normalises
+// the direction as ARay::SetDirection does (src/ARay.cxx:217-223). FMA
+// contraction is off so every position rounds as the written expression.
+constexpr int kSourceBlock = 256;
+
+__device__ __forceinline__ void rot3(const double* R, const double* l, double* m) {
+#pragma clang fp contract(off)
+  for (int r = 0; r < 3; r++) m[r] = l[0] * R[3 * r] + l[1] * R[3 * r + 1] + l[2] * R[3 * r + 2];
+}
+
+__device__ __forceinline__ void store_ray(const RayIO& io, int64_t i, const double* p, double t, const double* d,
+                                          double lam) {
+#pragma clang fp contract(off)
+  const double mag = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  double u0 = 0, u1 = 0, u2 = 0;  // SetDirection leaves a zero vector unset: TVector3's (0, 0, 0)
+  if (mag > 0) {
+    u0 = d[0] / mag;
+    u1 = d[1] / mag;
+    u2 = d[2] / mag;
+  }
+  io.x[i] = p[0];
+  io.y[i] = p[1];
+  io.z[i] = p[2];
+  io.t[i] = t;
+  io.dx[i] = u0;
+  io.dy[i] = u1;
+  io.dz[i] = u2;
+  const_cast<double*>(io.lambda)[i] = lam;
+  io.status[i] = RB_RUN;
+  io.npoints[i] = 1;
+  if (io.last_node) io.last_node[i] = -1;
+}
+
+__global__ void __launch_bounds__(kSourceBlock) k_shoot(rb_shooter_desc g, RayIO io, int64_t n) {
+#pragma clang fp contract(off)
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t id = g.id_base + uint64_t(i);
+    double loc[3] = {0., 0., 0.};        // the pattern point, local frame
+    double dl[3] = {g.dir[0], g.dir[1], g.dir[2]};
+    bool place = true;                   // false: the ray starts at the translation itself
+    bool turn = true;                    // the direction is rotated
+    bool cone = false;                   // RandomCone: direction = goal - start
+    switch (g.kind) {
+      case RB_SHOOT_RECTANGLE: {  // :426-431: i over nx outer, j over ny inner
+        const int64_t a = i / g.ny, b = i - a * g.ny;
+        const double deltax = g.nx == 1 ? g.a / 2 : g.a / (g.nx - 1);
+        const double deltay = g.ny == 1 ? g.b / 2 : g.b / (g.ny - 1);
+        loc[0] = double(a) * deltax - g.a / 2;
+        loc[1] = double(b) * deltay - g.b / 2;
+        break;
+      }
+      case RB_SHOOT_CIRCLE: {  // :154-177: the centre, then ring k (0-based) of nphi (k + 1) points
+        if (i == 0) {
+          place = false;
+          break;
+        }
+        const int64_t m = i - 1, nphi = g.ny;
+        const int64_t q = m / nphi;  // ring k starts at nphi k (k + 1) / 2: the largest k with k (k + 1) / 2 <= q
+        int64_t k = int64_t((sqrt(8.0 * double(q) + 1.0) - 1.0) * 0.5);
+        while ((k + 1) * (k + 2) / 2 <= q) k++;
+        while (k > 0 && k * (k + 1) / 2 > q) k--;
+        const int64_t j = m - nphi * (k * (k + 1) / 2);
+        const double r = g.a * double(k + 1) / g.nx;
+        const double phi = 2 * kPi / double(nphi) / double(k + 1) * double(j);
+        loc[0] = r * cos(phi);
+        loc[1] = r * sin(phi);
+        break;
+      }
+      case RB_SHOOT_RANDOM_CIRCLE:
+      case RB_SHOOT_RANDOM_CONE: {  // uniform in the disk of radius a (at z = b for the cone)
+        const double rho = g.a * sqrt(uniform01(g.seed, id, 0u, 2u));
+        const double phi = 2 * kPi * uniform01(g.seed, id, 1u, 2u);
+        loc[0] = rho * cos(phi);
+        loc[1] = rho * sin(phi);
+        if (g.kind == RB_SHOOT_RANDOM_CONE) {
+          loc[2] = g.b;
+          cone = true;
+        }
+        break;
+      }
+      case RB_SHOOT_RANDOM_RECTANGLE: {  // TRandom::Uniform(x1, x2) = x1 + (x2 - x1) U
+        const double lo_x = -g.a / 2., lo_y = -g.b / 2.;
+        loc[0] = lo_x + (g.a / 2. - lo_x) * uniform01(g.seed, id, 0u, 2u);
+        loc[1] = lo_y + (g.b / 2. - lo_y) * uniform01(g.seed, id, 1u, 2u);
+        break;
+      }
+      case RB_SHOOT_RANDOM_SPHERE: {  // isotropic; neither position nor direction is rotated
+        const double cz = 2 * uniform01(g.seed, id, 0u, 2u) - 1;
+        const double phi = 2 * kPi * uniform01(g.seed, id, 1u, 2u);
+        const double s = sqrt(fmax(0.0, 1 - cz * cz));
+        dl[0] = s * cos(phi);
+        dl[1] = s * sin(phi);
+        dl[2] = cz;
+        place = false;
+        turn = false;
+        break;
+      }
+      default: {  // RB_SHOOT_RANDOM_SPHERICAL_CONE, :365-371
+        const double c = cos(g.a * (kPi / 180.0));
+        const double th = acos(c + (1 - c) * uniform01(g.seed, id, 0u, 2u));
+        const double phi = 0 + (2 * kPi - 0) * uniform01(g.seed, id, 1u, 2u);
+        dl[0] = sin(th) * cos(phi);
+        dl[1] = sin(th) * sin(phi);
+        dl[2] = cos(th);
+        place = false;
+        break;
+      }
+    }
+    double p[3], d[3];
+    const double origin[3] = {0. + g.tr[0], 0. + g.tr[1], 0. + g.tr[2]};
+    if (place) {
+      rot3(g.rot, loc, p);
+      for (int k = 0; k < 3; k++) p[k] = p[k] + g.tr[k];
+    } else {
+      for (int k = 0; k < 3; k++) p[k] = origin[k];
+    }
+    if (cone) {  // :268-281: start at the translated origin, aim at the placed goal
+      for (int k = 0; k < 3; k++) d[k] = p[k] - origin[k];
+      for (int k = 0; k < 3; k++) p[k] = origin[k];
+    } else if (turn) {
+      rot3(g.rot, dl, d);
+    } else {
+      for (int k = 0; k < 3; k++) d[k] = dl[k];
+    }
+    store_ray(io, i, p, 0.0, d, g.lambda);
+  }
+}
+
+// CORSIKA bunches (src/ACorsikaIACTFile.cxx:71-131): count the rays of each
+// bunch, scan, then one lane per output ray finds its bunch by binary search
+// over the inclusive scan, so the writes of neighbouring lanes are adjacent.
+struct BunchIn {
+  const int32_t *tel, *arr;
+  const float *x, *y, *time, *cx, *cy, *cz, *lambda, *photons;
+};
+struct BunchParams {
+  double z, tel_z, index, lambda_min, lambda_max;
+  uint64_t seed, id_base;
+  int32_t tel, arr;
+};
+constexpr float kMaxBunchPhotons = 16777216.0f;  // 2^24: float(j) < photons counts exactly up to here
+
+__global__ void __launch_bounds__(kSourceBlock) k_bunch_count(BunchIn B, int64_t nb, int32_t tel, int32_t arr,
+                                                              int64_t* cnt, int32_t* bad) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nb; i += stride) {
+    int64_t c = 0;
+    if (B.tel[i] == tel && B.arr[i] == arr) {
+      const float p = B.photons[i];
+      if (p > kMaxBunchPhotons)
+        atomicOr(bad, 1);
+      else if (p > 0.f)  // #{j >= 0 : float(j) < p}; NaN and p <= 0 give none
+        c = int64_t(ceilf(p));
+    }
+    cnt[i] = c;
+  }
+}
+
+__global__ void __launch_bounds__(kSourceBlock) k_bunch_expand(BunchIn B, const int64_t* ends, int64_t nb,
+                                                               int64_t total, BunchParams Q, RayIO io) {
+#pragma clang fp contract(off)
+  const double m = 1e2 * 1.0, cm = 1.0, nm = 1e-9 * m, ns = 1e-9 * 1.0;  // AOpticsManager units
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < total; r += stride) {
+    int64_t lo = 0, hi = nb - 1;  // the first bunch whose inclusive end exceeds r
+    while (lo < hi) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      if (ends[mid] > r)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    const float cx = B.cx[lo], cy = B.cy[lo], cz = B.cz[lo], lambda = B.lambda[lo];
+    const double airmass = -1. / cz;
+    const double tel_dist = (Q.z - Q.tel_z * cm) * airmass;
+    const double speed = kC / Q.index;
+    double p[3], d[3] = {cx, cy, cz};
+    p[0] = B.x[lo] * cm - tel_dist * cx;
+    p[1] = B.y[lo] * cm - tel_dist * cy;
+    p[2] = Q.z;
+    const double pt = B.time[lo] * ns - tel_dist / speed;
+    double lam = lambda;
+    if (lambda == 0)
+      lam = 1. / (1. / Q.lambda_min -
+                  uniform01(Q.seed, Q.id_base + uint64_t(r), 0u, 3u) * (1. / Q.lambda_min - 1. / Q.lambda_max));
+    store_ray(io, r, p, pt, d, lam * nm);
+  }
+}
+
 // ---------------------------------------------------------------- spot analysis
 constexpr int kHistBlock = 256;
 constexpr int kSearchBlock = 1024;
@@ -628,7 +821,10 @@ int rbgpu_abi_sizes(int32_t* out) {
   out[9] = sizeof(rb_adjoint_desc);
   out[10] = sizeof(rb_hist2d_desc);
   out[11] = sizeof(rb_border_desc);
-  return 12;
+  out[12] = sizeof(rb_shooter_desc);
+  out[13] = sizeof(rb_bunches);
+  out[14] = sizeof(rb_bunch_opts);
+  return 15;
 }
 
 int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
@@ -818,6 +1014,165 @@ int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* g, rb_rays* rays, int
   hipLaunchKernelGGL(k_generate, dim3(unsigned(grid)), dim3(block), 0, ctx->stream, *g, io, n);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// Patterns above this many rays are refused (int64 index arithmetic in k_shoot
+// stays exact far beyond it).
+constexpr int64_t kMaxShooterRays = int64_t(1) << 40;
+
+int64_t rbgpu_shooter_count(const rb_shooter_desc* g, int64_t n) {
+  if (!g) return -1;
+  int64_t c;
+  switch (g->kind) {
+    case RB_SHOOT_RECTANGLE:  // :407-409
+      c = (g->a < 0 || g->b < 0 || g->nx < 1 || g->ny < 1) ? 0 : int64_t(g->nx) * g->ny;
+      break;
+    case RB_SHOOT_CIRCLE: {  // :128-130; ring i holds nphi (i + 1) points
+      if (g->a < 0 || g->nx < 1 || g->ny < 1) return 0;
+      const double est = double(g->ny) * double(g->nx) * (double(g->nx) + 1) / 2;
+      if (est > double(kMaxShooterRays)) return -1;
+      c = 1 + int64_t(g->ny) * (int64_t(g->nx) * (int64_t(g->nx) + 1) / 2);
+      break;
+    }
+    case RB_SHOOT_RANDOM_CIRCLE: c = g->a < 0 ? 0 : std::max<int64_t>(n, 0); break;        // :190-192
+    case RB_SHOOT_RANDOM_RECTANGLE: c = (g->a < 0 || g->b < 0 || n < 1) ? 0 : n; break;   // :294-296
+    case RB_SHOOT_RANDOM_CONE:
+    case RB_SHOOT_RANDOM_SPHERE:
+    case RB_SHOOT_RANDOM_SPHERICAL_CONE: c = std::max<int64_t>(n, 0); break;
+    default: return -1;
+  }
+  return c > kMaxShooterRays ? -1 : c;
+}
+
+static RayIO source_io(const rb_rays* rays) {
+  RayIO io{};
+  io.x = rays->x; io.y = rays->y; io.z = rays->z; io.t = rays->t;
+  io.dx = rays->dx; io.dy = rays->dy; io.dz = rays->dz; io.lambda = rays->lambda;
+  io.status = rays->status; io.npoints = rays->npoints; io.last_node = rays->last_node;
+  return io;
+}
+
+static bool source_arrays_present(const rb_rays* r) {
+  return r->x && r->y && r->z && r->t && r->dx && r->dy && r->dz && r->lambda && r->status && r->npoints;
+}
+
+int rbgpu_shoot_rays(rb_ctx* ctx, const rb_shooter_desc* g, rb_rays* rays, int64_t n) {
+  if (!ctx) return -1;
+  if (!g || !rays || n < 0) return fail(ctx, -1, "rbgpu_shoot_rays: null argument or negative n");
+  const int64_t want = rbgpu_shooter_count(g, n);
+  if (want < 0) return fail(ctx, -1, "rbgpu_shoot_rays: bad kind or a pattern above 2^40 rays");
+  if (want != n)
+    return fail(ctx, -1, "rbgpu_shoot_rays: n = " + std::to_string(n) + " but the pattern has " +
+                             std::to_string(want) + " rays");
+  ctx->last_ms = ctx->kernel_ms = 0;
+  ctx->launches = 0;
+  if (n == 0) return 0;
+  if (!source_arrays_present(rays)) return fail(ctx, -1, "rbgpu_shoot_rays: missing ray array");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!rays_accessible(rays, false)) return fail(ctx, -1, "rbgpu_shoot_rays: a ray array is not GPU memory");
+  const RayIO io = source_io(rays);
+  const int64_t grid = std::min<int64_t>((n + kSourceBlock - 1) / kSourceBlock, int64_t(ctx->cus) * 64);
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  hipLaunchKernelGGL(k_shoot, dim3(unsigned(grid)), dim3(kSourceBlock), 0, ctx->stream, *g, io, n);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ctx->kernel_ms = ms;
+  ctx->launches = 1;
+  return 0;
+}
+
+int rbgpu_bunches_to_rays(rb_ctx* ctx, const rb_bunches* b, int64_t nb, const rb_bunch_opts* o, rb_rays* rays,
+                          int64_t capacity, int64_t* n_rays) {
+  if (!ctx) return -1;
+  if (!b || !o || !n_rays || nb < 0 || capacity < 0)
+    return fail(ctx, -1, "rbgpu_bunches_to_rays: null argument or negative size");
+  *n_rays = 0;
+  if (!(o->refractive_index > 0) || !(o->lambda_min > 0) || !(o->lambda_max > 0))
+    return fail(ctx, -1, "rbgpu_bunches_to_rays: refractive index and wavelength bounds must be positive");
+  ctx->last_ms = ctx->kernel_ms = 0;
+  ctx->launches = 0;
+  if (nb == 0) return 0;
+  if (!b->tel_no || !b->array_no || !b->x || !b->y || !b->time || !b->cx || !b->cy || !b->cz || !b->lambda ||
+      !b->photons)
+    return fail(ctx, -1, "rbgpu_bunches_to_rays: missing bunch array");
+  if (rays && !source_arrays_present(rays)) return fail(ctx, -1, "rbgpu_bunches_to_rays: missing ray array");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (rays && !rays_accessible(rays, false))
+    return fail(ctx, -1, "rbgpu_bunches_to_rays: a ray array is not GPU memory");
+  DevRays dev;
+  BunchIn B{b->tel_no, b->array_no, b->x, b->y, b->time, b->cx, b->cy, b->cz, b->lambda, b->photons};
+  if (!o->device_ptrs) {
+    int32_t *tel, *arr;
+    HIPCHK(ctx, dev.alloc(&tel, nb));
+    HIPCHK(ctx, dev.alloc(&arr, nb));
+    const size_t ib = size_t(nb) * sizeof(int32_t), fb = size_t(nb) * sizeof(float);
+    HIPCHK(ctx, hipMemcpyAsync(tel, b->tel_no, ib, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(arr, b->array_no, ib, hipMemcpyHostToDevice, ctx->stream));
+    const float* src[8] = {b->x, b->y, b->time, b->cx, b->cy, b->cz, b->lambda, b->photons};
+    float* dst[8];
+    for (int k = 0; k < 8; k++) {
+      HIPCHK(ctx, dev.alloc(&dst[k], nb));
+      HIPCHK(ctx, hipMemcpyAsync(dst[k], src[k], fb, hipMemcpyHostToDevice, ctx->stream));
+    }
+    B = BunchIn{tel, arr, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7]};
+  } else {
+    const void* ptrs[10] = {B.tel, B.arr, B.x, B.y, B.time, B.cx, B.cy, B.cz, B.lambda, B.photons};
+    for (const void* p : ptrs)
+      if (!gpu_accessible(p)) return fail(ctx, -1, "rbgpu_bunches_to_rays: device_ptrs set but a bunch array is not GPU memory");
+  }
+  int64_t *cnt, *ends;
+  int32_t* bad;
+  HIPCHK(ctx, dev.alloc(&cnt, nb));
+  HIPCHK(ctx, dev.alloc(&ends, nb));
+  HIPCHK(ctx, dev.alloc(&bad, 1));
+  size_t scan_bytes = 0;
+  HIPCHK(ctx, rocprim::inclusive_scan(nullptr, scan_bytes, cnt, ends, size_t(nb), rocprim::plus<int64_t>(),
+                                      ctx->stream));
+  char* scan_tmp;
+  HIPCHK(ctx, dev.alloc(&scan_tmp, int64_t(std::max<size_t>(scan_bytes, 1))));
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(bad, 0, sizeof(int32_t), ctx->stream));
+  const int64_t cgrid = std::min<int64_t>((nb + kSourceBlock - 1) / kSourceBlock, int64_t(ctx->cus) * 16);
+  hipLaunchKernelGGL(k_bunch_count, dim3(unsigned(cgrid)), dim3(kSourceBlock), 0, ctx->stream, B, nb, o->tel_no,
+                     o->array_no, cnt, bad);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, rocprim::inclusive_scan(scan_tmp, scan_bytes, cnt, ends, size_t(nb), rocprim::plus<int64_t>(),
+                                      ctx->stream));
+  int64_t total = 0;
+  int32_t hbad = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&total, ends + (nb - 1), sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&hbad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (hbad) return fail(ctx, -1, "rbgpu_bunches_to_rays: a selected bunch holds more than 2^24 photons");
+  *n_rays = total;
+  ctx->launches = 2;
+  if (!rays) return 0;
+  if (total > capacity)
+    return fail(ctx, -1, "rbgpu_bunches_to_rays: " + std::to_string(total) + " rays exceed the capacity " +
+                             std::to_string(capacity));
+  if (total > 0) {
+    BunchParams Q{o->z, o->tel_z, o->refractive_index, o->lambda_min, o->lambda_max, o->seed, o->id_base,
+                  o->tel_no, o->array_no};
+    const RayIO io = source_io(rays);
+    const int64_t grid = std::min<int64_t>((total + kSourceBlock - 1) / kSourceBlock, int64_t(ctx->cus) * 64);
+    HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+    hipLaunchKernelGGL(k_bunch_expand, dim3(unsigned(grid)), dim3(kSourceBlock), 0, ctx->stream, B, ends, nb,
+                       total, Q, io);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+    ctx->launches = 3;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0, kms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (total > 0) HIPCHK(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
+  ctx->last_ms = ms;
+  ctx->kernel_ms = kms;
   return 0;
 }
 

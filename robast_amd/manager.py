@@ -120,6 +120,11 @@ class AOpticsManager:
         if isinstance(obj, ARayArray):
             self._trace_array(obj, record=self.record_path)
             return
+        from .sources import DeviceRays
+        if isinstance(obj, DeviceRays):  # rays already in HBM: traced in place
+            if obj.n:
+                self.context().trace(self.gpu_scene(), obj.rays(), obj.n, self.trace_opts(device_ptrs=1))
+            return
         raise TypeError(f"TraceNonSequential: unsupported argument {type(obj).__name__}")
 
     def _trace_array(self, arr, record):

@@ -3,7 +3,7 @@ as structure-of-arrays so 1e7-ray batches move to and from HBM in bulk.
 
 ARay (include/ARay.h:24-68): points (x, y, z, t), unit direction, lambda,
 status, node history. ARayArray (src/ARayArray.cxx:41-73): rays binned by
-status in input order. ARayShooter grids restate src/ARayShooter.cxx:401-459.
+status in input order. ARayShooter restates src/ARayShooter.cxx:122-459.
 """
 import math
 
@@ -220,38 +220,181 @@ class ARayArray:
 
 
 class ARayShooter:
-    """Grid sources of src/ARayShooter.cxx:401-459."""
+    """The ray sources of src/ARayShooter.cxx:122-459, with the reference's
+    signatures (rot, tr and v may be None or 0 for the reference's null
+    pointers). Without `ctx` each returns an ARayArray built on the host;
+    with `ctx` (a librbgpu Context) the rays are written into HBM by
+    rbgpu_shoot_rays and a DeviceRays batch is returned, ready for
+    AOpticsManager.TraceNonSequential. Random patterns draw from numpy's
+    generator seeded with `seed` on the host and from Philox streams keyed by
+    (seed, ray index) on the device; the reference draws from gRandom, so the
+    distributions agree and the individual rays do not."""
 
+    # ---- deterministic patterns
     @staticmethod
-    def Rectangle(lam, dx, dy, nx, ny, rot=None, tr=None, v=None):
+    def Rectangle(lam, dx, dy, nx, ny, rot=None, tr=None, v=None, ctx=None):
+        """:401-452: nx x ny grid, x_i = i dx / (nx - 1) - dx / 2 (dx / 2 steps for n = 1)."""
+        if ctx is not None:
+            return _device(ctx, A.RB_SHOOT_RECTANGLE, lam, dx, dy, nx, ny, rot, tr, v)
         if dx < 0 or dy < 0 or nx < 1 or ny < 1:
             return ARayArray(0)
-        d = np.array([0.0, 0.0, 1.0]) if v is None else np.asarray(v, dtype=float)
-        nd = rot.LocalToMasterVect(d) if rot is not None else d
         deltax = dx / 2 if nx == 1 else dx / (nx - 1)
         deltay = dy / 2 if ny == 1 else dy / (ny - 1)
         i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
         pts = np.stack([i.ravel() * deltax - dx / 2, j.ravel() * deltay - dy / 2, np.zeros(nx * ny)], axis=1)
-        if rot is not None:
-            pts = pts @ rot.rot.T
-        if tr is not None:
-            pts = pts @ tr.rot.T + tr.tr
-        n = len(pts)
-        return ARayArray.from_arrays(pts[:, 0], pts[:, 1], pts[:, 2], np.full(n, nd[0]), np.full(n, nd[1]),
-                                     np.full(n, nd[2]), np.full(n, float(lam)))
+        return _host(lam, _place(pts, rot, tr), _turn(_vector(v), rot))
 
     @staticmethod
-    def Square(lam, d, n, rot=None, tr=None, v=None):
-        return ARayShooter.Rectangle(lam, d, d, n, n, rot, tr, v)
+    def Square(lam, d, n, rot=None, tr=None, v=None, ctx=None):
+        """:455-459."""
+        return ARayShooter.Rectangle(lam, d, d, n, n, rot, tr, v, ctx=ctx)
 
     @staticmethod
-    def RandomSphere(lam, n, tr=None, seed=0):
-        """n rays from the origin (or tr) with isotropic directions
-        (src/ARayShooter.cxx RandomSphere; numpy's seeded generator replaces gRandom)."""
+    def Circle(lam, rmax, nr, nphi, rot=None, tr=None, v=None, ctx=None):
+        """:122-183: a ray at the centre, then rings r = rmax (i + 1) / nr of
+        nphi (i + 1) rays at phi = 2 pi / nphi / (i + 1) j."""
+        if ctx is not None:
+            return _device(ctx, A.RB_SHOOT_CIRCLE, lam, rmax, 0.0, nr, nphi, rot, tr, v)
+        if rmax < 0 or nr < 1 or nphi < 1:
+            return ARayArray(0)
+        xs, ys = [0.0], [0.0]
+        for i in range(nr):
+            r = rmax * (i + 1) / nr
+            j = np.arange(nphi * (i + 1), dtype=float)
+            phi = 2 * math.pi / nphi / (i + 1) * j
+            xs.append(r * np.cos(phi))
+            ys.append(r * np.sin(phi))
+        x, y = np.hstack(xs), np.hstack(ys)
+        pts = _place(np.stack([x, y, np.zeros(len(x))], axis=1), rot, tr)
+        pts[0] = _translation(tr)  # the centre is translated, not rotated (:150-157)
+        return _host(lam, pts, _turn(_vector(v), rot))
+
+    # ---- random patterns
+    @staticmethod
+    def RandomCircle(lam, rmax, n, rot=None, tr=None, v=None, seed=0, ctx=None):
+        """:186-237: uniform in the disk of radius rmax."""
+        if ctx is not None:
+            return _device(ctx, A.RB_SHOOT_RANDOM_CIRCLE, lam, rmax, 0.0, 0, 0, rot, tr, v, seed, n)
+        if rmax < 0 or n < 1:
+            return ARayArray(0)
+        rng = np.random.default_rng(seed)
+        rho, phi = rmax * np.sqrt(rng.uniform(0, 1, n)), rng.uniform(0, 2 * math.pi, n)
+        pts = np.stack([rho * np.cos(phi), rho * np.sin(phi), np.zeros(n)], axis=1)
+        return _host(lam, _place(pts, rot, tr), _turn(_vector(v), rot))
+
+    @staticmethod
+    def RandomCone(lam, r, d, n, rot=None, tr=None, seed=0, ctx=None):
+        """:240-285: from the (translated) origin toward a point uniform in the
+        disk of radius r at z = d (rotated, translated)."""
+        if ctx is not None:
+            return _device(ctx, A.RB_SHOOT_RANDOM_CONE, lam, r, d, 0, 0, rot, tr, None, seed, n)
+        if n < 1:
+            return ARayArray(0)
+        rng = np.random.default_rng(seed)
+        rho, phi = r * np.sqrt(rng.uniform(0, 1, n)), rng.uniform(0, 2 * math.pi, n)
+        goal = _place(np.stack([rho * np.cos(phi), rho * np.sin(phi), np.full(n, float(d))], axis=1), rot, tr)
+        start = np.tile(_translation(tr), (n, 1))
+        return _host(lam, start, goal - start)
+
+    @staticmethod
+    def RandomRectangle(lam, dx, dy, n, rot=None, tr=None, v=None, seed=0, ctx=None):
+        """:288-335: uniform in [-dx/2, dx/2] x [-dy/2, dy/2]."""
+        if ctx is not None:
+            return _device(ctx, A.RB_SHOOT_RANDOM_RECTANGLE, lam, dx, dy, 0, 0, rot, tr, v, seed, n)
+        if dx < 0 or dy < 0 or n < 1:
+            return ARayArray(0)
+        rng = np.random.default_rng(seed)
+        pts = np.stack([rng.uniform(-dx / 2, dx / 2, n), rng.uniform(-dy / 2, dy / 2, n), np.zeros(n)], axis=1)
+        return _host(lam, _place(pts, rot, tr), _turn(_vector(v), rot))
+
+    @staticmethod
+    def RandomSquare(lam, d, n, rot=None, tr=None, v=None, seed=0, ctx=None):
+        """:394-398."""
+        return ARayShooter.RandomRectangle(lam, d, d, n, rot, tr, v, seed=seed, ctx=ctx)
+
+    @staticmethod
+    def RandomSphere(lam, n, tr=None, seed=0, ctx=None):
+        """:338-356: isotropic directions from the origin (or tr)."""
+        if ctx is not None:
+            return _device(ctx, A.RB_SHOOT_RANDOM_SPHERE, lam, 0.0, 0.0, 0, 0, None, tr, None, seed, n)
+        if n < 1:
+            return ARayArray(0)
         rng = np.random.default_rng(seed)
         cos_t = rng.uniform(-1, 1, n)
         phi = rng.uniform(0, 2 * math.pi, n)
         sin_t = np.sqrt(1 - cos_t * cos_t)
-        o = np.zeros((n, 3)) if tr is None else np.tile(tr.tr, (n, 1))
-        return ARayArray.from_arrays(o[:, 0], o[:, 1], o[:, 2], sin_t * np.cos(phi), sin_t * np.sin(phi), cos_t,
-                                     np.full(n, float(lam)))
+        o = np.tile(_translation(tr), (n, 1))
+        return _host(lam, o, np.stack([sin_t * np.cos(phi), sin_t * np.sin(phi), cos_t], axis=1))
+
+    @staticmethod
+    def RandomSphericalCone(lam, n, theta, rot=None, tr=None, seed=0, ctx=None):
+        """:359-391: cos(angle to +z) uniform in [cos theta, 1] (theta in
+        degrees), then rotated; all from the origin (or tr)."""
+        if ctx is not None:
+            return _device(ctx, A.RB_SHOOT_RANDOM_SPHERICAL_CONE, lam, theta, 0.0, 0, 0, rot, tr, None, seed, n)
+        if n < 1:
+            return ARayArray(0)
+        rng = np.random.default_rng(seed)
+        th = np.arccos(rng.uniform(math.cos(theta * math.pi / 180.0), 1, n))
+        phi = rng.uniform(0, 2 * math.pi, n)
+        d = np.stack([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)], axis=1)
+        o = np.tile(_translation(tr), (n, 1))
+        return _host(lam, o, _turn(d, rot))
+
+
+def _rotation(rot):
+    if rot is None or (isinstance(rot, (int, float)) and rot == 0):
+        return None
+    return np.asarray(rot.rot, dtype=float)
+
+
+def _translation(tr):
+    if tr is None or (isinstance(tr, (int, float)) and tr == 0):
+        return np.zeros(3)
+    return np.asarray(tr.tr, dtype=float).reshape(3)
+
+
+def _vector(v):
+    if v is None or (isinstance(v, (int, float)) and v == 0):
+        return np.array([0.0, 0.0, 1.0])
+    if hasattr(v, "X"):
+        return np.array([v.X(), v.Y(), v.Z()], dtype=float)
+    return np.asarray(v, dtype=float).reshape(3)
+
+
+def _rot_rows(R, p):
+    """TGeoRotation::LocalToMaster row by row: m_i = l0 R_i0 + l1 R_i1 + l2 R_i2."""
+    return np.stack([p[..., 0] * R[i, 0] + p[..., 1] * R[i, 1] + p[..., 2] * R[i, 2] for i in range(3)], axis=-1)
+
+
+def _place(pts, rot, tr):
+    R = _rotation(rot)
+    if R is not None:
+        pts = _rot_rows(R, pts)
+    return pts + _translation(tr)
+
+
+def _turn(d, rot):
+    R = _rotation(rot)
+    return d if R is None else _rot_rows(R, d)
+
+
+def _host(lam, pts, d):
+    """new ARay(0, lam, x, y, z, 0, d) per point: SetDirection normalises a
+    non-zero vector and leaves a zero one (0, 0, 0) (src/ARay.cxx:28-36, 217-223)."""
+    n = len(pts)
+    d = np.broadcast_to(np.asarray(d, dtype=float), (n, 3))
+    mag = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+    u = np.zeros((n, 3))
+    ok = mag > 0
+    u[ok] = d[ok] / mag[ok, None]
+    a = ARayArray(n)
+    a.x[:], a.y[:], a.z[:] = pts[:, 0], pts[:, 1], pts[:, 2]
+    a.dx[:], a.dy[:], a.dz[:] = u[:, 0], u[:, 1], u[:, 2]
+    a.lambda_[:] = lam
+    return a
+
+
+def _device(ctx, kind, lam, a, b, nx, ny, rot, tr, v, seed=0, n=0):
+    from . import sources
+    return sources.shoot(ctx, sources.shooter_desc(kind, lam, a, b, nx, ny, rot, tr, v, seed), n)

@@ -30,8 +30,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_binding():
     lib = _lib.load()
     out = (C.c_int32 * 16)()
-    assert lib.rbgpu_abi_sizes(out) == 12
-    sizes = list(out)[:12]
+    assert lib.rbgpu_abi_sizes(out) == 15
+    sizes = list(out)[:15]
     assert sizes[0] == 96                              # rb_matrix
     assert sizes[1] == A.SHAPE_DTYPE.itemsize
     assert sizes[2] == A.VOLUME_DTYPE.itemsize
@@ -45,6 +45,9 @@ def test_struct_sizes_match_binding():
     assert sizes[10] == C.sizeof(A.rb_hist2d_desc) == 48
     assert sizes[11] == A.BORDER_DTYPE.itemsize == 16
     assert C.sizeof(A.rb_trace_opts) == 40
+    assert sizes[12] == C.sizeof(A.rb_shooter_desc) == 176
+    assert sizes[13] == C.sizeof(A.rb_bunches) == 11 * 8
+    assert sizes[14] == C.sizeof(A.rb_bunch_opts) == 72
 
 
 def test_null_context_is_an_error_not_a_crash():
@@ -53,3 +56,6 @@ def test_null_context_is_an_error_not_a_crash():
     assert lib.rbgpu_scene_upload(None, None, None) == -1
     assert lib.rbgpu_last_error(None) == b"null context"
     assert lib.rbgpu_containment_radius(None, None, 0, None, 0.8, None, None, None) == -1
+    assert lib.rbgpu_shoot_rays(None, None, None, 0) == -1
+    assert lib.rbgpu_bunches_to_rays(None, None, 0, None, None, 0, None) == -1
+    assert lib.rbgpu_shooter_count(None, 0) == -1
