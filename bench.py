@@ -123,6 +123,56 @@ def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3)
     return out
 
 
+def sources_bench(ctx, torch, A, rays, n, reps=3):
+    """Device ray sources writing n rays into resident buffers: ARayShooter's
+    RandomCircle (HESS1.C's 7.5 m source plane) and the CORSIKA bunch
+    conversion over a synthetic bunch table of about n photons."""
+    import numpy as np
+
+    from robast_amd import sources as S
+    out_bytes = 8 * 8 + 3 * 4                # x y z t dx dy dz lambda, status npoints last_node
+    g = S.shooter_desc(A.RB_SHOOT_RANDOM_CIRCLE, 400e-7, 750.0, 0.0, 0, 0, None, None, (0.0, 0.0, -1.0), seed=7)
+    shoot_ms = []
+    for _ in range(reps):
+        ctx.shoot(g, rays, n)
+        shoot_ms.append(ctx.last_kernel_ms())
+    rng = np.random.default_rng(5)
+    nb = n // 5                              # photons ~ U(0.5, 7.5): 4.5 rays per bunch on average
+    cx = rng.uniform(-0.05, 0.05, nb).astype(np.float32)
+    cy = rng.uniform(-0.05, 0.05, nb).astype(np.float32)
+    host = S.CorsikaBunches(np.zeros(nb), np.zeros(nb), rng.uniform(-1000, 1000, nb), rng.uniform(-1000, 1000, nb),
+                            np.full(nb, 1e6), rng.uniform(0, 50, nb), cx, cy, np.zeros(nb),
+                            rng.uniform(0.5, 7.5, nb))
+    keep = {f: torch.from_numpy(getattr(host, f)).cuda() for f in A.BUNCH_FIELDS_I + A.BUNCH_FIELDS_F}
+    d = A.rb_bunches()
+    for f, t in keep.items():
+        setattr(d, f, t.data_ptr())
+    o = S.bunch_opts(0, 0, 3000.0, 0.0, 1.0, 300.0, 700.0, seed=9, device_ptrs=True)
+    total = ctx.bunches_to_rays(d, nb, o)
+    bunch_rays = min(total, n)
+    sel = total if total <= n else None
+    conv_ms, expand_ms = [], []
+    if sel is not None:
+        for _ in range(reps):
+            ctx.bunches_to_rays(d, nb, o, rays, n)
+            conv_ms.append(ctx.last_trace_ms())
+            expand_ms.append(ctx.last_kernel_ms())
+    ks = min(shoot_ms)
+    res = {"shoot_random_circle": {"rays": n, "kernel_ms": round(ks, 3),
+                                   "mrays_s": round(n / (ks * 1e-3) / 1e6, 1),
+                                   "write_gb_s": round(n * out_bytes / (ks * 1e-3) / 1e9, 1),
+                                   "bytes_per_ray": out_bytes}}
+    if conv_ms:
+        ke = min(expand_ms)
+        res["corsika_bunches"] = {"bunches": nb, "rays": bunch_rays, "device_ms": round(min(conv_ms), 3),
+                                  "expand_kernel_ms": round(ke, 3),
+                                  "mrays_s": round(bunch_rays / (min(conv_ms) * 1e-3) / 1e6, 1),
+                                  "expand_write_gb_s": round(bunch_rays * out_bytes / (ke * 1e-3) / 1e9, 1),
+                                  "workload": "photons ~ U(0.5, 7.5) per bunch, wavelengths drawn (lambda = 0)"}
+    del keep
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +267,7 @@ def main():
             "fill_and_compact_ms": round(min(s - k for s, k in zip(spot_ms, search_ms)), 3),
             "workload": "AGeoUtil::ContainmentRadius(0.80) of the focused rays of the batch (all field angles), "
                         "TH2D 1000 x 1000 bins over +-120 cm"}
+    sources = sources_bench(ctx, torch, A, rays, N)
     del src, work, res_o, rays
     torch.cuda.empty_cache()
 
@@ -275,6 +326,7 @@ def main():
         "mixed_vs_fp64": {"status_flips": flips, "focused_hit_rms_um": round(rms_um, 6)},
         "adjoint": adjoint,
         "spot": spot,
+        "sources": sources,
         "configs": per_config,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,

@@ -405,14 +405,15 @@ enum { RB_ADJ_CENTROID = 0, RB_ADJ_MOMENTS = 1 };
 int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const rb_rays* rays, int64_t n,
                   const rb_adjoint_desc* adj, double* grad);
 
-/* Milliseconds the last rbgpu_trace (or rbgpu_adjoint, or
- * rbgpu_containment_radius) spent on the device, ordering (histogram fill)
- * included, timed with HIP events on the context's stream, and the number of
- * kernel launches it made. */
+/* Milliseconds the last rbgpu_trace (or rbgpu_adjoint,
+ * rbgpu_containment_radius, rbgpu_shoot_rays, rbgpu_bunches_to_rays) spent on
+ * the device, ordering (histogram fill, bunch counting) included, timed with
+ * HIP events on the context's stream, and the number of kernel launches it
+ * made. */
 double rbgpu_last_trace_ms(const rb_ctx* ctx);
-/* Milliseconds of the last call's trace (adjoint, containment search) kernel
- * alone, without the ordering (fill) kernels. No counterpart in the
- * reference (measurement). */
+/* Milliseconds of the last call's trace (adjoint, containment search, ray
+ * source, bunch expansion) kernels alone, without the ordering (fill,
+ * counting) kernels. No counterpart in the reference (measurement). */
 double rbgpu_last_kernel_ms(const rb_ctx* ctx);
 int rbgpu_last_trace_launches(const rb_ctx* ctx);
 

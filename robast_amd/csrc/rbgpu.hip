@@ -563,34 +563,71 @@ __global__ void __launch_bounds__(kSourceBlock) k_bunch_count(BunchIn B, int64_t
   }
 }
 
-__global__ void __launch_bounds__(kSourceBlock) k_bunch_expand(BunchIn B, const int64_t* ends, int64_t nb,
-                                                               int64_t total, BunchParams Q, RayIO io) {
-#pragma clang fp contract(off)
-  const double m = 1e2 * 1.0, cm = 1.0, nm = 1e-9 * m, ns = 1e-9 * 1.0;  // AOpticsManager units
+// Tile k of the output holds rays [k T, (k + 1) T). tile_b0[k] receives the
+// bunch of ray k T, and tile_b0[ntiles] the bunch of the last ray: each bunch
+// with rays writes the entries of the tile starts it covers, so every entry is
+// written once and no lane searches the whole table.
+constexpr int kExpandTile = 2 * kSourceBlock;  // rays per tile
+constexpr int kExpandLds = 2048;               // bunch ends a tile stages in LDS (16 KB)
+
+__global__ void __launch_bounds__(kSourceBlock) k_bunch_tiles(const int64_t* cnt, const int64_t* ends, int64_t nb,
+                                                              int64_t total, int64_t* tile_b0) {
+  const int64_t ntiles = (total + kExpandTile - 1) / kExpandTile;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < total; r += stride) {
-    int64_t lo = 0, hi = nb - 1;  // the first bunch whose inclusive end exceeds r
-    while (lo < hi) {
-      const int64_t mid = lo + (hi - lo) / 2;
-      if (ends[mid] > r)
-        hi = mid;
-      else
-        lo = mid + 1;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nb; i += stride) {
+    const int64_t c = cnt[i];
+    if (c == 0) continue;
+    const int64_t e = ends[i], s = e - c;
+    for (int64_t k = (s + kExpandTile - 1) / kExpandTile; k * kExpandTile < e; k++) tile_b0[k] = i;
+    if (s <= total - 1 && total - 1 < e) tile_b0[ntiles] = i;
+  }
+}
+
+// One block per tile: the tile's slice of the inclusive scan goes to LDS and
+// each lane finds its ray's bunch there, so neighbouring lanes write
+// neighbouring rays. A tile spanning more bunches than LDS holds (many
+// bunches of other telescopes between the selected ones) searches its slice
+// in global memory instead.
+__global__ void __launch_bounds__(kSourceBlock) k_bunch_expand(BunchIn B, const int64_t* ends,
+                                                               const int64_t* tile_b0, int64_t total, BunchParams Q,
+                                                               RayIO io) {
+#pragma clang fp contract(off)
+  __shared__ int64_t s_ends[kExpandLds];
+  const double m = 1e2 * 1.0, cm = 1.0, nm = 1e-9 * m, ns = 1e-9 * 1.0;  // AOpticsManager units
+  const int64_t ntiles = (total + kExpandTile - 1) / kExpandTile;
+  for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+    const int64_t t0 = k * kExpandTile, t1 = t0 + kExpandTile < total ? t0 + kExpandTile : total;
+    const int64_t b0 = tile_b0[k], nbt = tile_b0[k + 1] - b0 + 1;
+    const bool staged = nbt <= kExpandLds;
+    if (staged)
+      for (int64_t j = threadIdx.x; j < nbt; j += blockDim.x) s_ends[j] = ends[b0 + j];
+    __syncthreads();
+    for (int64_t r = t0 + threadIdx.x; r < t1; r += blockDim.x) {
+      int64_t lo = 0, hi = nbt - 1;  // the first bunch of the slice whose inclusive end exceeds r
+      while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if ((staged ? s_ends[mid] : ends[b0 + mid]) > r)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      const int64_t b = b0 + lo;
+      const float cx = B.cx[b], cy = B.cy[b], cz = B.cz[b], lambda = B.lambda[b];
+      const double airmass = -1. / cz;
+      const double tel_dist = (Q.z - Q.tel_z * cm) * airmass;
+      const double speed = kC / Q.index;
+      double p[3], d[3] = {cx, cy, cz};
+      p[0] = B.x[b] * cm - tel_dist * cx;
+      p[1] = B.y[b] * cm - tel_dist * cy;
+      p[2] = Q.z;
+      const double pt = B.time[b] * ns - tel_dist / speed;
+      double lam = lambda;
+      if (lambda == 0)
+        lam = 1. / (1. / Q.lambda_min -
+                    uniform01(Q.seed, Q.id_base + uint64_t(r), 0u, 3u) * (1. / Q.lambda_min - 1. / Q.lambda_max));
+      store_ray(io, r, p, pt, d, lam * nm);
     }
-    const float cx = B.cx[lo], cy = B.cy[lo], cz = B.cz[lo], lambda = B.lambda[lo];
-    const double airmass = -1. / cz;
-    const double tel_dist = (Q.z - Q.tel_z * cm) * airmass;
-    const double speed = kC / Q.index;
-    double p[3], d[3] = {cx, cy, cz};
-    p[0] = B.x[lo] * cm - tel_dist * cx;
-    p[1] = B.y[lo] * cm - tel_dist * cy;
-    p[2] = Q.z;
-    const double pt = B.time[lo] * ns - tel_dist / speed;
-    double lam = lambda;
-    if (lambda == 0)
-      lam = 1. / (1. / Q.lambda_min -
-                  uniform01(Q.seed, Q.id_base + uint64_t(r), 0u, 3u) * (1. / Q.lambda_min - 1. / Q.lambda_max));
-    store_ray(io, r, p, pt, d, lam * nm);
+    __syncthreads();
   }
 }
 
@@ -1158,13 +1195,19 @@ int rbgpu_bunches_to_rays(rb_ctx* ctx, const rb_bunches* b, int64_t nb, const rb
     BunchParams Q{o->z, o->tel_z, o->refractive_index, o->lambda_min, o->lambda_max, o->seed, o->id_base,
                   o->tel_no, o->array_no};
     const RayIO io = source_io(rays);
-    const int64_t grid = std::min<int64_t>((total + kSourceBlock - 1) / kSourceBlock, int64_t(ctx->cus) * 64);
+    const int64_t ntiles = (total + kExpandTile - 1) / kExpandTile;
+    int64_t* tile_b0;
+    HIPCHK(ctx, dev.alloc(&tile_b0, ntiles + 1));
     HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-    hipLaunchKernelGGL(k_bunch_expand, dim3(unsigned(grid)), dim3(kSourceBlock), 0, ctx->stream, B, ends, nb,
+    hipLaunchKernelGGL(k_bunch_tiles, dim3(unsigned(cgrid)), dim3(kSourceBlock), 0, ctx->stream, cnt, ends, nb, total,
+                       tile_b0);
+    HIPCHK(ctx, hipGetLastError());
+    const int64_t grid = std::min<int64_t>(ntiles, int64_t(ctx->cus) * 64);
+    hipLaunchKernelGGL(k_bunch_expand, dim3(unsigned(grid)), dim3(kSourceBlock), 0, ctx->stream, B, ends, tile_b0,
                        total, Q, io);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
-    ctx->launches = 3;
+    ctx->launches = 4;
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

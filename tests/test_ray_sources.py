@@ -428,6 +428,23 @@ def test_gpu_bunch_edge_cases(ctx):
     assert ctx.bunches_to_rays(b.desc(), b.n, opts(tel_no=9)) == 0
     with pytest.raises(RbError, match="positive"):
         ctx.bunches_to_rays(b.desc(), b.n, opts(refractive_index=0.0))
+    # a sparse selection: tiles of output rays span more bunches than LDS stages
+    sparse = bunch_table(300_000, seed=6, tels=tuple(range(100)), arrays=(2,))
+    o = opts()
+    n = ctx.bunches_to_rays(sparse.desc(), sparse.n, o)
+    ref = O.bunches_to_rays(sparse, o)
+    assert n == len(ref["x"]) > 5000
+    dr = S.DeviceRays(ctx, n)
+    ctx.bunches_to_rays(sparse.desc(), sparse.n, o, dr.rays(), n)
+    _match(_device_soa(dr), ref, exact=True)
+    # one bunch spanning many tiles
+    one = S.CorsikaBunches([1, 1, 1], [2, 2, 2], [1, 2, 3], [0, 0, 0], [0, 0, 0], [5, 6, 7], [0, 0.1, 0], [0, 0, 0.1],
+                           [0, 400, 0], [3.5, 5000.2, 700])
+    n = ctx.bunches_to_rays(one.desc(), one.n, o)
+    assert n == 4 + 5001 + 700
+    dr = S.DeviceRays(ctx, n)
+    ctx.bunches_to_rays(one.desc(), one.n, o, dr.rays(), n)
+    _match(_device_soa(dr), O.bunches_to_rays(one, o), exact=True)
     rays = S.GetRayArray(ctx, b, 1, 2, 3000.0, 1.0003, 120.0, 250.0, seed=0xC0)
     ref = O.bunches_to_rays(b, opts(lambda_max=100000.0))
     _match(_device_soa(rays), ref, exact=True)
