@@ -108,7 +108,24 @@ typedef struct {
   int32_t volume2;    /* component2's volume; -1 = the null node (leaving the top volume) */
   int32_t lambertian; /* EnableLambertian(kTRUE): diffuse reflection                 */
   double sigma;       /* SetGaussianRoughness(sigma) [rad]; 0 = smooth                 */
+  int32_t multilayer; /* SetMultilayer(): index into rb_scene_desc.grids; -1 = none    */
+  int32_t pad;
 } rb_border_desc;
+
+/* A multilayer coating (AMultilayer, include/AMultilayer.h) as the device
+ * reads it: its unpolarised reflectance and transmittance (CoherentTMMMixed)
+ * tabulated at the bin centres of an nx x ny grid over wavelength x [xlo,
+ * xhi) (cm) and incidence angle y [ylo, yhi) (rad), as
+ * AMultilayer::PreCalculateCoherentTMM fills its TH2Ds
+ * (include/AMultilayer.h:243-262), and read back with TH2::Interpolate.
+ * params[r_offset + (by - 1) nx + bx - 1] holds R of bin (bx, by), likewise
+ * T at t_offset. */
+typedef struct {
+  int32_t nx, ny;
+  double xlo, xhi;
+  double ylo, yhi;
+  int32_t r_offset, t_offset;
+} rb_grid2d_desc;
 
 /* One placed instance of a volume (a path in the TGeo node tree), with its
  * global matrix. Node 0 is the top volume. */
@@ -146,6 +163,8 @@ typedef struct {
   const double* table_xy;
   int32_t n_borders;
   const rb_border_desc* borders;
+  int32_t n_grids;
+  const rb_grid2d_desc* grids;
 } rb_scene_desc;
 
 /* Rays in structure-of-arrays form. On input each ray holds its last point
@@ -443,8 +462,8 @@ int rbgpu_containment_radius(rb_ctx* ctx, const rb_rays* rays, int64_t n, const 
 /* Writes sizeof() of the ABI structs (rb_matrix, rb_shape_desc,
  * rb_volume_desc, rb_node_desc, rb_table_desc, rb_scene_desc, rb_rays,
  * rb_trace_opts, rb_raygen_desc, rb_adjoint_desc, rb_hist2d_desc,
- * rb_border_desc, rb_shooter_desc, rb_bunches, rb_bunch_opts) into
- * out[0..14] for binding checks; returns 15.
+ * rb_border_desc, rb_shooter_desc, rb_bunches, rb_bunch_opts,
+ * rb_grid2d_desc) into out[0..15] for binding checks; returns 16.
  * No counterpart in the reference (binding self-check only). */
 int rbgpu_abi_sizes(int32_t* out);
 

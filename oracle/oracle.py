@@ -42,6 +42,8 @@ def lib():
         L.oracle_contains_batch.restype = C.c_int
         L.oracle_generate_rays.argtypes = [C.POINTER(A.rb_raygen_desc), C.POINTER(A.rb_rays), C.c_int64]
         L.oracle_generate_rays.restype = C.c_int
+        L.oracle_th2_interpolate.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+        L.oracle_th2_interpolate.restype = C.c_double
         L.oracle_shooter_count.argtypes = [C.POINTER(A.rb_shooter_desc), C.c_int64]
         L.oracle_shooter_count.restype = C.c_int64
         L.oracle_shoot_rays.argtypes = [C.POINTER(A.rb_shooter_desc), C.POINTER(A.rb_rays), C.c_int64]
@@ -133,6 +135,14 @@ def generate(gen, n):
     r = rays_struct(soa)
     lib().oracle_generate_rays(C.byref(gen), C.byref(r), n)
     return soa
+
+
+def th2_interpolate(grid, values, x, y):
+    """The oracle's TH2::Interpolate over values[ny, nx] (grid: a GRID_DTYPE record)."""
+    g = np.zeros(1, dtype=A.GRID_DTYPE)
+    g[0] = grid
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    return lib().oracle_th2_interpolate(g.ctypes.data, v.ctypes.data, float(x), float(y))
 
 
 def shooter_count(desc, n=0):

@@ -1580,6 +1580,41 @@ static V3 lambert_dir(V3 n, Rng& rng) {
 }
 
 // src/AOpticsManager.cxx:170-247
+// AMultilayer::CoherentTMMMixed once PreCalculateCoherentTMM has filled its
+// TH2Ds (include/AMultilayer.h:110-114, 243-262): TH2::Interpolate(lambda,
+// angle), restated from ROOT's documented behaviour -- bilinear between the
+// four nearest bin centres, a neighbour bin beyond the axis replaced by the
+// edge bin (so the outer half bins are flat along that axis), 0 outside the
+// axes. Written out step by step, independently of the device's form.
+static double th2_interpolate(const double* vals, const rb_grid2d_desc& g, double x, double y) {
+  const double wx = (g.xhi - g.xlo) / g.nx, wy = (g.yhi - g.ylo) / g.ny;
+  int bx, by;  // TAxis::FindFixBin
+  if (x < g.xlo) bx = 0; else if (!(x < g.xhi)) bx = g.nx + 1; else bx = 1 + int(g.nx * (x - g.xlo) / (g.xhi - g.xlo));
+  if (y < g.ylo) by = 0; else if (!(y < g.yhi)) by = g.ny + 1; else by = 1 + int(g.ny * (y - g.ylo) / (g.yhi - g.ylo));
+  if (bx > g.nx + 1) bx = g.nx + 1;
+  if (by > g.ny + 1) by = g.ny + 1;
+  if (bx < 1 || bx > g.nx || by < 1 || by > g.ny) return 0.0;  // "Cannot interpolate outside histogram domain"
+  // the quadrant of the bin holding (x, y) picks the lower-left of the four centres
+  const bool right = (g.xlo + bx * wx) - x <= wx / 2, upper = (g.ylo + by * wy) - y <= wy / 2;
+  const int ax = right ? bx : bx - 1, ay = upper ? by : by - 1;
+  const double x1 = g.xlo + (ax - 0.5) * wx, x2 = g.xlo + (ax + 0.5) * wx;
+  const double y1 = g.ylo + (ay - 0.5) * wy, y2 = g.ylo + (ay + 0.5) * wy;
+  auto clampx = [&](int b) { return std::min(std::max(b, 1), g.nx); };
+  auto clampy = [&](int b) { return std::min(std::max(b, 1), g.ny); };
+  auto at = [&](int i, int j) { return vals[int64_t(clampy(j) - 1) * g.nx + clampx(i) - 1]; };
+  const double q11 = at(ax, ay), q21 = at(ax + 1, ay), q12 = at(ax, ay + 1), q22 = at(ax + 1, ay + 1);
+  const double d = (x2 - x1) * (y2 - y1);
+  return q11 / d * (x2 - x) * (y2 - y) + q21 / d * (x - x1) * (y2 - y) + q12 / d * (x2 - x) * (y - y1) +
+         q22 / d * (x - x1) * (y - y1);
+}
+static void multilayer_rt(const rb_scene_desc* d, const rb_border_desc* bc, double lambda, double cos1, double& R,
+                          double& T) {
+  const rb_grid2d_desc& g = d->grids[bc->multilayer];
+  const double angle = std::acos(cos1);
+  R = th2_interpolate(d->params + g.r_offset, g, lambda, angle);
+  T = th2_interpolate(d->params + g.t_offset, g, lambda, angle);
+}
+
 static void do_reflection(double n1, Ray& ray, Nav& nv, int current, int next, const V3* normal, Rng& rng) {
   count_op(OP_REFLECT);
   const Scene& sc = *nv.sc;
@@ -1590,7 +1625,11 @@ static void do_reflection(double n1, Ray& ray, Nav& nv, int current, int next, c
   double cos1 = dot(d1, n);
   bool absorbed = false;
   if (kind_of(sc, next) == RB_KMIRROR) {
-    double ref = mirror_reflectance(sc.d, vol_of(sc, next), ray.lambda);
+    double ref, trans;
+    if (bc && bc->multilayer >= 0)  // src/AOpticsManager.cxx:194-198: the coating, unclamped
+      multilayer_rt(sc.d, bc, ray.lambda, cos1, ref, trans);
+    else
+      ref = mirror_reflectance(sc.d, vol_of(sc, next), ray.lambda);
     if (ref < rng.uniform()) {
       absorbed = true;
       ray.status = RB_ABSORB;
@@ -1605,17 +1644,31 @@ static void do_reflection(double n1, Ray& ray, Nav& nv, int current, int next, c
   ray.add_point(nv.p, t, next);
 }
 
-// src/AOpticsManager.cxx:52-167 (multilayer branch out of scope)
+// src/AOpticsManager.cxx:52-167
 static void do_fresnel(double n1, double n2, double k2, Ray& ray, Nav& nv, int current, int next, const Opts& o,
                        Rng& rng) {
   count_op(OP_REFRACT);
   double step = nv.step;
-  V3 n = facet_normal(nv, ray, find_border(*nv.sc, current, next), rng);
+  const rb_border_desc* bc = find_border(*nv.sc, current, next);
+  V3 n = facet_normal(nv, ray, bc, rng);
   V3 d1 = ray.d;
   double cos1 = dot(d1, n);
   double sin1 = std::sqrt(1 - cos1 * cos1);
   double sin2 = n1 * sin1 / n2;
   double cos2 = std::sqrt(1 - sin2 * sin2);
+  bool absorbed = false;
+  if (bc && bc->multilayer >= 0) {  // :79-96: reflect below R, transmit below R + T, else absorb
+    double R, T;
+    multilayer_rt(nv.sc->d, bc, ray.lambda, cos1, R, T);
+    double rnd = rng.uniform();
+    if (rnd < R) {
+      do_reflection(n1, ray, nv, current, next, &n, rng);
+      return;
+    } else if (!(rnd < R + T)) {
+      absorbed = true;
+    }
+    goto transmission_process;
+  }
   if (sin2 > 1.) {
     do_reflection(n1, ray, nv, current, next, &n, rng);
     return;
@@ -1649,6 +1702,7 @@ static void do_fresnel(double n1, double n2, double k2, Ray& ray, Nav& nv, int c
       return;
     }
   }
+transmission_process:
   V3 d2;
   if (sin1 != 0) {
     d2 = v3((d1.x - cos1 * n.x) * sin2 / sin1 + n.x * cos2, (d1.y - cos1 * n.y) * sin2 / sin1 + n.y * cos2,
@@ -1658,8 +1712,12 @@ static void do_fresnel(double n1, double n2, double k2, Ray& ray, Nav& nv, int c
   }
   double t = ray.t + step / (kC / n1);
   ray.add_point(nv.p, t, next);
-  ray.set_direction(d2);
-  nv.d = ray.d;
+  if (absorbed) {
+    ray.status = RB_ABSORB;
+  } else {
+    ray.set_direction(d2);  // a NaN d2 (transmission beyond the critical angle) is ignored, as by ARay
+    nv.d = ray.d;
+  }
 }
 
 // src/AOpticsManager.cxx:346-519 for one ray
@@ -2314,6 +2372,10 @@ int oracle_contains_batch(const rb_scene_desc* desc, int shape, int64_t n, const
 }
 
 double oracle_table_eval(const rb_scene_desc* desc, int table, double x) { return table_eval(desc, table, x); }
+
+double oracle_th2_interpolate(const rb_grid2d_desc* g, const double* vals, double x, double y) {
+  return th2_interpolate(vals, *g, x, y);
+}
 
 double oracle_lens_index(const rb_scene_desc* desc, int volume, double lambda) {
   return lens_index(desc, desc->volumes[volume], lambda);

@@ -47,7 +47,10 @@ VOLUME_DTYPE = np.dtype(
      ("index_model", "<i4"), ("index_ref", "<i4"), ("k_table", "<i4"),
      ("qe_lambda_table", "<i4"), ("qe_angle_table", "<i4"), ("border_first", "<i4"), ("border_count", "<i4")],
     align=True)
-BORDER_DTYPE = np.dtype([("volume2", "<i4"), ("lambertian", "<i4"), ("sigma", "<f8")], align=True)
+BORDER_DTYPE = np.dtype([("volume2", "<i4"), ("lambertian", "<i4"), ("sigma", "<f8"), ("multilayer", "<i4"),
+                         ("pad", "<i4")], align=True)
+GRID_DTYPE = np.dtype([("nx", "<i4"), ("ny", "<i4"), ("xlo", "<f8"), ("xhi", "<f8"), ("ylo", "<f8"), ("yhi", "<f8"),
+                       ("r_offset", "<i4"), ("t_offset", "<i4")], align=True)
 NODE_DTYPE = np.dtype(
     [("rot", "<f8", 9), ("tr", "<f8", 3), ("volume", "<i4"), ("parent", "<i4"),
      ("first_child", "<i4"), ("n_children", "<i4")], align=True)
@@ -55,7 +58,8 @@ TABLE_DTYPE = np.dtype([("offset", "<i4"), ("n", "<i4")], align=True)
 
 assert SHAPE_DTYPE.itemsize == 64
 assert VOLUME_DTYPE.itemsize == 48
-assert BORDER_DTYPE.itemsize == 16
+assert BORDER_DTYPE.itemsize == 24
+assert GRID_DTYPE.itemsize == 48
 assert NODE_DTYPE.itemsize == 112
 assert TABLE_DTYPE.itemsize == 8
 
@@ -73,6 +77,7 @@ class rb_scene_desc(C.Structure):
         ("n_tables", C.c_int32), ("tables", C.c_void_p),
         ("n_table_xy", C.c_int32), ("table_xy", C.c_void_p),
         ("n_borders", C.c_int32), ("borders", C.c_void_p),
+        ("n_grids", C.c_int32), ("grids", C.c_void_p),
     ]
 
 

@@ -64,6 +64,7 @@ struct DScene {
   const rb_table_desc* tables;
   const double* table_xy;
   const rb_border_desc* borders;
+  const rb_grid2d_desc* grids;  // multilayer (wavelength, angle) tables
   int32_t n_nodes;
   // BVH entries [stage_lo, stage_lo + stage_n) -- the top node's BVH, at
   // most kBvhStage entries -- are read from an LDS copy that each tracing

@@ -202,6 +202,19 @@ int validate(std::string& err, const rb_scene_desc* d) {
     const rb_border_desc& b = d->borders[i];
     if (b.volume2 < -2 || b.volume2 >= d->n_volumes) return fail_msg(err, -1, "scene: border volume out of range");
     if (!(b.sigma >= 0.0) || !std::isfinite(b.sigma)) return fail_msg(err, -1, "scene: border roughness must be >= 0");
+    if (b.multilayer < -1 || b.multilayer >= std::max(d->n_grids, 0))
+      return fail_msg(err, -1, "scene: border multilayer table out of range");
+  }
+  if (d->n_grids > 0 && !d->grids) return fail_msg(err, -1, "scene: missing multilayer tables");
+  for (int i = 0; i < d->n_grids; i++) {
+    const rb_grid2d_desc& g = d->grids[i];
+    const int64_t cells = int64_t(g.nx) * g.ny;
+    if (g.nx < 1 || g.ny < 1 || cells > (int64_t(1) << 26))
+      return fail_msg(err, -1, "scene: multilayer table needs 1 <= nx, ny and nx * ny <= 2^26");
+    if (!(g.xhi > g.xlo) || !(g.yhi > g.ylo) || !std::isfinite(g.xhi - g.xlo) || !std::isfinite(g.yhi - g.ylo))
+      return fail_msg(err, -1, "scene: multilayer table axes must be finite with hi > lo");
+    if (g.r_offset < 0 || g.t_offset < 0 || g.r_offset + cells > d->n_params || g.t_offset + cells > d->n_params)
+      return fail_msg(err, -1, "scene: multilayer table values out of range");
   }
   for (int i = 0; i < d->n_tables; i++) {
     const rb_table_desc& t = d->tables[i];
@@ -239,7 +252,7 @@ size_t put(std::vector<char>& blob, const X* src, size_t count) {
 
 // Offsets of the DScene arrays inside the packed blob.
 struct SceneLayout {
-  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy, borders;
+  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy, borders, grids;
   int32_t n_nodes;
   bool has_top_box = false;  // the top node has daughters: their bounding box follows
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
@@ -294,6 +307,7 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
   lay.tables = put(blob, d->tables, size_t(std::max(d->n_tables, 0)));
   lay.table_xy = put(blob, d->table_xy, size_t(std::max(d->n_table_xy, 0)));
   lay.borders = put(blob, d->borders, size_t(std::max(d->n_borders, 0)));
+  lay.grids = put(blob, d->grids, size_t(std::max(d->n_grids, 0)));
   lay.has_borders = d->n_borders > 0;
   lay.n_nodes = d->n_nodes;
   lay.has_top_box = d->n_nodes > 0 && nodes[0].bvh_root >= 0;
@@ -321,6 +335,7 @@ inline DScene scene_view(const char* base, const SceneLayout& lay) {
   s.tables = reinterpret_cast<const rb_table_desc*>(base + lay.tables);
   s.table_xy = reinterpret_cast<const double*>(base + lay.table_xy);
   s.borders = reinterpret_cast<const rb_border_desc*>(base + lay.borders);
+  s.grids = reinterpret_cast<const rb_grid2d_desc*>(base + lay.grids);
   s.n_nodes = lay.n_nodes;
   s.stage_lo = 0;
   s.stage_n = 0;

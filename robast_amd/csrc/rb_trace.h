@@ -627,6 +627,52 @@ __device__ __forceinline__ V3<double> facet_normal(const SC& S, const TraceParam
   return n;
 }
 
+// TH2::Interpolate(x, y) over a multilayer table (AMultilayer::CoherentTMMMixed
+// after PreCalculateCoherentTMM, include/AMultilayer.h:110-114, 243-262):
+// bilinear between the four nearest bin centres, the neighbour bin held
+// inside the axis, 0 outside either axis. Restated from ROOT's documented
+// behaviour; no FMA contraction, so the oracle's plain evaluation agrees bit
+// for bit.
+struct Th2Axis {
+  int b, i1, i2;
+  double c1, c2;
+};
+__device__ inline Th2Axis th2_axis(double v, int n, double lo, double hi) {
+#pragma clang fp contract(off)
+  const double w = (hi - lo) / n;
+  int b = v < lo ? 0 : !(v < hi) ? n + 1 : 1 + int(n * (v - lo) / (hi - lo));
+  b = b > n + 1 ? n + 1 : b;
+  const double up = lo + b * w;
+  const int b1 = (up - v) > w / 2 ? b - 1 : b;
+  Th2Axis a;
+  a.b = b;
+  a.c1 = lo + (b1 - 0.5) * w;
+  a.c2 = lo + (b1 + 0.5) * w;
+  a.i1 = b1 < 1 ? 1 : b1 > n ? n : b1;
+  a.i2 = b1 + 1 < 1 ? 1 : b1 + 1 > n ? n : b1 + 1;
+  return a;
+}
+__device__ RB_COLD_FN double th2_interpolate(const double* vals, const rb_grid2d_desc& g, double x, double y) {
+#pragma clang fp contract(off)
+  const Th2Axis ax = th2_axis(x, g.nx, g.xlo, g.xhi), ay = th2_axis(y, g.ny, g.ylo, g.yhi);
+  if (ax.b < 1 || ax.b > g.nx || ay.b < 1 || ay.b > g.ny) return 0.0;
+  const double q11 = vals[int64_t(ay.i1 - 1) * g.nx + ax.i1 - 1], q21 = vals[int64_t(ay.i1 - 1) * g.nx + ax.i2 - 1];
+  const double q12 = vals[int64_t(ay.i2 - 1) * g.nx + ax.i1 - 1], q22 = vals[int64_t(ay.i2 - 1) * g.nx + ax.i2 - 1];
+  const double d = (ax.c2 - ax.c1) * (ay.c2 - ay.c1);
+  return q11 / d * (ax.c2 - x) * (ay.c2 - y) + q21 / d * (x - ax.c1) * (ay.c2 - y) +
+         q12 / d * (ax.c2 - x) * (y - ay.c1) + q22 / d * (x - ax.c1) * (y - ay.c1);
+}
+// Reflectance and transmittance of border condition bc's multilayer at the
+// ray's wavelength and incidence angle acos(cos1).
+template <typename SC>
+__device__ inline void multilayer_rt(const SC& S, const rb_border_desc* bc, double lambda, double cos1, double& R,
+                                     double& T) {
+  const rb_grid2d_desc& g = S.grids[bc->multilayer];
+  const double angle = cold_acos(cos1);
+  R = th2_interpolate(S.params + g.r_offset, g, lambda, angle);
+  T = th2_interpolate(S.params + g.t_offset, g, lambda, angle);
+}
+
 __device__ __forceinline__ void add_point(const RayIO& io, int64_t i, Lane& L, V3<double> x, double t, int node) {
   if (io.path_cap > 0 && L.npoints - 1 < io.path_cap) {
     double* s = io.path_xyzt + (int64_t(io.path_cap) * i + (L.npoints - 1)) * 4;
@@ -656,6 +702,12 @@ __device__ void do_reflection(const SC& S, const TraceParams& P, const RayIO& io
     double ref = v.refl_table >= 0 ? table_eval(S, v.refl_table, L.lambda) : v.refl_const;
     ref = ref > 1 ? 1 : ref;
     ref = ref < 0 ? 0 : ref;
+    if constexpr (SC::borders_on) {
+      if (bc && bc->multilayer >= 0) {  // the coating's reflectance, unclamped (:194-198)
+        double T;
+        multilayer_rt(S, bc, L.lambda, cos1, ref, T);
+      }
+    }
     if (ref < draw(P, id, L)) {
       absorbed = true;
       L.status = RB_ABSORB;
@@ -682,21 +734,36 @@ __device__ void do_reflection(const SC& S, const TraceParams& P, const RayIO& io
   add_point(io, i, L, L.p, t, next);
 }
 
-// AOpticsManager::DoFresnel (src/AOpticsManager.cxx:52-167), no multilayer
-// branch. Refracts the ray, or returns true when it is reflected instead
-// (total internal reflection, or the Fresnel draw): the caller then runs
-// DoReflection with the same normal, so each is inlined once.
-template <typename Rec>
-__device__ bool do_fresnel(const TraceParams& P, const RayIO& io, int64_t i, uint64_t id, Lane& L, double n1,
-                           double n2, double k2, int next, V3<double> n, Rec& rec) {
+// AOpticsManager::DoFresnel (src/AOpticsManager.cxx:52-167). Refracts the
+// ray, or returns true when it is reflected instead (total internal
+// reflection, the Fresnel draw, or a multilayer's reflectance): the caller
+// then runs DoReflection with the same normal, so each is inlined once. A
+// multilayer condition (:79-96) replaces both the TIR test and the Fresnel
+// draw: one uniform against R, R + T; above R + T the ray is transmitted
+// and absorbed at the surface.
+template <typename Rec, typename SC>
+__device__ bool do_fresnel(const SC& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id, Lane& L,
+                           double n1, double n2, double k2, int next, V3<double> n, Rec& rec,
+                           const rb_border_desc* bc) {
   double step = L.step;
   V3<double> d1 = L.d;
   double cos1 = dot(d1, n);
   double sin1 = sqrt(1 - cos1 * cos1);
   double sin2 = n1 * sin1 / n2;
   double cos2 = sqrt(1 - sin2 * sin2);
-  if (sin2 > 1.) return true;
-  if (P.fresnel) {
+  bool coated = false, absorbed = false;
+  if constexpr (SC::borders_on) {
+    if (bc && bc->multilayer >= 0) {
+      double R, T;
+      multilayer_rt(S, bc, L.lambda, cos1, R, T);
+      const double u = draw(P, id, L);
+      if (u < R) return true;
+      coated = true;
+      absorbed = !(u < R + T);
+    }
+  }
+  if (!coated && sin2 > 1.) return true;
+  if (!coated && P.fresnel) {
     double Rs, Rp;
     if (k2 <= 0.) {
       double eta1S = n1 * cos1, eta2S = n2 * cos2, eta1P = n1 / cos1, eta2P = n2 / cos2;
@@ -728,8 +795,12 @@ __device__ bool do_fresnel(const TraceParams& P, const RayIO& io, int64_t i, uin
   }
   double t = L.t + step / (kC / n1);
   add_point(io, i, L, L.p, t, next);
-  set_direction(L, d2);
-  rec.refract();
+  if (absorbed) {
+    L.status = RB_ABSORB;
+  } else {
+    set_direction(L, d2);  // beyond the critical angle (a coated surface only) d2 is NaN and is ignored
+    rec.refract();
+  }
   return false;
 }
 
@@ -789,7 +860,7 @@ __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, in
       }
       const rb_border_desc* bc = find_border(S, current, next);
       const V3<double> n = facet_normal(S, P, L, bc);
-      if (mirror || do_fresnel(P, io, i, id, L, n1, n2, k2, next, n, rec))
+      if (mirror || do_fresnel(S, P, io, i, id, L, n1, n2, k2, next, n, rec, bc))
         do_reflection(S, P, io, i, id, L, n1, next, n, rec, bc);
     } else if ((c_free || c_lens) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
       double n1 = c_lens ? lens_index(S, vol_of(S, current), lambda) : 1.;

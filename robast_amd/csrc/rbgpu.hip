@@ -861,7 +861,8 @@ int rbgpu_abi_sizes(int32_t* out) {
   out[12] = sizeof(rb_shooter_desc);
   out[13] = sizeof(rb_bunches);
   out[14] = sizeof(rb_bunch_opts);
-  return 15;
+  out[15] = sizeof(rb_grid2d_desc);
+  return 16;
 }
 
 int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {

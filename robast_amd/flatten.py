@@ -33,10 +33,13 @@ class FlatScene:
     """Owns the numpy arrays behind an rb_scene_desc."""
 
     def __init__(self, nodes, children, volumes, shapes, params, tables, table_xy, node_names, volume_objs,
-                 borders=None):
+                 borders=None, grids=None):
         if borders is None:
             borders = np.zeros(0, dtype=A.BORDER_DTYPE)
+        if grids is None:
+            grids = np.zeros(0, dtype=A.GRID_DTYPE)
         self.borders = borders
+        self.grids = grids
         self.nodes = nodes
         self.children = children
         self.volumes = volumes
@@ -55,6 +58,7 @@ class FlatScene:
         d.n_tables, d.tables = len(tables), A.ptr(tables)
         d.n_table_xy, d.table_xy = len(table_xy), A.ptr(table_xy)
         d.n_borders, d.borders = len(borders), (A.ptr(borders) if len(borders) else None)
+        d.n_grids, d.grids = len(grids), (A.ptr(grids) if len(grids) else None)
         self.desc = d
 
     def desc_ptr(self):
@@ -178,20 +182,33 @@ def flatten(top_volume):
             vol_arr[i][k] = v
 
     # border-surface conditions, per component1 volume in registration order;
-    # a component2 that is not placed can never be entered (-2 matches nothing)
-    border_recs = []
+    # a component2 that is not placed can never be entered (-2 matches nothing).
+    # A multilayer becomes a (wavelength, angle) table of R and T in params,
+    # shared by every condition that holds the same AMultilayer object.
+    border_recs, grid_recs, grid_ids = [], [], {}
     for i, vol in enumerate(volume_objs):
         conds = getattr(vol, "border_conditions", [])
         vol_arr[i]["border_first"] = len(border_recs)
         vol_arr[i]["border_count"] = len(conds)
         for c in conds:
-            if c.multilayer is not None:
-                raise NotImplementedError("ABorderSurfaceCondition: multilayer coatings are not supported")
             v2 = -1 if c.component2 is None else volume_ids.get(id(c.component2), -2)
-            border_recs.append((v2, 1 if c.lambertian else 0, c.sigma))
+            g = -1
+            if c.multilayer is not None:
+                g = grid_ids.get(id(c.multilayer))
+                if g is None:
+                    nx, xlo, xhi, ny, ylo, yhi, R, T = c.multilayer.device_table()
+                    g = grid_ids[id(c.multilayer)] = len(grid_recs)
+                    grid_recs.append((nx, xlo, xhi, ny, ylo, yhi, add_params(np.ravel(R)), add_params(np.ravel(T))))
+            border_recs.append((v2, 1 if c.lambertian else 0, c.sigma, g))
     borders = np.zeros(len(border_recs), dtype=A.BORDER_DTYPE)
-    for i, (v2, lam, sig) in enumerate(border_recs):
+    for i, (v2, lam, sig, g) in enumerate(border_recs):
         borders[i]["volume2"], borders[i]["lambertian"], borders[i]["sigma"] = v2, lam, sig
+        borders[i]["multilayer"] = g
+    grids = np.zeros(len(grid_recs), dtype=A.GRID_DTYPE)
+    for i, (nx, xlo, xhi, ny, ylo, yhi, ro, to) in enumerate(grid_recs):
+        grids[i]["nx"], grids[i]["xlo"], grids[i]["xhi"] = nx, xlo, xhi
+        grids[i]["ny"], grids[i]["ylo"], grids[i]["yhi"] = ny, ylo, yhi
+        grids[i]["r_offset"], grids[i]["t_offset"] = ro, to
 
     shp = np.zeros(len(shapes), dtype=A.SHAPE_DTYPE)
     for i, (t, off, n, org, half) in enumerate(shapes):
@@ -204,7 +221,8 @@ def flatten(top_volume):
 
     par = np.array(params if params else [0.0], dtype=np.float64)
     txy = np.array(table_xy if table_xy else [0.0, 0.0], dtype=np.float64)
-    fs = FlatScene(nodes, children, vol_arr, shp, par, tab, txy, [r[3] for r in node_recs], volume_objs, borders)
+    fs = FlatScene(nodes, children, vol_arr, shp, par, tab, txy, [r[3] for r in node_recs], volume_objs, borders,
+                   grids)
     fs.desc.n_tables = len(tables)
     fs.desc.n_table_xy = len(table_xy)
     return fs

@@ -617,8 +617,9 @@ class ABorderSurfaceCondition:
     """include/ABorderSurfaceCondition.h:24-45 (src/ABorderSurfaceCondition.cxx):
     the surface between component1 and component2 (None = leaving the top
     volume), registered with component1. Gaussian roughness (rad) perturbs
-    the facet normal; Lambertian makes reflections diffuse. Multilayer
-    coatings are not supported by the flattener."""
+    the facet normal; Lambertian makes reflections diffuse; a multilayer
+    (robast_amd.multilayer.AMultilayer) sets the reflectance and
+    transmittance of the surface."""
 
     def __init__(self, component1, component2):
         self.component1 = component1

@@ -162,3 +162,30 @@ class ASchottFormula(ARefractiveIndex):
         lam = lam / um
         return math.sqrt(p[0] + p[1] * lam ** 2. + p[2] * lam ** -2. + p[3] * lam ** -4. +
                          p[4] * lam ** -6. + p[5] * lam ** -8.)
+
+
+class AFilmetrixDotCom(ARefractiveIndex):
+    """src/AFilmetrixDotCom.cxx: n and k tables from a filmetrics.com text
+    file (header `Wavelength(nm)<TAB>n<TAB>k`, optional UTF-8 BOM and CR;
+    then wavelength [nm], n, k per line), as linear TGraphs over wavelength
+    in cm."""
+
+    HEADER = "Wavelength(nm)\tn\tk"
+
+    def __init__(self, fname):
+        super().__init__()
+        with open(fname, encoding="utf-8-sig") as f:       # drops a UTF-8 BOM; CR LF read as LF
+            head = f.readline().rstrip("\n").rstrip("\r")
+            if head != self.HEADER:
+                raise ValueError(f"AFilmetrixDotCom: invalid data format in {fname}")
+            self.n_graph, self.k_graph = TGraph(), TGraph()
+            for line in f:
+                parts = line.split()
+                if len(parts) < 3:
+                    break
+                try:
+                    wl, n, k = (float(p) for p in parts[:3])
+                except ValueError:
+                    break
+                self.n_graph.SetPoint(self.n_graph.GetN(), wl * nm, n)
+                self.k_graph.SetPoint(self.k_graph.GetN(), wl * nm, k)

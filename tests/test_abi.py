@@ -30,8 +30,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_binding():
     lib = _lib.load()
     out = (C.c_int32 * 16)()
-    assert lib.rbgpu_abi_sizes(out) == 15
-    sizes = list(out)[:15]
+    assert lib.rbgpu_abi_sizes(out) == 16
+    sizes = list(out)[:16]
     assert sizes[0] == 96                              # rb_matrix
     assert sizes[1] == A.SHAPE_DTYPE.itemsize
     assert sizes[2] == A.VOLUME_DTYPE.itemsize
@@ -43,11 +43,12 @@ def test_struct_sizes_match_binding():
     assert sizes[8] == C.sizeof(A.rb_raygen_desc)
     assert sizes[9] == C.sizeof(A.rb_adjoint_desc)
     assert sizes[10] == C.sizeof(A.rb_hist2d_desc) == 48
-    assert sizes[11] == A.BORDER_DTYPE.itemsize == 16
+    assert sizes[11] == A.BORDER_DTYPE.itemsize == 24
     assert C.sizeof(A.rb_trace_opts) == 40
     assert sizes[12] == C.sizeof(A.rb_shooter_desc) == 176
     assert sizes[13] == C.sizeof(A.rb_bunches) == 11 * 8
     assert sizes[14] == C.sizeof(A.rb_bunch_opts) == 72
+    assert sizes[15] == A.GRID_DTYPE.itemsize == 48
 
 
 def test_null_context_is_an_error_not_a_crash():

@@ -1,0 +1,309 @@
+"""Multilayer border surfaces: AMultilayer's coherent transfer-matrix method
+(src/AMultilayer.cxx:240-483) tabulated over (wavelength, angle) as
+PreCalculateCoherentTMM does (include/AMultilayer.h:243-262), and the
+multilayer branches of DoFresnel (src/AOpticsManager.cxx:79-96) and
+DoReflection (:194-198) that read it.
+
+CPU (pins the TMM and the table reader): the reference's own known answers
+(testTMM, tutorials/unittest_robast.py:627-709: rs, rp, ts, tp of a four-medium
+stack, the reversed stack, scalar against vector evaluation, the
+precalculated table at 45 deg / 600); a bare interface against the closed
+Fresnel equations for a real and an absorbing medium; a quarter-wave coating's
+zero reflectance at its design wavelength; R + T = 1 for lossless stacks and
+< 1 for absorbing ones; TH2::Interpolate reproducing a bilinear function
+between bin centres, flat in the outer half bins, 0 outside, and the host
+and oracle readers agreeing bit for bit. The reference's
+testMirrorBoundaryMultilayer and testLensBoundaryMultilayer (:250-331, with
+its optical-constant files as fixtures) and a coated lens surface's
+reflect / transmit / absorb fractions run on the oracle and the GPU.
+
+GPU: per-ray parity with the oracle on a scene with a coated mirror and a
+lens coated on both faces, with a coarse precalculated table (interpolation
+between centres) and the default table."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from robast_amd import (ABorderSurfaceCondition, AFocalSurface, ALens, AMirror, ARayArray, ARefractiveIndex,
+                        TGeoBBox, TGeoTranslation)
+from robast_amd import _abi as A
+from robast_amd import geometry as G
+from robast_amd.materials import AFilmetrixDotCom
+from robast_amd.multilayer import DEFAULT_GRID, AMultilayer, kP, kS, th2_interpolate
+from test_reference_units import BACKENDS, make_the_world, run
+
+nm, um, mm, cm, m, deg = G.nm, G.um, G.mm, G.cm, G.m, G.deg
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "optical_constants")
+
+
+def material(name):
+    return AFilmetrixDotCom(os.path.join(DATA, f"{name}.txt"))
+
+
+# ---------------------------------------------------------------- TMM (CPU)
+
+def test_tmm_reference_known_answers():
+    # unittest_robast.py:627-709 (tmm's basic test)
+    med1, med2, med3, med4 = (ARefractiveIndex(1.), ARefractiveIndex(2., 4.), ARefractiveIndex(3., .3),
+                              ARefractiveIndex(1., .1))
+    multi = AMultilayer(med1, med4)
+    multi.InsertLayer(med2, 2)
+    multi.InsertLayer(med3, 3)
+    rs, rp = 0.37273208839139516, 0.37016110373044969
+    ts, tp = 0.22604491247079261, 0.22824374314132009
+    for pol, (r, t) in ((kS, (rs, ts)), (kP, (rp, tp))):
+        R, T = multi.CoherentTMM(pol, 0.1, 100)
+        assert abs(R - r) < 1e-12 and abs(T - t) < 1e-12
+    R, T = multi.CoherentTMMMixed(0.1, 100)
+    assert abs(R - (rs + rp) / 2) < 1e-12 and abs(T - (ts + tp) / 2) < 1e-12
+    reverse = AMultilayer(med4, med1)
+    reverse.InsertLayer(med3, 3)
+    reverse.InsertLayer(med2, 2)
+    for pol, (r, t) in ((kS, (rs, ts)), (kP, (rp, tp))):
+        R, T = reverse.CoherentTMM(pol, 0.1, 100, reverse=True)
+        assert abs(R - r) < 1e-12 and abs(T - t) < 1e-12
+    lam = np.arange(300, 800, dtype=float)
+    Rv, Tv = multi.CoherentTMMMixed(0.1, lam)
+    for k in (0, 123, 499):
+        r, t = multi.CoherentTMMMixed(0.1, lam[k])
+        assert abs(Rv[k] - r) < 1e-14 and abs(Tv[k] - t) < 1e-14
+    ang = np.arange(500) * math.pi / 2000
+    Ra, Ta = multi.CoherentTMMMixed(ang, 100)
+    r, t = multi.CoherentTMMMixed(ang[321], 100)
+    assert abs(Ra[321] - r) < 1e-14 and abs(Ta[321] - t) < 1e-14
+    r0, t0 = multi.CoherentTMMMixed(45 * deg, 600)
+    multi.PreCalculateCoherentTMM(801, 199.5, 1000.5, 90, -0.5 * deg, 89.5 * deg)
+    r1, t1 = multi.CoherentTMMMixed(45 * deg, 600)
+    assert abs(r0 - r1) < 1e-7 and abs(t0 - t1) < 1e-7
+
+
+@pytest.mark.parametrize("n2,k2", [(1.5, 0.0), (2.4, 0.3), (0.49, 4.86)])
+def test_bare_interface_is_fresnel(n2, k2):
+    """No films: the unpolarised R and T of the interface, by the Fresnel
+    equations with complex cos(theta_t) = sqrt(1 - (sin(theta_i) / N)^2)."""
+    ml = AMultilayer(ARefractiveIndex(1.0), ARefractiveIndex(n2, k2))
+    th = np.linspace(0, 89, 90) * deg
+    R, T = ml.CoherentTMMMixed(th, 500 * nm)
+    N = complex(n2, k2)
+    ci = np.cos(th)
+    ct = np.sqrt(1 - (np.sin(th) / N) ** 2 + 0j)
+    rs = (ci - N * ct) / (ci + N * ct)
+    rp = (N * ci - ct) / (N * ci + ct)
+    assert np.allclose(R, (abs(rs) ** 2 + abs(rp) ** 2) / 2, rtol=0, atol=1e-12)
+    if k2 == 0:
+        assert np.allclose(R + T, 1, rtol=0, atol=1e-12)
+        assert abs(R[0] - ((n2 - 1) / (n2 + 1)) ** 2) < 1e-15
+
+
+def test_quarter_wave_coating_and_energy():
+    glass, lam0 = ARefractiveIndex(1.69), 550 * nm
+    nc = math.sqrt(1.69)
+    coat = AMultilayer(ARefractiveIndex(1.0), glass)
+    coat.InsertLayer(ARefractiveIndex(nc), lam0 / (4 * nc))
+    assert coat.CoherentTMMMixed(0.0, lam0)[0] < 1e-15
+    assert coat.CoherentTMMMixed(0.0, 400 * nm)[0] > 1e-3
+    th = np.linspace(0, 85, 18) * deg
+    for l in (400 * nm, 550 * nm, 700 * nm):
+        R, T = coat.CoherentTMMMixed(th, l)
+        assert np.allclose(R + T, 1, rtol=0, atol=1e-12)
+    lossy = AMultilayer(ARefractiveIndex(1.0), glass)
+    lossy.InsertLayer(ARefractiveIndex(2.0, 0.2), 80 * nm)
+    R, T = lossy.CoherentTMMMixed(th, 500 * nm)
+    assert np.all(R + T < 1 - 1e-3) and np.all(R >= 0) and np.all(T >= 0)
+
+
+def test_th2_interpolate_semantics():
+    nx, xlo, xhi, ny, ylo, yhi = 8, -1.0, 3.0, 5, 10.0, 20.0
+    wx, wy = (xhi - xlo) / nx, (yhi - ylo) / ny
+    cx = xlo + (np.arange(1, nx + 1) - 0.5) * wx
+    cy = ylo + (np.arange(1, ny + 1) - 0.5) * wy
+    f = lambda x, y: 0.3 + 0.7 * x - 0.2 * y + 0.05 * x * y  # noqa: E731  bilinear
+    vals = f(cx[None, :], cy[:, None])
+    args = (nx, xlo, xhi, ny, ylo, yhi)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(cx[0], cx[-1], 500)
+    y = rng.uniform(cy[0], cy[-1], 500)
+    assert np.allclose(th2_interpolate(vals, *args, x, y), f(x, y), rtol=0, atol=1e-12)
+    assert np.allclose(th2_interpolate(vals, *args, cx[3], cy[2]), vals[2, 3], rtol=0, atol=1e-14)
+    # the outer half bins are flat along their axis
+    assert abs(th2_interpolate(vals, *args, xlo + 0.1 * wx, cy[1]) - f(cx[0], cy[1])) < 1e-12
+    assert abs(th2_interpolate(vals, *args, cx[2], yhi - 0.2 * wy) - f(cx[2], cy[-1])) < 1e-12
+    # outside either axis: 0; the upper edge is outside
+    for px, py in ((xlo - 1e-9, 15.0), (xhi, 15.0), (0.0, ylo - 1), (0.0, yhi), (np.nan, 15.0)):
+        assert th2_interpolate(vals, *args, px, py) == 0.0
+    # the oracle's reader, written separately, agrees bit for bit
+    from oracle import oracle as O
+    g = np.zeros((), dtype=A.GRID_DTYPE)
+    g["nx"], g["xlo"], g["xhi"], g["ny"], g["ylo"], g["yhi"] = args[0], xlo, xhi, ny, ylo, yhi
+    xs = np.concatenate([rng.uniform(xlo - 0.5, xhi + 0.5, 400), cx, [xlo, xhi]])
+    ys = np.concatenate([rng.uniform(ylo - 1, yhi + 1, 400), np.resize(cy, nx), [ylo, yhi]])
+    host = th2_interpolate(vals, *args, xs, ys)
+    ora = np.array([O.th2_interpolate(g, vals, a, b) for a, b in zip(xs, ys)])
+    assert np.array_equal(host, ora)
+
+
+def test_flattened_tables():
+    mgr = make_the_world()
+    ml = AMultilayer(ARefractiveIndex(1.0), ARefractiveIndex(1.5))
+    ml.InsertLayer(ARefractiveIndex(1.38), 100 * nm)
+    m1 = AMirror("m1", TGeoBBox("b1", 10 * cm, 10 * cm, 1 * cm))
+    m2 = AMirror("m2", TGeoBBox("b2", 10 * cm, 10 * cm, 1 * cm))
+    ABorderSurfaceCondition(mgr.GetTopVolume(), m1).SetMultilayer(ml)
+    ABorderSurfaceCondition(mgr.GetTopVolume(), m2).SetMultilayer(ml)
+    mgr.GetTopVolume().AddNode(m1, 1)
+    mgr.GetTopVolume().AddNode(m2, 1, TGeoTranslation(0, 0, 50 * cm))
+    mgr.CloseGeometry()
+    flat = mgr.flat()
+    assert len(flat.grids) == 1 and list(flat.borders["multilayer"]) == [0, 0]
+    g = flat.grids[0]
+    assert (g["nx"], g["ny"]) == (DEFAULT_GRID[0], DEFAULT_GRID[3])
+    R = flat.params[g["r_offset"]:g["r_offset"] + g["nx"] * g["ny"]].reshape(g["ny"], g["nx"])
+    lam = g["xlo"] + (np.arange(g["nx"]) + 0.5) * (g["xhi"] - g["xlo"]) / g["nx"]
+    th = g["ylo"] + (np.arange(g["ny"]) + 0.5) * (g["yhi"] - g["ylo"]) / g["ny"]
+    for i, j in ((0, 0), (400, 180), (800, 360)):
+        assert abs(R[j, i] - ml.CoherentTMMMixed(th[j], lam[i])[0]) < 1e-14
+
+
+# ---------------------------------------------------------------- tracing
+
+def rays_45(n, lam, z=0.51 * m):
+    s = 1 / math.sqrt(2)
+    return ARayArray.from_arrays(np.zeros(n), np.zeros(n), np.full(n, z), np.zeros(n), np.full(n, s), np.full(n, -s),
+                                 np.full(n, lam))
+
+
+def within_3sigma(k, n, p):
+    e = math.sqrt(max(n * p * (1 - p), 1.0))
+    return abs(k - n * p) < 3 * e + 1
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_mirror_boundary_multilayer(backend):
+    # unittest_robast.py:250-289: Al mirror under 25.4 nm of SiO2, 45 deg
+    G._reset_registry()
+    mgr = make_the_world()
+    mirror = AMirror("mirror", TGeoBBox("mirrorbox", 0.5 * m, 0.5 * m, 0.5 * m))
+    layer = AMultilayer(ARefractiveIndex(1.0, 0.0), material("Al"))
+    layer.InsertLayer(material("SiO2"), 25.4 * nm)
+    ABorderSurfaceCondition(mgr.GetTopVolume(), mirror).SetMultilayer(layer)
+    mgr.GetTopVolume().AddNode(mirror, 1)
+    mgr.CloseGeometry()
+    N = 20000
+    for wl in range(300, 1100, 100):
+        arr = rays_45(N, wl * nm)
+        run(mgr, arr, backend)
+        ex, ab = (arr.status == A.RB_EXIT).sum(), (arr.status == A.RB_ABSORB).sum()
+        assert ex + ab == N
+        R, _ = layer.CoherentTMMMixed(45 * deg, wl * nm)
+        assert within_3sigma(ex, N, R), (wl, ex, N * R)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_lens_boundary_multilayer(backend):
+    # unittest_robast.py:291-331 (an AMirror named "lens"): Si under SiO2 2 um + Si3N4 33 nm
+    G._reset_registry()
+    mgr = make_the_world()
+    lens = AMirror("lens", TGeoBBox("lensbox", 0.5 * m, 0.5 * m, 0.5 * m))
+    layer = AMultilayer(ARefractiveIndex(1.0, 0.0), material("Si"))
+    layer.InsertLayer(material("SiO2"), 2000 * nm)
+    layer.InsertLayer(material("Si3N4"), 33 * nm)
+    ABorderSurfaceCondition(mgr.GetTopVolume(), lens).SetMultilayer(layer)
+    mgr.GetTopVolume().AddNode(lens, 1)
+    mgr.CloseGeometry()
+    N = 20000
+    for wl in range(300, 600, 50):
+        arr = rays_45(N, wl * nm)
+        run(mgr, arr, backend)
+        ex, ab = (arr.status == A.RB_EXIT).sum(), (arr.status == A.RB_ABSORB).sum()
+        assert ex + ab == N
+        R, _ = layer.CoherentTMMMixed(45 * deg, wl * nm)
+        assert within_3sigma(ex, N, R), (wl, ex, N * R)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_coated_lens_fractions(backend):
+    """A coated lens face: a ray is reflected with probability R, absorbed in
+    the coating with 1 - R - T, and transmitted otherwise (the uncoated far
+    face does nothing with Fresnel reflection off)."""
+    G._reset_registry()
+    mgr = make_the_world()
+    mgr.DisableFresnelReflection(True)
+    lens = ALens("lens", TGeoBBox("lensbox", 0.5 * m, 0.5 * m, 1 * cm))
+    lens.SetRefractiveIndex(ARefractiveIndex(1.5))
+    layer = AMultilayer(ARefractiveIndex(1.0), ARefractiveIndex(1.5))
+    layer.InsertLayer(ARefractiveIndex(2.2, 0.15), 120 * nm)
+    ABorderSurfaceCondition(mgr.GetTopVolume(), lens).SetMultilayer(layer)
+    mgr.GetTopVolume().AddNode(lens, 1)
+    mgr.CloseGeometry()
+    N, lam = 40000, 450 * nm
+    arr = rays_45(N, lam, z=5 * cm)
+    run(mgr, arr, backend)
+    R, T = layer.CoherentTMMMixed(45 * deg, lam)
+    up = ((arr.status == A.RB_EXIT) & (arr.dz > 0)).sum()
+    down = ((arr.status == A.RB_EXIT) & (arr.dz < 0)).sum()
+    ab = (arr.status == A.RB_ABSORB).sum()
+    assert up + down + ab == N
+    assert within_3sigma(up, N, R) and within_3sigma(ab, N, 1 - R - T) and within_3sigma(down, N, T)
+    assert 0.05 < 1 - R - T < 0.9
+
+
+def _coated_scene(precalc):
+    G._reset_registry()
+    mgr = make_the_world()
+    world = mgr.GetTopVolume()
+    mirror = AMirror("mirror", TGeoBBox("mirrorbox", 30 * cm, 30 * cm, 1 * cm))
+    al = AMultilayer(ARefractiveIndex(1.0), ARefractiveIndex(0.49, 4.86))
+    al.InsertLayer(ARefractiveIndex(1.46), 90 * nm)
+    ABorderSurfaceCondition(world, mirror).SetMultilayer(al)
+    world.AddNode(mirror, 1, TGeoTranslation(0, 0, -20 * cm))
+    lens = ALens("lens", TGeoBBox("lensbox", 40 * cm, 40 * cm, 2 * cm))
+    lens.SetRefractiveIndex(ARefractiveIndex(1.6))
+    top = AMultilayer(ARefractiveIndex(1.0), ARefractiveIndex(1.6))
+    top.InsertLayer(ARefractiveIndex(1.38), 100 * nm)
+    bottom = AMultilayer(ARefractiveIndex(1.6), ARefractiveIndex(1.0))
+    bottom.InsertLayer(ARefractiveIndex(2.1, 0.05), 60 * nm)
+    if precalc:  # coarse tables: most rays fall between bin centres
+        for ml in (al, top):
+            ml.PreCalculateCoherentTMM(31, 250 * nm, 700 * nm, 19, -2.5 * deg, 92.5 * deg)
+    ABorderSurfaceCondition(world, lens).SetMultilayer(top)
+    ABorderSurfaceCondition(lens, world).SetMultilayer(bottom)
+    world.AddNode(lens, 1, TGeoTranslation(0, 0, 20 * cm))
+    focal = AFocalSurface("focal", TGeoBBox("focalbox", 80 * cm, 80 * cm, 1 * cm))
+    world.AddNode(focal, 1, TGeoTranslation(0, 0, 60 * cm))
+    mgr.CloseGeometry()
+    rng = np.random.default_rng(23)
+    n = 60000
+    x, y = rng.uniform(-30, 30, n) * cm, rng.uniform(-30, 30, n) * cm
+    d = np.stack([rng.normal(0, 0.4, n), rng.normal(0, 0.4, n), np.full(n, -1.0)], 1)
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    arr = ARayArray.from_arrays(x, y, np.full(n, 45 * cm), d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy(),
+                                rng.uniform(280, 680, n) * nm)
+    return mgr, arr
+
+
+@pytest.mark.parametrize("precalc", [False, True])
+def test_oracle_coated_scene_runs(precalc):
+    from oracle import oracle as O
+    mgr, arr = _coated_scene(precalc)
+    O.trace(mgr.flat(), arr.soa(), mgr.trace_opts(), nthreads=8)
+    c = np.bincount(arr.status, minlength=6)
+    assert c[A.RB_RUN] == 0 and c[A.RB_ABSORB] > 1000 and c[A.RB_FOCUS] > 1000 and c[A.RB_EXIT] > 1000
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precalc", [False, True])
+def test_gpu_coated_scene_per_ray(ctx, precalc):
+    from helpers import compare, copy_soa
+    from oracle import oracle as O
+    from robast_amd._lib import make_rays
+    print(f"multilayer scene per ray, precalculated={precalc}", flush=True)
+    mgr, arr = _coated_scene(precalc)
+    soa = arr.soa()
+    g, o = copy_soa(soa), copy_soa(soa)
+    scene = ctx.upload(mgr.flat())
+    ctx.trace(scene, make_rays(g), len(g["x"]), mgr.trace_opts())
+    O.trace(mgr.flat(), o, mgr.trace_opts(), nthreads=8)
+    compare(g, o)
+    assert (g["status"] == A.RB_ABSORB).sum() > 1000
