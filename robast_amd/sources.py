@@ -108,9 +108,9 @@ class CorsikaBunches:
         f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
         self.x, self.y, self.zem, self.time = f32(x), f32(y), f32(zem), f32(time)
         self.cx, self.cy = f32(cx), f32(cy)
-        if cz is None:  # the reader's cz = -1 * sqrt(1 - cx^2 - cy^2) (:452), in float
-            cx32, cy32 = self.cx, self.cy
-            cz = -np.sqrt(np.float32(1.0) - cx32 * cx32 - cy32 * cy32)
+        if cz is None:  # the reader's cz = -1. * sqrt(1. - cx * cx - cy * cy) (:452): float squares, double rest
+            cx2, cy2 = (self.cx * self.cx).astype(np.float64), (self.cy * self.cy).astype(np.float64)
+            cz = -1.0 * np.sqrt(1.0 - cx2 - cy2)
         self.cz = f32(cz)
         self.lambda_, self.photons = f32(lambda_), f32(photons)
         n = len(self.tel_no)
