@@ -4,15 +4,22 @@ aspheric disk, Winston cone, and union / subtraction / intersection
 composites up to two levels) under random rotations, as mirrors, lenses,
 obscurations, focal surfaces and plain components, hit by random rays with
 Fresnel reflection on. Objects occupy separate cells of a 3 x 3 x 3 grid
-(no overlapping nodes, whose resolution is ROOT-internal).
+(no overlapping nodes, whose resolution is ROOT-internal). A second set of
+the same scenes puts border conditions on about half of the mirrors and
+lenses: Gaussian roughness, Lambertian reflection, or a one- or two-film
+coating (read from the default table or a coarse precalculated one), on
+either side of the surface.
 
 CPU: the scenes build, flatten and trace in the oracle with every ray
 terminated. GPU: librbgpu reproduces the oracle per ray: status, point
 count and last node identical for every ray; terminal point within 1e-7 cm
 for rays of at most 12 points. Longer paths (up to 80 points, bouncing
 between curved lens and mirror surfaces) multiply a 1-ulp difference at every
-curved surface they meet; their terminal points are held to 1e-3 cm (the
-largest seen is 2.6e-5 cm, on a 35-point path)."""
+curved surface they meet: recording both paths of the worst ray (52 points)
+shows the difference growing from 1e-14 cm at the first hit to 1e-4 cm at
+the 49th, then carried over a 1 m flight to the world's wall as 1.2e-2 cm.
+Their terminal points are held to 5e-2 cm, with status, point count and last
+node still identical for every ray."""
 import math
 
 import numpy as np
@@ -78,8 +85,33 @@ def _shape(rng, k, tag):
     return kind, G.TGeoCompositeShape(n, f"({n}a - {n}t) * {n}s")
 
 
-def random_scene(seed):
-    """World of 100 cm half size; up to 27 objects, one per grid cell."""
+def _surface(rng, world, vol, k):
+    """A random border condition between the world and vol (either way):
+    Gaussian roughness, Lambertian reflection or a one- or two-film coating."""
+    from robast_amd.multilayer import AMultilayer
+    first, second = (world, vol) if rng.uniform() < 0.7 else (vol, world)
+    cond = G.ABorderSurfaceCondition(first, second)
+    pick = int(rng.integers(0, 3))
+    if pick == 0:
+        cond.SetGaussianRoughness(float(rng.uniform(0.2, 3.0)) * G.deg)
+    elif pick == 1:
+        cond.EnableLambertian(True)
+    else:
+        n_sub = float(rng.uniform(1.3, 1.8))
+        ml = AMultilayer(ARefractiveIndex(1.0), ARefractiveIndex(n_sub, float(rng.choice([0.0, 2.0]))))
+        for _ in range(int(rng.integers(1, 3))):
+            ml.InsertLayer(ARefractiveIndex(float(rng.uniform(1.3, 2.4)), float(rng.choice([0.0, 0.05]))),
+                           float(rng.uniform(40, 200)) * G.nm)
+        if rng.uniform() < 0.5:   # a coarse table: most rays fall between its bin centres
+            ml.PreCalculateCoherentTMM(25, 280e-7, 620e-7, 16, -3 * G.deg, 93 * G.deg)
+        cond.SetMultilayer(ml)
+        if rng.uniform() < 0.5:
+            cond.SetGaussianRoughness(float(rng.uniform(0.2, 2.0)) * G.deg)
+
+
+def random_scene(seed, surfaces=False):
+    """World of 100 cm half size; up to 27 objects, one per grid cell. With
+    surfaces, about half of the mirrors and lenses get a border condition."""
     G._reset_registry()
     rng = np.random.default_rng(seed)
     mgr = AOpticsManager("manager", f"random {seed}")
@@ -118,6 +150,8 @@ def random_scene(seed):
                                      float(rng.uniform(0, 360)))
                 mat = G.TGeoCombiTrans(ix * CELL, iy * CELL, iz * CELL, rot)
                 world.AddNode(vol, 1, mat)
+                if surfaces and role <= 2 and rng.uniform() < 0.6:
+                    _surface(rng, world, vol, k)
                 k += 1
     mgr.CloseGeometry()
     return mgr
@@ -139,9 +173,10 @@ def random_rays(seed, n):
     return soa
 
 
+@pytest.mark.parametrize("surfaces", [False, True])
 @pytest.mark.parametrize("seed", SEEDS)
-def test_random_scene_traces_in_oracle(seed):
-    mgr = random_scene(seed)
+def test_random_scene_traces_in_oracle(seed, surfaces):
+    mgr = random_scene(seed, surfaces)
     soa = random_rays(seed, 4000)
     O.trace(mgr.flat(), soa, mgr.trace_opts(), nthreads=8)
     c = np.bincount(soa["status"], minlength=6)
@@ -150,11 +185,14 @@ def test_random_scene_traces_in_oracle(seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("surfaces", [False, True])
 @pytest.mark.parametrize("seed", SEEDS)
-def test_random_scene_gpu_matches_oracle(ctx, seed):
+def test_random_scene_gpu_matches_oracle(ctx, seed, surfaces):
     from robast_amd._lib import make_rays
-    print(f"random scene {seed}", flush=True)
-    mgr = random_scene(seed)
+    print(f"random scene {seed}, border conditions {surfaces}", flush=True)
+    mgr = random_scene(seed, surfaces)
+    if surfaces:
+        assert len(mgr.flat().borders) > 0
     soa = random_rays(seed, 20000)
     g = copy_soa(soa)
     ctx.trace(ctx.upload(mgr.flat()), make_rays(g), len(g["x"]), mgr.trace_opts())
@@ -168,4 +206,4 @@ def test_random_scene_gpu_matches_oracle(ctx, seed):
     print(f"  max |dpos| short paths {dp[short].max():.2e} cm, long paths {dp[~short].max(initial=0):.2e} cm",
           flush=True)
     assert dp[short].max() <= 1e-7
-    assert dp[~short].max(initial=0) <= 1e-3
+    assert dp[~short].max(initial=0) <= 5e-2
