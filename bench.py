@@ -81,13 +81,15 @@ def cpu_baseline(mgr, gen, target_s=10.0):
 
 
 def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3):
-    """C1-C4 at 1e6 rays each: FP64 and mixed device Mrays/s, and the mixed
-    form's error against the FP64 result (status flips, focused-hit RMS)."""
+    """C1-C4 at 1e6 rays each, and C2 with rough, coated facets (border-surface
+    kernel form): FP64 and mixed device Mrays/s, and the mixed form's error
+    against the FP64 result (status flips, focused-hit RMS)."""
     import math
     dev = torch.device("cuda")
     out = {}
-    for name in ("c1", "c2", "c3", "c4"):
-        mgr = {"c1": configs.c1_scene, "c2": configs.c2_scene, "c3": configs.c3_scene, "c4": configs.c4_scene}[name]()
+    for name in ("c1", "c2", "c3", "c4", "c2_surfaces"):
+        mgr = {"c1": configs.c1_scene, "c2": configs.c2_scene, "c3": configs.c3_scene, "c4": configs.c4_scene,
+               "c2_surfaces": lambda: configs.c2_scene(surfaces=True)}[name]()
         scene = ctx.upload(mgr.flat())
         src = {f: torch.empty(n, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz",
                                                                             "lambda_")}
@@ -97,7 +99,7 @@ def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3)
             g = configs.c1_raygen(1.0, side)
             n_c = side * side
         else:
-            g = {"c2": configs.c2_raygen, "c3": configs.c3_raygen}.get(name)
+            g = {"c2": configs.c2_raygen, "c2_surfaces": configs.c2_raygen, "c3": configs.c3_raygen}.get(name)
             g = g() if g else configs.c4_raygen(mgr.c4_dz)
             n_c = n
         ctx.generate(g, make_rays_device(src), n_c)

@@ -132,11 +132,14 @@ def c2_reflectance():
     return g
 
 
-def c2_scene(deterministic=False, per_facet=False, facet_radius=None, facet_tilt=None, camera_dz=0.0):
+def c2_scene(deterministic=False, per_facet=False, facet_radius=None, facet_tilt=None, camera_dz=0.0, surfaces=False):
     """85-facet Davies-Cotton, f = 16 m. Facet reflectance linear in lambda
     0.80 -> 0.90 over 300-600 nm (R = 1 when deterministic). Camera disk
     r = 1.2 m at z = f; square 2.5 m obscuration plate 1 cm behind it.
-    per_facet: one volume per facet (needed for per-facet radius / tilt)."""
+    per_facet: one volume per facet (needed for per-facet radius / tilt).
+    surfaces: the facets' front surface gets a border condition, 0.5 deg
+    Gaussian roughness and an aluminium coating under 25 nm of silica
+    (constant indices n = 0.49 + 4.86 i and 1.46; the default table)."""
     mgr = AOpticsManager("manager", "Davies-Cotton")
     mgr.DisableFresnelReflection(True)
     world = AOpticalComponent("world", TGeoBBox("boxWorld", 20 * m, 20 * m, 20 * m))
@@ -161,6 +164,16 @@ def c2_scene(deterministic=False, per_facet=False, facet_radius=None, facet_tilt
             extra.rot = _small_rotation(tip, tilt)
             mat.rot = mat.rot @ extra.rot
         world.AddNode(mirror, i + 1, mat)
+    if surfaces:
+        from .geometry import ABorderSurfaceCondition
+        from .multilayer import AMultilayer
+        if per_facet:
+            raise ValueError("c2_scene: surfaces apply to the shared facet volume")
+        coat = AMultilayer(ARefractiveIndex(1.0), ARefractiveIndex(0.49, 4.86))
+        coat.InsertLayer(ARefractiveIndex(1.46), 25 * nm)
+        cond = ABorderSurfaceCondition(world, shared)
+        cond.SetGaussianRoughness(math.radians(0.5))
+        cond.SetMultilayer(coat)
     zc = C2_F + camera_dz
     cam = AFocalSurface("focalPlane", TGeoTube("tubeCamera", 0, C2_CAMERA_R, 1 * mm))
     world.AddNode(cam, 1, TGeoTranslation(0, 0, zc + 1 * mm))

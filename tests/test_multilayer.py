@@ -307,3 +307,22 @@ def test_gpu_coated_scene_per_ray(ctx, precalc):
     O.trace(mgr.flat(), o, mgr.trace_opts(), nthreads=8)
     compare(g, o)
     assert (g["status"] == A.RB_ABSORB).sum() > 1000
+
+
+@pytest.mark.gpu
+def test_gpu_c2_rough_coated_facets_per_ray(ctx):
+    """The benchmarked coated Davies-Cotton (0.5 deg roughness, Al under 25 nm
+    of silica on every facet) matches the oracle per ray."""
+    from helpers import compare
+    from oracle import oracle as O
+    from robast_amd import configs
+    from robast_amd._lib import make_rays
+    print("C2 with rough, coated facets: 1e5 rays per ray", flush=True)
+    mgr = configs.c2_scene(surfaces=True)
+    src = O.generate(configs.c2_raygen(), 100_000)
+    g = {k: v.copy() for k, v in src.items()}
+    o = {k: v.copy() for k, v in src.items()}
+    ctx.trace(ctx.upload(mgr.flat()), make_rays(g), 100_000, mgr.trace_opts())
+    O.trace(mgr.flat(), o, mgr.trace_opts(), nthreads=8)
+    compare(g, o)
+    assert (g["status"] == A.RB_FOCUS).sum() > 50_000 and (g["status"] == A.RB_ABSORB).sum() > 5_000
