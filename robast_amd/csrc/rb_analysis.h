@@ -5,8 +5,8 @@
 // The search reads the histogram only through SumInRadius, an integer count
 // (summed exactly in 64 bits), and is written without FMA contraction, so
 // that given the same counts and statistics it retraces a plain FP64 CPU
-// evaluation of the reference's loop step for step. SumInRadius runs over
-// the non-empty bins only, compacted row by row beforehand.
+// evaluation of the reference's loop step for step. SumInRadius reads each
+// row's bins in the circle as one run of columns, from per-row prefix sums.
 #pragma once
 
 #include "rb_device.h"
@@ -33,48 +33,150 @@ __device__ inline double hist_center(int b, int n, double lo, double hi) {
   return lo + (b - 1) * w + 0.5 * w;
 }
 
-// Sum over the block of one 64-bit value per thread; every thread gets it.
-__device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+// Sum over the wavefront of one 64-bit value per lane; every lane gets it.
+__device__ inline uint64_t wave_sum_u64(uint64_t v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned lo = __shfl_xor(unsigned(v), off, 64), hi = __shfl_xor(unsigned(v >> 32), off, 64);
+    v += (uint64_t(hi) << 32) | lo;
   }
-  const uint64_t t = red[0];
+  return v;
+}
+
+// Sum over the block (whole wavefronts) of one 64-bit value per thread:
+// wavefront sums by shuffles, then one LDS slot per wavefront. Every thread
+// gets the total. red holds blockDim.x / 64 entries.
+__device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* red) {
+  v = wave_sum_u64(v);
+  const int waves = int(blockDim.x) >> 6;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint64_t t = 0;
+  for (int k = 0; k < waves; k++) t += red[k];
   __syncthreads();
   return t;
 }
 
-// The non-empty bins, in bin order (row by row), with the first entry of
-// each row: SumInRadius visits only the rows that can reach the circle.
-struct SparseHist {
-  const int32_t* bin;        // bin index (by - 1) * nx + bx - 1 of entry e
-  const uint32_t* count;     // its content
-  const int32_t* row_start;  // first entry of row by - 1; row_start[ny] = entries
+// Per-row inclusive prefix sums of the bin contents: p[(by - 1) nx + bx - 1]
+// = contents of bins 1..bx of row by.
+struct RowPrefix {
+  const uint64_t* p;
 };
 
+// One SumInRadius call's circle, with the column geometry it needs: bin
+// centres cx(b) = xlo + (b - 1) w + 0.5 w as hist_center computes them, and
+// a = the number of columns whose centre lies left of x.
+struct Circle {
+  double x, y, r2, xlo, wx, inv_wx;
+  int nx, a;
+  __device__ double cx(int b) const {
+#pragma clang fp contract(off)
+    return xlo + (b - 1) * wx + 0.5 * wx;
+  }
+};
+
+// Contents of the bins of row `by` whose centres lie within r of (x, y), by
+// the reference's test d2 = (cx - x)^2 + (cy - y)^2 <= r^2 in FP64. Along a
+// row fl(cx - x) is monotone in the column, so the test holds on one run of
+// columns: on the columns left of x (1..a) it turns true once, on those from
+// a + 1 it turns false once. Each end is estimated from sqrt(r^2 - dy^2) and
+// settled with the exact test (a few steps, or a binary search past 8); two
+// prefix sums give the run's contents.
+__device__ inline uint64_t row_in_radius(const RowPrefix& P, const HistParams& H, const Circle& C, int by) {
+#pragma clang fp contract(off)
+  const double cy = hist_center(by, H.ny, H.ylo, H.yhi);
+  const double dy2 = (cy - C.y) * (cy - C.y);
+  if (!(dy2 <= C.r2)) return 0;  // (cx - x)^2 + dy2 >= dy2 > r^2 for every column
+  auto in = [&](int b) {
+    const double d = C.cx(b) - C.x;
+    return d * d + dy2 <= C.r2;
+  };
+  const double half = sqrt(C.r2 - dy2);
+  const int a = C.a, nx = C.nx;
+  // first inside column of 1..a (a + 1: none); the test is false..true there
+  double est = ceil((C.x - half - C.xlo) * C.inv_wx + 0.5);
+  int f = est < 1.0 ? 1 : est > double(a + 1) ? a + 1 : int(est);
+  if (f <= a && in(f)) {
+    int k = 0;
+    while (k < 8 && f > 1 && in(f - 1)) { f--; k++; }
+    if (k == 8 && f > 1 && in(f - 1)) {
+      int lo = 1, hi = f - 1;  // the first true lies in [lo, hi]
+      while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (in(mid)) hi = mid; else lo = mid + 1;
+      }
+      f = lo;
+    }
+  } else {
+    int k = 0;
+    while (k < 8 && f <= a && !in(f)) { f++; k++; }
+    if (f <= a && !in(f)) {
+      int lo = f + 1, hi = a + 1;  // the first true lies in [lo, hi] (a + 1: none)
+      while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (in(mid)) hi = mid; else lo = mid + 1;
+      }
+      f = lo;
+    }
+  }
+  // last inside column of a + 1..nx (a: none); the test is true..false there
+  est = floor((C.x + half - C.xlo) * C.inv_wx + 0.5);
+  int l = est < double(a) ? a : est > double(nx) ? nx : int(est);
+  if (l >= a + 1 && in(l)) {
+    int k = 0;
+    while (k < 8 && l < nx && in(l + 1)) { l++; k++; }
+    if (k == 8 && l < nx && in(l + 1)) {
+      int lo = l + 1, hi = nx;  // the last true lies in [lo, hi]
+      while (lo < hi) {
+        const int mid = hi - (hi - lo) / 2;
+        if (in(mid)) lo = mid; else hi = mid - 1;
+      }
+      l = lo;
+    }
+  } else {
+    int k = 0;
+    while (k < 8 && l >= a + 1 && !in(l)) { l--; k++; }
+    if (l >= a + 1 && !in(l)) {
+      int lo = a, hi = l - 1;  // the last true lies in [lo, hi] (a: none)
+      while (lo < hi) {
+        const int mid = hi - (hi - lo) / 2;
+        if (in(mid)) lo = mid; else hi = mid - 1;
+      }
+      l = lo;
+    }
+  }
+  const int b0 = f <= a ? f : a + 1, b1 = l >= a + 1 ? l : a;
+  if (b1 < b0) return 0;
+  const uint64_t* row = P.p + int64_t(by - 1) * nx;
+  return row[b1 - 1] - (b0 > 1 ? row[b0 - 2] : 0);
+}
+
 // SumInRadius (src/AGeoUtil.cxx:18-42): contents of the bins whose centres
-// lie within r of (x, y). All threads of the block call it together. Rows
-// whose centres lie more than r (plus one bin) from y hold no such bin and
-// are skipped; every other non-empty bin gets the reference's test, so the
-// sum is the full scan's.
-__device__ inline double sum_in_radius(const SparseHist& Hs, const HistParams& H, double x, double y, double r,
+// lie within r of (x, y). All threads of the block call it together and
+// share the rows; rows whose centres lie more than r (plus one bin) from y
+// hold no such bin and are skipped. The sum is the full scan's.
+__device__ inline double sum_in_radius(const RowPrefix& P, const HistParams& H, double x, double y, double r,
                                        uint64_t* red) {
 #pragma clang fp contract(off)
   const double r2 = r * r;
+  if (!(x == x) || !(y == y) || !(r2 == r2)) return 0.0;  // NaN: the reference's test fails for every bin
+  Circle C;
+  C.x = x;
+  C.y = y;
+  C.r2 = r2;
+  C.xlo = H.xlo;
+  C.wx = (H.xhi - H.xlo) / double(H.nx);
+  C.inv_wx = 1.0 / C.wx;
+  C.nx = H.nx;
+  const double ea = floor((x - H.xlo) * C.inv_wx + 0.5);  // columns with cx < x, then settled exactly
+  int a = ea < 0.0 ? 0 : ea > double(H.nx) ? H.nx : int(ea);
+  while (a < H.nx && C.cx(a + 1) < x) a++;
+  while (a > 0 && !(C.cx(a) < x)) a--;
+  C.a = a;
   const double w = (H.yhi - H.ylo) / double(H.ny);
   const double lo = fmin(fmax(floor((y - r - H.ylo) / w + 0.5) - 1.0, 1.0), double(H.ny));
   const double hi = fmin(fmax(ceil((y + r - H.ylo) / w + 0.5) + 1.0, 1.0), double(H.ny));
-  const int e0 = Hs.row_start[int(lo) - 1], e1 = Hs.row_start[int(hi)];
   uint64_t acc = 0;
-  for (int e = e0 + int(threadIdx.x); e < e1; e += blockDim.x) {
-    const int b = Hs.bin[e];
-    const int iy = b / H.nx, ix = b - iy * H.nx;
-    const double cx = hist_center(ix + 1, H.nx, H.xlo, H.xhi), cy = hist_center(iy + 1, H.ny, H.ylo, H.yhi);
-    const double d2 = (cx - x) * (cx - x) + (cy - y) * (cy - y);
-    if (d2 <= r2) acc += Hs.count[e];
-  }
+  for (int by = int(lo) + int(threadIdx.x); by <= int(hi); by += int(blockDim.x)) acc += row_in_radius(P, H, C, by);
   return double(block_sum_u64(acc, red));
 }
 
@@ -84,12 +186,12 @@ __device__ inline double sum_in_radius(const SparseHist& Hs, const HistParams& H
 constexpr int kMaxClimb = 1 << 16;
 
 // ContainmentRadius (src/AGeoUtil.cxx:198-308), run by every thread of one
-// block on the same (uniform) values; st = {sum w, x, x^2, y, y^2} of the fill
-// (TH1::GetMean / GetStdDev), integral = TH1::Integral. out = {r, x, y}.
-__device__ inline void containment_search(const SparseHist& Hs, const HistParams& H, const double* st,
+// block on the same (uniform) values; st = {sum w, x, x^2, y, y^2} of the
+// fill (TH1::GetMean / GetStdDev), integral = TH1::Integral. out = {r, x, y}.
+__device__ inline void containment_search(const RowPrefix& P, const HistParams& H, const double* st,
                                           double integral, double fraction, uint64_t* red, double* out) {
 #pragma clang fp contract(off)
-  auto sir = [&](double px, double py, double pr) { return sum_in_radius(Hs, H, px, py, pr, red); };
+  auto sir = [&](double px, double py, double pr) { return sum_in_radius(P, H, px, py, pr, red); };
   int no_shift = 0, no_stable = 0;
   const double sw = st[0];
   double x = sw == 0 ? 0. : st[1] / sw;

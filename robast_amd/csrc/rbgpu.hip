@@ -633,7 +633,7 @@ __global__ void __launch_bounds__(kSourceBlock) k_bunch_expand(BunchIn B, const 
 
 // ---------------------------------------------------------------- spot analysis
 constexpr int kHistBlock = 256;
-constexpr int kSearchBlock = 1024;
+constexpr int kSearchBlock = 512;  // 8 wavefronts: about one histogram row per thread per sum
 
 // TH2::Fill of the terminal (x, y) of the rays with status H.status: one
 // 32-bit atomic per in-range entry. Each block also sums its entries'
@@ -668,30 +668,36 @@ __global__ void __launch_bounds__(kHistBlock) k_hist2d(const double* x, const do
 
 // Compaction of the non-empty bins (after an exclusive scan of the flags):
 // entry pos[b] holds bin b; row_start[by - 1] is the first entry of row by.
-__global__ void __launch_bounds__(kHistBlock) k_hist_flags(const uint32_t* counts, int nb, int32_t* flag) {
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) flag[b] = counts[b] != 0u;
-}
-__global__ void __launch_bounds__(kHistBlock) k_hist_compact(const uint32_t* counts, const int32_t* pos, int nb,
-                                                             int nx, int ny, int32_t* bin, uint32_t* count,
-                                                             int32_t* row_start) {
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
-    const uint32_t c = counts[b];
-    if (c != 0u) {
-      bin[pos[b]] = b;
-      count[pos[b]] = c;
+// Row prefix sums of the histogram: one wavefront per row, an inclusive
+// 64-lane scan per chunk of 64 columns carried along the row.
+__global__ void __launch_bounds__(kHistBlock) k_hist_rowprefix(const uint32_t* counts, int nx, int ny,
+                                                               uint64_t* prefix) {
+  const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6, waves = int(blockDim.x) >> 6;
+  for (int row = int(blockIdx.x) * waves + wave; row < ny; row += int(gridDim.x) * waves) {
+    const uint32_t* c = counts + int64_t(row) * nx;
+    uint64_t* p = prefix + int64_t(row) * nx;
+    uint64_t carry = 0;
+    for (int base = 0; base < nx; base += 64) {
+      const int col = base + lane;
+      uint64_t v = col < nx ? uint64_t(c[col]) : 0;
+      for (int off = 1; off < 64; off <<= 1) {
+        const unsigned lo = __shfl_up(unsigned(v), off, 64), hi = __shfl_up(unsigned(v >> 32), off, 64);
+        if (lane >= off) v += (uint64_t(hi) << 32) | lo;
+      }
+      if (col < nx) p[col] = carry + v;
+      const unsigned lo = __shfl(unsigned(v), 63, 64), hi = __shfl(unsigned(v >> 32), 63, 64);
+      carry += (uint64_t(hi) << 32) | lo;
     }
-    if (b % nx == 0) row_start[b / nx] = pos[b];
-    if (b == nb - 1) row_start[ny] = pos[b] + (c != 0u ? 1 : 0);
   }
 }
 
 // One block: the statistics (block partials added in block order), the
 // integral, then the containment search. out = {r, x, y, sum w, sum x,
 // sum x^2, sum y, sum y^2}.
-__global__ void __launch_bounds__(kSearchBlock) k_containment(SparseHist Hs, const double* partial, int nparts,
+__global__ void __launch_bounds__(kSearchBlock) k_containment(RowPrefix P, const double* partial, int nparts,
                                                               HistParams H, double fraction, double* out) {
-  __shared__ uint64_t red[kSearchBlock];
   __shared__ double st_sh[5];
+  __shared__ uint64_t red[kSearchBlock / 64];
   if (threadIdx.x == 0) {
     double t[5] = {0., 0., 0., 0., 0.};
     for (int b = 0; b < nparts; b++)
@@ -701,12 +707,11 @@ __global__ void __launch_bounds__(kSearchBlock) k_containment(SparseHist Hs, con
   __syncthreads();
   double st[5];
   for (int k = 0; k < 5; k++) st[k] = st_sh[k];
-  const int ne = Hs.row_start[H.ny];
   uint64_t acc = 0;
-  for (int e = threadIdx.x; e < ne; e += blockDim.x) acc += Hs.count[e];
+  for (int row = threadIdx.x; row < H.ny; row += blockDim.x) acc += P.p[int64_t(row) * H.nx + H.nx - 1];
   const double integral = double(block_sum_u64(acc, red));
   double res[3];
-  containment_search(Hs, H, st, integral, fraction, red, res);
+  containment_search(P, H, st, integral, fraction, red, res);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 3; k++) out[k] = res[k];
     for (int k = 0; k < 5; k++) out[3 + k] = st[k];
@@ -1364,30 +1369,16 @@ int rbgpu_containment_radius(rb_ctx* ctx, const rb_rays* rays, int64_t n, const 
                        partial);
     HIPCHK(ctx, hipGetLastError());
   }
-  // non-empty bins, row by row
-  int32_t *flag, *pos, *sbin, *row_start;
-  uint32_t* scount;
-  HIPCHK(ctx, dev.alloc(&flag, nb));
-  HIPCHK(ctx, dev.alloc(&pos, nb));
-  HIPCHK(ctx, dev.alloc(&sbin, nb));
-  HIPCHK(ctx, dev.alloc(&scount, nb));
-  HIPCHK(ctx, dev.alloc(&row_start, int64_t(h->ny) + 1));
-  size_t scan_bytes = 0;
-  HIPCHK(ctx, rocprim::exclusive_scan(nullptr, scan_bytes, flag, pos, int32_t(0), size_t(nb), rocprim::plus<int32_t>(),
-                                      ctx->stream));
-  char* scan_tmp;
-  HIPCHK(ctx, dev.alloc(&scan_tmp, int64_t(scan_bytes)));
-  const int cgrid = std::max(1, std::min((nb + kHistBlock - 1) / kHistBlock, ctx->cus * 8));
-  hipLaunchKernelGGL(k_hist_flags, dim3(unsigned(cgrid)), dim3(kHistBlock), 0, ctx->stream, dcounts, nb, flag);
+  // per-row prefix sums of the contents
+  uint64_t* prefix;
+  HIPCHK(ctx, dev.alloc(&prefix, nb));
+  const int pgrid = std::max(1, std::min((h->ny + 3) / 4, ctx->cus * 8));
+  hipLaunchKernelGGL(k_hist_rowprefix, dim3(unsigned(pgrid)), dim3(kHistBlock), 0, ctx->stream, dcounts, h->nx, h->ny,
+                     prefix);
   HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, rocprim::exclusive_scan(scan_tmp, scan_bytes, flag, pos, int32_t(0), size_t(nb), rocprim::plus<int32_t>(),
-                                      ctx->stream));
-  hipLaunchKernelGGL(k_hist_compact, dim3(unsigned(cgrid)), dim3(kHistBlock), 0, ctx->stream, dcounts, pos, nb, h->nx,
-                     h->ny, sbin, scount, row_start);
-  HIPCHK(ctx, hipGetLastError());
-  SparseHist Hs{sbin, scount, row_start};
+  RowPrefix Pr{prefix};
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-  hipLaunchKernelGGL(k_containment, dim3(1), dim3(kSearchBlock), 0, ctx->stream, Hs, partial, grid, H, fraction,
+  hipLaunchKernelGGL(k_containment, dim3(1), dim3(kSearchBlock), 0, ctx->stream, Pr, partial, grid, H, fraction,
                      dout);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
@@ -1402,7 +1393,7 @@ int rbgpu_containment_radius(rb_ctx* ctx, const rb_rays* rays, int64_t n, const 
   HIPCHK(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
   ctx->last_ms = ms;
   ctx->kernel_ms = kms;
-  ctx->launches = n > 0 ? 5 : 4;  // fill, flags, scan, compaction, search
+  ctx->launches = n > 0 ? 3 : 2;  // fill, row prefix sums, search
   for (int k = 0; k < 3; k++) out[k] = res[k];
   if (stats)
     for (int k = 0; k < 5; k++) stats[k] = res[3 + k];
