@@ -266,7 +266,7 @@ def main():
         search_ms.append(ctx.last_kernel_ms())
     spot = {"d80_radius_cm": round(d80[0], 4), "centre_cm": [round(d80[1], 4), round(d80[2], 4)],
             "device_ms": round(min(spot_ms), 3), "search_ms": round(min(search_ms), 3),
-            "fill_and_compact_ms": round(min(s - k for s, k in zip(spot_ms, search_ms)), 3),
+            "fill_and_prefix_ms": round(min(s - k for s, k in zip(spot_ms, search_ms)), 3),
             "workload": "AGeoUtil::ContainmentRadius(0.80) of the focused rays of the batch (all field angles), "
                         "TH2D 1000 x 1000 bins over +-120 cm"}
     sources = sources_bench(ctx, torch, A, rays, N)
