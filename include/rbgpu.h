@@ -98,6 +98,9 @@ typedef struct {
   int32_t qe_angle_table;  /* focal surface: TGraph QE(angle [rad]), -1 = none   */
   int32_t border_first;    /* this volume's border-surface conditions:           */
   int32_t border_count;    /*   borders[border_first .. + border_count)          */
+  int32_t refl_grid;       /* mirror: TH2 R(lambda, angle) in grids (AMirror::SetReflectance(TH2),
+                              read with TH2::Interpolate before refl_table), -1 = none */
+  int32_t pad;
 } rb_volume_desc;
 
 /* ABorderSurfaceCondition(component1, component2) (include/ABorderSurfaceCondition.h:24-45),

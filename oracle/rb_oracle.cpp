@@ -1626,10 +1626,17 @@ static void do_reflection(double n1, Ray& ray, Nav& nv, int current, int next, c
   bool absorbed = false;
   if (kind_of(sc, next) == RB_KMIRROR) {
     double ref, trans;
-    if (bc && bc->multilayer >= 0)  // src/AOpticsManager.cxx:194-198: the coating, unclamped
+    const rb_volume_desc& mv = vol_of(sc, next);
+    if (bc && bc->multilayer >= 0) {  // src/AOpticsManager.cxx:194-198: the coating, unclamped
       multilayer_rt(sc.d, bc, ray.lambda, cos1, ref, trans);
-    else
-      ref = mirror_reflectance(sc.d, vol_of(sc, next), ray.lambda);
+    } else if (mv.refl_grid >= 0) {  // AMirror::GetReflectance (src/AMirror.cxx:39-60): the TH2 first, clamped
+      const rb_grid2d_desc& g = sc.d->grids[mv.refl_grid];
+      ref = th2_interpolate(sc.d->params + g.r_offset, g, ray.lambda, std::acos(cos1));
+      ref = ref > 1 ? 1 : ref;
+      ref = ref < 0 ? 0 : ref;
+    } else {
+      ref = mirror_reflectance(sc.d, mv, ray.lambda);
+    }
     if (ref < rng.uniform()) {
       absorbed = true;
       ray.status = RB_ABSORB;

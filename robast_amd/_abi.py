@@ -45,7 +45,8 @@ SHAPE_DTYPE = np.dtype(
 VOLUME_DTYPE = np.dtype(
     [("refl_const", "<f8"), ("shape", "<i4"), ("kind", "<i4"), ("refl_table", "<i4"),
      ("index_model", "<i4"), ("index_ref", "<i4"), ("k_table", "<i4"),
-     ("qe_lambda_table", "<i4"), ("qe_angle_table", "<i4"), ("border_first", "<i4"), ("border_count", "<i4")],
+     ("qe_lambda_table", "<i4"), ("qe_angle_table", "<i4"), ("border_first", "<i4"), ("border_count", "<i4"),
+     ("refl_grid", "<i4"), ("pad", "<i4")],
     align=True)
 BORDER_DTYPE = np.dtype([("volume2", "<i4"), ("lambertian", "<i4"), ("sigma", "<f8"), ("multilayer", "<i4"),
                          ("pad", "<i4")], align=True)
@@ -57,7 +58,7 @@ NODE_DTYPE = np.dtype(
 TABLE_DTYPE = np.dtype([("offset", "<i4"), ("n", "<i4")], align=True)
 
 assert SHAPE_DTYPE.itemsize == 64
-assert VOLUME_DTYPE.itemsize == 48
+assert VOLUME_DTYPE.itemsize == 56
 assert BORDER_DTYPE.itemsize == 24
 assert GRID_DTYPE.itemsize == 48
 assert NODE_DTYPE.itemsize == 112

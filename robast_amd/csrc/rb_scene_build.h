@@ -194,6 +194,8 @@ int validate(std::string& err, const rb_scene_desc* d) {
     if ((v.index_model == RB_INDEX_SELLMEIER || v.index_model == RB_INDEX_SCHOTT) &&
         (v.index_ref < 0 || v.index_ref + 6 > d->n_params))
       return fail_msg(err, -1, "scene: index formula parameters out of range");
+    if (v.refl_grid < -1 || v.refl_grid >= std::max(d->n_grids, 0))
+      return fail_msg(err, -1, "scene: mirror reflectance table out of range");
     if (v.border_count < 0 || v.border_first < 0 ||
         int64_t(v.border_first) + v.border_count > std::max(d->n_borders, 0) || (v.border_count > 0 && !d->borders))
       return fail_msg(err, -1, "scene: border-condition range out of bounds");
@@ -308,7 +310,10 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
   lay.table_xy = put(blob, d->table_xy, size_t(std::max(d->n_table_xy, 0)));
   lay.borders = put(blob, d->borders, size_t(std::max(d->n_borders, 0)));
   lay.grids = put(blob, d->grids, size_t(std::max(d->n_grids, 0)));
+  // the surface-extras kernel form also reads TH2 mirror reflectance
   lay.has_borders = d->n_borders > 0;
+  for (int i = 0; i < d->n_volumes; i++)
+    if (d->volumes[i].refl_grid >= 0) lay.has_borders = true;
   lay.n_nodes = d->n_nodes;
   lay.has_top_box = d->n_nodes > 0 && nodes[0].bvh_root >= 0;
   if (lay.has_top_box) {

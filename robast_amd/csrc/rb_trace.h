@@ -703,6 +703,12 @@ __device__ void do_reflection(const SC& S, const TraceParams& P, const RayIO& io
     ref = ref > 1 ? 1 : ref;
     ref = ref < 0 ? 0 : ref;
     if constexpr (SC::borders_on) {
+      if (v.refl_grid >= 0) {  // AMirror::GetReflectance's TH2 (src/AMirror.cxx:46-49), clamped
+        const rb_grid2d_desc& g = S.grids[v.refl_grid];
+        ref = th2_interpolate(S.params + g.r_offset, g, L.lambda, cold_acos(cos1));
+        ref = ref > 1 ? 1 : ref;
+        ref = ref < 0 ? 0 : ref;
+      }
       if (bc && bc->multilayer >= 0) {  // the coating's reflectance, unclamped (:194-198)
         double T;
         multilayer_rt(S, bc, L.lambda, cos1, ref, T);

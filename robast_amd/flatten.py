@@ -69,6 +69,18 @@ def flatten(top_volume):
     shapes, shape_ids, params = [], {}, []
     tables, table_ids, table_xy = [], {}, []
     volumes, volume_ids, volume_objs = [], {}, []
+    grid_recs, grid_ids = [], {}
+
+    def add_grid(key, nx, xlo, xhi, ny, ylo, yhi, r_values, t_values=None):
+        """A TH2-style table of bin-centre values (multilayer R and T, or a
+        mirror's TH2 reflectance), shared by every user of the same object."""
+        if key in grid_ids:
+            return grid_ids[key]
+        ro = add_params(np.ravel(r_values))
+        to = ro if t_values is None else add_params(np.ravel(t_values))
+        grid_ids[key] = len(grid_recs)
+        grid_recs.append((nx, xlo, xhi, ny, ylo, yhi, ro, to))
+        return grid_ids[key]
 
     def add_params(vals):
         off = len(params)
@@ -110,10 +122,13 @@ def flatten(top_volume):
         kind = volume_kind(vol)
         rec = dict(refl_const=1.0, shape=add_shape(vol.shape), kind=kind, refl_table=-1,
                    index_model=A.RB_INDEX_NONE, index_ref=-1, k_table=-1,
-                   qe_lambda_table=-1, qe_angle_table=-1)
+                   qe_lambda_table=-1, qe_angle_table=-1, refl_grid=-1)
         if kind == A.RB_KMIRROR:
             rec["refl_const"] = vol.reflectance
             rec["refl_table"] = add_table(vol.reflectance_graph)
+            h = getattr(vol, "reflectance_th2", None)
+            if h is not None:
+                rec["refl_grid"] = add_grid(id(h), h.nx, h.xlo, h.xhi, h.ny, h.ylo, h.yhi, h.contents())
         elif kind == A.RB_KLENS and vol.index is not None:
             ix = vol.index
             if isinstance(ix, ASellmeierFormula):
@@ -185,7 +200,7 @@ def flatten(top_volume):
     # a component2 that is not placed can never be entered (-2 matches nothing).
     # A multilayer becomes a (wavelength, angle) table of R and T in params,
     # shared by every condition that holds the same AMultilayer object.
-    border_recs, grid_recs, grid_ids = [], [], {}
+    border_recs = []
     for i, vol in enumerate(volume_objs):
         conds = getattr(vol, "border_conditions", [])
         vol_arr[i]["border_first"] = len(border_recs)
@@ -197,8 +212,7 @@ def flatten(top_volume):
                 g = grid_ids.get(id(c.multilayer))
                 if g is None:
                     nx, xlo, xhi, ny, ylo, yhi, R, T = c.multilayer.device_table()
-                    g = grid_ids[id(c.multilayer)] = len(grid_recs)
-                    grid_recs.append((nx, xlo, xhi, ny, ylo, yhi, add_params(np.ravel(R)), add_params(np.ravel(T))))
+                    g = add_grid(id(c.multilayer), nx, xlo, xhi, ny, ylo, yhi, R, T)
             border_recs.append((v2, 1 if c.lambertian else 0, c.sigma, g))
     borders = np.zeros(len(border_recs), dtype=A.BORDER_DTYPE)
     for i, (v2, lam, sig, g) in enumerate(border_recs):

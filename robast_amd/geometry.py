@@ -670,15 +670,25 @@ class AMirror(AOpticalComponent):
         self.reflectance_graph = None
 
     def SetReflectance(self, r):
+        """A constant, a TGraph R(lambda), or a TH2 R(lambda, angle [rad])
+        (robast_amd.geoutil.TH2D with SetBinContent); src/AMirror.cxx:39-60
+        reads the TH2 first, then the TGraph, then the constant."""
+        from .geoutil import TH2D
         from .materials import TGraph
         if isinstance(r, TGraph):
             self.reflectance_graph = r
+        elif isinstance(r, TH2D):
+            self.reflectance_th2 = r
         else:
             self.reflectance = float(r)
             self.reflectance_graph = None
 
     def GetReflectance(self, lam, angle=0.0):
-        r = self.reflectance_graph.Eval(lam) if self.reflectance_graph is not None else self.reflectance
+        h = getattr(self, "reflectance_th2", None)
+        if h is not None:
+            r = h.Interpolate(lam, angle)
+        else:
+            r = self.reflectance_graph.Eval(lam) if self.reflectance_graph is not None else self.reflectance
         return min(1.0, max(0.0, r))
 
 

@@ -12,6 +12,8 @@ GetStdDev, Integral and GetBinContent.
 """
 import math
 
+import numpy as np
+
 from . import _abi as A
 
 
@@ -24,7 +26,27 @@ class TH2D:
         self.ny, self.ylo, self.yhi = int(ny), float(ylo), float(yhi)
         self.counts = None   # [ny, nx] uint32 after ContainmentRadius
         self.stats = None    # {sum w, sum x, sum x^2, sum y, sum y^2}
+        self.values = None   # [ny, nx] float64 bin contents set with SetBinContent
         self._source = None
+
+    def SetBinContent(self, bx, by, v):
+        """Set bin (bx, by) (1-based; under/overflow bins are not kept)."""
+        if self.values is None:
+            self.values = np.zeros((self.ny, self.nx))
+        if 1 <= bx <= self.nx and 1 <= by <= self.ny:
+            self.values[by - 1, bx - 1] = float(v)
+
+    def contents(self):
+        """Bin contents [ny, nx]: those set with SetBinContent, else the fill's counts."""
+        if self.values is not None:
+            return self.values
+        self._filled()
+        return self.counts.astype(np.float64)
+
+    def Interpolate(self, x, y):
+        """TH2::Interpolate: bilinear between the nearest bin centres, 0 outside the axes."""
+        from .multilayer import th2_interpolate
+        return float(th2_interpolate(self.contents(), self.nx, self.xlo, self.xhi, self.ny, self.ylo, self.yhi, x, y))
 
     def FillRays(self, ctx, rays, n, status=A.RB_FOCUS, device_ptrs=False):
         """Fill with the terminal (x, y) of the n rays (rb_rays) with `status`;
@@ -47,9 +69,8 @@ class TH2D:
         return float(self.counts.sum(dtype="uint64"))
 
     def GetBinContent(self, bx, by):
-        self._filled()
         if 1 <= bx <= self.nx and 1 <= by <= self.ny:
-            return float(self.counts[by - 1, bx - 1])
+            return float(self.contents()[by - 1, bx - 1])
         return 0.0
 
     def GetMean(self, axis=1):

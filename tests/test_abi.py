@@ -34,7 +34,7 @@ def test_struct_sizes_match_binding():
     sizes = list(out)[:16]
     assert sizes[0] == 96                              # rb_matrix
     assert sizes[1] == A.SHAPE_DTYPE.itemsize
-    assert sizes[2] == A.VOLUME_DTYPE.itemsize
+    assert sizes[2] == A.VOLUME_DTYPE.itemsize == 56
     assert sizes[3] == A.NODE_DTYPE.itemsize
     assert sizes[4] == A.TABLE_DTYPE.itemsize
     assert sizes[5] == C.sizeof(A.rb_scene_desc)
