@@ -45,9 +45,24 @@ def _algorithmic_flops(key):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(mgr, gen, target_s=10.0):
+def vs_gpu(soa, gpu):
+    """Error of the GPU's FP64 results against the oracle on the rays both
+    traced (the same seeded rays, keyed by the same ids): status flips and
+    the focused-hit RMS and maximum distance."""
+    import numpy as np
+    m = min(len(soa["x"]), len(gpu["x"]))
+    so, sg = soa["status"][:m], gpu["status"][:m]
+    both = (so == 3) & (sg == 3)
+    d2 = sum((soa[k][:m] - gpu[k][:m]) ** 2 for k in ("x", "y", "z"))[both]
+    return {"rays": int(m), "status_flips": int((so != sg).sum()),
+            "focused_hit_rms_um": round(float(np.sqrt(d2.mean())) * 1e4 if d2.size else 0.0, 9),
+            "focused_hit_max_um": round(float(np.sqrt(d2.max())) * 1e4 if d2.size else 0.0, 9)}
+
+
+def cpu_baseline(mgr, gen, target_s=10.0, gpu_main=None, gpu_configs=None):
     """The CPU FP64 oracle (a restatement, not the ROOT-built reference) on a
-    bounded sample of the same workload, all usable host cores (max 16)."""
+    bounded sample of the same workload, all usable host cores (max 16); the
+    GPU's FP64 results on the same rays are checked against it (vs_gpu)."""
     from oracle import oracle as O
     cores = min(len(os.sched_getaffinity(0)), 16)
     flat = mgr.flat()
@@ -63,6 +78,8 @@ def cpu_baseline(mgr, gen, target_s=10.0):
     out = {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
            "sample": f"first {n} rays of the same seeded Davies-Cotton batch, oracle/rb_oracle.cpp, "
                      f"std::thread x {cores}, {dt:.1f} s"}
+    if gpu_main is not None:
+        out["vs_gpu"] = vs_gpu(soa, gpu_main)
     # the other configurations, on samples of about 2 s each
     from robast_amd import configs
     per = {}
@@ -76,11 +93,13 @@ def cpu_baseline(mgr, gen, target_s=10.0):
         t0 = time.perf_counter()
         O.trace(m.flat(), soa, m.trace_opts(), nthreads=cores)
         per[name] = {"value": round(k / (time.perf_counter() - t0) / 1e6, 4), "rays": k}
+        if gpu_configs and name in gpu_configs:
+            per[name]["vs_gpu"] = vs_gpu(soa, gpu_configs[name])
     out["configs"] = per
     return out
 
 
-def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3):
+def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3, hosts=None):
     """C1-C4 at 1e6 rays each, and C2 with rough, coated facets (border-surface
     kernel form): FP64 and mixed device Mrays/s, and the mixed form's error
     against the FP64 result (status flips, focused-hit RMS)."""
@@ -115,6 +134,8 @@ def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3)
                 best = ctx.last_trace_ms() if best is None else min(best, ctx.last_trace_ms())
             res[prec] = (best, work)
         a, b = res["fp64"][1], res["mixed"][1]
+        if hosts is not None:
+            hosts[name] = {k: a[k].cpu().numpy() for k in ("status", "x", "y", "z")}
         both = (a["status"] == A.RB_FOCUS) & (b["status"] == A.RB_FOCUS)
         d2 = (a["x"] - b["x"]) ** 2 + (a["y"] - b["y"]) ** 2 + (a["z"] - b["z"]) ** 2
         out[name] = {"rays": n_c, "fp64_mrays_s": round(n_c / res["fp64"][0] / 1e3, 2),
@@ -269,8 +290,9 @@ def main():
             "fill_and_prefix_ms": round(min(s - k for s, k in zip(spot_ms, search_ms)), 3),
             "workload": "AGeoUtil::ContainmentRadius(0.80) of the focused rays of the batch (all field angles), "
                         "TH2D 1000 x 1000 bins over +-120 cm"}
+    gpu_main = None if args.no_cpu_baseline else {k: a[k].cpu().numpy() for k in ("status", "x", "y", "z")}
     sources = sources_bench(ctx, torch, A, rays, N)
-    del src, work, res_o, rays
+    del src, work, res_o, rays, a, b
     torch.cuda.empty_cache()
 
     adjoint = None
@@ -295,7 +317,9 @@ def main():
                                "spot RMS^2 over 8 angles 0-3.5 deg, unit reflectance, FP64 forward, "
                                "adjoint FP32 per ray with FP64 accumulation"}
 
-    per_config = None if args.no_configs else config_sweep(ctx, torch, A, configs, make_rays_device)
+    gpu_configs = {}
+    per_config = None if args.no_configs else config_sweep(ctx, torch, A, configs, make_rays_device,
+                                                           hosts=gpu_configs)
 
     achieved = BYTES_PER_RAY * N / (kms * 1e-3) / 1e9
     traffic = _profile_traffic(f"k_trace_c2_{args.precision}_pmc.txt")
@@ -344,7 +368,7 @@ def main():
                                         "k_trace launch time; MI355X vector FP64 peak"}
     if not args.no_cpu_baseline:
         mgr.SetPrecision(False)
-        out["cpu_baseline"] = cpu_baseline(mgr, gen)
+        out["cpu_baseline"] = cpu_baseline(mgr, gen, gpu_main=gpu_main, gpu_configs=gpu_configs)
     print(json.dumps(out), flush=True)
 
 
