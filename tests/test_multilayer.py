@@ -14,8 +14,9 @@ zero reflectance at its design wavelength; R + T = 1 for lossless stacks and
 between bin centres, flat in the outer half bins, 0 outside, and the host
 and oracle readers agreeing bit for bit. The reference's
 testMirrorBoundaryMultilayer and testLensBoundaryMultilayer (:250-331, with
-its optical-constant files as fixtures) and a coated lens surface's
-reflect / transmit / absorb fractions run on the oracle and the GPU.
+its optical-constant files as fixtures), its Fresnel-against-TMM check for an
+absorbing TiO2 lens (:162-183), and a coated lens surface's reflect /
+transmit / absorb fractions run on the oracle and the GPU.
 
 GPU: per-ray parity with the oracle on a scene with a coated mirror and a
 lens coated on both faces, with a coarse precalculated table (interpolation
@@ -326,3 +327,27 @@ def test_gpu_c2_rough_coated_facets_per_ray(ctx):
     O.trace(mgr.flat(), o, mgr.trace_opts(), nthreads=8)
     compare(g, o)
     assert (g["status"] == A.RB_FOCUS).sum() > 50_000 and (g["status"] == A.RB_ABSORB).sum() > 5_000
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fresnel_matches_bare_interface_tmm(backend):
+    """unittest_robast.py:162-183: DoFresnel's reflectance off an absorbing
+    TiO2 lens (its complex-index form, src/AOpticsManager.cxx:116-135) equals
+    the TMM of the bare air | TiO2 interface at 45 deg, within 3 sigma, at
+    200 and 800 nm; the trace stops after the lens is entered (SetLimit(3))."""
+    G._reset_registry()
+    mgr = make_the_world()
+    tio2 = material("TiO2")
+    lens = ALens("lens", TGeoBBox("lensbox", 0.5 * m, 0.5 * m, 0.5 * m))
+    lens.SetRefractiveIndex(tio2)
+    mgr.GetTopVolume().AddNode(lens, 1)
+    mgr.CloseGeometry()
+    mgr.SetLimit(3)
+    layer = AMultilayer(ARefractiveIndex(1.0, 0.0), tio2)
+    N = 20000
+    for wl in (200 * nm, 800 * nm):
+        ref, _ = layer.CoherentTMMMixed(45 * deg, wl)
+        arr = rays_45(N, wl)
+        run(mgr, arr, backend)
+        n = (arr.status == A.RB_EXIT).sum()
+        assert within_3sigma(n, N, ref), (wl, n, N * ref)
