@@ -15,7 +15,7 @@ from .geometry import (AFocalSurface, AGeoAsphericDisk, AGeoWinstonConePoly, ALe
                        TGeoTube, cm, m, mm, nm, um)
 from .manager import AOpticsManager
 from .materials import ASellmeierFormula, ARefractiveIndex, TGraph
-from .rays import ARayArray, ARayShooter
+from .rays import ARayShooter
 
 # ------------------------------------------------------------------ C1
 C1_F = 8 * m

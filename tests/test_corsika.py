@@ -16,13 +16,11 @@ bit."""
 import math
 import os
 import struct
-import subprocess
 
 import numpy as np
 import pytest
 
 from robast_amd.corsika import ACorsikaIACTFile
-from robast_amd import _abi as A
 from robast_amd import sources as S
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
