@@ -171,12 +171,14 @@ def bunches_to_rays(bunches, opts):
     return soa
 
 
-def shape_query(flat, shape, p, d):
-    out = np.zeros(6)
+def shape_query(flat, shape, p, d, exit_normal=False):
+    """(inside, distance in from outside, distance out from inside, normal):
+    the normal at the entry crossing, or with exit_normal at the exit one."""
+    out = np.zeros(9)
     p = np.ascontiguousarray(p, dtype=float)
     d = np.ascontiguousarray(d, dtype=float)
     lib().oracle_shape_query(C.byref(flat.desc), shape, p.ctypes.data, d.ctypes.data, out.ctypes.data)
-    return bool(out[0]), out[1], out[2], out[3:6]
+    return bool(out[0]), out[1], out[2], (out[6:9] if exit_normal else out[3:6])
 
 
 def contains_batch(flat, shape, pts):

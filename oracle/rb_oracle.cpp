@@ -2100,6 +2100,9 @@ int oracle_shape_query(const rb_scene_desc* desc, int shape, const double* p, co
   out[2] = b.s;
   V3 n = shape_normal(sc, shape, a.leaf, axpy(q, a.s < kBig ? a.s : 0, dd), dd);
   out[3] = n.x; out[4] = n.y; out[5] = n.z;
+  // the normal where the ray leaves the shape (from an inside point)
+  V3 m = shape_normal(sc, shape, b.leaf, axpy(q, b.s < kBig ? b.s : 0, dd), dd);
+  out[6] = m.x; out[7] = m.y; out[8] = m.z;
   return 0;
 }
 
