@@ -275,6 +275,22 @@ void rbgpu_scene_free(rb_scene* scene);
 int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
                 rb_rays* rays, int64_t n);
 
+/* rbgpu_trace without any allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same kernels, same
+ * results ray for ray as rbgpu_trace. The ray arrays must be device memory
+ * (opts->device_ptrs set) and n <= RB_NOALLOC_MAX_RAYS, else -1 with a
+ * message. The coherence order's keys and indices and the persistent
+ * schedule's ray counter live in one file-scope device array of about
+ * 65 MiB (RB_NOALLOC_MAX_RAYS x 16 B + 1 MiB), which each GPU the library
+ * runs on holds until the process ends. Every call on a GPU shares that
+ * array: calls that overlap on one GPU (two contexts, two host threads)
+ * race. Calls no hipMalloc, hipFree or allocating library routine.
+ * Blocks until done. */
+#define RB_NOALLOC_MAX_RAYS 4194304
+int rbgpu_trace_noalloc(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
+                        rb_rays* rays, int64_t n);
+
 /* Fill n rays on the device (pointers in `rays` must be device pointers). */
 int rbgpu_generate_rays(rb_ctx* ctx, const rb_raygen_desc* gen, rb_rays* rays,
                         int64_t n);

@@ -20,7 +20,11 @@ EXPORTS = (
     "rbgpu_scene_free", "rbgpu_trace", "rbgpu_generate_rays", "rbgpu_last_trace_ms",
     "rbgpu_last_trace_launches", "rbgpu_abi_sizes", "rbgpu_adjoint", "rbgpu_last_kernel_ms",
     "rbgpu_containment_radius", "rbgpu_shooter_count", "rbgpu_shoot_rays", "rbgpu_bunches_to_rays",
+    "rbgpu_trace_noalloc",
 )
+
+# rbgpu_trace_noalloc's largest batch (RB_NOALLOC_MAX_RAYS in include/rbgpu.h)
+NOALLOC_MAX_RAYS = 4194304
 
 
 def load():
@@ -50,6 +54,9 @@ def load():
     lib.rbgpu_trace.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays),
                                 C.c_int64]
     lib.rbgpu_trace.restype = C.c_int
+    lib.rbgpu_trace_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays),
+                                        C.c_int64]
+    lib.rbgpu_trace_noalloc.restype = C.c_int
     lib.rbgpu_generate_rays.argtypes = [C.c_void_p, C.POINTER(A.rb_raygen_desc), C.POINTER(A.rb_rays), C.c_int64]
     lib.rbgpu_generate_rays.restype = C.c_int
     lib.rbgpu_last_trace_ms.argtypes = [C.c_void_p]
@@ -107,6 +114,12 @@ class Context:
     def trace(self, scene, rays, n, opts):
         self._check(self.lib.rbgpu_trace(self.h, scene.h, C.byref(opts), C.byref(rays), C.c_int64(n)),
                     "rbgpu_trace")
+
+    def trace_noalloc(self, scene, rays, n, opts):
+        """rbgpu_trace_noalloc: rbgpu_trace on device-resident rays with no
+        allocation (opts.device_ptrs must be set; n <= NOALLOC_MAX_RAYS)."""
+        self._check(self.lib.rbgpu_trace_noalloc(self.h, scene.h, C.byref(opts), C.byref(rays), C.c_int64(n)),
+                    "rbgpu_trace_noalloc")
 
     def generate(self, gen, rays, n):
         self._check(self.lib.rbgpu_generate_rays(self.h, C.byref(gen), C.byref(rays), C.c_int64(n)),

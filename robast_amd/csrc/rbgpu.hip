@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,8 @@ struct rb_ctx {
   size_t order_cap = 0;
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
+  void* scratch = nullptr;  // this device's copy of the allocation-free entry point's scratch array
+  hipError_t scratch_err = hipSuccess;
   int launches = 0;
   std::string err;
 };
@@ -719,10 +722,16 @@ __global__ void __launch_bounds__(kSearchBlock) k_containment(RowPrefix P, const
 }
 
 // ---------------------------------------------------------------- launches
+// Clears the persistent schedule's ray counter with a store from a kernel,
+// for the entry point whose callers may capture its stream into a graph.
+__global__ void k_clear_counter(unsigned long long* c) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *c = 0ull;
+}
+
 // The trace kernel for the scene's depth and the lane schedule.
 template <bool MIXED, int MAXD, bool BORDERS>
 static int launch_trace(rb_ctx* ctx, const rb_scene* scene, bool persistent, const RayIO& io, int64_t n,
-                        const TraceParams& P, const uint32_t* perm) {
+                        const TraceParams& P, const uint32_t* perm, unsigned long long* counter, bool clear_by_kernel) {
   const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
   if (persistent) {
@@ -730,9 +739,12 @@ static int launch_trace(rb_ctx* ctx, const rb_scene* scene, bool persistent, con
     HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                     &per_cu, (const void*)k_trace_persist<MIXED, MAXD, BORDERS>, block, 0));
     const int64_t pgrid = std::min<int64_t>(grid, int64_t(std::max(per_cu, 1)) * ctx->cus);
-    HIPCHK(ctx, hipMemsetAsync(ctx->next_ray, 0, sizeof(unsigned long long), ctx->stream));
+    if (clear_by_kernel)
+      hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(64), 0, ctx->stream, counter);
+    else
+      HIPCHK(ctx, hipMemsetAsync(counter, 0, sizeof(unsigned long long), ctx->stream));
     hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
-                       scene_of<MAXD, BORDERS>(scene), io, n, P, ctx->next_ray, perm);
+                       scene_of<MAXD, BORDERS>(scene), io, n, P, counter, perm);
   } else {
     hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<MAXD, BORDERS>(scene), io, n, P, perm);
@@ -766,6 +778,34 @@ static bool rays_accessible(const rb_rays* r, bool with_path) {
 // Batches at least this large are traced in coherence order by default.
 constexpr int64_t kAutoOrderMin = int64_t(1) << 20;
 
+// Sort bits of the coherence order of n rays: the tile index in the high
+// bits, as many Morton bits below it as fit in 32.
+struct OrderBits {
+  int mbits;
+  unsigned end_bit;
+};
+static OrderBits order_bits(int64_t n) {
+  unsigned tile_bits = 0;
+  while ((int64_t(1) << tile_bits) < ((n - 1) >> kOrderTileBits) + 1) tile_bits++;
+  const int mbits = std::min(30, 32 - int(tile_bits));
+  return {mbits, unsigned(mbits) + tile_bits};
+}
+
+// k_order_key over the rays: 8 blocks of 256 per CU, each striding over the rays.
+static int launch_order_key(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, int mbits,
+                            uint32_t* key, uint32_t* idx) {
+  float s[3];
+  for (int k = 0; k < 3; k++) {
+    const float ext = scene->top_hi[k] - scene->top_lo[k];
+    s[k] = ext > 0.0f ? 1023.0f / ext : 0.0f;
+  }
+  const int64_t grid = std::min<int64_t>((n + 255) / 256, int64_t(ctx->cus) * 8);
+  hipLaunchKernelGGL(k_order_key, dim3(unsigned(grid)), dim3(256), 0, ctx->stream, scene->s, io, n, scene->top_lo[0],
+                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], mbits, key, idx);
+  HIPCHK(ctx, hipGetLastError());
+  return 0;
+}
+
 // Coherence order of n rays: keys from k_order_key, then a radix sort of
 // (key, index) pairs. *perm receives the sorted indices (device).
 static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm) {
@@ -781,13 +821,9 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
   uint32_t* idx_in = reinterpret_cast<uint32_t*>(key_out + ctx->order_cap);
   uint32_t* idx_out = idx_in + ctx->order_cap;
   const unsigned count = unsigned(n);
-  // sort on the Morton bits and as many tile-index bits as there are tiles
-  unsigned tile_bits = 0;
-  while ((int64_t(1) << tile_bits) < ((n - 1) >> kOrderTileBits) + 1) tile_bits++;
-  const int mbits = std::min(30, 32 - int(tile_bits));
-  const unsigned begin_bit = 0u, end_bit = unsigned(mbits) + tile_bits;
+  const OrderBits ob = order_bits(n);
   size_t bytes = 0;
-  HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, idx_in, idx_out, count, begin_bit, end_bit,
+  HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, idx_in, idx_out, count, 0u, ob.end_bit,
                                         ctx->stream));
   if (bytes > ctx->sort_tmp_bytes) {
     if (ctx->sort_tmp) HIPCHK(ctx, hipFree(ctx->sort_tmp));
@@ -796,21 +832,153 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
     HIPCHK(ctx, hipMalloc(&ctx->sort_tmp, bytes));
     ctx->sort_tmp_bytes = bytes;
   }
-  float s[3];
-  for (int k = 0; k < 3; k++) {
-    const float ext = scene->top_hi[k] - scene->top_lo[k];
-    s[k] = ext > 0.0f ? 1023.0f / ext : 0.0f;
-  }
-  // 8 blocks of 256 per CU, each striding over the rays
-  const int64_t grid = std::min<int64_t>((n + 255) / 256, int64_t(ctx->cus) * 8);
-  hipLaunchKernelGGL(k_order_key, dim3(unsigned(grid)), dim3(256), 0, ctx->stream, scene->s, io, n, scene->top_lo[0],
-                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], mbits, key_in, idx_in);
-  HIPCHK(ctx, hipGetLastError());
+  const int rc = launch_order_key(ctx, scene, io, n, ob.mbits, key_in, idx_in);
+  if (rc) return rc;
   bytes = ctx->sort_tmp_bytes;
-  HIPCHK(ctx, rocprim::radix_sort_pairs(ctx->sort_tmp, bytes, key_in, key_out, idx_in, idx_out, count, begin_bit, end_bit,
+  HIPCHK(ctx, rocprim::radix_sort_pairs(ctx->sort_tmp, bytes, key_in, key_out, idx_in, idx_out, count, 0u, ob.end_bit,
                                         ctx->stream));
   *perm = idx_out;
   return 0;
+}
+
+// ---------------------------------------------------------------- allocation-free trace
+// rbgpu_trace_noalloc works in this file-scope device array instead of
+// allocating: the persistent schedule's ray counter (first 256 B), the
+// coherence order's keys and indices, double buffered (16 B per ray), and
+// the radix sort's temporary storage. One copy per GPU, held until the
+// process ends; every call on a GPU shares it.
+constexpr int64_t kNoAllocMaxRays = RB_NOALLOC_MAX_RAYS;
+static_assert(kNoAllocMaxRays == (int64_t(1) << 22), "scratch sized for 2^22 rays");
+constexpr size_t kScratchCounterBytes = 256;
+constexpr size_t kScratchOrderBytes = 4 * sizeof(uint32_t) * size_t(kNoAllocMaxRays);
+constexpr size_t kScratchSortBytes = size_t(1) << 20;
+constexpr size_t kScratchBytes = kScratchCounterBytes + kScratchOrderBytes + kScratchSortBytes;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The device address of `scratch` on each GPU, resolved once per device by
+// rbgpu_ctx_create with that device current. In host code the array's name
+// is not a device address; kernels and copies take this pointer.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static hipError_t resolve_scratch(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+// Coherence order inside the scratch array (at most kNoAllocMaxRays rays):
+// the double-buffered radix sort, whose temporary storage is only its
+// histograms, so the keys and indices need no second copy.
+static int order_rays_scratch(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm) {
+  unsigned char* base = static_cast<unsigned char*>(ctx->scratch) + kScratchCounterBytes;
+  uint32_t* key_a = reinterpret_cast<uint32_t*>(base);
+  uint32_t* key_b = key_a + kNoAllocMaxRays;
+  uint32_t* idx_a = key_b + kNoAllocMaxRays;
+  uint32_t* idx_b = idx_a + kNoAllocMaxRays;
+  void* tmp = base + kScratchOrderBytes;
+  rocprim::double_buffer<uint32_t> keys(key_a, key_b);
+  rocprim::double_buffer<uint32_t> idx(idx_a, idx_b);
+  const unsigned count = unsigned(n);
+  const OrderBits ob = order_bits(n);
+  size_t bytes = 0;
+  HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, keys, idx, count, 0u, ob.end_bit, ctx->stream));
+  if (bytes > kScratchSortBytes)
+    return fail(ctx, -2, "rbgpu_trace_noalloc: the radix sort needs " + std::to_string(bytes) +
+                             " B of temporary storage, more than the scratch array's " +
+                             std::to_string(kScratchSortBytes));
+  const int rc = launch_order_key(ctx, scene, io, n, ob.mbits, key_a, idx_a);
+  if (rc) return rc;
+  bytes = kScratchSortBytes;
+  HIPCHK(ctx, rocprim::radix_sort_pairs(tmp, bytes, keys, idx, count, 0u, ob.end_bit, ctx->stream));
+  *perm = idx.current();
+  return 0;
+}
+
+// Ordering, schedule and kernel form of one trace, enqueued on the
+// context's stream between its timing events. With noalloc the order
+// buffers and the ray counter live in the scratch array and nothing is
+// allocated; otherwise they are the context's own, grown on demand.
+static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const RayIO& io, int64_t n,
+                        const TraceParams& P, bool noalloc, bool* coherent_out) {
+  const bool mixed = opts->precision == RB_PRECISION_MIXED;
+  const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
+                        (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
+  const bool persistent = opts->schedule == RB_SCHEDULE_PERSISTENT ||
+                          (opts->schedule == RB_SCHEDULE_AUTO && scene->has_lens);
+  *coherent_out = coherent;
+  const uint32_t* perm = nullptr;
+  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (coherent) {
+    const int rc = noalloc ? order_rays_scratch(ctx, scene, io, n, &perm) : order_rays(ctx, scene, io, n, &perm);
+    if (rc) return rc;
+  }
+  unsigned long long* counter = nullptr;
+  if (noalloc) {
+    counter = static_cast<unsigned long long*>(ctx->scratch);
+  } else {
+    if (persistent && !ctx->next_ray) HIPCHK(ctx, hipMalloc(&ctx->next_ray, sizeof(unsigned long long)));
+    counter = ctx->next_ray;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
+  // kernels: composites at most one level deep and no border conditions;
+  // deeper composites; border conditions (the general form)
+  const int form = scene->has_borders ? 2 : scene->max_depth <= 1 ? 0 : 1;
+  int rc;
+  if (mixed)
+    rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
+       : form == 1 ? launch_trace<true, kMaxCsgDepth, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
+                   : launch_trace<true, kMaxCsgDepth, true>(ctx, scene, persistent, io, n, P, perm, counter, noalloc);
+  else
+    rc = form == 0 ? launch_trace<false, 1, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
+       : form == 1 ? launch_trace<false, kMaxCsgDepth, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
+                   : launch_trace<false, kMaxCsgDepth, true>(ctx, scene, persistent, io, n, P, perm, counter, noalloc);
+  if (rc) return rc;
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  return 0;
+}
+
+// Checks shared by the trace entry points; 1 = nothing to trace (n == 0).
+static int check_trace_args(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const rb_rays* rays,
+                            int64_t n, const char* who) {
+  const std::string w(who);
+  if (!scene || !opts || !rays || n < 0) return fail(ctx, -1, w + ": null argument or negative n");
+  if (opts->precision != RB_PRECISION_FP64 && opts->precision != RB_PRECISION_MIXED)
+    return fail(ctx, -1, w + ": unknown precision mode");
+  if (opts->order != RB_ORDER_AUTO && opts->order != RB_ORDER_INPUT && opts->order != RB_ORDER_COHERENT)
+    return fail(ctx, -1, w + ": unknown order");
+  if (opts->schedule != RB_SCHEDULE_AUTO && opts->schedule != RB_SCHEDULE_LANE &&
+      opts->schedule != RB_SCHEDULE_PERSISTENT)
+    return fail(ctx, -1, w + ": unknown schedule");
+  ctx->launches = 0;
+  ctx->last_ms = 0;
+  ctx->kernel_ms = 0;
+  if (n == 0) return 1;  // an empty batch: nothing to read, arrays may be null
+  if (!rays->x || !rays->y || !rays->z || !rays->t || !rays->dx || !rays->dy || !rays->dz || !rays->lambda ||
+      !rays->status || !rays->npoints)
+    return fail(ctx, -1, w + ": missing ray array");
+  if (rays->path_capacity < 0 || (rays->path_capacity > 0 && (!rays->path_xyzt || !rays->path_node)))
+    return fail(ctx, -1, w + ": bad path record");
+  return 0;
+}
+
+static TraceParams trace_params(const rb_trace_opts* opts) {
+  TraceParams P;
+  P.seed = opts->seed;
+  P.id_base = opts->ray_id_base;
+  P.limit = opts->limit > 0 ? opts->limit : 100;
+  P.fresnel = opts->fresnel ? 1 : 0;
+  return P;
 }
 
 extern "C" {
@@ -830,6 +998,9 @@ int rbgpu_ctx_create(int device, rb_ctx** out) {
     rbgpu_ctx_destroy(c);
     return -2;
   }
+  // rbgpu_trace_noalloc's scratch on this device (the device is current);
+  // a failure is reported by that entry point and leaves the others usable
+  c->scratch_err = resolve_scratch(device, &c->scratch);
   *out = c;
   return 0;
 }
@@ -918,29 +1089,10 @@ void rbgpu_scene_free(rb_scene* s) {
 
 int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, rb_rays* rays, int64_t n) {
   if (!ctx) return -1;
-  if (!scene || !opts || !rays || n < 0) return fail(ctx, -1, "rbgpu_trace: null argument or negative n");
-  if (opts->precision != RB_PRECISION_FP64 && opts->precision != RB_PRECISION_MIXED)
-    return fail(ctx, -1, "rbgpu_trace: unknown precision mode");
-  if (opts->order != RB_ORDER_AUTO && opts->order != RB_ORDER_INPUT && opts->order != RB_ORDER_COHERENT)
-    return fail(ctx, -1, "rbgpu_trace: unknown order");
-  if (opts->schedule != RB_SCHEDULE_AUTO && opts->schedule != RB_SCHEDULE_LANE &&
-      opts->schedule != RB_SCHEDULE_PERSISTENT)
-    return fail(ctx, -1, "rbgpu_trace: unknown schedule");
-  ctx->launches = 0;
-  ctx->last_ms = 0;
-  ctx->kernel_ms = 0;
-  if (n == 0) return 0;  // an empty batch: nothing to read, arrays may be null
-  if (!rays->x || !rays->y || !rays->z || !rays->t || !rays->dx || !rays->dy || !rays->dz || !rays->lambda ||
-      !rays->status || !rays->npoints)
-    return fail(ctx, -1, "rbgpu_trace: missing ray array");
-  if (rays->path_capacity < 0 || (rays->path_capacity > 0 && (!rays->path_xyzt || !rays->path_node)))
-    return fail(ctx, -1, "rbgpu_trace: bad path record");
+  const int chk = check_trace_args(ctx, scene, opts, rays, n, "rbgpu_trace");
+  if (chk) return chk > 0 ? 0 : chk;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  TraceParams P;
-  P.seed = opts->seed;
-  P.id_base = opts->ray_id_base;
-  P.limit = opts->limit > 0 ? opts->limit : 100;
-  P.fresnel = opts->fresnel ? 1 : 0;
+  const TraceParams P = trace_params(opts);
 
   RayIO io;
   DevRays dev;
@@ -982,35 +1134,9 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
     io.path_node = rays->path_node;
   }
 
-  const bool mixed = opts->precision == RB_PRECISION_MIXED;
-  const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
-                        (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
-  const bool persistent = opts->schedule == RB_SCHEDULE_PERSISTENT ||
-                          (opts->schedule == RB_SCHEDULE_AUTO && scene->has_lens);
-  const uint32_t* perm = nullptr;
-  HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  if (coherent) {
-    const int rc = order_rays(ctx, scene, io, n, &perm);
-    if (rc) return rc;
-  }
-  if (persistent && !ctx->next_ray) HIPCHK(ctx, hipMalloc(&ctx->next_ray, sizeof(unsigned long long)));
-  HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-  // kernels: composites at most one level deep and no border conditions;
-  // deeper composites; border conditions (the general form)
-  const int form = scene->has_borders ? 2 : scene->max_depth <= 1 ? 0 : 1;
-  int rc;
-  if (mixed)
-    rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, persistent, io, n, P, perm)
-       : form == 1 ? launch_trace<true, kMaxCsgDepth, false>(ctx, scene, persistent, io, n, P, perm)
-                   : launch_trace<true, kMaxCsgDepth, true>(ctx, scene, persistent, io, n, P, perm);
-  else
-    rc = form == 0 ? launch_trace<false, 1, false>(ctx, scene, persistent, io, n, P, perm)
-       : form == 1 ? launch_trace<false, kMaxCsgDepth, false>(ctx, scene, persistent, io, n, P, perm)
-                   : launch_trace<false, kMaxCsgDepth, true>(ctx, scene, persistent, io, n, P, perm);
+  bool coherent = false;
+  const int rc = trace_launch(ctx, scene, opts, io, n, P, false, &coherent);
   if (rc) return rc;
-  HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
-  HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   ctx->launches = coherent ? 3 : 1;  // key kernel + radix sort (counted once) + trace
 
   if (host) {
@@ -1026,6 +1152,42 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
       HIPCHK(ctx, hipMemcpyAsync(rays->path_node, io.path_node, ni * cap, hipMemcpyDeviceToHost, ctx->stream));
     }
   }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0, kms = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  HIPCHK(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
+  ctx->last_ms = ms;
+  ctx->kernel_ms = kms;
+  return 0;
+}
+
+int rbgpu_trace_noalloc(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, rb_rays* rays, int64_t n) {
+  if (!ctx) return -1;
+  const int chk = check_trace_args(ctx, scene, opts, rays, n, "rbgpu_trace_noalloc");
+  if (chk) return chk > 0 ? 0 : chk;
+  if (n > kNoAllocMaxRays)
+    return fail(ctx, -1, "rbgpu_trace_noalloc: " + std::to_string(n) + " rays exceed the scratch maximum of " +
+                             std::to_string(kNoAllocMaxRays) + " (use rbgpu_trace, or split the batch)");
+  if (!opts->device_ptrs)
+    return fail(ctx, -1, "rbgpu_trace_noalloc: the ray arrays must be device memory (set device_ptrs)");
+  if (ctx->scratch_err != hipSuccess || !ctx->scratch)
+    return fail(ctx, -2, std::string("rbgpu_trace_noalloc: scratch array address unresolved: ") +
+                             hipGetErrorString(ctx->scratch_err));
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!rays_accessible(rays, rays->path_capacity > 0))
+    return fail(ctx, -1, "rbgpu_trace_noalloc: a ray array is not GPU memory");
+  const TraceParams P = trace_params(opts);
+  RayIO io;
+  io.x = rays->x; io.y = rays->y; io.z = rays->z; io.t = rays->t;
+  io.dx = rays->dx; io.dy = rays->dy; io.dz = rays->dz; io.lambda = rays->lambda;
+  io.status = rays->status; io.npoints = rays->npoints; io.last_node = rays->last_node;
+  io.path_cap = int32_t(rays->path_capacity);
+  io.path_xyzt = rays->path_xyzt;
+  io.path_node = rays->path_node;
+  bool coherent = false;
+  const int rc = trace_launch(ctx, scene, opts, io, n, P, true, &coherent);
+  if (rc) return rc;
+  ctx->launches = coherent ? 3 : 1;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   float ms = 0, kms = 0;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

@@ -54,6 +54,7 @@ def test_struct_sizes_match_binding():
 def test_null_context_is_an_error_not_a_crash():
     lib = _lib.load()
     assert lib.rbgpu_trace(None, None, None, None, 0) == -1
+    assert lib.rbgpu_trace_noalloc(None, None, None, None, 0) == -1
     assert lib.rbgpu_scene_upload(None, None, None) == -1
     assert lib.rbgpu_last_error(None) == b"null context"
     assert lib.rbgpu_containment_radius(None, None, 0, None, 0.8, None, None, None) == -1
