@@ -28,6 +28,8 @@ def lib():
         L.oracle_trace.argtypes = [C.POINTER(A.rb_scene_desc), C.POINTER(A.rb_trace_opts), C.POINTER(A.rb_rays),
                                    C.c_int64, C.c_int]
         L.oracle_trace.restype = C.c_int
+        L.oracle_set_navigator.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p]
+        L.oracle_set_navigator.restype = None
         L.oracle_shape_query.argtypes = [C.POINTER(A.rb_scene_desc), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_shape_query.restype = C.c_int
         L.oracle_table_eval.argtypes = [C.POINTER(A.rb_scene_desc), C.c_int, C.c_double]
@@ -87,6 +89,34 @@ def rays_struct(soa):
         r.path_xyzt = A.ptr(soa["path_xyzt"])
         r.path_node = A.ptr(soa["path_node"])
     return r
+
+
+NAVIGATOR_DEFAULTS = (2e-6, 100.0, 1e-8)
+
+
+def set_navigator(step_back=0.0, push_scale=0.0, bool_push=0.0):
+    """Sets the oracle's navigator constants (0 keeps one): the step back
+    after a reflection (cm), the scale of the locate push after a crossing
+    (push = scale (1 + |T| + step) 1e-10 cm) and the boolean iteration's
+    inside-test offset (cm). Returns the constants in force."""
+    out = np.zeros(3)
+    lib().oracle_set_navigator(step_back, push_scale, bool_push, out.ctypes.data)
+    return tuple(float(v) for v in out)
+
+
+class navigator:
+    """Context manager: the oracle traces with other navigator constants
+    inside the block and with the defaults after it."""
+
+    def __init__(self, step_back=0.0, push_scale=0.0, bool_push=0.0):
+        self.args = (step_back, push_scale, bool_push)
+
+    def __enter__(self):
+        return set_navigator(*self.args)
+
+    def __exit__(self, *exc):
+        set_navigator(*NAVIGATOR_DEFAULTS)
+        return False
 
 
 def trace(flat, soa, opts, nthreads=1):

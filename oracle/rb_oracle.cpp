@@ -38,7 +38,13 @@ constexpr double kBig = 1e30;        // TGeoShape::Big()
 constexpr double kTol = 1e-10;       // TGeoShape::Tolerance()
 constexpr double kEpsilon = 1e-6;    // src/AOpticsManager.cxx:20
 constexpr double kC = 2.99792458e10; // TMath::C() * AOpticsManager::m(), cm/s
-constexpr double kBoolPush = 1e-8;   // inside-test offset of the boolean iteration
+// The navigator constants ROOT fixes internally, which only an import of
+// ROOT could pin. They are settable (oracle_set_navigator) so that the
+// sensitivity of the results to each can be measured; the defaults are the
+// rules the device follows.
+static double g_step_back = kEpsilon + 1e-6;  // TGeoNavigator::Step() after SetStep(kEpsilon), cm
+static double g_push_scale = 100.;             // CrossBoundaryAndLocate push: g_push_scale (1 + |T| + step) kTol
+static double g_bool_push = 1e-8;              // inside-test offset of the boolean iteration, cm
 constexpr double kPi = 3.14159265358979323846;
 
 // Operation counts for the algorithmic FLOP figure (oracle_trace_counts):
@@ -1010,7 +1016,7 @@ static Hit child_in(const Scene& sc, int child, const rb_matrix& m, V3 q, V3 d, 
 }
 
 // TGeoUnion/TGeoSubtraction/TGeoIntersection::DistFromOutside restated as
-// boundary-to-boundary iteration (inside tests taken kBoolPush past each point)
+// boundary-to-boundary iteration (inside tests taken g_bool_push past each point)
 static Hit comp_dist_out(const Scene& sc, int shape, V3 q, V3 d) {
   Comp c = comp_of(sc, shape);
   if (c.op == RB_OP_UNION) {
@@ -1022,7 +1028,7 @@ static Hit comp_dist_out(const Scene& sc, int shape, V3 q, V3 d) {
   uint32_t leaf = 0;
   for (int it = 0; it < 64; it++) {
     V3 x = axpy(q, s, d);
-    V3 xp = axpy(x, kBoolPush, d);
+    V3 xp = axpy(x, g_bool_push, d);
     bool inA = contains(sc, c.left, m2l(*c.lm, xp));
     bool inB = contains(sc, c.right, m2l(*c.rm, xp));
     double step;
@@ -1057,7 +1063,7 @@ static Hit comp_dist_out(const Scene& sc, int shape, V3 q, V3 d) {
         leaf = hb.leaf;
       }
     }
-    if (step < kBoolPush) step = kBoolPush;
+    if (step < g_bool_push) step = g_bool_push;
     s += step;
   }
   return {kBig, 0};
@@ -1079,7 +1085,7 @@ static Hit comp_dist_in(const Scene& sc, int shape, V3 q, V3 d) {
   uint32_t leaf = 0;
   for (int it = 0; it < 64; it++) {
     V3 x = axpy(q, s, d);
-    V3 xp = axpy(x, kBoolPush, d);
+    V3 xp = axpy(x, g_bool_push, d);
     bool inA = contains(sc, c.left, m2l(*c.lm, xp));
     bool inB = contains(sc, c.right, m2l(*c.rm, xp));
     if (!inA && !inB) return {s, leaf};
@@ -1087,7 +1093,7 @@ static Hit comp_dist_in(const Scene& sc, int shape, V3 q, V3 d) {
     if (inA) ha = child_in(sc, c.left, *c.lm, x, d, 0);
     if (inB) hb = child_in(sc, c.right, *c.rm, x, d, 1);
     Hit h = (inA && (!inB || ha.s >= hb.s)) ? ha : hb;
-    double step = h.s < kBoolPush ? kBoolPush : h.s;
+    double step = h.s < g_bool_push ? g_bool_push : h.s;
     leaf = h.leaf;
     s += step;
   }
@@ -1359,7 +1365,7 @@ static void init_track(Nav& nv, V3 p, V3 d) {
 static double locate_push(const Scene& sc, int n, double step) {
   const double* t = node(sc, n).global.tr;
   double trmax = 1. + std::fabs(t[0]) + std::fabs(t[1]) + std::fabs(t[2]);
-  return 100. * (trmax + step) * kTol;
+  return g_push_scale * (trmax + step) * kTol;
 }
 
 // TGeoNavigator::FindNextBoundaryAndStep(); returns the next node (-1 = null)
@@ -1454,7 +1460,7 @@ static V3 find_normal(const Nav& nv) {
 // TGeoNavigator::Step() after SetStep(kEpsilon): moves fStep + 1e-6 and relocates
 static void step_back(Nav& nv, V3 back_dir) {
   const Scene& sc = *nv.sc;
-  double s = kEpsilon + 1e-6;
+  double s = g_step_back;
   nv.p = axpy(nv.p, s, back_dir);
   int start = nv.cur < 0 ? 0 : nv.cur;
   if (nv.cur < 0) {
@@ -1887,7 +1893,7 @@ static bool dual_ray(const rb_scene_desc* d, const rb_adjoint_desc* A, V3 origin
     const int32_t* np = A->node_param + 3 * pnode[j];
     if (np[0] < 0 || np[1] < 0 || np[2] < 0) return false;
   }
-  const double eps = kEpsilon + 1e-6;
+  const double eps = g_step_back;
   DV3 o = dv(origin), din = dv(d0);
   for (int j = 0; j < K; j++) {
     const int32_t* np = A->node_param + 3 * pnode[j];
@@ -2028,6 +2034,20 @@ extern "C" {
 
 // Trace n rays with `nthreads` host threads in contiguous chunks of
 // (n / nthreads), the last taking the remainder (src/AOpticsManager.cxx:533-541).
+// Sets the navigator constants (a value <= 0 keeps the current one) and
+// returns the ones in force in out[3] = {step back, push scale, bool push}.
+// Not to be called while a trace runs.
+void oracle_set_navigator(double step_back, double push_scale, double bool_push, double* out) {
+  if (step_back > 0) g_step_back = step_back;
+  if (push_scale > 0) g_push_scale = push_scale;
+  if (bool_push > 0) g_bool_push = bool_push;
+  if (out) {
+    out[0] = g_step_back;
+    out[1] = g_push_scale;
+    out[2] = g_bool_push;
+  }
+}
+
 int oracle_trace(const rb_scene_desc* desc, const rb_trace_opts* opts, rb_rays* rays, int64_t n, int nthreads) {
   if (!desc || !opts || !rays || n < 0) return -1;
   Scene sc{desc, {}};

@@ -111,7 +111,7 @@ def main():
         if name == "c1":
             idx = np.nonzero(out["status"] == A.RB_FOCUS)[0][::100]
             for i in idx:
-                end, tt = c1_exact_chain(mgr.flat(), src, i)
+                end, tt = c1_exact_chain(mgr.flat(), src, i, 2e-6)
                 for k, key in enumerate(("x", "y", "z")):
                     assert abs(float(end[k]) - out[key][i]) < 1e-9
                 assert abs(float(tt) - out["t"][i]) < 1e-18

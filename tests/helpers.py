@@ -33,10 +33,11 @@ def counts(status):
     return np.bincount(np.asarray(status), minlength=6)
 
 
-def c1_exact_chain(flat, src, i):
+def c1_exact_chain(flat, src, i, step_back):
     """50-digit construction of ray i of a C1 (paraboloid) batch: the hit on
     the inner surface z = a r^2 + b + sag/2, the normal, the specular
-    reflection, the 2e-6 cm step back along -d1 and the focal-plane (z = f)
+    reflection, the step back along -d1 by `step_back` cm (the navigator's
+    move after a reflection, an input here) and the focal-plane (z = f)
     crossing. Returns (end point [x, y, z], arrival time)."""
     from robast_amd import configs
     mp.mp.dps = 50
@@ -64,7 +65,7 @@ def c1_exact_chain(flat, src, i):
     d2 = [d[k] - 2 * cs * n[k] for k in range(3)]
     m = mp.sqrt(sum(v * v for v in d2))
     d2 = [v / m for v in d2]
-    hb = [h[k] - mp.mpf("2e-6") * d[k] for k in range(3)]
+    hb = [h[k] - mp.mpf(step_back) * d[k] for k in range(3)]
     s2 = (f - hb[2]) / d2[2]
     end = [hb[k] + s2 * d2[k] for k in range(3)]
     return end, t / c + (s2 + mp.mpf("1e-10")) / c

@@ -152,20 +152,25 @@ def test_c1_on_axis_status_counts_closed_form():
     assert np.hypot(soa["x"][f], soa["y"][f]).max() < 1e-6
 
 
-@pytest.mark.parametrize("deg", [0.0, 1.5])
-def test_c1_reflection_chain_matches_exact_geometry(deg):
-    """Exact (50-digit) paraboloid hit, normal, specular reflection, the
-    2e-6 cm step back along -d1 and the focal-plane crossing, for a sample."""
+@pytest.mark.parametrize("deg, step_back", [(0.0, 2e-6), (1.5, 2e-6), (1.5, 1e-6), (0.5, 5e-6)])
+def test_c1_reflection_chain_matches_exact_geometry(deg, step_back):
+    """Exact (50-digit) paraboloid hit, normal, specular reflection, the step
+    back along -d1 and the focal-plane crossing, for a sample. The step back
+    is an input of both sides: the oracle is traced with each value (the
+    default 2e-6 cm and two others), so the geometry is checked, not the
+    constant."""
     mp.mp.dps = 50
     mgr = configs.c1_scene()
     flat = mgr.flat()
     soa = _c1_soa(deg)
     src = {k: v.copy() for k, v in soa.items()}
-    O.trace(flat, soa, mgr.trace_opts(), nthreads=8)
+    with O.navigator(step_back=step_back) as consts:
+        assert consts[0] == step_back
+        O.trace(flat, soa, mgr.trace_opts(), nthreads=8)
     idx = np.nonzero(soa["status"] == A.RB_FOCUS)[0][::997]
     assert len(idx) > 50
     for i in idx:
-        end, tt = c1_exact_chain(flat, src, i)
+        end, tt = c1_exact_chain(flat, src, i, step_back)
         for k, key in enumerate(("x", "y", "z")):
             assert abs(float(end[k]) - soa[key][i]) < 1e-9, (i, key, float(end[k]), soa[key][i])
         assert abs(float(tt) - soa["t"][i]) < 1e-18
