@@ -76,8 +76,8 @@ def cpu_baseline(mgr, gen, target_s=10.0, gpu_main=None, gpu_configs=None):
     O.trace(flat, soa, mgr.trace_opts(), nthreads=cores)
     dt = time.perf_counter() - t0
     out = {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
-           "sample": f"first {n} rays of the same seeded Davies-Cotton batch, oracle/rb_oracle.cpp, "
-                     f"std::thread x {cores}, {dt:.1f} s"}
+           "sample": f"{n} rays of the same seeded Davies-Cotton ray stream (the timed GPU batch is its first "
+                     f"1e7), oracle/rb_oracle.cpp, std::thread x {cores}, {dt:.1f} s"}
     if gpu_main is not None:
         out["vs_gpu"] = vs_gpu(soa, gpu_main)
     # the other configurations, on samples of about 2 s each
@@ -99,14 +99,21 @@ def cpu_baseline(mgr, gen, target_s=10.0, gpu_main=None, gpu_configs=None):
     return out
 
 
-def config_sweep(ctx, torch, A, configs, make_rays_device, n=1_000_000, steps=3, hosts=None):
-    """C1-C4 at 1e6 rays each, and C2 with rough, coated facets (border-surface
-    kernel form): FP64 and mixed device Mrays/s, and the mixed form's error
+# rays per configuration in the sweep: C2-C4 at the 1e7 rays their
+# workloads are specified at, C1 as a 1000 x 1000 grid (its own workload is
+# 4 angles x 99,856 rays)
+SWEEP_RAYS = {"c1": 1_000_000, "c2": 10_000_000, "c3": 10_000_000, "c4": 10_000_000, "c2_surfaces": 10_000_000}
+
+
+def config_sweep(ctx, torch, A, configs, make_rays_device, sizes=None, steps=3, hosts=None):
+    """C1-C4, and C2 with rough, coated facets (border-surface kernel form), at
+    SWEEP_RAYS rays: FP64 and mixed device Mrays/s, and the mixed form's error
     against the FP64 result (status flips, focused-hit RMS)."""
     import math
     dev = torch.device("cuda")
     out = {}
     for name in ("c1", "c2", "c3", "c4", "c2_surfaces"):
+        n = (sizes or SWEEP_RAYS)[name]
         mgr = {"c1": configs.c1_scene, "c2": configs.c2_scene, "c3": configs.c3_scene, "c4": configs.c4_scene,
                "c2_surfaces": lambda: configs.c2_scene(surfaces=True)}[name]()
         scene = ctx.upload(mgr.flat())

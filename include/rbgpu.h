@@ -190,7 +190,9 @@ typedef struct {
   int32_t* last_node;
   /* optional bounded path record (ARay's TGeoTrack points and node history):
    * path_capacity == 0 disables it; otherwise path_xyzt is [n][capacity][4]
-   * and path_node [n][capacity]; slot k holds point k+1 (point 0 is the input) */
+   * and path_node [n][capacity]; slot k holds the (k+1)-th point the trace
+   * adds (a ray entering with npoints = m: its point m + k); the points it
+   * held before the call are not repeated */
   int32_t path_capacity;
   int32_t pad;
   double* path_xyzt;

@@ -1478,14 +1478,16 @@ struct Ray {
   V3 d;      // direction (unit)
   double lambda;
   int status, npoints, last_node;
-  double* path;     // optional [cap][4]
+  double* path;     // optional [cap][4]: slot k holds the (k+1)-th point this trace adds
   int32_t* pnode;
   int cap;
+  int npoints_in;   // points the ray held when the trace began
   void add_point(V3 p, double tt, int nd) {
-    if (path && npoints - 1 < cap) {
-      double* s = path + 4 * (npoints - 1);
+    const int k = npoints - npoints_in;
+    if (path && k < cap) {
+      double* s = path + 4 * k;
       s[0] = p.x; s[1] = p.y; s[2] = p.z; s[3] = tt;
-      pnode[npoints - 1] = nd;
+      pnode[k] = nd;
     }
     x = p;
     t = tt;
@@ -1971,6 +1973,7 @@ static void dual_range(const Scene& sc, const Opts& o, const rb_trace_opts* opts
     ray.lambda = r->lambda[i];
     ray.status = RB_RUN;
     ray.npoints = 1;
+    ray.npoints_in = 1;
     ray.last_node = -1;
     ray.d = v3(r->dx[i], r->dy[i], r->dz[i]);
     ray.cap = kCap;
@@ -2015,6 +2018,7 @@ static void trace_range(const Scene& sc, const Opts& o, const rb_trace_opts* opt
     ray.lambda = r->lambda[i];
     ray.status = r->status[i];
     ray.npoints = r->npoints[i];
+    ray.npoints_in = ray.npoints;
     ray.last_node = r->last_node ? r->last_node[i] : -1;
     ray.d = v3(r->dx[i], r->dy[i], r->dz[i]);  // already unit, as ARay stores it
     ray.cap = r->path_capacity;

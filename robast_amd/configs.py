@@ -300,12 +300,14 @@ def c4_positions(rings=C4_RINGS, pitch=None):
     return pts
 
 
-def c4_scene(rings=C4_RINGS):
+def c4_scene(rings=C4_RINGS, deterministic=False):
     """BK7 window (Sellmeier; k from a 1 m absorption length at 400 nm, so the
     length scales with lambda) above a hexagonal array of hexagonal Winston
     cones, each a mirror block "pgon:rot30 - hexV" with wall reflectance 0.9
     and a thin hexagonal focal surface at its exit (HexWinstonCone.C mode 0).
-    The cone length follows from r1, r2: 2 dz = 40.1 mm."""
+    The cone length follows from r1, r2: 2 dz = 40.1 mm. deterministic: unit
+    wall reflectance, no absorption in the window and Fresnel reflection off,
+    so no ray draws a random number."""
     mgr = AOpticsManager("manager", "light guides")
     world = AOpticalComponent("world", TGeoBBox("worldbox", 1 * m, 1 * m, 1 * m))
     mgr.SetTopVolume(world)
@@ -318,7 +320,7 @@ def c4_scene(rings=C4_RINGS):
     pgon.DefineSection(2, -dz * 0.0, 0, C4_R1 * 0.99)
     pgon.DefineSection(3, dz * 0.999, 0, C4_R1 * 1.001)
     mirror = AMirror("coneMirror", TGeoCompositeShape("coneComp1", "pgon:rot30 - hexV"))
-    mirror.SetReflectance(C4_WALL_R)
+    mirror.SetReflectance(1.0 if deterministic else C4_WALL_R)
     pmt_shape = TGeoPgon("pgonPMT", 0, 360, 6, 2)
     pmt_shape.DefineSection(0, -dz - 0.01 * mm, 0, C4_R2 * 1.01)
     pmt_shape.DefineSection(1, -dz, 0, C4_R2 * 1.01)
@@ -332,10 +334,13 @@ def c4_scene(rings=C4_RINGS):
     bk7 = ASellmeierFormula(*C4_BK7)
     kg = TGraph()
     kg.SetPoint(0, 400 * nm, ARefractiveIndex.AbsorptionLengthToExtinctionCoefficient(C4_ABS_400, 400 * nm))
-    bk7.SetExtinctionCoefficient(kg)
+    if not deterministic:
+        bk7.SetExtinctionCoefficient(kg)
     lens = ALens("lens", TGeoTube("lensV", 0, rmax, 0.5 * cm))
     lens.SetRefractiveIndex(bk7)
     world.AddNode(lens, 1, TGeoTranslation(0, 0, dz + 2 * cm))
+    if deterministic:
+        mgr.DisableFresnelReflection(True)
     mgr.CloseGeometry()
     mgr.c4_dz = dz
     mgr.c4_rmax = rmax

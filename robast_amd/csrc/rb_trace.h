@@ -673,11 +673,17 @@ __device__ inline void multilayer_rt(const SC& S, const rb_border_desc* bc, doub
   T = th2_interpolate(S.params + g.t_offset, g, lambda, angle);
 }
 
+// The path record's slot k holds the (k+1)-th point this trace adds; the
+// ray's point count at entry is still in io.npoints[i] (written back only
+// when the trace ends), so it is read there rather than kept in a register.
 __device__ __forceinline__ void add_point(const RayIO& io, int64_t i, Lane& L, V3<double> x, double t, int node) {
-  if (io.path_cap > 0 && L.npoints - 1 < io.path_cap) {
-    double* s = io.path_xyzt + (int64_t(io.path_cap) * i + (L.npoints - 1)) * 4;
-    s[0] = x.x; s[1] = x.y; s[2] = x.z; s[3] = t;
-    io.path_node[int64_t(io.path_cap) * i + (L.npoints - 1)] = node;
+  if (io.path_cap > 0) {
+    const int k = L.npoints - io.npoints[i];
+    if (k < io.path_cap) {
+      double* s = io.path_xyzt + (int64_t(io.path_cap) * i + k) * 4;
+      s[0] = x.x; s[1] = x.y; s[2] = x.z; s[3] = t;
+      io.path_node[int64_t(io.path_cap) * i + k] = node;
+    }
   }
   L.t = t;
   L.last_node = node;

@@ -140,9 +140,10 @@ class AOpticsManager:
             cap = self.limit
             path_xyzt = np.zeros((n, cap, 4))
             path_node = np.full((n, cap), -1, dtype=np.int32)
+        npoints_in = arr.npoints.copy()
         rays = make_rays(arr.soa(), path_xyzt, path_node)
         ctx = self.context()
         ctx.trace(self.gpu_scene(), rays, n, self.trace_opts())
         if record:
-            arr.path_xyzt, arr.path_node = path_xyzt, path_node
+            arr.path_xyzt, arr.path_node, arr.npoints_in = path_xyzt, path_node, npoints_in
         arr.node_names = flat.node_names

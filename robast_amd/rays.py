@@ -130,8 +130,9 @@ class ARayArray:
         self.last_node = np.full(n, -1, dtype=np.int32)
         self.ids = np.arange(n, dtype=np.int64)
         self.first = None          # first points, kept for GetFirstPoint
-        self.path_xyzt = None      # optional [n, cap, 4]
+        self.path_xyzt = None      # optional [n, cap, 4]: the points the last trace added
         self.path_node = None      # optional [n, cap]
+        self.npoints_in = None     # point counts when the last recorded trace began
         self.node_names = None
 
     @classmethod
@@ -197,18 +198,24 @@ class ARayArray:
         return self._bin(A.RB_ABSORB)
 
     def ray(self, i):
-        """Materialise ray i as an ARay (points from the path record when kept)."""
+        """Materialise ray i as an ARay: the points it held when added (an ARay
+        passed to Add keeps its whole history; otherwise its first point),
+        then the points the last recorded trace added, then its last point."""
         r = ARay(int(self.ids[i]), self.lambda_[i])
-        if self.first is not None:
+        obj = getattr(self, "_objs", {}).get(i)
+        if obj is not None:
+            r.points, r.nodes = list(obj.points), list(obj.nodes)
+        elif self.first is not None:
             r.points = [tuple(self.first[i])]
         else:
             r.points = []
         if self.path_xyzt is not None:
             cap = self.path_xyzt.shape[1]
-            k = min(int(self.npoints[i]) - 1, cap)
+            base = int(self.npoints_in[i]) if self.npoints_in is not None else 1
+            k = min(int(self.npoints[i]) - base, cap)
             r.points += [tuple(p) for p in self.path_xyzt[i, :k]]
             names = self.node_names or []
-            r.nodes = [names[j] if 0 <= j < len(names) else None for j in self.path_node[i, :k]]
+            r.nodes += [names[j] if 0 <= j < len(names) else None for j in self.path_node[i, :k]]
         if not r.points or r.points[-1] != (self.x[i], self.y[i], self.z[i], self.t[i]):
             r.points.append((self.x[i], self.y[i], self.z[i], self.t[i]))
         r.direction = (self.dx[i], self.dy[i], self.dz[i])

@@ -69,3 +69,54 @@ def c1_exact_chain(flat, src, i, step_back):
     s2 = (f - hb[2]) / d2[2]
     end = [hb[k] + s2 * d2[k] for k in range(3)]
     return end, t / c + (s2 + mp.mpf("1e-10")) / c
+
+
+def sensitivity(flat, soa, opts, delta=1e-10, trials=2, seed=0):
+    """Per-ray conditioning of the trace, measured with the oracle: the rays
+    are re-traced `trials` times with their start points moved by `delta` cm
+    in random directions. Returns (gain, unstable): gain[i] is the largest
+    terminal-point move over delta (how much ray i amplifies a small
+    difference along its path; the terminal point of a path that bounces
+    between curved surfaces can move by 1e9 delta), unstable[i] is true when
+    a move changed the ray's status, point count or last node."""
+    from oracle import oracle as O
+    base = copy_soa(soa)
+    O.trace(flat, base, opts, nthreads=8)
+    rng = np.random.default_rng(seed)
+    n = len(soa["x"])
+    gain = np.zeros(n)
+    unstable = np.zeros(n, dtype=bool)
+    for _ in range(trials):
+        s = copy_soa(soa)
+        u = rng.normal(size=(n, 3))
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        s["x"] += delta * u[:, 0]
+        s["y"] += delta * u[:, 1]
+        s["z"] += delta * u[:, 2]
+        O.trace(flat, s, opts, nthreads=8)
+        unstable |= (s["status"] != base["status"]) | (s["npoints"] != base["npoints"]) | \
+            (s["last_node"] != base["last_node"])
+        mv = np.sqrt((s["x"] - base["x"]) ** 2 + (s["y"] - base["y"]) ** 2 + (s["z"] - base["z"]) ** 2)
+        gain = np.maximum(gain, mv / delta)
+    return gain, unstable
+
+
+def conditioned_compare(g, o, gain, unstable, eps=1e-12, floor=1e-9):
+    """Per-ray parity with bars from each ray's own conditioning: statuses,
+    point counts and last nodes identical except on rays that a 1e-10 cm
+    input move also changes; terminal points within floor + eps * gain *
+    npoints cm, i.e. a rounding-level difference of eps cm at each recorded
+    point, amplified by the ray's measured gain. Returns (integer
+    mismatches, largest |dpos| / bar over the matched rays)."""
+    bad = (g["status"] != o["status"]) | (g["npoints"] != o["npoints"]) | (g["last_node"] != o["last_node"])
+    assert np.all(unstable[bad]), f"integer mismatch on well-conditioned rays {np.nonzero(bad & ~unstable)[0][:5]}"
+    dp = np.sqrt((g["x"] - o["x"]) ** 2 + (g["y"] - o["y"]) ** 2 + (g["z"] - o["z"]) ** 2)
+    # a navigation step that degraded to TGeoShape::Big leaves the point near
+    # 1e31 cm: compared relative to its size
+    pn = np.sqrt(o["x"] ** 2 + o["y"] ** 2 + o["z"] ** 2)
+    bar = floor + eps * np.maximum(gain, 1.0) * o["npoints"] + 1e-12 * np.where(pn > 1e4, pn, 0.0)
+    ratio = (dp / bar)[~bad]
+    worst = int(np.argmax(dp / bar * ~bad))
+    assert ratio.max(initial=0) <= 1.0, \
+        f"ray {worst}: |dpos| {dp[worst]:.3e} > bar {bar[worst]:.3e} (gain {gain[worst]:.3e}, {o['npoints'][worst]} points)"
+    return int(bad.sum()), float(ratio.max(initial=0))

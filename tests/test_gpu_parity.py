@@ -211,9 +211,8 @@ def test_coherence_order_gives_identical_results(ctx, config):
     """Tracing in coherence order (sorted by first daughter-box entry, traced
     through a permutation) and in input order gives bit-identical rays, one
     lane per ray or from the persistent work queue. The two schedules are
-    separately compiled kernels, whose FMA contractions may differ: across
-    them statuses, point counts and last nodes are identical and terminal
-    points agree to 1e-9 cm (the parity bar is 1e-7 cm)."""
+    separately compiled kernels; with no FMA contraction every operation
+    rounds as written in both, so they agree bit for bit too."""
     import torch
     print(f"order input vs coherent, lane vs persistent {config}", flush=True)
     N = 1 << 20
@@ -238,11 +237,8 @@ def test_coherence_order_gives_identical_results(ctx, config):
         for k in a:
             assert torch.equal(a[k], b[k]), (schedule, k)
     a, b = out[A.RB_ORDER_INPUT, A.RB_SCHEDULE_LANE], out[A.RB_ORDER_INPUT, A.RB_SCHEDULE_PERSISTENT]
-    for k in ("status", "npoints", "last_node", "lambda_"):
+    for k in a:
         assert torch.equal(a[k], b[k]), k
-    dp = torch.sqrt((a["x"] - b["x"]) ** 2 + (a["y"] - b["y"]) ** 2 + (a["z"] - b["z"]) ** 2)
-    dd = torch.sqrt((a["dx"] - b["dx"]) ** 2 + (a["dy"] - b["dy"]) ** 2 + (a["dz"] - b["dz"]) ** 2)
-    assert float(dp.max()) <= 1e-9 and float(dd.max()) <= 1e-9
 
 
 def test_host_memory_passed_as_device_pointers_is_an_error(ctx):
