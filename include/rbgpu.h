@@ -403,7 +403,8 @@ int rbgpu_bunches_to_rays(rb_ctx* ctx, const rb_bunches* bunches, int64_t n_bunc
  * is weight * |h - centroid|^2 / 2 on its hit (h_x, h_y), so weight = 2 / N
  * and centroid = mean hit give d(spot RMS^2)/dparams. */
 typedef struct {
-  const int32_t* node_param;   /* device [n_nodes][3]: tip, tilt, R parameter index or -1 */
+  const int32_t* node_param;   /* device [n_nodes][3]: tip, tilt, R parameter index or -1; an index
+                                  outside [0, n_params) counts as -1 (the ray gives no gradient) */
   const double* param_values;  /* device [n_params]: current tip, tilt (rad), R (cm), ... */
   int32_t n_params;
   int32_t z_param;             /* index of the focal-plane z offset parameter, or -1 */

@@ -1893,7 +1893,8 @@ static bool dual_ray(const rb_scene_desc* d, const rb_adjoint_desc* A, V3 origin
                      const int32_t* pnode, int K, V3 hit, double* grad) {
   for (int j = 0; j < K; j++) {
     const int32_t* np = A->node_param + 3 * pnode[j];
-    if (np[0] < 0 || np[1] < 0 || np[2] < 0) return false;
+    for (int k = 0; k < 3; k++)
+      if (np[k] < 0 || np[k] >= A->n_params) return false;
   }
   const double eps = g_step_back;
   DV3 o = dv(origin), din = dv(d0);

@@ -150,6 +150,11 @@ def node_param_table(flat, nfacets=C5_NFACETS):
 def adjoint_desc(node_param, values, n_params, z_param, focal_node, cx, cy, weight, mode=A.RB_ADJ_CENTROID):
     """rb_adjoint_desc over caller-owned buffers (host or device pointers, as
     the tensors are)."""
+    idx = node_param.reshape(-1)
+    if idx.numel() and (int(idx.max()) >= n_params or int(idx.min()) < -1):
+        raise ValueError(f"adjoint_desc: a node parameter index lies outside [-1, {n_params})")
+    if not (-1 <= z_param < n_params):
+        raise ValueError(f"adjoint_desc: z_param {z_param} outside [-1, {n_params})")
     d = A.rb_adjoint_desc()
     d.mode = mode
     d.node_param = node_param.data_ptr()

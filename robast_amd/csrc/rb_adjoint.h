@@ -48,10 +48,14 @@ __device__ __forceinline__ V3f addf(V3f a, V3f b) { return {a.x + b.x, a.y + b.y
 __device__ __forceinline__ V3f mulf(V3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 
 // True if every reflection of the chain is on a facet with parameters.
+// True if every reflection of the chain is on a node whose three
+// parameter indices lie in [0, n_params): an index out of that range would
+// read param_values and write the LDS accumulators out of bounds.
 __device__ __forceinline__ bool chain_covered(const AdjParams& A, const ChainRecord& rec) {
   for (int j = 0; j < rec.n; j++) {
     const int32_t* np = A.node_param + 3 * rec.node[j];
-    if (np[0] < 0 || np[1] < 0 || np[2] < 0) return false;
+    for (int k = 0; k < 3; k++)
+      if (np[k] < 0 || np[k] >= A.n_params) return false;
   }
   return true;
 }

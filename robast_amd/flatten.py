@@ -49,6 +49,7 @@ class FlatScene:
         self.table_xy = table_xy
         self.node_names = node_names
         self.volume_objs = volume_objs
+        self.coating_grid = None  # (nx, lambda lo, lambda hi) of default coating tables, if any
         d = A.rb_scene_desc()
         d.n_nodes, d.nodes = len(nodes), A.ptr(nodes)
         d.n_children, d.children = len(children), A.ptr(children)
@@ -65,7 +66,10 @@ class FlatScene:
         return C.byref(self.desc)
 
 
-def flatten(top_volume):
+def flatten(top_volume, lambda_range=None):
+    """The closed geometry under top_volume as an rb_scene_desc (FlatScene).
+    lambda_range (cm): the wavelengths the default tables of coatings that
+    were not precalculated must cover (multilayer.default_grid)."""
     shapes, shape_ids, params = [], {}, []
     tables, table_ids, table_xy = [], {}, []
     volumes, volume_ids, volume_objs = [], {}, []
@@ -201,6 +205,7 @@ def flatten(top_volume):
     # A multilayer becomes a (wavelength, angle) table of R and T in params,
     # shared by every condition that holds the same AMultilayer object.
     border_recs = []
+    coating_grid = None  # the default grid of the coatings that were not precalculated
     for i, vol in enumerate(volume_objs):
         conds = getattr(vol, "border_conditions", [])
         vol_arr[i]["border_first"] = len(border_recs)
@@ -211,8 +216,10 @@ def flatten(top_volume):
             if c.multilayer is not None:
                 g = grid_ids.get(id(c.multilayer))
                 if g is None:
-                    nx, xlo, xhi, ny, ylo, yhi, R, T = c.multilayer.device_table()
+                    nx, xlo, xhi, ny, ylo, yhi, R, T = c.multilayer.device_table(lambda_range)
                     g = add_grid(id(c.multilayer), nx, xlo, xhi, ny, ylo, yhi, R, T)
+                    if c.multilayer.table is None:
+                        coating_grid = (nx, xlo, xhi)
             border_recs.append((v2, 1 if c.lambertian else 0, c.sigma, g))
     borders = np.zeros(len(border_recs), dtype=A.BORDER_DTYPE)
     for i, (v2, lam, sig, g) in enumerate(border_recs):
@@ -239,6 +246,7 @@ def flatten(top_volume):
                    grids)
     fs.desc.n_tables = len(tables)
     fs.desc.n_table_xy = len(table_xy)
+    fs.coating_grid = coating_grid
     return fs
 
 

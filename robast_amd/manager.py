@@ -45,6 +45,22 @@ class AOpticsManager:
         self._scene = None
         return self._flat
 
+    def flat_for(self, lam_min, lam_max):
+        """The flattened scene for rays with wavelengths in [lam_min, lam_max]
+        (cm). Coatings that were not precalculated, which the reference
+        evaluates exactly at any wavelength, are read on the device from a
+        table; when the batch reaches outside the band that table covers,
+        the scene is flattened again with the band widened to it."""
+        from .multilayer import grid_covers
+        flat = self.flat()
+        g = flat.coating_grid
+        if g is not None and not grid_covers(g, lam_min, lam_max):
+            lo = min(lam_min, g[1] + 0.5 * (g[2] - g[1]) / g[0])
+            hi = max(lam_max, g[2] - 0.5 * (g[2] - g[1]) / g[0])
+            self._flat = flatten(self.top, lambda_range=(lo, hi))
+            self._scene = None
+        return self._flat
+
     def SetNsegments(self, n):
         pass
 
@@ -123,6 +139,9 @@ class AOpticsManager:
         from .sources import DeviceRays
         if isinstance(obj, DeviceRays):  # rays already in HBM: traced in place
             if obj.n:
+                if self.flat().coating_grid is not None:
+                    lam = obj.t["lambda_"][:obj.n]
+                    self.flat_for(float(lam.min()), float(lam.max()))
                 self.context().trace(self.gpu_scene(), obj.rays(), obj.n, self.trace_opts(device_ptrs=1))
             return
         raise TypeError(f"TraceNonSequential: unsupported argument {type(obj).__name__}")
@@ -134,7 +153,7 @@ class AOpticsManager:
             return
         if arr.first is None:
             arr.first = np.stack([arr.x, arr.y, arr.z, arr.t], axis=1).copy()
-        flat = self.flat()
+        flat = self.flat_for(float(arr.lambda_.min()), float(arr.lambda_.max()))
         path_xyzt = path_node = None
         if record:
             cap = self.limit

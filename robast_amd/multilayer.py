@@ -17,7 +17,10 @@ The device evaluates a multilayer through a TH2-style table of the
 unpolarised reflectance and transmittance on bin centres, as
 AMultilayer::PreCalculateCoherentTMM (include/AMultilayer.h:243-262) builds it
 and CoherentTMMMixed then reads it with TH2::Interpolate. A stack that was not
-precalculated is tabulated on DEFAULT_GRID when the scene is flattened.
+precalculated, which the reference evaluates exactly at any wavelength, is
+tabulated on 1 nm x 0.25 deg bins over 200-1000 nm widened to the traced
+rays' wavelengths (default_grid): AOpticsManager re-flattens the scene when a
+batch reaches outside the band its table covers.
 """
 import math
 
@@ -28,6 +31,29 @@ kS, kP = 0, 1
 NM = 1e-9 * (1e2 * 1.0)                       # AOpticsManager::nm() in cm
 # default table: 1 nm steps over 200-1000 nm, 0.25 deg steps over 0-90 deg (bin centres)
 DEFAULT_GRID = (801, 199.5 * NM, 1000.5 * NM, 361, math.radians(-0.125), math.radians(90.125))
+
+
+def default_grid(lambda_range=None):
+    """DEFAULT_GRID, its wavelength axis widened in whole 1 nm bins to hold
+    lambda_range = (lo, hi) in cm with a bin to spare on either side: every
+    wavelength in the range then lies between two bin centres."""
+    nx, xlo, xhi, ny, ylo, yhi = DEFAULT_GRID
+    if lambda_range is None:
+        return DEFAULT_GRID
+    lo, hi = float(lambda_range[0]), float(lambda_range[1])
+    below = max(0, math.ceil((xlo + NM - lo) / NM)) if lo < xlo + NM else 0
+    above = max(0, math.ceil((hi - (xhi - NM)) / NM)) if hi > xhi - NM else 0
+    # never below 1 nm: the first bin centre stays positive
+    below = min(below, int((xlo - 0.5 * NM) / NM))
+    return (nx + below + above, xlo - below * NM, xhi + above * NM, ny, ylo, yhi)
+
+
+def grid_covers(grid, lo, hi):
+    """True if every wavelength in [lo, hi] (cm) lies between the first and
+    the last bin centre of the table's wavelength axis."""
+    nx, xlo, xhi = grid[0], grid[1], grid[2]
+    w = (xhi - xlo) / nx
+    return xlo + 0.5 * w <= lo and hi <= xhi - 0.5 * w
 
 
 def _forward(n, th):
@@ -205,10 +231,11 @@ class AMultilayer:
         R, T = self.tabulate(lam_nbins, lam_min, lam_max, th_nbins, th_min, th_max)
         self.table = (int(lam_nbins), float(lam_min), float(lam_max), int(th_nbins), float(th_min), float(th_max), R, T)
 
-    def device_table(self):
-        """The table the device reads: the precalculated one, or DEFAULT_GRID."""
+    def device_table(self, lambda_range=None):
+        """The table the device reads: the precalculated one, or the default
+        grid widened to lambda_range (default_grid)."""
         if self.table is None:
-            nx, xlo, xhi, ny, ylo, yhi = DEFAULT_GRID
+            nx, xlo, xhi, ny, ylo, yhi = default_grid(lambda_range)
             R, T = self.tabulate(nx, xlo, xhi, ny, ylo, yhi)
             return (nx, xlo, xhi, ny, ylo, yhi, R, T)
         return self.table

@@ -202,6 +202,38 @@ def test_mirror_boundary_multilayer(backend):
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
+def test_coating_outside_the_default_band(backend):
+    """A coating that was not precalculated, at wavelengths outside the
+    200-1000 nm default table (where the reference still evaluates the TMM
+    exactly): the scene is flattened again with the table widened to the
+    batch, and the reflected fraction follows the exact TMM. Inside the band
+    nothing is rebuilt."""
+    G._reset_registry()
+    mgr = make_the_world()
+    mirror = AMirror("mirror", TGeoBBox("mirrorbox", 0.5 * m, 0.5 * m, 0.5 * m))
+    layer = AMultilayer(ARefractiveIndex(1.0, 0.0), material("Al"))
+    layer.InsertLayer(material("SiO2"), 400 * nm)
+    ABorderSurfaceCondition(mgr.GetTopVolume(), mirror).SetMultilayer(layer)
+    mgr.GetTopVolume().AddNode(mirror, 1)
+    mgr.CloseGeometry()
+    first = mgr.flat()
+    assert first.coating_grid[0] == DEFAULT_GRID[0]
+    assert mgr.flat_for(300 * nm, 900 * nm) is first
+    N = 20000
+    for wl in (190.25, 1200.0, 1733.3, 2400.0):
+        arr = rays_45(N, wl * nm)
+        run(mgr, arr, backend)
+        g = mgr.flat().coating_grid
+        w = (g[2] - g[1]) / g[0]
+        assert g[1] + 0.5 * w <= wl * nm <= g[2] - 0.5 * w
+        ex, ab = (arr.status == A.RB_EXIT).sum(), (arr.status == A.RB_ABSORB).sum()
+        assert ex + ab == N
+        R, _ = layer.CoherentTMMMixed(45 * deg, wl * nm)
+        assert 0.2 < R < 1.0
+        assert within_3sigma(ex, N, R), (wl, ex, N * R)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
 def test_lens_boundary_multilayer(backend):
     # unittest_robast.py:291-331 (an AMirror named "lens"): Si under SiO2 2 um + Si3N4 33 nm
     G._reset_registry()

@@ -18,8 +18,10 @@ BACKENDS = ["oracle", pytest.param("gpu", marks=pytest.mark.gpu)]
 
 def run(mgr, arr, backend):
     if backend == "oracle":
+        # the scene TraceNonSequential would upload for this batch's wavelengths
         from oracle import oracle as O
-        O.trace(mgr.flat(), arr.soa(), mgr.trace_opts(), nthreads=8)
+        flat = mgr.flat_for(float(arr.lambda_.min()), float(arr.lambda_.max())) if arr.n else mgr.flat()
+        O.trace(flat, arr.soa(), mgr.trace_opts(), nthreads=8)
     else:
         mgr.TraceNonSequential(arr)
 
