@@ -32,7 +32,7 @@ from oracle import oracle as O
 from robast_amd import _abi as A
 from robast_amd import geometry as G
 from robast_amd.manager import AOpticsManager
-from robast_amd.materials import ARefractiveIndex, ASellmeierFormula, TGraph
+from robast_amd.materials import ARefractiveIndex, ASchottFormula, ASellmeierFormula, TGraph
 
 SEEDS = [11, 23, 37, 41, 59]
 # the per-point rounding-level difference the parity bars allow, in cm
@@ -140,11 +140,16 @@ def random_scene(seed, surfaces=False):
                         vol.SetReflectance(graph)
                 elif role == 2:
                     vol = G.ALens(name, sh)
-                    if rng.uniform() < 0.5:
+                    u_idx = rng.uniform()
+                    if u_idx < 0.4:
                         vol.SetRefractiveIndex(ARefractiveIndex(float(rng.uniform(1.3, 1.8))))
-                    else:
+                    elif u_idx < 0.7:
                         vol.SetRefractiveIndex(ASellmeierFormula(1.03961212, 0.231792344, 1.01046945,
                                                                  0.00600069867, 0.0200179144, 103.560653))
+                    else:
+                        # N-BK7 in Schott's form (src/ASchottFormula.cxx:43-55)
+                        vol.SetRefractiveIndex(ASchottFormula(2.2718929, -1.0108077e-2, 1.0592509e-2,
+                                                              2.0816965e-4, -7.6472538e-6, 4.9240991e-7))
                 elif role == 3:
                     vol = G.AObscuration(name, sh)
                 elif role == 4:

@@ -10,6 +10,7 @@ from robast_amd import (AFocalSurface, ALens, AMirror, AOpticalComponent, AOptic
                         ARayShooter, ARefractiveIndex, TGeoBBox, TGeoSphere, TGeoTranslation, TGraph)
 from robast_amd import _abi as A
 from robast_amd import geometry as G
+from robast_amd.materials import ASchottFormula
 
 nm, um, mm, m, deg = G.nm, G.um, G.mm, G.m, G.deg
 
@@ -61,6 +62,34 @@ def test_snells_law(backend):
     assert arr.status[0] == A.RB_FOCUS
     assert round(arr.dx[0] - math.sin(th) / idx, 7) == 0
     assert round(arr.dy[0], 7) == 0
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_snells_law_schott_dispersion(backend):
+    """test_snells_law through a lens whose index follows Schott's formula
+    (src/ASchottFormula.cxx:43-55; an N-BK7 fit): at each wavelength the
+    refracted direction is sin(30 deg) / n(lambda), n from the closed form."""
+    a = (2.2718929, -1.0108077e-2, 1.0592509e-2, 2.0816965e-4, -7.6472538e-6, 4.9240991e-7)
+    mgr = make_the_world()
+    mgr.DisableFresnelReflection(True)
+    lens = ALens("lens", TGeoBBox("lensbox", 0.5 * m, 0.5 * m, 1 * mm))
+    lens.SetRefractiveIndex(ASchottFormula(*a))
+    mgr.GetTopVolume().AddNode(lens, 1)
+    lens.AddNode(AFocalSurface("focal", TGeoBBox("focalbox", 0.5 * m, 0.5 * m, 0.1 * mm)), 1)
+    mgr.CloseGeometry()
+    th = 30 * deg
+    lams = (350 * nm, 450 * nm, 587.6 * nm, 800 * nm)
+    arr = ARayArray()
+    for k, lam in enumerate(lams):
+        arr.Add(ARay(k, lam, 0, 0, 2 * mm, 0, math.sin(th), 0, -math.cos(th)))
+    run(mgr, arr, backend)
+    for k, lam in enumerate(lams):
+        l = lam / um
+        n = math.sqrt(a[0] + a[1] * l ** 2 + a[2] * l ** -2 + a[3] * l ** -4 + a[4] * l ** -6 + a[5] * l ** -8)
+        assert arr.status[k] == A.RB_FOCUS
+        assert abs(arr.dx[k] - math.sin(th) / n) < 1e-12, (lam, arr.dx[k], math.sin(th) / n)
+    # dispersion: the blue ray bends most
+    assert arr.dx[0] < arr.dx[1] < arr.dx[2] < arr.dx[3]
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
