@@ -49,9 +49,12 @@ enum {
   RB_SHAPE_TUBE = 2,       /* rmin rmax dz                          (TGeoTube)        */
   RB_SHAPE_SPHERE = 3,     /* rmin rmax theta1 theta2 [deg] cos1 cos2 sin1 sin2 (TGeoSphere, full phi) */
   RB_SHAPE_PARABOLOID = 4, /* rlo rhi dz a b                        (TGeoParaboloid)  */
-  RB_SHAPE_PGON = 5,       /* phi1 dphi nedges nz {z rmin rmax}*nz {cos sin}*nedges (TGeoPgon;
-                              side k's outward normal at phi1 + dphi/nedges (k + 1/2)) */
-  RB_SHAPE_PCON = 6,       /* phi1 dphi nz {z rmin rmax}*nz         (TGeoPcon)        */
+  RB_SHAPE_PGON = 5,       /* phi1 dphi nedges nz {z rmin rmax}*nz {cos sin}*nedges
+                              {cos sin}(phi1) {cos sin}(phi1 + dphi)   (TGeoPgon; side k's outward
+                              normal at phi1 + dphi/nedges (k + 1/2); z non-decreasing;
+                              dphi >= 360 is the full polygon) */
+  RB_SHAPE_PCON = 6,       /* phi1 dphi nz {z rmin rmax}*nz {cos sin}(phi1) {cos sin}(phi1 + dphi)
+                              (TGeoPcon; z non-decreasing; dphi >= 360 is the full cone) */
   RB_SHAPE_COMPOSITE = 7,  /* op left right lmat[12] rmat[12]       (TGeoBoolNode)    */
   RB_SHAPE_ASPHERIC = 8,   /* z1 c1 z2 c2 rmax rmin kappa1 kappa2 n1 n2 K1[n1] K2[n2]
                               Kn1[n1] Kn2[n2] with Kn_k = K_k rmax^2k (AGeoAsphericDisk) */

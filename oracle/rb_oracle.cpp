@@ -648,6 +648,187 @@ static V3 pgon_normal(const double* p, V3 q, V3 d) {
 }
 
 
+// ---------------------------------------------------------------- TGeoPcon, general TGeoPgon
+// TGeoPcon (any rmin, any phi range) and TGeoPgon with rmin > 0 or a phi
+// range. Containment follows the shapes' definitions, as ROOT's
+// TGeoPcon::Contains / TGeoPgon::Contains state them: the z section, phi by
+// atan2 within [phi1, phi1 + dphi], and the radius -- for a Pgon the
+// projection on the side of the point's phi sector -- between rmin and rmax
+// interpolated linearly in z. Distances are found by brute force over the
+// surfaces: every crossing of the ray with a section plane, with the inner
+// and outer cone (or side planes) of every segment, or with a phi
+// half-plane is a candidate, and the answer is the first candidate at which
+// containment changes, tested 1e-9 cm before and after it.
+// params: Pcon phi1 dphi nz {z rmin rmax}*nz {cos sin}*2;
+//         Pgon phi1 dphi nedges nz {z rmin rmax}*nz {cos sin}*nedges {cos sin}*2
+struct GPoly {
+  const double* z;
+  int nz, ne;
+  bool pgon;
+  double phi1, dphi;
+};
+static GPoly gpoly_p(const double* p, bool pgon) {
+  GPoly g;
+  g.pgon = pgon;
+  g.phi1 = p[0];
+  g.dphi = p[1];
+  g.ne = pgon ? int(p[2]) : 0;
+  g.nz = int(p[pgon ? 3 : 2]);
+  g.z = p + (pgon ? 4 : 3);
+  return g;
+}
+// full phi and rmin = 0 throughout: the convex-frustum form above
+static bool pgon_simple(const double* p) {
+  if (p[1] < 360.0) return false;
+  for (int k = 0; k < int(p[3]); k++)
+    if (p[4 + 3 * k + 1] != 0.0) return false;
+  return true;
+}
+// phi of q in [phi1, phi1 + 360) (degrees)
+static double gpoly_phi(const GPoly& g, V3 q) {
+  double phi = std::atan2(q.y, q.x) * 180 / kPi;
+  while (phi < g.phi1) phi += 360.0;
+  while (phi >= g.phi1 + 360.0) phi -= 360.0;
+  return phi;
+}
+static double gpoly_radius(const GPoly& g, V3 q, double phi) {
+  if (!g.pgon) return std::sqrt(q.x * q.x + q.y * q.y);
+  const double div = std::min(g.dphi, 360.0) / g.ne;
+  const int ip = std::min(int((phi - g.phi1) / div), g.ne - 1);
+  const double ph = (g.phi1 + div * (ip + 0.5)) * kPi / 180;
+  return q.x * std::cos(ph) + q.y * std::sin(ph);
+}
+static bool gpoly_contains(const GPoly& g, V3 q) {
+  if (q.z < g.z[0] || q.z > g.z[3 * (g.nz - 1)]) return false;
+  const double phi = gpoly_phi(g, q);
+  if (g.dphi < 360.0 && phi - g.phi1 > g.dphi) return false;
+  const double r = gpoly_radius(g, q, phi);
+  for (int i = 0; i + 1 < g.nz; i++) {
+    const double za = g.z[3 * i], zb = g.z[3 * (i + 1)];
+    if (q.z < za || q.z > zb) continue;
+    double rmin, rmax;
+    if (zb - za < 1e-8) {
+      rmin = std::min(g.z[3 * i + 1], g.z[3 * (i + 1) + 1]);
+      rmax = std::max(g.z[3 * i + 2], g.z[3 * (i + 1) + 2]);
+    } else {
+      const double f = (q.z - za) / (zb - za);
+      rmin = g.z[3 * i + 1] + f * (g.z[3 * (i + 1) + 1] - g.z[3 * i + 1]);
+      rmax = g.z[3 * i + 2] + f * (g.z[3 * (i + 1) + 2] - g.z[3 * i + 2]);
+    }
+    if (r >= rmin && r <= rmax) return true;
+  }
+  return false;
+}
+// every parameter t at which the ray meets one of the solid's surfaces
+static void gpoly_candidates(const GPoly& g, V3 q, V3 d, std::vector<double>& ts) {
+  ts.clear();
+  if (d.z != 0)
+    for (int k = 0; k < g.nz; k++) ts.push_back((g.z[3 * k] - q.z) / d.z);
+  for (int i = 0; i + 1 < g.nz; i++) {
+    const double z0 = g.z[3 * i], z1 = g.z[3 * (i + 1)];
+    if (!(z1 > z0)) continue;
+    for (int col = 1; col <= 2; col++) {
+      const double r0 = g.z[3 * i + col], r1 = g.z[3 * (i + 1) + col];
+      if (col == 1 && r0 == 0 && r1 == 0) continue;
+      const double s = (r1 - r0) / (z1 - z0);
+      if (g.pgon) {
+        const double div = std::min(g.dphi, 360.0) / g.ne;
+        for (int k = 0; k < g.ne; k++) {
+          const double ph = (g.phi1 + div * (k + 0.5)) * kPi / 180;
+          const double cx = std::cos(ph), sy = std::sin(ph);
+          const double f0 = cx * q.x + sy * q.y - s * (q.z - z0) - r0, fd = cx * d.x + sy * d.y - s * d.z;
+          if (fd != 0) ts.push_back(-f0 / fd);
+        }
+      } else {
+        // x^2 + y^2 = (r0 + s (z - z0))^2 along the ray: both nappes
+        const double R0 = r0 + s * (q.z - z0);
+        const double a = d.x * d.x + d.y * d.y - s * s * d.z * d.z;
+        const double b = 2 * (q.x * d.x + q.y * d.y - s * d.z * R0);
+        const double c = q.x * q.x + q.y * q.y - R0 * R0;
+        if (std::fabs(a) < 1e-14) {
+          if (b != 0) ts.push_back(-c / b);
+        } else {
+          const double disc = b * b - 4 * a * c;
+          if (disc >= 0) {
+            ts.push_back((-b - std::sqrt(disc)) / (2 * a));
+            ts.push_back((-b + std::sqrt(disc)) / (2 * a));
+          }
+        }
+      }
+    }
+  }
+  if (g.dphi < 360.0) {
+    for (double phd : {g.phi1, g.phi1 + g.dphi}) {
+      const double ph = phd * kPi / 180;
+      const double nx = -std::sin(ph), ny = std::cos(ph);
+      const double fd = nx * d.x + ny * d.y;
+      if (fd != 0) ts.push_back(-(nx * q.x + ny * q.y) / fd);
+    }
+  }
+  std::sort(ts.begin(), ts.end());
+}
+// from outside (entering) or inside (leaving): the first candidate at which
+// containment changes that way
+static double gpoly_dist(const GPoly& g, V3 q, V3 d, bool leaving) {
+  std::vector<double> ts;
+  gpoly_candidates(g, q, d, ts);
+  const double e = 1e-9;
+  for (double t : ts) {
+    if (t < -e) continue;
+    const bool before = gpoly_contains(g, axpy(q, t - e, d)), after = gpoly_contains(g, axpy(q, t + e, d));
+    if (leaving ? (before && !after) : (!before && after)) return std::max(t, 0.0);
+  }
+  return leaving ? 0.0 : kBig;
+}
+// the nearest surface: a section plane, the inner or outer surface of the
+// point's segment, or a phi half-plane
+static V3 gpoly_normal(const GPoly& g, V3 q, V3 d) {
+  double best = kBig;
+  V3 n = v3(0, 0, 1);
+  for (int k = 0; k < g.nz; k++) best = std::min(best, std::fabs(q.z - g.z[3 * k]));
+  const double phi = gpoly_phi(g, q);
+  double cx, sy;
+  if (g.pgon) {
+    const double div = std::min(g.dphi, 360.0) / g.ne;
+    const int ip = std::min(int((std::min(phi - g.phi1, std::min(g.dphi, 360.0))) / div), g.ne - 1);
+    const double ph = (g.phi1 + div * (ip + 0.5)) * kPi / 180;
+    cx = std::cos(ph);
+    sy = std::sin(ph);
+  } else {
+    cx = std::cos(phi * kPi / 180);
+    sy = std::sin(phi * kPi / 180);
+  }
+  const double r = q.x * cx + q.y * sy;
+  for (int i = 0; i + 1 < g.nz; i++) {
+    const double z0 = g.z[3 * i], z1 = g.z[3 * (i + 1)];
+    if (!(z1 > z0) || q.z < z0 - 1e-9 || q.z > z1 + 1e-9) continue;
+    for (int col = 1; col <= 2; col++) {
+      const double r0 = g.z[3 * i + col], r1 = g.z[3 * (i + 1) + col];
+      if (col == 1 && r0 == 0 && r1 == 0) continue;
+      const double s = (r1 - r0) / (z1 - z0), len = std::sqrt(1 + s * s);
+      const double dist = std::fabs(r - r0 - s * (q.z - z0)) / len;
+      if (dist < best) {
+        best = dist;
+        n = v3(cx / len, sy / len, -s / len);
+      }
+    }
+  }
+  if (g.dphi < 360.0) {
+    for (double phd : {g.phi1, g.phi1 + g.dphi}) {
+      const double ph = phd * kPi / 180;
+      if (std::cos(ph) * q.x + std::sin(ph) * q.y < 0) continue;
+      const double nx = -std::sin(ph), ny = std::cos(ph);
+      const double dist = std::fabs(nx * q.x + ny * q.y);
+      if (dist < best) {
+        best = dist;
+        n = v3(nx, ny, 0);
+      }
+    }
+  }
+  if (dot(n, d) < 0) n = mul(n, -1);
+  return n;
+}
+
 // ---------------------------------------------------------------- aspheric disk
 // AGeoAsphericDisk (src/AGeoAsphericDisk.cxx). params: z1 c1 z2 c2 rmax rmin
 // kappa1 kappa2 n1 n2 K1[n1] K2[n2]; surface s in {0, 1} = ROOT's 1, 2.
@@ -990,7 +1171,8 @@ static bool contains(const Scene& sc, int shape, V3 q) {
     case RB_SHAPE_TUBE: return tube_contains(p, q);
     case RB_SHAPE_SPHERE: return sphere_contains(p, q);
     case RB_SHAPE_PARABOLOID: return para_contains(p, q);
-    case RB_SHAPE_PGON: return pgon_contains(p, q);
+    case RB_SHAPE_PGON: return pgon_simple(p) ? pgon_contains(p, q) : gpoly_contains(gpoly_p(p, true), q);
+    case RB_SHAPE_PCON: return gpoly_contains(gpoly_p(p, false), q);
     case RB_SHAPE_ASPHERIC: return asph_contains(p, q);
     case RB_SHAPE_WINSTON: return wins_contains(p, q);
     case RB_SHAPE_COMPOSITE: {
@@ -1109,7 +1291,9 @@ static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
     case RB_SHAPE_TUBE: return {tube_dist_out(p, q, d), 0};
     case RB_SHAPE_SPHERE: return {sphere_dist_out(p, q, d), 0};
     case RB_SHAPE_PARABOLOID: return {para_dist_out(p, q, d), 0};
-    case RB_SHAPE_PGON: return {pgon_dist_out(p, q, d), 0};
+    case RB_SHAPE_PGON:
+      return {pgon_simple(p) ? pgon_dist_out(p, q, d) : gpoly_dist(gpoly_p(p, true), q, d, false), 0};
+    case RB_SHAPE_PCON: return {gpoly_dist(gpoly_p(p, false), q, d, false), 0};
     case RB_SHAPE_ASPHERIC: return {asph_dist(p, q, d), 0};
     case RB_SHAPE_WINSTON: return {wins_dist_out(p, q, d), 0};
     case RB_SHAPE_COMPOSITE: return comp_dist_out(sc, shape, q, d);
@@ -1125,7 +1309,9 @@ static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
     case RB_SHAPE_TUBE: return {tube_dist_in(p, q, d), 0};
     case RB_SHAPE_SPHERE: return {sphere_dist_in(p, q, d), 0};
     case RB_SHAPE_PARABOLOID: return {para_dist_in(p, q, d), 0};
-    case RB_SHAPE_PGON: return {pgon_dist_in(p, q, d), 0};
+    case RB_SHAPE_PGON:
+      return {pgon_simple(p) ? pgon_dist_in(p, q, d) : gpoly_dist(gpoly_p(p, true), q, d, true), 0};
+    case RB_SHAPE_PCON: return {gpoly_dist(gpoly_p(p, false), q, d, true), 0};
     case RB_SHAPE_ASPHERIC: return {asph_dist(p, q, d), 0};
     case RB_SHAPE_WINSTON: return {wins_dist_in(p, q, d), 0};
     case RB_SHAPE_COMPOSITE: return comp_dist_in(sc, shape, q, d);
@@ -1157,7 +1343,8 @@ static V3 shape_normal(const Scene& sc, int shape, uint32_t leaf, V3 q, V3 d) {
     case RB_SHAPE_TUBE: n = tube_normal(p, q, d); break;
     case RB_SHAPE_SPHERE: n = sphere_normal(p, q, d); break;
     case RB_SHAPE_PARABOLOID: n = para_normal(p, q, d); break;
-    case RB_SHAPE_PGON: n = pgon_normal(p, q, d); break;
+    case RB_SHAPE_PGON: n = pgon_simple(p) ? pgon_normal(p, q, d) : gpoly_normal(gpoly_p(p, true), q, d); break;
+    case RB_SHAPE_PCON: n = gpoly_normal(gpoly_p(p, false), q, d); break;
     case RB_SHAPE_ASPHERIC: n = asph_normal(p, q, d); break;
     case RB_SHAPE_WINSTON: n = wins_normal(p, q, d); break;
     default: n = v3(0, 0, 1);

@@ -74,15 +74,19 @@ struct DScene {
   // whether it handles border-surface conditions (roughness, Lambertian)
   static constexpr int max_depth = kMaxCsgDepth;
   static constexpr bool borders_on = true;
+  static constexpr bool ext_on = true;
 };
 constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // The same scene, for kernels compiled for scenes whose composites nest at
 // most MAXD levels: the deeper composite call sites, and the registers they
 // make the kernel spill around them, are left out; likewise the border-surface
-// code unless BORDERS.
-template <int MAXD, bool BORDERS = false> struct DSceneOf : DScene {
+// code unless BORDERS, and the TGeoPcon / general TGeoPgon code unless EXT
+// (compiled into the deep and the border-surface forms; a scene holding such
+// a shape is traced with the deep form).
+template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1 || BORDERS)> struct DSceneOf : DScene {
   static constexpr int max_depth = MAXD;
   static constexpr bool borders_on = BORDERS;
+  static constexpr bool ext_on = EXT;
 };
 
 template <typename T>
@@ -602,6 +606,323 @@ template <typename T> __device__ V3<T> pgon_normal(const double* p, V3<T> q, V3<
   return n;
 }
 
+// ------------------------------------------------------------ TGeoPcon, and TGeoPgon with rmin > 0 or a phi range
+// Pcon params:  phi1 dphi nz {z rmin rmax}*nz {cos sin}(phi1) {cos sin}(phi1 + dphi)
+// Pgon params:  phi1 dphi nedges nz {z rmin rmax}*nz {cos sin}*nedges {cos sin}(phi1) {cos sin}(phi1 + dphi)
+// (z non-decreasing). Per z segment the solid is the region between the
+// inner and the outer surface (cone frustums; for a Pgon pyramid frustums
+// whose sides are the polygon's), inside the phi wedge. Along a ray the
+// slab, the outer region and a wedge of at most 180 deg are convex (one
+// interval each); the inner region and, for a wedge wider than 180 deg, the
+// missing wedge are removed from it, so a segment holds at most three
+// intervals of the ray and the solid holds their union over the segments.
+#ifndef RB_POLY_FN
+#define RB_POLY_FN __noinline__
+#endif
+constexpr int kShapePgonGeneral = 64;  // DShape.type of a TGeoPgon with rmin > 0 or a phi range (set at upload)
+
+struct PolyView {
+  const double* z;      // {z rmin rmax} * nz
+  const double* faces;  // Pgon: outward side normals {cos sin} * ne
+  const double* wedge;  // {cos sin} of phi1, then of phi1 + dphi
+  int nz, ne;
+  bool full;            // dphi >= 360
+  bool reflex;          // dphi > 180: the wedge is the union of its two half-planes
+};
+template <bool PGON> __device__ __forceinline__ PolyView poly_view(const double* p) {
+  PolyView v;
+  if (PGON) {
+    v.ne = int(p[2]);
+    v.nz = int(p[3]);
+    v.z = p + 4;
+    v.faces = v.z + 3 * v.nz;
+    v.wedge = v.faces + 2 * v.ne;
+  } else {
+    v.ne = 0;
+    v.nz = int(p[2]);
+    v.z = p + 3;
+    v.faces = nullptr;
+    v.wedge = v.z + 3 * v.nz;
+  }
+  v.full = p[1] >= 360.0;
+  v.reflex = p[1] > 180.0;
+  return v;
+}
+// inward normals (into the wedge) of the two phi half-planes
+template <typename T> __device__ __forceinline__ void poly_wedge(const PolyView& v, T& n1x, T& n1y, T& n2x, T& n2y) {
+  n1x = T(-v.wedge[1]);
+  n1y = T(v.wedge[0]);
+  n2x = T(v.wedge[3]);
+  n2y = T(-v.wedge[2]);
+}
+// [a, b] clipped to {t : n . (q + t d)_xy >= 0}; false if it becomes empty
+template <typename T> __device__ __forceinline__ bool half_clip(T nx, T ny, V3<T> q, V3<T> d, T& a, T& b) {
+  const T f0 = nx * q.x + ny * q.y, fd = nx * d.x + ny * d.y;
+  if (fd == T(0)) return f0 >= T(0) && a <= b;
+  const T t = -f0 / fd;
+  if (fd > T(0)) a = t > a ? t : a;
+  else b = t < b ? t : b;
+  return a <= b;
+}
+// [a, b] clipped to the slab of segment i; false if empty or a zero-length segment
+template <typename T> __device__ __forceinline__ bool poly_slab(const PolyView& v, int i, V3<T> q, V3<T> d, T& a, T& b) {
+  const T z0 = T(v.z[3 * i]), z1 = T(v.z[3 * (i + 1)]);
+  if (!(z1 - z0 > T(0))) return false;
+  a = T(-kBig);
+  b = T(kBig);
+  if (d.z != T(0)) {
+    T u = (z0 - q.z) / d.z, w = (z1 - q.z) / d.z;
+    if (u > w) { T x = u; u = w; w = x; }
+    a = u;
+    b = w;
+    return true;
+  }
+  return q.z >= z0 && q.z <= z1;
+}
+// [a, b] clipped to the convex region radial(p) <= R(z) of segment i, where R
+// interpolates column col (1 = rmin, 2 = rmax) linearly between its sections
+// and radial is the distance from the axis (Pcon) or the projection on the
+// side normals (Pgon); false if it becomes empty.
+template <typename T, bool PGON>
+__device__ bool poly_radial(const PolyView& v, int i, int col, V3<T> q, V3<T> d, T& a, T& b) {
+  const T z0 = T(v.z[3 * i]), z1 = T(v.z[3 * (i + 1)]);
+  const T r0 = T(v.z[3 * i + col]);
+  const T s = (T(v.z[3 * (i + 1) + col]) - r0) / (z1 - z0);
+  if (PGON) {
+    for (int k = 0; k < v.ne; k++) {
+      const T cx = T(v.faces[2 * k]), sy = T(v.faces[2 * k + 1]);
+      const T f0 = cx * q.x + sy * q.y - s * (q.z - z0) - r0;
+      const T fd = cx * d.x + sy * d.y - s * d.z;
+      if (fd == T(0)) {
+        if (f0 > T(0)) return false;
+        continue;
+      }
+      const T t = -f0 / fd;
+      if (fd > T(0)) b = t < b ? t : b;
+      else a = t > a ? t : a;
+    }
+    return a <= b;
+  }
+  // cone frustum x^2 + y^2 <= R(z)^2 with R(z) = R0 + s dz t >= 0 inside the slab:
+  // g(t) = A t^2 + 2 B t + C <= 0
+  const T R0 = r0 + s * (q.z - z0);
+  const T A = d.x * d.x + d.y * d.y - s * s * d.z * d.z;
+  const T B = q.x * d.x + q.y * d.y - s * d.z * R0;
+  const T C = q.x * q.x + q.y * q.y - R0 * R0;
+  // the real nappe: R(t) >= 0
+  const T rd = s * d.z;
+  if (rd != T(0)) {
+    const T t = -R0 / rd;
+    if (rd > T(0)) a = t > a ? t : a;
+    else b = t < b ? t : b;
+  } else if (R0 < T(0)) {
+    return false;
+  }
+  if (!(a <= b)) return false;
+  const T eps = T(1e-12);
+  if (fabs(A) <= eps * (d.x * d.x + d.y * d.y + s * s * d.z * d.z)) {
+    // parallel to a generator: 2 B t + C <= 0
+    if (B == T(0)) return C <= T(0);
+    const T t = -C / (T(2) * B);
+    if (B > T(0)) b = t < b ? t : b;
+    else a = t > a ? t : a;
+    return a <= b;
+  }
+  const T disc = B * B - A * C;
+  if (A > T(0)) {
+    if (disc < T(0)) return false;
+    const T h = sqrt(disc);
+    // roots without cancellation
+    const T qq = B >= T(0) ? -(B + h) : -(B - h);
+    T u = qq / A, w = qq != T(0) ? C / qq : T(0);
+    if (u > w) { T x = u; u = w; w = x; }
+    a = u > a ? u : a;
+    b = w < b ? w : b;
+    return a <= b;
+  }
+  // A < 0: g <= 0 outside the roots; the part on the real nappe is one half-line
+  if (disc < T(0)) return true;
+  const T h = sqrt(disc);
+  const T qq = B >= T(0) ? -(B + h) : -(B - h);
+  T u = qq / A, w = qq != T(0) ? C / qq : T(0);
+  if (u > w) { T x = u; u = w; w = x; }
+  if (R0 + rd * w >= T(0)) a = w > a ? w : a;
+  else b = u < b ? u : b;
+  return a <= b;
+}
+// True if segment i has an inner surface (rmin > 0 at either section)
+__device__ __forceinline__ bool poly_has_inner(const PolyView& v, int i) {
+  return v.z[3 * i + 1] > 0.0 || v.z[3 * (i + 1) + 1] > 0.0;
+}
+// Calls f(lo, hi) for each interval of the ray inside segment i.
+template <typename T, bool PGON, typename F>
+__device__ __forceinline__ void poly_pieces(const PolyView& v, int i, V3<T> q, V3<T> d, F& f) {
+  T a, b;
+  if (!poly_slab(v, i, q, d, a, b)) return;
+  if (!poly_radial<T, PGON>(v, i, 2, q, d, a, b)) return;
+  T n1x = T(0), n1y = T(0), n2x = T(0), n2y = T(0);
+  if (!v.full) {
+    poly_wedge(v, n1x, n1y, n2x, n2y);
+    if (!v.reflex && (!half_clip(n1x, n1y, q, d, a, b) || !half_clip(n2x, n2y, q, d, a, b))) return;
+  }
+  // removed: the inner region [c, e], and for a reflex wedge the missing wedge [m0, m1]
+  T c = T(kBig), e = T(-kBig), m0 = T(kBig), m1 = T(-kBig);
+  if (poly_has_inner(v, i)) {
+    T u = T(-kBig), w = T(kBig);
+    if (poly_radial<T, PGON>(v, i, 1, q, d, u, w)) { c = u; e = w; }
+  }
+  if (!v.full && v.reflex) {
+    T u = T(-kBig), w = T(kBig);
+    if (half_clip(-n1x, -n1y, q, d, u, w) && half_clip(-n2x, -n2y, q, d, u, w)) { m0 = u; m1 = w; }
+  }
+  // [a, b] minus [c, e] minus [m0, m1]
+  T lo[2], hi[2];
+  int np = 0;
+  if (c <= e && c < b && e > a) {
+    if (a < c) { lo[np] = a; hi[np] = c; np++; }
+    if (e < b) { lo[np] = e; hi[np] = b; np++; }
+  } else {
+    lo[0] = a; hi[0] = b; np = 1;
+  }
+  for (int k = 0; k < np; k++) {
+    if (m0 <= m1 && m0 < hi[k] && m1 > lo[k]) {
+      if (lo[k] < m0) f(lo[k], m0);
+      if (m1 < hi[k]) f(m1, hi[k]);
+    } else {
+      f(lo[k], hi[k]);
+    }
+  }
+}
+template <typename T> struct PolyFirstEntry {
+  T best;
+  __device__ __forceinline__ void operator()(T lo, T hi) {
+    if (hi <= T(0)) return;
+    const T t = lo < T(0) ? T(0) : lo;
+    best = t < best ? t : best;
+  }
+};
+template <typename T> struct PolyAdvance {
+  T cur;
+  bool moved;
+  __device__ __forceinline__ void operator()(T lo, T hi) {
+    if (!moved && lo <= cur + T(1e-9) && hi > cur + T(1e-9)) {
+      cur = hi;
+      moved = true;
+    }
+  }
+};
+template <typename T, bool PGON> __device__ RB_POLY_FN T poly_dist_out(const double* p, V3<T> q, V3<T> d) {
+  const PolyView v = poly_view<PGON>(p);
+  PolyFirstEntry<T> f{T(kBig)};
+  for (int i = 0; i + 1 < v.nz; i++) poly_pieces<T, PGON>(v, i, q, d, f);
+  return f.best;
+}
+// the end of the run of touching intervals that holds the start point
+template <typename T, bool PGON> __device__ RB_POLY_FN T poly_dist_in(const double* p, V3<T> q, V3<T> d) {
+  const PolyView v = poly_view<PGON>(p);
+  T cur = T(0);
+  for (int guard = 0; guard < 3 * v.nz + 2; guard++) {
+    PolyAdvance<T> f{cur, false};
+    for (int i = 0; i + 1 < v.nz && !f.moved; i++) poly_pieces<T, PGON>(v, i, q, d, f);
+    if (!f.moved) break;
+    cur = f.cur;
+  }
+  return cur;
+}
+// radial measure: distance from the axis (Pcon); for a Pgon the projection on
+// the side of the point's own sector, the largest over all sides
+template <typename T, bool PGON> __device__ __forceinline__ T poly_rho(const PolyView& v, V3<T> q, int& face) {
+  face = -1;
+  if (!PGON) return sqrt(q.x * q.x + q.y * q.y);
+  T r = T(-kBig);
+  for (int k = 0; k < v.ne; k++) {
+    const T w = q.x * T(v.faces[2 * k]) + q.y * T(v.faces[2 * k + 1]);
+    if (w > r) { r = w; face = k; }
+  }
+  return r;
+}
+template <typename T> __device__ __forceinline__ bool poly_in_wedge(const PolyView& v, V3<T> q) {
+  if (v.full) return true;
+  T n1x, n1y, n2x, n2y;
+  poly_wedge(v, n1x, n1y, n2x, n2y);
+  const bool h1 = n1x * q.x + n1y * q.y >= T(0), h2 = n2x * q.x + n2y * q.y >= T(0);
+  return v.reflex ? (h1 || h2) : (h1 && h2);
+}
+template <typename T, bool PGON> __device__ RB_POLY_FN bool poly_contains(const double* p, V3<T> q) {
+  const PolyView v = poly_view<PGON>(p);
+  if (q.z < T(v.z[0]) || q.z > T(v.z[3 * (v.nz - 1)])) return false;
+  if (!poly_in_wedge(v, q)) return false;
+  int face;
+  const T rho = poly_rho<T, PGON>(v, q, face);
+  for (int i = 0; i + 1 < v.nz; i++) {
+    const T za = T(v.z[3 * i]), zb = T(v.z[3 * (i + 1)]);
+    if (q.z < za || q.z > zb) continue;
+    T rmin, rmax;
+    if (zb - za < T(1e-8)) {
+      rmin = T(fmin(v.z[3 * i + 1], v.z[3 * (i + 1) + 1]));
+      rmax = T(fmax(v.z[3 * i + 2], v.z[3 * (i + 1) + 2]));
+    } else {
+      const T f = (q.z - za) / (zb - za);
+      rmin = T(v.z[3 * i + 1]) + f * (T(v.z[3 * (i + 1) + 1]) - T(v.z[3 * i + 1]));
+      rmax = T(v.z[3 * i + 2]) + f * (T(v.z[3 * (i + 1) + 2]) - T(v.z[3 * i + 2]));
+    }
+    if (rho >= rmin && rho <= rmax) return true;
+  }
+  return false;
+}
+// normal of the nearest surface: a section plane, the inner or outer surface
+// of the point's segment, or a phi half-plane
+template <typename T, bool PGON> __device__ RB_POLY_FN V3<T> poly_normal(const double* p, V3<T> q, V3<T> d) {
+  const PolyView v = poly_view<PGON>(p);
+  T best = T(kBig);
+  V3<T> n = mk<T>(T(0), T(0), T(1));
+  for (int k = 0; k < v.nz; k++) {
+    const T dz = fabs(q.z - T(v.z[3 * k]));
+    if (dz < best) best = dz;
+  }
+  int face;
+  const T rho = poly_rho<T, PGON>(v, q, face);
+  T cx, sy;
+  if (PGON) {
+    cx = T(v.faces[2 * face]);
+    sy = T(v.faces[2 * face + 1]);
+  } else {
+    cx = rho > T(0) ? q.x / rho : T(1);
+    sy = rho > T(0) ? q.y / rho : T(0);
+  }
+  for (int i = 0; i + 1 < v.nz; i++) {
+    const T z0 = T(v.z[3 * i]), z1 = T(v.z[3 * (i + 1)]);
+    if (!(z1 - z0 > T(0)) || q.z < z0 - T(1e-9) || q.z > z1 + T(1e-9)) continue;
+    for (int col = 1; col <= 2; col++) {
+      if (col == 1 && !poly_has_inner(v, i)) continue;
+      const T r0 = T(v.z[3 * i + col]);
+      const T s = (T(v.z[3 * (i + 1) + col]) - r0) / (z1 - z0);
+      const T len = sqrt(T(1) + s * s);
+      const T dist = fabs(rho - r0 - s * (q.z - z0)) / len;
+      if (dist < best) {
+        best = dist;
+        n = mk<T>(cx / len, sy / len, -s / len);
+      }
+    }
+  }
+  if (!v.full) {
+    T nx[2], ny[2];
+    poly_wedge(v, nx[0], ny[0], nx[1], ny[1]);
+    for (int k = 0; k < 2; k++) {
+      // the half-plane reaches out from the axis along (cos phi, sin phi)
+      const T ux = T(v.wedge[2 * k]), uy = T(v.wedge[2 * k + 1]);
+      if (ux * q.x + uy * q.y < T(0)) continue;
+      const T dist = fabs(nx[k] * q.x + ny[k] * q.y);
+      if (dist < best) {
+        best = dist;
+        n = mk<T>(nx[k], ny[k], T(0));
+      }
+    }
+  }
+  if (dot(n, d) < T(0)) n = scale(n, T(-1));
+  return n;
+}
+
 // ------------------------------------------------------------ aspheric disk
 // AGeoAsphericDisk (src/AGeoAsphericDisk.cxx). params: z1 c1 z2 c2 rmax rmin
 // kappa1 kappa2 n1 n2 K1[n1] K2[n2]. Both distances are the minimum over the
@@ -948,49 +1269,57 @@ template <typename T> __device__ RB_HEAVY_FN V3<T> wins_normal(const double* p, 
 #endif
 
 // ------------------------------------------------------------ dispatch
-template <typename T> __device__ RB_PRIM_FN bool prim_contains(int type, const double* p, V3<T> q) {
+template <typename T, bool EXT> __device__ RB_PRIM_FN bool prim_contains(int type, const double* p, V3<T> q) {
   switch (type) {
     case RB_SHAPE_BOX: return box_contains<T>(p, q);
     case RB_SHAPE_TUBE: return tube_contains<T>(p, q);
     case RB_SHAPE_SPHERE: return sphere_contains<T>(p, q);
     case RB_SHAPE_PARABOLOID: return para_contains<T>(p, q);
     case RB_SHAPE_PGON: return pgon_contains<T>(p, q);
+    case kShapePgonGeneral: if constexpr (EXT) return poly_contains<T, true>(p, q); break;
+    case RB_SHAPE_PCON: if constexpr (EXT) return poly_contains<T, false>(p, q); break;
     case RB_SHAPE_ASPHERIC: return asph_contains<T>(p, q);
     case RB_SHAPE_WINSTON: return wins_contains<T>(p, q);
   }
   return false;
 }
-template <typename T> __device__ RB_PRIM_FN T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T, bool EXT> __device__ RB_PRIM_FN T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_dist_out<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_dist_out<T>(p, q, d);
     case RB_SHAPE_SPHERE: return sphere_dist<T, true>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_dist_out<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_dist_out<T>(p, q, d);
+    case kShapePgonGeneral: if constexpr (EXT) return poly_dist_out<T, true>(p, q, d); break;
+    case RB_SHAPE_PCON: if constexpr (EXT) return poly_dist_out<T, false>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_dist_out<T>(p, q, d);
   }
   return T(kBig);
 }
-template <typename T> __device__ RB_PRIM_FN T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T, bool EXT> __device__ RB_PRIM_FN T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_dist_in<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_dist_in<T>(p, q, d);
     case RB_SHAPE_SPHERE: return sphere_dist<T, false>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_dist_in<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_dist_in<T>(p, q, d);
+    case kShapePgonGeneral: if constexpr (EXT) return poly_dist_in<T, true>(p, q, d); break;
+    case RB_SHAPE_PCON: if constexpr (EXT) return poly_dist_in<T, false>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_dist_in<T>(p, q, d);
   }
   return T(0);
 }
-template <typename T> __device__ RB_PRIM_FN V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T, bool EXT> __device__ RB_PRIM_FN V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_normal<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_normal<T>(p, q, d);
     case RB_SHAPE_SPHERE: return sphere_normal<T>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_normal<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_normal<T>(p, q, d);
+    case kShapePgonGeneral: if constexpr (EXT) return poly_normal<T, true>(p, q, d); break;
+    case RB_SHAPE_PCON: if constexpr (EXT) return poly_normal<T, false>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_normal<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_normal<T>(p, q, d);
   }
@@ -1004,29 +1333,29 @@ template <typename T> __device__ RB_PRIM_FN V3<T> prim_normal(int type, const do
 template <typename T> __device__ __forceinline__ T bool_push() { return T(kBoolPush); }
 template <> __device__ __forceinline__ float bool_push<float>() { return 2e-4f; }
 
-template <typename T, int D> struct Csg {
+template <typename T, int D, bool EXT> struct Csg {
   __device__ RB_CSG_FN static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return prim_contains<T>(sh.type, p, q);
+    if (sh.type != RB_SHAPE_COMPOSITE) return prim_contains<T, EXT>(sh.type, p, q);
     int op = int(p[0]), L = int(p[1]), R = int(p[2]);
-    bool a = Csg<T, D - 1>::contains(S, L, m2l<T>(p + 3, p + 12, q));
-    if (op == RB_OP_UNION) return a || Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, q));
-    if (op == RB_OP_SUBTRACTION) return a && !Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, q));
-    return a && Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, q));
+    bool a = Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, q));
+    if (op == RB_OP_UNION) return a || Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, q));
+    if (op == RB_OP_SUBTRACTION) return a && !Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, q));
+    return a && Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, q));
   }
   __device__ static Hit child(const DScene& S, const double* p, int side, bool out, V3<T> q, V3<T> d) {
     const double* m = p + (side ? 15 : 3);
     int c = int(p[side ? 2 : 1]);
     V3<T> lq = m2l<T>(m, m + 9, q), ld = m2lv<T>(m, d);
-    Hit h = out ? Csg<T, D - 1>::dist_out(S, c, lq, ld) : Csg<T, D - 1>::dist_in(S, c, lq, ld);
+    Hit h = out ? Csg<T, D - 1, EXT>::dist_out(S, c, lq, ld) : Csg<T, D - 1, EXT>::dist_in(S, c, lq, ld);
     h.leaf = leaf_push(h.leaf, side);
     return h;
   }
   __device__ RB_CSG_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T>(sh.type, p, q, d)), 0u, 0u};
+    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T, EXT>(sh.type, p, q, d)), 0u, 0u};
     int op = int(p[0]), L = int(p[1]), R = int(p[2]);
     if (op == RB_OP_UNION) {
       Hit a = child(S, p, 0, true, q, d), b = child(S, p, 1, true, q, d);
@@ -1038,8 +1367,8 @@ template <typename T, int D> struct Csg {
     for (int it = 0; it < 64; it++) {
       V3<T> x = axpy(q, s, d);
       V3<T> xp = axpy(x, push, d);
-      bool inA = Csg<T, D - 1>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
-      bool inB = Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
+      bool inA = Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
+      bool inB = Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
       Hit h;
       if (op == RB_OP_SUBTRACTION) {
         if (inA && !inB) return {double(s), last.leaf, last.exiting};
@@ -1068,7 +1397,7 @@ template <typename T, int D> struct Csg {
   __device__ RB_CSG_FN static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T>(sh.type, p, q, d)), 0u, 1u};
+    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T, EXT>(sh.type, p, q, d)), 0u, 1u};
     int op = int(p[0]), L = int(p[1]), R = int(p[2]);
     if (op == RB_OP_INTERSECTION) {
       Hit a = child(S, p, 0, false, q, d), b = child(S, p, 1, false, q, d);
@@ -1084,8 +1413,8 @@ template <typename T, int D> struct Csg {
     for (int it = 0; it < 64; it++) {
       V3<T> x = axpy(q, s, d);
       V3<T> xp = axpy(x, push, d);
-      bool inA = Csg<T, D - 1>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
-      bool inB = Csg<T, D - 1>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
+      bool inA = Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
+      bool inB = Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
       if (!inA && !inB) return {double(s), last.leaf, last.exiting};
       Hit ha = {0.0, 0u, 1u}, hb = {0.0, 0u, 1u};
       if (inA) ha = child(S, p, 0, false, x, d);
@@ -1101,10 +1430,10 @@ template <typename T, int D> struct Csg {
   __device__ RB_CSG_FN static V3<T> normal(const DScene& S, int shape, uint32_t path, int depth, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return prim_normal<T>(sh.type, p, q, d);
+    if (sh.type != RB_SHAPE_COMPOSITE) return prim_normal<T, EXT>(sh.type, p, q, d);
     int side = depth > 0 ? int(path & 1u) : 0;
     const double* m = p + (side ? 15 : 3);
-    V3<T> n = Csg<T, D - 1>::normal(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0,
+    V3<T> n = Csg<T, D - 1, EXT>::normal(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0,
                                     m2l<T>(m, m + 9, q), m2lv<T>(m, d));
     return l2mv<T>(m, n);
   }
@@ -1118,32 +1447,32 @@ template <typename T, int D> struct Csg {
     if (sh.type != RB_SHAPE_COMPOSITE) {
       T t0 = s0 - back > T(0) ? s0 - back : T(0);
       V3<T> q0 = axpy(q, t0, d);
-      T t = exiting ? prim_dist_in<T>(sh.type, p, q0, d) : prim_dist_out<T>(sh.type, p, q0, d);
+      T t = exiting ? prim_dist_in<T, EXT>(sh.type, p, q0, d) : prim_dist_out<T, EXT>(sh.type, p, q0, d);
       return t >= T(kBig) ? kBig : double(t0 + t);
     }
     int side = depth > 0 ? int(path & 1u) : 0;
     const double* m = p + (side ? 15 : 3);
-    return Csg<T, D - 1>::polish(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0, exiting,
+    return Csg<T, D - 1, EXT>::polish(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0, exiting,
                                  m2l<T>(m, m + 9, q), m2lv<T>(m, d), s0, back);
   }
 };
 // depth exhausted: primitives only (the host rejects deeper expressions)
-template <typename T> struct Csg<T, 0> {
+template <typename T, bool EXT> struct Csg<T, 0, EXT> {
   __device__ RB_CSG0_FN static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
-    return prim_contains<T>(sh.type, S.params + sh.off, q);
+    return prim_contains<T, EXT>(sh.type, S.params + sh.off, q);
   }
   __device__ RB_CSG0_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
-    return {double(prim_dist_out<T>(sh.type, S.params + sh.off, q, d)), 0u, 0u};
+    return {double(prim_dist_out<T, EXT>(sh.type, S.params + sh.off, q, d)), 0u, 0u};
   }
   __device__ RB_CSG0_FN static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
-    return {double(prim_dist_in<T>(sh.type, S.params + sh.off, q, d)), 0u, 1u};
+    return {double(prim_dist_in<T, EXT>(sh.type, S.params + sh.off, q, d)), 0u, 1u};
   }
   __device__ RB_CSG0_FN static V3<T> normal(const DScene& S, int shape, uint32_t, int, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
-    return prim_normal<T>(sh.type, S.params + sh.off, q, d);
+    return prim_normal<T, EXT>(sh.type, S.params + sh.off, q, d);
   }
   __device__ RB_CSG0_FN static double polish(const DScene& S, int shape, uint32_t, int, bool exiting, V3<T> q, V3<T> d, T s0,
                                   T back) {
@@ -1151,7 +1480,7 @@ template <typename T> struct Csg<T, 0> {
     const double* p = S.params + sh.off;
     T t0 = s0 - back > T(0) ? s0 - back : T(0);
     V3<T> q0 = axpy(q, t0, d);
-    T t = exiting ? prim_dist_in<T>(sh.type, p, q0, d) : prim_dist_out<T>(sh.type, p, q0, d);
+    T t = exiting ? prim_dist_in<T, EXT>(sh.type, p, q0, d) : prim_dist_out<T, EXT>(sh.type, p, q0, d);
     return t >= T(kBig) ? kBig : double(t0 + t);
   }
 };
@@ -1164,26 +1493,26 @@ __device__ __forceinline__ V3<T> shape_normal(const SC& S, int shape, uint32_t l
   const uint32_t path = leaf & 0xffffffu;
   const int depth = int(leaf >> 24);
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0>::normal(S, shape, path, depth, q, d);
-    case 1: return Csg<T, 1>::normal(S, shape, path, depth, q, d);
+    case 0: return Csg<T, 0, SC::ext_on>::normal(S, shape, path, depth, q, d);
+    case 1: return Csg<T, 1, SC::ext_on>::normal(S, shape, path, depth, q, d);
   }
-  return Csg<T, SC::max_depth>::normal(S, shape, path, depth, q, d);
+  return Csg<T, SC::max_depth, SC::ext_on>::normal(S, shape, path, depth, q, d);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ bool shape_contains(const SC& S, int shape, V3<T> q) {
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0>::contains(S, shape, q);
-    case 1: return Csg<T, 1>::contains(S, shape, q);
+    case 0: return Csg<T, 0, SC::ext_on>::contains(S, shape, q);
+    case 1: return Csg<T, 1, SC::ext_on>::contains(S, shape, q);
   }
-  return Csg<T, SC::max_depth>::contains(S, shape, q);
+  return Csg<T, SC::max_depth, SC::ext_on>::contains(S, shape, q);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ Hit shape_dist(const SC& S, int shape, V3<T> q, V3<T> d, bool out) {
   switch (S.shapes[shape].depth) {
-    case 0: return out ? Csg<T, 0>::dist_out(S, shape, q, d) : Csg<T, 0>::dist_in(S, shape, q, d);
-    case 1: return out ? Csg<T, 1>::dist_out(S, shape, q, d) : Csg<T, 1>::dist_in(S, shape, q, d);
+    case 0: return out ? Csg<T, 0, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 0, SC::ext_on>::dist_in(S, shape, q, d);
+    case 1: return out ? Csg<T, 1, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 1, SC::ext_on>::dist_in(S, shape, q, d);
   }
-  return out ? Csg<T, SC::max_depth>::dist_out(S, shape, q, d) : Csg<T, SC::max_depth>::dist_in(S, shape, q, d);
+  return out ? Csg<T, SC::max_depth, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, SC::max_depth, SC::ext_on>::dist_in(S, shape, q, d);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t leaf, bool exiting, V3<T> q,
@@ -1191,10 +1520,10 @@ __device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t 
   const uint32_t path = leaf & 0xffffffu;
   const int depth = int(leaf >> 24);
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0>::polish(S, shape, path, depth, exiting, q, d, s0, back);
-    case 1: return Csg<T, 1>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+    case 0: return Csg<T, 0, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+    case 1: return Csg<T, 1, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
   }
-  return Csg<T, SC::max_depth>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+  return Csg<T, SC::max_depth, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
 }
 
 // ------------------------------------------------------------ physics

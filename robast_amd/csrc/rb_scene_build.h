@@ -122,6 +122,15 @@ struct BvhBuilder {
   }
 };
 
+// A TGeoPgon the convex-frustum code takes: full phi and rmin = 0 throughout.
+inline bool pgon_is_simple(const double* p) {
+  if (p[1] < 360.0) return false;
+  const int nz = int(p[3]);
+  for (int k = 0; k < nz; k++)
+    if (p[4 + 3 * k + 1] != 0.0) return false;
+  return true;
+}
+
 int csg_depth(const rb_scene_desc* d, int shape, int guard) {
   if (guard > 16) return 99;
   const rb_shape_desc& s = d->shapes[shape];
@@ -152,13 +161,21 @@ int validate(std::string& err, const rb_scene_desc* d) {
       case RB_SHAPE_PARABOLOID:
         if (s.n_params != 5) return fail_msg(err, -1, "scene: paraboloid needs 5 parameters");
         break;
-      case RB_SHAPE_PGON: {
+      case RB_SHAPE_PGON:
+      case RB_SHAPE_PCON: {
+        const bool pgon = s.type == RB_SHAPE_PGON;
         const double* p = d->params + s.param_offset;
-        int nz = int(p[3]);
-        if (nz < 2 || int(p[2]) < 3 || s.n_params != 4 + 3 * nz + 2 * int(p[2]) || p[1] != 360.0)
-          return fail_msg(err, -3, "scene: unsupported TGeoPgon parameters");
-        for (int k = 0; k < nz; k++)
-          if (p[4 + 3 * k + 1] != 0.0) return fail_msg(err, -3, "scene: TGeoPgon with rmin > 0 is unsupported");
+        const int nz = int(p[pgon ? 3 : 2]), ne = pgon ? int(p[2]) : 0;
+        if (nz < 2 || (pgon && ne < 1) || s.n_params != (pgon ? 4 : 3) + 3 * nz + 2 * ne + 4)
+          return fail_msg(err, -1, pgon ? "scene: TGeoPgon parameter count" : "scene: TGeoPcon parameter count");
+        if (!(p[1] > 0.0)) return fail_msg(err, -1, "scene: TGeoPcon/TGeoPgon needs dphi > 0");
+        const double* z = p + (pgon ? 4 : 3);
+        for (int k = 0; k < nz; k++) {
+          if (k > 0 && z[3 * k] < z[3 * (k - 1)]) return fail_msg(err, -1, "scene: TGeoPcon/TGeoPgon sections must not decrease in z");
+          if (!(z[3 * k + 1] >= 0.0) || !(z[3 * k + 2] >= z[3 * k + 1]))
+            return fail_msg(err, -1, "scene: TGeoPcon/TGeoPgon needs 0 <= rmin <= rmax");
+        }
+        if (pgon && p[1] >= 360.0 && ne < 3) return fail_msg(err, -1, "scene: a full TGeoPgon needs 3 or more sides");
         break;
       }
       case RB_SHAPE_ASPHERIC: {
@@ -262,6 +279,7 @@ struct SceneLayout {
   int32_t max_depth = 0;                  // deepest boolean nesting of any shape
   bool has_lens = false;                  // a placed node is an ALens
   bool has_borders = false;               // border-surface conditions are defined
+  bool has_poly = false;                  // a TGeoPcon or a general TGeoPgon (the deep kernel form's code)
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -297,6 +315,9 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
   lay.max_depth = 0;
   for (int i = 0; i < d->n_shapes; i++) {
     shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset, csg_depth(d, i, 0), 0};
+    if (shapes[i].type == RB_SHAPE_PGON && !pgon_is_simple(d->params + d->shapes[i].param_offset))
+      shapes[i].type = kShapePgonGeneral;  // rmin > 0 or a phi range: the general polygon code
+    if (shapes[i].type == kShapePgonGeneral || shapes[i].type == RB_SHAPE_PCON) lay.has_poly = true;
     lay.max_depth = std::max(lay.max_depth, shapes[i].depth);
   }
   blob.clear();

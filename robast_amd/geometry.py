@@ -286,18 +286,30 @@ class TGeoPcon(TGeoShape):
         self.rmax = [0.0] * self.nz
 
     def DefineSection(self, i, z, rmin, rmax):
+        """TGeoPcon::DefineSection: once the last section is defined, sections
+        given in decreasing z are put in increasing order."""
         self.z[i], self.rmin[i], self.rmax[i] = float(z), float(rmin), float(rmax)
+        if i == self.nz - 1 and self.z[0] > self.z[i]:
+            self.z.reverse()
+            self.rmin.reverse()
+            self.rmax.reverse()
 
     def _check(self):
-        if any(r != 0 for r in self.rmin) or self.dphi != 360.0:
-            raise NotImplementedError(f"{type(self).__name__} with rmin > 0 or a phi range is not supported")
+        for i in range(1, self.nz):
+            if self.z[i] < self.z[i - 1]:
+                raise ValueError(f"{type(self).__name__} {self.GetName()}: sections must not decrease in z")
+
+    def _wedge(self):
+        """{cos sin} of phi1 and of phi1 + dphi (the phi half-planes)."""
+        a, b = math.radians(self.phi1), math.radians(self.phi1 + self.dphi)
+        return [math.cos(a), math.sin(a), math.cos(b), math.sin(b)]
 
     def params(self):
         self._check()
         p = [self.phi1, self.dphi, self.nz]
         for i in range(self.nz):
             p += [self.z[i], self.rmin[i], self.rmax[i]]
-        return p
+        return p + self._wedge()
 
     def local_bbox(self):
         r = max(self.rmax)
@@ -329,10 +341,11 @@ class TGeoPgon(TGeoPcon):
         for k in range(self.nedges):
             ph = math.radians(self.phi1 + div * (k + 0.5))
             p += [math.cos(ph), math.sin(ph)]
-        return p
+        return p + self._wedge()
 
     def local_bbox(self):
-        r = max(self.rmax) / math.cos(math.pi / self.nedges)
+        # the polygon's corners lie at rmax / cos(half a sector) from the axis
+        r = max(self.rmax) / math.cos(math.radians(min(self.dphi, 360.0) / self.nedges / 2))
         z0, z1 = min(self.z), max(self.z)
         return np.array([0, 0, 0.5 * (z0 + z1)]), np.array([r, r, 0.5 * (z1 - z0)])
 

@@ -36,6 +36,35 @@ def _shapes():
     pg.DefineSection(1, 0, 0, 7)
     pg.DefineSection(2, 4, 0, 4)
     out["pgon"] = pg
+    pc = G.TGeoPcon("pc", 0, 360, 4)
+    pc.DefineSection(0, -6, 0, 5)
+    pc.DefineSection(1, -1, 2, 8)
+    pc.DefineSection(2, 3, 3, 6)
+    pc.DefineSection(3, 5, 0, 9)
+    out["pcon"] = pc
+    pw = G.TGeoPcon("pw", 20, 120, 3)
+    pw.DefineSection(0, -5, 0, 7)
+    pw.DefineSection(1, 1, 1, 9)
+    pw.DefineSection(2, 4, 0, 4)
+    out["pcon_wedge"] = pw
+    pr = G.TGeoPcon("pr", -30, 270, 2)
+    pr.DefineSection(0, -4, 3, 9)
+    pr.DefineSection(1, 5, 2, 6)
+    out["pcon_reflex"] = pr
+    ph = G.TGeoPgon("ph", 10, 360, 5, 3)
+    ph.DefineSection(0, -6, 2, 7)
+    ph.DefineSection(1, 0, 4, 9)
+    ph.DefineSection(2, 5, 1, 5)
+    out["pgon_hollow"] = ph
+    pgw = G.TGeoPgon("pgw", 40, 150, 3, 2)
+    pgw.DefineSection(0, -5, 2, 8)
+    pgw.DefineSection(1, 4, 3, 10)
+    out["pgon_wedge"] = pgw
+    pgr = G.TGeoPgon("pgr", 0, 300, 5, 3)
+    pgr.DefineSection(0, -5, 0, 6)
+    pgr.DefineSection(1, 0, 0, 9)
+    pgr.DefineSection(2, 5, 0, 7)
+    out["pgon_reflex"] = pgr
     G.TGeoBBox("cb", 6, 6, 6)
     G.TGeoTube("ct", 0, 3, 9)
     G.TGeoTranslation("ctr", 1.5, 0, 0).RegisterYourself()
@@ -118,10 +147,40 @@ def _contains_closed_form(name, q):
         r = x * np.cos(ph0) + y * np.sin(ph0)
         rmax = np.where(z <= 0, 5 + (z + 5) / 5 * 2, 7 + z / 4 * (-3))
         return (z >= -5) & (z <= 4) & (r <= rmax)
+    if name in _POLY:
+        phi1, dphi, nedges, sections = _POLY[name]
+        zs, rmins, rmaxs = (np.array(v, dtype=float) for v in zip(*sections))
+        rel = (np.degrees(np.arctan2(y, x)) - phi1) % 360.0
+        in_phi = rel <= dphi
+        if nedges:
+            div = min(dphi, 360.0) / nedges
+            mid = np.radians(phi1 + (np.minimum(np.floor(rel / div), nedges - 1) + 0.5) * div)
+            r = x * np.cos(mid) + y * np.sin(mid)
+        else:
+            r = np.sqrt(r2)
+        inside = np.zeros(len(q), dtype=bool)
+        for i in range(len(zs) - 1):
+            seg = (z >= zs[i]) & (z <= zs[i + 1])
+            f = (z - zs[i]) / (zs[i + 1] - zs[i])
+            lo = rmins[i] + f * (rmins[i + 1] - rmins[i])
+            hi = rmaxs[i] + f * (rmaxs[i + 1] - rmaxs[i])
+            inside |= seg & (r >= lo) & (r <= hi)
+        return inside & in_phi
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("name", ["box", "tube", "sphere", "paraboloid", "pgon"])
+# (phi1, dphi, nedges (0: Pcon), sections) of the polycones and polygons above
+_POLY = {
+    "pcon": (0, 360, 0, [(-6, 0, 5), (-1, 2, 8), (3, 3, 6), (5, 0, 9)]),
+    "pcon_wedge": (20, 120, 0, [(-5, 0, 7), (1, 1, 9), (4, 0, 4)]),
+    "pcon_reflex": (-30, 270, 0, [(-4, 3, 9), (5, 2, 6)]),
+    "pgon_hollow": (10, 360, 5, [(-6, 2, 7), (0, 4, 9), (5, 1, 5)]),
+    "pgon_wedge": (40, 150, 3, [(-5, 2, 8), (4, 3, 10)]),
+    "pgon_reflex": (0, 300, 5, [(-5, 0, 6), (0, 0, 9), (5, 0, 7)]),
+}
+
+
+@pytest.mark.parametrize("name", ["box", "tube", "sphere", "paraboloid", "pgon"] + sorted(_POLY))
 def test_contains_matches_closed_form(name):
     flat, sid = _shape_scene(SHAPES[name])
     q = np.random.default_rng(5).uniform(-12, 12, (20000, 3))

@@ -1,5 +1,6 @@
 """Seeded random scenes: every supported shape (box with an offset origin,
-tube with a bore, theta-limited sphere, spherical cap, paraboloid, Pgon,
+tube with a bore, theta-limited sphere, spherical cap, paraboloid, Pgon and
+Pcon -- full, hollow, over a wedge of at most 180 deg and over a wider one --,
 aspheric disk, Winston cone, and union / subtraction / intersection
 composites up to two levels) under random rotations, as mirrors, lenses,
 obscurations, focal surfaces and plain components, hit by random rays with
@@ -58,10 +59,20 @@ def _shape(rng, k, tag):
     if kind == "paraboloid":
         return kind, G.TGeoParaboloid(n, u(0, 2), u(6, 11), u(3, 8))
     if kind == "pgon":
-        pg = G.TGeoPgon(n, u(0, 60), 360, int(rng.integers(3, 9)), 3)
-        pg.DefineSection(0, -u(4, 8), 0, u(4, 8))
-        pg.DefineSection(1, 0, 0, u(6, 10))
-        pg.DefineSection(2, u(4, 8), 0, u(3, 7))
+        # a full polygon, or (TGeoPgon / TGeoPcon) hollow, over a wedge of at
+        # most 180 deg, or over a wider one
+        variant = int(rng.integers(0, 7))
+        pcon = variant >= 4
+        dphi = (360.0, 360.0, u(60, 180), u(200, 330))[variant % 4 if not pcon else (variant - 3)]
+        hollow = variant in (1, 4, 6)
+        rin = (lambda a, b: u(a, b)) if hollow else (lambda a, b: 0.0)
+        if pcon:
+            pg = G.TGeoPcon(n, u(0, 360), dphi, 3)
+        else:
+            pg = G.TGeoPgon(n, u(0, 60), dphi, int(rng.integers(3, 9)), 3)
+        pg.DefineSection(0, -u(4, 8), rin(1, 3), u(4, 8))
+        pg.DefineSection(1, 0, rin(1, 4), u(6, 10))
+        pg.DefineSection(2, u(4, 8), rin(0, 2), u(3, 7))
         return kind, pg
     if kind == "aspheric":
         a = G.AGeoAsphericDisk(n, -u(1, 3), 1.0 / u(30, 60), u(1, 3), 1.0 / u(40, 80), u(8, 12), u(0, 3))
