@@ -58,8 +58,11 @@ enum {
   RB_SHAPE_COMPOSITE = 7,  /* op left right lmat[12] rmat[12]       (TGeoBoolNode)    */
   RB_SHAPE_ASPHERIC = 8,   /* z1 c1 z2 c2 rmax rmin kappa1 kappa2 n1 n2 K1[n1] K2[n2]
                               Kn1[n1] Kn2[n2] with Kn_k = K_k rmax^2k (AGeoAsphericDisk) */
-  RB_SHAPE_WINSTON = 9     /* r1 r2 polyN theta dz f cos(theta) sin(theta) tan(pi/N)
+  RB_SHAPE_WINSTON = 9,    /* r1 r2 polyN theta dz f cos(theta) sin(theta) tan(pi/N)
                               {cos sin}(2 pi i / N) * N          (AGeoWinstonConePoly) */
+  RB_SHAPE_WINSTON2D = 10  /* r1 r2 2 theta dz f cos(theta) sin(theta) 0 1 0 -1 0 dy
+                              (AGeoWinstonCone2D: the profile of AGeoWinstonConePoly with two
+                              sides, at phi = 0 and pi, extruded over |y| <= dy) */
 };
 
 /* boolean operators of RB_SHAPE_COMPOSITE (TGeoUnion/TGeoSubtraction/TGeoIntersection) */

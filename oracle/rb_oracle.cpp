@@ -1137,6 +1137,79 @@ static V3 wins_normal(const double* p, V3 q, V3 d) {
   return n;
 }
 
+// ---------------------------------------------------------------- Winston cone (2D trough)
+// AGeoWinstonCone2D (src/AGeoWinstonCone2D.cxx): Contains (:175-190),
+// DistFromInside (:203-238), DistFromOutside (:241-322), ComputeNormal
+// (:120-172), with DistToParabola at phi = 0 and pi over an open angle of pi.
+// params: r1 r2 2 theta dz f cos sin 0 {1 0} {-1 0} dy
+static bool wins2d_contains(const double* p, V3 q) {
+  Wins w = wins_p(p);
+  if (std::fabs(q.y) > p[13] || std::fabs(q.z) > w.dz) return false;
+  double r;
+  if (!wins_r(w, q.z, r)) return false;
+  return std::fabs(q.x) <= r;
+}
+static double wins2d_dist_in(const double* p, V3 q, V3 d) {
+  Wins w = wins_p(p);
+  const double dy = p[13];
+  double s[4] = {kBig, kBig, kBig, kBig};
+  if (d.z < 0) s[0] = -(q.z + w.dz) / d.z;
+  else if (d.z > 0) s[0] = (w.dz - q.z) / d.z;
+  if (d.y < 0) s[1] = -(q.y + dy) / d.y;
+  else if (d.y > 0) s[1] = (dy - q.y) / d.y;
+  s[2] = wins_parabola(w, q, d, 0., kPi);
+  s[3] = wins_parabola(w, q, d, kPi, kPi);
+  return *std::min_element(s, s + 4);
+}
+static double wins2d_dist_out(const double* p, V3 q, V3 d) {
+  Wins w = wins_p(p);
+  const double dy = p[13];
+  if (q.z <= -w.dz) {
+    if (d.z <= 0) return kBig;
+    double sn = -(w.dz + q.z) / d.z;
+    if (std::fabs(q.x + sn * d.x) <= w.r2 && std::fabs(q.y + sn * d.y) <= dy) return sn;
+  } else if (q.z >= w.dz) {
+    if (d.z >= 0) return kBig;
+    double sn = (w.dz - q.z) / d.z;
+    if (std::fabs(q.x + sn * d.x) <= w.r1 && std::fabs(q.y + sn * d.y) <= dy) return sn;
+  }
+  if (q.y <= -dy) {
+    if (d.y <= 0) return kBig;
+    double sn = -(dy + q.y) / d.y, zn = q.z + sn * d.z, r;
+    if (std::fabs(zn) <= w.dz && wins_r(w, zn, r) && std::fabs(q.x + sn * d.x) <= r) return sn;
+  } else if (q.y >= dy) {
+    if (d.y >= 0) return kBig;
+    double sn = (dy - q.y) / d.y, zn = q.z + sn * d.z, r;
+    if (std::fabs(zn) <= w.dz && wins_r(w, zn, r) && std::fabs(q.x + sn * d.x) <= r) return sn;
+  }
+  double best = kBig;
+  for (double phi : {0., kPi}) {
+    double sn = wins_parabola(w, q, d, phi, kPi);
+    if (std::fabs(q.y + sn * d.y) <= dy) best = std::min(best, sn);
+  }
+  return best;
+}
+static V3 wins2d_normal(const double* p, V3 q, V3 d) {
+  Wins w = wins_p(p);
+  double saf[3] = {std::fabs(std::fabs(p[13]) - std::fabs(q.y)), std::fabs(std::fabs(w.dz) - std::fabs(q.z)), kBig};
+  double r;
+  if (wins_r(w, q.z, r)) saf[2] = std::fabs(r - std::fabs(q.x));
+  const int i = int(std::min_element(saf, saf + 3) - saf);
+  V3 n;
+  if (i == 0) {
+    n = v3(0, 1, 0);
+  } else if (i == 1) {
+    n = v3(0, 0, 1);
+  } else {
+    double g = 0;
+    wins_drdz(w, q.z, g);
+    n = q.x > 0 ? v3(1, 0, -g) : v3(1, 0, g);
+    n = mul(n, 1 / std::sqrt(dot(n, n)));
+  }
+  if (dot(n, d) < 0) n = mul(n, -1);
+  return n;
+}
+
 // ---------------------------------------------------------------- dispatch
 static bool contains(const Scene& sc, int shape, V3 q);
 static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d);
@@ -1159,12 +1232,12 @@ static Comp comp_of(const Scene& sc, int shape) {
 }
 
 static const int kInOp[] = {-1, OP_IN_BOX, OP_IN_TUBE, OP_IN_SPHERE, OP_IN_PARA, OP_IN_PGON, OP_IN_PGON, -1,
-                            OP_IN_ASPH, OP_IN_WINS};
+                            OP_IN_ASPH, OP_IN_WINS, OP_IN_WINS};
 static const int kDistOp[] = {-1, OP_DIST_BOX, OP_DIST_TUBE, OP_DIST_SPHERE, OP_DIST_PARA, OP_DIST_PGON,
-                              OP_DIST_PGON, -1, -1, OP_DIST_WINS_CAP};
+                              OP_DIST_PGON, -1, -1, OP_DIST_WINS_CAP, OP_DIST_WINS_CAP};
 static bool contains(const Scene& sc, int shape, V3 q) {
   const double* p = sc.P(shape);
-  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON && kInOp[sc.type(shape)] >= 0)
+  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON2D && kInOp[sc.type(shape)] >= 0)
     count_op(kInOp[sc.type(shape)]);
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return box_contains(p, q);
@@ -1173,6 +1246,7 @@ static bool contains(const Scene& sc, int shape, V3 q) {
     case RB_SHAPE_PARABOLOID: return para_contains(p, q);
     case RB_SHAPE_PGON: return pgon_simple(p) ? pgon_contains(p, q) : gpoly_contains(gpoly_p(p, true), q);
     case RB_SHAPE_PCON: return gpoly_contains(gpoly_p(p, false), q);
+    case RB_SHAPE_WINSTON2D: return wins2d_contains(p, q);
     case RB_SHAPE_ASPHERIC: return asph_contains(p, q);
     case RB_SHAPE_WINSTON: return wins_contains(p, q);
     case RB_SHAPE_COMPOSITE: {
@@ -1284,7 +1358,7 @@ static Hit comp_dist_in(const Scene& sc, int shape, V3 q, V3 d) {
 
 static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
   const double* p = sc.P(shape);
-  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON && kDistOp[sc.type(shape)] >= 0)
+  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON2D && kDistOp[sc.type(shape)] >= 0)
     count_op(kDistOp[sc.type(shape)]);
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return {box_dist_out(p, q, d), 0};
@@ -1294,6 +1368,7 @@ static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
     case RB_SHAPE_PGON:
       return {pgon_simple(p) ? pgon_dist_out(p, q, d) : gpoly_dist(gpoly_p(p, true), q, d, false), 0};
     case RB_SHAPE_PCON: return {gpoly_dist(gpoly_p(p, false), q, d, false), 0};
+    case RB_SHAPE_WINSTON2D: return {wins2d_dist_out(p, q, d), 0};
     case RB_SHAPE_ASPHERIC: return {asph_dist(p, q, d), 0};
     case RB_SHAPE_WINSTON: return {wins_dist_out(p, q, d), 0};
     case RB_SHAPE_COMPOSITE: return comp_dist_out(sc, shape, q, d);
@@ -1302,7 +1377,7 @@ static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
 }
 static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
   const double* p = sc.P(shape);
-  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON && kDistOp[sc.type(shape)] >= 0)
+  if (sc.type(shape) >= 0 && sc.type(shape) <= RB_SHAPE_WINSTON2D && kDistOp[sc.type(shape)] >= 0)
     count_op(kDistOp[sc.type(shape)]);
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return {box_dist_in(p, q, d), 0};
@@ -1312,6 +1387,7 @@ static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
     case RB_SHAPE_PGON:
       return {pgon_simple(p) ? pgon_dist_in(p, q, d) : gpoly_dist(gpoly_p(p, true), q, d, true), 0};
     case RB_SHAPE_PCON: return {gpoly_dist(gpoly_p(p, false), q, d, true), 0};
+    case RB_SHAPE_WINSTON2D: return {wins2d_dist_in(p, q, d), 0};
     case RB_SHAPE_ASPHERIC: return {asph_dist(p, q, d), 0};
     case RB_SHAPE_WINSTON: return {wins_dist_in(p, q, d), 0};
     case RB_SHAPE_COMPOSITE: return comp_dist_in(sc, shape, q, d);
@@ -1345,6 +1421,7 @@ static V3 shape_normal(const Scene& sc, int shape, uint32_t leaf, V3 q, V3 d) {
     case RB_SHAPE_PARABOLOID: n = para_normal(p, q, d); break;
     case RB_SHAPE_PGON: n = pgon_simple(p) ? pgon_normal(p, q, d) : gpoly_normal(gpoly_p(p, true), q, d); break;
     case RB_SHAPE_PCON: n = gpoly_normal(gpoly_p(p, false), q, d); break;
+    case RB_SHAPE_WINSTON2D: n = wins2d_normal(p, q, d); break;
     case RB_SHAPE_ASPHERIC: n = asph_normal(p, q, d); break;
     case RB_SHAPE_WINSTON: n = wins_normal(p, q, d); break;
     default: n = v3(0, 0, 1);

@@ -2,7 +2,8 @@
 (AOpticsManager::TraceNonSequential), built on librbgpu (HIP kernels for
 gfx950 behind the C ABI in include/rbgpu.h)."""
 from . import _abi
-from .geometry import (ABorderSurfaceCondition, AFocalSurface, AGeoAsphericDisk, AGeoWinstonConePoly, ALens, AMirror,
+from .geometry import (ABorderSurfaceCondition, AFocalSurface, AGeoAsphericDisk, AGeoWinstonCone2D,
+                       AGeoWinstonConePoly, ALens, AMirror,
                        AObscuration, AOpticalComponent, TGeoBBox,
                        TGeoCombiTrans, TGeoCompositeShape, TGeoHMatrix, TGeoParaboloid, TGeoPcon, TGeoPgon,
                        TGeoRotation, TGeoSphere, TGeoTranslation, TGeoTube, TGeoVolume, cm, deg, m, mm, nm,

@@ -24,6 +24,7 @@ RB_SHAPE_PCON = 6
 RB_SHAPE_COMPOSITE = 7
 RB_SHAPE_ASPHERIC = 8
 RB_SHAPE_WINSTON = 9
+RB_SHAPE_WINSTON2D = 10
 
 RB_OP_UNION, RB_OP_SUBTRACTION, RB_OP_INTERSECTION = 0, 1, 2
 

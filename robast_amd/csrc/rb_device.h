@@ -34,7 +34,7 @@ constexpr double kEps = 1e-6;        // AOpticsManager kEpsilon (src/AOpticsMana
 constexpr double kC = 2.99792458e10; // speed of light, cm/s
 constexpr double kBoolPush = 1e-8;   // inside-test offset in boolean-shape iteration
 constexpr double kPi = 3.14159265358979323846;
-constexpr int kMaxCsgDepth = 2;   // boolean nesting levels unrolled on the device
+constexpr int kMaxCsgDepth = 3;   // boolean nesting levels unrolled on the device
 
 // ------------------------------------------------------------ scene on HBM
 struct DNode {                 // 128 B
@@ -80,9 +80,10 @@ constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // The same scene, for kernels compiled for scenes whose composites nest at
 // most MAXD levels: the deeper composite call sites, and the registers they
 // make the kernel spill around them, are left out; likewise the border-surface
-// code unless BORDERS, and the TGeoPcon / general TGeoPgon code unless EXT
-// (compiled into the deep and the border-surface forms; a scene holding such
-// a shape is traced with the deep form).
+// code unless BORDERS, and the TGeoPcon / general TGeoPgon /
+// AGeoWinstonCone2D code unless EXT (compiled into the deep and the
+// border-surface forms; a scene holding such a shape is traced with the deep
+// form).
 template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1 || BORDERS)> struct DSceneOf : DScene {
   static constexpr int max_depth = MAXD;
   static constexpr bool borders_on = BORDERS;
@@ -1249,6 +1250,74 @@ template <typename T> __device__ RB_HEAVY_FN V3<T> wins_normal(const double* p, 
   return nn;
 }
 
+// ------------------------------------------------------------ Winston cone (2D trough)
+// AGeoWinstonCone2D (src/AGeoWinstonCone2D.cxx): the compound-parabolic
+// profile |x| <= R(z) of the cone above, extruded over |y| <= dy. params:
+// r1 r2 2 theta dz f cos(theta) sin(theta) 0 {1, 0} {-1, 0} dy -- the
+// polygonal cone's layout with its two sides at phi = 0 and pi, then dy.
+// The side crossings accept the half-plane of their side, as
+// DistToParabola(point, dir, phi, pi) does in both distances (:203-322).
+template <typename T> __device__ RB_POLY_FN bool wins2d_contains(const double* p, V3<T> q) {
+  return fabs(q.y) <= T(p[13]) && wins_contains<T>(p, q);
+}
+template <typename T> __device__ RB_POLY_FN T wins2d_dist_in(const double* p, V3<T> q, V3<T> d) {
+  const T dy = T(p[13]);
+  T best = wins_dist_in<T>(p, q, d);
+  if (d.y < T(0)) {
+    const T s = -(q.y + dy) / d.y;
+    best = s < best ? s : best;
+  } else if (d.y > T(0)) {
+    const T s = (dy - q.y) / d.y;
+    best = s < best ? s : best;
+  }
+  return best;
+}
+// the end caps (within the apertures and |y| <= dy), the y walls (within
+// |z| <= dz and |x| <= R(z)), then the two parabolic sides within |y| <= dy
+template <typename T> __device__ RB_POLY_FN T wins2d_dist_out(const double* p, V3<T> q, V3<T> d) {
+  const T dz = T(p[4]), dy = T(p[13]);
+  if (q.z <= -dz || q.z >= dz) {
+    const bool below = q.z <= -dz;
+    if (below ? d.z <= T(0) : d.z >= T(0)) return T(kBig);
+    const T s = below ? -(dz + q.z) / d.z : (dz - q.z) / d.z;
+    if (fabs(q.x + s * d.x) <= T(p[below ? 1 : 0]) && fabs(q.y + s * d.y) <= dy) return s;
+  }
+  if (q.y <= -dy || q.y >= dy) {
+    const bool below = q.y <= -dy;
+    if (below ? d.y <= T(0) : d.y >= T(0)) return T(kBig);
+    const T s = below ? -(dy + q.y) / d.y : (dy - q.y) / d.y;
+    const T zn = q.z + s * d.z;
+    T r;
+    if (fabs(zn) <= dz && wins_r<T>(p, zn, r) && fabs(q.x + s * d.x) <= r) return s;
+  }
+  T best = T(kBig);
+  for (int i = 0; i < 2; i++) {
+    const T s = wins_parabola<T>(p, i, q, d, true);
+    if (s < best && fabs(q.y + s * d.y) <= dy) best = s;
+  }
+  return best;
+}
+// the nearest of the y walls, the end caps and the parabolic side, in that
+// order of preference (ComputeNormal, :120-172)
+template <typename T> __device__ RB_POLY_FN V3<T> wins2d_normal(const double* p, V3<T> q, V3<T> d) {
+  const T saf0 = fabs(T(p[13]) - fabs(q.y)), saf1 = fabs(T(p[4]) - fabs(q.z));
+  T r, saf2 = T(kBig);
+  if (wins_r<T>(p, q.z, r)) saf2 = fabs(r - fabs(q.x));
+  V3<T> n;
+  if (saf0 <= saf1 && saf0 <= saf2) {
+    n = mk<T>(T(0), T(1), T(0));
+  } else if (saf1 <= saf2) {
+    n = mk<T>(T(0), T(0), T(1));
+  } else {
+    T g = T(0);
+    wins_drdz<T>(p, q.z, g);
+    n = mk<T>(T(1), T(0), q.x > T(0) ? -g : g);
+    n = scale(n, T(1) / sqrt(dot(n, n)));
+  }
+  if (dot(n, d) < T(0)) n = scale(n, T(-1));
+  return n;
+}
+
 // Call structure of the shape code. The node-level queries and the
 // primitive dispatch are inlined; the composite levels (Csg<T, D>) stay real
 // calls, which keeps each boolean level's register allocation to itself.
@@ -1278,6 +1347,7 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN bool prim_contains(int typ
     case RB_SHAPE_PGON: return pgon_contains<T>(p, q);
     case kShapePgonGeneral: if constexpr (EXT) return poly_contains<T, true>(p, q); break;
     case RB_SHAPE_PCON: if constexpr (EXT) return poly_contains<T, false>(p, q); break;
+    case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_contains<T>(p, q); break;
     case RB_SHAPE_ASPHERIC: return asph_contains<T>(p, q);
     case RB_SHAPE_WINSTON: return wins_contains<T>(p, q);
   }
@@ -1292,6 +1362,7 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN T prim_dist_out(int type, 
     case RB_SHAPE_PGON: return pgon_dist_out<T>(p, q, d);
     case kShapePgonGeneral: if constexpr (EXT) return poly_dist_out<T, true>(p, q, d); break;
     case RB_SHAPE_PCON: if constexpr (EXT) return poly_dist_out<T, false>(p, q, d); break;
+    case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_dist_out<T>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_dist_out<T>(p, q, d);
   }
@@ -1306,6 +1377,7 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN T prim_dist_in(int type, c
     case RB_SHAPE_PGON: return pgon_dist_in<T>(p, q, d);
     case kShapePgonGeneral: if constexpr (EXT) return poly_dist_in<T, true>(p, q, d); break;
     case RB_SHAPE_PCON: if constexpr (EXT) return poly_dist_in<T, false>(p, q, d); break;
+    case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_dist_in<T>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_dist_in<T>(p, q, d);
   }
@@ -1320,6 +1392,7 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN V3<T> prim_normal(int type
     case RB_SHAPE_PGON: return pgon_normal<T>(p, q, d);
     case kShapePgonGeneral: if constexpr (EXT) return poly_normal<T, true>(p, q, d); break;
     case RB_SHAPE_PCON: if constexpr (EXT) return poly_normal<T, false>(p, q, d); break;
+    case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_normal<T>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_normal<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_normal<T>(p, q, d);
   }
@@ -1495,6 +1568,9 @@ __device__ __forceinline__ V3<T> shape_normal(const SC& S, int shape, uint32_t l
   switch (S.shapes[shape].depth) {
     case 0: return Csg<T, 0, SC::ext_on>::normal(S, shape, path, depth, q, d);
     case 1: return Csg<T, 1, SC::ext_on>::normal(S, shape, path, depth, q, d);
+    case 2:
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on>::normal(S, shape, path, depth, q, d);
+      break;
   }
   return Csg<T, SC::max_depth, SC::ext_on>::normal(S, shape, path, depth, q, d);
 }
@@ -1503,6 +1579,9 @@ __device__ __forceinline__ bool shape_contains(const SC& S, int shape, V3<T> q) 
   switch (S.shapes[shape].depth) {
     case 0: return Csg<T, 0, SC::ext_on>::contains(S, shape, q);
     case 1: return Csg<T, 1, SC::ext_on>::contains(S, shape, q);
+    case 2:
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on>::contains(S, shape, q);
+      break;
   }
   return Csg<T, SC::max_depth, SC::ext_on>::contains(S, shape, q);
 }
@@ -1511,6 +1590,9 @@ __device__ __forceinline__ Hit shape_dist(const SC& S, int shape, V3<T> q, V3<T>
   switch (S.shapes[shape].depth) {
     case 0: return out ? Csg<T, 0, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 0, SC::ext_on>::dist_in(S, shape, q, d);
     case 1: return out ? Csg<T, 1, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 1, SC::ext_on>::dist_in(S, shape, q, d);
+    case 2:
+      if constexpr (SC::max_depth >= 3) return out ? Csg<T, 2, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 2, SC::ext_on>::dist_in(S, shape, q, d);
+      break;
   }
   return out ? Csg<T, SC::max_depth, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, SC::max_depth, SC::ext_on>::dist_in(S, shape, q, d);
 }
@@ -1522,6 +1604,9 @@ __device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t 
   switch (S.shapes[shape].depth) {
     case 0: return Csg<T, 0, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
     case 1: return Csg<T, 1, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+    case 2:
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+      break;
   }
   return Csg<T, SC::max_depth, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
 }

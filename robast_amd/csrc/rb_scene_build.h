@@ -190,6 +190,12 @@ int validate(std::string& err, const rb_scene_desc* d) {
           return fail_msg(err, -1, "scene: Winston cone parameter count");
         break;
       }
+      case RB_SHAPE_WINSTON2D: {
+        const double* p = d->params + s.param_offset;
+        if (s.n_params != 14 || int(p[2]) != 2 || !(p[13] > 0.0) || !(p[0] > p[1]) || !(p[1] > 0.0))
+          return fail_msg(err, -1, "scene: AGeoWinstonCone2D needs 14 parameters, r1 > r2 > 0 and dy > 0");
+        break;
+      }
       case RB_SHAPE_COMPOSITE:
         if (s.n_params != 27) return fail_msg(err, -1, "scene: composite needs 27 parameters");
         if (csg_depth(d, i, 0) > kMaxCsgDepth)
@@ -279,7 +285,8 @@ struct SceneLayout {
   int32_t max_depth = 0;                  // deepest boolean nesting of any shape
   bool has_lens = false;                  // a placed node is an ALens
   bool has_borders = false;               // border-surface conditions are defined
-  bool has_poly = false;                  // a TGeoPcon or a general TGeoPgon (the deep kernel form's code)
+  bool has_ext = false;                   // a TGeoPcon, a general TGeoPgon or an AGeoWinstonCone2D
+                                          // (code compiled into the deep kernel forms only)
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -317,7 +324,8 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
     shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset, csg_depth(d, i, 0), 0};
     if (shapes[i].type == RB_SHAPE_PGON && !pgon_is_simple(d->params + d->shapes[i].param_offset))
       shapes[i].type = kShapePgonGeneral;  // rmin > 0 or a phi range: the general polygon code
-    if (shapes[i].type == kShapePgonGeneral || shapes[i].type == RB_SHAPE_PCON) lay.has_poly = true;
+    if (shapes[i].type == kShapePgonGeneral || shapes[i].type == RB_SHAPE_PCON || shapes[i].type == RB_SHAPE_WINSTON2D)
+      lay.has_ext = true;
     lay.max_depth = std::max(lay.max_depth, shapes[i].depth);
   }
   blob.clear();

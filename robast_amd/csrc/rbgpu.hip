@@ -46,7 +46,7 @@ struct rb_scene {
   int max_depth = 0;  // deepest boolean nesting among the scene's shapes
   bool has_lens = false;  // a placed volume refracts (ALens)
   bool has_borders = false;  // border-surface conditions are defined
-  bool has_poly = false;     // a TGeoPcon or a general TGeoPgon: traced with the deep kernel form
+  bool has_ext = false;      // a TGeoPcon, general TGeoPgon or AGeoWinstonCone2D: traced with the deep kernel form
   bool has_top_box = false;
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
 };
@@ -931,9 +931,9 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   }
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
   // kernels: composites at most one level deep, no border conditions and no
-  // TGeoPcon / general TGeoPgon; deeper composites or those shapes; border
-  // conditions (the general form)
-  const int form = scene->has_borders ? 2 : (scene->max_depth <= 1 && !scene->has_poly) ? 0 : 1;
+  // TGeoPcon / general TGeoPgon / AGeoWinstonCone2D; deeper composites or
+  // those shapes; border conditions (the general form)
+  const int form = scene->has_borders ? 2 : (scene->max_depth <= 1 && !scene->has_ext) ? 0 : 1;
   int rc;
   if (mixed)
     rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
@@ -1071,7 +1071,7 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
   sc->max_depth = lay.max_depth;
   sc->has_lens = lay.has_lens;
   sc->has_borders = lay.has_borders;
-  sc->has_poly = lay.has_poly;
+  sc->has_ext = lay.has_ext;
   sc->has_top_box = lay.has_top_box;
   for (int k = 0; k < 3; k++) {
     sc->top_lo[k] = lay.top_lo[k];
@@ -1456,7 +1456,7 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
     if (rc) return rc;
   }
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-  const bool shallow = scene->max_depth <= 1 && !scene->has_poly;
+  const bool shallow = scene->max_depth <= 1 && !scene->has_ext;
   if (opts->precision == RB_PRECISION_MIXED && shallow)
     hipLaunchKernelGGL((k_adjoint<true, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene), io,
                        n, P, A, grad, perm);

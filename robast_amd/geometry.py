@@ -452,6 +452,41 @@ class AGeoWinstonConePoly(TGeoShape):
         return np.zeros(3), np.array([r, r, self.dz])
 
 
+class AGeoWinstonCone2D(TGeoShape):
+    """Two-dimensional Winston cone, a trough (src/AGeoWinstonCone2D.cxx:551-567):
+    r1, r2 are the half widths of the entrance and exit apertures in x, and
+    y the half length of the trough; theta = asin(r2/r1), half length
+    dz = (r1 + r2) / tan(theta) / 2, focal length f = r2 (1 + sin theta).
+    The wide end is at z = +dz."""
+    shape_type = A.RB_SHAPE_WINSTON2D
+
+    def __init__(self, *args):
+        name = ""
+        if args and isinstance(args[0], str):
+            name, args = args[0], args[1:]
+        super().__init__(name)
+        r1, r2, y = abs(float(args[0])), abs(float(args[1])), abs(float(args[2]))
+        self.r1, self.r2 = (r1, r2) if r1 > r2 else (r2, r1)
+        self.dy = y
+        self.theta = math.asin(self.r2 / self.r1)
+        self.dz = (self.r1 + self.r2) / math.tan(self.theta) / 2.0
+        self.f = self.r2 * (1 + math.sin(self.theta))
+
+    def GetDZ(self):
+        return self.dz
+
+    def GetDY(self):
+        return self.dy
+
+    def params(self):
+        # the polygonal cone's layout with two sides, at phi = 0 and pi, then dy
+        return [self.r1, self.r2, 2.0, self.theta, self.dz, self.f, math.cos(self.theta), math.sin(self.theta), 0.0,
+                1.0, 0.0, -1.0, 0.0, self.dy]
+
+    def local_bbox(self):
+        return np.zeros(3), np.array([self.r1, self.dy, self.dz])
+
+
 class TGeoBoolNode:
     def __init__(self, op, left, right, lmat, rmat):
         self.op, self.left, self.right = op, left, right

@@ -94,7 +94,7 @@ int emul_trace(const rb_scene_desc* d, const rb_trace_opts* o, rb_rays* r, int64
   io.path_node = r->path_node;
   rb_emul_fallbacks = 0;
   const bool mixed = o->precision == RB_PRECISION_MIXED;
-  const int form = lay.has_borders ? 2 : (lay.max_depth <= 1 && !lay.has_poly) ? 0 : 1;
+  const int form = lay.has_borders ? 2 : (lay.max_depth <= 1 && !lay.has_ext) ? 0 : 1;
   if (mixed)
     form == 0 ? run<true, 1, false>(base, P, io, n)
     : form == 1 ? run<true, kMaxCsgDepth, false>(base, P, io, n)

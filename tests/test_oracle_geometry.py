@@ -65,6 +65,7 @@ def _shapes():
     pgr.DefineSection(1, 0, 0, 9)
     pgr.DefineSection(2, 5, 0, 7)
     out["pgon_reflex"] = pgr
+    out["winston2d"] = G.AGeoWinstonCone2D("w2d", 7.5, 3.0, 9.0)
     G.TGeoBBox("cb", 6, 6, 6)
     G.TGeoTube("ct", 0, 3, 9)
     G.TGeoTranslation("ctr", 1.5, 0, 0).RegisterYourself()

@@ -93,6 +93,28 @@ def test_winston_contains_matches_cpc_polygon():
         np.testing.assert_array_equal(got[clear], want[clear])
 
 
+def test_winston2d_contains_matches_cpc_trough():
+    """AGeoWinstonCone2D (src/AGeoWinstonCone2D.cxx:175-190): |y| <= dy,
+    |z| <= dz and |x| within the exact CPC profile r(z)."""
+    G._reset_registry()
+    w = G.AGeoWinstonCone2D("w2", 2.5, 1.0, 3.0)
+    flat, sid = _shape_scene(w)
+    r_of, dz, _, _ = _cpc_profile(w, dps=20)
+    assert w.dz == pytest.approx(float(dz), rel=1e-14)
+    zs = np.linspace(-w.dz, w.dz, 161)
+    rz = np.array([float(r_of(z)) for z in zs])
+    rng = np.random.default_rng(21)
+    q = np.column_stack([rng.uniform(-3, 3, 40000), rng.uniform(-3.5, 3.5, 40000),
+                         rng.uniform(-1.1 * w.dz, 1.1 * w.dz, 40000)])
+    r_lin = np.interp(q[:, 2], zs, rz)
+    clear = (np.abs(np.abs(q[:, 0]) - r_lin) > 1e-3) & (np.abs(np.abs(q[:, 2]) - w.dz) > 1e-9) & \
+        (np.abs(np.abs(q[:, 1]) - 3.0) > 1e-9)
+    want = (np.abs(q[:, 2]) <= w.dz) & (np.abs(q[:, 0]) <= r_lin) & (np.abs(q[:, 1]) <= 3.0)
+    got = O.contains_batch(flat, sid, q)
+    assert clear.sum() > 39000 and want[clear].sum() > 5000
+    np.testing.assert_array_equal(got[clear], want[clear])
+
+
 def _exact_exit_distance(w, r_of, p, d, tmax):
     """Distance from an inside point to the first wall or end cap, in mpmath:
     a 2000-step scan of h(t) = max_k apothem_k(p + t d) - r(z) and of the

@@ -1,7 +1,7 @@
 """Seeded random scenes: every supported shape (box with an offset origin,
 tube with a bore, theta-limited sphere, spherical cap, paraboloid, Pgon and
 Pcon -- full, hollow, over a wedge of at most 180 deg and over a wider one --,
-aspheric disk, Winston cone, and union / subtraction / intersection
+aspheric disk, Winston cone (polygonal and the 2D trough), and union / subtraction / intersection
 composites up to two levels) under random rotations, as mirrors, lenses,
 obscurations, focal surfaces and plain components, hit by random rays with
 Fresnel reflection on. Objects occupy separate cells of a 3 x 3 x 3 grid
@@ -80,6 +80,8 @@ def _shape(rng, k, tag):
         a.SetPolynomials(2, [u(-1e-4, 1e-4), u(-1e-7, 1e-7)], 1, [u(-1e-4, 1e-4)])
         return kind, a
     if kind == "winston":
+        if rng.uniform() < 0.35:  # the trough, AGeoWinstonCone2D
+            return kind, G.AGeoWinstonCone2D(n, u(4, 7), u(1.5, 3), u(3, 8))
         return kind, G.AGeoWinstonConePoly(n, u(4, 7), u(1.5, 3), int(rng.integers(4, 9)))
     if kind == "box_minus_tube":
         G.TGeoBBox(n + "b", u(5, 9), u(5, 9), u(5, 9))
