@@ -47,7 +47,8 @@ enum {
 enum {
   RB_SHAPE_BOX = 1,        /* dx dy dz ox oy oz                     (TGeoBBox)        */
   RB_SHAPE_TUBE = 2,       /* rmin rmax dz                          (TGeoTube)        */
-  RB_SHAPE_SPHERE = 3,     /* rmin rmax theta1 theta2 [deg] cos1 cos2 sin1 sin2 (TGeoSphere, full phi) */
+  RB_SHAPE_SPHERE = 3,     /* rmin rmax theta1 theta2 [deg] cos1 cos2 sin1 sin2 phi1 dphi [deg]
+                              {cos sin}(phi1) {cos sin}(phi1 + dphi)  (TGeoSphere; dphi >= 360: full) */
   RB_SHAPE_PARABOLOID = 4, /* rlo rhi dz a b                        (TGeoParaboloid)  */
   RB_SHAPE_PGON = 5,       /* phi1 dphi nedges nz {z rmin rmax}*nz {cos sin}*nedges
                               {cos sin}(phi1) {cos sin}(phi1 + dphi)   (TGeoPgon; side k's outward

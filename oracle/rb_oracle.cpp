@@ -247,7 +247,8 @@ static V3 tube_normal(const double* p, V3 q, V3 d) {
 }
 
 // ---------------------------------------------------------------- sphere
-// rmin rmax theta1 theta2 (deg), full phi. Constraint functions (<= 0 inside):
+// rmin rmax theta1 theta2 (deg), full phi (a phi range: sphw_* below).
+// Constraint functions (<= 0 inside):
 //   g_out = r^2 - rmax^2 ; g_in = rmin^2 - r^2
 //   g_t1 = z - c1 r   (theta >= theta1)  when theta1 > 0
 //   g_t2 = c2 r - z   (theta <= theta2)  when theta2 < 180
@@ -1137,6 +1138,104 @@ static V3 wins_normal(const double* p, V3 q, V3 d) {
   return n;
 }
 
+// ---------------------------------------------------------------- sphere with a phi range
+// TGeoSphere(rmin, rmax, theta1, theta2, phi1, phi2) with dphi < 360: the
+// shell segment restated from its definition (r in [rmin, rmax], theta by
+// acos(z / r) in [theta1, theta2], phi by atan2 in [phi1, phi1 + dphi]);
+// distances by brute force over every crossing of the ray with the two
+// spheres, the two theta cones (either nappe) and the two phi planes,
+// taking the first at which containment changes, tested 1e-9 cm either side.
+// params: as the sphere, then phi1 dphi {cos sin}(phi1) {cos sin}(phi1 + dphi)
+static bool sphw_wedge_full(const double* p) { return p[9] >= 360.0; }
+static bool sphw_contains(const double* p, V3 q) {
+  const double r = std::sqrt(dot(q, q));
+  if (r < p[0] || r > p[1]) return false;
+  if (r > 0) {
+    const double th = std::acos(std::max(-1.0, std::min(1.0, q.z / r))) * 180 / kPi;
+    if (th < p[2] || th > p[3]) return false;
+  }
+  double phi = std::atan2(q.y, q.x) * 180 / kPi;
+  while (phi < p[8]) phi += 360.0;
+  while (phi >= p[8] + 360.0) phi -= 360.0;
+  return phi - p[8] <= p[9];
+}
+static double sphw_dist(const double* p, V3 q, V3 d, bool leaving) {
+  std::vector<double> ts;
+  const double b = dot(q, d), qq = dot(q, q);
+  for (double R : {p[0], p[1]}) {
+    if (!(R > 0)) continue;
+    const double disc = b * b - (qq - R * R);
+    if (disc >= 0) {
+      ts.push_back(-b - std::sqrt(disc));
+      ts.push_back(-b + std::sqrt(disc));
+    }
+  }
+  for (int k = 0; k < 2; k++) {
+    const double thd = p[2 + k];
+    if ((k == 0 && thd <= 0) || (k == 1 && thd >= 180)) continue;
+    const double c = std::cos(thd * kPi / 180), c2 = c * c;
+    // z^2 = c^2 r^2 along the ray, both nappes
+    const double A = d.z * d.z - c2 * dot(d, d), B = 2 * (q.z * d.z - c2 * dot(q, d)), C = q.z * q.z - c2 * qq;
+    if (std::fabs(A) < 1e-14) {
+      if (B != 0) ts.push_back(-C / B);
+    } else {
+      const double disc = B * B - 4 * A * C;
+      if (disc >= 0) {
+        ts.push_back((-B - std::sqrt(disc)) / (2 * A));
+        ts.push_back((-B + std::sqrt(disc)) / (2 * A));
+      }
+    }
+  }
+  for (double phd : {p[8], p[8] + p[9]}) {
+    const double ph = phd * kPi / 180, nx = -std::sin(ph), ny = std::cos(ph);
+    const double fd = nx * d.x + ny * d.y;
+    if (fd != 0) ts.push_back(-(nx * q.x + ny * q.y) / fd);
+  }
+  std::sort(ts.begin(), ts.end());
+  const double e = 1e-9;
+  for (double t : ts) {
+    if (t < -e) continue;
+    const bool before = sphw_contains(p, axpy(q, t - e, d)), after = sphw_contains(p, axpy(q, t + e, d));
+    if (leaving ? (before && !after) : (!before && after)) return std::max(t, 0.0);
+  }
+  return leaving ? 0.0 : kBig;
+}
+// the nearest of the spheres, theta cones and phi half-planes
+static V3 sphw_normal(const double* p, V3 q, V3 d) {
+  const double r = std::sqrt(dot(q, q)), rho = std::sqrt(q.x * q.x + q.y * q.y);
+  double best = kBig;
+  V3 n = v3(0, 0, 1);
+  for (double R : {p[0], p[1]}) {
+    if (R > 0 && std::fabs(r - R) < best) {
+      best = std::fabs(r - R);
+      n = r > 0 ? mul(q, 1 / r) : v3(0, 0, 1);
+    }
+  }
+  const double th = r > 0 ? std::acos(std::max(-1.0, std::min(1.0, q.z / r))) : 0;
+  for (int k = 0; k < 2; k++) {
+    const double thd = p[2 + k];
+    if ((k == 0 && thd <= 0) || (k == 1 && thd >= 180)) continue;
+    const double tk = thd * kPi / 180;
+    const double dist = r * std::fabs(std::sin(th - tk));
+    if (dist < best) {
+      best = dist;
+      const double cp = rho > 0 ? q.x / rho : 1, sp = rho > 0 ? q.y / rho : 0;
+      n = v3(std::cos(tk) * cp, std::cos(tk) * sp, -std::sin(tk));  // the cone's normal, up to sign
+    }
+  }
+  for (double phd : {p[8], p[8] + p[9]}) {
+    const double ph = phd * kPi / 180;
+    if (std::cos(ph) * q.x + std::sin(ph) * q.y < 0) continue;
+    const double dist = std::fabs(-std::sin(ph) * q.x + std::cos(ph) * q.y);
+    if (dist < best) {
+      best = dist;
+      n = v3(-std::sin(ph), std::cos(ph), 0);
+    }
+  }
+  if (dot(n, d) < 0) n = mul(n, -1);
+  return n;
+}
+
 // ---------------------------------------------------------------- Winston cone (2D trough)
 // AGeoWinstonCone2D (src/AGeoWinstonCone2D.cxx): Contains (:175-190),
 // DistFromInside (:203-238), DistFromOutside (:241-322), ComputeNormal
@@ -1242,7 +1341,7 @@ static bool contains(const Scene& sc, int shape, V3 q) {
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return box_contains(p, q);
     case RB_SHAPE_TUBE: return tube_contains(p, q);
-    case RB_SHAPE_SPHERE: return sphere_contains(p, q);
+    case RB_SHAPE_SPHERE: return sphw_wedge_full(p) ? sphere_contains(p, q) : sphw_contains(p, q);
     case RB_SHAPE_PARABOLOID: return para_contains(p, q);
     case RB_SHAPE_PGON: return pgon_simple(p) ? pgon_contains(p, q) : gpoly_contains(gpoly_p(p, true), q);
     case RB_SHAPE_PCON: return gpoly_contains(gpoly_p(p, false), q);
@@ -1363,7 +1462,7 @@ static Hit dist_out(const Scene& sc, int shape, V3 q, V3 d) {
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return {box_dist_out(p, q, d), 0};
     case RB_SHAPE_TUBE: return {tube_dist_out(p, q, d), 0};
-    case RB_SHAPE_SPHERE: return {sphere_dist_out(p, q, d), 0};
+    case RB_SHAPE_SPHERE: return {sphw_wedge_full(p) ? sphere_dist_out(p, q, d) : sphw_dist(p, q, d, false), 0};
     case RB_SHAPE_PARABOLOID: return {para_dist_out(p, q, d), 0};
     case RB_SHAPE_PGON:
       return {pgon_simple(p) ? pgon_dist_out(p, q, d) : gpoly_dist(gpoly_p(p, true), q, d, false), 0};
@@ -1382,7 +1481,7 @@ static Hit dist_in(const Scene& sc, int shape, V3 q, V3 d) {
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: return {box_dist_in(p, q, d), 0};
     case RB_SHAPE_TUBE: return {tube_dist_in(p, q, d), 0};
-    case RB_SHAPE_SPHERE: return {sphere_dist_in(p, q, d), 0};
+    case RB_SHAPE_SPHERE: return {sphw_wedge_full(p) ? sphere_dist_in(p, q, d) : sphw_dist(p, q, d, true), 0};
     case RB_SHAPE_PARABOLOID: return {para_dist_in(p, q, d), 0};
     case RB_SHAPE_PGON:
       return {pgon_simple(p) ? pgon_dist_in(p, q, d) : gpoly_dist(gpoly_p(p, true), q, d, true), 0};
@@ -1417,7 +1516,7 @@ static V3 shape_normal(const Scene& sc, int shape, uint32_t leaf, V3 q, V3 d) {
   switch (sc.type(shape)) {
     case RB_SHAPE_BOX: n = box_normal(p, q, d); break;
     case RB_SHAPE_TUBE: n = tube_normal(p, q, d); break;
-    case RB_SHAPE_SPHERE: n = sphere_normal(p, q, d); break;
+    case RB_SHAPE_SPHERE: n = sphw_wedge_full(p) ? sphere_normal(p, q, d) : sphw_normal(p, q, d); break;
     case RB_SHAPE_PARABOLOID: n = para_normal(p, q, d); break;
     case RB_SHAPE_PGON: n = pgon_simple(p) ? pgon_normal(p, q, d) : gpoly_normal(gpoly_p(p, true), q, d); break;
     case RB_SHAPE_PCON: n = gpoly_normal(gpoly_p(p, false), q, d); break;

@@ -252,13 +252,21 @@ template <typename T> __device__ V3<T> tube_normal(const double* p, V3<T> q, V3<
 }
 
 // ------------------------------------------------------------ sphere
-// params: rmin rmax theta1 theta2 (deg); constraint g <= 0 inside:
+// params: rmin rmax theta1 theta2 (deg) cos1 cos2 sin1 sin2 phi1 dphi (deg)
+// {cos sin}(phi1) {cos sin}(phi1 + dphi); constraint g <= 0 inside:
 //   0: r - rmax   1: rmin - r   2: z - c1 r (theta >= theta1)   3: c2 r - z (theta <= theta2)
+//   4, 5: -n.x for the inward normals n of the phi half-planes (W: a phi range)
+// The phi range is a wedge: both of 4 and 5 hold for dphi <= 180 deg, either
+// of them for a wider one. A sphere with a phi range has DShape.type
+// kShapeSphereWedge (set at upload) and is compiled into the EXT forms only.
+constexpr int kShapeSphereWedge = 65;
 template <typename T> struct SphP {
   T rmin, rmax, c1, c2;
   bool t1, t2;
+  T n1x, n1y, n2x, n2y;  // W: inward normals of the phi half-planes
+  bool reflex;           // W: dphi > 180 deg (the union of the two half-spaces)
 };
-template <typename T> __device__ SphP<T> sph_p(const double* p) {
+template <typename T, bool W = false> __device__ SphP<T> sph_p(const double* p) {
   SphP<T> s;
   s.rmin = T(p[0]);
   s.rmax = T(p[1]);
@@ -266,12 +274,24 @@ template <typename T> __device__ SphP<T> sph_p(const double* p) {
   s.t2 = p[3] < 180.0;
   s.c1 = T(p[4]);
   s.c2 = T(p[5]);
+  if (W) {
+    s.n1x = T(-p[11]);
+    s.n1y = T(p[10]);
+    s.n2x = T(p[13]);
+    s.n2y = T(-p[12]);
+    s.reflex = p[9] > 180.0;
+  }
   return s;
 }
-template <typename T> __device__ bool sphere_contains(const double* p, V3<T> q) {
-  SphP<T> s = sph_p<T>(p);
+template <typename T> __device__ __forceinline__ bool sph_in_wedge(const SphP<T>& s, V3<T> q, T lim) {
+  const bool h1 = s.n1x * q.x + s.n1y * q.y >= -lim, h2 = s.n2x * q.x + s.n2y * q.y >= -lim;
+  return s.reflex ? (h1 || h2) : (h1 && h2);
+}
+template <typename T, bool W = false> __device__ bool sphere_contains(const double* p, V3<T> q) {
+  SphP<T> s = sph_p<T, W>(p);
   T r2 = dot(q, q);
   if (r2 < s.rmin * s.rmin || r2 > s.rmax * s.rmax) return false;
+  if (W && !sph_in_wedge(s, q, T(0))) return false;
   if (r2 < T(1e-20)) return true;
   T r = sqrt(r2);
   if (s.t1 && q.z > s.c1 * r) return false;
@@ -280,6 +300,12 @@ template <typename T> __device__ bool sphere_contains(const double* p, V3<T> q) 
 }
 // evaluate constraint k and its rate along d at x
 template <typename T> __device__ __forceinline__ void sph_g(const SphP<T>& s, V3<T> x, V3<T> d, int k, T& g, T& dg) {
+  if (k >= 4) {
+    const T nx = k == 4 ? s.n1x : s.n2x, ny = k == 4 ? s.n1y : s.n2y;
+    g = -(nx * x.x + ny * x.y);
+    dg = -(nx * d.x + ny * d.y);
+    return;
+  }
   T r = sqrt(dot(x, x));
   T rd = r > T(0) ? dot(x, d) / r : T(0);
   if (k == 0) { g = r - s.rmax; dg = rd; }
@@ -289,15 +315,17 @@ template <typename T> __device__ __forceinline__ void sph_g(const SphP<T>& s, V3
 }
 // A candidate root t on surface w: accept for dist_out if entering through w
 // and every other active constraint holds; for dist_in if leaving through w.
-template <typename T, bool OUT>
+// With a reflex wedge, crossing one phi plane enters or leaves the solid only
+// while the point is outside the other half-space.
+template <typename T, bool OUT, bool W = false>
 __device__ __forceinline__ void sph_try(const SphP<T>& s, V3<T> q, V3<T> d, T t, int w, T& best) {
   if (!(t >= T(0)) || t >= best) return;
   V3<T> x = axpy(q, t, d);
   T g, dg;
   sph_g(s, x, d, w, g, dg);
+  T lim = T(1e-9) * (T(1) + s.rmax);
   if (OUT) {
     if (dg >= T(0)) return;
-    T lim = T(1e-9) * (T(1) + s.rmax);
     for (int k = 0; k < 4; k++) {
       if (k == w) continue;
       if (k == 1 && !(s.rmin > T(0))) continue;
@@ -307,15 +335,29 @@ __device__ __forceinline__ void sph_try(const SphP<T>& s, V3<T> q, V3<T> d, T t,
       sph_g(s, x, d, k, gk, dgk);
       if (gk > lim) return;
     }
+    if (W) {
+      if (w >= 4) {
+        T go, dgo;
+        sph_g(s, x, d, 9 - w, go, dgo);
+        if (s.reflex ? !(go > T(0)) : go > lim) return;
+      } else if (!sph_in_wedge(s, x, lim)) {
+        return;
+      }
+    }
   } else {
     if (dg <= T(0)) return;
+    if (W && w >= 4 && s.reflex) {
+      T go, dgo;
+      sph_g(s, x, d, 9 - w, go, dgo);
+      if (!(go > T(0))) return;
+    }
   }
   best = t;
 }
-template <typename T, bool OUT>
+template <typename T, bool OUT, bool W = false>
 __device__ void sph_cone(const SphP<T>& s, V3<T> q, V3<T> d, T c, int w, T& best) {
   if (fabs(c) < T(1e-15)) {
-    if (d.z != T(0)) sph_try<T, OUT>(s, q, d, -q.z / d.z, w, best);
+    if (d.z != T(0)) sph_try<T, OUT, W>(s, q, d, -q.z / d.z, w, best);
     return;
   }
   T c2 = c * c;
@@ -339,34 +381,42 @@ __device__ void sph_cone(const SphP<T>& s, V3<T> q, V3<T> d, T c, int w, T& best
   for (int i = 0; i < m; i++) {
     T r = i == 0 ? r0 : r1;
     T z = q.z + r * d.z;
-    if ((c > T(0) && z >= T(0)) || (c < T(0) && z <= T(0))) sph_try<T, OUT>(s, q, d, r, w, best);
+    if ((c > T(0) && z >= T(0)) || (c < T(0) && z <= T(0))) sph_try<T, OUT, W>(s, q, d, r, w, best);
   }
 }
-template <typename T, bool OUT> __device__ T sphere_dist(const double* p, V3<T> q, V3<T> d) {
-  SphP<T> s = sph_p<T>(p);
+template <typename T, bool OUT, bool W = false> __device__ T sphere_dist(const double* p, V3<T> q, V3<T> d) {
+  SphP<T> s = sph_p<T, W>(p);
   T best = T(kBig);
   T b = dot(q, d), qq = dot(q, q);
   T disc = b * b - (qq - s.rmax * s.rmax);
   if (disc > T(0)) {
     T sq = sqrt(disc);
-    sph_try<T, OUT>(s, q, d, -b - sq, 0, best);
-    sph_try<T, OUT>(s, q, d, -b + sq, 0, best);
+    sph_try<T, OUT, W>(s, q, d, -b - sq, 0, best);
+    sph_try<T, OUT, W>(s, q, d, -b + sq, 0, best);
   }
   if (s.rmin > T(0)) {
     disc = b * b - (qq - s.rmin * s.rmin);
     if (disc > T(0)) {
       T sq = sqrt(disc);
-      sph_try<T, OUT>(s, q, d, -b - sq, 1, best);
-      sph_try<T, OUT>(s, q, d, -b + sq, 1, best);
+      sph_try<T, OUT, W>(s, q, d, -b - sq, 1, best);
+      sph_try<T, OUT, W>(s, q, d, -b + sq, 1, best);
     }
   }
-  if (s.t1) sph_cone<T, OUT>(s, q, d, s.c1, 2, best);
-  if (s.t2) sph_cone<T, OUT>(s, q, d, s.c2, 3, best);
+  if (s.t1) sph_cone<T, OUT, W>(s, q, d, s.c1, 2, best);
+  if (s.t2) sph_cone<T, OUT, W>(s, q, d, s.c2, 3, best);
+  if (W) {
+    // the phi planes through the axis
+    for (int k = 4; k < 6; k++) {
+      const T nx = k == 4 ? s.n1x : s.n2x, ny = k == 4 ? s.n1y : s.n2y;
+      const T fd = nx * d.x + ny * d.y;
+      if (fd != T(0)) sph_try<T, OUT, W>(s, q, d, -(nx * q.x + ny * q.y) / fd, k, best);
+    }
+  }
   if (!OUT && best >= T(kBig)) best = T(0);
   return best;
 }
-template <typename T> __device__ V3<T> sphere_normal(const double* p, V3<T> q, V3<T> d) {
-  SphP<T> s = sph_p<T>(p);
+template <typename T, bool W = false> __device__ V3<T> sphere_normal(const double* p, V3<T> q, V3<T> d) {
+  SphP<T> s = sph_p<T, W>(p);
   T r = sqrt(dot(q, q));
   T rho = sqrt(q.x * q.x + q.y * q.y);
   T st1 = T(p[6]), st2 = T(p[7]);
@@ -382,6 +432,20 @@ template <typename T> __device__ V3<T> sphere_normal(const double* p, V3<T> q, V
     T c = k == 2 ? s.c1 : s.c2, st = k == 2 ? st1 : st2;
     T cp = rho > T(0) ? q.x / rho : T(1), sp = rho > T(0) ? q.y / rho : T(0);
     n = mk<T>(-c * cp, -c * sp, st);
+  }
+  if (W) {
+    // a phi half-plane, where the point projects onto it
+    T best = dist[k];
+    for (int j = 0; j < 2; j++) {
+      const T ux = T(p[10 + 2 * j]), uy = T(p[11 + 2 * j]);
+      if (ux * q.x + uy * q.y < T(0)) continue;
+      const T nx = j == 0 ? s.n1x : s.n2x, ny = j == 0 ? s.n1y : s.n2y;
+      const T dj = fabs(nx * q.x + ny * q.y);
+      if (dj < best) {
+        best = dj;
+        n = mk<T>(nx, ny, T(0));
+      }
+    }
   }
   if (dot(n, d) < T(0)) n = scale(n, T(-1));
   return n;
@@ -1348,6 +1412,7 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN bool prim_contains(int typ
     case kShapePgonGeneral: if constexpr (EXT) return poly_contains<T, true>(p, q); break;
     case RB_SHAPE_PCON: if constexpr (EXT) return poly_contains<T, false>(p, q); break;
     case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_contains<T>(p, q); break;
+    case kShapeSphereWedge: if constexpr (EXT) return sphere_contains<T, true>(p, q); break;
     case RB_SHAPE_ASPHERIC: return asph_contains<T>(p, q);
     case RB_SHAPE_WINSTON: return wins_contains<T>(p, q);
   }
@@ -1363,6 +1428,7 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN T prim_dist_out(int type, 
     case kShapePgonGeneral: if constexpr (EXT) return poly_dist_out<T, true>(p, q, d); break;
     case RB_SHAPE_PCON: if constexpr (EXT) return poly_dist_out<T, false>(p, q, d); break;
     case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_dist_out<T>(p, q, d); break;
+    case kShapeSphereWedge: if constexpr (EXT) return sphere_dist<T, true, true>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_dist_out<T>(p, q, d);
   }
@@ -1378,6 +1444,7 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN T prim_dist_in(int type, c
     case kShapePgonGeneral: if constexpr (EXT) return poly_dist_in<T, true>(p, q, d); break;
     case RB_SHAPE_PCON: if constexpr (EXT) return poly_dist_in<T, false>(p, q, d); break;
     case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_dist_in<T>(p, q, d); break;
+    case kShapeSphereWedge: if constexpr (EXT) return sphere_dist<T, false, true>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_dist_in<T>(p, q, d);
   }
@@ -1393,6 +1460,7 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN V3<T> prim_normal(int type
     case kShapePgonGeneral: if constexpr (EXT) return poly_normal<T, true>(p, q, d); break;
     case RB_SHAPE_PCON: if constexpr (EXT) return poly_normal<T, false>(p, q, d); break;
     case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_normal<T>(p, q, d); break;
+    case kShapeSphereWedge: if constexpr (EXT) return sphere_normal<T, true>(p, q, d); break;
     case RB_SHAPE_ASPHERIC: return asph_normal<T>(p, q, d);
     case RB_SHAPE_WINSTON: return wins_normal<T>(p, q, d);
   }

@@ -155,9 +155,12 @@ int validate(std::string& err, const rb_scene_desc* d) {
       case RB_SHAPE_TUBE:
         if (s.n_params != 3) return fail_msg(err, -1, "scene: tube needs 3 parameters");
         break;
-      case RB_SHAPE_SPHERE:
-        if (s.n_params != 8) return fail_msg(err, -1, "scene: sphere needs 8 parameters");
+      case RB_SHAPE_SPHERE: {
+        const double* p = d->params + s.param_offset;
+        if (s.n_params != 14) return fail_msg(err, -1, "scene: sphere needs 14 parameters");
+        if (!(p[9] > 0.0)) return fail_msg(err, -1, "scene: sphere needs dphi > 0");
         break;
+      }
       case RB_SHAPE_PARABOLOID:
         if (s.n_params != 5) return fail_msg(err, -1, "scene: paraboloid needs 5 parameters");
         break;
@@ -285,8 +288,9 @@ struct SceneLayout {
   int32_t max_depth = 0;                  // deepest boolean nesting of any shape
   bool has_lens = false;                  // a placed node is an ALens
   bool has_borders = false;               // border-surface conditions are defined
-  bool has_ext = false;                   // a TGeoPcon, a general TGeoPgon or an AGeoWinstonCone2D
-                                          // (code compiled into the deep kernel forms only)
+  bool has_ext = false;                   // a TGeoPcon, a general TGeoPgon, an AGeoWinstonCone2D or a
+                                          // TGeoSphere with a phi range (code compiled into the deep
+                                          // kernel forms only)
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -324,7 +328,10 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
     shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset, csg_depth(d, i, 0), 0};
     if (shapes[i].type == RB_SHAPE_PGON && !pgon_is_simple(d->params + d->shapes[i].param_offset))
       shapes[i].type = kShapePgonGeneral;  // rmin > 0 or a phi range: the general polygon code
-    if (shapes[i].type == kShapePgonGeneral || shapes[i].type == RB_SHAPE_PCON || shapes[i].type == RB_SHAPE_WINSTON2D)
+    if (shapes[i].type == RB_SHAPE_SPHERE && d->params[d->shapes[i].param_offset + 9] < 360.0)
+      shapes[i].type = kShapeSphereWedge;  // a phi range: the wedge-constrained sphere code
+    if (shapes[i].type == kShapePgonGeneral || shapes[i].type == RB_SHAPE_PCON ||
+        shapes[i].type == RB_SHAPE_WINSTON2D || shapes[i].type == kShapeSphereWedge)
       lay.has_ext = true;
     lay.max_depth = std::max(lay.max_depth, shapes[i].depth);
   }

@@ -226,13 +226,17 @@ class TGeoSphere(TGeoShape):
         super().__init__(name)
         vals = [float(v) for v in args] + [0.0, 180.0, 0.0, 360.0][len(args) - 2:]
         self.rmin, self.rmax, self.theta1, self.theta2, self.phi1, self.phi2 = vals[:6]
-        if not (self.phi1 == 0.0 and self.phi2 == 360.0):
-            raise NotImplementedError("TGeoSphere with a phi range is not supported")
+        # the phi range [phi1, phi2], going round once when phi2 <= phi1
+        self.dphi = self.phi2 - self.phi1
+        if self.dphi <= 0:
+            self.dphi += 360.0
 
     def params(self):
         t1, t2 = math.radians(self.theta1), math.radians(self.theta2)
+        f1, f2 = math.radians(self.phi1), math.radians(self.phi1 + self.dphi)
         return [self.rmin, self.rmax, self.theta1, self.theta2,
-                math.cos(t1), math.cos(t2), math.sin(t1), math.sin(t2)]
+                math.cos(t1), math.cos(t2), math.sin(t1), math.sin(t2),
+                self.phi1, self.dphi, math.cos(f1), math.sin(f1), math.cos(f2), math.sin(f2)]
 
     def local_bbox(self):
         r = self.rmax

@@ -30,6 +30,8 @@ def _shapes():
     out["tube"] = G.TGeoTube("t", 3, 7, 5)
     out["sphere"] = G.TGeoSphere("s", 4, 9, 30, 120)
     out["sphere_cap"] = G.TGeoSphere("s2", 6, 9, 140, 180)
+    out["sphere_wedge"] = G.TGeoSphere("sw", 4, 9, 30, 120, 20, 140)
+    out["sphere_reflex"] = G.TGeoSphere("sr", 0, 8, 0, 150, -60, 200)
     out["paraboloid"] = G.TGeoParaboloid("p", 1, 6, 4)
     pg = G.TGeoPgon("pg", 0, 360, 6, 3)
     pg.DefineSection(0, -5, 0, 5)
@@ -138,6 +140,11 @@ def _contains_closed_form(name, q):
     if name == "sphere":
         th = np.degrees(np.arccos(np.clip(z / np.maximum(R, 1e-300), -1, 1)))
         return (R >= 4) & (R <= 9) & (th >= 30) & (th <= 120)
+    if name in ("sphere_wedge", "sphere_reflex"):
+        rmin, rmax, t1, t2, f1, f2 = (4, 9, 30, 120, 20, 140) if name == "sphere_wedge" else (0, 8, 0, 150, -60, 200)
+        th = np.degrees(np.arccos(np.clip(z / np.maximum(R, 1e-300), -1, 1)))
+        rel = (np.degrees(np.arctan2(y, x)) - f1) % 360.0
+        return (R >= rmin) & (R <= rmax) & (th >= t1) & (th <= t2) & (rel <= f2 - f1)
     if name == "paraboloid":
         a = 2 * 4 / (36 - 1)
         b = -4 * (1 + 36) / (36 - 1)
@@ -181,7 +188,8 @@ _POLY = {
 }
 
 
-@pytest.mark.parametrize("name", ["box", "tube", "sphere", "paraboloid", "pgon"] + sorted(_POLY))
+@pytest.mark.parametrize("name", ["box", "tube", "sphere", "sphere_wedge", "sphere_reflex", "paraboloid", "pgon"] +
+                         sorted(_POLY))
 def test_contains_matches_closed_form(name):
     flat, sid = _shape_scene(SHAPES[name])
     q = np.random.default_rng(5).uniform(-12, 12, (20000, 3))

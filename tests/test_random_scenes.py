@@ -1,5 +1,6 @@
 """Seeded random scenes: every supported shape (box with an offset origin,
-tube with a bore, theta-limited sphere, spherical cap, paraboloid, Pgon and
+tube with a bore, theta-limited sphere with and without a phi range,
+spherical cap, paraboloid, Pgon and
 Pcon -- full, hollow, over a wedge of at most 180 deg and over a wider one --,
 aspheric disk, Winston cone (polygonal and the 2D trough), and union / subtraction / intersection
 composites up to two levels) under random rotations, as mirrors, lenses,
@@ -53,6 +54,9 @@ def _shape(rng, k, tag):
     if kind == "tube":
         return kind, G.TGeoTube(n, u(1, 4), u(6, 11), u(3, 9))
     if kind == "sphere":
+        if rng.uniform() < 0.4:  # a phi range, narrower or wider than 180 deg
+            f1 = u(-180, 180)
+            return kind, G.TGeoSphere(n, u(3, 6), u(8, 12), u(10, 60), u(100, 170), f1, f1 + u(60, 320))
         return kind, G.TGeoSphere(n, u(3, 6), u(8, 12), u(10, 60), u(100, 170))
     if kind == "cap":
         return kind, G.TGeoSphere(n, u(8, 10), u(11, 13), u(120, 150), 180)
