@@ -232,7 +232,7 @@ enum { RB_ORDER_AUTO = 0, RB_ORDER_INPUT = 1, RB_ORDER_COHERENT = 2 };
  * from a work counter as soon as theirs terminates, so a wavefront does not
  * wait for its longest path. AUTO picks PERSISTENT for scenes with refracting
  * volumes (multi-bounce paths of very unequal length) and LANE otherwise. */
-enum { RB_SCHEDULE_AUTO = 0, RB_SCHEDULE_LANE = 1, RB_SCHEDULE_PERSISTENT = 2 };
+enum { RB_SCHEDULE_AUTO = 0, RB_SCHEDULE_LANE = 1, RB_SCHEDULE_PERSISTENT = 2, RB_SCHEDULE_REGROUP = 3 };
 
 /* ---- synthetic ray sources (device-side ARayShooter equivalents) ---- */
 enum {

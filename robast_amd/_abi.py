@@ -9,7 +9,7 @@ import numpy as np
 
 # ray status (include/ARay.h:26)
 RB_RUN, RB_STOP, RB_EXIT, RB_FOCUS, RB_SUSPEND, RB_ABSORB = range(6)
-RB_SCHEDULE_AUTO, RB_SCHEDULE_LANE, RB_SCHEDULE_PERSISTENT = range(3)
+RB_SCHEDULE_AUTO, RB_SCHEDULE_LANE, RB_SCHEDULE_PERSISTENT, RB_SCHEDULE_REGROUP = range(4)
 STATUS_NAMES = ("running", "stopped", "exited", "focused", "suspended", "absorbed")
 
 # volume kinds (include/AOpticsManager.h:45-53)

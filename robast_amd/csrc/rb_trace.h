@@ -271,9 +271,14 @@ __device__ __forceinline__ void rank_fp64(int c, const Hit& h, double& best, int
 // the best crossing (a shape is never met before its box). If a box that
 // did not fit could still hold a nearer crossing, the one-pass walk decides.
 // The ranking rule makes the result independent of the test order.
+//
+// `hint` (the daughter the ray last interacted with, or -1) is ranked before
+// the walk: a ray that keeps meeting one daughter (a light guide's wall) then
+// starts the walk with that crossing as its bound, which prunes the boxes
+// beyond it.
 template <typename SC>
 __device__ void search_boundary_fp64(const SC& S, int cur, V3<double> p, V3<double> d, double& best,
-                                     int& crossed, Hit& hit, Cand cand) {
+                                     int& crossed, Hit& hit, Cand cand, int hint) {
   const DNode& cn = S.nodes[cur];
   Hit hx = node_dist<double>(S, cur, p, d, 0.0, false);
   best = kBig;
@@ -284,6 +289,9 @@ __device__ void search_boundary_fp64(const SC& S, int cur, V3<double> p, V3<doub
     hit = hx;
   }
   if (cn.n_children == 0) return;
+  // (a ray's last node arrives from the caller: bounded before it is read)
+  const bool hinted = hint > 0 && hint < S.n_nodes && S.nodes[hint].parent == cur;
+  if (hinted) rank_fp64(hint, node_dist<double>(S, hint, p, d, 0.0, true), best, crossed, hit);
   const float ox = float(p.x), oy = float(p.y), oz = float(p.z);
   const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
 #if RB_POSTPONE
@@ -292,6 +300,7 @@ __device__ void search_boundary_fp64(const SC& S, int cur, V3<double> p, V3<doub
   for (int k = 0; k < nc; k++) {
     if (cand.tn[k * kCandStride] > float(best) * 1.0001f + 1e-2f) break;
     const int c = cand.node[k * kCandStride];
+    if (hinted && c == hint) continue;
     rank_fp64(c, node_dist<double>(S, c, p, d, 0.0, true), best, crossed, hit);
   }
   if (!(cut <= float(best) * 1.0001f + 1e-2f)) return;
@@ -439,7 +448,10 @@ __device__ bool search_boundary_mixed(const SC& S, int cur, V3<double> p, V3<dou
 
 // TGeoNavigator::FindNextBoundaryAndStep; returns the next node (-1 = null).
 // MIXED: the search runs in FP32 and the winning crossing is polished in FP64.
-template <bool MIXED, typename SC>
+// HINT: the FP64 search ranks the daughter of the last interaction first
+// (the result does not change; the work-queue kernels, whose rays bounce
+// many times inside light guides, compile it in).
+template <bool MIXED, bool HINT = false, typename SC>
 __device__ int next_boundary(const SC& S, Lane& L) {
   double extra = L.on_b ? kTol : 0.0;
   L.on_b = false;
@@ -470,7 +482,7 @@ __device__ int next_boundary(const SC& S, Lane& L) {
 #ifdef RB_HOST_EMULATION
     if (MIXED) rb_emul_fallbacks++;
 #endif
-    search_boundary_fp64(S, cur, L.p, L.d, best, crossed, hit, L.cand);
+    search_boundary_fp64(S, cur, L.p, L.d, best, crossed, hit, L.cand, HINT ? L.last_node : -1);
   }
   if (crossed == -1 && best <= kTol) {
     // TGeoNavigator: do not stay on the exit surface; step by the tolerance
@@ -826,7 +838,7 @@ __device__ __forceinline__ void begin_lane(const SC& S, Lane& L) {
 
 // One segment of the per-ray state machine: next boundary, then the
 // interaction the (current, next) volume kinds select.
-template <bool MIXED, typename Rec, typename SC>
+template <bool MIXED, bool HINT = false, typename Rec, typename SC>
 __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
   const uint64_t id = L.ray;
   const double lambda = L.lambda;
@@ -835,7 +847,7 @@ __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, in
     const double t1 = L.t;
     const int current = L.cur;
     const int np0 = L.npoints;
-    const int next = next_boundary<MIXED>(S, L);
+    const int next = next_boundary<MIXED, HINT>(S, L);
     const double step = L.step;
     const int tc = kind_of(S, current), tn = kind_of(S, next);
 
@@ -901,10 +913,10 @@ __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, in
   }
 }
 
-template <bool MIXED, typename Rec, typename SC>
+template <bool MIXED, bool HINT = false, typename Rec, typename SC>
 __device__ void trace_lane(const SC& S, const TraceParams& P, const RayIO& io, int64_t i, Lane& L, Rec& rec) {
   begin_lane(S, L);
-  while (L.status == RB_RUN) step_lane<MIXED>(S, P, io, i, L, rec);
+  while (L.status == RB_RUN) step_lane<MIXED, HINT>(S, P, io, i, L, rec);
 }
 
 }  // namespace rbd

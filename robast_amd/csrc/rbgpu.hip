@@ -273,7 +273,116 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
       continue;
     }
     if (active) {
-      step_lane<MIXED>(S, P, io, i, L, rec);
+      step_lane<MIXED, true>(S, P, io, i, L, rec);
+      if (L.status != RB_RUN) {
+        store_lane(io, i, L);
+        active = false;
+      }
+    }
+  }
+}
+
+// Regrouping form: the persistent schedule, with the block's live rays
+// regrouped before every step so that each wavefront holds rays in the same
+// phase. Each lane's phase key is 0 (in a non-refracting volume: a navigation
+// search among daughters), 1 (inside a lens: its exit and the Fresnel
+// physics) or 2 (no ray). A counting sort over the block -- per-wave ballot
+// counts, then each lane's destination from the counts of the lower keys and
+// waves and its rank among its wave's lanes of its key -- moves every lane's
+// ray state through LDS to its new lane. Lanes then step their rays as in
+// k_trace_persist; free lanes claim the next rays (one atomic per wave).
+// A ray is stepped by whichever lane holds it, always with its own Philox
+// stream (keyed by its index) and draw counter, so results equal k_trace's.
+struct RegroupLDS {
+  double px[kTraceBlock], py[kTraceBlock], pz[kTraceBlock];
+  double dx[kTraceBlock], dy[kTraceBlock], dz[kTraceBlock];
+  double t[kTraceBlock], lambda[kTraceBlock], step[kTraceBlock];
+  uint64_t ray[kTraceBlock];
+  int32_t cur[kTraceBlock], status[kTraceBlock], npoints[kTraceBlock], last_node[kTraceBlock];
+  int32_t hit_node[kTraceBlock], flags[kTraceBlock];  // flags: bit 0 active, bit 1 on_b
+  uint32_t draw[kTraceBlock], hit_leaf[kTraceBlock];
+  int32_t count[kTraceBlock / 64][3];
+  int32_t exhausted;
+};
+
+template <bool MIXED, int MAXD, bool BORDERS>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS> S, RayIO io,
+                                                                          int64_t n, TraceParams P,
+                                                                          unsigned long long* next_ray,
+                                                                          const uint32_t* perm) {
+  __shared__ int32_t cand_node[kCand * kCandStride];
+  __shared__ float cand_tn[kCand * kCandStride];
+  __shared__ RegroupLDS G;
+  stage_bvh(S);
+  if (threadIdx.x == 0) G.exhausted = 0;
+  __syncthreads();
+  Lane L;
+  L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
+  NoRecord rec;
+  bool active = false;
+  const int wave = int(threadIdx.x) >> 6;
+  for (;;) {
+    // free lanes claim the next rays of the trace order, one atomic per wave
+    const bool need = !active && !G.exhausted;
+    const unsigned long long want = __ballot(need);
+    if (want != 0ull) {
+      const int leader = __ffsll(want) - 1;
+      const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(want >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(want), 0u));
+      unsigned long long base = 0;
+      if (need && rank == 0) base = atomicAdd(next_ray, (unsigned long long)__popcll(want));
+      const unsigned lo = __shfl(unsigned(base), leader), hi = __shfl(unsigned(base >> 32), leader);
+      if (need) {
+        const int64_t c = int64_t((static_cast<unsigned long long>(hi) << 32) | lo) + rank;
+        int64_t i;
+        if (c >= n) {
+          G.exhausted = 1;
+        } else if (i = perm ? int64_t(perm[c]) : c; io.status[i] == RB_RUN) {
+          load_lane(io, i, L);
+          L.ray = P.id_base + uint64_t(i);
+          begin_lane(S, L);
+          active = true;
+        }
+      }
+    }
+    // the block leaves when none of its lanes holds a ray and the order is used up
+    if (!__syncthreads_or(active ? 1 : 0) && G.exhausted) break;
+    // counting sort of the block's lanes by phase key
+    const int key = !active ? 2 : kind_of(S, L.cur) == RB_KLENS ? 1 : 0;
+    const unsigned long long m0 = __ballot(key == 0), m1 = __ballot(key == 1);
+    const unsigned long long mine = key == 0 ? m0 : key == 1 ? m1 : ~(m0 | m1);
+    if ((threadIdx.x & 63) == 0) {
+      G.count[wave][0] = __popcll(m0);
+      G.count[wave][1] = __popcll(m1);
+      G.count[wave][2] = 64 - __popcll(m0) - __popcll(m1);
+    }
+    __syncthreads();
+    int dest = int(__builtin_amdgcn_mbcnt_hi(unsigned(mine >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(mine), 0u)));
+    for (int w = 0; w < kTraceBlock / 64; w++) {
+      for (int k = 0; k < key; k++) dest += G.count[w][k];
+      if (w < wave) dest += G.count[w][key];
+    }
+    G.px[dest] = L.p.x; G.py[dest] = L.p.y; G.pz[dest] = L.p.z;
+    G.dx[dest] = L.d.x; G.dy[dest] = L.d.y; G.dz[dest] = L.d.z;
+    G.t[dest] = L.t; G.lambda[dest] = L.lambda; G.step[dest] = L.step;
+    G.ray[dest] = L.ray;
+    G.cur[dest] = L.cur; G.status[dest] = L.status; G.npoints[dest] = L.npoints; G.last_node[dest] = L.last_node;
+    G.hit_node[dest] = L.hit_node; G.flags[dest] = (active ? 1 : 0) | (L.on_b ? 2 : 0);
+    G.draw[dest] = L.draw; G.hit_leaf[dest] = L.hit_leaf;
+    __syncthreads();
+    const int me = int(threadIdx.x);
+    L.p = mk(G.px[me], G.py[me], G.pz[me]);
+    L.d = mk(G.dx[me], G.dy[me], G.dz[me]);
+    L.t = G.t[me]; L.lambda = G.lambda[me]; L.step = G.step[me];
+    L.ray = G.ray[me];
+    L.cur = G.cur[me]; L.status = G.status[me]; L.npoints = G.npoints[me]; L.last_node = G.last_node[me];
+    L.hit_node = G.hit_node[me];
+    active = (G.flags[me] & 1) != 0;
+    L.on_b = (G.flags[me] & 2) != 0;
+    L.draw = G.draw[me]; L.hit_leaf = G.hit_leaf[me];
+    // (the next iteration's first barrier keeps these reads ahead of the next writes)
+    if (active) {
+      const int64_t i = int64_t(L.ray - P.id_base);
+      step_lane<MIXED, true>(S, P, io, i, L, rec);
       if (L.status != RB_RUN) {
         store_lane(io, i, L);
         active = false;
@@ -731,21 +840,27 @@ __global__ void k_clear_counter(unsigned long long* c) {
 
 // The trace kernel for the scene's depth and the lane schedule.
 template <bool MIXED, int MAXD, bool BORDERS>
-static int launch_trace(rb_ctx* ctx, const rb_scene* scene, bool persistent, const RayIO& io, int64_t n,
+static int launch_trace(rb_ctx* ctx, const rb_scene* scene, int schedule, const RayIO& io, int64_t n,
                         const TraceParams& P, const uint32_t* perm, unsigned long long* counter, bool clear_by_kernel) {
   const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
-  if (persistent) {
+  if (schedule != RB_SCHEDULE_LANE) {
+    const bool regroup = schedule == RB_SCHEDULE_REGROUP;
+    const void* fn = regroup ? (const void*)k_trace_regroup<MIXED, MAXD, BORDERS>
+                             : (const void*)k_trace_persist<MIXED, MAXD, BORDERS>;
     int per_cu = 0;
-    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, (const void*)k_trace_persist<MIXED, MAXD, BORDERS>, block, 0));
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
     const int64_t pgrid = std::min<int64_t>(grid, int64_t(std::max(per_cu, 1)) * ctx->cus);
     if (clear_by_kernel)
       hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(64), 0, ctx->stream, counter);
     else
       HIPCHK(ctx, hipMemsetAsync(counter, 0, sizeof(unsigned long long), ctx->stream));
-    hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
-                       scene_of<MAXD, BORDERS>(scene), io, n, P, counter, perm);
+    if (regroup)
+      hipLaunchKernelGGL((k_trace_regroup<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
+                         scene_of<MAXD, BORDERS>(scene), io, n, P, counter, perm);
+    else
+      hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
+                         scene_of<MAXD, BORDERS>(scene), io, n, P, counter, perm);
   } else {
     hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<MAXD, BORDERS>(scene), io, n, P, perm);
@@ -913,8 +1028,9 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   const bool mixed = opts->precision == RB_PRECISION_MIXED;
   const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
                         (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
-  const bool persistent = opts->schedule == RB_SCHEDULE_PERSISTENT ||
-                          (opts->schedule == RB_SCHEDULE_AUTO && scene->has_lens);
+  const int schedule = opts->schedule != RB_SCHEDULE_AUTO ? opts->schedule
+                       : scene->has_lens ? RB_SCHEDULE_PERSISTENT : RB_SCHEDULE_LANE;
+  const bool persistent = schedule != RB_SCHEDULE_LANE;
   *coherent_out = coherent;
   const uint32_t* perm = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -936,13 +1052,13 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   const int form = scene->has_borders ? 2 : (scene->max_depth <= 1 && !scene->has_ext) ? 0 : 1;
   int rc;
   if (mixed)
-    rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
-       : form == 1 ? launch_trace<true, kMaxCsgDepth, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
-                   : launch_trace<true, kMaxCsgDepth, true>(ctx, scene, persistent, io, n, P, perm, counter, noalloc);
+    rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, schedule, io, n, P, perm, counter, noalloc)
+       : form == 1 ? launch_trace<true, kMaxCsgDepth, false>(ctx, scene, schedule, io, n, P, perm, counter, noalloc)
+                   : launch_trace<true, kMaxCsgDepth, true>(ctx, scene, schedule, io, n, P, perm, counter, noalloc);
   else
-    rc = form == 0 ? launch_trace<false, 1, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
-       : form == 1 ? launch_trace<false, kMaxCsgDepth, false>(ctx, scene, persistent, io, n, P, perm, counter, noalloc)
-                   : launch_trace<false, kMaxCsgDepth, true>(ctx, scene, persistent, io, n, P, perm, counter, noalloc);
+    rc = form == 0 ? launch_trace<false, 1, false>(ctx, scene, schedule, io, n, P, perm, counter, noalloc)
+       : form == 1 ? launch_trace<false, kMaxCsgDepth, false>(ctx, scene, schedule, io, n, P, perm, counter, noalloc)
+                   : launch_trace<false, kMaxCsgDepth, true>(ctx, scene, schedule, io, n, P, perm, counter, noalloc);
   if (rc) return rc;
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
@@ -960,7 +1076,7 @@ static int check_trace_args(rb_ctx* ctx, const rb_scene* scene, const rb_trace_o
   if (opts->order != RB_ORDER_AUTO && opts->order != RB_ORDER_INPUT && opts->order != RB_ORDER_COHERENT)
     return fail(ctx, -1, w + ": unknown order");
   if (opts->schedule != RB_SCHEDULE_AUTO && opts->schedule != RB_SCHEDULE_LANE &&
-      opts->schedule != RB_SCHEDULE_PERSISTENT)
+      opts->schedule != RB_SCHEDULE_PERSISTENT && opts->schedule != RB_SCHEDULE_REGROUP)
     return fail(ctx, -1, w + ": unknown schedule");
   ctx->launches = 0;
   ctx->last_ms = 0;

@@ -27,7 +27,9 @@ struct Slots {
   std::vector<float> tn = std::vector<float>(size_t(kCand) * kCandStride);
 };
 
-template <bool MIXED, int MAXD, bool BORDERS>
+// HINT: the FP64 search ranks the last daughter first, as the work-queue
+// kernels compile it (the library picks those for scenes with a lens)
+template <bool MIXED, int MAXD, bool BORDERS, bool HINT>
 void run(const DScene& base, const TraceParams& P, const RayIO& io, int64_t n) {
   DSceneOf<MAXD, BORDERS> S;
   static_cast<DScene&>(S) = base;
@@ -49,7 +51,7 @@ void run(const DScene& base, const TraceParams& P, const RayIO& io, int64_t n) {
     L.hit_leaf = 0;
     L.ray = P.id_base + uint64_t(i);
     NoRecord rec;
-    trace_lane<MIXED>(S, P, io, i, L, rec);
+    trace_lane<MIXED, HINT>(S, P, io, i, L, rec);
     io.x[i] = L.p.x;
     io.y[i] = L.p.y;
     io.z[i] = L.p.z;
@@ -95,14 +97,19 @@ int emul_trace(const rb_scene_desc* d, const rb_trace_opts* o, rb_rays* r, int64
   rb_emul_fallbacks = 0;
   const bool mixed = o->precision == RB_PRECISION_MIXED;
   const int form = lay.has_borders ? 2 : (lay.max_depth <= 1 && !lay.has_ext) ? 0 : 1;
+  const bool hint = lay.has_lens;
   if (mixed)
-    form == 0 ? run<true, 1, false>(base, P, io, n)
-    : form == 1 ? run<true, kMaxCsgDepth, false>(base, P, io, n)
-                : run<true, kMaxCsgDepth, true>(base, P, io, n);
+    form == 0 ? (hint ? run<true, 1, false, true>(base, P, io, n) : run<true, 1, false, false>(base, P, io, n))
+    : form == 1 ? (hint ? run<true, kMaxCsgDepth, false, true>(base, P, io, n)
+                        : run<true, kMaxCsgDepth, false, false>(base, P, io, n))
+                : (hint ? run<true, kMaxCsgDepth, true, true>(base, P, io, n)
+                        : run<true, kMaxCsgDepth, true, false>(base, P, io, n));
   else
-    form == 0 ? run<false, 1, false>(base, P, io, n)
-    : form == 1 ? run<false, kMaxCsgDepth, false>(base, P, io, n)
-                : run<false, kMaxCsgDepth, true>(base, P, io, n);
+    form == 0 ? (hint ? run<false, 1, false, true>(base, P, io, n) : run<false, 1, false, false>(base, P, io, n))
+    : form == 1 ? (hint ? run<false, kMaxCsgDepth, false, true>(base, P, io, n)
+                        : run<false, kMaxCsgDepth, false, false>(base, P, io, n))
+                : (hint ? run<false, kMaxCsgDepth, true, true>(base, P, io, n)
+                        : run<false, kMaxCsgDepth, true, false>(base, P, io, n));
   if (fallbacks) *fallbacks = rb_emul_fallbacks;
   return 0;
 }

@@ -210,7 +210,8 @@ def test_c4_light_guides_sample_per_ray(ctx):
 def test_coherence_order_gives_identical_results(ctx, config):
     """Tracing in coherence order (sorted by first daughter-box entry, traced
     through a permutation) and in input order gives bit-identical rays, one
-    lane per ray or from the persistent work queue. The two schedules are
+    lane per ray, from the persistent work queue, or with the block's rays
+    regrouped by phase before every step. The schedules are
     separately compiled kernels; with no FMA contraction every operation
     rounds as written in both, so they agree bit for bit too."""
     import torch
@@ -223,7 +224,8 @@ def test_coherence_order_gives_identical_results(ctx, config):
     out = {}
     for order, schedule in ((A.RB_ORDER_INPUT, A.RB_SCHEDULE_LANE), (A.RB_ORDER_COHERENT, A.RB_SCHEDULE_LANE),
                             (A.RB_ORDER_COHERENT, A.RB_SCHEDULE_PERSISTENT),
-                            (A.RB_ORDER_INPUT, A.RB_SCHEDULE_PERSISTENT)):
+                            (A.RB_ORDER_INPUT, A.RB_SCHEDULE_PERSISTENT),
+                            (A.RB_ORDER_COHERENT, A.RB_SCHEDULE_REGROUP), (A.RB_ORDER_INPUT, A.RB_SCHEDULE_REGROUP)):
         t = {f: torch.empty(N, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_")}
         for f in ("status", "npoints", "last_node"):
             t[f] = torch.empty(N, dtype=torch.int32, device=dev)
@@ -232,13 +234,14 @@ def test_coherence_order_gives_identical_results(ctx, config):
         ctx.trace(scene, make_rays_device(t), N, mgr.trace_opts(device_ptrs=1, order=order, schedule=schedule))
         assert ctx.last_trace_launches() == (3 if order == A.RB_ORDER_COHERENT else 1)
         out[order, schedule] = t
-    for schedule in (A.RB_SCHEDULE_LANE, A.RB_SCHEDULE_PERSISTENT):
+    for schedule in (A.RB_SCHEDULE_LANE, A.RB_SCHEDULE_PERSISTENT, A.RB_SCHEDULE_REGROUP):
         a, b = out[A.RB_ORDER_INPUT, schedule], out[A.RB_ORDER_COHERENT, schedule]
         for k in a:
             assert torch.equal(a[k], b[k]), (schedule, k)
-    a, b = out[A.RB_ORDER_INPUT, A.RB_SCHEDULE_LANE], out[A.RB_ORDER_INPUT, A.RB_SCHEDULE_PERSISTENT]
-    for k in a:
-        assert torch.equal(a[k], b[k]), k
+    for schedule in (A.RB_SCHEDULE_PERSISTENT, A.RB_SCHEDULE_REGROUP):
+        a, b = out[A.RB_ORDER_INPUT, A.RB_SCHEDULE_LANE], out[A.RB_ORDER_INPUT, schedule]
+        for k in a:
+            assert torch.equal(a[k], b[k]), (schedule, k)
 
 
 def test_host_memory_passed_as_device_pointers_is_an_error(ctx):
