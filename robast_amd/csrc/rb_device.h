@@ -1493,7 +1493,11 @@ template <typename T, int D, bool EXT> struct Csg {
     h.leaf = leaf_push(h.leaf, side);
     return h;
   }
-  __device__ RB_CSG_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  // Entry distance from outside. `lim`: a caller that rejects any crossing
+  // at or beyond lim (the FP64 search: its best + tolerance) gets kBig as
+  // soon as the iteration has passed it; the accumulated distance only grows,
+  // so this changes no accepted result.
+  __device__ RB_CSG_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d, double lim = kBig) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T, EXT>(sh.type, p, q, d)), 0u, 0u};
@@ -1508,14 +1512,20 @@ template <typename T, int D, bool EXT> struct Csg {
     for (int it = 0; it < 64; it++) {
       V3<T> x = axpy(q, s, d);
       V3<T> xp = axpy(x, push, d);
-      bool inA = Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
-      bool inB = Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
       Hit h;
       if (op == RB_OP_SUBTRACTION) {
-        if (inA && !inB) return {double(s), last.leaf, last.exiting};
-        h = inB ? child(S, p, 1, false, x, d) : child(S, p, 0, true, x, d);
-        if (!inB && h.s >= kBig) return {kBig, 0u, 0u};
+        // inside B: leave B first; A is tested only outside B
+        const bool inB = Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
+        if (inB) {
+          h = child(S, p, 1, false, x, d);
+        } else {
+          if (Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, xp))) return {double(s), last.leaf, last.exiting};
+          h = child(S, p, 0, true, x, d);
+          if (h.s >= kBig) return {kBig, 0u, 0u};
+        }
       } else {
+        const bool inA = Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
+        const bool inB = Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
         if (inA && inB) return {double(s), last.leaf, last.exiting};
         Hit ha = {0.0, 0u, 0u}, hb = {0.0, 0u, 0u};
         if (!inA) {
@@ -1532,6 +1542,7 @@ template <typename T, int D, bool EXT> struct Csg {
       last = h;
       if (step < push) step = push;
       s += step;
+      if (double(s) >= lim) return {kBig, 0u, 0u};
     }
     return {kBig, 0u, 0u};
   }
@@ -1603,7 +1614,7 @@ template <typename T, bool EXT> struct Csg<T, 0, EXT> {
     DShape sh = S.shapes[shape];
     return prim_contains<T, EXT>(sh.type, S.params + sh.off, q);
   }
-  __device__ RB_CSG0_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d) {
+  __device__ RB_CSG0_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d, double = kBig) {
     DShape sh = S.shapes[shape];
     return {double(prim_dist_out<T, EXT>(sh.type, S.params + sh.off, q, d)), 0u, 0u};
   }
@@ -1654,15 +1665,15 @@ __device__ __forceinline__ bool shape_contains(const SC& S, int shape, V3<T> q) 
   return Csg<T, SC::max_depth, SC::ext_on>::contains(S, shape, q);
 }
 template <typename T, typename SC>
-__device__ __forceinline__ Hit shape_dist(const SC& S, int shape, V3<T> q, V3<T> d, bool out) {
+__device__ __forceinline__ Hit shape_dist(const SC& S, int shape, V3<T> q, V3<T> d, bool out, double lim = kBig) {
   switch (S.shapes[shape].depth) {
     case 0: return out ? Csg<T, 0, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 0, SC::ext_on>::dist_in(S, shape, q, d);
-    case 1: return out ? Csg<T, 1, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 1, SC::ext_on>::dist_in(S, shape, q, d);
+    case 1: return out ? Csg<T, 1, SC::ext_on>::dist_out(S, shape, q, d, lim) : Csg<T, 1, SC::ext_on>::dist_in(S, shape, q, d);
     case 2:
-      if constexpr (SC::max_depth >= 3) return out ? Csg<T, 2, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 2, SC::ext_on>::dist_in(S, shape, q, d);
+      if constexpr (SC::max_depth >= 3) return out ? Csg<T, 2, SC::ext_on>::dist_out(S, shape, q, d, lim) : Csg<T, 2, SC::ext_on>::dist_in(S, shape, q, d);
       break;
   }
-  return out ? Csg<T, SC::max_depth, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, SC::max_depth, SC::ext_on>::dist_in(S, shape, q, d);
+  return out ? Csg<T, SC::max_depth, SC::ext_on>::dist_out(S, shape, q, d, lim) : Csg<T, SC::max_depth, SC::ext_on>::dist_in(S, shape, q, d);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t leaf, bool exiting, V3<T> q,

@@ -171,13 +171,15 @@ __device__ __forceinline__ double locate_push(const DScene& S, int n, double ste
 // distance from a ray (origin p, unit d, FP64) to node c's shape, the
 // search itself in precision T; the origin is first rebased to `base`
 // along the ray (the node's box entry) so FP32 works near the node's frame
+// lim: crossings at or beyond it may be reported as kBig (see Csg::dist_out)
 template <typename T, typename SC>
-__device__ RB_NODE_FN Hit node_dist(const SC& S, int c, V3<double> p, V3<double> d, double base, bool out) {
+__device__ RB_NODE_FN Hit node_dist(const SC& S, int c, V3<double> p, V3<double> d, double base, bool out,
+                                    double lim = kBig) {
   const DNode& nd = S.nodes[c];
   V3<double> pb = axpy(p, base, d);
   V3<double> q = m2l<double>(nd.rot, nd.tr, pb), ld = m2lv<double>(nd.rot, d);
   V3<T> qt = mk<T>(T(q.x), T(q.y), T(q.z)), dt = mk<T>(T(ld.x), T(ld.y), T(ld.z));
-  Hit h = shape_dist<T>(S, nd.shape, qt, dt, out);
+  Hit h = shape_dist<T>(S, nd.shape, qt, dt, out, lim - base);
   if (h.s < kBig) h.s += base;
   return h;
 }
@@ -291,7 +293,7 @@ __device__ void search_boundary_fp64(const SC& S, int cur, V3<double> p, V3<doub
   if (cn.n_children == 0) return;
   // (a ray's last node arrives from the caller: bounded before it is read)
   const bool hinted = hint > 0 && hint < S.n_nodes && S.nodes[hint].parent == cur;
-  if (hinted) rank_fp64(hint, node_dist<double>(S, hint, p, d, 0.0, true), best, crossed, hit);
+  if (hinted) rank_fp64(hint, node_dist<double>(S, hint, p, d, 0.0, true, best + kTol), best, crossed, hit);
   const float ox = float(p.x), oy = float(p.y), oz = float(p.z);
   const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
 #if RB_POSTPONE
@@ -301,7 +303,7 @@ __device__ void search_boundary_fp64(const SC& S, int cur, V3<double> p, V3<doub
     if (cand.tn[k * kCandStride] > float(best) * 1.0001f + 1e-2f) break;
     const int c = cand.node[k * kCandStride];
     if (hinted && c == hint) continue;
-    rank_fp64(c, node_dist<double>(S, c, p, d, 0.0, true), best, crossed, hit);
+    rank_fp64(c, node_dist<double>(S, c, p, d, 0.0, true, best + kTol), best, crossed, hit);
   }
   if (!(cut <= float(best) * 1.0001f + 1e-2f)) return;
   // a dropped box may hold a nearer crossing: rank all daughters afresh
@@ -322,7 +324,7 @@ __device__ void search_boundary_fp64(const SC& S, int cur, V3<double> p, V3<doub
       i = b.escape;
       continue;
     }
-    if (b.prim >= 0) rank_fp64(b.prim, node_dist<double>(S, b.prim, p, d, 0.0, true), best, crossed, hit);
+    if (b.prim >= 0) rank_fp64(b.prim, node_dist<double>(S, b.prim, p, d, 0.0, true, best + kTol), best, crossed, hit);
     i++;
   }
 }
