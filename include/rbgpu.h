@@ -403,12 +403,29 @@ int rbgpu_bunches_to_rays(rb_ctx* ctx, const rb_bunches* bunches, int64_t n_bunc
                           rb_rays* rays, int64_t capacity, int64_t* n_rays);
 
 /* ---- adjoint of a focal-plane loss (differentiable alignment) ------------
- * Supported optics: chains of up to 4 reflections on spherical facets, each
- * the node shape `sphere(rmin = R) : T(0, 0, R) * cut` placed with rotation
- * M = M0 Rx(tip) Ry(tilt) (a Davies-Cotton facet), ending on a focal surface
- * whose entry face is the plane z = z_cam. The loss seen by each focused ray
- * is weight * |h - centroid|^2 / 2 on its hit (h_x, h_y), so weight = 2 / N
- * and centroid = mean hit give d(spot RMS^2)/dparams. */
+ * Supported optics: chains of up to 4 reflections, each on
+ *  - a spherical facet (node_param): the node shape
+ *    `sphere(rmin = R) : T(0, 0, R) * cut` placed with rotation
+ *    M = M0 Rx(tip) Ry(tilt) (a Davies-Cotton facet), or
+ *  - an AGeoAsphericDisk surface (surface_param): the disk is the node's own
+ *    shape; the surface the ray reflects on (F1 or F2, picked as the trace's
+ *    normal picks it, src/AGeoAsphericDisk.cxx:246-328) has parameters
+ *    (c, kappa, K_1..K_n): its sag F(r) = z + r^2 c / (1 + sqrt(1 - r^2 c^2 kappa))
+ *    + sum_k K_k r^2k (:126-153), n <= RB_ADJ_MAX_POLY, with kappa = conic + 1
+ *    (SetConicConstants, :1007-1013), so d/dkappa is d/dconic;
+ * ending on a focal surface: the plane z = z_cam, or, when the focal node's
+ * shape is an AGeoAsphericDisk, the disk face the ray enters (a surface or a
+ * wall; z_param then moves the focal node along its own z axis). Rays that
+ * reflect elsewhere (a mirror disk's side wall, a node without parameters)
+ * or refract give no gradient. The loss seen by each focused ray, on its hit (h_x, h_y) and
+ * with u = h - centroid:
+ *   CENTROID, MOMENTS: weight * |u|^2 / 2 (weight = 2 / N and centroid = mean
+ *     hit give d(spot RMS^2)/dparams);
+ *   ENCIRCLED: weight * sigmoid((r0^2 - |u|^2) / (2 r0 w)), a smooth count of
+ *     the hits within r0 of the centroid point (width w), which is a fixed
+ *     point here (the target); weight = -1 / N makes minimising it maximise
+ *     the encircled energy fraction. */
+#define RB_ADJ_MAX_POLY 10
 typedef struct {
   const int32_t* node_param;   /* device [n_nodes][3]: tip, tilt, R parameter index or -1; an index
                                   outside [0, n_params) counts as -1 (the ray gives no gradient) */
@@ -428,6 +445,15 @@ typedef struct {
   int64_t group_size;
   int32_t n_groups;
   int32_t pad;
+  /* AGeoAsphericDisk mirrors: device [n_nodes][2] = (first, count) or NULL.
+   * A node with first >= 0 has the parameters param[first + 0] = c (1/cm),
+   * [first + 1] = kappa, [first + 1 + k] = K_k (cm^(1-2k)) of the surface a
+   * ray reflects on; count must be >= 2 + n of that surface and
+   * first + count <= n_params. The gradient is with respect to those values;
+   * the caller keeps them equal to the scene's (param_values is not read for
+   * these nodes). */
+  const int32_t* surface_param;
+  double ee_radius, ee_width;    /* RB_ADJ_ENCIRCLED: r0 > 0 and w > 0 (cm) */
 } rb_adjoint_desc;
 
 /* RB_ADJ_CENTROID: grad[p] += weight * sum over focused rays of
@@ -443,7 +469,9 @@ typedef struct {
  *   (2 / N) (sum u.dh/dp - m_x sum dh_x/dp - m_y sum dh_y/dp).
  * A reference point near the centroid keeps the FP32 products small. The
  * accumulators of all groups must fit: groups x (3 P + 4) <= 6400. */
-enum { RB_ADJ_CENTROID = 0, RB_ADJ_MOMENTS = 1 };
+/* RB_ADJ_ENCIRCLED: grad[p] += sum over focused rays of d(loss)/dp, and
+ * grad[n_params] += the summed loss (grad has n_params + 1 entries). */
+enum { RB_ADJ_CENTROID = 0, RB_ADJ_MOMENTS = 1, RB_ADJ_ENCIRCLED = 2 };
 
 /* Re-trace the n rays (left unchanged) and accumulate into grad (device,
  * FP64; added to, not overwritten) as `adj->mode` selects. Per-ray

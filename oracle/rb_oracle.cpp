@@ -963,14 +963,15 @@ static bool asph_contains(const double* p, V3 q) {
   return !(q.z < f1 || f2 < q.z);
 }
 // ComputeNormal (:246-327): closest of the four surfaces, rotated by phi
-static V3 asph_normal(const double* p, V3 q, V3 d) {
-  Asph a = asph_p(p);
+// ComputeNormal's choice of face (:246-328): the one with the smallest
+// safety, 0 inner wall, 1 outer wall, 2 + s surface s; *df receives the
+// surfaces' slopes dF/dr (kBig where undefined).
+static int asph_face(const Asph& a, V3 q, double* df) {
   double r = std::sqrt(q.x * q.x + q.y * q.y);
-  double phi = std::atan2(q.y, q.x);
   double saf[4];
   saf[0] = a.rmin > 0 ? std::fabs(r - a.rmin) : kBig;
   saf[1] = std::fabs(r - a.rmax);
-  double df[2] = {kBig, kBig};
+  df[0] = df[1] = kBig;
   for (int s = 0; s < 2; s++) {
     double f;
     if (!asph_sag(a, s, r, f)) { saf[2 + s] = kBig; continue; }
@@ -980,6 +981,13 @@ static V3 asph_normal(const double* p, V3 q, V3 d) {
   int k = 0;
   for (int i = 1; i < 4; i++)
     if (saf[i] < saf[k]) k = i;
+  return k;
+}
+static V3 asph_normal(const double* p, V3 q, V3 d) {
+  Asph a = asph_p(p);
+  double phi = std::atan2(q.y, q.x);
+  double df[2];
+  const int k = asph_face(a, q, df);
   double n0 = 0, n2 = 0;
   if (k <= 1) {
     n0 = 1;
@@ -2173,14 +2181,19 @@ static void trace_ray(const Scene& sc, const Opts& o, Ray& ray, Rng& rng) {
 }
 
 // ---------------------------------------------------------------- dual-number adjoint reference
-// Forward-mode restatement of the focal-plane loss gradient for chains of
-// spherical-facet reflections, used to check the device adjoint. Each focused
-// ray is traced as above with its path recorded; the chain is then recomputed
-// with dual numbers whose tangents are the (tip, tilt, R) of each facet the
-// ray reflects on and the focal-plane z. Nothing here is hand-differentiated:
-// the facet rotation is rebuilt as M0 Rx(tip) Ry(tilt) and every step follows
-// the forward formulas of reflect() and the navigator's step back.
-constexpr int kDualN = 13;  // 3 per reflection (at most 4) + focal-plane z
+// Forward-mode restatement of the focal-plane loss gradient, used to check
+// the device adjoint. Each focused ray is traced as above with its path
+// recorded; the chain is then recomputed with dual numbers whose tangents are
+// the parameters of each mirror the ray reflects on -- (tip, tilt, R) of a
+// spherical facet, (c, kappa, K_1..K_n) of an AGeoAsphericDisk surface --
+// and the focal-surface z. Nothing here is hand-differentiated: the facet
+// rotation is rebuilt as M0 Rx(tip) Ry(tilt), the crossings are solved again
+// (the sphere's quadratic; Newton steps on the disk surface's sag from the
+// recorded crossing, CalcF and CalcdF/dr of src/AGeoAsphericDisk.cxx:96-153),
+// and every step follows the forward formulas of reflect() and the
+// navigator's step back.
+constexpr int kDualPer = 2 + RB_ADJ_MAX_POLY;    // tangents per reflection
+constexpr int kDualN = 4 * kDualPer + 1;         // at most 4 reflections + focal z
 struct Dual {
   double v;
   double d[kDualN];
@@ -2249,76 +2262,214 @@ static inline DV3 operator-(const DV3& a, const DV3& b) { return {a.x - b.x, a.y
 static inline DV3 operator*(const DV3& a, const Dual& s) { return {a.x * s, a.y * s, a.z * s}; }
 static inline Dual ddot(const DV3& a, const DV3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline V3 val(const DV3& a) { return v3(a.x.v, a.y.v, a.z.v); }
+// local = R^T (p - T) and master = R local (+ T), R constant
+static inline DV3 dm2lv(const rb_matrix& m, const DV3& q) {
+  const double* r = m.rot;
+  return {dc(r[0]) * q.x + dc(r[3]) * q.y + dc(r[6]) * q.z, dc(r[1]) * q.x + dc(r[4]) * q.y + dc(r[7]) * q.z,
+          dc(r[2]) * q.x + dc(r[5]) * q.y + dc(r[8]) * q.z};
+}
+static inline DV3 dl2mv(const rb_matrix& m, const DV3& q) {
+  const double* r = m.rot;
+  return {dc(r[0]) * q.x + dc(r[1]) * q.y + dc(r[2]) * q.z, dc(r[3]) * q.x + dc(r[4]) * q.y + dc(r[5]) * q.z,
+          dc(r[6]) * q.x + dc(r[7]) * q.y + dc(r[8]) * q.z};
+}
+static inline DV3 dtr(const rb_matrix& m) { return dv(v3(m.tr[0], m.tr[1], m.tr[2])); }
+
+// A disk surface's coefficients as duals (c, kappa, K_k), its offset z.
+struct DSurf {
+  Dual c, kap, K[RB_ADJ_MAX_POLY];
+  double z;
+  int n;
+};
+static DSurf dsurf(const Asph& a, int s, int base) {
+  DSurf f;
+  f.c = base >= 0 ? dvar(a.c[s], base) : dc(a.c[s]);
+  f.kap = base >= 0 ? dvar(a.kap[s], base + 1) : dc(a.kap[s]);
+  f.n = a.n[s];
+  for (int k = 0; k < f.n; k++) f.K[k] = base >= 0 ? dvar(a.K[s][k], base + 2 + k) : dc(a.K[s][k]);
+  f.z = a.z[s];
+  return f;
+}
+// G = q_z - CalcF(r) and a = CalcdFdr(r) / r at the local point q
+static void dsurf_eval(const DSurf& f, const DV3& q, Dual& G, Dual& a) {
+  const Dual w = q.x * q.x + q.y * q.y;
+  const Dual sq = dsqrt(dc(1) - w * f.c * f.c * f.kap);
+  Dual F = w * f.c / (dc(1) + sq);
+  a = f.c / sq;
+  Dual wk1 = dc(1);  // w^(k-1)
+  for (int k = 1; k <= f.n; k++) {
+    F = F + f.K[k - 1] * wk1 * w;
+    a = a + dc(2.0 * k) * f.K[k - 1] * wk1;
+    wk1 = wk1 * w;
+  }
+  G = q.z - dc(f.z) - F;
+}
+// The ray o + t e (local) on the surface: Newton steps in t from t0, the
+// recorded crossing, to machine precision; with dual numbers the tangent
+// converges with the value.
+static Dual dsurf_cross(const DSurf& f, const DV3& o, const DV3& e, double t0) {
+  Dual t = dc(t0);
+  for (int it = 0; it < 5; it++) {
+    const DV3 q = o + e * t;
+    Dual G, a;
+    dsurf_eval(f, q, G, a);
+    const Dual gp = e.z - a * (q.x * e.x + q.y * e.y);
+    t = t - G / gp;
+  }
+  return t;
+}
+
+// How the chain's reflection (or the focal hit) is differentiated; as the
+// device classifies it.
+enum { kDNone = 0, kDFacet = 1, kDAsph = 2, kDPlane = 3 };
+static int dual_kind(const rb_scene_desc* d, const rb_adjoint_desc* A, int node, V3 P, int& surf) {
+  if (A->node_param) {
+    const int32_t* np = A->node_param + 3 * node;
+    bool ok = true;
+    for (int k = 0; k < 3; k++) ok = ok && np[k] >= 0 && np[k] < A->n_params;
+    if (ok) return kDFacet;
+  }
+  if (!A->surface_param) return kDNone;
+  const int32_t first = A->surface_param[2 * node], count = A->surface_param[2 * node + 1];
+  const rb_shape_desc& sh = d->shapes[d->volumes[d->nodes[node].volume].shape];
+  if (first < 0 || sh.type != RB_SHAPE_ASPHERIC) return kDNone;
+  const Asph a = asph_p(d->params + sh.param_offset);
+  double df[2];
+  const int face = asph_face(a, m2l(d->nodes[node].global, P), df);
+  if (face < 2) return kDNone;
+  surf = face - 2;
+  if (a.n[surf] > RB_ADJ_MAX_POLY || count < 2 + a.n[surf] || int64_t(first) + count > A->n_params) return kDNone;
+  return kDAsph;
+}
 
 // Loss gradient of one focused ray; returns false when the chain holds a
-// reflection on a node without parameters (the device adjoint skips those too).
+// reflection that cannot be differentiated (the device adjoint skips those too).
 static bool dual_ray(const rb_scene_desc* d, const rb_adjoint_desc* A, V3 origin, V3 d0, const double* path,
                      const int32_t* pnode, int K, V3 hit, double* grad) {
-  for (int j = 0; j < K; j++) {
-    const int32_t* np = A->node_param + 3 * pnode[j];
-    for (int k = 0; k < 3; k++)
-      if (np[k] < 0 || np[k] >= A->n_params) return false;
-  }
   const double eps = g_step_back;
+  if (K > 4) return false;
+  int kinds[4] = {kDNone, kDNone, kDNone, kDNone}, surfs[4] = {0, 0, 0, 0};
+  // the recorded points are the crossings stepped back by eps along the
+  // incoming direction: moved forward again once that direction is known
   DV3 o = dv(origin), din = dv(d0);
   for (int j = 0; j < K; j++) {
-    const int32_t* np = A->node_param + 3 * pnode[j];
-    const rb_matrix& g = d->nodes[pnode[j]].global;
-    const double tip0 = A->param_values[np[0]], tilt0 = A->param_values[np[1]];
-    // M0 = M Ry(tilt0)^T Rx(tip0)^T: the placement without the tip/tilt
-    double rx[9] = {1, 0, 0, 0, std::cos(tip0), -std::sin(tip0), 0, std::sin(tip0), std::cos(tip0)};
-    double ry[9] = {std::cos(tilt0), 0, std::sin(tilt0), 0, 1, 0, -std::sin(tilt0), 0, std::cos(tilt0)};
-    double e[9], m0[9];
-    for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 3; c++) {
-        double s = 0;
-        for (int k = 0; k < 3; k++) s += rx[c * 3 + k] * ry[k * 3 + r];  // (Rx Ry)^T
-        e[r * 3 + c] = s;
-      }
-    for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 3; c++) {
-        double s = 0;
-        for (int k = 0; k < 3; k++) s += g.rot[r * 3 + k] * e[k * 3 + c];
-        m0[r * 3 + c] = s;
-      }
-    const Dual tip = dvar(tip0, 3 * j), tilt = dvar(tilt0, 3 * j + 1), R = dvar(A->param_values[np[2]], 3 * j + 2);
-    // local axis of the sphere centre: Rx(tip) Ry(tilt) e_z = (sin tilt, -sin tip cos tilt, cos tip cos tilt)
-    const Dual ct = dcos(tilt);
-    const DV3 a = {dsin(tilt), dc(0) - dsin(tip) * ct, dcos(tip) * ct};
-    DV3 C;
-    C.x = dc(g.tr[0]) + (dc(m0[0]) * a.x + dc(m0[1]) * a.y + dc(m0[2]) * a.z) * R;
-    C.y = dc(g.tr[1]) + (dc(m0[3]) * a.x + dc(m0[4]) * a.y + dc(m0[5]) * a.z) * R;
-    C.z = dc(g.tr[2]) + (dc(m0[6]) * a.x + dc(m0[7]) * a.y + dc(m0[8]) * a.z) * R;
-    // |o + t din - C| = R; the root nearest the recorded crossing
-    const DV3 oc = o - C;
-    const Dual b = ddot(oc, din), c = ddot(oc, oc) - R * R;
-    const Dual q = dsqrt(b * b - c);
-    const Dual t1 = dc(0) - b - q, t2 = dc(0) - b + q;
+    const int node = pnode[j];
+    const rb_matrix& g = d->nodes[node].global;
     const V3 rec = axpy(v3(path[4 * j], path[4 * j + 1], path[4 * j + 2]), eps, val(din));
-    const V3 p1 = axpy(val(o), t1.v, val(din)), p2 = axpy(val(o), t2.v, val(din));
-    const V3 e1 = sub(p1, rec), e2 = sub(p2, rec);
-    const Dual t = dot(e1, e1) <= dot(e2, e2) ? t1 : t2;
-    const DV3 P = o + din * t;
-    const DV3 n = (P - C) * (dc(1) / R);
+    kinds[j] = dual_kind(d, A, node, rec, surfs[j]);
+    if (kinds[j] == kDNone) return false;
+    const int base = kDualPer * j;
+    DV3 P, n;
+    if (kinds[j] == kDFacet) {
+      const int32_t* np = A->node_param + 3 * node;
+      const double tip0 = A->param_values[np[0]], tilt0 = A->param_values[np[1]];
+      // M0 = M Ry(tilt0)^T Rx(tip0)^T: the placement without the tip/tilt
+      double rx[9] = {1, 0, 0, 0, std::cos(tip0), -std::sin(tip0), 0, std::sin(tip0), std::cos(tip0)};
+      double ry[9] = {std::cos(tilt0), 0, std::sin(tilt0), 0, 1, 0, -std::sin(tilt0), 0, std::cos(tilt0)};
+      double e[9], m0[9];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+          double s = 0;
+          for (int k = 0; k < 3; k++) s += rx[c * 3 + k] * ry[k * 3 + r];  // (Rx Ry)^T
+          e[r * 3 + c] = s;
+        }
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+          double s = 0;
+          for (int k = 0; k < 3; k++) s += g.rot[r * 3 + k] * e[k * 3 + c];
+          m0[r * 3 + c] = s;
+        }
+      const Dual tip = dvar(tip0, base), tilt = dvar(tilt0, base + 1), R = dvar(A->param_values[np[2]], base + 2);
+      // local axis of the sphere centre: Rx(tip) Ry(tilt) e_z = (sin tilt, -sin tip cos tilt, cos tip cos tilt)
+      const Dual ct = dcos(tilt);
+      const DV3 a = {dsin(tilt), dc(0) - dsin(tip) * ct, dcos(tip) * ct};
+      DV3 C;
+      C.x = dc(g.tr[0]) + (dc(m0[0]) * a.x + dc(m0[1]) * a.y + dc(m0[2]) * a.z) * R;
+      C.y = dc(g.tr[1]) + (dc(m0[3]) * a.x + dc(m0[4]) * a.y + dc(m0[5]) * a.z) * R;
+      C.z = dc(g.tr[2]) + (dc(m0[6]) * a.x + dc(m0[7]) * a.y + dc(m0[8]) * a.z) * R;
+      // |o + t din - C| = R; the root nearest the recorded crossing
+      const DV3 oc = o - C;
+      const Dual b = ddot(oc, din), c = ddot(oc, oc) - R * R;
+      const Dual q = dsqrt(b * b - c);
+      const Dual t1 = dc(0) - b - q, t2 = dc(0) - b + q;
+      const V3 p1 = axpy(val(o), t1.v, val(din)), p2 = axpy(val(o), t2.v, val(din));
+      const V3 e1 = sub(p1, rec), e2 = sub(p2, rec);
+      const Dual t = dot(e1, e1) <= dot(e2, e2) ? t1 : t2;
+      P = o + din * t;
+      n = (P - C) * (dc(1) / R);
+    } else {
+      const rb_shape_desc& sh = d->shapes[d->volumes[d->nodes[node].volume].shape];
+      const DSurf f = dsurf(asph_p(d->params + sh.param_offset), surfs[j], base);
+      const DV3 ol = dm2lv(g, o - dtr(g)), el = dm2lv(g, din);
+      const Dual t = dsurf_cross(f, ol, el, dot(sub(rec, val(o)), val(din)));
+      const DV3 q = ol + el * t;
+      Dual G, a;
+      dsurf_eval(f, q, G, a);
+      const DV3 gl = {dc(0) - a * q.x, dc(0) - a * q.y, dc(1)};
+      n = dl2mv(g, gl * (dc(1) / dsqrt(ddot(gl, gl))));
+      P = dtr(g) + dl2mv(g, q);
+    }
     const Dual k = ddot(din, n);
     DV3 dout = din - n * (dc(2) * k);
     dout = dout * (dc(1) / dsqrt(ddot(dout, dout)));
     o = P - din * dc(eps);
     din = dout;
   }
-  // focal plane z = hit.z, moved by the z parameter
-  const Dual zc = A->z_param >= 0 ? dvar(hit.z, kDualN - 1) : dc(hit.z);
-  const Dual s = (zc - o.z) / din.z;
-  const DV3 h = o + din * s;
-  const double gx = A->weight * (h.x.v - A->centroid_x), gy = A->weight * (h.y.v - A->centroid_y);
+  // the focal surface: the plane z = hit.z moved by the z parameter, or the
+  // focal node's disk surface moved along its z axis
+  const Dual dz = A->z_param >= 0 ? dvar(0.0, kDualN - 1) : dc(0.0);
+  DV3 h;
+  const rb_node_desc& fn = d->nodes[A->focal_node];
+  const rb_shape_desc& fsh = d->shapes[d->volumes[fn.volume].shape];
+  if (fsh.type == RB_SHAPE_ASPHERIC) {
+    const Asph a = asph_p(d->params + fsh.param_offset);
+    double df[2];
+    const int face = asph_face(a, m2l(fn.global, hit), df);
+    DV3 ol = dm2lv(fn.global, o - dtr(fn.global)), el = dm2lv(fn.global, din);
+    ol.z = ol.z - dz;
+    const double s0 = dot(sub(hit, val(o)), val(din));
+    Dual s;
+    if (face >= 2) {
+      s = dsurf_cross(dsurf(a, face - 2, -1), ol, el, s0);
+    } else {  // a wall, r = rmin or rmax: Newton steps on q_x^2 + q_y^2 - r^2
+      const double rw = face == 0 ? a.rmin : a.rmax;
+      s = dc(s0);
+      for (int it = 0; it < 5; it++) {
+        const DV3 q = ol + el * s;
+        const Dual G = q.x * q.x + q.y * q.y - dc(rw * rw);
+        const Dual gp = dc(2) * (q.x * el.x + q.y * el.y);
+        s = s - G / gp;
+      }
+    }
+    h = o + din * s;
+  } else {
+    const Dual s = (dc(hit.z) + dz - o.z) / din.z;
+    h = o + din * s;
+  }
+  // the loss adjoint on (h_x, h_y)
+  const double ux = h.x.v - A->centroid_x, uy = h.y.v - A->centroid_y;
+  double gx = A->weight * ux, gy = A->weight * uy;
+  if (A->mode == RB_ADJ_ENCIRCLED) {
+    const double r0 = A->ee_radius, w = A->ee_width;
+    const double sg = 1.0 / (1.0 + std::exp(-(r0 * r0 - ux * ux - uy * uy) / (2 * r0 * w)));
+    const double f = -A->weight * sg * (1 - sg) / (r0 * w);
+    gx = f * ux;
+    gy = f * uy;
+  }
   auto put = [&](int param, int k) {
     if (param >= 0) grad[param] += gx * h.x.d[k] + gy * h.y.d[k];
   };
   for (int j = 0; j < K; j++) {
-    const int32_t* np = A->node_param + 3 * pnode[j];
-    put(np[0], 3 * j);
-    put(np[1], 3 * j + 1);
-    put(np[2], 3 * j + 2);
+    const int base = kDualPer * j;
+    if (kinds[j] == kDFacet) {
+      const int32_t* np = A->node_param + 3 * pnode[j];
+      for (int k = 0; k < 3; k++) put(np[k], base + k);
+    } else {
+      const rb_shape_desc& sh = d->shapes[d->volumes[d->nodes[pnode[j]].volume].shape];
+      const int n = int(d->params[sh.param_offset + 8 + surfs[j]]);
+      const int first = A->surface_param[2 * pnode[j]];
+      for (int k = 0; k < 2 + n; k++) put(first + k, base + k);
+    }
   }
   put(A->z_param, kDualN - 1);
   return true;

@@ -34,7 +34,8 @@ RB_PRECISION_FP64, RB_PRECISION_MIXED = 0, 1
 
 RB_ORDER_AUTO, RB_ORDER_INPUT, RB_ORDER_COHERENT = 0, 1, 2
 
-RB_ADJ_CENTROID, RB_ADJ_MOMENTS = 0, 1
+RB_ADJ_CENTROID, RB_ADJ_MOMENTS, RB_ADJ_ENCIRCLED = 0, 1, 2
+RB_ADJ_MAX_POLY = 10
 
 RB_RAYGEN_GRID, RB_RAYGEN_DISK, RB_RAYGEN_CONE = 0, 1, 2
 (RB_SHOOT_RECTANGLE, RB_SHOOT_CIRCLE, RB_SHOOT_RANDOM_CIRCLE, RB_SHOOT_RANDOM_CONE, RB_SHOOT_RANDOM_RECTANGLE,
@@ -117,6 +118,7 @@ class rb_adjoint_desc(C.Structure):
         ("z_param", C.c_int32), ("focal_node", C.c_int32), ("mode", C.c_int32),
         ("centroid_x", C.c_double), ("centroid_y", C.c_double), ("weight", C.c_double),
         ("ref_xy", C.c_void_p), ("group_size", C.c_int64), ("n_groups", C.c_int32), ("pad", C.c_int32),
+        ("surface_param", C.c_void_p), ("ee_radius", C.c_double), ("ee_width", C.c_double),
     ]
 
 

@@ -229,31 +229,47 @@ def c3_coefficients(seed=0xC3):
     return out
 
 
-def c3_scene(seed=0xC3):
+def c3_figure(seed=0xC3):
+    """The mirrors' figures as ((c, kappa, K[]) primary, (c, kappa, K[])
+    secondary): base curvature 1/(2F), kappa = conic + 1, r^2k coefficients."""
+    co = c3_coefficients(seed)
+    (kp, polp), (ks, pols) = co["primary"], co["secondary"]
+    return ((0.5 / C3_FP, kp + 1.0, list(polp)), (0.5 / C3_FS, ks + 1.0, list(pols)))
+
+
+def c3_scene(seed=0xC3, figure=None, focal_dz=0.0):
     """Primary (R 4.8 m, hole 2.2 m) and secondary (R 2.7 m) 1 um thick
     AGeoAsphericDisk mirrors with base curvature 1/(2F) from the tutorial,
     seeded conics and r^2..r^16 terms; spherical-cap focal surface (R_c = 1 m,
     0.4 m across) with an obscuration disk behind it. World -> top -> optics,
-    as the tutorial nests them."""
-    co = c3_coefficients(seed)
+    as the tutorial nests them. figure: the mirrors' (c, kappa, K[]) in place
+    of the seeded ones (both surfaces of each disk; c3_figure's layout);
+    focal_dz moves the focal surface along z."""
+    if figure is None:  # the seeded conics as drawn (kappa - 1 would round)
+        co = c3_coefficients(seed)
+        (conic_p, polp), (conic_s, pols) = co["primary"], co["secondary"]
+        cp, cs = 0.5 / C3_FP, 0.5 / C3_FS
+    else:
+        (cp, kap_p, polp), (cs, kap_s, pols) = figure
+        conic_p, conic_s = kap_p - 1.0, kap_s - 1.0
     mgr = AOpticsManager("manager", "SC")
     world = AOpticalComponent("world", TGeoBBox("worldbox", 30 * m, 30 * m, 30 * m))
     mgr.SetTopVolume(world)
     top = AOpticalComponent("top", TGeoBBox("topbox", 25 * m, 25 * m, 25 * m))
-    cp, cs = 0.5 / C3_FP, 0.5 / C3_FS
-    kp, polp = co["primary"]
     prim = AGeoAsphericDisk("primaryV", -1 * um, cp, 0.0, cp, C3_RP_MAX, C3_RP_MIN)
-    prim.SetConicConstants(kp, kp)
-    prim.SetPolynomials(C3_NPOL, polp, C3_NPOL, polp)
+    prim.SetConicConstants(conic_p, conic_p)
+    prim.SetPolynomials(len(polp), polp, len(polp), polp)
     top.AddNode(AMirror("primaryMirror", prim), 1)
-    ks, pols = co["secondary"]
     sec = AGeoAsphericDisk("secondaryV", C3_ZS, cs, C3_ZS + 1 * um, cs, C3_RS_MAX, 0.0)
-    sec.SetConicConstants(ks, ks)
-    sec.SetPolynomials(C3_NPOL, pols, C3_NPOL, pols)
+    sec.SetConicConstants(conic_s, conic_s)
+    sec.SetPolynomials(len(pols), pols, len(pols), pols)
     top.AddNode(AMirror("secondaryMirror", sec), 1)
     cf = -1.0 / C3_FOCAL_RC
     foc = AGeoAsphericDisk("focalV", C3_ZF - 1 * mm, cf, C3_ZF, cf, C3_FOCAL_R, 0.0)
-    top.AddNode(AFocalSurface("focalPlane", foc), 1)
+    if focal_dz:
+        top.AddNode(AFocalSurface("focalPlane", foc), 1, TGeoTranslation(0.0, 0.0, float(focal_dz)))
+    else:
+        top.AddNode(AFocalSurface("focalPlane", foc), 1)
     zlow = foc.CalcF1(C3_FOCAL_R)
     obs = AObscuration("focalObs", TGeoTube("focalObsV", 0, C3_FOCAL_R * 1.25, 0.5 * cm))
     top.AddNode(obs, 1, TGeoTranslation(0, 0, zlow - 1 * cm))

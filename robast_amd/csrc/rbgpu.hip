@@ -1511,12 +1511,15 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   if (!ctx) return -1;
   if (!scene || !opts || !rays || !adj || !grad || n < 0) return fail(ctx, -1, "rbgpu_adjoint: null argument");
   if (!opts->device_ptrs) return fail(ctx, -1, "rbgpu_adjoint: rays must be device pointers");
-  if (!adj->node_param || !adj->param_values || adj->n_params <= 0 || adj->n_params > kMaxAdjParams)
-    return fail(ctx, -1, "rbgpu_adjoint: parameter table missing or larger than " + std::to_string(kMaxAdjParams));
+  if ((!adj->node_param && !adj->surface_param) || !adj->param_values || adj->n_params <= 0 ||
+      adj->n_params > kMaxAdjParams)
+    return fail(ctx, -1, "rbgpu_adjoint: parameter tables missing or larger than " + std::to_string(kMaxAdjParams));
   if (adj->z_param >= adj->n_params || adj->focal_node >= scene->s.n_nodes)
     return fail(ctx, -1, "rbgpu_adjoint: z parameter or focal node out of range");
-  if (adj->mode != RB_ADJ_CENTROID && adj->mode != RB_ADJ_MOMENTS)
+  if (adj->mode != RB_ADJ_CENTROID && adj->mode != RB_ADJ_MOMENTS && adj->mode != RB_ADJ_ENCIRCLED)
     return fail(ctx, -1, "rbgpu_adjoint: unknown mode");
+  if (adj->mode == RB_ADJ_ENCIRCLED && !(adj->ee_radius > 0 && adj->ee_width > 0))
+    return fail(ctx, -1, "rbgpu_adjoint: the encircled-energy radius and width must be positive");
   if (scene->has_borders)
     return fail(ctx, -3, "rbgpu_adjoint: rough or Lambertian border surfaces are not differentiated");
   const int groups = adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 ? adj->n_groups : 1;
@@ -1530,7 +1533,8 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
       !rays->status || !rays->npoints)
     return fail(ctx, -1, "rbgpu_adjoint: missing ray array");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (!rays_accessible(rays, false) || !gpu_accessible(adj->node_param) || !gpu_accessible(adj->param_values) ||
+  if (!rays_accessible(rays, false) || (adj->node_param && !gpu_accessible(adj->node_param)) ||
+      (adj->surface_param && !gpu_accessible(adj->surface_param)) || !gpu_accessible(adj->param_values) ||
       !gpu_accessible(grad) || (adj->mode == RB_ADJ_MOMENTS && adj->n_groups > 0 && !gpu_accessible(adj->ref_xy)))
     return fail(ctx, -1, "rbgpu_adjoint: rays, parameter table, values and grad must be GPU memory");
   ctx->launches = 0;
@@ -1550,6 +1554,9 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   io.path_cap = 0;
   AdjParams A;
   A.node_param = adj->node_param;
+  A.surface_param = adj->surface_param;
+  A.ee_r0 = adj->ee_radius;
+  A.ee_w = adj->ee_width;
   A.values = adj->param_values;
   A.n_params = adj->n_params;
   A.z_param = adj->z_param;

@@ -1,0 +1,281 @@
+"""Differentiable figure of AGeoAsphericDisk mirrors (the Schwarzschild-Couder
+geometry): the gradient of the focal-surface loss with respect to each
+mirror's curvature, kappa (= conic + 1) and r^2k coefficients and the focal
+surface's z offset.
+
+The CPU dual-number reference (oracle_adjoint_dual, Newton steps on the disk
+surface's sag carried out in dual numbers) is pinned first, with no GPU:
+against the closed form of a single ray reflected by one disk onto a plane,
+differentiated at 40 digits by mpmath; and against central finite
+differences (relative step 1e-7) of the oracle's own FP64 traces of the
+spot RMS^2 and of the smooth encircled-energy loss. The GPU adjoint
+(rbgpu_adjoint, through the C ABI) is then checked against the dual
+reference and against central differences of the GPU FP64 forward trace, and
+drives an Adam fit of the figure.
+"""
+import mpmath as mp
+import numpy as np
+import pytest
+
+from helpers import copy_soa
+from oracle import oracle as O
+from robast_amd import _abi as A
+from robast_amd import aspheric_fit as F
+from robast_amd import geometry as G
+from robast_amd.manager import AOpticsManager
+
+P = F.PER_MIRROR
+FD_INDEX = [0, 1, 2, 5, P - 1, P, P + 1, P + 3, 2 * P - 1, F.Z_PARAM]
+
+
+def fd_step(p, k):
+    """Central-difference step: 1e-7 of the parameter's size, 1e-7 m for the
+    focal offset."""
+    return 1e-5 if k == F.Z_PARAM else 1e-7 * abs(p[k])
+
+
+def _srcs(n, angles=(0.0, 3.0)):
+    return [O.generate(F.raygen(a, k), n) for k, a in enumerate(angles)]
+
+
+def _oracle_hits(p, srcs):
+    mgr = F.figure_scene(p)
+    flat = mgr.flat()
+    _, focal = F.surface_table(flat)
+    out = []
+    for s in srcs:
+        o = copy_soa(s)
+        O.trace(flat, o, mgr.trace_opts(), nthreads=8)
+        m = (o["status"] == A.RB_FOCUS) & (o["last_node"] == focal)
+        out.append((o["x"][m], o["y"][m], m))
+    return out
+
+
+def _spot_loss(p, srcs):
+    total, stats, masks = 0.0, [], []
+    for hx, hy, m in _oracle_hits(p, srcs):
+        cx, cy = hx.mean(), hy.mean()
+        total += ((hx - cx) ** 2 + (hy - cy) ** 2).mean()
+        stats.append((len(hx), cx, cy))
+        masks.append(m)
+    return total, stats, masks
+
+
+def _ee_loss(p, srcs, targets, r0, w):
+    total, counts, masks = 0.0, [], []
+    for (hx, hy, m), (tx, ty) in zip(_oracle_hits(p, srcs), targets):
+        u = (r0 * r0 - (hx - tx) ** 2 - (hy - ty) ** 2) / (2 * r0 * w)
+        total -= (1.0 / (1.0 + np.exp(-u))).mean()
+        counts.append(len(hx))
+        masks.append(m)
+    return total, counts, masks
+
+
+def _dual(p, srcs, per_set):
+    """per_set: (mode, cx, cy, weight, r0, w) per ray set."""
+    mgr = F.figure_scene(p)
+    flat = mgr.flat()
+    table, focal = F.surface_table(flat)
+    values = np.ascontiguousarray(p, dtype=float)
+    g, used = np.zeros(F.NPARAM), 0
+    for s, (mode, cx, cy, wt, r0, w) in zip(srcs, per_set):
+        d = F.figure_desc(table, values, focal, mode, cx, cy, wt, r0, w)
+        gi, u = O.adjoint_dual(flat, s, mgr.trace_opts(), d, F.NPARAM)
+        g += gi
+        used += u
+    return g, used
+
+
+def _check_fd(loss_of, p0, g, masks):
+    for k in FD_INDEX:
+        h = fd_step(p0, k)
+        pp, pm = p0.copy(), p0.copy()
+        pp[k] += h
+        pm[k] -= h
+        Lp, mp_ = loss_of(pp)
+        Lm, mm_ = loss_of(pm)
+        assert all(np.array_equal(a, b) for a, b in zip(mp_, masks)) and all(
+            np.array_equal(a, b) for a, b in zip(mm_, masks)), ("a ray changed status under the FD step", k)
+        fd = (Lp - Lm) / (2 * h)
+        assert fd != 0.0
+        assert g[k] == pytest.approx(fd, rel=1e-4), (k, g[k], fd)
+
+
+# ---------------------------------------------------------------- CPU
+
+
+def _one_disk(c, kap, K, plane_z):
+    """One thin disk mirror (surfaces F1 at -1 um and F2 at 0, the same
+    figure) under a focal plane whose bottom face is z = plane_z."""
+    G._reset_registry()
+    mgr = AOpticsManager("manager", "one disk")
+    world = G.AOpticalComponent("world", G.TGeoBBox("worldbox", 1000.0, 1000.0, 1000.0))
+    mgr.SetTopVolume(world)
+    disk = G.AGeoAsphericDisk("diskV", -1e-4, c, 0.0, c, 60.0, 0.0)
+    disk.SetConicConstants(kap - 1.0, kap - 1.0)
+    disk.SetPolynomials(len(K), K, len(K), K)
+    world.AddNode(G.AMirror("disk", disk), 1)
+    world.AddNode(G.AFocalSurface("plane", G.TGeoBBox("planebox", 500.0, 500.0, 1.0)), 1,
+                  G.TGeoTranslation(0.0, 0.0, plane_z + 1.0))
+    mgr.CloseGeometry()
+    return mgr
+
+
+def test_dual_single_ray_closed_form():
+    """A ray going straight down at height r reflects off the disk's top
+    surface F(r) = w c / (1 + sqrt(1 - w c^2 kappa)) + sum K_k w^k (w = r^2)
+    and meets the plane z = L at
+      h_x = r - (L - F(r) - eps) 2 F'(r) / (1 - F'(r)^2)
+    (eps = 2e-6 cm, the step back). Its derivatives in c, kappa, K_k and L,
+    taken by mpmath at 40 digits from this formula alone, are the dual
+    reference's Jacobian row to 1e-8."""
+    c, kap, K, L, r = 1.0 / 400.0, 0.4, [2e-6, -3e-10, 5e-14], 150.0, 23.0
+    mgr = _one_disk(c, kap, K, L)
+    flat = mgr.flat()
+    names = flat.node_names
+    disk, plane = names.index("disk_1"), names.index("plane_1")
+    table = np.full((len(flat.nodes), 2), -1, dtype=np.int32)
+    table[disk] = (0, 2 + len(K))
+    n_params = 2 + len(K) + 1
+    values = np.array([c, kap, *K, 0.0])
+    soa = O.new_soa(1)
+    soa["x"][0], soa["z"][0], soa["dz"][0] = r, 100.0, -1.0
+    soa["lambda_"][0] = 400e-7
+    soa["status"][0] = A.RB_RUN
+    o = copy_soa(soa)
+    O.trace(flat, o, mgr.trace_opts())
+    assert o["status"][0] == A.RB_FOCUS and o["last_node"][0] == plane
+    d = A.rb_adjoint_desc()
+    d.mode = A.RB_ADJ_CENTROID
+    d.surface_param = table.ctypes.data
+    d.param_values = values.ctypes.data
+    d.n_params = n_params
+    d.z_param = n_params - 1
+    d.focal_node = plane
+    d.centroid_x, d.centroid_y, d.weight = o["x"][0] - 1.0, o["y"][0], 1.0
+    g, used = O.adjoint_dual(flat, soa, mgr.trace_opts(), d, n_params, 1)
+    assert used == 1
+
+    mp.mp.dps = 40
+    eps = mp.mpf("2e-6")
+
+    def hx(*args):
+        cc, kk = args[0], args[1]
+        Ks, LL = args[2:2 + len(K)], args[-1]
+        rr = mp.mpf(r)
+        w = rr * rr
+        sq = mp.sqrt(1 - w * cc * cc * kk)
+        Fv = w * cc / (1 + sq) + sum(Ks[i] * w ** (i + 1) for i in range(len(K)))
+        dF = rr * cc / sq + sum(2 * (i + 1) * Ks[i] * rr ** (2 * i + 1) for i in range(len(K)))
+        return rr - (LL - Fv - eps) * 2 * dF / (1 - dF * dF)
+
+    x0 = [mp.mpf(v) for v in (c, kap, *K, L)]
+    assert abs(float(hx(*x0)) - o["x"][0]) < 1e-9
+    for k in range(n_params):
+        order = [0] * n_params
+        order[k] = 1
+        want = float(mp.diff(hx, x0, tuple(order)))
+        assert g[k] == pytest.approx(want, rel=1e-8), (k, g[k], want)
+
+
+def test_dual_spot_against_central_fd():
+    """Dual gradient of the spot RMS^2 (2 angles x 20k rays) vs central FD of
+    the oracle-traced loss: 1e-4 relative on curvature, kappa, r^2k
+    coefficients of both mirrors and the focal offset. Every focused ray
+    contributes, those meeting the focal disk's rim included."""
+    srcs = _srcs(20000)
+    p0 = F.figure_params()
+    L0, stats, masks = _spot_loss(p0, srcs)
+    g, used = _dual(p0, srcs, [(A.RB_ADJ_CENTROID, cx, cy, 2.0 / n, 0.0, 0.0) for n, cx, cy in stats])
+    assert used == sum(n for n, _, _ in stats)
+    _check_fd(lambda p: _spot_loss(p, srcs)[::2], p0, g, masks)
+
+
+def test_dual_encircled_energy_against_central_fd():
+    """The smooth encircled-energy loss (r0 = 15 cm, width 2 cm, about fixed
+    target points): dual gradient vs central FD, 1e-4 relative."""
+    srcs = _srcs(20000)
+    p0 = F.figure_params()
+    _, stats, _ = _spot_loss(p0, srcs)
+    targets = [(cx, cy) for _, cx, cy in stats]
+    r0, w = 15.0, 2.0
+    L0, counts, masks = _ee_loss(p0, srcs, targets, r0, w)
+    assert -len(srcs) < L0 < 0
+    g, used = _dual(p0, srcs, [(A.RB_ADJ_ENCIRCLED, tx, ty, -1.0 / n, r0, w) for n, (tx, ty) in zip(counts, targets)])
+    assert used == sum(counts)
+    _check_fd(lambda p: _ee_loss(p, srcs, targets, r0, w)[::2], p0, g, masks)
+
+
+# ---------------------------------------------------------------- GPU
+
+
+def _gpu_model(ctx, n, loss="spot"):
+    return F.AsphericFigureFit(ctx, angles_deg=(0.0, 3.0), rays_per_angle=n, loss=loss, ee_radius=15.0, ee_width=2.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("loss", ["spot", "ee"])
+def test_gpu_figure_adjoint_matches_dual(ctx, loss):
+    """rbgpu_adjoint (FP64 aspheric chain) vs the dual reference on the same
+    rays: 1e-6 relative per parameter; the forward loss equals the oracle's."""
+    print(f"C3 figure adjoint ({loss}) vs dual, 2 angles x 20k rays", flush=True)
+    model = _gpu_model(ctx, 20000, loss)
+    p0 = F.figure_params()
+    L = model.loss(p0)
+    g = model.gradient()
+    srcs = [{k: v.cpu().numpy().copy() for k, v in s.items()} for s in model.src]
+    if loss == "spot":
+        L_o, stats, _ = _spot_loss(p0, srcs)
+        per = [(A.RB_ADJ_CENTROID, cx, cy, 2.0 / n, 0.0, 0.0) for n, cx, cy in model.stats]
+        assert [n for n, _, _ in stats] == [n for n, _, _ in model.stats]
+    else:
+        L_o, counts, _ = _ee_loss(p0, srcs, model.targets, 15.0, 2.0)
+        per = [(A.RB_ADJ_ENCIRCLED, tx, ty, -1.0 / n, 15.0, 2.0) for (n, _, _), (tx, ty) in
+               zip(model.stats, model.targets)]
+        assert model.last_adjoint_loss == pytest.approx(L, rel=1e-9)
+    assert L == pytest.approx(L_o, rel=1e-9)
+    gd, used = _dual(p0, srcs, per)
+    assert used == sum(n for n, _, _ in model.stats)
+    rel = np.abs(g - gd) / np.abs(gd)
+    print(f"  max relative |g - g_dual| per parameter {rel.max():.3e}", flush=True)
+    assert rel.max() <= 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_figure_adjoint_against_central_fd(ctx):
+    """Adjoint gradient vs central FD of the GPU FP64 forward spot loss at
+    2 angles x 200k rays: 1e-4 relative."""
+    print("C3 figure adjoint vs central FD, 2 angles x 200k rays", flush=True)
+    model = _gpu_model(ctx, 200000)
+    p0 = F.figure_params()
+    model.loss(p0)
+    g = model.gradient()
+    worst = 0.0
+    for k in FD_INDEX:
+        h = fd_step(p0, k)
+        pp, pm = p0.copy(), p0.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (model.loss(pp) - model.loss(pm)) / (2 * h)
+        rel = abs(g[k] - fd) / abs(fd)
+        worst = max(worst, rel)
+        assert rel <= 1e-4, (k, g[k], fd)
+    print(f"  worst relative error {worst:.3e}", flush=True)
+
+
+@pytest.mark.gpu
+def test_gpu_figure_adam_fit_reduces_spot(ctx):
+    """60 Adam steps on the adjoint gradient of the spot loss (4 angles x 50k
+    rays), moving the conics and r^2 .. r^8 terms of both mirrors and the
+    focal offset, shrink the summed spot RMS^2 of the seeded figure."""
+    print("C3 figure Adam 60 steps, 4 angles x 50k rays", flush=True)
+    model = F.AsphericFigureFit(ctx, rays_per_angle=50000)
+    p0 = F.figure_params()
+    mask = np.zeros(F.NPARAM, dtype=bool)
+    for o in (0, P):
+        mask[o + 1:o + 6] = True
+    mask[F.Z_PARAM] = True
+    p, hist = F.adam_fit(model, p0, steps=60, lr=0.1, mask=mask)
+    print(f"  loss {hist[0]:.3f} -> {hist[-1]:.3f} cm^2", flush=True)
+    assert hist[-1] < 0.5 * hist[0]
+    assert np.array_equal(p[~mask], p0[~mask])
