@@ -150,6 +150,7 @@ def config_sweep(ctx, torch, A, configs, make_rays_device, sizes=None, steps=3, 
                      "mixed_status_flips": int((a["status"] != b["status"]).sum()),
                      "mixed_focused_hit_rms_um": round(float(torch.sqrt(d2[both].mean())) * 1e4, 6),
                      "focused_fraction": round(float((a["status"] == A.RB_FOCUS).double().mean()), 6)}
+        print(f"bench: config {name} {out[name]['fp64_mrays_s']} Mrays/s", file=sys.stderr, flush=True)
     return out
 
 
@@ -273,6 +274,7 @@ def main():
     res_o = {f: v.clone() for f, v in work.items()}
     ms_per_step, kms, dms = run(args.precision, args.steps, args.warmup)
     value = N / (ms_per_step * 1e-3) / 1e6
+    print(f"bench: headline {value:.1f} Mrays/s, trace kernel {kms:.3f} ms", file=sys.stderr, flush=True)
     status = torch.bincount(work["status"], minlength=6).cpu().tolist()
     a, b = (work, res_o) if args.precision == "fp64" else (res_o, work)
     flips = int((a["status"] != b["status"]).sum())
@@ -316,6 +318,7 @@ def main():
             adj_ms.append(model.last_adjoint_ms)
         torch.cuda.synchronize()
         ms_eval = (time.perf_counter() - t0) * 1e3 / args.steps
+        print(f"bench: adjoint {ms_eval:.2f} ms per evaluation", file=sys.stderr, flush=True)
         adjoint = {"grad_evals_per_s": round(1e3 / ms_eval, 3), "ms_per_eval": round(ms_eval, 3),
                    "adjoint_device_ms": round(sum(adj_ms) / len(adj_ms), 3),
                    "form": "one adjoint launch over the 8 angles returning loss and gradient (moments form)",
