@@ -214,6 +214,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--order", choices=("auto", "input", "coherent"), default="auto",
+                    help="trace order of the headline batch (auto: coherent for >= 2^20 rays)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
@@ -246,7 +248,8 @@ def main():
 
     def run(precision, steps, warmup):
         mgr.SetPrecision(precision == "mixed")
-        opts = mgr.trace_opts(device_ptrs=1)
+        opts = mgr.trace_opts(device_ptrs=1, order={"auto": A.RB_ORDER_AUTO, "input": A.RB_ORDER_INPUT,
+                                                    "coherent": A.RB_ORDER_COHERENT}[args.order])
 
         for _ in range(warmup):
             for f in src:
@@ -349,7 +352,9 @@ def main():
         "config": {"workload": "Davies-Cotton 85 hexagonal facets f=16 m, 1e7 rays, stochastic reflectance "
                                "0.80-0.90, camera r=1.2 m + 2.5 m square obscuration",
                    "rays": N, "precision": args.precision, "limit": mgr.limit,
-                   "order": "coherent (auto for >= 2^20 rays): sorted by first daughter-box entry"},
+                   "order": {"auto": "coherent (auto for >= 2^20 rays): sorted by first daughter-box entry",
+                             "input": "input order", "coherent": "coherent: sorted by first daughter-box entry"}[
+                       args.order]},
         "kernel_ms": round(kms, 3),
         "kernel_mrays_s": round(N / (kms * 1e-3) / 1e6, 3),
         "device_ms": round(dms, 3),
