@@ -230,8 +230,12 @@ enum { RB_ORDER_AUTO = 0, RB_ORDER_INPUT = 1, RB_ORDER_COHERENT = 2 };
 /* Lane schedule. LANE: one lane per ray for the whole trace. PERSISTENT: a
  * grid sized to the machine whose lanes take the next ray (in trace order)
  * from a work counter as soon as theirs terminates, so a wavefront does not
- * wait for its longest path. AUTO picks PERSISTENT for scenes with refracting
- * volumes (multi-bounce paths of very unequal length) and LANE otherwise. */
+ * wait for its longest path. REGROUP: PERSISTENT with the block's live rays
+ * sorted by the kind of step they take next before every step, so that each
+ * wavefront runs one kind. AUTO picks REGROUP for scenes with refracting
+ * volumes (multi-bounce paths of very unequal length) and a volume with at
+ * least 256 daughters, PERSISTENT for other scenes with refracting volumes,
+ * and LANE otherwise. */
 enum { RB_SCHEDULE_AUTO = 0, RB_SCHEDULE_LANE = 1, RB_SCHEDULE_PERSISTENT = 2, RB_SCHEDULE_REGROUP = 3 };
 
 /* ---- synthetic ray sources (device-side ARayShooter equivalents) ---- */

@@ -41,7 +41,8 @@ struct DNode {                 // 128 B
   double rot[9];
   double tr[3];
   int32_t shape, kind, volume, parent;
-  int32_t first_child, n_children, bvh_root, pad;
+  int32_t first_child, n_children, bvh_root;
+  int32_t cell;                // this node's neighbour cell among its parent's daughters, or -1
 };
 struct DBvh {                  // 32 B, FP32 boxes expanded outward, depth-first order
   float lo[3];
@@ -65,6 +66,13 @@ struct DScene {
   const double* table_xy;
   const rb_border_desc* borders;
   const rb_grid2d_desc* grids;  // multilayer (wavelength, angle) tables
+  // Neighbour cells of the daughters of nodes with many daughters. cells[k]
+  // (DBvh layout): the cell box C (the daughter's box grown by a tenth of
+  // its extent per axis), escape = first neighbour record, prim = neighbour
+  // count (-1: too many, not used). cell_nbrs: one record per daughter whose
+  // box meets C (the daughter itself included): its box and prim = node id.
+  const DBvh* cells;
+  const DBvh* cell_nbrs;
   int32_t n_nodes;
   // BVH entries [stage_lo, stage_lo + stage_n) -- the top node's BVH, at
   // most kBvhStage entries -- are read from an LDS copy that each tracing

@@ -122,6 +122,80 @@ struct BvhBuilder {
   }
 };
 
+// Neighbour cells of node n's daughters. A daughter c's cell box C(c) is
+// its FP32 box grown by a tenth of its extent on each axis; its neighbours
+// are the daughters whose FP32 boxes meet C(c), c included. Any daughter
+// that contains a point of C(c), or is crossed by a ray inside C(c), has a
+// box meeting C(c) (boxes enclose their shapes), so the neighbours are a
+// complete candidate set there, whatever the BVH would have visited.
+// A cell with more than kCellMaxNbrs neighbours is marked unused.
+constexpr int kCellMinChildren = 256;
+constexpr int kCellMaxChildren = 1 << 20;
+constexpr int kCellMaxNbrs = 48;
+
+inline void build_cells(const rb_scene_desc* d, const rb_node_desc& n, std::vector<DNode>& nodes,
+                        std::vector<DBvh>& cells, std::vector<DBvh>& nbrs) {
+  const int m = n.n_children;
+  std::vector<DBvh> box(m), cbox(m);
+  for (int k = 0; k < m; k++) {
+    const int c = d->children[n.first_child + k];
+    const Box b = node_box(d, c);
+    DBvh e{};
+    for (int a = 0; a < 3; a++) {
+      e.lo[a] = down(b.lo[a]);
+      e.hi[a] = up(b.hi[a]);
+    }
+    e.prim = c;
+    e.escape = 0;
+    box[k] = e;
+    DBvh g = e;
+    for (int a = 0; a < 3; a++) {
+      const double grow = 0.1 * (double(e.hi[a]) - double(e.lo[a]));
+      g.lo[a] = down(double(e.lo[a]) - grow);
+      g.hi[a] = up(double(e.hi[a]) + grow);
+    }
+    cbox[k] = g;
+  }
+  // sweep on x: daughters ordered by box lo.x; the widest box bounds the scan
+  std::vector<int> order(m);
+  for (int k = 0; k < m; k++) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    return box[a].lo[0] < box[b].lo[0] || (box[a].lo[0] == box[b].lo[0] && a < b);
+  });
+  std::vector<float> lox(m);
+  float widest = 0.0f;
+  for (int k = 0; k < m; k++) {
+    lox[k] = box[order[k]].lo[0];
+    widest = std::max(widest, box[order[k]].hi[0] - box[order[k]].lo[0]);
+  }
+  std::vector<int> found;
+  for (int k = 0; k < m; k++) {
+    const DBvh& g = cbox[k];
+    found.clear();
+    const float from = std::nextafter(g.lo[0] - widest, -INFINITY);
+    auto it = std::lower_bound(lox.begin(), lox.end(), from);
+    for (size_t j = size_t(it - lox.begin()); j < size_t(m) && lox[j] <= g.hi[0]; j++) {
+      const DBvh& e = box[order[j]];
+      bool meet = true;
+      for (int a = 0; a < 3; a++) meet = meet && e.lo[a] <= g.hi[a] && g.lo[a] <= e.hi[a];
+      if (meet) found.push_back(order[j]);
+    }
+    DBvh cell = g;
+    const int c = box[k].prim;
+    if (int(found.size()) > kCellMaxNbrs) {
+      cell.escape = 0;
+      cell.prim = -1;
+    } else {
+      std::sort(found.begin(), found.end(), [&](int a, int b) { return box[a].prim < box[b].prim; });
+      cell.escape = int(nbrs.size());
+      cell.prim = int(found.size());
+      for (int j : found) nbrs.push_back(box[j]);
+    }
+    nodes[c].cell = int(cells.size());
+    cells.push_back(cell);
+  }
+}
+
 // A TGeoPgon the convex-frustum code takes: full phi and rmin = 0 throughout.
 inline bool pgon_is_simple(const double* p) {
   if (p[1] < 360.0) return false;
@@ -280,13 +354,14 @@ size_t put(std::vector<char>& blob, const X* src, size_t count) {
 
 // Offsets of the DScene arrays inside the packed blob.
 struct SceneLayout {
-  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy, borders, grids;
+  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy, borders, grids, cells, cell_nbrs;
   int32_t n_nodes;
   bool has_top_box = false;  // the top node has daughters: their bounding box follows
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
   int32_t top_bvh_lo = 0, top_bvh_n = 0;  // the top node's BVH entries
   int32_t max_depth = 0;                  // deepest boolean nesting of any shape
   bool has_lens = false;                  // a placed node is an ALens
+  bool has_cells = false;                 // some node's daughters have neighbour cells
   bool has_borders = false;               // border-surface conditions are defined
   bool has_ext = false;                   // a TGeoPcon, a general TGeoPgon, an AGeoWinstonCone2D or a
                                           // TGeoSphere with a phi range (code compiled into the deep
@@ -311,7 +386,7 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
     o.first_child = n.first_child;
     o.n_children = n.n_children;
     o.bvh_root = -1;
-    o.pad = 0;
+    o.cell = -1;
     if (o.kind == RB_KLENS) lay.has_lens = true;
     if (n.n_children > 0) {
       std::vector<std::pair<int, Box>> items;
@@ -321,6 +396,14 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
       }
       o.bvh_root = bvh.build(items);
     }
+  }
+  // neighbour cells (exact candidate sets for a point or a ray segment
+  // inside a daughter's cell box): built for nodes with many daughters
+  std::vector<DBvh> cells, nbrs;
+  for (int i = 0; i < d->n_nodes; i++) {
+    const rb_node_desc& n = d->nodes[i];
+    if (n.n_children < kCellMinChildren || n.n_children > kCellMaxChildren) continue;
+    build_cells(d, n, nodes, cells, nbrs);
   }
   std::vector<DShape> shapes(d->n_shapes);
   lay.max_depth = 0;
@@ -346,6 +429,9 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
   lay.table_xy = put(blob, d->table_xy, size_t(std::max(d->n_table_xy, 0)));
   lay.borders = put(blob, d->borders, size_t(std::max(d->n_borders, 0)));
   lay.grids = put(blob, d->grids, size_t(std::max(d->n_grids, 0)));
+  lay.cells = put(blob, cells.data(), cells.size());
+  lay.has_cells = !cells.empty();
+  lay.cell_nbrs = put(blob, nbrs.data(), nbrs.size());
   // the surface-extras kernel form also reads TH2 mirror reflectance
   lay.has_borders = d->n_borders > 0;
   for (int i = 0; i < d->n_volumes; i++)
@@ -377,6 +463,8 @@ inline DScene scene_view(const char* base, const SceneLayout& lay) {
   s.table_xy = reinterpret_cast<const double*>(base + lay.table_xy);
   s.borders = reinterpret_cast<const rb_border_desc*>(base + lay.borders);
   s.grids = reinterpret_cast<const rb_grid2d_desc*>(base + lay.grids);
+  s.cells = reinterpret_cast<const DBvh*>(base + lay.cells);
+  s.cell_nbrs = reinterpret_cast<const DBvh*>(base + lay.cell_nbrs);
   s.n_nodes = lay.n_nodes;
   s.stage_lo = 0;
   s.stage_n = 0;

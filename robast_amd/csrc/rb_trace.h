@@ -114,14 +114,39 @@ __device__ __forceinline__ bool box_has(const DBvh& b, float x, float y, float z
   return x >= b.lo[0] && x <= b.hi[0] && y >= b.lo[1] && y <= b.hi[1] && z >= b.lo[2] && z <= b.hi[2];
 }
 
-// First daughter of n (lowest node id, excluding skip) whose shape contains
-// p; stackless walk of n's BVH (depth-first order with escape links).
+// The neighbour cell of daughter `hint` of node n when the FP32 point (fx,
+// fy, fz) lies in its cell box (the cell's record), else a record with
+// prim < 0. The hint arrives from the ray's state: bounded before it is read.
 template <typename SC>
-__device__ int first_daughter_containing(const SC& S, int n, V3<double> p, int skip) {
+__device__ __forceinline__ DBvh cell_at(const SC& S, int n, int hint, float fx, float fy, float fz) {
+  DBvh none{};
+  none.prim = -1;
+  if (hint < 0 || hint >= S.n_nodes) return none;
+  const DNode& h = S.nodes[hint];
+  if (h.parent != n || h.cell < 0) return none;
+  const DBvh c = S.cells[h.cell];
+  return box_has(c, fx, fy, fz) ? c : none;
+}
+
+// First daughter of n (lowest node id, excluding skip) whose shape contains
+// p; stackless walk of n's BVH (depth-first order with escape links). When
+// p lies in the cell of daughter `hint`, only that cell's neighbours can
+// contain it (their boxes are the ones that meet the cell), and they are
+// tested instead of the walk: the same candidates, the same answer.
+template <typename SC>
+__device__ int first_daughter_containing(const SC& S, int n, V3<double> p, int skip, int hint = -1) {
   const DNode& nd = S.nodes[n];
   if (nd.n_children == 0) return -1;
   const float fx = float(p.x), fy = float(p.y), fz = float(p.z);
   int found = 0x7fffffff;
+  const DBvh cell = cell_at(S, n, hint, fx, fy, fz);
+  if (cell.prim >= 0) {
+    for (int k = 0; k < cell.prim; k++) {
+      const DBvh e = S.cell_nbrs[cell.escape + k];
+      if (e.prim != skip && e.prim < found && box_has(e, fx, fy, fz) && node_contains(S, e.prim, p)) found = e.prim;
+    }
+    return found == 0x7fffffff ? -1 : found;
+  }
   int i = nd.bvh_root;
   const int end = bvh_node(S, i).escape;
   while (i < end) {
@@ -140,8 +165,10 @@ __device__ int first_daughter_containing(const SC& S, int n, V3<double> p, int s
 #ifndef RB_SEARCH_FN
 #define RB_SEARCH_FN RB_NODE_FN
 #endif
+// hint: a daughter whose neighbour cell may hold p (the node the ray just
+// left or reflected on), used at the level of its parent.
 template <typename SC>
-__device__ RB_SEARCH_FN int search_node(const SC& S, int start, V3<double> p, int skip, bool forced) {
+__device__ RB_SEARCH_FN int search_node(const SC& S, int start, V3<double> p, int skip, bool forced, int hint = -1) {
   int n = start;
   bool inside = forced || n == skip;
   while (!inside) {
@@ -155,10 +182,11 @@ __device__ RB_SEARCH_FN int search_node(const SC& S, int start, V3<double> p, in
     }
   }
   for (int guard = 0; guard < 64; guard++) {
-    int c = first_daughter_containing(S, n, p, skip);
+    int c = first_daughter_containing(S, n, p, skip, hint);
     if (c < 0) return n;
     n = c;
     skip = -1;
+    hint = -1;
   }
   return n;
 }
@@ -250,6 +278,53 @@ __device__ __forceinline__ int gather_leaves(const DScene& S, int root, float ox
   return nc;
 }
 
+// Exit distance of the ray (p, d) from a cell box, or -1 when p is not
+// inside it (FP64 against the box's FP32 bounds).
+__device__ __forceinline__ double cell_exit(const DBvh& c, V3<double> p, V3<double> d) {
+  const double pp[3] = {p.x, p.y, p.z}, dd[3] = {d.x, d.y, d.z};
+  double t = kBig;
+  for (int a = 0; a < 3; a++) {
+    const double lo = double(c.lo[a]), hi = double(c.hi[a]);
+    if (!(pp[a] >= lo && pp[a] <= hi)) return -1.0;
+    if (dd[a] > 0.0) t = fmin(t, (hi - pp[a]) / dd[a]);
+    else if (dd[a] < 0.0) t = fmin(t, (lo - pp[a]) / dd[a]);
+  }
+  return t;
+}
+
+// gather_leaves over a neighbour cell's records instead of a BVH walk.
+template <typename SC>
+__device__ __forceinline__ int gather_cell(const SC& S, const DBvh& cell, float ox, float oy, float oz, float ix,
+                                           float iy, float iz, float lim, Cand cand, float* cut) {
+  int nc = 0;
+  float drop = 3.0e38f;
+  for (int j = 0; j < cell.prim; j++) {
+    const DBvh b = S.cell_nbrs[cell.escape + j];
+    float tn;
+    if (!box_pass(b, ox, oy, oz, ix, iy, iz, fminf(lim, drop), tn)) continue;
+    int k = nc < kCand ? nc : kCand - 1;
+    if (nc == kCand) {
+      const float last = cand.tn[(kCand - 1) * kCandStride];
+      if (tn >= last) {
+        drop = fminf(drop, tn);
+        continue;
+      }
+      drop = fminf(drop, last);
+    } else {
+      nc++;
+    }
+    while (k > 0 && cand.tn[(k - 1) * kCandStride] > tn) {
+      cand.tn[k * kCandStride] = cand.tn[(k - 1) * kCandStride];
+      cand.node[k * kCandStride] = cand.node[(k - 1) * kCandStride];
+      k--;
+    }
+    cand.tn[k * kCandStride] = tn;
+    cand.node[k * kCandStride] = b.prim;
+  }
+  *cut = drop;
+  return nc;
+}
+
 // FP64 ranking rule: a daughter must beat the best by the tolerance; within
 // the tolerance the lower node id wins, as the reference's loop over
 // daughters in index order decides.
@@ -298,6 +373,28 @@ __device__ void search_boundary_fp64(const SC& S, int cur, V3<double> p, V3<doub
   const float ix = 1.0f / float(d.x), iy = 1.0f / float(d.y), iz = 1.0f / float(d.z);
 #if RB_POSTPONE
   float cut;
+  if (hinted) {
+    // The hint's neighbour cell: a daughter crossed at a point of the cell
+    // box has a box that meets it, so while the best crossing lies inside
+    // the box (the ray, convexly, stays inside up to its exit distance) the
+    // cell's neighbours are a complete candidate set, and the walk is not
+    // needed. Otherwise the walk below ranks every daughter again.
+    const DBvh cell = cell_at(S, cur, hint, ox, oy, oz);
+    if (cell.prim >= 0) {
+      const double texit = cell_exit(cell, p, d);
+      if (texit > 0.0) {
+        const float lim = float(fmin(best, texit)) * 1.0001f + 1e-2f;
+        const int nl = gather_cell(S, cell, ox, oy, oz, ix, iy, iz, lim, cand, &cut);
+        for (int k = 0; k < nl; k++) {
+          if (cand.tn[k * kCandStride] > float(best) * 1.0001f + 1e-2f) break;
+          const int c = cand.node[k * kCandStride];
+          if (c == hint) continue;
+          rank_fp64(c, node_dist<double>(S, c, p, d, 0.0, true, best + kTol), best, crossed, hit);
+        }
+        if (best + 1e-6 * (1.0 + best) < texit && !(cut <= float(best) * 1.0001f + 1e-2f)) return;
+      }
+    }
+  }
   const int nc = gather_leaves(S, cn.bvh_root, ox, oy, oz, ix, iy, iz, float(best) * 1.0001f + 1e-2f, cand, &cut);
   for (int k = 0; k < nc; k++) {
     if (cand.tn[k * kCandStride] > float(best) * 1.0001f + 1e-2f) break;
@@ -450,9 +547,10 @@ __device__ bool search_boundary_mixed(const SC& S, int cur, V3<double> p, V3<dou
 
 // TGeoNavigator::FindNextBoundaryAndStep; returns the next node (-1 = null).
 // MIXED: the search runs in FP32 and the winning crossing is polished in FP64.
-// HINT: the FP64 search ranks the daughter of the last interaction first
-// (the result does not change; the work-queue kernels, whose rays bounce
-// many times inside light guides, compile it in).
+// HINT: the FP64 search ranks the daughter of the last interaction first,
+// and searches and relocates within the neighbour cells of the daughters it
+// leaves or reflects on (the results do not change; the work-queue kernels,
+// whose rays bounce many times inside light guides, compile it in).
 template <bool MIXED, bool HINT = false, typename SC>
 __device__ int next_boundary(const SC& S, Lane& L) {
   double extra = L.on_b ? kTol : 0.0;
@@ -509,7 +607,7 @@ __device__ int next_boundary(const SC& S, Lane& L) {
       return -1;
     }
     V3<double> pp = axpy(L.p, locate_push(S, cn.parent, L.step), L.d);
-    L.cur = search_node(S, cn.parent, pp, cur, false);
+    L.cur = search_node(S, cn.parent, pp, cur, false, HINT ? cur : -1);
     return L.cur;
   }
   L.hit_node = crossed;
@@ -709,8 +807,10 @@ __device__ __forceinline__ void set_direction(Lane& L, V3<double> v) {
   if (mag > 0) L.d = mk(v.x / mag, v.y / mag, v.z / mag);
 }
 
-// AOpticsManager::DoReflection (src/AOpticsManager.cxx:170-247)
-template <typename Rec, typename SC>
+// AOpticsManager::DoReflection (src/AOpticsManager.cxx:170-247). CELLS: the
+// relocation after the step back tests the neighbour cell of the node
+// reflected on (the work-queue kernels compile it in).
+template <bool CELLS = false, typename Rec, typename SC>
 __device__ void do_reflection(const SC& S, const TraceParams& P, const RayIO& io, int64_t i, uint64_t id,
                               Lane& L, double n1, int next, V3<double> n, Rec& rec, const rb_border_desc* bc) {
   double step = L.step;
@@ -755,7 +855,7 @@ __device__ void do_reflection(const SC& S, const TraceParams& P, const RayIO& io
   // TGeoNavigator::Step() of kEpsilon backwards moves kEpsilon + 1e-6, then relocates
   L.p = axpy(L.p, -(kEps + 1e-6), d1);
   L.cur = L.cur < 0 ? (node_contains(S, 0, L.p) ? search_node(S, 0, L.p, -1, true) : -1)
-                    : search_node(S, L.cur, L.p, -1, false);
+                    : search_node(S, L.cur, L.p, -1, false, CELLS ? next : -1);
   L.on_b = true;
   add_point(io, i, L, L.p, t, next);
 }
@@ -887,7 +987,7 @@ __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, in
       const rb_border_desc* bc = find_border(S, current, next);
       const V3<double> n = facet_normal(S, P, L, bc);
       if (mirror || do_fresnel(S, P, io, i, id, L, n1, n2, k2, next, n, rec, bc))
-        do_reflection(S, P, io, i, id, L, n1, next, n, rec, bc);
+        do_reflection<HINT>(S, P, io, i, id, L, n1, next, n, rec, bc);
     } else if ((c_free || c_lens) && (tn == RB_KOBS || tn == RB_KFOCUS)) {
       double n1 = c_lens ? lens_index(S, vol_of(S, current), lambda) : 1.;
       add_point(io, i, L, L.p, t1 + step / (kC / n1), next);

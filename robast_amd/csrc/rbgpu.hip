@@ -45,6 +45,7 @@ struct rb_scene {
   DScene s{};
   int max_depth = 0;  // deepest boolean nesting among the scene's shapes
   bool has_lens = false;  // a placed volume refracts (ALens)
+  bool has_cells = false; // some node's daughters have neighbour cells
   bool has_borders = false;  // border-surface conditions are defined
   bool has_ext = false;      // a TGeoPcon, general TGeoPgon or AGeoWinstonCone2D: traced with the deep kernel form
   bool has_top_box = false;
@@ -284,9 +285,11 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
 
 // Regrouping form: the persistent schedule, with the block's live rays
 // regrouped before every step so that each wavefront holds rays in the same
-// phase. Each lane's phase key is 0 (in a non-refracting volume: a navigation
-// search among daughters), 1 (inside a lens: its exit and the Fresnel
-// physics) or 2 (no ray). A counting sort over the block -- per-wave ballot
+// phase. Each lane's phase key is 0 (in a non-refracting volume, inside the
+// neighbour cell of the daughter it last met: a search among that cell's
+// neighbours), 1 (in a non-refracting volume otherwise: a search of the
+// whole BVH), 2 (inside a lens: its exit and the Fresnel physics) or 3 (no
+// ray). A counting sort over the block -- per-wave ballot
 // counts, then each lane's destination from the counts of the lower keys and
 // waves and its rank among its wave's lanes of its key -- moves every lane's
 // ray state through LDS to its new lane. Lanes then step their rays as in
@@ -301,7 +304,7 @@ struct RegroupLDS {
   int32_t cur[kTraceBlock], status[kTraceBlock], npoints[kTraceBlock], last_node[kTraceBlock];
   int32_t hit_node[kTraceBlock], flags[kTraceBlock];  // flags: bit 0 active, bit 1 on_b
   uint32_t draw[kTraceBlock], hit_leaf[kTraceBlock];
-  int32_t count[kTraceBlock / 64][3];
+  int32_t count[kTraceBlock / 64][4];
   int32_t exhausted;
 };
 
@@ -347,13 +350,22 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_regroup(DSc
     // the block leaves when none of its lanes holds a ray and the order is used up
     if (!__syncthreads_or(active ? 1 : 0) && G.exhausted) break;
     // counting sort of the block's lanes by phase key
-    const int key = !active ? 2 : kind_of(S, L.cur) == RB_KLENS ? 1 : 0;
-    const unsigned long long m0 = __ballot(key == 0), m1 = __ballot(key == 1);
-    const unsigned long long mine = key == 0 ? m0 : key == 1 ? m1 : ~(m0 | m1);
+    int key = 3;
+    if (active) {
+      if (kind_of(S, L.cur) == RB_KLENS) {
+        key = 2;
+      } else {
+        const DBvh cell = cell_at(S, L.cur, L.last_node, float(L.p.x), float(L.p.y), float(L.p.z));
+        key = cell.prim >= 0 ? 0 : 1;
+      }
+    }
+    const unsigned long long m0 = __ballot(key == 0), m1 = __ballot(key == 1), m2 = __ballot(key == 2);
+    const unsigned long long mine = key == 0 ? m0 : key == 1 ? m1 : key == 2 ? m2 : ~(m0 | m1 | m2);
     if ((threadIdx.x & 63) == 0) {
       G.count[wave][0] = __popcll(m0);
       G.count[wave][1] = __popcll(m1);
-      G.count[wave][2] = 64 - __popcll(m0) - __popcll(m1);
+      G.count[wave][2] = __popcll(m2);
+      G.count[wave][3] = 64 - __popcll(m0) - __popcll(m1) - __popcll(m2);
     }
     __syncthreads();
     int dest = int(__builtin_amdgcn_mbcnt_hi(unsigned(mine >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(mine), 0u)));
@@ -1029,7 +1041,8 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
                         (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
   const int schedule = opts->schedule != RB_SCHEDULE_AUTO ? opts->schedule
-                       : scene->has_lens ? RB_SCHEDULE_PERSISTENT : RB_SCHEDULE_LANE;
+                       : scene->has_lens ? (scene->has_cells ? RB_SCHEDULE_REGROUP : RB_SCHEDULE_PERSISTENT)
+                                         : RB_SCHEDULE_LANE;
   const bool persistent = schedule != RB_SCHEDULE_LANE;
   *coherent_out = coherent;
   const uint32_t* perm = nullptr;
@@ -1186,6 +1199,7 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
   sc->s.stage_n = std::min(lay.top_bvh_n, int32_t(kBvhStage));
   sc->max_depth = lay.max_depth;
   sc->has_lens = lay.has_lens;
+  sc->has_cells = lay.has_cells;
   sc->has_borders = lay.has_borders;
   sc->has_ext = lay.has_ext;
   sc->has_top_box = lay.has_top_box;
