@@ -296,31 +296,43 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
 // k_trace_persist; free lanes claim the next rays (one atomic per wave).
 // A ray is stepped by whichever lane holds it, always with its own Philox
 // stream (keyed by its index) and draw counter, so results equal k_trace's.
+// Threads per block of the regrouping form: the rays of one block are
+// regrouped together (more rays sort into more uniform wavefronts), and
+// every step waits at the block's barriers for its slowest wave. Each 256
+// lanes keep their own candidate slots.
+#ifndef RB_REGROUP_BLOCK
+#define RB_REGROUP_BLOCK 256
+#endif
+constexpr int kRegroupBlock = RB_REGROUP_BLOCK;
+static_assert(kRegroupBlock % 64 == 0 && (kRegroupBlock <= kCandStride || kRegroupBlock % kCandStride == 0),
+              "regroup block: whole waves, whole slot groups");
+constexpr int kRegroupSlots = (kRegroupBlock + kCandStride - 1) / kCandStride;
 struct RegroupLDS {
-  double px[kTraceBlock], py[kTraceBlock], pz[kTraceBlock];
-  double dx[kTraceBlock], dy[kTraceBlock], dz[kTraceBlock];
-  double t[kTraceBlock], lambda[kTraceBlock], step[kTraceBlock];
-  uint64_t ray[kTraceBlock];
-  int32_t cur[kTraceBlock], status[kTraceBlock], npoints[kTraceBlock], last_node[kTraceBlock];
-  int32_t hit_node[kTraceBlock], flags[kTraceBlock];  // flags: bit 0 active, bit 1 on_b
-  uint32_t draw[kTraceBlock], hit_leaf[kTraceBlock];
-  int32_t count[kTraceBlock / 64][4];
+  double px[kRegroupBlock], py[kRegroupBlock], pz[kRegroupBlock];
+  double dx[kRegroupBlock], dy[kRegroupBlock], dz[kRegroupBlock];
+  double t[kRegroupBlock], lambda[kRegroupBlock], step[kRegroupBlock];
+  uint64_t ray[kRegroupBlock];
+  int32_t cur[kRegroupBlock], status[kRegroupBlock], npoints[kRegroupBlock], last_node[kRegroupBlock];
+  int32_t hit_node[kRegroupBlock], flags[kRegroupBlock];  // flags: bit 0 active, bit 1 on_b
+  uint32_t draw[kRegroupBlock], hit_leaf[kRegroupBlock];
+  int32_t count[kRegroupBlock / 64][4];
   int32_t exhausted;
 };
 
 template <bool MIXED, int MAXD, bool BORDERS>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS> S, RayIO io,
+__global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS> S, RayIO io,
                                                                           int64_t n, TraceParams P,
                                                                           unsigned long long* next_ray,
                                                                           const uint32_t* perm) {
-  __shared__ int32_t cand_node[kCand * kCandStride];
-  __shared__ float cand_tn[kCand * kCandStride];
+  __shared__ int32_t cand_node[kRegroupSlots * kCand * kCandStride];
+  __shared__ float cand_tn[kRegroupSlots * kCand * kCandStride];
   __shared__ RegroupLDS G;
   stage_bvh(S);
   if (threadIdx.x == 0) G.exhausted = 0;
   __syncthreads();
   Lane L;
-  L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
+  const int slot = int(threadIdx.x) % kCandStride + int(threadIdx.x) / kCandStride * kCand * kCandStride;
+  L.cand = {cand_node + slot, cand_tn + slot};
   NoRecord rec;
   bool active = false;
   const int wave = int(threadIdx.x) >> 6;
@@ -369,7 +381,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_regroup(DSc
     }
     __syncthreads();
     int dest = int(__builtin_amdgcn_mbcnt_hi(unsigned(mine >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(mine), 0u)));
-    for (int w = 0; w < kTraceBlock / 64; w++) {
+    for (int w = 0; w < kRegroupBlock / 64; w++) {
       for (int k = 0; k < key; k++) dest += G.count[w][k];
       if (w < wave) dest += G.count[w][key];
     }
@@ -860,15 +872,16 @@ static int launch_trace(rb_ctx* ctx, const rb_scene* scene, int schedule, const 
     const bool regroup = schedule == RB_SCHEDULE_REGROUP;
     const void* fn = regroup ? (const void*)k_trace_regroup<MIXED, MAXD, BORDERS>
                              : (const void*)k_trace_persist<MIXED, MAXD, BORDERS>;
+    const int pblock = regroup ? kRegroupBlock : block;
     int per_cu = 0;
-    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
-    const int64_t pgrid = std::min<int64_t>(grid, int64_t(std::max(per_cu, 1)) * ctx->cus);
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, pblock, 0));
+    const int64_t pgrid = std::min<int64_t>((n + pblock - 1) / pblock, int64_t(std::max(per_cu, 1)) * ctx->cus);
     if (clear_by_kernel)
       hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(64), 0, ctx->stream, counter);
     else
       HIPCHK(ctx, hipMemsetAsync(counter, 0, sizeof(unsigned long long), ctx->stream));
     if (regroup)
-      hipLaunchKernelGGL((k_trace_regroup<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
+      hipLaunchKernelGGL((k_trace_regroup<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(pblock), 0, ctx->stream,
                          scene_of<MAXD, BORDERS>(scene), io, n, P, counter, perm);
     else
       hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
