@@ -73,6 +73,9 @@ struct DScene {
   // box meets C (the daughter itself included): its box and prim = node id.
   const DBvh* cells;
   const DBvh* cell_nbrs;
+  // per shape, its local bounding box grown by 1e-3 cm (+1e-6 relative):
+  // lo[3], hi[3]. A boolean node tests a child's box before the child.
+  const double* sbox;
   int32_t n_nodes;
   // BVH entries [stage_lo, stage_lo + stage_n) -- the top node's BVH, at
   // most kBvhStage entries -- are read from an LDS copy that each tracing
@@ -92,10 +95,15 @@ constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // AGeoWinstonCone2D code unless EXT (compiled into the deep and the
 // border-surface forms; a scene holding such a shape is traced with the deep
 // form).
-template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1 || BORDERS)> struct DSceneOf : DScene {
+// BOX: boolean nodes test a child's bounding box before evaluating the child
+// (compiled into the work-queue kernels: in the lane kernel the extra code
+// cost more in register spills than it saved).
+template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1 || BORDERS), bool BOX = false>
+struct DSceneOf : DScene {
   static constexpr int max_depth = MAXD;
   static constexpr bool borders_on = BORDERS;
   static constexpr bool ext_on = EXT;
+  static constexpr bool box_on = BOX;
 };
 
 template <typename T>
@@ -1482,22 +1490,58 @@ template <typename T, bool EXT> __device__ RB_PRIM_FN V3<T> prim_normal(int type
 template <typename T> __device__ __forceinline__ T bool_push() { return T(kBoolPush); }
 template <> __device__ __forceinline__ float bool_push<float>() { return 2e-4f; }
 
-template <typename T, int D, bool EXT> struct Csg {
+// A point in a shape's local box (lo[3], hi[3]; grown on upload), and a
+// ray that may enter it: false only when the ray (origin q, direction d,
+// t >= 0) certainly misses the box.
+template <typename T>
+__device__ __forceinline__ bool sbox_has(const double* b, V3<T> q) {
+  return q.x >= T(b[0]) && q.x <= T(b[3]) && q.y >= T(b[1]) && q.y <= T(b[4]) && q.z >= T(b[2]) && q.z <= T(b[5]);
+}
+template <typename T>
+__device__ __forceinline__ bool sbox_ray(const double* b, V3<T> q, V3<T> d) {
+  const T o[3] = {q.x, q.y, q.z}, v[3] = {d.x, d.y, d.z};
+  T t0 = T(0), t1 = T(kBig);
+  for (int a = 0; a < 3; a++) {
+    const T lo = T(b[a]), hi = T(b[3 + a]);
+    if (v[a] == T(0)) {
+      if (o[a] < lo || o[a] > hi) return false;
+    } else {
+      const T inv = T(1) / v[a];
+      T ta = (lo - o[a]) * inv, tb = (hi - o[a]) * inv;
+      if (ta > tb) { const T x = ta; ta = tb; tb = x; }
+      t0 = ta > t0 ? ta : t0;
+      t1 = tb < t1 ? tb : t1;
+    }
+  }
+  return t0 <= t1 * T(1.0001) + T(1e-3);
+}
+
+template <typename T, int D, bool EXT, bool BOX = false> struct Csg {
+  // child `side` (0 left, 1 right) contains q: outside its box it does not
+  __device__ __forceinline__ static bool child_in(const DScene& S, const double* p, int side, V3<T> q) {
+    const double* m = p + (side ? 15 : 3);
+    const int c = int(p[side ? 2 : 1]);
+    const V3<T> lq = m2l<T>(m, m + 9, q);
+    return (!BOX || sbox_has<T>(S.sbox + 6 * c, lq)) && Csg<T, D - 1, EXT, BOX>::contains(S, c, lq);
+  }
   __device__ RB_CSG_FN static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return prim_contains<T, EXT>(sh.type, p, q);
-    int op = int(p[0]), L = int(p[1]), R = int(p[2]);
-    bool a = Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, q));
-    if (op == RB_OP_UNION) return a || Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, q));
-    if (op == RB_OP_SUBTRACTION) return a && !Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, q));
-    return a && Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, q));
+    int op = int(p[0]);
+    bool a = child_in(S, p, 0, q);
+    if (op == RB_OP_UNION) return a || child_in(S, p, 1, q);
+    if (op == RB_OP_SUBTRACTION) return a && !child_in(S, p, 1, q);
+    return a && child_in(S, p, 1, q);
   }
+  // distance to child `side`; an entry (out) along a ray that misses the
+  // child's box is kBig without evaluating the child
   __device__ static Hit child(const DScene& S, const double* p, int side, bool out, V3<T> q, V3<T> d) {
     const double* m = p + (side ? 15 : 3);
     int c = int(p[side ? 2 : 1]);
     V3<T> lq = m2l<T>(m, m + 9, q), ld = m2lv<T>(m, d);
-    Hit h = out ? Csg<T, D - 1, EXT>::dist_out(S, c, lq, ld) : Csg<T, D - 1, EXT>::dist_in(S, c, lq, ld);
+    if (BOX && out && !sbox_ray<T>(S.sbox + 6 * c, lq, ld)) return {kBig, 0u, 0u};
+    Hit h = out ? Csg<T, D - 1, EXT, BOX>::dist_out(S, c, lq, ld) : Csg<T, D - 1, EXT, BOX>::dist_in(S, c, lq, ld);
     h.leaf = leaf_push(h.leaf, side);
     return h;
   }
@@ -1509,7 +1553,7 @@ template <typename T, int D, bool EXT> struct Csg {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T, EXT>(sh.type, p, q, d)), 0u, 0u};
-    int op = int(p[0]), L = int(p[1]), R = int(p[2]);
+    const int op = int(p[0]);
     if (op == RB_OP_UNION) {
       Hit a = child(S, p, 0, true, q, d), b = child(S, p, 1, true, q, d);
       return b.s < a.s ? b : a;
@@ -1523,17 +1567,17 @@ template <typename T, int D, bool EXT> struct Csg {
       Hit h;
       if (op == RB_OP_SUBTRACTION) {
         // inside B: leave B first; A is tested only outside B
-        const bool inB = Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
+        const bool inB = child_in(S, p, 1, xp);
         if (inB) {
           h = child(S, p, 1, false, x, d);
         } else {
-          if (Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, xp))) return {double(s), last.leaf, last.exiting};
+          if (child_in(S, p, 0, xp)) return {double(s), last.leaf, last.exiting};
           h = child(S, p, 0, true, x, d);
           if (h.s >= kBig) return {kBig, 0u, 0u};
         }
       } else {
-        const bool inA = Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
-        const bool inB = Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
+        const bool inA = child_in(S, p, 0, xp);
+        const bool inB = child_in(S, p, 1, xp);
         if (inA && inB) return {double(s), last.leaf, last.exiting};
         Hit ha = {0.0, 0u, 0u}, hb = {0.0, 0u, 0u};
         if (!inA) {
@@ -1558,7 +1602,7 @@ template <typename T, int D, bool EXT> struct Csg {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
     if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T, EXT>(sh.type, p, q, d)), 0u, 1u};
-    int op = int(p[0]), L = int(p[1]), R = int(p[2]);
+    const int op = int(p[0]);
     if (op == RB_OP_INTERSECTION) {
       Hit a = child(S, p, 0, false, q, d), b = child(S, p, 1, false, q, d);
       return b.s < a.s ? b : a;
@@ -1573,8 +1617,8 @@ template <typename T, int D, bool EXT> struct Csg {
     for (int it = 0; it < 64; it++) {
       V3<T> x = axpy(q, s, d);
       V3<T> xp = axpy(x, push, d);
-      bool inA = Csg<T, D - 1, EXT>::contains(S, L, m2l<T>(p + 3, p + 12, xp));
-      bool inB = Csg<T, D - 1, EXT>::contains(S, R, m2l<T>(p + 15, p + 24, xp));
+      bool inA = child_in(S, p, 0, xp);
+      bool inB = child_in(S, p, 1, xp);
       if (!inA && !inB) return {double(s), last.leaf, last.exiting};
       Hit ha = {0.0, 0u, 1u}, hb = {0.0, 0u, 1u};
       if (inA) ha = child(S, p, 0, false, x, d);
@@ -1593,7 +1637,7 @@ template <typename T, int D, bool EXT> struct Csg {
     if (sh.type != RB_SHAPE_COMPOSITE) return prim_normal<T, EXT>(sh.type, p, q, d);
     int side = depth > 0 ? int(path & 1u) : 0;
     const double* m = p + (side ? 15 : 3);
-    V3<T> n = Csg<T, D - 1, EXT>::normal(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0,
+    V3<T> n = Csg<T, D - 1, EXT, BOX>::normal(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0,
                                     m2l<T>(m, m + 9, q), m2lv<T>(m, d));
     return l2mv<T>(m, n);
   }
@@ -1612,12 +1656,12 @@ template <typename T, int D, bool EXT> struct Csg {
     }
     int side = depth > 0 ? int(path & 1u) : 0;
     const double* m = p + (side ? 15 : 3);
-    return Csg<T, D - 1, EXT>::polish(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0, exiting,
+    return Csg<T, D - 1, EXT, BOX>::polish(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0, exiting,
                                  m2l<T>(m, m + 9, q), m2lv<T>(m, d), s0, back);
   }
 };
 // depth exhausted: primitives only (the host rejects deeper expressions)
-template <typename T, bool EXT> struct Csg<T, 0, EXT> {
+template <typename T, bool EXT, bool BOX> struct Csg<T, 0, EXT, BOX> {
   __device__ RB_CSG0_FN static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
     return prim_contains<T, EXT>(sh.type, S.params + sh.off, q);
@@ -1653,35 +1697,35 @@ __device__ __forceinline__ V3<T> shape_normal(const SC& S, int shape, uint32_t l
   const uint32_t path = leaf & 0xffffffu;
   const int depth = int(leaf >> 24);
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0, SC::ext_on>::normal(S, shape, path, depth, q, d);
-    case 1: return Csg<T, 1, SC::ext_on>::normal(S, shape, path, depth, q, d);
+    case 0: return Csg<T, 0, SC::ext_on, SC::box_on>::normal(S, shape, path, depth, q, d);
+    case 1: return Csg<T, 1, SC::ext_on, SC::box_on>::normal(S, shape, path, depth, q, d);
     case 2:
-      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on>::normal(S, shape, path, depth, q, d);
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on, SC::box_on>::normal(S, shape, path, depth, q, d);
       break;
   }
-  return Csg<T, SC::max_depth, SC::ext_on>::normal(S, shape, path, depth, q, d);
+  return Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::normal(S, shape, path, depth, q, d);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ bool shape_contains(const SC& S, int shape, V3<T> q) {
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0, SC::ext_on>::contains(S, shape, q);
-    case 1: return Csg<T, 1, SC::ext_on>::contains(S, shape, q);
+    case 0: return Csg<T, 0, SC::ext_on, SC::box_on>::contains(S, shape, q);
+    case 1: return Csg<T, 1, SC::ext_on, SC::box_on>::contains(S, shape, q);
     case 2:
-      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on>::contains(S, shape, q);
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on, SC::box_on>::contains(S, shape, q);
       break;
   }
-  return Csg<T, SC::max_depth, SC::ext_on>::contains(S, shape, q);
+  return Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::contains(S, shape, q);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ Hit shape_dist(const SC& S, int shape, V3<T> q, V3<T> d, bool out, double lim = kBig) {
   switch (S.shapes[shape].depth) {
-    case 0: return out ? Csg<T, 0, SC::ext_on>::dist_out(S, shape, q, d) : Csg<T, 0, SC::ext_on>::dist_in(S, shape, q, d);
-    case 1: return out ? Csg<T, 1, SC::ext_on>::dist_out(S, shape, q, d, lim) : Csg<T, 1, SC::ext_on>::dist_in(S, shape, q, d);
+    case 0: return out ? Csg<T, 0, SC::ext_on, SC::box_on>::dist_out(S, shape, q, d) : Csg<T, 0, SC::ext_on, SC::box_on>::dist_in(S, shape, q, d);
+    case 1: return out ? Csg<T, 1, SC::ext_on, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, 1, SC::ext_on, SC::box_on>::dist_in(S, shape, q, d);
     case 2:
-      if constexpr (SC::max_depth >= 3) return out ? Csg<T, 2, SC::ext_on>::dist_out(S, shape, q, d, lim) : Csg<T, 2, SC::ext_on>::dist_in(S, shape, q, d);
+      if constexpr (SC::max_depth >= 3) return out ? Csg<T, 2, SC::ext_on, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, 2, SC::ext_on, SC::box_on>::dist_in(S, shape, q, d);
       break;
   }
-  return out ? Csg<T, SC::max_depth, SC::ext_on>::dist_out(S, shape, q, d, lim) : Csg<T, SC::max_depth, SC::ext_on>::dist_in(S, shape, q, d);
+  return out ? Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::dist_in(S, shape, q, d);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t leaf, bool exiting, V3<T> q,
@@ -1689,13 +1733,13 @@ __device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t 
   const uint32_t path = leaf & 0xffffffu;
   const int depth = int(leaf >> 24);
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
-    case 1: return Csg<T, 1, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+    case 0: return Csg<T, 0, SC::ext_on, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+    case 1: return Csg<T, 1, SC::ext_on, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
     case 2:
-      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
       break;
   }
-  return Csg<T, SC::max_depth, SC::ext_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+  return Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
 }
 
 // ------------------------------------------------------------ physics

@@ -354,7 +354,7 @@ size_t put(std::vector<char>& blob, const X* src, size_t count) {
 
 // Offsets of the DScene arrays inside the packed blob.
 struct SceneLayout {
-  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy, borders, grids, cells, cell_nbrs;
+  size_t nodes, children, bvh, shapes, params, volumes, tables, table_xy, borders, grids, cells, cell_nbrs, sbox;
   int32_t n_nodes;
   bool has_top_box = false;  // the top node has daughters: their bounding box follows
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
@@ -406,6 +406,16 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
     build_cells(d, n, nodes, cells, nbrs);
   }
   std::vector<DShape> shapes(d->n_shapes);
+  // each shape's local box, grown as node boxes are, for the boolean nodes'
+  // child tests
+  std::vector<double> sbox(size_t(std::max(d->n_shapes, 0)) * 6);
+  for (int i = 0; i < d->n_shapes; i++)
+    for (int k = 0; k < 3; k++) {
+      const double lo = d->shapes[i].bbox_origin[k] - d->shapes[i].bbox_half[k];
+      const double hi = d->shapes[i].bbox_origin[k] + d->shapes[i].bbox_half[k];
+      sbox[size_t(i) * 6 + k] = lo - (1e-3 + 1e-6 * std::fabs(lo));
+      sbox[size_t(i) * 6 + 3 + k] = hi + (1e-3 + 1e-6 * std::fabs(hi));
+    }
   lay.max_depth = 0;
   for (int i = 0; i < d->n_shapes; i++) {
     shapes[i] = DShape{d->shapes[i].type, d->shapes[i].param_offset, csg_depth(d, i, 0), 0};
@@ -432,6 +442,7 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
   lay.cells = put(blob, cells.data(), cells.size());
   lay.has_cells = !cells.empty();
   lay.cell_nbrs = put(blob, nbrs.data(), nbrs.size());
+  lay.sbox = put(blob, sbox.data(), sbox.size());
   // the surface-extras kernel form also reads TH2 mirror reflectance
   lay.has_borders = d->n_borders > 0;
   for (int i = 0; i < d->n_volumes; i++)
@@ -465,6 +476,7 @@ inline DScene scene_view(const char* base, const SceneLayout& lay) {
   s.grids = reinterpret_cast<const rb_grid2d_desc*>(base + lay.grids);
   s.cells = reinterpret_cast<const DBvh*>(base + lay.cells);
   s.cell_nbrs = reinterpret_cast<const DBvh*>(base + lay.cell_nbrs);
+  s.sbox = reinterpret_cast<const double*>(base + lay.sbox);
   s.n_nodes = lay.n_nodes;
   s.stage_lo = 0;
   s.stage_n = 0;

@@ -55,8 +55,9 @@ struct rb_scene {
 namespace {
 
 // The scene as the kernels compiled for composites at most MAXD deep take it.
-template <int MAXD, bool BORDERS = false> DSceneOf<MAXD, BORDERS> scene_of(const rb_scene* sc) {
-  DSceneOf<MAXD, BORDERS> s;
+template <int MAXD, bool BORDERS = false, bool BOX = false>
+DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), BOX> scene_of(const rb_scene* sc) {
+  DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), BOX> s;
   static_cast<DScene&>(s) = sc->s;
   return s;
 }
@@ -235,7 +236,7 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
 // it, with the same per-ray Philox stream, so results equal k_trace's. Every
 // wave leaves when the counter has passed n and none of its lanes holds a ray.
 template <bool MIXED, int MAXD, bool BORDERS>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSceneOf<MAXD, BORDERS> S, RayIO io,
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), true> S, RayIO io,
                                                                           int64_t n,
                                                                           TraceParams P,
                                                                           unsigned long long* next_ray,
@@ -320,7 +321,7 @@ struct RegroupLDS {
 };
 
 template <bool MIXED, int MAXD, bool BORDERS>
-__global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS> S, RayIO io,
+__global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), true> S, RayIO io,
                                                                           int64_t n, TraceParams P,
                                                                           unsigned long long* next_ray,
                                                                           const uint32_t* perm) {
@@ -882,10 +883,10 @@ static int launch_trace(rb_ctx* ctx, const rb_scene* scene, int schedule, const 
       HIPCHK(ctx, hipMemsetAsync(counter, 0, sizeof(unsigned long long), ctx->stream));
     if (regroup)
       hipLaunchKernelGGL((k_trace_regroup<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(pblock), 0, ctx->stream,
-                         scene_of<MAXD, BORDERS>(scene), io, n, P, counter, perm);
+                         scene_of<MAXD, BORDERS, true>(scene), io, n, P, counter, perm);
     else
       hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
-                         scene_of<MAXD, BORDERS>(scene), io, n, P, counter, perm);
+                         scene_of<MAXD, BORDERS, true>(scene), io, n, P, counter, perm);
   } else {
     hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<MAXD, BORDERS>(scene), io, n, P, perm);

@@ -27,11 +27,13 @@ struct Slots {
   std::vector<float> tn = std::vector<float>(size_t(kCand) * kCandStride);
 };
 
-// HINT: the FP64 search ranks the last daughter first, as the work-queue
-// kernels compile it (the library picks those for scenes with a lens)
+// HINT: the work-queue kernels' form (the library picks those for scenes
+// with a lens): the FP64 search ranks the last daughter first, searches and
+// relocates within neighbour cells, and boolean nodes test their children's
+// boxes first
 template <bool MIXED, int MAXD, bool BORDERS, bool HINT>
 void run(const DScene& base, const TraceParams& P, const RayIO& io, int64_t n) {
-  DSceneOf<MAXD, BORDERS> S;
+  DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), HINT> S;
   static_cast<DScene&>(S) = base;
   Slots slots;
   for (int64_t i = 0; i < n; i++) {
