@@ -6,7 +6,8 @@
 // Built by tests/test_emulation.py as a program: `emul_trace IN OUT` reads
 // the flattened scene, the options and the rays from IN and writes the
 // traced rays to OUT (the layout is in tests/test_emulation.py);
-// `emul_trace --oracle IN OUT` traces them with the oracle
+// `emul_trace --lane IN OUT` runs the lane kernel's form for scenes with a
+// lens too; `emul_trace --oracle IN OUT` traces them with the oracle
 // (oracle/rb_oracle.cpp, linked in), so the oracle runs under the same
 // sanitizers. Never part of librbgpu.
 #define RB_HOST_EMULATION 1
@@ -73,8 +74,10 @@ extern "C" {
 
 // Traces n host rays in place with the device code; returns 0, or the
 // scene packer's error code. *fallbacks counts the mixed form's reruns in
-// FP64.
-int emul_trace(const rb_scene_desc* d, const rb_trace_opts* o, rb_rays* r, int64_t n, int64_t* fallbacks) {
+// FP64. Scenes with a lens run the work-queue kernels' form unless
+// lane_form.
+int emul_trace(const rb_scene_desc* d, const rb_trace_opts* o, rb_rays* r, int64_t n, int64_t* fallbacks,
+               bool lane_form) {
   std::vector<char> blob;
   SceneLayout lay;
   std::string err;
@@ -99,7 +102,7 @@ int emul_trace(const rb_scene_desc* d, const rb_trace_opts* o, rb_rays* r, int64
   rb_emul_fallbacks = 0;
   const bool mixed = o->precision == RB_PRECISION_MIXED;
   const int form = lay.has_borders ? 2 : (lay.max_depth <= 1 && !lay.has_ext) ? 0 : 1;
-  const bool hint = lay.has_lens;
+  const bool hint = lay.has_lens && !lane_form;
   if (mixed)
     form == 0 ? (hint ? run<true, 1, false, true>(base, P, io, n) : run<true, 1, false, false>(base, P, io, n))
     : form == 1 ? (hint ? run<true, kMaxCsgDepth, false, true>(base, P, io, n)
@@ -146,11 +149,12 @@ struct Reader {
 
 int main(int argc, char** argv) {
   const bool use_oracle = argc == 4 && std::string(argv[1]) == "--oracle";
-  if (argc != 3 && !use_oracle) {
-    std::fprintf(stderr, "usage: emul_trace [--oracle] IN OUT\n");
+  const bool lane_form = argc == 4 && std::string(argv[1]) == "--lane";
+  if (argc != 3 && !use_oracle && !lane_form) {
+    std::fprintf(stderr, "usage: emul_trace [--oracle | --lane] IN OUT\n");
     return 2;
   }
-  if (use_oracle) argv++;
+  if (use_oracle || lane_form) argv++;
   FILE* in = std::fopen(argv[1], "rb");
   if (!in) return 2;
   Reader R{in};
@@ -201,7 +205,8 @@ int main(int argc, char** argv) {
   r.path_xyzt = r.path_capacity ? path_xyzt.data() : nullptr;
   r.path_node = r.path_capacity ? path_node.data() : nullptr;
   int64_t fallbacks = 0;
-  const int rc = use_oracle ? oracle_trace(&d, opts.data(), &r, n, 2) : emul_trace(&d, opts.data(), &r, n, &fallbacks);
+  const int rc = use_oracle ? oracle_trace(&d, opts.data(), &r, n, 2)
+                            : emul_trace(&d, opts.data(), &r, n, &fallbacks, lane_form);
   if (rc) return 4;
   FILE* out = std::fopen(argv[2], "wb");
   if (!out) return 2;

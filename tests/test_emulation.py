@@ -55,10 +55,11 @@ def _put(fh, a, count=None):
     fh.write(a.tobytes())
 
 
-def emul_trace(exe, flat, soa, opts, tmpdir, oracle=False):
+def emul_trace(exe, flat, soa, opts, tmpdir, oracle=False, lane=False):
     """Traces soa (host SoA dict, with optional path_xyzt / path_node) in
     place with the sanitized device code (with oracle: the sanitized
-    oracle); returns the mixed form's count of FP64 reruns."""
+    oracle; with lane: the lane kernel's form even for scenes with a lens);
+    returns the mixed form's count of FP64 reruns."""
     n = len(soa["x"])
     fin, fout = os.path.join(tmpdir, "in.bin"), os.path.join(tmpdir, "out.bin")
     with open(fin, "wb") as fh:
@@ -77,7 +78,8 @@ def emul_trace(exe, flat, soa, opts, tmpdir, oracle=False):
         _put(fh, path.reshape(-1) if path is not None else np.zeros(0))
         _put(fh, soa["path_node"].reshape(-1) if path is not None else np.zeros(0, dtype=np.int32))
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
-    r = subprocess.run([exe] + (["--oracle"] if oracle else []) + [fin, fout], capture_output=True, text=True,
+    r = subprocess.run([exe] + (["--oracle"] if oracle else ["--lane"] if lane else []) + [fin, fout],
+                       capture_output=True, text=True,
                        env=env, timeout=600)
     assert r.returncode == 0, f"emul_trace exited {r.returncode}:\n{r.stderr[-4000:]}"
     raw = open(fout, "rb").read()
@@ -192,3 +194,27 @@ def test_oracle_under_sanitizers_is_the_oracle(emul, tmp_path, name, n):
     O.trace(mgr.flat(), o, mgr.trace_opts(), nthreads=8)
     for k in o:
         assert np.array_equal(e[k], o[k]), k
+
+
+@pytest.mark.parametrize("name", ["c4", "c4_det", "r41"])
+def test_work_queue_form_equals_lane_form_under_sanitizers(emul, tmp_path, name):
+    """The work-queue kernels' form of the device code -- the last-daughter
+    hint, the searches and relocations within neighbour cells, the boolean
+    nodes' child-box tests -- gives the lane kernel's results bit for bit
+    (path record included), on C4 (2,055 daughters: cells built), its
+    deterministic variant, and a random scene with a lens and composites."""
+    if name == "r41":
+        from test_random_scenes import random_rays, random_scene
+        mgr, soa = random_scene(41, True), random_rays(41, 1500)
+    else:
+        mgr = configs.c4_scene(deterministic=name == "c4_det")
+        soa = _inputs("c4", mgr, 1500)
+    a, b = copy_soa(soa), copy_soa(soa)
+    for d in (a, b):
+        d["path_xyzt"] = np.zeros((len(d["x"]), 8, 4))
+        d["path_node"] = np.full((len(d["x"]), 8), -1, dtype=np.int32)
+    emul_trace(emul, mgr.flat(), a, mgr.trace_opts(), str(tmp_path))
+    emul_trace(emul, mgr.flat(), b, mgr.trace_opts(), str(tmp_path), lane=True)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), (name, k)
+    assert a["npoints"].max() >= 4
