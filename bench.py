@@ -329,6 +329,33 @@ def main():
                    "workload": "Davies-Cotton alignment: 170 tip/tilt + 85 radii + camera z, loss = sum of "
                                "spot RMS^2 over 8 angles 0-3.5 deg, unit reflectance, FP64 forward, "
                                "adjoint FP32 per ray with FP64 accumulation"}
+        del model
+        torch.cuda.empty_cache()
+        # the aspheric figure fit: forward traces (spot RMS^2 per angle) and
+        # the adjoint through both AGeoAsphericDisk mirrors and the curved
+        # focal surface
+        from robast_amd import aspheric_fit as AF
+        fig = AF.AsphericFigureFit(ctx, rays_per_angle=N // len(AF.FIT_ANGLES_DEG))
+        q0 = AF.figure_params()
+        fig.loss_and_gradient(q0)            # warm-up evaluation (sets the reference centroids)
+        torch.cuda.synchronize()
+        fig_ms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fig.loss_and_gradient(q0)
+            fig_ms.append(fig.last_adjoint_ms)
+        torch.cuda.synchronize()
+        ms_fig = (time.perf_counter() - t0) * 1e3 / args.steps
+        print(f"bench: figure adjoint {ms_fig:.2f} ms per evaluation", file=sys.stderr, flush=True)
+        adjoint["figure"] = {"grad_evals_per_s": round(1e3 / ms_fig, 3), "ms_per_eval": round(ms_fig, 3),
+                             "adjoint_device_ms": round(sum(fig_ms) / len(fig_ms), 3),
+                             "rays": fig.n * len(fig.angles), "params": AF.NPARAM,
+                             "workload": "Schwarzschild-Couder figure: curvature, kappa and r^2..r^16 "
+                                         "coefficients of both aspheric mirrors + focal offset, loss = sum "
+                                         "of spot RMS^2 over 4 angles 0-4.5 deg; one moments-form adjoint "
+                                         "launch over the 4 angles returning loss and gradient (FP64 chain)"}
+        del fig
+        torch.cuda.empty_cache()
 
     gpu_configs = {}
     per_config = None if args.no_configs else config_sweep(ctx, torch, A, configs, make_rays_device,

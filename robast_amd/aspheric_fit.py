@@ -22,7 +22,8 @@ Losses, per incidence angle k with N_k focused rays and hits h:
           point t_k (the first evaluation's centroid),
           -(1/N_k) sum sigmoid((r0^2 - |h - t_k|^2) / (2 r0 w)),
 summed over the angles. `FigureLoss` wraps either as a
-torch.autograd.Function.
+torch.autograd.Function; the spot loss then takes one adjoint launch per
+evaluation (the moments form), which also returns the loss.
 """
 import math
 
@@ -119,20 +120,25 @@ class AsphericFigureFit:
         self.kind = loss
         self.ee_radius, self.ee_width = float(ee_radius), float(ee_width)
         self.dev = torch.device("cuda")
+        # all angles' rays in one buffer (angle k at [k n, (k + 1) n)), so one
+        # adjoint launch can cover them; self.src[k] are views of angle k
+        self.all = self._alloc(len(self.angles) * self.n)
         self.src = []
         for k, a in enumerate(self.angles):
-            t = self._alloc()
+            t = {f: v[k * self.n:(k + 1) * self.n] for f, v in self.all.items()}
             self.ctx.generate(raygen(a, k), make_rays_device(t), self.n)
             self.src.append(t)
         self.work = self._alloc()
         self.targets = None
         self.table = None
+        self.c0 = None
         self.last_adjoint_ms = 0.0
 
-    def _alloc(self):
-        t = {f: torch.empty(self.n, dtype=torch.float64, device=self.dev) for f in _FIELDS_F64}
+    def _alloc(self, n=None):
+        n = self.n if n is None else n
+        t = {f: torch.empty(n, dtype=torch.float64, device=self.dev) for f in _FIELDS_F64}
         for f in _FIELDS_I32:
-            t[f] = torch.empty(self.n, dtype=torch.int32, device=self.dev)
+            t[f] = torch.empty(n, dtype=torch.int32, device=self.dev)
         return t
 
     def set_params(self, params):
@@ -182,6 +188,42 @@ class AsphericFigureFit:
                 total -= float(torch.sigmoid(u).mean())
         return total
 
+    def loss_and_gradient(self, params):
+        """Spot loss and gradient from one adjoint launch over all angles (no
+        forward trace): the moments form of rbgpu_adjoint, one group of sums
+        per angle about the previous evaluation's centroid, the centroid
+        terms formed here in FP64. The "ee" loss takes loss() and gradient()."""
+        if self.kind != "spot":
+            return self.loss(params), self.gradient()
+        if self.c0 is None:
+            self.loss(params)
+            self.c0 = [(cx, cy) for _, cx, cy in self.stats]
+        self.set_params(params)
+        P, G = NPARAM, len(self.src)
+        acc = torch.zeros((G, 3 * P + 4), dtype=torch.float64, device=self.dev)
+        ref = torch.tensor(self.c0, dtype=torch.float64, device=self.dev)
+        d = figure_desc(self.table, self.values, self.focal_node, A.RB_ADJ_MOMENTS, 0.0, 0.0, 0.0)
+        d.ref_xy = ref.data_ptr()
+        d.group_size = self.n
+        d.n_groups = G
+        self.ctx.adjoint(self.scene, make_rays_device(self.all), G * self.n, self.mgr.trace_opts(device_ptrs=1), d,
+                         acc.data_ptr())
+        self.last_adjoint_ms = self.ctx.last_trace_ms()
+        a = acc.cpu().numpy()
+        loss, grad, stats = 0.0, np.zeros(P), []
+        for k in range(G):
+            n, sx, sy, s2 = a[k, 3 * P:]
+            if n == 0:
+                stats.append((0, 0.0, 0.0))
+                continue
+            ux, uy = sx / n, sy / n
+            loss += s2 / n - ux * ux - uy * uy
+            grad += 2.0 / n * (a[k, :P] - ux * a[k, P:2 * P] - uy * a[k, 2 * P:3 * P])
+            stats.append((int(n), self.c0[k][0] + ux, self.c0[k][1] + uy))
+        self.stats = stats
+        self.c0 = [(cx, cy) if cnt else c for (cnt, cx, cy), c in zip(stats, self.c0)]
+        return loss, grad
+
     def gradient(self):
         """dL/dparams at the parameters of the last loss() call."""
         opts = self.mgr.trace_opts(device_ptrs=1)
@@ -209,8 +251,8 @@ class FigureLoss(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, params, model):
-        loss = model.loss(params.detach().cpu().numpy())
-        ctx.grad = torch.from_numpy(model.gradient()).to(params.device)
+        loss, grad = model.loss_and_gradient(params.detach().cpu().numpy())
+        ctx.grad = torch.from_numpy(grad).to(params.device)
         return torch.tensor(loss, dtype=torch.float64, device=params.device)
 
     @staticmethod

@@ -279,3 +279,25 @@ def test_gpu_figure_adam_fit_reduces_spot(ctx):
     print(f"  loss {hist[0]:.3f} -> {hist[-1]:.3f} cm^2", flush=True)
     assert hist[-1] < 0.5 * hist[0]
     assert np.array_equal(p[~mask], p0[~mask])
+
+
+@pytest.mark.gpu
+def test_gpu_figure_moments_form_matches_centroid_form(ctx):
+    """The one-launch moments form of the figure fit (loss and gradient from
+    rbgpu_adjoint alone) equals the forward loss and the centroid-form
+    gradient, at the seeded figure and at moved parameters."""
+    print("C3 figure moments form vs forward + centroid form, 4 angles x 50k rays", flush=True)
+    model = F.AsphericFigureFit(ctx, rays_per_angle=50000)
+    p0 = F.figure_params()
+    loss_f = model.loss(p0)
+    g_c = model.gradient()
+    loss_m, g_m = model.loss_and_gradient(p0)
+    assert loss_m == pytest.approx(loss_f, rel=1e-10)
+    rel = np.abs(g_m - g_c) / np.abs(g_c)
+    print(f"  loss {loss_m:.6f} vs {loss_f:.6f}, max relative gradient difference {rel.max():.2e}", flush=True)
+    assert rel.max() <= 1e-8
+    p1 = p0 + 0.5 * F.param_scale(p0)
+    loss1, g1 = model.loss_and_gradient(p1)
+    assert loss1 == pytest.approx(model.loss(p1), rel=1e-10)
+    g1c = model.gradient()
+    assert (np.abs(g1 - g1c) / np.abs(g1c)).max() <= 1e-8
