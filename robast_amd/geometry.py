@@ -409,13 +409,23 @@ class AGeoAsphericDisk(TGeoShape):
         return [self.z1, self.c1, self.z2, self.c2, self.rmax, self.rmin, self.conic1 + 1, self.conic2 + 1,
                 float(len(self.k1)), float(len(self.k2)), *self.k1, *self.k2, *kn1, *kn2]
 
+    def _sag_samples(self, which, rs):
+        """_sag over an array of radii, the points where it is undefined
+        dropped."""
+        z, c, kap, ks = ((self.z1, self.c1, self.conic1 + 1, self.k1) if which == 1 else
+                         (self.z2, self.c2, self.conic2 + 1, self.k2))
+        rs = rs[1 - rs * rs * c * c * kap >= 0]
+        v = z + rs * rs * c / (1 + np.sqrt(1 - rs * rs * c * c * kap))
+        for i, k in enumerate(ks):
+            v = v + k * rs ** (2 * (i + 1))
+        return v
+
     def local_bbox(self):
         rs = np.linspace(self.rmin, self.rmax, 4001)
-        f1 = [self._sag(1, r) for r in rs]
-        f2 = [self._sag(2, r) for r in rs]
-        f1 = [v for v in f1 if v is not None] or [self.z1]
-        f2 = [v for v in f2 if v is not None] or [self.z2]
-        zmin, zmax = min(f1), max(f2)
+        f1 = self._sag_samples(1, rs)
+        f2 = self._sag_samples(2, rs)
+        zmin = float(f1.min()) if f1.size else self.z1
+        zmax = float(f2.max()) if f2.size else self.z2
         pad = 1e-3 * (zmax - zmin) + 0.1
         zmin, zmax = zmin - pad, zmax + pad
         return np.array([0, 0, 0.5 * (zmin + zmax)]), np.array([self.rmax, self.rmax, 0.5 * (zmax - zmin)])
