@@ -33,6 +33,8 @@ struct rb_ctx {
   size_t order_cap = 0;
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
+  void* rec_buf = nullptr;  // the rays' staged copy (RayRec, one line per ray), grown on demand
+  size_t rec_cap = 0;
   void* scratch = nullptr;  // this device's copy of the allocation-free entry point's scratch array
   hipError_t scratch_err = hipSuccess;
   int launches = 0;
@@ -111,6 +113,17 @@ constexpr int kTraceBlock = kCandStride;
 #define RB_TRACE_ATTR
 #endif
 
+// A ray's input state in one 128-byte line. Traced in coherence order, the
+// rays of a wavefront lie far apart in the input, so each of the eleven
+// state arrays costs the lane a line of its own; the ordering kernel, which
+// reads the rays in input order anyway, writes this copy so that the trace
+// reads one line per ray instead.
+struct alignas(128) RayRec {
+  double x, y, z, t, dx, dy, dz, lambda;
+  int32_t status, npoints, last_node, pad;
+};
+static_assert(sizeof(RayRec) == 128, "one line per ray");
+
 __device__ __forceinline__ void load_lane(const RayIO& io, int64_t i, Lane& L) {
   L.p = mk(io.x[i], io.y[i], io.z[i]);
   L.d = mk(io.dx[i], io.dy[i], io.dz[i]);
@@ -123,6 +136,29 @@ __device__ __forceinline__ void load_lane(const RayIO& io, int64_t i, Lane& L) {
   L.step = 0;
   L.hit_node = -1;
   L.hit_leaf = 0;
+}
+// Whether ray i is running and, if it is, its state into L: from its line of
+// the staged copy when there is one, else from the state arrays.
+__device__ __forceinline__ bool fetch_ray(const RayIO& io, const RayRec* rr, int64_t i, Lane& L) {
+  if (rr) {
+    const RayRec& r = rr[i];
+    if (r.status != RB_RUN) return false;
+    L.p = mk(r.x, r.y, r.z);
+    L.d = mk(r.dx, r.dy, r.dz);
+    L.t = r.t;
+    L.lambda = r.lambda;
+    L.status = RB_RUN;
+    L.npoints = r.npoints;
+    L.last_node = r.last_node;
+    L.draw = 0;
+    L.step = 0;
+    L.hit_node = -1;
+    L.hit_leaf = 0;
+    return true;
+  }
+  if (io.status[i] != RB_RUN) return false;
+  load_lane(io, i, L);
+  return true;
 }
 // The index handed back through an empty asm: the store addresses are then
 // formed again after the trace instead of being kept (and spilled) across it
@@ -166,20 +202,27 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
   v = (v | (v << 2)) & 0x09249249u;
   return v;
 }
-// Rays are reordered only within tiles of 2^kOrderTileBits consecutive
-// rays: the tile index forms the key's high bits. A tile's ray state (40 MB
-// at 2^19 rays) stays in the MALL while its permuted loads and stores run,
-// and a tile still holds enough rays for coherent wavefronts. Measured on the
+// Where the trace reads the rays' state arrays through the permutation (the
+// allocation-free entry point, which has no room for the staged copy), rays
+// are reordered only within tiles of 2^kOrderTileBits consecutive rays: the
+// tile index forms the key's high bits. A tile's ray state (40 MB at 2^19
+// rays) stays in the MALL while its permuted loads and stores run, and a
+// tile still holds enough rays for coherent wavefronts. Measured on the
 // 1e7-ray Davies-Cotton (FP64, ordering included): input order 14.8 ms;
 // global reorder 13.0; tiles of 2^15 / 2^17 / 2^19 / 2^21 rays 14.8 / 13.0 /
 // 12.1 / 12.3. Moving the ray state into sorted order and back with
 // separate gather/scatter passes instead of permuted loads in the trace
 // kernel cost 14.2-14.8 ms: the trace gets faster but the two passes of
 // scattered 8-byte accesses cost more than they save.
+// With the staged copy (RayRec, written by the key kernel in input order,
+// one line read per ray) the order is global (kOrderGlobalBits): 1e7-ray
+// Davies-Cotton 7.54 -> 6.82 ms against 2^19-ray tiles without the copy,
+// 7.52 with the copy and 2^19-ray tiles, 6.93 with 2^21.
 #ifndef RB_ORDER_TILE_BITS
 #define RB_ORDER_TILE_BITS 19
 #endif
 constexpr int kOrderTileBits = RB_ORDER_TILE_BITS;
+constexpr int kOrderGlobalBits = 31;  // one tile: batches are below 2^31 rays
 // Keys are 32-bit: the tile index in the high bits and the top
 // (32 - tile bits) bits of the 30-bit Morton code below it (all 30 up to
 // 4 tiles; 27 = 9 bits per axis at 1e7 rays), so that the radix sort makes
@@ -189,17 +232,33 @@ constexpr int kOrderTileBits = RB_ORDER_TILE_BITS;
 // and LDS answers them at a fraction of the L2 latency. Blocks stride over
 // the rays so that the staging is paid once per block.
 __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n, float lox, float loy, float loz,
-                                                   float sx, float sy, float sz, int mbits, uint32_t* key,
-                                                   uint32_t* idx) {
+                                                   float sx, float sy, float sz, int tile_bits, int mbits,
+                                                   uint32_t* key, uint32_t* idx, RayRec* rr) {
   stage_bvh(S);
   const int root = S.nodes[0].bvh_root;
   const int end = bvh_node(S, root).escape;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     uint32_t k = 0x3fffffffu;  // not running or no daughter box crossed: the last cell
-    if (io.status[i] == RB_RUN) {
-      const float ox = float(io.x[i]), oy = float(io.y[i]), oz = float(io.z[i]);
-      const float dx = float(io.dx[i]), dy = float(io.dy[i]), dz = float(io.dz[i]);
+    const int32_t status = io.status[i];
+    const double x = io.x[i], y = io.y[i], z = io.z[i], ddx = io.dx[i], ddy = io.dy[i], ddz = io.dz[i];
+    if (rr) {
+      // the whole line from registers, in 16-byte stores: no line is left
+      // partly written
+      const double t = io.t[i], lambda = io.lambda[i];
+      const int32_t npoints = io.npoints[i], last_node = io.last_node ? io.last_node[i] : -1;
+      double2* o = reinterpret_cast<double2*>(rr + i);
+      o[0] = make_double2(x, y);
+      o[1] = make_double2(z, t);
+      o[2] = make_double2(ddx, ddy);
+      o[3] = make_double2(ddz, lambda);
+      int4* q = reinterpret_cast<int4*>(o + 4);
+      q[0] = make_int4(status, npoints, last_node, 0);
+      q[1] = q[2] = q[3] = make_int4(0, 0, 0, 0);
+    }
+    if (status == RB_RUN) {
+      const float ox = float(x), oy = float(y), oz = float(z);
+      const float dx = float(ddx), dy = float(ddy), dz = float(ddz);
       const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
       float best = 3.0e38f;
       int b = root;
@@ -222,7 +281,7 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
       }
     }
     // tile index above the top mbits (<= 30) Morton bits
-    key[i] = (uint32_t(i >> kOrderTileBits) << mbits) | (k >> (30 - mbits));
+    key[i] = (uint32_t(i >> tile_bits) << mbits) | (k >> (30 - mbits));
     idx[i] = uint32_t(i);
   }
 }
@@ -240,7 +299,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
                                                                           int64_t n,
                                                                           TraceParams P,
                                                                           unsigned long long* next_ray,
-                                                                          const uint32_t* perm) {
+                                                                          const uint32_t* perm, const RayRec* rr) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   stage_bvh(S);
@@ -262,8 +321,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
         const int64_t c = int64_t((static_cast<unsigned long long>(hi) << 32) | lo) + rank;
         if (c >= n) {
           exhausted = true;
-        } else if (i = perm ? int64_t(perm[c]) : c; io.status[i] == RB_RUN) {
-          load_lane(io, i, L);
+        } else if (i = perm ? int64_t(perm[c]) : c; fetch_ray(io, rr, i, L)) {
           L.ray = P.id_base + uint64_t(i);
           begin_lane(S, L);
           active = true;
@@ -324,7 +382,7 @@ template <bool MIXED, int MAXD, bool BORDERS>
 __global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), true> S, RayIO io,
                                                                           int64_t n, TraceParams P,
                                                                           unsigned long long* next_ray,
-                                                                          const uint32_t* perm) {
+                                                                          const uint32_t* perm, const RayRec* rr) {
   __shared__ int32_t cand_node[kRegroupSlots * kCand * kCandStride];
   __shared__ float cand_tn[kRegroupSlots * kCand * kCandStride];
   __shared__ RegroupLDS G;
@@ -352,8 +410,7 @@ __global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(D
         int64_t i;
         if (c >= n) {
           G.exhausted = 1;
-        } else if (i = perm ? int64_t(perm[c]) : c; io.status[i] == RB_RUN) {
-          load_lane(io, i, L);
+        } else if (i = perm ? int64_t(perm[c]) : c; fetch_ray(io, rr, i, L)) {
           L.ray = P.id_base + uint64_t(i);
           begin_lane(S, L);
           active = true;
@@ -420,17 +477,17 @@ __global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(D
 // order); the ray keeps its own index for its Philox stream and its slots.
 template <bool MIXED, int MAXD, bool BORDERS>
 __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD, BORDERS> S, RayIO io, int64_t n,
-                                                                  TraceParams P, const uint32_t* perm) {
+                                                                  TraceParams P, const uint32_t* perm,
+                                                                  const RayRec* rr) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   stage_bvh(S);
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int64_t i = perm ? int64_t(perm[t]) : t;
-  if (io.status[i] != RB_RUN) return;
   Lane L;
+  if (!fetch_ray(io, rr, i, L)) return;
   L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
-  load_lane(io, i, L);
   L.ray = P.id_base + uint64_t(i);
   NoRecord rec;
   trace_lane<MIXED>(S, P, io, i, L, rec);
@@ -443,7 +500,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MA
 template <bool MIXED, int MAXD>
 __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DSceneOf<MAXD> S, RayIO io, int64_t n,
                                                                     TraceParams P, AdjParams A, double* grad,
-                                                                    const uint32_t* perm) {
+                                                                    const uint32_t* perm, const RayRec* rr) {
   __shared__ double acc[kMaxAdjAcc];
   const int nacc = adj_acc_size(A.mode, A.n_params, A.n_groups);
   __shared__ int32_t cand_node[kCand * kCandStride];
@@ -452,20 +509,10 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DSceneOf<
   stage_bvh(S);
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t i = t < n ? (perm ? int64_t(perm[t]) : t) : n;
-  if (i < n && io.status[i] == RB_RUN) {
-    Lane L;
+  Lane L;
+  if (i < n && fetch_ray(io, rr, i, L)) {
     L.cand = {cand_node + threadIdx.x, cand_tn + threadIdx.x};
-    L.p = mk(io.x[i], io.y[i], io.z[i]);
-    L.d = mk(io.dx[i], io.dy[i], io.dz[i]);
-    L.t = io.t[i];
-    L.lambda = io.lambda[i];
-    L.status = RB_RUN;
-    L.npoints = io.npoints[i];
-    L.last_node = -1;
-    L.draw = 0;
-    L.step = 0;
-    L.hit_node = -1;
-    L.hit_leaf = 0;
+    L.last_node = -1;  // the recomputed chain starts without a hint
     L.ray = P.id_base + uint64_t(i);
     const V3<double> origin = L.p, d0 = L.d;
     ChainRecord rec;
@@ -866,7 +913,8 @@ __global__ void k_clear_counter(unsigned long long* c) {
 // The trace kernel for the scene's depth and the lane schedule.
 template <bool MIXED, int MAXD, bool BORDERS>
 static int launch_trace(rb_ctx* ctx, const rb_scene* scene, int schedule, const RayIO& io, int64_t n,
-                        const TraceParams& P, const uint32_t* perm, unsigned long long* counter, bool clear_by_kernel) {
+                        const TraceParams& P, const uint32_t* perm, const RayRec* rr, unsigned long long* counter,
+                        bool clear_by_kernel) {
   const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
   if (schedule != RB_SCHEDULE_LANE) {
@@ -883,13 +931,13 @@ static int launch_trace(rb_ctx* ctx, const rb_scene* scene, int schedule, const 
       HIPCHK(ctx, hipMemsetAsync(counter, 0, sizeof(unsigned long long), ctx->stream));
     if (regroup)
       hipLaunchKernelGGL((k_trace_regroup<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(pblock), 0, ctx->stream,
-                         scene_of<MAXD, BORDERS, true>(scene), io, n, P, counter, perm);
+                         scene_of<MAXD, BORDERS, true>(scene), io, n, P, counter, perm, rr);
     else
       hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
-                         scene_of<MAXD, BORDERS, true>(scene), io, n, P, counter, perm);
+                         scene_of<MAXD, BORDERS, true>(scene), io, n, P, counter, perm, rr);
   } else {
     hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
-                       scene_of<MAXD, BORDERS>(scene), io, n, P, perm);
+                       scene_of<MAXD, BORDERS>(scene), io, n, P, perm, rr);
   }
   HIPCHK(ctx, hipGetLastError());
   return 0;
@@ -923,19 +971,20 @@ constexpr int64_t kAutoOrderMin = int64_t(1) << 20;
 // Sort bits of the coherence order of n rays: the tile index in the high
 // bits, as many Morton bits below it as fit in 32.
 struct OrderBits {
+  int tile_bits;  // rays per tile: 2^tile_bits
   int mbits;
   unsigned end_bit;
 };
-static OrderBits order_bits(int64_t n) {
-  unsigned tile_bits = 0;
-  while ((int64_t(1) << tile_bits) < ((n - 1) >> kOrderTileBits) + 1) tile_bits++;
-  const int mbits = std::min(30, 32 - int(tile_bits));
-  return {mbits, unsigned(mbits) + tile_bits};
+static OrderBits order_bits(int64_t n, int tile_bits) {
+  unsigned index_bits = 0;
+  while ((int64_t(1) << index_bits) < ((n - 1) >> tile_bits) + 1) index_bits++;
+  const int mbits = std::min(30, 32 - int(index_bits));
+  return {tile_bits, mbits, unsigned(mbits) + index_bits};
 }
 
 // k_order_key over the rays: 8 blocks of 256 per CU, each striding over the rays.
-static int launch_order_key(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, int mbits,
-                            uint32_t* key, uint32_t* idx) {
+static int launch_order_key(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const OrderBits& ob,
+                            uint32_t* key, uint32_t* idx, RayRec* rr) {
   float s[3];
   for (int k = 0; k < 3; k++) {
     const float ext = scene->top_hi[k] - scene->top_lo[k];
@@ -943,14 +992,18 @@ static int launch_order_key(rb_ctx* ctx, const rb_scene* scene, const RayIO& io,
   }
   const int64_t grid = std::min<int64_t>((n + 255) / 256, int64_t(ctx->cus) * 8);
   hipLaunchKernelGGL(k_order_key, dim3(unsigned(grid)), dim3(256), 0, ctx->stream, scene->s, io, n, scene->top_lo[0],
-                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], mbits, key, idx);
+                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], ob.tile_bits, ob.mbits, key, idx, rr);
   HIPCHK(ctx, hipGetLastError());
   return 0;
 }
 
 // Coherence order of n rays: keys from k_order_key, then a radix sort of
-// (key, index) pairs. *perm receives the sorted indices (device).
-static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm) {
+// (key, index) pairs. *perm receives the sorted indices (device). With rr,
+// *rr receives the rays' staged copy (one line per ray, in input order),
+// written by the key kernel into a buffer of the context's; where that
+// buffer cannot be had (128 B per ray), *rr is null and the order is tiled.
+static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm,
+                      const RayRec** rr = nullptr) {
   if (size_t(n) > ctx->order_cap) {
     if (ctx->order_buf) HIPCHK(ctx, hipFree(ctx->order_buf));
     ctx->order_buf = nullptr;
@@ -962,8 +1015,24 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
   uint32_t* key_out = key_in + ctx->order_cap;
   uint32_t* idx_in = reinterpret_cast<uint32_t*>(key_out + ctx->order_cap);
   uint32_t* idx_out = idx_in + ctx->order_cap;
+  RayRec* rec = nullptr;
+  if (rr) {
+    if (size_t(n) > ctx->rec_cap) {
+      if (ctx->rec_buf) HIPCHK(ctx, hipFree(ctx->rec_buf));
+      ctx->rec_buf = nullptr;
+      ctx->rec_cap = 0;
+      if (hipMalloc(&ctx->rec_buf, sizeof(RayRec) * size_t(n)) == hipSuccess) {
+        ctx->rec_cap = size_t(n);
+      } else {
+        (void)hipGetLastError();  // out of memory: trace without the copy
+        ctx->rec_buf = nullptr;
+      }
+    }
+    rec = ctx->rec_cap ? static_cast<RayRec*>(ctx->rec_buf) : nullptr;
+    *rr = rec;
+  }
   const unsigned count = unsigned(n);
-  const OrderBits ob = order_bits(n);
+  const OrderBits ob = order_bits(n, rec ? kOrderGlobalBits : kOrderTileBits);
   size_t bytes = 0;
   HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, key_in, key_out, idx_in, idx_out, count, 0u, ob.end_bit,
                                         ctx->stream));
@@ -974,7 +1043,7 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
     HIPCHK(ctx, hipMalloc(&ctx->sort_tmp, bytes));
     ctx->sort_tmp_bytes = bytes;
   }
-  const int rc = launch_order_key(ctx, scene, io, n, ob.mbits, key_in, idx_in);
+  const int rc = launch_order_key(ctx, scene, io, n, ob, key_in, idx_in, rec);
   if (rc) return rc;
   bytes = ctx->sort_tmp_bytes;
   HIPCHK(ctx, rocprim::radix_sort_pairs(ctx->sort_tmp, bytes, key_in, key_out, idx_in, idx_out, count, 0u, ob.end_bit,
@@ -1030,14 +1099,14 @@ static int order_rays_scratch(rb_ctx* ctx, const rb_scene* scene, const RayIO& i
   rocprim::double_buffer<uint32_t> keys(key_a, key_b);
   rocprim::double_buffer<uint32_t> idx(idx_a, idx_b);
   const unsigned count = unsigned(n);
-  const OrderBits ob = order_bits(n);
+  const OrderBits ob = order_bits(n, kOrderTileBits);
   size_t bytes = 0;
   HIPCHK(ctx, rocprim::radix_sort_pairs(nullptr, bytes, keys, idx, count, 0u, ob.end_bit, ctx->stream));
   if (bytes > kScratchSortBytes)
     return fail(ctx, -2, "rbgpu_trace_noalloc: the radix sort needs " + std::to_string(bytes) +
                              " B of temporary storage, more than the scratch array's " +
                              std::to_string(kScratchSortBytes));
-  const int rc = launch_order_key(ctx, scene, io, n, ob.mbits, key_a, idx_a);
+  const int rc = launch_order_key(ctx, scene, io, n, ob, key_a, idx_a, nullptr);
   if (rc) return rc;
   bytes = kScratchSortBytes;
   HIPCHK(ctx, rocprim::radix_sort_pairs(tmp, bytes, keys, idx, count, 0u, ob.end_bit, ctx->stream));
@@ -1060,9 +1129,10 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   const bool persistent = schedule != RB_SCHEDULE_LANE;
   *coherent_out = coherent;
   const uint32_t* perm = nullptr;
+  const RayRec* rr = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (coherent) {
-    const int rc = noalloc ? order_rays_scratch(ctx, scene, io, n, &perm) : order_rays(ctx, scene, io, n, &perm);
+    const int rc = noalloc ? order_rays_scratch(ctx, scene, io, n, &perm) : order_rays(ctx, scene, io, n, &perm, &rr);
     if (rc) return rc;
   }
   unsigned long long* counter = nullptr;
@@ -1079,13 +1149,13 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   const int form = scene->has_borders ? 2 : (scene->max_depth <= 1 && !scene->has_ext) ? 0 : 1;
   int rc;
   if (mixed)
-    rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, schedule, io, n, P, perm, counter, noalloc)
-       : form == 1 ? launch_trace<true, kMaxCsgDepth, false>(ctx, scene, schedule, io, n, P, perm, counter, noalloc)
-                   : launch_trace<true, kMaxCsgDepth, true>(ctx, scene, schedule, io, n, P, perm, counter, noalloc);
+    rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc)
+       : form == 1 ? launch_trace<true, kMaxCsgDepth, false>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc)
+                   : launch_trace<true, kMaxCsgDepth, true>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc);
   else
-    rc = form == 0 ? launch_trace<false, 1, false>(ctx, scene, schedule, io, n, P, perm, counter, noalloc)
-       : form == 1 ? launch_trace<false, kMaxCsgDepth, false>(ctx, scene, schedule, io, n, P, perm, counter, noalloc)
-                   : launch_trace<false, kMaxCsgDepth, true>(ctx, scene, schedule, io, n, P, perm, counter, noalloc);
+    rc = form == 0 ? launch_trace<false, 1, false>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc)
+       : form == 1 ? launch_trace<false, kMaxCsgDepth, false>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc)
+                   : launch_trace<false, kMaxCsgDepth, true>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc);
   if (rc) return rc;
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
@@ -1156,6 +1226,7 @@ void rbgpu_ctx_destroy(rb_ctx* c) {
   if (c->next_ray) (void)hipFree(c->next_ray);
   if (c->order_buf) (void)hipFree(c->order_buf);
   if (c->sort_tmp) (void)hipFree(c->sort_tmp);
+  if (c->rec_buf) (void)hipFree(c->rec_buf);
   if (c->evk0) (void)hipEventDestroy(c->evk0);
   if (c->evk1) (void)hipEventDestroy(c->evk1);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1601,25 +1672,26 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
                         (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
   const uint32_t* perm = nullptr;
+  const RayRec* rr = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (coherent) {
-    const int rc = order_rays(ctx, scene, io, n, &perm);
+    const int rc = order_rays(ctx, scene, io, n, &perm, &rr);
     if (rc) return rc;
   }
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
   const bool shallow = scene->max_depth <= 1 && !scene->has_ext;
   if (opts->precision == RB_PRECISION_MIXED && shallow)
     hipLaunchKernelGGL((k_adjoint<true, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene), io,
-                       n, P, A, grad, perm);
+                       n, P, A, grad, perm, rr);
   else if (opts->precision == RB_PRECISION_MIXED)
     hipLaunchKernelGGL((k_adjoint<true, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
-                       scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm);
+                       scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm, rr);
   else if (shallow)
     hipLaunchKernelGGL((k_adjoint<false, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene),
-                       io, n, P, A, grad, perm);
+                       io, n, P, A, grad, perm, rr);
   else
     hipLaunchKernelGGL((k_adjoint<false, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
-                       scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm);
+                       scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm, rr);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
