@@ -170,6 +170,18 @@ __device__ __forceinline__ int64_t opaque_index(int64_t i) {
   if (RB_ADDR_BARRIER) asm volatile("" : "+v"(i));
   return i;
 }
+// The traced state into the ray's staged line (all 128 bytes, in 16-byte
+// stores), copied to the state arrays afterwards by k_unstage.
+__device__ __forceinline__ void store_staged(RayRec* rr, int64_t i, const Lane& L) {
+  double2* o = reinterpret_cast<double2*>(rr + i);
+  o[0] = make_double2(L.p.x, L.p.y);
+  o[1] = make_double2(L.p.z, L.t);
+  o[2] = make_double2(L.d.x, L.d.y);
+  o[3] = make_double2(L.d.z, L.lambda);
+  int4* q = reinterpret_cast<int4*>(o + 4);
+  q[0] = make_int4(L.status, L.npoints, L.last_node, 0);
+  q[1] = q[2] = q[3] = make_int4(0, 0, 0, 0);
+}
 __device__ __forceinline__ void store_lane(const RayIO& io, int64_t i, const Lane& L) {
   io.x[i] = L.p.x;
   io.y[i] = L.p.y;
@@ -286,6 +298,23 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
   }
 }
 
+// The traced rays' staged lines back into the state arrays, in input order
+// (coalesced on both sides). Lines of rays that were not running hold their
+// input state, which is written back unchanged.
+__global__ void __launch_bounds__(256) k_unstage(const RayRec* rr, RayIO io, int64_t n) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double2* o = reinterpret_cast<const double2*>(rr + i);
+    const double2 a = o[0], b = o[1], c = o[2], d = o[3];
+    const int4 q = reinterpret_cast<const int4*>(o + 4)[0];
+    io.x[i] = a.x; io.y[i] = a.y; io.z[i] = b.x; io.t[i] = b.y;
+    io.dx[i] = c.x; io.dy[i] = c.y; io.dz[i] = d.x;
+    io.status[i] = q.x;
+    io.npoints[i] = q.y;
+    if (io.last_node) io.last_node[i] = q.z;
+  }
+}
+
 // Persistent form: a grid sized to the machine, each lane taking the next
 // ray from a global counter as soon as its current one terminates, so a
 // wave's lanes stay busy instead of waiting for the longest path among 64
@@ -299,7 +328,7 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
                                                                           int64_t n,
                                                                           TraceParams P,
                                                                           unsigned long long* next_ray,
-                                                                          const uint32_t* perm, const RayRec* rr) {
+                                                                          const uint32_t* perm, RayRec* rr) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   stage_bvh(S);
@@ -335,7 +364,10 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSc
     if (active) {
       step_lane<MIXED, true>(S, P, io, i, L, rec);
       if (L.status != RB_RUN) {
-        store_lane(io, i, L);
+        if (rr)
+          store_staged(rr, i, L);
+        else
+          store_lane(io, i, L);
         active = false;
       }
     }
@@ -382,7 +414,7 @@ template <bool MIXED, int MAXD, bool BORDERS>
 __global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), true> S, RayIO io,
                                                                           int64_t n, TraceParams P,
                                                                           unsigned long long* next_ray,
-                                                                          const uint32_t* perm, const RayRec* rr) {
+                                                                          const uint32_t* perm, RayRec* rr) {
   __shared__ int32_t cand_node[kRegroupSlots * kCand * kCandStride];
   __shared__ float cand_tn[kRegroupSlots * kCand * kCandStride];
   __shared__ RegroupLDS G;
@@ -466,7 +498,10 @@ __global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(D
       const int64_t i = int64_t(L.ray - P.id_base);
       step_lane<MIXED, true>(S, P, io, i, L, rec);
       if (L.status != RB_RUN) {
-        store_lane(io, i, L);
+        if (rr)
+          store_staged(rr, i, L);
+        else
+          store_lane(io, i, L);
         active = false;
       }
     }
@@ -478,7 +513,7 @@ __global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(D
 template <bool MIXED, int MAXD, bool BORDERS>
 __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD, BORDERS> S, RayIO io, int64_t n,
                                                                   TraceParams P, const uint32_t* perm,
-                                                                  const RayRec* rr) {
+                                                                  RayRec* rr) {
   __shared__ int32_t cand_node[kCand * kCandStride];
   __shared__ float cand_tn[kCand * kCandStride];
   stage_bvh(S);
@@ -491,7 +526,10 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MA
   L.ray = P.id_base + uint64_t(i);
   NoRecord rec;
   trace_lane<MIXED>(S, P, io, i, L, rec);
-  store_lane(io, opaque_index(i), L);
+  if (rr)
+    store_staged(rr, opaque_index(i), L);
+  else
+    store_lane(io, opaque_index(i), L);
 }
 
 // Adjoint of the focal-plane loss: each lane recomputes its ray's bounce
@@ -913,7 +951,7 @@ __global__ void k_clear_counter(unsigned long long* c) {
 // The trace kernel for the scene's depth and the lane schedule.
 template <bool MIXED, int MAXD, bool BORDERS>
 static int launch_trace(rb_ctx* ctx, const rb_scene* scene, int schedule, const RayIO& io, int64_t n,
-                        const TraceParams& P, const uint32_t* perm, const RayRec* rr, unsigned long long* counter,
+                        const TraceParams& P, const uint32_t* perm, RayRec* rr, unsigned long long* counter,
                         bool clear_by_kernel) {
   const int block = kTraceBlock;
   const int64_t grid = (n + block - 1) / block;
@@ -1003,7 +1041,7 @@ static int launch_order_key(rb_ctx* ctx, const rb_scene* scene, const RayIO& io,
 // written by the key kernel into a buffer of the context's; where that
 // buffer cannot be had (128 B per ray), *rr is null and the order is tiled.
 static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm,
-                      const RayRec** rr = nullptr) {
+                      RayRec** rr = nullptr) {
   if (size_t(n) > ctx->order_cap) {
     if (ctx->order_buf) HIPCHK(ctx, hipFree(ctx->order_buf));
     ctx->order_buf = nullptr;
@@ -1129,7 +1167,7 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   const bool persistent = schedule != RB_SCHEDULE_LANE;
   *coherent_out = coherent;
   const uint32_t* perm = nullptr;
-  const RayRec* rr = nullptr;
+  RayRec* rr = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (coherent) {
     const int rc = noalloc ? order_rays_scratch(ctx, scene, io, n, &perm) : order_rays(ctx, scene, io, n, &perm, &rr);
@@ -1159,6 +1197,11 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   if (rc) return rc;
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->evk1, ctx->stream));
+  if (rr) {
+    const int64_t grid = std::min<int64_t>((n + 255) / 256, int64_t(ctx->cus) * 8);
+    hipLaunchKernelGGL(k_unstage, dim3(unsigned(grid)), dim3(256), 0, ctx->stream, rr, io, n);
+    HIPCHK(ctx, hipGetLastError());
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   return 0;
 }
@@ -1672,7 +1715,7 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
                         (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
   const uint32_t* perm = nullptr;
-  const RayRec* rr = nullptr;
+  RayRec* rr = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (coherent) {
     const int rc = order_rays(ctx, scene, io, n, &perm, &rr);
