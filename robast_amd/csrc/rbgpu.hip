@@ -248,8 +248,8 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
   stage_bvh(S);
   const int root = S.nodes[0].bvh_root;
   const int end = bvh_node(S, root).escape;
-  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
     uint32_t k = 0x3fffffffu;  // not running or no daughter box crossed: the last cell
     const int32_t status = io.status[i];
     const double x = io.x[i], y = io.y[i], z = io.z[i], ddx = io.dx[i], ddy = io.dy[i], ddz = io.dz[i];
@@ -1003,6 +1003,14 @@ static bool rays_accessible(const rb_rays* r, bool with_path) {
 }
 
 // Batches at least this large are traced in coherence order by default.
+// Batches at least this large are traced in coherence order by default.
+// Whether the order pays depends on the input: at 2^19 rays, randomly placed
+// beams gain (Davies-Cotton 0.65 -> 0.57 ms, Schwarzschild-Couder 1.34 ->
+// 0.96, Winston cone array 6.82 -> 6.66), while a grid of parallel rays,
+// already coherent as given, loses (paraboloid 0.18 -> 0.35); at 2^17 every
+// case loses. Telling the two apart on the device (a count of rays whose key
+// is below their predecessor's, read back before the sort) cost 0.16 ms of
+// host round trip per sorted batch, more than it saved.
 constexpr int64_t kAutoOrderMin = int64_t(1) << 20;
 
 // Sort bits of the coherence order of n rays: the tile index in the high
@@ -1158,22 +1166,27 @@ static int order_rays_scratch(rb_ctx* ctx, const rb_scene* scene, const RayIO& i
 // buffers and the ray counter live in the scratch array and nothing is
 // allocated; otherwise they are the context's own, grown on demand.
 static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const RayIO& io, int64_t n,
-                        const TraceParams& P, bool noalloc, bool* coherent_out) {
+                        const TraceParams& P, bool noalloc, int* launches) {
   const bool mixed = opts->precision == RB_PRECISION_MIXED;
-  const bool coherent = scene->has_top_box && n < (int64_t(1) << 31) &&
-                        (opts->order == RB_ORDER_COHERENT || (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
+  const bool coherent =
+      scene->has_top_box && n < (int64_t(1) << 31) &&
+      (opts->order == RB_ORDER_COHERENT ||
+       (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
   const int schedule = opts->schedule != RB_SCHEDULE_AUTO ? opts->schedule
                        : scene->has_lens ? (scene->has_cells ? RB_SCHEDULE_REGROUP : RB_SCHEDULE_PERSISTENT)
                                          : RB_SCHEDULE_LANE;
   const bool persistent = schedule != RB_SCHEDULE_LANE;
-  *coherent_out = coherent;
   const uint32_t* perm = nullptr;
   RayRec* rr = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (coherent) {
-    const int rc = noalloc ? order_rays_scratch(ctx, scene, io, n, &perm) : order_rays(ctx, scene, io, n, &perm, &rr);
+    const int rc = noalloc ? order_rays_scratch(ctx, scene, io, n, &perm)
+                           : order_rays(ctx, scene, io, n, &perm, &rr);
     if (rc) return rc;
   }
+  // the key kernel, the radix sort (counted once), the trace and the
+  // copy-back of the staged lines
+  *launches = (perm ? 2 : 0) + 1 + (rr ? 1 : 0);
   unsigned long long* counter = nullptr;
   if (noalloc) {
     counter = static_cast<unsigned long long*>(ctx->scratch);
@@ -1396,10 +1409,10 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, r
     io.path_node = rays->path_node;
   }
 
-  bool coherent = false;
-  const int rc = trace_launch(ctx, scene, opts, io, n, P, false, &coherent);
+  int launches = 0;
+  const int rc = trace_launch(ctx, scene, opts, io, n, P, false, &launches);
   if (rc) return rc;
-  ctx->launches = coherent ? 3 : 1;  // key kernel + radix sort (counted once) + trace
+  ctx->launches = launches;
 
   if (host) {
     double* hs[7] = {rays->x, rays->y, rays->z, rays->t, rays->dx, rays->dy, rays->dz};
@@ -1446,10 +1459,10 @@ int rbgpu_trace_noalloc(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   io.path_cap = int32_t(rays->path_capacity);
   io.path_xyzt = rays->path_xyzt;
   io.path_node = rays->path_node;
-  bool coherent = false;
-  const int rc = trace_launch(ctx, scene, opts, io, n, P, true, &coherent);
+  int launches = 0;
+  const int rc = trace_launch(ctx, scene, opts, io, n, P, true, &launches);
   if (rc) return rc;
-  ctx->launches = coherent ? 3 : 1;
+  ctx->launches = launches;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   float ms = 0, kms = 0;
   HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1745,7 +1758,7 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   HIPCHK(ctx, hipEventElapsedTime(&kms, ctx->evk0, ctx->evk1));
   ctx->last_ms = ms;
   ctx->kernel_ms = kms;
-  ctx->launches = coherent ? 3 : 1;
+  ctx->launches = (perm ? 2 : 0) + 1;  // key kernel, radix sort, adjoint
   return 0;
 }
 

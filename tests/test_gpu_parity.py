@@ -232,7 +232,7 @@ def test_coherence_order_gives_identical_results(ctx, config):
         ctx.generate(gen, make_rays_device(t), N)
         t["status"][::7] = A.RB_STOP   # some rays not running: left untouched in every order
         ctx.trace(scene, make_rays_device(t), N, mgr.trace_opts(device_ptrs=1, order=order, schedule=schedule))
-        assert ctx.last_trace_launches() == (3 if order == A.RB_ORDER_COHERENT else 1)
+        assert ctx.last_trace_launches() == (4 if order == A.RB_ORDER_COHERENT else 1)  # key, sort, trace, copy-back
         out[order, schedule] = t
     for schedule in (A.RB_SCHEDULE_LANE, A.RB_SCHEDULE_PERSISTENT, A.RB_SCHEDULE_REGROUP):
         a, b = out[A.RB_ORDER_INPUT, schedule], out[A.RB_ORDER_COHERENT, schedule]
@@ -284,8 +284,9 @@ def test_empty_and_ragged_batches(ctx, n):
 
 
 def test_auto_order_just_above_threshold(ctx):
-    """A batch of 2^20 + 3 rays (auto order sorts it, in a partial last tile)
-    equals the same batch traced in input order."""
+    """A batch of 2^20 + 3 random Davies-Cotton rays (auto order sorts it:
+    key kernel, sort, trace, copy-back of the staged lines) equals the same
+    batch traced in input order."""
     import torch
     N = (1 << 20) + 3
     mgr = configs.c2_scene()
@@ -298,7 +299,8 @@ def test_auto_order_just_above_threshold(ctx):
             t[f] = torch.empty(N, dtype=torch.int32, device=dev)
         ctx.generate(configs.c2_raygen(), make_rays_device(t), N)
         ctx.trace(scene, make_rays_device(t), N, mgr.trace_opts(device_ptrs=1, order=order))
-        assert ctx.last_trace_launches() == (3 if order == A.RB_ORDER_AUTO else 1)
+        assert ctx.last_trace_launches() == (4 if order == A.RB_ORDER_AUTO else 1)
         out[order] = t
     for k in out[A.RB_ORDER_AUTO]:
         assert torch.equal(out[A.RB_ORDER_AUTO][k], out[A.RB_ORDER_INPUT][k]), k
+
