@@ -223,8 +223,10 @@ typedef struct {
  * so per-ray results are the same in any order; the order only changes how
  * coherent the rays sharing a wavefront are. COHERENT sorts running rays by
  * a Morton code of the point where each first enters a daughter box of the
- * top volume, and traces them in that order through a permutation. AUTO
- * sorts batches of 2^20 rays or more. */
+ * top volume, and traces them in that order through a permutation; each
+ * ray's state is staged in a 128-byte line of a context buffer (128 B per
+ * ray, grown on demand), read and written by the trace and copied back to
+ * the ray arrays after it. AUTO sorts batches of 2^20 rays or more. */
 enum { RB_ORDER_AUTO = 0, RB_ORDER_INPUT = 1, RB_ORDER_COHERENT = 2 };
 
 /* Lane schedule. LANE: one lane per ray for the whole trace. PERSISTENT: a
@@ -296,8 +298,10 @@ int rbgpu_trace(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
  * message. The coherence order's keys and indices and the persistent
  * schedule's ray counter live in one file-scope device array of about
  * 65 MiB (RB_NOALLOC_MAX_RAYS x 16 B + 1 MiB), which each GPU the library
- * runs on holds until the process ends. Every call on a GPU shares that
- * array: calls that overlap on one GPU (two contexts, two host threads)
+ * runs on holds until the process ends. A sorted trace here has no staged
+ * ray lines: the kernels read and write the ray arrays through the
+ * permutation, sorted within tiles of 2^19 rays. Every call on a GPU shares
+ * that array: calls that overlap on one GPU (two contexts, two host threads)
  * race. Calls no hipMalloc, hipFree or allocating library routine.
  * Blocks until done. */
 #define RB_NOALLOC_MAX_RAYS 4194304
