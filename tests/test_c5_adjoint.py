@@ -287,3 +287,24 @@ def test_gpu_moments_form_matches_centroid_form(ctx):
     g1c = model.gradient()
     nz = g1c != 0
     assert (np.abs(g1 - g1c)[nz] / np.abs(g1c[nz])).max() <= 1e-5
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_coherent_order_matches_input_order(ctx):
+    """The moments-form adjoint over rays in coherence order (read through
+    their staged lines) equals the same launch in input order: loss and
+    gradient to 1e-12 (the FP64 block sums are added in another order)."""
+    print("C5 moments form, coherent vs input order, 8 angles x 50k rays", flush=True)
+    model = AL.DaviesCottonAlignment(ctx, rays_per_angle=50000)
+    p0 = AL.c5_initial_params()
+    model.loss_and_gradient(p0)
+    c0 = list(model.c0)
+    out = {}
+    for order in (A.RB_ORDER_INPUT, A.RB_ORDER_COHERENT):
+        model.c0 = list(c0)
+        model._opts = lambda order=order: model.mgr.trace_opts(device_ptrs=1, order=order)
+        out[order] = model.loss_and_gradient(p0)
+        assert ctx.last_trace_launches() == (3 if order == A.RB_ORDER_COHERENT else 1)
+    (l1, g1), (l2, g2) = out[A.RB_ORDER_INPUT], out[A.RB_ORDER_COHERENT]
+    assert l2 == pytest.approx(l1, rel=1e-12)
+    assert np.abs(g2 - g1).max() <= 1e-12 * np.abs(g1).max()

@@ -244,6 +244,34 @@ def test_coherence_order_gives_identical_results(ctx, config):
             assert torch.equal(a[k], b[k]), (schedule, k)
 
 
+def test_coherent_order_with_path_record(ctx):
+    """A coherent-order trace (rays read and written through their staged
+    lines) with the path record on equals the input-order trace: terminal
+    states, every recorded point and node, and the record's indexing from the
+    point count a ray enters with (some rays enter with 3 points)."""
+    import torch
+    N, cap = 100_000, 4
+    print(f"C2 path record, coherent vs input order, n={N}", flush=True)
+    mgr = configs.c2_scene()
+    scene = ctx.upload(mgr.flat())
+    dev = torch.device("cuda")
+    out = {}
+    for order in (A.RB_ORDER_INPUT, A.RB_ORDER_COHERENT):
+        t = {f: torch.empty(N, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_")}
+        for f in ("status", "npoints", "last_node"):
+            t[f] = torch.empty(N, dtype=torch.int32, device=dev)
+        t["path_xyzt"] = torch.zeros((N, cap, 4), dtype=torch.float64, device=dev)
+        t["path_node"] = torch.full((N, cap), -1, dtype=torch.int32, device=dev)
+        ctx.generate(configs.c2_raygen(), make_rays_device(t), N)
+        t["npoints"][::5] = 3
+        ctx.trace(scene, make_rays_device(t), N, mgr.trace_opts(device_ptrs=1, order=order))
+        out[order] = t
+    a, b = out[A.RB_ORDER_INPUT], out[A.RB_ORDER_COHERENT]
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
+    assert int((a["path_node"][:, 1] >= 0).sum()) > N // 2
+
+
 def test_host_memory_passed_as_device_pointers_is_an_error(ctx):
     """device_ptrs = 1 with pageable host arrays returns an error (checked on
     the host before any launch) instead of faulting in a kernel."""
