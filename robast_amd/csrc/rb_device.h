@@ -92,13 +92,13 @@ constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // most MAXD levels: the deeper composite call sites, and the registers they
 // make the kernel spill around them, are left out; likewise the border-surface
 // code unless BORDERS, and the TGeoPcon / general TGeoPgon /
-// AGeoWinstonCone2D code unless EXT (compiled into the deep and the
-// border-surface forms; a scene holding such a shape is traced with the deep
-// form).
+// AGeoWinstonCone2D code unless EXT (compiled into the deep forms, with or
+// without border surfaces; a scene holding such a shape is traced with a
+// deep form).
 // BOX: boolean nodes test a child's bounding box before evaluating the child
 // (compiled into the work-queue kernels: in the lane kernel the extra code
 // cost more in register spills than it saved).
-template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1 || BORDERS), bool BOX = false>
+template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1), bool BOX = false>
 struct DSceneOf : DScene {
   static constexpr int max_depth = MAXD;
   static constexpr bool borders_on = BORDERS;

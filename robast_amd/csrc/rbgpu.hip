@@ -58,8 +58,8 @@ namespace {
 
 // The scene as the kernels compiled for composites at most MAXD deep take it.
 template <int MAXD, bool BORDERS = false, bool BOX = false>
-DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), BOX> scene_of(const rb_scene* sc) {
-  DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), BOX> s;
+DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX> scene_of(const rb_scene* sc) {
+  DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX> s;
   static_cast<DScene&>(s) = sc->s;
   return s;
 }
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(256) k_unstage(const RayRec* rr, RayIO io, int
 // it, with the same per-ray Philox stream, so results equal k_trace's. Every
 // wave leaves when the counter has passed n and none of its lanes holds a ray.
 template <bool MIXED, int MAXD, bool BORDERS>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), true> S, RayIO io,
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace_persist(DSceneOf<MAXD, BORDERS, (MAXD > 1), true> S, RayIO io,
                                                                           int64_t n,
                                                                           TraceParams P,
                                                                           unsigned long long* next_ray,
@@ -410,7 +410,7 @@ struct RegroupLDS {
 };
 
 template <bool MIXED, int MAXD, bool BORDERS>
-__global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), true> S, RayIO io,
+__global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(DSceneOf<MAXD, BORDERS, (MAXD > 1), true> S, RayIO io,
                                                                           int64_t n, TraceParams P,
                                                                           unsigned long long* next_ray,
                                                                           const uint32_t* perm, RayRec* rr) {
@@ -1195,12 +1195,18 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
     counter = ctx->next_ray;
   }
   HIPCHK(ctx, hipEventRecord(ctx->evk0, ctx->stream));
-  // kernels: composites at most one level deep, no border conditions and no
-  // TGeoPcon / general TGeoPgon / AGeoWinstonCone2D; deeper composites or
-  // those shapes; border conditions (the general form)
-  const int form = scene->has_borders ? 2 : (scene->max_depth <= 1 && !scene->has_ext) ? 0 : 1;
+  // kernels: composites at most one level deep and no TGeoPcon / general
+  // TGeoPgon / AGeoWinstonCone2D (form 0, or 3 with border conditions);
+  // deeper composites or those shapes (form 1, or 2 with border conditions).
+  // The one-level border form measured 9.45 ms against 11.04 for the deep
+  // one on the coated Davies-Cotton (1e7 rays): a 336-byte frame, not 1,688.
+  const bool shallow = scene->max_depth <= 1 && !scene->has_ext;
+  const int form = scene->has_borders ? (shallow ? 3 : 2) : shallow ? 0 : 1;
   int rc;
-  if (mixed)
+  if (form == 3)
+    rc = mixed ? launch_trace<true, 1, true>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc)
+               : launch_trace<false, 1, true>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc);
+  else if (mixed)
     rc = form == 0 ? launch_trace<true, 1, false>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc)
        : form == 1 ? launch_trace<true, kMaxCsgDepth, false>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc)
                    : launch_trace<true, kMaxCsgDepth, true>(ctx, scene, schedule, io, n, P, perm, rr, counter, noalloc);

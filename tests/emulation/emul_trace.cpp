@@ -34,7 +34,7 @@ struct Slots {
 // boxes first
 template <bool MIXED, int MAXD, bool BORDERS, bool HINT>
 void run(const DScene& base, const TraceParams& P, const RayIO& io, int64_t n) {
-  DSceneOf<MAXD, BORDERS, (MAXD > 1 || BORDERS), HINT> S;
+  DSceneOf<MAXD, BORDERS, (MAXD > 1), HINT> S;
   static_cast<DScene&>(S) = base;
   Slots slots;
   for (int64_t i = 0; i < n; i++) {
@@ -101,9 +101,15 @@ int emul_trace(const rb_scene_desc* d, const rb_trace_opts* o, rb_rays* r, int64
   io.path_node = r->path_node;
   rb_emul_fallbacks = 0;
   const bool mixed = o->precision == RB_PRECISION_MIXED;
-  const int form = lay.has_borders ? 2 : (lay.max_depth <= 1 && !lay.has_ext) ? 0 : 1;
+  const bool shallow = lay.max_depth <= 1 && !lay.has_ext;
+  const int form = lay.has_borders ? (shallow ? 3 : 2) : shallow ? 0 : 1;
   const bool hint = lay.has_lens && !lane_form;
-  if (mixed)
+  if (form == 3) {
+    if (mixed)
+      hint ? run<true, 1, true, true>(base, P, io, n) : run<true, 1, true, false>(base, P, io, n);
+    else
+      hint ? run<false, 1, true, true>(base, P, io, n) : run<false, 1, true, false>(base, P, io, n);
+  } else if (mixed)
     form == 0 ? (hint ? run<true, 1, false, true>(base, P, io, n) : run<true, 1, false, false>(base, P, io, n))
     : form == 1 ? (hint ? run<true, kMaxCsgDepth, false, true>(base, P, io, n)
                         : run<true, kMaxCsgDepth, false, false>(base, P, io, n))
