@@ -86,6 +86,7 @@ struct DScene {
   static constexpr int max_depth = kMaxCsgDepth;
   static constexpr bool borders_on = true;
   static constexpr bool ext_on = true;
+  static constexpr bool lens_on = true;
 };
 constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // The same scene, for kernels compiled for scenes whose composites nest at
@@ -98,12 +99,16 @@ constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // BOX: boolean nodes test a child's bounding box before evaluating the child
 // (compiled into the work-queue kernels: in the lane kernel the extra code
 // cost more in register spills than it saved).
-template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1), bool BOX = false>
+// LENS: the refracting-volume physics (absorption in the volume, Fresnel
+// reflection and refraction at its faces); the lane kernels for scenes with
+// no ALens are compiled without it (a smaller frame: 288 bytes against 336).
+template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1), bool BOX = false, bool LENS = true>
 struct DSceneOf : DScene {
   static constexpr int max_depth = MAXD;
   static constexpr bool borders_on = BORDERS;
   static constexpr bool ext_on = EXT;
   static constexpr bool box_on = BOX;
+  static constexpr bool lens_on = LENS;
 };
 
 template <typename T>

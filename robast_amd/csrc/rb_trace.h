@@ -953,7 +953,7 @@ __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, in
     const double step = L.step;
     const int tc = kind_of(S, current), tn = kind_of(S, next);
 
-    if (tc == RB_KLENS) {
+    if (SC::lens_on && tc == RB_KLENS) {
       const rb_volume_desc& v = vol_of(S, current);
       double k = lens_k(S, v, lambda);
       double absl = k <= 0 ? 0.0 : lambda / (4 * kPi * k);
@@ -972,14 +972,15 @@ __device__ void step_lane(const SC& S, const TraceParams& P, const RayIO& io, in
     // (current, next) transition table; the mirror and the Fresnel cases
     // share one normal and one DoReflection
     const bool c_free = tc == RB_KNULL || tc == RB_KOPT || tc == RB_KOTHER;
-    const bool c_lens = tc == RB_KLENS;
+    // (scenes traced without the lens physics have no RB_KLENS node)
+    const bool c_lens = SC::lens_on && tc == RB_KLENS;
     const bool n_free = tn == RB_KNULL || tn == RB_KOPT || tn == RB_KOTHER;
     const bool mirror = (c_free || c_lens) && tn == RB_KMIRROR;
-    const bool fresnel = (c_free && tn == RB_KLENS) || (c_lens && (tn == RB_KLENS || n_free));
+    const bool fresnel = SC::lens_on && ((c_free && tn == RB_KLENS) || (c_lens && (tn == RB_KLENS || n_free)));
     if (mirror || fresnel) {
       const double n1 = c_lens ? lens_index(S, vol_of(S, current), lambda) : 1.;
       double n2 = 1., k2 = 0.;
-      if (tn == RB_KLENS) {
+      if (SC::lens_on && tn == RB_KLENS) {
         const rb_volume_desc& v = vol_of(S, next);
         n2 = lens_index(S, v, lambda);
         k2 = lens_k(S, v, lambda);

@@ -57,9 +57,9 @@ struct rb_scene {
 namespace {
 
 // The scene as the kernels compiled for composites at most MAXD deep take it.
-template <int MAXD, bool BORDERS = false, bool BOX = false>
-DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX> scene_of(const rb_scene* sc) {
-  DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX> s;
+template <int MAXD, bool BORDERS = false, bool BOX = false, bool LENS = true>
+DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX, LENS> scene_of(const rb_scene* sc) {
+  DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX, LENS> s;
   static_cast<DScene&>(s) = sc->s;
   return s;
 }
@@ -509,8 +509,8 @@ __global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(D
 
 // One lane per ray. With `perm`, lane t traces ray perm[t] (the coherence
 // order); the ray keeps its own index for its Philox stream and its slots.
-template <bool MIXED, int MAXD, bool BORDERS>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD, BORDERS> S, RayIO io, int64_t n,
+template <bool MIXED, int MAXD, bool BORDERS, bool LENS>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD, BORDERS, (MAXD > 1), false, LENS> S, RayIO io, int64_t n,
                                                                   TraceParams P, const uint32_t* perm,
                                                                   RayRec* rr) {
   __shared__ int32_t cand_node[kCand * kCandStride];
@@ -972,9 +972,13 @@ static int launch_trace(rb_ctx* ctx, const rb_scene* scene, int schedule, const 
     else
       hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
                          scene_of<MAXD, BORDERS, true>(scene), io, n, P, counter, perm, rr);
-  } else {
-    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+  } else if (scene->has_lens) {
+    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS, true>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<MAXD, BORDERS>(scene), io, n, P, perm, rr);
+  } else {
+    // no refracting volume: the form without the lens physics
+    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS, false>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                       scene_of<MAXD, BORDERS, false, false>(scene), io, n, P, perm, rr);
   }
   HIPCHK(ctx, hipGetLastError());
   return 0;
