@@ -534,8 +534,8 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MA
 // Adjoint of the focal-plane loss: each lane recomputes its ray's bounce
 // chain (nothing is stored by the forward pass) and accumulates parameter
 // gradients in LDS (FP64), flushed with one global atomic per parameter.
-template <bool MIXED, int MAXD>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DSceneOf<MAXD> S, RayIO io, int64_t n,
+template <bool MIXED, int MAXD, bool LENS = true>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DSceneOf<MAXD, false, (MAXD > 1), false, LENS> S, RayIO io, int64_t n,
                                                                     TraceParams P, AdjParams A, double* grad,
                                                                     const uint32_t* perm, const RayRec* rr) {
   __shared__ double acc[kMaxAdjAcc];
@@ -1753,9 +1753,15 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   else if (opts->precision == RB_PRECISION_MIXED)
     hipLaunchKernelGGL((k_adjoint<true, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm, rr);
+  else if (shallow && !scene->has_lens)  // no refracting volume: the form without the lens physics
+    hipLaunchKernelGGL((k_adjoint<false, 1, false>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                       scene_of<1, false, false, false>(scene), io, n, P, A, grad, perm, rr);
   else if (shallow)
     hipLaunchKernelGGL((k_adjoint<false, 1>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream, scene_of<1>(scene),
                        io, n, P, A, grad, perm, rr);
+  else if (!scene->has_lens)
+    hipLaunchKernelGGL((k_adjoint<false, kMaxCsgDepth, false>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                       scene_of<kMaxCsgDepth, false, false, false>(scene), io, n, P, A, grad, perm, rr);
   else
     hipLaunchKernelGGL((k_adjoint<false, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm, rr);
