@@ -55,6 +55,11 @@ struct DShape {
   int32_t depth;  // boolean nesting levels below this shape (0 = primitive)
   int32_t pad;
 };
+// Shape sets a kernel form is compiled for (the primitive dispatch leaves the
+// others out): kShapesExt the TGeoPcon, general TGeoPgon, AGeoWinstonCone2D
+// and phi-limited TGeoSphere code; kShapesSolver the AGeoAsphericDisk and
+// AGeoWinstonConePoly code.
+constexpr unsigned kShapesExt = 1u, kShapesSolver = 2u;
 struct DScene {
   const DNode* nodes;
   const int32_t* children;
@@ -87,6 +92,7 @@ struct DScene {
   static constexpr bool borders_on = true;
   static constexpr bool ext_on = true;
   static constexpr bool lens_on = true;
+  static constexpr unsigned shape_set = kShapesExt | kShapesSolver;
 };
 constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // The same scene, for kernels compiled for scenes whose composites nest at
@@ -102,13 +108,18 @@ constexpr int kBvhStage = 256;  // 8 KB of DBvh per block
 // LENS: the refracting-volume physics (absorption in the volume, Fresnel
 // reflection and refraction at its faces); the lane kernels for scenes with
 // no ALens are compiled without it (a smaller frame: 288 bytes against 336).
-template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1), bool BOX = false, bool LENS = true>
+// SOLVER: the AGeoAsphericDisk and AGeoWinstonConePoly code (iterative
+// roots); the lane kernels for scenes with neither (and no ALens) are
+// compiled without it (160 bytes against 288, no VGPR spills).
+template <int MAXD, bool BORDERS = false, bool EXT = (MAXD > 1), bool BOX = false, bool LENS = true,
+          bool SOLVER = true>
 struct DSceneOf : DScene {
   static constexpr int max_depth = MAXD;
   static constexpr bool borders_on = BORDERS;
   static constexpr bool ext_on = EXT;
   static constexpr bool box_on = BOX;
   static constexpr bool lens_on = LENS;
+  static constexpr unsigned shape_set = (EXT ? kShapesExt : 0u) | (SOLVER ? kShapesSolver : 0u);
 };
 
 template <typename T>
@@ -1423,67 +1434,67 @@ template <typename T> __device__ RB_POLY_FN V3<T> wins2d_normal(const double* p,
 #endif
 
 // ------------------------------------------------------------ dispatch
-template <typename T, bool EXT> __device__ RB_PRIM_FN bool prim_contains(int type, const double* p, V3<T> q) {
+template <typename T, unsigned SH> __device__ RB_PRIM_FN bool prim_contains(int type, const double* p, V3<T> q) {
   switch (type) {
     case RB_SHAPE_BOX: return box_contains<T>(p, q);
     case RB_SHAPE_TUBE: return tube_contains<T>(p, q);
     case RB_SHAPE_SPHERE: return sphere_contains<T>(p, q);
     case RB_SHAPE_PARABOLOID: return para_contains<T>(p, q);
     case RB_SHAPE_PGON: return pgon_contains<T>(p, q);
-    case kShapePgonGeneral: if constexpr (EXT) return poly_contains<T, true>(p, q); break;
-    case RB_SHAPE_PCON: if constexpr (EXT) return poly_contains<T, false>(p, q); break;
-    case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_contains<T>(p, q); break;
-    case kShapeSphereWedge: if constexpr (EXT) return sphere_contains<T, true>(p, q); break;
-    case RB_SHAPE_ASPHERIC: return asph_contains<T>(p, q);
-    case RB_SHAPE_WINSTON: return wins_contains<T>(p, q);
+    case kShapePgonGeneral: if constexpr ((SH & kShapesExt) != 0u) return poly_contains<T, true>(p, q); break;
+    case RB_SHAPE_PCON: if constexpr ((SH & kShapesExt) != 0u) return poly_contains<T, false>(p, q); break;
+    case RB_SHAPE_WINSTON2D: if constexpr ((SH & kShapesExt) != 0u) return wins2d_contains<T>(p, q); break;
+    case kShapeSphereWedge: if constexpr ((SH & kShapesExt) != 0u) return sphere_contains<T, true>(p, q); break;
+    case RB_SHAPE_ASPHERIC: if constexpr ((SH & kShapesSolver) != 0u) return asph_contains<T>(p, q); break;
+    case RB_SHAPE_WINSTON: if constexpr ((SH & kShapesSolver) != 0u) return wins_contains<T>(p, q); break;
   }
   return false;
 }
-template <typename T, bool EXT> __device__ RB_PRIM_FN T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T, unsigned SH> __device__ RB_PRIM_FN T prim_dist_out(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_dist_out<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_dist_out<T>(p, q, d);
     case RB_SHAPE_SPHERE: return sphere_dist<T, true>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_dist_out<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_dist_out<T>(p, q, d);
-    case kShapePgonGeneral: if constexpr (EXT) return poly_dist_out<T, true>(p, q, d); break;
-    case RB_SHAPE_PCON: if constexpr (EXT) return poly_dist_out<T, false>(p, q, d); break;
-    case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_dist_out<T>(p, q, d); break;
-    case kShapeSphereWedge: if constexpr (EXT) return sphere_dist<T, true, true>(p, q, d); break;
-    case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
-    case RB_SHAPE_WINSTON: return wins_dist_out<T>(p, q, d);
+    case kShapePgonGeneral: if constexpr ((SH & kShapesExt) != 0u) return poly_dist_out<T, true>(p, q, d); break;
+    case RB_SHAPE_PCON: if constexpr ((SH & kShapesExt) != 0u) return poly_dist_out<T, false>(p, q, d); break;
+    case RB_SHAPE_WINSTON2D: if constexpr ((SH & kShapesExt) != 0u) return wins2d_dist_out<T>(p, q, d); break;
+    case kShapeSphereWedge: if constexpr ((SH & kShapesExt) != 0u) return sphere_dist<T, true, true>(p, q, d); break;
+    case RB_SHAPE_ASPHERIC: if constexpr ((SH & kShapesSolver) != 0u) return asph_dist<T>(p, q, d); break;
+    case RB_SHAPE_WINSTON: if constexpr ((SH & kShapesSolver) != 0u) return wins_dist_out<T>(p, q, d); break;
   }
   return T(kBig);
 }
-template <typename T, bool EXT> __device__ RB_PRIM_FN T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T, unsigned SH> __device__ RB_PRIM_FN T prim_dist_in(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_dist_in<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_dist_in<T>(p, q, d);
     case RB_SHAPE_SPHERE: return sphere_dist<T, false>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_dist_in<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_dist_in<T>(p, q, d);
-    case kShapePgonGeneral: if constexpr (EXT) return poly_dist_in<T, true>(p, q, d); break;
-    case RB_SHAPE_PCON: if constexpr (EXT) return poly_dist_in<T, false>(p, q, d); break;
-    case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_dist_in<T>(p, q, d); break;
-    case kShapeSphereWedge: if constexpr (EXT) return sphere_dist<T, false, true>(p, q, d); break;
-    case RB_SHAPE_ASPHERIC: return asph_dist<T>(p, q, d);
-    case RB_SHAPE_WINSTON: return wins_dist_in<T>(p, q, d);
+    case kShapePgonGeneral: if constexpr ((SH & kShapesExt) != 0u) return poly_dist_in<T, true>(p, q, d); break;
+    case RB_SHAPE_PCON: if constexpr ((SH & kShapesExt) != 0u) return poly_dist_in<T, false>(p, q, d); break;
+    case RB_SHAPE_WINSTON2D: if constexpr ((SH & kShapesExt) != 0u) return wins2d_dist_in<T>(p, q, d); break;
+    case kShapeSphereWedge: if constexpr ((SH & kShapesExt) != 0u) return sphere_dist<T, false, true>(p, q, d); break;
+    case RB_SHAPE_ASPHERIC: if constexpr ((SH & kShapesSolver) != 0u) return asph_dist<T>(p, q, d); break;
+    case RB_SHAPE_WINSTON: if constexpr ((SH & kShapesSolver) != 0u) return wins_dist_in<T>(p, q, d); break;
   }
   return T(0);
 }
-template <typename T, bool EXT> __device__ RB_PRIM_FN V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
+template <typename T, unsigned SH> __device__ RB_PRIM_FN V3<T> prim_normal(int type, const double* p, V3<T> q, V3<T> d) {
   switch (type) {
     case RB_SHAPE_BOX: return box_normal<T>(p, q, d);
     case RB_SHAPE_TUBE: return tube_normal<T>(p, q, d);
     case RB_SHAPE_SPHERE: return sphere_normal<T>(p, q, d);
     case RB_SHAPE_PARABOLOID: return para_normal<T>(p, q, d);
     case RB_SHAPE_PGON: return pgon_normal<T>(p, q, d);
-    case kShapePgonGeneral: if constexpr (EXT) return poly_normal<T, true>(p, q, d); break;
-    case RB_SHAPE_PCON: if constexpr (EXT) return poly_normal<T, false>(p, q, d); break;
-    case RB_SHAPE_WINSTON2D: if constexpr (EXT) return wins2d_normal<T>(p, q, d); break;
-    case kShapeSphereWedge: if constexpr (EXT) return sphere_normal<T, true>(p, q, d); break;
-    case RB_SHAPE_ASPHERIC: return asph_normal<T>(p, q, d);
-    case RB_SHAPE_WINSTON: return wins_normal<T>(p, q, d);
+    case kShapePgonGeneral: if constexpr ((SH & kShapesExt) != 0u) return poly_normal<T, true>(p, q, d); break;
+    case RB_SHAPE_PCON: if constexpr ((SH & kShapesExt) != 0u) return poly_normal<T, false>(p, q, d); break;
+    case RB_SHAPE_WINSTON2D: if constexpr ((SH & kShapesExt) != 0u) return wins2d_normal<T>(p, q, d); break;
+    case kShapeSphereWedge: if constexpr ((SH & kShapesExt) != 0u) return sphere_normal<T, true>(p, q, d); break;
+    case RB_SHAPE_ASPHERIC: if constexpr ((SH & kShapesSolver) != 0u) return asph_normal<T>(p, q, d); break;
+    case RB_SHAPE_WINSTON: if constexpr ((SH & kShapesSolver) != 0u) return wins_normal<T>(p, q, d); break;
   }
   return mk<T>(T(0), T(0), T(1));
 }
@@ -1521,18 +1532,18 @@ __device__ __forceinline__ bool sbox_ray(const double* b, V3<T> q, V3<T> d) {
   return t0 <= t1 * T(1.0001) + T(1e-3);
 }
 
-template <typename T, int D, bool EXT, bool BOX = false> struct Csg {
+template <typename T, int D, unsigned SH, bool BOX = false> struct Csg {
   // child `side` (0 left, 1 right) contains q: outside its box it does not
   __device__ __forceinline__ static bool child_in(const DScene& S, const double* p, int side, V3<T> q) {
     const double* m = p + (side ? 15 : 3);
     const int c = int(p[side ? 2 : 1]);
     const V3<T> lq = m2l<T>(m, m + 9, q);
-    return (!BOX || sbox_has<T>(S.sbox + 6 * c, lq)) && Csg<T, D - 1, EXT, BOX>::contains(S, c, lq);
+    return (!BOX || sbox_has<T>(S.sbox + 6 * c, lq)) && Csg<T, D - 1, SH, BOX>::contains(S, c, lq);
   }
   __device__ RB_CSG_FN static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return prim_contains<T, EXT>(sh.type, p, q);
+    if (sh.type != RB_SHAPE_COMPOSITE) return prim_contains<T, SH>(sh.type, p, q);
     int op = int(p[0]);
     bool a = child_in(S, p, 0, q);
     if (op == RB_OP_UNION) return a || child_in(S, p, 1, q);
@@ -1546,7 +1557,7 @@ template <typename T, int D, bool EXT, bool BOX = false> struct Csg {
     int c = int(p[side ? 2 : 1]);
     V3<T> lq = m2l<T>(m, m + 9, q), ld = m2lv<T>(m, d);
     if (BOX && out && !sbox_ray<T>(S.sbox + 6 * c, lq, ld)) return {kBig, 0u, 0u};
-    Hit h = out ? Csg<T, D - 1, EXT, BOX>::dist_out(S, c, lq, ld) : Csg<T, D - 1, EXT, BOX>::dist_in(S, c, lq, ld);
+    Hit h = out ? Csg<T, D - 1, SH, BOX>::dist_out(S, c, lq, ld) : Csg<T, D - 1, SH, BOX>::dist_in(S, c, lq, ld);
     h.leaf = leaf_push(h.leaf, side);
     return h;
   }
@@ -1557,7 +1568,7 @@ template <typename T, int D, bool EXT, bool BOX = false> struct Csg {
   __device__ RB_CSG_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d, double lim = kBig) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T, EXT>(sh.type, p, q, d)), 0u, 0u};
+    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_out<T, SH>(sh.type, p, q, d)), 0u, 0u};
     const int op = int(p[0]);
     if (op == RB_OP_UNION) {
       Hit a = child(S, p, 0, true, q, d), b = child(S, p, 1, true, q, d);
@@ -1606,7 +1617,7 @@ template <typename T, int D, bool EXT, bool BOX = false> struct Csg {
   __device__ RB_CSG_FN static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T, EXT>(sh.type, p, q, d)), 0u, 1u};
+    if (sh.type != RB_SHAPE_COMPOSITE) return {double(prim_dist_in<T, SH>(sh.type, p, q, d)), 0u, 1u};
     const int op = int(p[0]);
     if (op == RB_OP_INTERSECTION) {
       Hit a = child(S, p, 0, false, q, d), b = child(S, p, 1, false, q, d);
@@ -1639,10 +1650,10 @@ template <typename T, int D, bool EXT, bool BOX = false> struct Csg {
   __device__ RB_CSG_FN static V3<T> normal(const DScene& S, int shape, uint32_t path, int depth, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
     const double* p = S.params + sh.off;
-    if (sh.type != RB_SHAPE_COMPOSITE) return prim_normal<T, EXT>(sh.type, p, q, d);
+    if (sh.type != RB_SHAPE_COMPOSITE) return prim_normal<T, SH>(sh.type, p, q, d);
     int side = depth > 0 ? int(path & 1u) : 0;
     const double* m = p + (side ? 15 : 3);
-    V3<T> n = Csg<T, D - 1, EXT, BOX>::normal(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0,
+    V3<T> n = Csg<T, D - 1, SH, BOX>::normal(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0,
                                     m2l<T>(m, m + 9, q), m2lv<T>(m, d));
     return l2mv<T>(m, n);
   }
@@ -1656,32 +1667,32 @@ template <typename T, int D, bool EXT, bool BOX = false> struct Csg {
     if (sh.type != RB_SHAPE_COMPOSITE) {
       T t0 = s0 - back > T(0) ? s0 - back : T(0);
       V3<T> q0 = axpy(q, t0, d);
-      T t = exiting ? prim_dist_in<T, EXT>(sh.type, p, q0, d) : prim_dist_out<T, EXT>(sh.type, p, q0, d);
+      T t = exiting ? prim_dist_in<T, SH>(sh.type, p, q0, d) : prim_dist_out<T, SH>(sh.type, p, q0, d);
       return t >= T(kBig) ? kBig : double(t0 + t);
     }
     int side = depth > 0 ? int(path & 1u) : 0;
     const double* m = p + (side ? 15 : 3);
-    return Csg<T, D - 1, EXT, BOX>::polish(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0, exiting,
+    return Csg<T, D - 1, SH, BOX>::polish(S, int(p[side ? 2 : 1]), path >> 1, depth > 0 ? depth - 1 : 0, exiting,
                                  m2l<T>(m, m + 9, q), m2lv<T>(m, d), s0, back);
   }
 };
 // depth exhausted: primitives only (the host rejects deeper expressions)
-template <typename T, bool EXT, bool BOX> struct Csg<T, 0, EXT, BOX> {
+template <typename T, unsigned SH, bool BOX> struct Csg<T, 0, SH, BOX> {
   __device__ RB_CSG0_FN static bool contains(const DScene& S, int shape, V3<T> q) {
     DShape sh = S.shapes[shape];
-    return prim_contains<T, EXT>(sh.type, S.params + sh.off, q);
+    return prim_contains<T, SH>(sh.type, S.params + sh.off, q);
   }
   __device__ RB_CSG0_FN static Hit dist_out(const DScene& S, int shape, V3<T> q, V3<T> d, double = kBig) {
     DShape sh = S.shapes[shape];
-    return {double(prim_dist_out<T, EXT>(sh.type, S.params + sh.off, q, d)), 0u, 0u};
+    return {double(prim_dist_out<T, SH>(sh.type, S.params + sh.off, q, d)), 0u, 0u};
   }
   __device__ RB_CSG0_FN static Hit dist_in(const DScene& S, int shape, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
-    return {double(prim_dist_in<T, EXT>(sh.type, S.params + sh.off, q, d)), 0u, 1u};
+    return {double(prim_dist_in<T, SH>(sh.type, S.params + sh.off, q, d)), 0u, 1u};
   }
   __device__ RB_CSG0_FN static V3<T> normal(const DScene& S, int shape, uint32_t, int, V3<T> q, V3<T> d) {
     DShape sh = S.shapes[shape];
-    return prim_normal<T, EXT>(sh.type, S.params + sh.off, q, d);
+    return prim_normal<T, SH>(sh.type, S.params + sh.off, q, d);
   }
   __device__ RB_CSG0_FN static double polish(const DScene& S, int shape, uint32_t, int, bool exiting, V3<T> q, V3<T> d, T s0,
                                   T back) {
@@ -1689,7 +1700,7 @@ template <typename T, bool EXT, bool BOX> struct Csg<T, 0, EXT, BOX> {
     const double* p = S.params + sh.off;
     T t0 = s0 - back > T(0) ? s0 - back : T(0);
     V3<T> q0 = axpy(q, t0, d);
-    T t = exiting ? prim_dist_in<T, EXT>(sh.type, p, q0, d) : prim_dist_out<T, EXT>(sh.type, p, q0, d);
+    T t = exiting ? prim_dist_in<T, SH>(sh.type, p, q0, d) : prim_dist_out<T, SH>(sh.type, p, q0, d);
     return t >= T(kBig) ? kBig : double(t0 + t);
   }
 };
@@ -1702,35 +1713,35 @@ __device__ __forceinline__ V3<T> shape_normal(const SC& S, int shape, uint32_t l
   const uint32_t path = leaf & 0xffffffu;
   const int depth = int(leaf >> 24);
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0, SC::ext_on, SC::box_on>::normal(S, shape, path, depth, q, d);
-    case 1: return Csg<T, 1, SC::ext_on, SC::box_on>::normal(S, shape, path, depth, q, d);
+    case 0: return Csg<T, 0, SC::shape_set, SC::box_on>::normal(S, shape, path, depth, q, d);
+    case 1: return Csg<T, 1, SC::shape_set, SC::box_on>::normal(S, shape, path, depth, q, d);
     case 2:
-      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on, SC::box_on>::normal(S, shape, path, depth, q, d);
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::shape_set, SC::box_on>::normal(S, shape, path, depth, q, d);
       break;
   }
-  return Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::normal(S, shape, path, depth, q, d);
+  return Csg<T, SC::max_depth, SC::shape_set, SC::box_on>::normal(S, shape, path, depth, q, d);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ bool shape_contains(const SC& S, int shape, V3<T> q) {
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0, SC::ext_on, SC::box_on>::contains(S, shape, q);
-    case 1: return Csg<T, 1, SC::ext_on, SC::box_on>::contains(S, shape, q);
+    case 0: return Csg<T, 0, SC::shape_set, SC::box_on>::contains(S, shape, q);
+    case 1: return Csg<T, 1, SC::shape_set, SC::box_on>::contains(S, shape, q);
     case 2:
-      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on, SC::box_on>::contains(S, shape, q);
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::shape_set, SC::box_on>::contains(S, shape, q);
       break;
   }
-  return Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::contains(S, shape, q);
+  return Csg<T, SC::max_depth, SC::shape_set, SC::box_on>::contains(S, shape, q);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ Hit shape_dist(const SC& S, int shape, V3<T> q, V3<T> d, bool out, double lim = kBig) {
   switch (S.shapes[shape].depth) {
-    case 0: return out ? Csg<T, 0, SC::ext_on, SC::box_on>::dist_out(S, shape, q, d) : Csg<T, 0, SC::ext_on, SC::box_on>::dist_in(S, shape, q, d);
-    case 1: return out ? Csg<T, 1, SC::ext_on, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, 1, SC::ext_on, SC::box_on>::dist_in(S, shape, q, d);
+    case 0: return out ? Csg<T, 0, SC::shape_set, SC::box_on>::dist_out(S, shape, q, d) : Csg<T, 0, SC::shape_set, SC::box_on>::dist_in(S, shape, q, d);
+    case 1: return out ? Csg<T, 1, SC::shape_set, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, 1, SC::shape_set, SC::box_on>::dist_in(S, shape, q, d);
     case 2:
-      if constexpr (SC::max_depth >= 3) return out ? Csg<T, 2, SC::ext_on, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, 2, SC::ext_on, SC::box_on>::dist_in(S, shape, q, d);
+      if constexpr (SC::max_depth >= 3) return out ? Csg<T, 2, SC::shape_set, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, 2, SC::shape_set, SC::box_on>::dist_in(S, shape, q, d);
       break;
   }
-  return out ? Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::dist_in(S, shape, q, d);
+  return out ? Csg<T, SC::max_depth, SC::shape_set, SC::box_on>::dist_out(S, shape, q, d, lim) : Csg<T, SC::max_depth, SC::shape_set, SC::box_on>::dist_in(S, shape, q, d);
 }
 template <typename T, typename SC>
 __device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t leaf, bool exiting, V3<T> q,
@@ -1738,13 +1749,13 @@ __device__ __forceinline__ double shape_polish(const SC& S, int shape, uint32_t 
   const uint32_t path = leaf & 0xffffffu;
   const int depth = int(leaf >> 24);
   switch (S.shapes[shape].depth) {
-    case 0: return Csg<T, 0, SC::ext_on, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
-    case 1: return Csg<T, 1, SC::ext_on, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+    case 0: return Csg<T, 0, SC::shape_set, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+    case 1: return Csg<T, 1, SC::shape_set, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
     case 2:
-      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::ext_on, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+      if constexpr (SC::max_depth >= 3) return Csg<T, 2, SC::shape_set, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
       break;
   }
-  return Csg<T, SC::max_depth, SC::ext_on, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
+  return Csg<T, SC::max_depth, SC::shape_set, SC::box_on>::polish(S, shape, path, depth, exiting, q, d, s0, back);
 }
 
 // ------------------------------------------------------------ physics

@@ -366,6 +366,7 @@ struct SceneLayout {
   bool has_ext = false;                   // a TGeoPcon, a general TGeoPgon, an AGeoWinstonCone2D or a
                                           // TGeoSphere with a phi range (code compiled into the deep
                                           // kernel forms only)
+  bool has_solver = false;                // an AGeoAsphericDisk or an AGeoWinstonConePoly
 };
 
 // Validate d, build BVHs and pack everything into blob. 0 = ok.
@@ -426,6 +427,7 @@ inline int pack_scene(const rb_scene_desc* d, std::vector<char>& blob, SceneLayo
     if (shapes[i].type == kShapePgonGeneral || shapes[i].type == RB_SHAPE_PCON ||
         shapes[i].type == RB_SHAPE_WINSTON2D || shapes[i].type == kShapeSphereWedge)
       lay.has_ext = true;
+    if (shapes[i].type == RB_SHAPE_ASPHERIC || shapes[i].type == RB_SHAPE_WINSTON) lay.has_solver = true;
     lay.max_depth = std::max(lay.max_depth, shapes[i].depth);
   }
   blob.clear();

@@ -50,6 +50,7 @@ struct rb_scene {
   bool has_cells = false; // some node's daughters have neighbour cells
   bool has_borders = false;  // border-surface conditions are defined
   bool has_ext = false;      // a TGeoPcon, general TGeoPgon or AGeoWinstonCone2D: traced with the deep kernel form
+  bool has_solver = false;   // an AGeoAsphericDisk or AGeoWinstonConePoly
   bool has_top_box = false;
   float top_lo[3] = {0, 0, 0}, top_hi[3] = {0, 0, 0};
 };
@@ -57,9 +58,9 @@ struct rb_scene {
 namespace {
 
 // The scene as the kernels compiled for composites at most MAXD deep take it.
-template <int MAXD, bool BORDERS = false, bool BOX = false, bool LENS = true>
-DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX, LENS> scene_of(const rb_scene* sc) {
-  DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX, LENS> s;
+template <int MAXD, bool BORDERS = false, bool BOX = false, bool LENS = true, bool SOLVER = true>
+DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX, LENS, SOLVER> scene_of(const rb_scene* sc) {
+  DSceneOf<MAXD, BORDERS, (MAXD > 1), BOX, LENS, SOLVER> s;
   static_cast<DScene&>(s) = sc->s;
   return s;
 }
@@ -509,8 +510,8 @@ __global__ void __launch_bounds__(kRegroupBlock) RB_TRACE_ATTR k_trace_regroup(D
 
 // One lane per ray. With `perm`, lane t traces ray perm[t] (the coherence
 // order); the ray keeps its own index for its Philox stream and its slots.
-template <bool MIXED, int MAXD, bool BORDERS, bool LENS>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD, BORDERS, (MAXD > 1), false, LENS> S, RayIO io, int64_t n,
+template <bool MIXED, int MAXD, bool BORDERS, bool LENS, bool SOLVER>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MAXD, BORDERS, (MAXD > 1), false, LENS, SOLVER> S, RayIO io, int64_t n,
                                                                   TraceParams P, const uint32_t* perm,
                                                                   RayRec* rr) {
   __shared__ int32_t cand_node[kCand * kCandStride];
@@ -973,12 +974,16 @@ static int launch_trace(rb_ctx* ctx, const rb_scene* scene, int schedule, const 
       hipLaunchKernelGGL((k_trace_persist<MIXED, MAXD, BORDERS>), dim3(unsigned(pgrid)), dim3(block), 0, ctx->stream,
                          scene_of<MAXD, BORDERS, true>(scene), io, n, P, counter, perm, rr);
   } else if (scene->has_lens) {
-    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS, true>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS, true, true>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<MAXD, BORDERS>(scene), io, n, P, perm, rr);
-  } else {
+  } else if (scene->has_solver) {
     // no refracting volume: the form without the lens physics
-    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS, false>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
-                       scene_of<MAXD, BORDERS, false, false>(scene), io, n, P, perm, rr);
+    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS, false, true>), dim3(unsigned(grid)), dim3(block), 0,
+                       ctx->stream, scene_of<MAXD, BORDERS, false, false>(scene), io, n, P, perm, rr);
+  } else {
+    // neither a refracting volume nor an aspheric or Winston shape
+    hipLaunchKernelGGL((k_trace<MIXED, MAXD, BORDERS, false, false>), dim3(unsigned(grid)), dim3(block), 0,
+                       ctx->stream, scene_of<MAXD, BORDERS, false, false, false>(scene), io, n, P, perm, rr);
   }
   HIPCHK(ctx, hipGetLastError());
   return 0;
@@ -1364,6 +1369,7 @@ int rbgpu_scene_upload(rb_ctx* ctx, const rb_scene_desc* d, rb_scene** out) {
   sc->has_cells = lay.has_cells;
   sc->has_borders = lay.has_borders;
   sc->has_ext = lay.has_ext;
+  sc->has_solver = lay.has_solver;
   sc->has_top_box = lay.has_top_box;
   for (int k = 0; k < 3; k++) {
     sc->top_lo[k] = lay.top_lo[k];

@@ -32,9 +32,9 @@ struct Slots {
 // with a lens): the FP64 search ranks the last daughter first, searches and
 // relocates within neighbour cells, and boolean nodes test their children's
 // boxes first
-template <bool MIXED, int MAXD, bool BORDERS, bool HINT, bool LENS = true>
+template <bool MIXED, int MAXD, bool BORDERS, bool HINT, bool LENS = true, bool SOLVER = true>
 void run(const DScene& base, const TraceParams& P, const RayIO& io, int64_t n) {
-  DSceneOf<MAXD, BORDERS, (MAXD > 1), HINT, LENS> S;
+  DSceneOf<MAXD, BORDERS, (MAXD > 1), HINT, LENS, SOLVER> S;
   static_cast<DScene&>(S) = base;
   Slots slots;
   for (int64_t i = 0; i < n; i++) {
@@ -105,8 +105,21 @@ int emul_trace(const rb_scene_desc* d, const rb_trace_opts* o, rb_rays* r, int64
   const int form = lay.has_borders ? (shallow ? 3 : 2) : shallow ? 0 : 1;
   const bool hint = lay.has_lens && !lane_form;
   // the lane kernels of scenes without a refracting volume leave the lens
-  // physics out (LENS = false), as the library's do
-  if (!lay.has_lens) {
+  // physics out (LENS = false), and those of scenes without an aspheric or
+  // Winston shape too the iterative solvers (SOLVER = false), as the
+  // library's do
+  if (!lay.has_lens && !lay.has_solver) {
+    if (mixed)
+      form == 0 ? run<true, 1, false, false, false, false>(base, P, io, n)
+      : form == 1 ? run<true, kMaxCsgDepth, false, false, false, false>(base, P, io, n)
+      : form == 2 ? run<true, kMaxCsgDepth, true, false, false, false>(base, P, io, n)
+                  : run<true, 1, true, false, false, false>(base, P, io, n);
+    else
+      form == 0 ? run<false, 1, false, false, false, false>(base, P, io, n)
+      : form == 1 ? run<false, kMaxCsgDepth, false, false, false, false>(base, P, io, n)
+      : form == 2 ? run<false, kMaxCsgDepth, true, false, false, false>(base, P, io, n)
+                  : run<false, 1, true, false, false, false>(base, P, io, n);
+  } else if (!lay.has_lens) {
     if (mixed)
       form == 0 ? run<true, 1, false, false, false>(base, P, io, n)
       : form == 1 ? run<true, kMaxCsgDepth, false, false, false>(base, P, io, n)
