@@ -535,8 +535,8 @@ __global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_trace(DSceneOf<MA
 // Adjoint of the focal-plane loss: each lane recomputes its ray's bounce
 // chain (nothing is stored by the forward pass) and accumulates parameter
 // gradients in LDS (FP64), flushed with one global atomic per parameter.
-template <bool MIXED, int MAXD, bool LENS = true>
-__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DSceneOf<MAXD, false, (MAXD > 1), false, LENS> S, RayIO io, int64_t n,
+template <bool MIXED, int MAXD, bool LENS = true, bool SOLVER = true>
+__global__ void __launch_bounds__(kTraceBlock) RB_TRACE_ATTR k_adjoint(DSceneOf<MAXD, false, (MAXD > 1), false, LENS, SOLVER> S, RayIO io, int64_t n,
                                                                     TraceParams P, AdjParams A, double* grad,
                                                                     const uint32_t* perm, const RayRec* rr) {
   __shared__ double acc[kMaxAdjAcc];
@@ -1769,6 +1769,9 @@ int rbgpu_adjoint(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts,
   else if (opts->precision == RB_PRECISION_MIXED)
     hipLaunchKernelGGL((k_adjoint<true, kMaxCsgDepth>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<kMaxCsgDepth>(scene), io, n, P, A, grad, perm, rr);
+  else if (shallow && !scene->has_lens && !scene->has_solver)  // neither lens physics nor iterative solvers
+    hipLaunchKernelGGL((k_adjoint<false, 1, false, false>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
+                       scene_of<1, false, false, false, false>(scene), io, n, P, A, grad, perm, rr);
   else if (shallow && !scene->has_lens)  // no refracting volume: the form without the lens physics
     hipLaunchKernelGGL((k_adjoint<false, 1, false>), dim3(unsigned(grid)), dim3(block), 0, ctx->stream,
                        scene_of<1, false, false, false>(scene), io, n, P, A, grad, perm, rr);
