@@ -1012,15 +1012,21 @@ static bool rays_accessible(const rb_rays* r, bool with_path) {
 }
 
 // Batches at least this large are traced in coherence order by default.
-// Batches at least this large are traced in coherence order by default.
 // Whether the order pays depends on the input: at 2^19 rays, randomly placed
 // beams gain (Davies-Cotton 0.65 -> 0.57 ms, Schwarzschild-Couder 1.34 ->
-// 0.96, Winston cone array 6.82 -> 6.66), while a grid of parallel rays,
-// already coherent as given, loses (paraboloid 0.18 -> 0.35); at 2^17 every
-// case loses. Telling the two apart on the device (a count of rays whose key
-// is below their predecessor's, read back before the sort) cost 0.16 ms of
-// host round trip per sorted batch, more than it saved.
+// 0.96), while a grid of parallel rays, already coherent as given, loses
+// (paraboloid 0.18 -> 0.35); at 2^17 every case loses. Telling the two apart
+// on the device (a count of rays whose key is below their predecessor's,
+// read back before the sort) cost 0.16 ms of host round trip per sorted
+// batch, more than it saved.
 constexpr int64_t kAutoOrderMin = int64_t(1) << 20;
+// Scenes traced with a work-queue schedule (a refracting volume) cost tens
+// of times more per ray, so the order's fixed cost matters less: the Winston
+// cone array at 1e6 rays 11.1-11.2 ms as given, 10.39 sorted (ordering
+// included, two runs); at 2^19 and 2^18 rays the two runs disagree (6.63
+// against 6.67 and 6.82 against 6.66; 4.39 against 4.47 and 4.53 against
+// 4.42), so the threshold sits above them.
+constexpr int64_t kAutoOrderMinQueue = int64_t(3) << 18;
 
 // Sort bits of the coherence order of n rays: the tile index in the high
 // bits, as many Morton bits below it as fit in 32.
@@ -1187,14 +1193,14 @@ static int order_rays_scratch(rb_ctx* ctx, const rb_scene* scene, const RayIO& i
 static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts* opts, const RayIO& io, int64_t n,
                         const TraceParams& P, bool noalloc, int* launches) {
   const bool mixed = opts->precision == RB_PRECISION_MIXED;
-  const bool coherent =
-      scene->has_top_box && n < (int64_t(1) << 31) &&
-      (opts->order == RB_ORDER_COHERENT ||
-       (opts->order == RB_ORDER_AUTO && n >= kAutoOrderMin));
   const int schedule = opts->schedule != RB_SCHEDULE_AUTO ? opts->schedule
                        : scene->has_lens ? (scene->has_cells ? RB_SCHEDULE_REGROUP : RB_SCHEDULE_PERSISTENT)
                                          : RB_SCHEDULE_LANE;
   const bool persistent = schedule != RB_SCHEDULE_LANE;
+  const bool coherent =
+      scene->has_top_box && n < (int64_t(1) << 31) &&
+      (opts->order == RB_ORDER_COHERENT ||
+       (opts->order == RB_ORDER_AUTO && n >= (persistent ? kAutoOrderMinQueue : kAutoOrderMin)));
   const uint32_t* perm = nullptr;
   RayRec* rr = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));

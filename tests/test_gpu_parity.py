@@ -244,6 +244,30 @@ def test_coherence_order_gives_identical_results(ctx, config):
             assert torch.equal(a[k], b[k]), (schedule, k)
 
 
+def test_auto_order_work_queue_threshold(ctx):
+    """The Winston cone array (a lens scene, traced with the work-queue
+    schedule) is sorted by the auto order from 3 x 2^18 rays: just above that
+    the trace runs the key kernel, sort, trace and copy-back, just below it
+    the trace alone, and both equal input order."""
+    import torch
+    mgr = configs.c4_scene()
+    scene = ctx.upload(mgr.flat())
+    dev = torch.device("cuda")
+    for N, launches in ((3 * (1 << 18) + 5, 4), (3 * (1 << 18) - 5, 1)):
+        print(f"C4 auto order n={N}", flush=True)
+        out = {}
+        for order in (A.RB_ORDER_AUTO, A.RB_ORDER_INPUT):
+            t = {f: torch.empty(N, dtype=torch.float64, device=dev) for f in ("x", "y", "z", "t", "dx", "dy", "dz", "lambda_")}
+            for f in ("status", "npoints", "last_node"):
+                t[f] = torch.empty(N, dtype=torch.int32, device=dev)
+            ctx.generate(configs.c4_raygen(mgr.c4_dz), make_rays_device(t), N)
+            ctx.trace(scene, make_rays_device(t), N, mgr.trace_opts(device_ptrs=1, order=order))
+            assert ctx.last_trace_launches() == (launches if order == A.RB_ORDER_AUTO else 1)
+            out[order] = t
+        for k in out[A.RB_ORDER_AUTO]:
+            assert torch.equal(out[A.RB_ORDER_AUTO][k], out[A.RB_ORDER_INPUT][k]), (N, k)
+
+
 def test_coherent_order_with_path_record(ctx):
     """A coherent-order trace (rays read and written through their staged
     lines) with the path record on equals the input-order trace: terminal
