@@ -203,10 +203,15 @@ __device__ __forceinline__ void stage_bvh(const DScene& S) {
   __syncthreads();
 }
 
-// Coherence key of each ray: a 30-bit Morton code of the point where it first
-// enters one of the top volume's daughter boxes (box tests only, over the top
-// node's BVH), quantised over the daughters' joint box. Rays that are not
+// Coherence key of each ray: a 30-bit Morton code of the point where it
+// enters the first of the top volume's daughter boxes that the walk of the
+// top node's BVH meets (box tests only; the first in walk order, not the
+// nearest), quantised over the daughters' joint box. Rays that are not
 // running, or cross no daughter box, get the last cell and sort last.
+// Stopping at the first box met rather than walking on for the nearest,
+// measured: Winston cone array at 1e7 rays 83.7 -> 80.7 ms per trace (there
+// every ray's nearest box is the window above the cones), Davies-Cotton
+// 5.77 -> 5.78 and Schwarzschild-Couder 12.28 -> 12.24.
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {
   v &= 0x3ffu;
   v = (v | (v << 16)) & 0x030000ffu;
@@ -281,7 +286,10 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
           b = box.escape;
           continue;
         }
-        if (box.prim >= 0) best = fminf(best, tn);
+        if (box.prim >= 0) {
+          best = tn;
+          break;
+        }
         b++;
       }
       if (best < 3.0e38f) {
