@@ -227,7 +227,7 @@ typedef struct {
  * ray's state is staged in a 128-byte line of a context buffer (128 B per
  * ray, grown on demand), read and written by the trace and copied back to
  * the ray arrays after it. AUTO sorts batches of 2^20 rays or more, or of
- * 786,432 (3 x 2^18) or more when the work-queue schedule traces them. */
+ * 2^18 or more when the work-queue schedule traces them. */
 enum { RB_ORDER_AUTO = 0, RB_ORDER_INPUT = 1, RB_ORDER_COHERENT = 2 };
 
 /* Lane schedule. LANE: one lane per ray for the whole trace. PERSISTENT: a

@@ -204,14 +204,15 @@ __device__ __forceinline__ void stage_bvh(const DScene& S) {
 }
 
 // Coherence key of each ray: a 30-bit Morton code of the point where it
-// enters the first of the top volume's daughter boxes that the walk of the
-// top node's BVH meets (box tests only; the first in walk order, not the
-// nearest), quantised over the daughters' joint box. Rays that are not
-// running, or cross no daughter box, get the last cell and sort last.
-// Stopping at the first box met rather than walking on for the nearest,
-// measured: Winston cone array at 1e7 rays 83.7 -> 80.7 ms per trace (there
-// every ray's nearest box is the window above the cones), Davies-Cotton
-// 5.77 -> 5.78 and Schwarzschild-Couder 12.28 -> 12.24.
+// enters one of the top volume's daughter boxes (box tests only, over the
+// top node's BVH), quantised over the daughters' joint box: the nearest box
+// it crosses, or with first_box the first one the walk meets. Rays that are
+// not running, or cross no daughter box, get the last cell and sort last.
+// The work-queue schedules take first_box: Winston cone array at 1e7 rays
+// 83.7 -> 80.7 ms per trace (there every ray's nearest box is the window
+// above the cones). The lane kernel keeps the nearest box: Davies-Cotton's
+// trace kernel 4.46 against 4.55 ms with first_box (its key kernel is
+// 0.05 ms faster), Schwarzschild-Couder unchanged.
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {
   v &= 0x3ffu;
   v = (v | (v << 16)) & 0x030000ffu;
@@ -250,7 +251,7 @@ constexpr int kOrderGlobalBits = 31;  // one tile: batches are below 2^31 rays
 // loads, and LDS answers them at a fraction of the L2 latency.
 __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n, float lox, float loy, float loz,
                                                    float sx, float sy, float sz, int tile_bits, int mbits,
-                                                   uint32_t* key, uint32_t* idx, RayRec* rr) {
+                                                   uint32_t* key, uint32_t* idx, RayRec* rr, bool first_box) {
   stage_bvh(S);
   const int root = S.nodes[0].bvh_root;
   const int end = bvh_node(S, root).escape;
@@ -287,8 +288,8 @@ __global__ void __launch_bounds__(256) k_order_key(DScene S, RayIO io, int64_t n
           continue;
         }
         if (box.prim >= 0) {
-          best = tn;
-          break;
+          best = fminf(best, tn);
+          if (first_box) break;
         }
         b++;
       }
@@ -1030,11 +1031,10 @@ static bool rays_accessible(const rb_rays* r, bool with_path) {
 constexpr int64_t kAutoOrderMin = int64_t(1) << 20;
 // Scenes traced with a work-queue schedule (a refracting volume) cost tens
 // of times more per ray, so the order's fixed cost matters less: the Winston
-// cone array at 1e6 rays 11.1-11.2 ms as given, 10.39 sorted (ordering
-// included, two runs); at 2^19 and 2^18 rays the two runs disagree (6.63
-// against 6.67 and 6.82 against 6.66; 4.39 against 4.47 and 4.53 against
-// 4.42), so the threshold sits above them.
-constexpr int64_t kAutoOrderMinQueue = int64_t(3) << 18;
+// cone array (keyed by the first daughter box met) at 2^18 rays 4.40 ms as
+// given, 4.28 sorted; 2^19 6.74 and 6.43; 1e6 11.41 and 10.31 (ordering
+// included).
+constexpr int64_t kAutoOrderMinQueue = int64_t(1) << 18;
 
 // Sort bits of the coherence order of n rays: the tile index in the high
 // bits, as many Morton bits below it as fit in 32.
@@ -1054,7 +1054,7 @@ static OrderBits order_bits(int64_t n, int tile_bits) {
 // per thread measured 0.58 ms at 1e7 rays against 0.89 for 8 blocks of 256
 // per CU striding over them (the BVH staging is then paid per block, from L2).
 static int launch_order_key(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const OrderBits& ob,
-                            uint32_t* key, uint32_t* idx, RayRec* rr) {
+                            uint32_t* key, uint32_t* idx, RayRec* rr, bool first_box) {
   float s[3];
   for (int k = 0; k < 3; k++) {
     const float ext = scene->top_hi[k] - scene->top_lo[k];
@@ -1062,7 +1062,8 @@ static int launch_order_key(rb_ctx* ctx, const rb_scene* scene, const RayIO& io,
   }
   const int64_t grid = (n + 255) / 256;
   hipLaunchKernelGGL(k_order_key, dim3(unsigned(grid)), dim3(256), 0, ctx->stream, scene->s, io, n, scene->top_lo[0],
-                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], ob.tile_bits, ob.mbits, key, idx, rr);
+                     scene->top_lo[1], scene->top_lo[2], s[0], s[1], s[2], ob.tile_bits, ob.mbits, key, idx, rr,
+                     first_box);
   HIPCHK(ctx, hipGetLastError());
   return 0;
 }
@@ -1083,7 +1084,7 @@ using OrderSortConfig = rocprim::radix_sort_config<
 // written by the key kernel into a buffer of the context's; where that
 // buffer cannot be had (128 B per ray), *rr is null and the order is tiled.
 static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm,
-                      RayRec** rr = nullptr) {
+                      RayRec** rr = nullptr, bool first_box = false) {
   if (size_t(n) > ctx->order_cap) {
     if (ctx->order_buf) HIPCHK(ctx, hipFree(ctx->order_buf));
     ctx->order_buf = nullptr;
@@ -1123,7 +1124,7 @@ static int order_rays(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64
     HIPCHK(ctx, hipMalloc(&ctx->sort_tmp, bytes));
     ctx->sort_tmp_bytes = bytes;
   }
-  const int rc = launch_order_key(ctx, scene, io, n, ob, key_in, idx_in, rec);
+  const int rc = launch_order_key(ctx, scene, io, n, ob, key_in, idx_in, rec, first_box);
   if (rc) return rc;
   bytes = ctx->sort_tmp_bytes;
   HIPCHK(ctx, rocprim::radix_sort_pairs<OrderSortConfig>(ctx->sort_tmp, bytes, key_in, key_out, idx_in, idx_out, count, 0u, ob.end_bit,
@@ -1169,7 +1170,8 @@ static hipError_t resolve_scratch(int device, void** out) {
 // Coherence order inside the scratch array (at most kNoAllocMaxRays rays):
 // the double-buffered radix sort, whose temporary storage is only its
 // histograms, so the keys and indices need no second copy.
-static int order_rays_scratch(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm) {
+static int order_rays_scratch(rb_ctx* ctx, const rb_scene* scene, const RayIO& io, int64_t n, const uint32_t** perm,
+                              bool first_box) {
   unsigned char* base = static_cast<unsigned char*>(ctx->scratch) + kScratchCounterBytes;
   uint32_t* key_a = reinterpret_cast<uint32_t*>(base);
   uint32_t* key_b = key_a + kNoAllocMaxRays;
@@ -1186,7 +1188,7 @@ static int order_rays_scratch(rb_ctx* ctx, const rb_scene* scene, const RayIO& i
     return fail(ctx, -2, "rbgpu_trace_noalloc: the radix sort needs " + std::to_string(bytes) +
                              " B of temporary storage, more than the scratch array's " +
                              std::to_string(kScratchSortBytes));
-  const int rc = launch_order_key(ctx, scene, io, n, ob, key_a, idx_a, nullptr);
+  const int rc = launch_order_key(ctx, scene, io, n, ob, key_a, idx_a, nullptr, first_box);
   if (rc) return rc;
   bytes = kScratchSortBytes;
   HIPCHK(ctx, rocprim::radix_sort_pairs(tmp, bytes, keys, idx, count, 0u, ob.end_bit, ctx->stream));
@@ -1213,8 +1215,10 @@ static int trace_launch(rb_ctx* ctx, const rb_scene* scene, const rb_trace_opts*
   RayRec* rr = nullptr;
   HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if (coherent) {
-    const int rc = noalloc ? order_rays_scratch(ctx, scene, io, n, &perm)
-                           : order_rays(ctx, scene, io, n, &perm, &rr);
+    // the key: the first daughter box met for the work-queue schedules, the
+    // nearest for the lane kernel
+    const int rc = noalloc ? order_rays_scratch(ctx, scene, io, n, &perm, persistent)
+                           : order_rays(ctx, scene, io, n, &perm, &rr, persistent);
     if (rc) return rc;
   }
   // the key kernel, the radix sort (counted once), the trace and the

@@ -246,14 +246,14 @@ def test_coherence_order_gives_identical_results(ctx, config):
 
 def test_auto_order_work_queue_threshold(ctx):
     """The Winston cone array (a lens scene, traced with the work-queue
-    schedule) is sorted by the auto order from 3 x 2^18 rays: just above that
+    schedule) is sorted by the auto order from 2^18 rays: just above that
     the trace runs the key kernel, sort, trace and copy-back, just below it
     the trace alone, and both equal input order."""
     import torch
     mgr = configs.c4_scene()
     scene = ctx.upload(mgr.flat())
     dev = torch.device("cuda")
-    for N, launches in ((3 * (1 << 18) + 5, 4), (3 * (1 << 18) - 5, 1)):
+    for N, launches in (((1 << 18) + 5, 4), ((1 << 18) - 5, 1)):
         print(f"C4 auto order n={N}", flush=True)
         out = {}
         for order in (A.RB_ORDER_AUTO, A.RB_ORDER_INPUT):
