@@ -114,11 +114,12 @@ constexpr int kTraceBlock = kCandStride;
 #define RB_TRACE_ATTR
 #endif
 
-// A ray's input state in one 128-byte line. Traced in coherence order, the
-// rays of a wavefront lie far apart in the input, so each of the eleven
-// state arrays costs the lane a line of its own; the ordering kernel, which
-// reads the rays in input order anyway, writes this copy so that the trace
-// reads one line per ray instead.
+// A ray's state in one 128-byte line. Traced in coherence order, the rays of
+// a wavefront lie far apart in the input, so each of the eleven state arrays
+// costs the lane a line of its own; the ordering kernel, which reads the
+// rays in input order anyway, writes this copy so that the trace reads one
+// line per ray instead, and writes the traced state back into the same line
+// (k_unstage then copies the lines to the state arrays, coalesced).
 struct alignas(128) RayRec {
   double x, y, z, t, dx, dy, dz, lambda;
   int32_t status, npoints, last_node, pad;
@@ -245,7 +246,8 @@ constexpr int kOrderGlobalBits = 31;  // one tile: batches are below 2^31 rays
 // Keys are 32-bit: the tile index in the high bits and as many of the top
 // bits of the 30-bit Morton code as fit below it (all 30 for a global order
 // or up to 4 tiles; 27 = 9 bits per axis for 2^19-ray tiles at 1e7 rays),
-// so that the radix sort makes 4 passes over 8-byte pairs instead of 5 over
+// so that the radix sort moves 8-byte pairs (3 passes of 10 bits for the
+// global order, 4 of 8 in the allocation-free entry point) instead of
 // 12-byte ones. The walk reads the top node's BVH through the same LDS copy
 // as the tracing kernels (stage_bvh / bvh_node): it is a chain of dependent
 // loads, and LDS answers them at a fraction of the L2 latency.
