@@ -93,13 +93,14 @@ class FacetSceneUpdater:
         self.nfacets = nfacets
         index = {name: k for k, name in enumerate(flat.node_names)}
         self.nodes = [index[f"mirror{i}_{i + 1}"] for i in range(nfacets)]
-        self.base_rot = [np.asarray(configs._dc_facet_matrix(x, y).rot) for x, y in configs.dc_facet_positions()]
+        self.base_rot = np.stack([np.asarray(configs._dc_facet_matrix(x, y).rot) for x, y in configs.dc_facet_positions()])
         self.comp = []      # (composite object, composite shape index, sphere shape index)
         for k in self.nodes:
             v = int(flat.nodes[k]["volume"])
             cs = int(flat.volumes[v]["shape"])
             sph = int(flat.params[flat.shapes[cs]["param_offset"] + 1])
             self.comp.append((flat.volume_objs[v].shape, cs, sph))
+        self.radius = np.array([comp.node.left.rmin for comp, _, _ in self.comp])  # the spheres' current R
 
     def _put_shape(self, idx, sh, params):
         f = self.flat
@@ -116,18 +117,17 @@ class FacetSceneUpdater:
         p = np.asarray(params, dtype=float)
         nf = self.nfacets
         f = self.flat
-        for i, k in enumerate(self.nodes):
-            rot = self.base_rot[i] @ configs._small_rotation(float(p[2 * i]), float(p[2 * i + 1]))
-            f.nodes[k]["rot"] = rot.ravel()
+        f.nodes["rot"][self.nodes] = configs.facet_rotations(self.base_rot[:nf], p[:2 * nf]).reshape(nf, 9)
+        for i in np.nonzero(p[2 * nf:3 * nf] != self.radius)[0]:  # the facets whose R changed
             comp, cs, sph = self.comp[i]
             s = comp.node.left
             R = float(p[2 * nf + i])
-            if s.rmin != R:
-                s.rmin, s.rmax = R, R + configs.C2_MIRROR_T
-                s.theta1 = 180.0 - math.asin(configs.C2_MIRROR_D / math.sqrt(3) / R) * 180.0 / math.pi
-                comp.node.lmat.tr = np.array([0.0, 0.0, R])
-                self._put_shape(sph, s, s.params())
-                self._put_shape(cs, comp, comp.params())
+            s.rmin, s.rmax = R, R + configs.C2_MIRROR_T
+            s.theta1 = 180.0 - math.asin(configs.C2_MIRROR_D / math.sqrt(3) / R) * 180.0 / math.pi
+            comp.node.lmat.tr = np.array([0.0, 0.0, R])
+            self._put_shape(sph, s, s.params())
+            self._put_shape(cs, comp, comp.params())
+            self.radius[i] = R
         cam = f.node_names.index("focalPlane_1")
         obs = f.node_names.index("cameraBox_1")
         zc = configs.C2_F + float(p[3 * nf])

@@ -147,6 +147,9 @@ def c2_scene(deterministic=False, per_facet=False, facet_radius=None, facet_tilt
     refl = None if deterministic else c2_reflectance()
     pos = dc_facet_positions()
     shared = None
+    rots = None
+    if facet_tilt is not None:
+        rots = facet_rotations([_dc_facet_matrix(x, y).rot for x, y in pos], facet_tilt)
     for i, (x, y) in enumerate(pos):
         if per_facet or shared is None:
             rad = facet_radius[i] if facet_radius is not None else C2_MIRROR_R
@@ -158,11 +161,8 @@ def c2_scene(deterministic=False, per_facet=False, facet_radius=None, facet_tilt
         else:
             mirror = shared
         mat = _dc_facet_matrix(x, y)
-        if facet_tilt is not None:
-            tip, tilt = facet_tilt[i]
-            extra = TGeoRotation()
-            extra.rot = _small_rotation(tip, tilt)
-            mat.rot = mat.rot @ extra.rot
+        if rots is not None:
+            mat.rot = rots[i]
         world.AddNode(mirror, i + 1, mat)
     if surfaces:
         from .geometry import ABorderSurfaceCondition
@@ -183,12 +183,19 @@ def c2_scene(deterministic=False, per_facet=False, facet_radius=None, facet_tilt
     return mgr
 
 
-def _small_rotation(tip, tilt):
-    """Rotation by `tip` about local x then `tilt` about local y (radians)."""
-    cx, sx, cy, sy = math.cos(tip), math.sin(tip), math.cos(tilt), math.sin(tilt)
-    rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
-    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
-    return rx @ ry
+def facet_rotations(base, tilts):
+    """base @ Rx(tip) @ Ry(tilt) for every facet: base (N, 3, 3) placement
+    rotations, tilts (N, 2) (tip, tilt) in radians, a rotation by `tip` about
+    the facet's local x then `tilt` about its local y. One vectorised
+    evaluation, shared by the scene builder and the alignment's scene updater
+    so that both give the same bits."""
+    t = np.asarray(tilts, dtype=float).reshape(-1, 2)
+    cx, sx, cy, sy = np.cos(t[:, 0]), np.sin(t[:, 0]), np.cos(t[:, 1]), np.sin(t[:, 1])
+    small = np.zeros((len(t), 3, 3))
+    small[:, 0, 0], small[:, 0, 2] = cy, sy
+    small[:, 1, 0], small[:, 1, 1], small[:, 1, 2] = sx * sy, cx, -(sx * cy)
+    small[:, 2, 0], small[:, 2, 1], small[:, 2, 2] = -(cx * sy), sx, cx * cy
+    return np.matmul(np.asarray(base, dtype=float).reshape(-1, 3, 3), small)
 
 
 def c2_raygen(seed=0xC2, theta_max_deg=4.0):
