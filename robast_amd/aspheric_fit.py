@@ -73,6 +73,41 @@ def figure_scene(params):
     return configs.c3_scene(figure=fig, focal_dz=float(p[Z_PARAM]))
 
 
+class FigureSceneUpdater:
+    """Moves an existing Schwarzschild-Couder scene to new figure parameters
+    by patching its flattened arrays in place: both mirror disks' parameters
+    and bounding boxes, through the same shape code a rebuild uses, and the
+    focal surface's z offset, so the arrays equal a rebuild's bit for bit."""
+
+    def __init__(self, mgr):
+        self.flat = flat = mgr.flat()
+        index = {name: k for k, name in enumerate(flat.node_names)}
+        self.mirrors = []       # (AGeoAsphericDisk, its shape index)
+        for name in ("primaryMirror_1", "secondaryMirror_1"):
+            v = int(flat.nodes[index[name]]["volume"])
+            self.mirrors.append((flat.volume_objs[v].shape, int(flat.volumes[v]["shape"])))
+        self.focal = index["focalPlane_1"]
+
+    def update(self, params):
+        p = np.asarray(params, dtype=float)
+        f = self.flat
+        for m, (sh, idx) in enumerate(self.mirrors):
+            o = m * PER_MIRROR
+            c, conic, pol = float(p[o]), float(p[o + 1]) - 1.0, [float(v) for v in p[o + 2:o + PER_MIRROR]]
+            sh.c1 = sh.c2 = c
+            sh.SetConicConstants(conic, conic)
+            sh.SetPolynomials(len(pol), pol, len(pol), pol)
+            off, n = int(f.shapes[idx]["param_offset"]), int(f.shapes[idx]["n_params"])
+            vals = sh.params()
+            assert len(vals) == n
+            f.params[off:off + n] = vals
+            org, half = sh.local_bbox()
+            f.shapes[idx]["bbox_origin"] = org
+            f.shapes[idx]["bbox_half"] = half
+        f.nodes[self.focal]["tr"] = [0.0, 0.0, float(p[Z_PARAM])]
+        return f
+
+
 def surface_table(flat):
     """[n_nodes][2] int32 (first, count) of each mirror's parameter block, -1
     elsewhere; plus the focal node index."""
@@ -129,6 +164,7 @@ class AsphericFigureFit:
             self.ctx.generate(raygen(a, k), make_rays_device(t), self.n)
             self.src.append(t)
         self.work = self._alloc()
+        self.updater = None
         self.targets = None
         self.table = None
         self.c0 = None
@@ -146,8 +182,12 @@ class AsphericFigureFit:
         if p.shape != (NPARAM,):
             raise ValueError(f"AsphericFigureFit: {NPARAM} parameters expected, got {p.shape}")
         self.params = p.copy()
-        self.mgr = figure_scene(p)
-        flat = self.mgr.flat()
+        if self.updater is None:
+            self.mgr = figure_scene(p)
+            self.updater = FigureSceneUpdater(self.mgr)
+            flat = self.mgr.flat()
+        else:
+            flat = self.updater.update(p)
         if self.table is None:
             table, self.focal_node = surface_table(flat)
             self.table = torch.from_numpy(table).to(self.dev)

@@ -301,3 +301,21 @@ def test_gpu_figure_moments_form_matches_centroid_form(ctx):
     assert loss1 == pytest.approx(model.loss(p1), rel=1e-10)
     g1c = model.gradient()
     assert (np.abs(g1 - g1c) / np.abs(g1c)).max() <= 1e-8
+
+
+def test_figure_scene_updater_equals_rebuild():
+    """Patching the flattened Schwarzschild-Couder scene to new figure
+    parameters (curvatures, conics, r^2k terms, focal offset, including a
+    zero offset) gives the same arrays, bit for bit, as building the scene at
+    those parameters."""
+    p0 = F.figure_params()
+    up = F.FigureSceneUpdater(F.figure_scene(p0))
+    rng = np.random.default_rng(3)
+    for it in range(3):
+        p = p0 * (1 + rng.normal(0, 1e-3, F.NPARAM))
+        p[F.Z_PARAM] = rng.normal(0, 1.0) if it else 0.0
+        got, want = up.update(p), F.figure_scene(p).flat()
+        for name in ("nodes", "children", "volumes", "shapes", "params", "tables", "table_xy"):
+            a, b = getattr(got, name), getattr(want, name)
+            assert a.dtype == b.dtype and a.shape == b.shape, name
+            assert a.tobytes() == b.tobytes(), (it, name)
