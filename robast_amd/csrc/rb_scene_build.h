@@ -123,7 +123,7 @@ struct BvhBuilder {
 };
 
 // Neighbour cells of node n's daughters. A daughter c's cell box C(c) is
-// its FP32 box grown by a tenth of its extent on each axis; its neighbours
+// its FP32 box grown by kCellGrow of its extent on each axis; its neighbours
 // are the daughters whose FP32 boxes meet C(c), c included. Any daughter
 // that contains a point of C(c), or is crossed by a ray inside C(c), has a
 // box meeting C(c) (boxes enclose their shapes), so the neighbours are a
@@ -132,6 +132,9 @@ struct BvhBuilder {
 constexpr int kCellMinChildren = 256;
 constexpr int kCellMaxChildren = 1 << 20;
 constexpr int kCellMaxNbrs = 48;
+// Measured on the Winston cone array (1e7 rays): growth 0.03 / 0.05 / 0.1 /
+// 0.2 / 0.3 -> 78.0 / 78.0 / 78.9 / 78.9 / 80.9 ms per trace kernel.
+constexpr double kCellGrow = 0.05;
 
 inline void build_cells(const rb_scene_desc* d, const rb_node_desc& n, std::vector<DNode>& nodes,
                         std::vector<DBvh>& cells, std::vector<DBvh>& nbrs) {
@@ -150,7 +153,7 @@ inline void build_cells(const rb_scene_desc* d, const rb_node_desc& n, std::vect
     box[k] = e;
     DBvh g = e;
     for (int a = 0; a < 3; a++) {
-      const double grow = 0.1 * (double(e.hi[a]) - double(e.lo[a]));
+      const double grow = kCellGrow * (double(e.hi[a]) - double(e.lo[a]));
       g.lo[a] = down(double(e.lo[a]) - grow);
       g.hi[a] = up(double(e.hi[a]) + grow);
     }
